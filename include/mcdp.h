@@ -1,0 +1,148 @@
+/*
+ * mcdp.h -- C ABI of the MI355X Monte Carlo DAG propagation library (libmcdp.so).
+ *
+ * Plain C types only: no C++ or torch types cross this boundary.  Every entry
+ * point returns 0 (MCDP_OK) on success or a nonzero mcdp_status; the message of
+ * the last failure on the calling thread is available from mcdp_last_error().
+ * Bindings turn a nonzero status into the reference's exception type
+ * (std::runtime_error -> Python RuntimeError).
+ *
+ * Each entry point names the reference interface it replaces, as a
+ * file:line into WonJayne/mc_dagprop (src/mc_dagprop/monte_carlo/_core.cpp).
+ *
+ * Layout: per-sample arrays are sample-contiguous ("event-major"):
+ *   realized[e * ld + s], cause[e * ld + s]   for event e,   sample s
+ *   durations[l * ld + s]                     for activity l, sample s
+ * so a wave of 64 samples reads and writes 512-byte runs.  Sample s of the
+ * reference's SimResult (_core.cpp:65-69) is column s.
+ */
+#ifndef MCDP_H
+#define MCDP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCDP_ABI_VERSION 1
+
+typedef enum mcdp_status {
+    MCDP_OK = 0,
+    MCDP_E_INVALID = 1,       /* invalid argument or context                     */
+    MCDP_E_RESERVED_TYPE = 2, /* a distribution on activity type -1 (_core.cpp:196-198) */
+    MCDP_E_MAX_DELAY = 3,     /* max_delay < 0 (_core.cpp:199-201)                */
+    MCDP_E_CYCLE = 4,         /* cycle in precedence list (_core.cpp:262-264)     */
+    MCDP_E_DEVICE = 5,        /* no HIP device, or a HIP call failed              */
+    MCDP_E_NOMEM = 6          /* host or device allocation failed                 */
+} mcdp_status;
+
+typedef struct mcdp_generator mcdp_generator;
+typedef struct mcdp_propagator mcdp_propagator;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *mcdp_last_error(void);
+/* MCDP_ABI_VERSION of the loaded library. */
+int mcdp_abi_version(void);
+
+/* ---- GenericDelayGenerator (_core.cpp:146-159, binding :519-542) -------- */
+
+/* GenericDelayGenerator() (_core.cpp:151). */
+int mcdp_generator_create(mcdp_generator **out);
+void mcdp_generator_destroy(mcdp_generator *gen);
+/* GenericDelayGenerator::set_seed (_core.cpp:153).  Accepted and ignored: the
+ * reference never draws from the generator's own engine (:148, :313). */
+int mcdp_generator_set_seed(mcdp_generator *gen, int32_t seed);
+/* add_constant (_core.cpp:154): extra = factor * minimal_duration. */
+int mcdp_generator_add_constant(mcdp_generator *gen, int32_t activity_type, double factor);
+/* add_exponential (_core.cpp:155): X ~ Exp(mean lambda_) conditioned on
+ * X <= max_scale; extra = X * minimal_duration (:78-90). */
+int mcdp_generator_add_exponential(mcdp_generator *gen, int32_t activity_type, double lambda_, double max_scale);
+/* add_gamma (_core.cpp:156-158): X ~ Gamma(shape, scale) conditioned on
+ * X <= max_scale (pass INFINITY for none); extra = X * minimal_duration. */
+int mcdp_generator_add_gamma(mcdp_generator *gen, int32_t activity_type, double shape, double scale,
+                             double max_scale);
+/* add_empirical_absolute (_core.cpp:529-534, dist :110-126): extra = values[i],
+ * P(i) = weights[i] / sum(weights).  n == 1 always draws bin 0. */
+int mcdp_generator_add_empirical_absolute(mcdp_generator *gen, int32_t activity_type, const double *values,
+                                          const double *weights, int64_t n);
+/* add_empirical_relative (_core.cpp:536-541, dist :129-141): extra =
+ * factors[i] * minimal_duration. */
+int mcdp_generator_add_empirical_relative(mcdp_generator *gen, int32_t activity_type, const double *factors,
+                                          const double *weights, int64_t n);
+
+/* ---- MonteCarloPropagator (Simulator, _core.cpp:162-362) ----------------- */
+
+/* Simulator(context, generator) (_core.cpp:193-307): validates, orders and
+ * compiles the DAG on the host.  Device upload happens on the first run.
+ *   events:      n_events earliest times (EventTimestamp.earliest; latest and
+ *                actual are not read by the Monte Carlo engine)
+ *   activities:  n_activities (idx, minimal_duration, activity_type); the
+ *                (src, dst) keys of the reference's activity map are not read
+ *   precedence:  n_entries entries; entry i is (entry_target[i],
+ *                [(pred_event[k], pred_activity[k]) for k in
+ *                 entry_offsets[i] .. entry_offsets[i+1])]
+ * The generator is copied; it may be destroyed afterwards. */
+int mcdp_propagator_create(int32_t n_events, const double *earliest, int32_t n_activities,
+                           const int32_t *activity_idx, const double *minimal_duration,
+                           const int32_t *activity_type, int32_t n_entries, const int32_t *entry_target,
+                           const int64_t *entry_offsets, const int32_t *pred_event, const int32_t *pred_activity,
+                           double max_delay, const mcdp_generator *gen, mcdp_propagator **out);
+void mcdp_propagator_destroy(mcdp_propagator *prop);
+/* node_count / activity_count (_core.cpp:309-310). */
+int32_t mcdp_propagator_node_count(const mcdp_propagator *prop);
+int32_t mcdp_propagator_activity_count(const mcdp_propagator *prop);
+
+/* Simulator::run_many (_core.cpp:355-361) on device buffers, asynchronous on
+ * `stream` (a hipStream_t, NULL = default stream).  seeds_dev: int32[n] device;
+ * outputs event-major with leading dimension ld >= n.  durations_dev or
+ * cause_dev may be NULL to skip that output; realized_dev may be NULL only if
+ * the context keeps every predecessor in LDS (otherwise MCDP_E_INVALID). */
+int mcdp_run_many_device(mcdp_propagator *prop, const int32_t *seeds_dev, int64_t n, int64_t ld,
+                         double *realized_dev, double *durations_dev, int32_t *cause_dev, void *stream);
+
+/* Simulator::run_many (_core.cpp:355-361) on host buffers, synchronous.
+ * Outputs event-major with ld = n: realized[E*n], durations[L*n], cause[E*n]. */
+int mcdp_run_many_host(mcdp_propagator *prop, const int32_t *seeds, int64_t n, double *realized,
+                       double *durations, int32_t *cause);
+
+/* The propagation half of Simulator::run (_core.cpp:316-350) with injected
+ * durations (durations_dev[l * ld + s]); device buffers, asynchronous. */
+int mcdp_propagate_device(mcdp_propagator *prop, const double *durations_dev, int64_t n, int64_t ld,
+                          double *realized_dev, int32_t *cause_dev, void *stream);
+/* Same on host buffers (ld = n), synchronous. */
+int mcdp_propagate_host(mcdp_propagator *prop, const double *durations, int64_t n, double *realized,
+                        int32_t *cause);
+
+/* Plan facts for tests and tuning: out[0..7] = {steps, waves per workgroup,
+ * LDS slots, LDS bytes per workgroup, predecessors read from global memory,
+ * stale predecessors, orphan activities, max events per wave and step}. */
+int mcdp_propagator_plan_info(const mcdp_propagator *prop, int64_t *out, int n_out);
+
+/* ---- Device plumbing for callers without their own HIP allocator -------- */
+/* (no counterpart in the reference, which has no device) */
+int mcdp_device_count(int *count);
+int mcdp_malloc(void **ptr, size_t bytes);
+int mcdp_free(void *ptr);
+int mcdp_memcpy_htod(void *dst_dev, const void *src_host, size_t bytes);
+int mcdp_memcpy_dtoh(void *dst_host, const void *src_dev, size_t bytes);
+int mcdp_memset(void *dst_dev, int value, size_t bytes);
+int mcdp_stream_create(void **stream);
+int mcdp_stream_destroy(void *stream);
+int mcdp_stream_synchronize(void *stream);
+int mcdp_event_create(void **event);
+int mcdp_event_destroy(void *event);
+int mcdp_event_record(void *event, void *stream);
+int mcdp_event_elapsed_ms(void *start, void *stop, float *ms);
+
+/* ---- Test hooks (no counterpart in the reference) ------------------------ */
+/* Runs the device Philox-4x32-10 (variant 4) or Philox-2x32-10 (variant 2) on
+ * n host-supplied counters/keys and copies the words back to the host.
+ * variant 4: ctr[4n], key[2n], out[4n]; variant 2: ctr[2n], key[n], out[2n]. */
+int mcdp_debug_philox(int variant, int64_t n, const uint32_t *ctr, const uint32_t *key, uint32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MCDP_H */

@@ -1,0 +1,298 @@
+"""ctypes view of libmcdp's C ABI (``include/mcdp.h``).
+
+The pybind11 module ``mc_dagprop_amd.monte_carlo._core`` is the drop-in API;
+this module is the thin, torch-free route to the same library for callers that
+keep results on the device (the benchmark, statistics consumers) and for tests
+that check the C ABI itself.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+
+from .workloads import CONSTANT, EMPIRICAL_ABSOLUTE, EXPONENTIAL, GAMMA, FlatDag
+
+LIB_PATH = Path(__file__).resolve().parent / "libmcdp.so"
+HEADER_PATH = Path(__file__).resolve().parent.parent / "include" / "mcdp.h"
+
+_P = C.c_void_p
+_i32, _i64, _f64, _u32 = C.c_int32, C.c_int64, C.c_double, C.c_uint32
+_pi32, _pi64, _pf64, _pu32 = C.POINTER(_i32), C.POINTER(_i64), C.POINTER(_f64), C.POINTER(_u32)
+
+_SIGNATURES = {
+    "mcdp_last_error": (C.c_char_p, []),
+    "mcdp_abi_version": (C.c_int, []),
+    "mcdp_generator_create": (C.c_int, [C.POINTER(_P)]),
+    "mcdp_generator_destroy": (None, [_P]),
+    "mcdp_generator_set_seed": (C.c_int, [_P, _i32]),
+    "mcdp_generator_add_constant": (C.c_int, [_P, _i32, _f64]),
+    "mcdp_generator_add_exponential": (C.c_int, [_P, _i32, _f64, _f64]),
+    "mcdp_generator_add_gamma": (C.c_int, [_P, _i32, _f64, _f64, _f64]),
+    "mcdp_generator_add_empirical_absolute": (C.c_int, [_P, _i32, _pf64, _pf64, _i64]),
+    "mcdp_generator_add_empirical_relative": (C.c_int, [_P, _i32, _pf64, _pf64, _i64]),
+    "mcdp_propagator_create": (
+        C.c_int,
+        [_i32, _pf64, _i32, _pi32, _pf64, _pi32, _i32, _pi32, _pi64, _pi32, _pi32, _f64, _P, C.POINTER(_P)],
+    ),
+    "mcdp_propagator_destroy": (None, [_P]),
+    "mcdp_propagator_node_count": (_i32, [_P]),
+    "mcdp_propagator_activity_count": (_i32, [_P]),
+    "mcdp_run_many_device": (C.c_int, [_P, _P, _i64, _i64, _P, _P, _P, _P]),
+    "mcdp_run_many_host": (C.c_int, [_P, _pi32, _i64, _pf64, _pf64, _pi32]),
+    "mcdp_propagate_device": (C.c_int, [_P, _P, _i64, _i64, _P, _P, _P]),
+    "mcdp_propagate_host": (C.c_int, [_P, _pf64, _i64, _pf64, _pi32]),
+    "mcdp_propagator_plan_info": (C.c_int, [_P, _pi64, C.c_int]),
+    "mcdp_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "mcdp_malloc": (C.c_int, [C.POINTER(_P), C.c_size_t]),
+    "mcdp_free": (C.c_int, [_P]),
+    "mcdp_memcpy_htod": (C.c_int, [_P, _P, C.c_size_t]),
+    "mcdp_memcpy_dtoh": (C.c_int, [_P, _P, C.c_size_t]),
+    "mcdp_memset": (C.c_int, [_P, C.c_int, C.c_size_t]),
+    "mcdp_stream_create": (C.c_int, [C.POINTER(_P)]),
+    "mcdp_stream_destroy": (C.c_int, [_P]),
+    "mcdp_stream_synchronize": (C.c_int, [_P]),
+    "mcdp_event_create": (C.c_int, [C.POINTER(_P)]),
+    "mcdp_event_destroy": (C.c_int, [_P]),
+    "mcdp_event_record": (C.c_int, [_P, _P]),
+    "mcdp_event_elapsed_ms": (C.c_int, [_P, _P, C.POINTER(C.c_float)]),
+    "mcdp_debug_philox": (C.c_int, [C.c_int, _i64, _pu32, _pu32, _pu32]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded libmcdp.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python mc_dagprop_amd/build.py`")
+        handle = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def header_symbols() -> list[str]:
+    """Every function name declared in include/mcdp.h."""
+    text = HEADER_PATH.read_text()
+    return sorted(set(re.findall(r"\b(mcdp_[a-z0-9_]+)\s*\(", text)))
+
+
+class McdpError(RuntimeError):
+    """A nonzero mcdp_status; the message is mcdp_last_error()."""
+
+    def __init__(self, code: int, message: str) -> None:
+        super().__init__(message)
+        self.code = code
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise McdpError(rc, lib().mcdp_last_error().decode())
+
+
+def _ptr(a: np.ndarray, t):
+    return a.ctypes.data_as(t)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(lib().mcdp_device_count(C.byref(n)))
+    return n.value
+
+
+class DeviceBuffer:
+    """Device allocation owned by this object (freed on close / garbage collection)."""
+
+    def __init__(self, nbytes: int) -> None:
+        self.nbytes = int(nbytes)
+        self.ptr = C.c_void_p(0)
+        if self.nbytes:
+            check(lib().mcdp_malloc(C.byref(self.ptr), self.nbytes))
+
+    @staticmethod
+    def from_array(a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        buf = DeviceBuffer(a.nbytes)
+        if a.nbytes:
+            check(lib().mcdp_memcpy_htod(buf.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes))
+        return buf
+
+    def to_array(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        if out.nbytes:
+            check(lib().mcdp_memcpy_dtoh(out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes))
+        return out
+
+    def close(self) -> None:
+        if self.ptr and self.ptr.value:
+            lib().mcdp_free(self.ptr)
+            self.ptr = C.c_void_p(0)
+
+    def __del__(self) -> None:  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Generator:
+    """GenericDelayGenerator over the C ABI."""
+
+    def __init__(self, dists=()) -> None:
+        self.h = C.c_void_p(0)
+        check(lib().mcdp_generator_create(C.byref(self.h)))
+        for d in dists:
+            self.add(d)
+
+    def add(self, d) -> None:
+        L = lib()
+        a, b, c = d.params
+        if d.kind == CONSTANT:
+            check(L.mcdp_generator_add_constant(self.h, d.activity_type, a))
+        elif d.kind == EXPONENTIAL:
+            check(L.mcdp_generator_add_exponential(self.h, d.activity_type, a, b))
+        elif d.kind == GAMMA:
+            check(L.mcdp_generator_add_gamma(self.h, d.activity_type, a, b, c))
+        else:
+            v = np.ascontiguousarray(d.values, dtype=np.float64)
+            w = np.ascontiguousarray(d.weights, dtype=np.float64)
+            fn = (
+                L.mcdp_generator_add_empirical_absolute
+                if d.kind == EMPIRICAL_ABSOLUTE
+                else L.mcdp_generator_add_empirical_relative
+            )
+            check(fn(self.h, d.activity_type, _ptr(v, _pf64), _ptr(w, _pf64), v.size))
+
+    def __del__(self) -> None:  # pragma: no cover
+        try:
+            lib().mcdp_generator_destroy(self.h)
+        except Exception:
+            pass
+
+
+class Propagator:
+    """MonteCarloPropagator over the C ABI, built from a :class:`FlatDag`."""
+
+    def __init__(self, dag: FlatDag) -> None:
+        gen = Generator(dag.dists)
+        self._keep = [
+            np.ascontiguousarray(x, dtype=t)
+            for x, t in (
+                (dag.earliest, np.float64),
+                (dag.act_idx, np.int32),
+                (dag.act_dur, np.float64),
+                (dag.act_type, np.int32),
+                (dag.prec_target, np.int32),
+                (dag.prec_off, np.int64),
+                (dag.prec_src, np.int32),
+                (dag.prec_edge, np.int32),
+            )
+        ]
+        e, ai, ad, at, pt, po, ps, pe = self._keep
+        self.h = C.c_void_p(0)
+        check(
+            lib().mcdp_propagator_create(
+                e.size,
+                _ptr(e, _pf64),
+                ai.size,
+                _ptr(ai, _pi32),
+                _ptr(ad, _pf64),
+                _ptr(at, _pi32),
+                pt.size,
+                _ptr(pt, _pi32),
+                _ptr(po, _pi64),
+                _ptr(ps, _pi32),
+                _ptr(pe, _pi32),
+                dag.max_delay,
+                gen.h,
+                C.byref(self.h),
+            )
+        )
+        self.E = lib().mcdp_propagator_node_count(self.h)
+        self.L = lib().mcdp_propagator_activity_count(self.h)
+
+    def plan_info(self) -> dict:
+        v = np.zeros(8, dtype=np.int64)
+        check(lib().mcdp_propagator_plan_info(self.h, _ptr(v, _pi64), 8))
+        names = ["steps", "waves", "lds_slots", "lds_bytes", "global_preds", "stale_preds", "orphans", "max_events_per_wave"]
+        return dict(zip(names, map(int, v)))
+
+    def run_many(self, seeds) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """Host run_many: (realized (n,E), durations (n,L), cause (n,E)) as transposed views."""
+        s = np.ascontiguousarray(seeds, dtype=np.int32)
+        n = s.size
+        r = np.empty((self.E, n))
+        d = np.empty((self.L, n))
+        c = np.empty((self.E, n), dtype=np.int32)
+        check(lib().mcdp_run_many_host(self.h, _ptr(s, _pi32), n, _ptr(r, _pf64), _ptr(d, _pf64), _ptr(c, _pi32)))
+        return r.T, d.T, c.T
+
+    def propagate(self, durations: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+        """Host propagation with injected durations (n, L): (realized (n,E), cause (n,E))."""
+        dT = np.ascontiguousarray(np.asarray(durations, dtype=np.float64).T)
+        n = dT.shape[1] if dT.ndim == 2 else 0
+        r = np.empty((self.E, n))
+        c = np.empty((self.E, n), dtype=np.int32)
+        check(lib().mcdp_propagate_host(self.h, _ptr(dT, _pf64), n, _ptr(r, _pf64), _ptr(c, _pi32)))
+        return r.T, c.T
+
+    def run_many_device(self, seeds: DeviceBuffer, n: int, ld: int, realized, durations, cause, stream=None) -> None:
+        def p(b):
+            return b.ptr if b is not None else None
+
+        check(lib().mcdp_run_many_device(self.h, seeds.ptr, n, ld, p(realized), p(durations), p(cause), stream))
+
+    def propagate_device(self, durations: DeviceBuffer, n: int, ld: int, realized, cause, stream=None) -> None:
+        def p(b):
+            return b.ptr if b is not None else None
+
+        check(lib().mcdp_propagate_device(self.h, durations.ptr, n, ld, p(realized), p(cause), stream))
+
+    def __del__(self) -> None:  # pragma: no cover
+        try:
+            lib().mcdp_propagator_destroy(self.h)
+        except Exception:
+            pass
+
+
+def debug_philox(variant: int, ctr: np.ndarray, key: np.ndarray) -> np.ndarray:
+    """Device Philox-4x32-10 (variant 4) or -2x32-10 (variant 2) on n counters."""
+    ctr = np.ascontiguousarray(ctr, dtype=np.uint32)
+    key = np.ascontiguousarray(key, dtype=np.uint32)
+    n = ctr.shape[0]
+    out = np.empty_like(ctr)
+    check(lib().mcdp_debug_philox(variant, n, _ptr(ctr, _pu32), _ptr(key, _pu32), _ptr(out, _pu32)))
+    return out
+
+
+class Timer:
+    """HIP events on a stream: elapsed milliseconds between two records."""
+
+    def __init__(self) -> None:
+        self.a, self.b = C.c_void_p(0), C.c_void_p(0)
+        check(lib().mcdp_event_create(C.byref(self.a)))
+        check(lib().mcdp_event_create(C.byref(self.b)))
+
+    def start(self, stream=None) -> None:
+        check(lib().mcdp_event_record(self.a, stream))
+
+    def stop(self, stream=None) -> None:
+        check(lib().mcdp_event_record(self.b, stream))
+
+    def elapsed_ms(self) -> float:
+        ms = C.c_float(0)
+        check(lib().mcdp_event_elapsed_ms(self.a, self.b, C.byref(ms)))
+        return float(ms.value)
+
+
+def synchronize(stream=None) -> None:
+    check(lib().mcdp_stream_synchronize(stream))

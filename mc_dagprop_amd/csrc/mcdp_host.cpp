@@ -1,0 +1,947 @@
+// Host side of libmcdp: the delay-generator registry, the context compiler
+// that turns a DAG into a levelled, wave-scheduled device plan, and the C ABI
+// declared in include/mcdp.h.
+//
+// Reference semantics followed here (WonJayne/mc_dagprop,
+// src/mc_dagprop/monte_carlo/_core.cpp):
+//   * validation order and messages: :196-201, :262-264;
+//   * link table: link_count = max(idx) + 1, gaps are {0.0, type -1} (:213-229);
+//   * the last precedence entry of a target supplies its predecessor list and
+//     indegree, every entry adds adjacency, Kahn order with a FIFO (:232-264);
+//   * per-event predecessor order is the input order (:266-285);
+//   * links without a distribution keep minimal_duration (:303-306).
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/mcdp.h"
+#include "mcdp_kernels.hpp"
+#include "mcdp_plan.hpp"
+
+using namespace mcdp;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    return fail(MCDP_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                          \
+    do {                                             \
+        hipError_t _e = (expr);                      \
+        if (_e != hipSuccess) return hip_fail(_e, what); \
+    } while (0)
+
+// API-level distribution kinds (the reference's DistVar alternatives, :143).
+enum ApiKind { kApiConstant, kApiExponential, kApiGamma, kApiEmpAbs, kApiEmpRel };
+
+struct DistSpec {
+    ApiKind kind;
+    double a = 0.0, b = 0.0, c = 0.0;
+    std::vector<double> values, weights;
+};
+
+// ---- regularized lower incomplete gamma P(a, x) ---------------------------
+// Series  P = e^{-x} x^a / Gamma(a+1) * sum_n x^n / ((a+1)...(a+n))  for x < a+1,
+// continued fraction for Q = 1 - P otherwise (Abramowitz & Stegun 6.5.29,
+// 6.5.31; modified Lentz evaluation).  Used only to choose a sampling method.
+double reg_lower_gamma(double a, double x) {
+    if (!(x > 0.0)) return 0.0;
+    if (std::isinf(x)) return 1.0;
+    const double lg = std::lgamma(a);
+    if (x < a + 1.0) {
+        double term = 1.0 / a, sum = term, ap = a;
+        for (int n = 0; n < 100000; ++n) {
+            ap += 1.0;
+            term *= x / ap;
+            sum += term;
+            if (std::fabs(term) < std::fabs(sum) * 1e-17) break;
+        }
+        return std::min(1.0, sum * std::exp(-x + a * std::log(x) - lg));
+    }
+    const double tiny = 1e-300;
+    double b = x + 1.0 - a, c = 1.0 / tiny, d = 1.0 / b, h = d;
+    for (int i = 1; i < 100000; ++i) {
+        const double an = -i * (i - a);
+        b += 2.0;
+        d = an * d + b;
+        if (std::fabs(d) < tiny) d = tiny;
+        c = b + an / c;
+        if (std::fabs(c) < tiny) c = tiny;
+        d = 1.0 / d;
+        const double del = d * c;
+        h *= del;
+        if (std::fabs(del - 1.0) < 1e-16) break;
+    }
+    const double q = std::exp(-x + a * std::log(x) - lg) * h;
+    return std::max(0.0, 1.0 - q);
+}
+
+// ---- Vose alias table (M. D. Vose, IEEE TSE 17(9), 1991) -------------------
+// P(bin j) = (prob[j] + sum_{i: alias[i] = j} (1 - prob[i])) / n = w_j / sum(w).
+// Zero-weight bins get prob 0 and are never an alias target.
+void build_alias(const std::vector<double> &w, std::vector<double> &prob, std::vector<int> &alias) {
+    const size_t n = w.size();
+    long double total = 0.0L;
+    for (double x : w) total += x;
+    std::vector<long double> p(n);
+    std::vector<size_t> small, large;
+    size_t heaviest = 0;
+    for (size_t i = 0; i < n; ++i) {
+        p[i] = (long double)w[i] * (long double)n / total;
+        if (w[i] > w[heaviest]) heaviest = i;
+    }
+    // Zero-weight bins are pushed last so they pair first (stack order).
+    for (size_t i = 0; i < n; ++i)
+        if (p[i] < 1.0L && w[i] != 0.0) small.push_back(i);
+    for (size_t i = 0; i < n; ++i)
+        if (w[i] == 0.0) small.push_back(i);
+    for (size_t i = 0; i < n; ++i)
+        if (p[i] >= 1.0L) large.push_back(i);
+    prob.assign(n, 1.0);
+    alias.resize(n);
+    for (size_t i = 0; i < n; ++i) alias[i] = (int)i;
+    while (!small.empty() && !large.empty()) {
+        const size_t l = small.back();
+        small.pop_back();
+        const size_t g = large.back();
+        large.pop_back();
+        prob[l] = (double)p[l];
+        alias[l] = (int)g;
+        p[g] = (p[g] + p[l]) - 1.0L;
+        if (p[g] < 1.0L)
+            small.push_back(g);
+        else
+            large.push_back(g);
+    }
+    for (size_t g : large) prob[g] = 1.0;
+    for (size_t l : small) {  // rounding leftovers: p[l] is 1 up to rounding
+        if (w[l] == 0.0) {
+            prob[l] = 0.0;
+            alias[l] = (int)heaviest;
+        } else {
+            prob[l] = 1.0;
+        }
+    }
+}
+
+bool all_finite_nonneg(const std::vector<double> &w, double &sum) {
+    sum = 0.0;
+    for (double x : w) {
+        if (!std::isfinite(x) || x < 0.0) return false;
+        sum += x;
+    }
+    return std::isfinite(sum) && sum > 0.0;
+}
+
+int compile_exponential(double lam, double m, DistRec &D) {
+    if (std::isnan(lam) || lam < 0.0) return fail(MCDP_E_INVALID, "exponential lambda_ must be non-negative");
+    if (std::isnan(m)) m = std::numeric_limits<double>::infinity();  // x > NaN never rejects (:86)
+    if (m < 0.0) return fail(MCDP_E_INVALID, "exponential max_scale must be non-negative");
+    if (lam == 0.0 || m == 0.0) {  // X == 0 always
+        D.kind = kConstant;
+        D.p[0] = 0.0;
+    } else if (std::isinf(m)) {
+        D.kind = kExpInf;
+        D.p[0] = lam;
+    } else {
+        D.kind = kExpTrunc;
+        D.p[0] = lam;
+        D.p[1] = -std::expm1(-m / lam);
+        D.p[2] = m;
+    }
+    return MCDP_OK;
+}
+
+int compile_gamma(double k, double th, double m, DistRec &D) {
+    if (std::isnan(k) || !(k > 0.0) || std::isinf(k)) return fail(MCDP_E_INVALID, "gamma shape must be positive and finite");
+    if (std::isnan(th) || !(th > 0.0) || std::isinf(th)) return fail(MCDP_E_INVALID, "gamma scale must be positive and finite");
+    if (std::isnan(m)) m = std::numeric_limits<double>::infinity();
+    if (m < 0.0) return fail(MCDP_E_INVALID, "gamma max_scale must be non-negative");
+    if (m == 0.0) {
+        D.kind = kConstant;
+        D.p[0] = 0.0;
+        return MCDP_OK;
+    }
+    if (k == 1.0) return compile_exponential(th, m, D);  // Gamma(1, th) = Exp(mean th)
+    const double mp = m / th;
+    const double p_rej = std::isinf(m) ? 1.0 : reg_lower_gamma(k, mp);
+    double p_pow = 0.0;
+    if (!std::isinf(m)) {
+        const double lp = std::lgamma(k + 1.0) + std::log(std::max(p_rej, 1e-300)) - k * std::log(mp);
+        p_pow = std::min(1.0, std::exp(lp));
+    }
+    if (p_rej >= 0.25 || p_rej >= p_pow) {
+        if (p_rej < kMinAcceptance) return fail(MCDP_E_INVALID, "gamma max_scale truncation too severe to sample");
+        D.p[0] = th;
+        D.p[3] = m;
+        if (k == 2.0) {
+            D.kind = kGammaErlang2;
+        } else if (k == 0.5) {
+            D.kind = kGammaHalf;
+        } else {
+            const double kp = k < 1.0 ? k + 1.0 : k;
+            D.kind = kGammaMT;
+            D.boost = k < 1.0 ? 1 : 0;
+            D.p[1] = kp - 1.0 / 3.0;
+            D.p[2] = 1.0 / std::sqrt(9.0 * D.p[1]);
+            D.p[4] = 1.0 / k;
+        }
+    } else {
+        if (p_pow < kMinAcceptance) return fail(MCDP_E_INVALID, "gamma max_scale truncation too severe to sample");
+        D.kind = kGammaPow;
+        D.p[0] = th;
+        D.p[1] = mp;
+        D.p[2] = 1.0 / k;
+        D.p[3] = m;
+    }
+    return MCDP_OK;
+}
+
+int compile_empirical(const DistSpec &s, DistRec &D, std::vector<AliasRec> &table) {
+    const size_t n = s.values.size();
+    const bool abs = s.kind == kApiEmpAbs;
+    if (n == 0) return fail(MCDP_E_INVALID, "empirical distribution needs at least one value");
+    if (n == 1) {  // libstdc++ discrete_distribution with < 2 weights always yields 0
+        D.kind = abs ? kEmpiricalAbs1 : kEmpiricalRel1;
+        D.p[0] = s.values[0];
+        return MCDP_OK;
+    }
+    double sum;
+    if (!all_finite_nonneg(s.weights, sum))
+        return fail(MCDP_E_INVALID, "empirical weights must be finite, non-negative and not all zero");
+    if (n > (size_t)std::numeric_limits<int32_t>::max() / 2) return fail(MCDP_E_INVALID, "empirical table too large");
+    std::vector<double> prob;
+    std::vector<int> alias;
+    build_alias(s.weights, prob, alias);
+    D.kind = abs ? kEmpiricalAbs : kEmpiricalRel;
+    D.n = (int32_t)n;
+    D.table_off = (int32_t)table.size();
+    for (size_t i = 0; i < n; ++i) table.push_back(AliasRec{prob[i], s.values[i], s.values[(size_t)alias[i]], 0.0});
+    return MCDP_OK;
+}
+
+int compile_dist(const DistSpec &s, DistRec &D, std::vector<AliasRec> &table) {
+    D = DistRec{};
+    switch (s.kind) {
+        case kApiConstant:
+            D.kind = kConstant;
+            D.p[0] = s.a;
+            return MCDP_OK;
+        case kApiExponential: return compile_exponential(s.a, s.b, D);
+        case kApiGamma: return compile_gamma(s.a, s.b, s.c, D);
+        default: return compile_empirical(s, D, table);
+    }
+}
+
+// Relative cost of one draw (VALU work), for balancing waves within a step.
+int draw_cost(const DistRec *D) {
+    if (!D) return 1;
+    switch (D->kind) {
+        case kConstant:
+        case kEmpiricalAbs1:
+        case kEmpiricalRel1: return 2;
+        case kEmpiricalAbs:
+        case kEmpiricalRel: return 12;
+        case kExpInf:
+        case kExpTrunc: return 16;
+        case kGammaErlang2: return 18;
+        case kGammaHalf: return 24;
+        default: return 40;
+    }
+}
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace
+
+struct mcdp_generator {
+    std::map<int32_t, DistSpec> dists;  // last add_* per type wins
+};
+
+struct DeviceCopy {
+    int device = -1;
+    void *blob = nullptr;
+    WalkParams base{};
+};
+
+struct mcdp_propagator {
+    int32_t E = 0, L = 0;
+    // host plan
+    std::vector<int32_t> step_off;
+    std::vector<EventRec> ev;
+    std::vector<PredRec> pr;
+    std::vector<OrphanRec> orph;
+    std::vector<DistRec> dist;
+    std::vector<AliasRec> alias;
+    int32_t n_steps = 0, waves = 1, slots = 0, max_per_wave = 0;
+    int64_t n_global = 0, n_stale = 0;
+    // device copies, one per device used
+    std::mutex mu;
+    std::vector<DeviceCopy> dev;
+
+    ~mcdp_propagator() {
+        for (auto &d : dev) {
+            if (d.blob) {
+                int cur = -1;
+                (void)hipGetDevice(&cur);
+                (void)hipSetDevice(d.device);
+                (void)hipFree(d.blob);
+                if (cur >= 0) (void)hipSetDevice(cur);
+            }
+        }
+    }
+};
+
+namespace {
+
+constexpr size_t kLdsBudgetBytes = 64 * 1024;  // LDS per workgroup for predecessor slots
+
+int compile_propagator(mcdp_propagator &P, int32_t n_events, const double *earliest, int32_t n_acts,
+                       const int32_t *act_idx, const double *act_dur, const int32_t *act_type, int32_t n_entries,
+                       const int32_t *entry_target, const int64_t *entry_off, const int32_t *pred_event,
+                       const int32_t *pred_act, double max_delay, const mcdp_generator &gen) {
+    // (1) validation, in the reference's order.
+    if (gen.dists.count(-1)) return fail(MCDP_E_RESERVED_TYPE, "Activity type -1 is reserved for no delay");
+    if (max_delay < 0.0) return fail(MCDP_E_MAX_DELAY, "max_delay must be non-negative");
+    if (n_events < 0 || n_acts < 0 || n_entries < 0) return fail(MCDP_E_INVALID, "negative count");
+    const int32_t E = n_events;
+
+    // (2) distributions.
+    std::map<int32_t, int32_t> type_to_dist;
+    for (const auto &kv : gen.dists) {
+        DistRec D;
+        int rc = compile_dist(kv.second, D, P.alias);
+        if (rc) return fail(rc, g_last_error + " (activity_type " + std::to_string(kv.first) + ")");
+        type_to_dist[kv.first] = (int32_t)P.dist.size();
+        P.dist.push_back(D);
+    }
+    if (P.dist.size() >= kNoDist) return fail(MCDP_E_INVALID, "too many distributions");
+
+    // (3) link table.
+    int64_t max_idx = -1;
+    for (int32_t i = 0; i < n_acts; ++i) {
+        if (act_idx[i] < 0) return fail(MCDP_E_INVALID, "activity index must be non-negative");
+        max_idx = std::max<int64_t>(max_idx, act_idx[i]);
+    }
+    const int32_t L = (int32_t)(max_idx + 1);
+    std::vector<double> dmin((size_t)L, 0.0);
+    std::vector<int32_t> ltype((size_t)L, -1);
+    std::vector<uint8_t> seen((size_t)L, 0);
+    for (int32_t i = 0; i < n_acts; ++i) {
+        const int32_t l = act_idx[i];
+        if (seen[(size_t)l] && (std::memcmp(&dmin[(size_t)l], &act_dur[i], sizeof(double)) != 0 || ltype[(size_t)l] != act_type[i]))
+            return fail(MCDP_E_INVALID, "two activities share index " + std::to_string(l) + " with different attributes");
+        seen[(size_t)l] = 1;
+        dmin[(size_t)l] = act_dur[i];
+        ltype[(size_t)l] = act_type[i];
+    }
+    std::vector<uint32_t> ldist((size_t)L, kNoDist);
+    for (int32_t l = 0; l < L; ++l) {
+        auto it = type_to_dist.find(ltype[(size_t)l]);
+        if (it != type_to_dist.end()) ldist[(size_t)l] = (uint32_t)it->second;
+    }
+
+    // (4) topology.
+    if (n_entries > 0 && entry_off[0] != 0) return fail(MCDP_E_INVALID, "entry_offsets[0] must be 0");
+    for (int32_t i = 0; i < n_entries; ++i) {
+        if (entry_off[i + 1] < entry_off[i]) return fail(MCDP_E_INVALID, "entry_offsets must be non-decreasing");
+        if (entry_target[i] < 0 || entry_target[i] >= E) return fail(MCDP_E_INVALID, "precedence target out of range");
+        for (int64_t k = entry_off[i]; k < entry_off[i + 1]; ++k)
+            if (pred_event[k] < 0 || pred_event[k] >= E) return fail(MCDP_E_INVALID, "predecessor event out of range");
+    }
+    std::vector<int32_t> win((size_t)E, -1), indeg((size_t)E, 0);
+    std::vector<int64_t> adj_off((size_t)E + 1, 0);
+    for (int32_t i = 0; i < n_entries; ++i) {
+        win[(size_t)entry_target[i]] = i;
+        indeg[(size_t)entry_target[i]] = (int32_t)(entry_off[i + 1] - entry_off[i]);
+        for (int64_t k = entry_off[i]; k < entry_off[i + 1]; ++k) adj_off[(size_t)pred_event[k] + 1]++;
+    }
+    for (int32_t i = 0; i < E; ++i) adj_off[(size_t)i + 1] += adj_off[(size_t)i];
+    std::vector<int32_t> adj((size_t)adj_off[(size_t)E]);
+    {
+        std::vector<int64_t> fill(adj_off.begin(), adj_off.end() - 1);
+        for (int32_t i = 0; i < n_entries; ++i)
+            for (int64_t k = entry_off[i]; k < entry_off[i + 1]; ++k)
+                adj[(size_t)fill[(size_t)pred_event[k]]++] = entry_target[i];
+    }
+    std::vector<int32_t> order;
+    order.reserve((size_t)E);
+    {
+        std::vector<int32_t> q;
+        q.reserve((size_t)E);
+        for (int32_t i = 0; i < E; ++i)
+            if (indeg[(size_t)i] == 0) q.push_back(i);
+        size_t head = 0;
+        while (head < q.size()) {
+            const int32_t n = q[head++];
+            order.push_back(n);
+            for (int64_t k = adj_off[(size_t)n]; k < adj_off[(size_t)n + 1]; ++k)
+                if (--indeg[(size_t)adj[(size_t)k]] == 0) q.push_back(adj[(size_t)k]);
+        }
+    }
+    if ((int32_t)order.size() != E) return fail(MCDP_E_CYCLE, "Invalid DAG: cycle detected in precedence list");
+
+    // (5) levels.  A predecessor evaluated after its consumer in the
+    // reference's order is read before it is written there, i.e. at its
+    // earliest time ("stale"); every other predecessor is a true dependency.
+    std::vector<int32_t> pos((size_t)E), level((size_t)E, 0);
+    for (int32_t i = 0; i < E; ++i) pos[(size_t)order[(size_t)i]] = i;
+    int32_t n_levels = E > 0 ? 1 : 0;
+    for (int32_t e : order) {
+        const int32_t w = win[(size_t)e];
+        if (w < 0) continue;
+        int32_t lv = 0;
+        for (int64_t k = entry_off[w]; k < entry_off[w + 1]; ++k) {
+            const int32_t s = pred_event[k];
+            if (pos[(size_t)s] < pos[(size_t)e]) lv = std::max(lv, level[(size_t)s] + 1);
+        }
+        level[(size_t)e] = lv;
+        n_levels = std::max(n_levels, lv + 1);
+    }
+
+    // (6) steps and wave assignment.
+    std::vector<std::vector<int32_t>> by_level((size_t)n_levels);
+    for (int32_t e : order) by_level[(size_t)level[(size_t)e]].push_back(e);
+    size_t max_width = 0;
+    for (auto &v : by_level) max_width = std::max(max_width, v.size());
+    int32_t W = 1;
+    while (W < 8 && (size_t)W * 4 < max_width) W *= 2;
+    const int32_t K = kMaxEventsPerWave;
+    P.waves = W;
+
+    auto event_cost = [&](int32_t e) {
+        int c = 1;
+        const int32_t w = win[(size_t)e];
+        if (w < 0) return c;
+        for (int64_t k = entry_off[w]; k < entry_off[w + 1]; ++k) {
+            const int32_t l = pred_act[k];
+            const DistRec *D = (l >= 0 && l < L && ldist[(size_t)l] != kNoDist) ? &P.dist[ldist[(size_t)l]] : nullptr;
+            c += 1 + draw_cost(D);
+        }
+        return c;
+    };
+
+    std::vector<int32_t> ev_step((size_t)E, -1);
+    std::vector<std::vector<int32_t>> step_waves;  // flattened: per step, per wave, events
+    P.step_off.assign(1, 0);
+    std::vector<int32_t> flat_events;
+    int32_t step = 0;
+    for (auto &lev : by_level) {
+        for (size_t c0 = 0; c0 < lev.size(); c0 += (size_t)W * K) {
+            const size_t c1 = std::min(lev.size(), c0 + (size_t)W * K);
+            std::vector<std::pair<int, int32_t>> items;
+            for (size_t i = c0; i < c1; ++i) items.emplace_back(event_cost(lev[i]), lev[i]);
+            std::stable_sort(items.begin(), items.end(), [](auto &a, auto &b) { return a.first > b.first; });
+            std::vector<std::vector<int32_t>> per((size_t)W);
+            std::vector<int64_t> load((size_t)W, 0);
+            for (auto &it : items) {
+                int best = -1;
+                for (int w = 0; w < W; ++w)
+                    if ((int)per[(size_t)w].size() < K && (best < 0 || load[(size_t)w] < load[(size_t)best])) best = w;
+                per[(size_t)best].push_back(it.second);
+                load[(size_t)best] += it.first;
+            }
+            for (int w = 0; w < W; ++w) {
+                for (int32_t e : per[(size_t)w]) {
+                    flat_events.push_back(e);
+                    ev_step[(size_t)e] = step;
+                }
+                P.max_per_wave = std::max<int32_t>(P.max_per_wave, (int32_t)per[(size_t)w].size());
+                P.step_off.push_back((int32_t)flat_events.size());
+            }
+            ++step;
+        }
+    }
+    P.n_steps = step;
+
+    // (7) LDS slots: an event read by a later step occupies a slot from the
+    // commit of its step to the end of the last step that reads it.
+    std::vector<int32_t> last_use((size_t)E, -1);
+    for (int32_t e = 0; e < E; ++e) {
+        const int32_t w = win[(size_t)e];
+        if (w < 0) continue;
+        for (int64_t k = entry_off[w]; k < entry_off[w + 1]; ++k) {
+            const int32_t s = pred_event[k];
+            if (pos[(size_t)s] < pos[(size_t)e]) last_use[(size_t)s] = std::max(last_use[(size_t)s], ev_step[(size_t)e]);
+        }
+    }
+    const int32_t slot_cap = (int32_t)(kLdsBudgetBytes / (kLanes * sizeof(double)));
+    std::vector<int32_t> slot_of((size_t)E, -1);
+    {
+        std::vector<std::vector<int32_t>> release((size_t)std::max(P.n_steps, 1));
+        std::vector<int32_t> free_slots;
+        int32_t used = 0;
+        for (int32_t t = 0; t < P.n_steps; ++t) {
+            for (int32_t e : release[(size_t)t]) free_slots.push_back(slot_of[(size_t)e]);
+            const int32_t b = P.step_off[(size_t)t * W], en = P.step_off[(size_t)(t + 1) * W];
+            for (int32_t i = b; i < en; ++i) {
+                const int32_t e = flat_events[(size_t)i];
+                if (last_use[(size_t)e] < 0) continue;
+                int32_t s = -1;
+                if (!free_slots.empty()) {
+                    s = free_slots.back();
+                    free_slots.pop_back();
+                } else if (used < slot_cap) {
+                    s = used++;
+                }
+                slot_of[(size_t)e] = s;
+                if (s >= 0) release[(size_t)last_use[(size_t)e]].push_back(e);
+            }
+        }
+        P.slots = used;
+    }
+
+    // (8) records.
+    P.ev.reserve(flat_events.size());
+    std::vector<uint8_t> referenced((size_t)L, 0);
+    for (int32_t e : flat_events) {
+        EventRec er{};
+        er.earliest = earliest[e];
+        er.upper = earliest[e] + max_delay;
+        er.event = e;
+        er.slot = slot_of[(size_t)e];
+        er.pred_begin = (int32_t)P.pr.size();
+        const int32_t w = win[(size_t)e];
+        if (w >= 0) {
+            for (int64_t k = entry_off[w]; k < entry_off[w + 1]; ++k) {
+                PredRec r{};
+                const int32_t s = pred_event[k], l = pred_act[k];
+                r.src_event = s;
+                r.src_value = earliest[s];
+                if (pos[(size_t)s] < pos[(size_t)e]) {
+                    r.src_slot = slot_of[(size_t)s] >= 0 ? slot_of[(size_t)s] : kSrcGlobal;
+                    if (r.src_slot == kSrcGlobal) ++P.n_global;
+                } else {
+                    r.src_slot = kSrcStale;
+                    ++P.n_stale;
+                }
+                r.edge = l;
+                if (l >= 0 && l < L) {
+                    r.info = ldist[(size_t)l] | kInfoEdgeValid;
+                    r.dmin = dmin[(size_t)l];
+                    if (!referenced[(size_t)l]) {
+                        referenced[(size_t)l] = 1;
+                        r.info |= kInfoOwner;
+                    }
+                } else {
+                    r.info = kNoDist;  // undefined in the reference: contributes 0.0
+                    r.dmin = 0.0;
+                }
+                P.pr.push_back(r);
+            }
+        }
+        er.pred_count = (int32_t)P.pr.size() - er.pred_begin;
+        P.ev.push_back(er);
+    }
+    for (int32_t l = 0; l < L; ++l)
+        if (!referenced[(size_t)l]) P.orph.push_back(OrphanRec{l, ldist[(size_t)l], dmin[(size_t)l]});
+    if (P.pr.size() > (size_t)std::numeric_limits<int32_t>::max())
+        return fail(MCDP_E_INVALID, "too many predecessor entries");
+
+    P.E = E;
+    P.L = L;
+    return MCDP_OK;
+}
+
+// Uploads the plan of `P` to the current device (once per device).
+int ensure_device(mcdp_propagator &P, DeviceCopy *&out) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return fail(MCDP_E_DEVICE, "no HIP device available: the Monte Carlo engine runs on the GPU only");
+    int device = 0;
+    HIP_TRY(hipGetDevice(&device), "hipGetDevice");
+    for (auto &d : P.dev)
+        if (d.device == device) {
+            out = &d;
+            return MCDP_OK;
+        }
+    const size_t a = 256;
+    size_t off = 0;
+    const size_t o_step = off;
+    off = align_up(off + P.step_off.size() * sizeof(int32_t), a);
+    const size_t o_ev = off;
+    off = align_up(off + P.ev.size() * sizeof(EventRec), a);
+    const size_t o_pr = off;
+    off = align_up(off + P.pr.size() * sizeof(PredRec), a);
+    const size_t o_or = off;
+    off = align_up(off + P.orph.size() * sizeof(OrphanRec), a);
+    const size_t o_di = off;
+    off = align_up(off + P.dist.size() * sizeof(DistRec), a);
+    const size_t o_al = off;
+    off = align_up(off + P.alias.size() * sizeof(AliasRec), a);
+    const size_t total = std::max<size_t>(off, a);
+    std::vector<unsigned char> host(total, 0);
+    auto put = [&](size_t o, const void *src, size_t bytes) {
+        if (bytes) std::memcpy(host.data() + o, src, bytes);
+    };
+    put(o_step, P.step_off.data(), P.step_off.size() * sizeof(int32_t));
+    put(o_ev, P.ev.data(), P.ev.size() * sizeof(EventRec));
+    put(o_pr, P.pr.data(), P.pr.size() * sizeof(PredRec));
+    put(o_or, P.orph.data(), P.orph.size() * sizeof(OrphanRec));
+    put(o_di, P.dist.data(), P.dist.size() * sizeof(DistRec));
+    put(o_al, P.alias.data(), P.alias.size() * sizeof(AliasRec));
+    void *blob = nullptr;
+    hipError_t e = hipMalloc(&blob, total);
+    if (e != hipSuccess) return fail(MCDP_E_NOMEM, std::string("hipMalloc (plan): ") + hipGetErrorString(e));
+    e = hipMemcpy(blob, host.data(), total, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(blob);
+        return hip_fail(e, "hipMemcpy (plan)");
+    }
+    DeviceCopy d;
+    d.device = device;
+    d.blob = blob;
+    auto *b = static_cast<unsigned char *>(blob);
+    d.base.step_off = reinterpret_cast<const int32_t *>(b + o_step);
+    d.base.ev = reinterpret_cast<const EventRec *>(b + o_ev);
+    d.base.pr = reinterpret_cast<const PredRec *>(b + o_pr);
+    d.base.orph = reinterpret_cast<const OrphanRec *>(b + o_or);
+    d.base.dist = reinterpret_cast<const DistRec *>(b + o_di);
+    d.base.alias = reinterpret_cast<const AliasRec *>(b + o_al);
+    d.base.n_steps = P.n_steps;
+    d.base.n_orphans = (int32_t)P.orph.size();
+    d.base.waves = P.waves;
+    P.dev.push_back(d);
+    out = &P.dev.back();
+    return MCDP_OK;
+}
+
+int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double *dur_in, int64_t n, int64_t ld,
+           double *realized, double *durations, int32_t *cause, hipStream_t stream) {
+    if (n < 0) return fail(MCDP_E_INVALID, "n must be non-negative");
+    if (n > 0 && ld < n) return fail(MCDP_E_INVALID, "ld must be >= n");
+    if (injected && !dur_in && P.L > 0 && n > 0) return fail(MCDP_E_INVALID, "durations input is null");
+    if (!injected && !seeds && n > 0) return fail(MCDP_E_INVALID, "seeds is null");
+    if (!realized && P.n_global > 0 && n > 0)
+        return fail(MCDP_E_INVALID, "this context reads predecessors from the realized output; realized must not be null");
+    std::lock_guard<std::mutex> lock(P.mu);
+    DeviceCopy *d = nullptr;
+    int rc = ensure_device(P, d);
+    if (rc) return rc;
+    if (n == 0 || P.E == 0 && (injected || P.orph.empty())) return MCDP_OK;
+    WalkParams p = d->base;
+    p.seeds = seeds;
+    p.dur_in = dur_in;
+    p.n = n;
+    p.ld = ld;
+    p.n_tiles = (n + kLanes - 1) / kLanes;
+    p.realized = realized;
+    p.durations = injected ? nullptr : durations;
+    p.cause = cause;
+    const size_t lds = (size_t)P.slots * kLanes * sizeof(double);
+    int cus = 0, bpc = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device), "hipDeviceGetAttribute");
+    HIP_TRY(walk_occupancy(injected, P.waves, lds, &bpc), "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    if (bpc < 1) return fail(MCDP_E_DEVICE, "walk kernel cannot be resident with " + std::to_string(lds) + " bytes of LDS");
+    const int64_t grid = std::min<int64_t>(p.n_tiles, (int64_t)cus * bpc);
+    HIP_TRY(launch_walk(p, injected, (int)grid, lds, stream), "walk kernel launch");
+    return MCDP_OK;
+}
+
+// Host-buffer convenience: runs in chunks of samples so device memory stays
+// bounded, and copies each chunk into the caller's event-major arrays.
+int run_host(mcdp_propagator &P, bool injected, const int32_t *seeds, const double *dur_in, int64_t n,
+             double *realized, double *durations, int32_t *cause) {
+    if (n < 0) return fail(MCDP_E_INVALID, "n must be non-negative");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return fail(MCDP_E_DEVICE, "no HIP device available: the Monte Carlo engine runs on the GPU only");
+    if (n == 0) return MCDP_OK;
+    const int64_t E = P.E, L = P.L;
+    const int64_t per_sample = 12 * E + 8 * L + 8;
+    int64_t chunk = std::max<int64_t>(kLanes, ((int64_t)1 << 31) / std::max<int64_t>(per_sample, 1));
+    chunk = std::min<int64_t>(n, chunk / kLanes * kLanes);
+    double *d_real = nullptr, *d_dur = nullptr;
+    int32_t *d_cause = nullptr, *d_seeds = nullptr;
+    auto cleanup = [&]() {
+        if (d_real) (void)hipFree(d_real);
+        if (d_dur) (void)hipFree(d_dur);
+        if (d_cause) (void)hipFree(d_cause);
+        if (d_seeds) (void)hipFree(d_seeds);
+    };
+    auto alloc = [&](void **ptr, size_t bytes) -> int {
+        if (bytes == 0) return MCDP_OK;
+        hipError_t e = hipMalloc(ptr, bytes);
+        if (e != hipSuccess) return fail(MCDP_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        return MCDP_OK;
+    };
+    int rc = MCDP_OK;
+    if ((rc = alloc((void **)&d_real, sizeof(double) * E * chunk)) ||
+        (rc = alloc((void **)&d_cause, sizeof(int32_t) * E * chunk)) ||
+        (rc = alloc((void **)&d_dur, sizeof(double) * L * chunk)) ||
+        (!injected && (rc = alloc((void **)&d_seeds, sizeof(int32_t) * chunk)))) {
+        cleanup();
+        return rc;
+    }
+    for (int64_t c0 = 0; c0 < n && rc == MCDP_OK; c0 += chunk) {
+        const int64_t m = std::min(chunk, n - c0);
+        hipError_t e = hipSuccess;
+        if (injected) {
+            if (L > 0)
+                e = hipMemcpy2D(d_dur, sizeof(double) * chunk, dur_in + c0, sizeof(double) * n, sizeof(double) * m,
+                                (size_t)L, hipMemcpyHostToDevice);
+        } else {
+            e = hipMemcpy(d_seeds, seeds + c0, sizeof(int32_t) * m, hipMemcpyHostToDevice);
+        }
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "hipMemcpy (inputs)");
+            break;
+        }
+        rc = launch(P, injected, d_seeds, d_dur, m, chunk, d_real, injected ? nullptr : d_dur, d_cause, nullptr);
+        if (rc) break;
+        e = hipDeviceSynchronize();
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "walk kernel");
+            break;
+        }
+        auto copy_out = [&](void *dst, const void *src, size_t elem, int64_t rows) -> hipError_t {
+            if (!dst || rows == 0) return hipSuccess;
+            return hipMemcpy2D(static_cast<char *>(dst) + elem * c0, elem * n, src, elem * chunk, elem * m,
+                               (size_t)rows, hipMemcpyDeviceToHost);
+        };
+        if ((e = copy_out(realized, d_real, sizeof(double), E)) != hipSuccess ||
+            (e = copy_out(cause, d_cause, sizeof(int32_t), E)) != hipSuccess ||
+            (!injected && (e = copy_out(durations, d_dur, sizeof(double), L)) != hipSuccess))
+            rc = hip_fail(e, "hipMemcpy (outputs)");
+    }
+    cleanup();
+    return rc;
+}
+
+}  // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+const char *mcdp_last_error(void) { return g_last_error.c_str(); }
+int mcdp_abi_version(void) { return MCDP_ABI_VERSION; }
+
+int mcdp_generator_create(mcdp_generator **out) {
+    if (!out) return fail(MCDP_E_INVALID, "out is null");
+    *out = new (std::nothrow) mcdp_generator();
+    return *out ? MCDP_OK : fail(MCDP_E_NOMEM, "out of host memory");
+}
+void mcdp_generator_destroy(mcdp_generator *gen) { delete gen; }
+int mcdp_generator_set_seed(mcdp_generator *gen, int32_t) {
+    return gen ? MCDP_OK : fail(MCDP_E_INVALID, "generator is null");
+}
+int mcdp_generator_add_constant(mcdp_generator *gen, int32_t t, double factor) {
+    if (!gen) return fail(MCDP_E_INVALID, "generator is null");
+    DistSpec s;
+    s.kind = kApiConstant;
+    s.a = factor;
+    gen->dists[t] = s;
+    return MCDP_OK;
+}
+int mcdp_generator_add_exponential(mcdp_generator *gen, int32_t t, double lambda_, double max_scale) {
+    if (!gen) return fail(MCDP_E_INVALID, "generator is null");
+    DistSpec s;
+    s.kind = kApiExponential;
+    s.a = lambda_;
+    s.b = max_scale;
+    gen->dists[t] = s;
+    return MCDP_OK;
+}
+int mcdp_generator_add_gamma(mcdp_generator *gen, int32_t t, double shape, double scale, double max_scale) {
+    if (!gen) return fail(MCDP_E_INVALID, "generator is null");
+    DistSpec s;
+    s.kind = kApiGamma;
+    s.a = shape;
+    s.b = scale;
+    s.c = max_scale;
+    gen->dists[t] = s;
+    return MCDP_OK;
+}
+static int add_empirical(mcdp_generator *gen, int32_t t, const double *v, const double *w, int64_t n, ApiKind k) {
+    if (!gen) return fail(MCDP_E_INVALID, "generator is null");
+    if (n < 0 || (n > 0 && (!v || !w))) return fail(MCDP_E_INVALID, "invalid empirical table");
+    DistSpec s;
+    s.kind = k;
+    s.values.assign(v, v + n);
+    s.weights.assign(w, w + n);
+    gen->dists[t] = std::move(s);
+    return MCDP_OK;
+}
+int mcdp_generator_add_empirical_absolute(mcdp_generator *gen, int32_t t, const double *values, const double *weights,
+                                          int64_t n) {
+    return add_empirical(gen, t, values, weights, n, kApiEmpAbs);
+}
+int mcdp_generator_add_empirical_relative(mcdp_generator *gen, int32_t t, const double *factors,
+                                          const double *weights, int64_t n) {
+    return add_empirical(gen, t, factors, weights, n, kApiEmpRel);
+}
+
+int mcdp_propagator_create(int32_t n_events, const double *earliest, int32_t n_activities,
+                           const int32_t *activity_idx, const double *minimal_duration,
+                           const int32_t *activity_type, int32_t n_entries, const int32_t *entry_target,
+                           const int64_t *entry_offsets, const int32_t *pred_event, const int32_t *pred_activity,
+                           double max_delay, const mcdp_generator *gen, mcdp_propagator **out) {
+    if (!out) return fail(MCDP_E_INVALID, "out is null");
+    *out = nullptr;
+    if (!gen) return fail(MCDP_E_INVALID, "generator is null");
+    if ((n_events > 0 && !earliest) || (n_activities > 0 && (!activity_idx || !minimal_duration || !activity_type)) ||
+        (n_entries > 0 && (!entry_target || !entry_offsets)) ||
+        (n_entries > 0 && entry_offsets[n_entries] > 0 && (!pred_event || !pred_activity)))
+        return fail(MCDP_E_INVALID, "null input array");
+    try {
+        std::unique_ptr<mcdp_propagator> P(new mcdp_propagator());
+        int rc = compile_propagator(*P, n_events, earliest, n_activities, activity_idx, minimal_duration,
+                                    activity_type, n_entries, entry_target, entry_offsets, pred_event,
+                                    pred_activity, max_delay, *gen);
+        if (rc) return rc;
+        *out = P.release();
+        return MCDP_OK;
+    } catch (const std::bad_alloc &) {
+        return fail(MCDP_E_NOMEM, "out of host memory");
+    }
+}
+void mcdp_propagator_destroy(mcdp_propagator *prop) { delete prop; }
+int32_t mcdp_propagator_node_count(const mcdp_propagator *prop) { return prop ? prop->E : -1; }
+int32_t mcdp_propagator_activity_count(const mcdp_propagator *prop) { return prop ? prop->L : -1; }
+
+int mcdp_run_many_device(mcdp_propagator *prop, const int32_t *seeds_dev, int64_t n, int64_t ld,
+                         double *realized_dev, double *durations_dev, int32_t *cause_dev, void *stream) {
+    if (!prop) return fail(MCDP_E_INVALID, "propagator is null");
+    return launch(*prop, false, seeds_dev, nullptr, n, ld, realized_dev, durations_dev, cause_dev,
+                  static_cast<hipStream_t>(stream));
+}
+int mcdp_propagate_device(mcdp_propagator *prop, const double *durations_dev, int64_t n, int64_t ld,
+                          double *realized_dev, int32_t *cause_dev, void *stream) {
+    if (!prop) return fail(MCDP_E_INVALID, "propagator is null");
+    return launch(*prop, true, nullptr, durations_dev, n, ld, realized_dev, nullptr, cause_dev,
+                  static_cast<hipStream_t>(stream));
+}
+int mcdp_run_many_host(mcdp_propagator *prop, const int32_t *seeds, int64_t n, double *realized, double *durations,
+                       int32_t *cause) {
+    if (!prop) return fail(MCDP_E_INVALID, "propagator is null");
+    if (n > 0 && !seeds) return fail(MCDP_E_INVALID, "seeds is null");
+    return run_host(*prop, false, seeds, nullptr, n, realized, durations, cause);
+}
+int mcdp_propagate_host(mcdp_propagator *prop, const double *durations, int64_t n, double *realized,
+                        int32_t *cause) {
+    if (!prop) return fail(MCDP_E_INVALID, "propagator is null");
+    if (n > 0 && prop->L > 0 && !durations) return fail(MCDP_E_INVALID, "durations is null");
+    return run_host(*prop, true, nullptr, durations, n, realized, nullptr, cause);
+}
+
+int mcdp_propagator_plan_info(const mcdp_propagator *prop, int64_t *out, int n_out) {
+    if (!prop || !out) return fail(MCDP_E_INVALID, "null argument");
+    const int64_t v[8] = {prop->n_steps,
+                          prop->waves,
+                          prop->slots,
+                          (int64_t)prop->slots * kLanes * (int64_t)sizeof(double),
+                          prop->n_global,
+                          prop->n_stale,
+                          (int64_t)prop->orph.size(),
+                          prop->max_per_wave};
+    for (int i = 0; i < n_out && i < 8; ++i) out[i] = v[i];
+    return MCDP_OK;
+}
+
+// ---- device plumbing --------------------------------------------------------
+int mcdp_device_count(int *count) {
+    if (!count) return fail(MCDP_E_INVALID, "count is null");
+    *count = 0;
+    hipError_t e = hipGetDeviceCount(count);
+    if (e != hipSuccess) *count = 0;
+    return MCDP_OK;
+}
+int mcdp_malloc(void **ptr, size_t bytes) {
+    if (!ptr) return fail(MCDP_E_INVALID, "ptr is null");
+    hipError_t e = hipMalloc(ptr, bytes);
+    return e == hipSuccess ? MCDP_OK : fail(MCDP_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+}
+int mcdp_free(void *ptr) {
+    HIP_TRY(hipFree(ptr), "hipFree");
+    return MCDP_OK;
+}
+int mcdp_memcpy_htod(void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy HtoD");
+    return MCDP_OK;
+}
+int mcdp_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy DtoH");
+    return MCDP_OK;
+}
+int mcdp_memset(void *dst, int value, size_t bytes) {
+    HIP_TRY(hipMemset(dst, value, bytes), "hipMemset");
+    return MCDP_OK;
+}
+int mcdp_stream_create(void **stream) {
+    if (!stream) return fail(MCDP_E_INVALID, "stream is null");
+    hipStream_t s = nullptr;
+    HIP_TRY(hipStreamCreate(&s), "hipStreamCreate");
+    *stream = s;
+    return MCDP_OK;
+}
+int mcdp_stream_destroy(void *stream) {
+    HIP_TRY(hipStreamDestroy(static_cast<hipStream_t>(stream)), "hipStreamDestroy");
+    return MCDP_OK;
+}
+int mcdp_stream_synchronize(void *stream) {
+    HIP_TRY(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "hipStreamSynchronize");
+    return MCDP_OK;
+}
+int mcdp_event_create(void **event) {
+    if (!event) return fail(MCDP_E_INVALID, "event is null");
+    hipEvent_t e = nullptr;
+    HIP_TRY(hipEventCreate(&e), "hipEventCreate");
+    *event = e;
+    return MCDP_OK;
+}
+int mcdp_event_destroy(void *event) {
+    HIP_TRY(hipEventDestroy(static_cast<hipEvent_t>(event)), "hipEventDestroy");
+    return MCDP_OK;
+}
+int mcdp_event_record(void *event, void *stream) {
+    HIP_TRY(hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(stream)), "hipEventRecord");
+    return MCDP_OK;
+}
+int mcdp_event_elapsed_ms(void *start, void *stop, float *ms) {
+    if (!ms) return fail(MCDP_E_INVALID, "ms is null");
+    HIP_TRY(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop)),
+            "hipEventElapsedTime");
+    return MCDP_OK;
+}
+
+int mcdp_debug_philox(int variant, int64_t n, const uint32_t *ctr, const uint32_t *key, uint32_t *out) {
+    if (variant != 2 && variant != 4) return fail(MCDP_E_INVALID, "variant must be 2 or 4");
+    if (n < 0 || (n > 0 && (!ctr || !key || !out))) return fail(MCDP_E_INVALID, "invalid arguments");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(MCDP_E_DEVICE, "no HIP device available");
+    if (n == 0) return MCDP_OK;
+    const size_t cw = (size_t)variant, kw = (size_t)(variant / 2);
+    uint32_t *d = nullptr;
+    const size_t bytes = sizeof(uint32_t) * (size_t)n * (cw + kw + cw);
+    HIP_TRY(hipMalloc(&d, bytes), "hipMalloc");
+    uint32_t *dc = d, *dk = d + (size_t)n * cw, *dout = dk + (size_t)n * kw;
+    hipError_t e = hipMemcpy(dc, ctr, sizeof(uint32_t) * n * cw, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dk, key, sizeof(uint32_t) * n * kw, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_philox_debug(variant, n, dc, dk, dout, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(out, dout, sizeof(uint32_t) * n * cw, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "philox debug");
+    return MCDP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,206 @@
+// Counter-based delay sampling on gfx950.
+//
+// The reference draws every delay from one sequential Xoshiro256++ stream per
+// run (_core.cpp:313, _custom_rng.hpp:551-600) through libstdc++'s
+// exponential / gamma / discrete distributions (_core.cpp:78-141).  A GPU lane
+// cannot share a sequential stream, so each draw here is a pure function of
+// (seed, activity, attempt): Philox (Salmon, Moraes, Dror, Shaw, "Parallel
+// random numbers: as easy as 1, 2, 3", SC'11) keyed by the run's seed, with
+// the activity index and the attempt number as counter.  A duration therefore
+// does not depend on which lane, tile, batch or launch computes it, and
+// run_many(seeds)[i] equals run(seeds[i]) exactly.
+//
+// Distributions are matched in law, not stream: each method below samples
+// exactly the distribution the reference samples.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mcdp_plan.hpp"
+
+namespace mcdp {
+
+// ---- Philox -----------------------------------------------------------------
+// Multipliers and Weyl key increments as published with Random123.
+constexpr uint32_t kPhiloxM4a = 0xD2511F53u, kPhiloxM4b = 0xCD9E8D57u;
+constexpr uint32_t kPhiloxM2 = 0xD256D193u;
+constexpr uint32_t kPhiloxW0 = 0x9E3779B9u, kPhiloxW1 = 0xBB67AE85u;
+
+struct Words4 {
+    uint32_t x, y, z, w;
+};
+struct Words2 {
+    uint32_t x, y;
+};
+
+// Philox-4x32-10.  A 32x32->64 product is one v_mad_u64_u32.
+__device__ __forceinline__ Words4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                                uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += kPhiloxW0;
+            k1 += kPhiloxW1;
+        }
+        const uint64_t p0 = (uint64_t)kPhiloxM4a * c0;
+        const uint64_t p1 = (uint64_t)kPhiloxM4b * c2;
+        const uint32_t y0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t y2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = y0;
+        c2 = y2;
+    }
+    return Words4{c0, c1, c2, c3};
+}
+
+// Philox-2x32-10.
+__device__ __forceinline__ Words2 philox2x32_10(uint32_t c0, uint32_t c1, uint32_t k) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) k += kPhiloxW0;
+        const uint64_t p = (uint64_t)kPhiloxM2 * c0;
+        const uint32_t y0 = (uint32_t)(p >> 32) ^ k ^ c1;
+        c1 = (uint32_t)p;
+        c0 = y0;
+    }
+    return Words2{c0, c1};
+}
+
+// ---- uniforms ---------------------------------------------------------------
+// 53 random bits m = (hi >> 11) * 2^32 + lo; the fma is exact (m < 2^53).
+__device__ __forceinline__ double bits53(uint32_t hi, uint32_t lo) {
+    return __builtin_fma((double)(hi >> 11), 4294967296.0, (double)lo);
+}
+// (m + 1) 2^-53, in (0, 1]: safe argument for log.
+__device__ __forceinline__ double u01_open0(uint32_t hi, uint32_t lo) { return (bits53(hi, lo) + 1.0) * 0x1p-53; }
+// m 2^-53, in [0, 1).
+__device__ __forceinline__ double u01(uint32_t hi, uint32_t lo) { return bits53(hi, lo) * 0x1p-53; }
+// (w + 1/2) 2^-32, in (0, 1): 32-bit resolution, used for an angle and for
+// the acceptance test of Marsaglia-Tsang.
+__device__ __forceinline__ double u01_mid32(uint32_t w) { return ((double)w + 0.5) * 0x1p-32; }
+
+// Key of a run: the reference seeds from the C int seed converted to uint64
+// (sign extension, _custom_rng.hpp:570); the 64-bit key is that value.
+struct SeedKey {
+    uint32_t lo, hi;
+};
+__device__ __forceinline__ SeedKey seed_key(int32_t seed) {
+    return SeedKey{(uint32_t)seed, seed < 0 ? 0xFFFFFFFFu : 0u};
+}
+
+// ---- per-kind draws of the multiplier X (or the absolute extra) -------------
+
+// One Marsaglia-Tsang attempt for shape k' = d + 1/3 >= 1, scale 1
+// (Marsaglia & Tsang, ACM TOMS 26(3), 2000).  The normal comes from one
+// Box-Muller branch.  Returns false on rejection.
+__device__ __forceinline__ bool mt_attempt(const Words4 &w, double d, double c, double &x) {
+    const double u1 = u01_open0(w.x, w.y);
+    const double u2 = u01_mid32(w.z);
+    const double u3 = u01_mid32(w.w);
+    const double z = sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
+    double v = 1.0 + c * z;
+    if (!(v > 0.0)) return false;
+    v = v * v * v;
+    const double z2 = z * z;
+    x = d * v;
+    if (u3 < 1.0 - 0.0331 * (z2 * z2)) return true;
+    return log(u3) < 0.5 * z2 + d * (1.0 - v + log(v));
+}
+
+// Draws the extra delay of one activity for one sample.  `edge` is the
+// activity index, `dmin` its minimal duration, `D` its compiled distribution
+// (wave-uniform).  Returns base + extra rounded exactly as the reference
+// (_core.cpp:75, 88, 103, 125, 140, 328): extra = X * d (one rounding), then
+// d + extra (one rounding); this file is compiled with -ffp-contract=off.
+__device__ __forceinline__ double draw_duration(const DistRec &D, const AliasRec *__restrict__ alias, uint32_t edge,
+                                                double dmin, SeedKey key) {
+    switch (D.kind) {
+        case kConstant: {
+            const double extra = dmin * D.p[0];
+            return dmin + extra;
+        }
+        case kEmpiricalAbs1: return dmin + D.p[0];
+        case kEmpiricalRel1: {
+            const double extra = D.p[0] * dmin;
+            return dmin + extra;
+        }
+        case kExpInf: {
+            const Words2 w = philox2x32_10(edge, kTagMain, key.lo);
+            const double X = -D.p[0] * log(u01_open0(w.x, w.y));
+            const double extra = X * dmin;
+            return dmin + extra;
+        }
+        case kExpTrunc: {
+            // Inverse of the truncated CDF (1 - e^{-x/l}) / (1 - e^{-m/l}).
+            const Words2 w = philox2x32_10(edge, kTagMain, key.lo);
+            double X = -D.p[0] * log1p(-u01(w.x, w.y) * D.p[1]);
+            X = (X > D.p[2]) ? D.p[2] : X;
+            const double extra = X * dmin;
+            return dmin + extra;
+        }
+        case kEmpiricalAbs:
+        case kEmpiricalRel: {
+            // Vose alias table: one uniform picks the bin and the coin.
+            const Words2 w = philox2x32_10(edge, kTagMain, key.lo);
+            const double xn = u01(w.x, w.y) * (double)D.n;
+            int i = (int)xn;
+            i = (i >= D.n) ? D.n - 1 : i;
+            const double frac = xn - (double)i;
+            const AliasRec *rec = alias + D.table_off + i;
+            const double v = (frac < rec->prob) ? rec->v_self : rec->v_alias;
+            if (D.kind == kEmpiricalAbs) return dmin + v;
+            const double extra = v * dmin;
+            return dmin + extra;
+        }
+        default: break;
+    }
+    // Gamma family: rejection on the truncation bound max_scale (D.p[3]),
+    // with the attempt number in the counter so every lane's draw is fixed
+    // by (seed, edge) alone.
+    const double m = D.p[3];
+    double X = 0.0;
+    uint32_t j = 0;
+    for (;;) {
+        const Words4 w = philox4x32_10(edge, j, kTagMain, 0u, key.lo, key.hi);
+        bool ok = true;
+        switch (D.kind) {
+            case kGammaErlang2:  // sum of two unit exponentials
+                X = -D.p[0] * log(u01_open0(w.x, w.y) * u01_open0(w.z, w.w));
+                break;
+            case kGammaHalf: {  // Gamma(1/2) = N(0,1)^2 / 2
+                const double cs = cospi(2.0 * u01_mid32(w.z));
+                X = D.p[0] * (-log(u01_open0(w.x, w.y)) * (cs * cs));
+                break;
+            }
+            case kGammaMT: {
+                double y;
+                ok = mt_attempt(w, D.p[1], D.p[2], y);
+                if (ok) {
+                    if (D.boost) {  // shape < 1: Gamma(k) = Gamma(k + 1) U^{1/k}
+                        const Words2 b = philox2x32_10(edge, (j << 8) | kTagBoost, key.lo);
+                        y *= exp(log(u01_open0(b.x, b.y)) * D.p[4]);
+                    }
+                    X = D.p[0] * y;
+                }
+                break;
+            }
+            default: {  // kGammaPow: density x^{k-1} e^{-x} on [0, m'] by
+                        // proposal m' U^{1/k}, accepted with probability e^{-x}
+                const double y = D.p[1] * exp(log(u01_open0(w.x, w.y)) * D.p[2]);
+                ok = u01_open0(w.z, w.w) <= exp(-y);
+                X = (y * D.p[0] > m) ? m : y * D.p[0];
+                break;
+            }
+        }
+        if (ok && !(X > m)) break;
+        if (++j >= kMaxAttempts) {
+            X = m;
+            break;
+        }
+    }
+    const double extra = X * dmin;
+    return dmin + extra;
+}
+
+}  // namespace mcdp

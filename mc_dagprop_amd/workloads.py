@@ -1,0 +1,241 @@
+"""Flat DAG descriptions and the synthetic benchmark workloads.
+
+A :class:`FlatDag` holds a DAG context and a delay generator as plain numpy
+arrays: the form the C ABI (``include/mcdp.h``) takes.  It can also build the
+Python API objects (``DagContext``, ``GenericDelayGenerator``).
+
+The layered generators reproduce the benchmark workloads of the path:
+
+* ``layered_small``   -- 20 layers x 10 events, 1-3 predecessors from the previous
+  layer (~380 activities), 3 activity types (constant 1.5, exponential mean 60,
+  gamma(2, 30));
+* ``layered_network`` -- 200 layers x 100 events, 1-4 predecessors (~50,000
+  activities), 8 activity types (constant 1.5, exponential means 30 and 120,
+  gamma (2, 30) and (0.5, 200), 64-bin empirical absolute and relative tables on
+  [0, 900] s with Dirichlet(1) weights, one type without a distribution).
+
+Timestamps: minimal_duration ~ U[30, 600) s; earliest = critical-path time +
+U[0, 120) s slack; latest = earliest + 3600; actual = 0; max_delay = 1800.
+All randomness comes from one SplitMix64 stream (seed 20260000 + variant) so
+the workloads are identical on every machine and numpy version.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# Distribution kinds, in the order of the reference's DistVar alternatives
+# (_core.cpp:143).
+CONSTANT, EXPONENTIAL, GAMMA, EMPIRICAL_ABSOLUTE, EMPIRICAL_RELATIVE = range(5)
+
+
+@dataclass
+class DistSpec:
+    activity_type: int
+    kind: int
+    params: tuple[float, float, float] = (0.0, 0.0, float("inf"))
+    values: np.ndarray | None = None
+    weights: np.ndarray | None = None
+
+
+@dataclass
+class FlatDag:
+    earliest: np.ndarray
+    latest: np.ndarray
+    actual: np.ndarray
+    act_src: np.ndarray
+    act_dst: np.ndarray
+    act_idx: np.ndarray
+    act_dur: np.ndarray
+    act_type: np.ndarray
+    prec_target: np.ndarray
+    prec_off: np.ndarray
+    prec_src: np.ndarray
+    prec_edge: np.ndarray
+    max_delay: float
+    dists: list[DistSpec] = field(default_factory=list)
+
+    @property
+    def n_events(self) -> int:
+        return int(self.earliest.shape[0])
+
+    @property
+    def link_count(self) -> int:
+        return int(self.act_idx.max()) + 1 if self.act_idx.size else 0
+
+    # -- builders ---------------------------------------------------------
+    @staticmethod
+    def from_lists(events, activities, precedence_list, max_delay, dists=()) -> "FlatDag":
+        """events: [(earliest, latest, actual)]; activities: {(src, dst): (idx, dur, type)};
+        precedence_list: [(target, [(src, edge), ...])]."""
+        ev = np.asarray(events, dtype=np.float64).reshape(-1, 3)
+        keys = list(activities.keys())
+        vals = [activities[k] for k in keys]
+        off = [0]
+        src, edge = [], []
+        for _, preds in precedence_list:
+            for s, e in preds:
+                src.append(s)
+                edge.append(e)
+            off.append(len(src))
+        return FlatDag(
+            earliest=ev[:, 0].copy(),
+            latest=ev[:, 1].copy(),
+            actual=ev[:, 2].copy(),
+            act_src=np.asarray([k[0] for k in keys], dtype=np.int32),
+            act_dst=np.asarray([k[1] for k in keys], dtype=np.int32),
+            act_idx=np.asarray([v[0] for v in vals], dtype=np.int32),
+            act_dur=np.asarray([v[1] for v in vals], dtype=np.float64),
+            act_type=np.asarray([v[2] for v in vals], dtype=np.int32),
+            prec_target=np.asarray([t for t, _ in precedence_list], dtype=np.int32),
+            prec_off=np.asarray(off, dtype=np.int64),
+            prec_src=np.asarray(src, dtype=np.int32),
+            prec_edge=np.asarray(edge, dtype=np.int32),
+            max_delay=float(max_delay),
+            dists=list(dists),
+        )
+
+    def to_api(self, api):
+        """(DagContext, GenericDelayGenerator) built through module ``api``."""
+        events = [
+            api.Event(str(i), api.EventTimestamp(float(a), float(b), float(c)))
+            for i, (a, b, c) in enumerate(zip(self.earliest, self.latest, self.actual))
+        ]
+        acts = {
+            (int(s), int(d)): api.Activity(idx=int(i), minimal_duration=float(u), activity_type=int(t))
+            for s, d, i, u, t in zip(self.act_src, self.act_dst, self.act_idx, self.act_dur, self.act_type)
+        }
+        prec = []
+        for i, tgt in enumerate(self.prec_target):
+            a, b = int(self.prec_off[i]), int(self.prec_off[i + 1])
+            prec.append((int(tgt), [(int(s), int(e)) for s, e in zip(self.prec_src[a:b], self.prec_edge[a:b])]))
+        ctx = api.DagContext(events=events, activities=acts, precedence_list=prec, max_delay=self.max_delay)
+        return ctx, self.generator(api)
+
+    def generator(self, api):
+        gen = api.GenericDelayGenerator()
+        for d in self.dists:
+            a, b, c = d.params
+            if d.kind == CONSTANT:
+                gen.add_constant(d.activity_type, a)
+            elif d.kind == EXPONENTIAL:
+                gen.add_exponential(d.activity_type, a, b)
+            elif d.kind == GAMMA:
+                gen.add_gamma(d.activity_type, a, b, c)
+            elif d.kind == EMPIRICAL_ABSOLUTE:
+                gen.add_empirical_absolute(d.activity_type, list(map(float, d.values)), list(map(float, d.weights)))
+            else:
+                gen.add_empirical_relative(d.activity_type, list(map(float, d.values)), list(map(float, d.weights)))
+        return gen
+
+
+class SplitMix64:
+    """SplitMix64 (Steele, Lea, Flood 2014): the workload generators' only source of randomness."""
+
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed: int) -> None:
+        self.s = seed & self.MASK
+
+    def next(self) -> int:
+        self.s = (self.s + 0x9E3779B97F4A7C15) & self.MASK
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & self.MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & self.MASK
+        return z ^ (z >> 31)
+
+    def uniform(self, lo: float = 0.0, hi: float = 1.0) -> float:
+        return lo + (hi - lo) * ((self.next() >> 11) * 2.0**-53)
+
+    def below(self, n: int) -> int:
+        return min(int(((self.next() >> 11) * 2.0**-53) * n), n - 1)
+
+
+def _histogram_tables(rng: SplitMix64) -> tuple[np.ndarray, np.ndarray]:
+    centres = 7.03125 + 14.0625 * np.arange(64)
+    w = np.array([-np.log1p(-rng.uniform()) for _ in range(64)])  # Gamma(1) = Exp(1) => Dirichlet(1)
+    return centres, w / w.sum()
+
+
+def layered(layers: int, width: int, max_preds: int, n_types: int, variant: int, dists: list[DistSpec]) -> FlatDag:
+    rng = SplitMix64(20260000 + variant)
+    E = layers * width
+    earliest = np.zeros(E)
+    act_src, act_dst, act_dur, act_type = [], [], [], []
+    prec_target, prec_off, prec_src, prec_edge = [], [0], [], []
+    for j in range(width):
+        earliest[j] = rng.uniform(0.0, 120.0)
+    for layer in range(1, layers):
+        for j in range(width):
+            e = layer * width + j
+            k = 1 + rng.below(max_preds)
+            pool = list(range((layer - 1) * width, layer * width))
+            preds = []
+            for i in range(k):  # partial Fisher-Yates, draw order kept
+                r = i + rng.below(width - i)
+                pool[i], pool[r] = pool[r], pool[i]
+                preds.append(pool[i])
+            arrive = -np.inf
+            for p in preds:
+                idx = len(act_src)
+                dmin = rng.uniform(30.0, 600.0)
+                t = rng.below(n_types)
+                act_src.append(p)
+                act_dst.append(e)
+                act_dur.append(dmin)
+                act_type.append(t)
+                prec_src.append(p)
+                prec_edge.append(idx)
+                arrive = max(arrive, earliest[p] + dmin)
+            earliest[e] = arrive + rng.uniform(0.0, 120.0)
+            prec_target.append(e)
+            prec_off.append(len(prec_src))
+    A = len(act_src)
+    return FlatDag(
+        earliest=earliest,
+        latest=earliest + 3600.0,
+        actual=np.zeros(E),
+        act_src=np.asarray(act_src, dtype=np.int32),
+        act_dst=np.asarray(act_dst, dtype=np.int32),
+        act_idx=np.arange(A, dtype=np.int32),
+        act_dur=np.asarray(act_dur),
+        act_type=np.asarray(act_type, dtype=np.int32),
+        prec_target=np.asarray(prec_target, dtype=np.int32),
+        prec_off=np.asarray(prec_off, dtype=np.int64),
+        prec_src=np.asarray(prec_src, dtype=np.int32),
+        prec_edge=np.asarray(prec_edge, dtype=np.int32),
+        max_delay=1800.0,
+        dists=dists,
+    )
+
+
+INF = float("inf")
+
+
+def layered_small() -> FlatDag:
+    """200 events / ~380 activities, 3 activity types (the CPU-sized workload)."""
+    dists = [
+        DistSpec(0, CONSTANT, (1.5, 0.0, INF)),
+        DistSpec(1, EXPONENTIAL, (60.0, INF, INF)),
+        DistSpec(2, GAMMA, (2.0, 30.0, INF)),
+    ]
+    return layered(20, 10, 3, 3, 0, dists)
+
+
+def layered_network(layers: int = 200, width: int = 100) -> FlatDag:
+    """20,000 events / ~50,000 activities, 8 activity types (the network-scale workload)."""
+    rng = SplitMix64(20260001 + 1000)
+    v_abs, w_abs = _histogram_tables(rng)
+    v_rel, w_rel = _histogram_tables(rng)
+    dists = [
+        DistSpec(0, CONSTANT, (1.5, 0.0, INF)),
+        DistSpec(1, EXPONENTIAL, (30.0, INF, INF)),
+        DistSpec(2, EXPONENTIAL, (120.0, INF, INF)),
+        DistSpec(3, GAMMA, (2.0, 30.0, INF)),
+        DistSpec(4, GAMMA, (0.5, 200.0, INF)),
+        DistSpec(5, EMPIRICAL_ABSOLUTE, values=v_abs, weights=w_abs),
+        DistSpec(6, EMPIRICAL_RELATIVE, values=v_rel, weights=w_rel),
+    ]  # type 7 has no distribution: duration = minimal_duration
+    return layered(layers, width, 4, 8, 1, dists)

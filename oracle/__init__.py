@@ -1,0 +1,282 @@
+"""CPU oracle of the Monte Carlo DAG propagation path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` import this package, and only as the checker.  The product
+(``mc_dagprop_amd``) never imports it.
+
+Parts (the C sources say, function by function, which reference lines they
+follow and what pins them):
+
+* :class:`OracleCtx` -- ``mcdp_oracle.c``: the reference's context compile
+  (_core.cpp:193-307) and propagation loop (_core.cpp:316-350) restated in C.
+  Pinned against the compiled reference (:class:`RefSim`) and the reference's
+  own test KATs.
+* :func:`philox4x32` / :func:`philox2x32` -- Philox-4x32-10 / -2x32-10, pinned by
+  the Random123 known-answer vectors.
+* :class:`RefSim` -- the reference engine itself, compiled from its sources by
+  ``oracle/Makefile`` into ``oracle/_ref/`` (the stream-exact durations of
+  Xoshiro256++ + libstdc++ come only from here).
+* Closed forms of the distributions the reference samples (_core.cpp:72-141):
+  truncated exponential and gamma CDFs, the empirical pmf, and the
+  no-randomness durations (constant / no distribution).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ORACLE_LIB = HERE / "_build" / "liboracle_mcdp.so"
+REF_LIB = HERE / "_ref" / "libref_mcdp.so"
+
+_P = C.c_void_p
+_pi32, _pi64, _pf64, _pu32 = (C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_uint32))
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+_olib = None
+
+
+def oracle_lib() -> C.CDLL:
+    global _olib
+    if _olib is None:
+        if not ORACLE_LIB.exists():
+            raise FileNotFoundError(f"{ORACLE_LIB} missing: run `make -C oracle oracle`")
+        L = C.CDLL(str(ORACLE_LIB))
+        L.orc_compile.restype = C.c_int
+        L.orc_compile.argtypes = [
+            C.c_int, _pf64, C.c_int, _pi32, _pf64, _pi32, C.c_int, _pi32, _pi64, _pi32, _pi32,
+            C.c_double, C.c_int, _pi32, C.POINTER(_P), C.c_char_p, C.c_int,
+        ]
+        L.orc_free.argtypes = [_P]
+        L.orc_event_count.argtypes = [_P]
+        L.orc_link_count.argtypes = [_P]
+        L.orc_get_order.argtypes = [_P, _pi32]
+        L.orc_base_durations.argtypes = [_P, C.c_int, _pi32, _pi32, _pf64, _pf64]
+        L.orc_propagate.argtypes = [_P, _pf64, C.c_int64, C.c_int64, _pf64, _pi32]
+        L.orc_philox4x32_many.argtypes = [C.c_int, C.c_int64, _pu32, _pu32, _pu32]
+        L.orc_philox2x32_many.argtypes = [C.c_int, C.c_int64, _pu32, _pu32, _pu32]
+        _olib = L
+    return _olib
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code: int, msg: str) -> None:
+        super().__init__(msg)
+        self.code = code
+
+
+class OracleCtx:
+    """The reference's compiled context (Kahn order + CSR) and its propagation loop, in C."""
+
+    def __init__(self, dag) -> None:
+        L = oracle_lib()
+        self.dag = dag
+        reg = np.asarray(sorted({d.activity_type for d in dag.dists}), dtype=np.int32)
+        self._keep = [
+            np.ascontiguousarray(dag.earliest, dtype=np.float64),
+            np.ascontiguousarray(dag.act_idx, dtype=np.int32),
+            np.ascontiguousarray(dag.act_dur, dtype=np.float64),
+            np.ascontiguousarray(dag.act_type, dtype=np.int32),
+            np.ascontiguousarray(dag.prec_target, dtype=np.int32),
+            np.ascontiguousarray(dag.prec_off, dtype=np.int64),
+            np.ascontiguousarray(dag.prec_src, dtype=np.int32),
+            np.ascontiguousarray(dag.prec_edge, dtype=np.int32),
+            reg,
+        ]
+        e, ai, ad, at, pt, po, ps, pe, rg = self._keep
+        err = C.create_string_buffer(256)
+        self.h = _P(0)
+        rc = L.orc_compile(
+            e.size, _ptr(e, _pf64), ai.size, _ptr(ai, _pi32), _ptr(ad, _pf64), _ptr(at, _pi32), pt.size,
+            _ptr(pt, _pi32), _ptr(po, _pi64), _ptr(ps, _pi32), _ptr(pe, _pi32), float(dag.max_delay), rg.size,
+            _ptr(rg, _pi32), C.byref(self.h), err, 256,
+        )
+        if rc:
+            raise OracleError(rc, err.value.decode())
+        self.E = L.orc_event_count(self.h)
+        self.L = L.orc_link_count(self.h)
+
+    @property
+    def order(self) -> np.ndarray:
+        o = np.empty(self.E, dtype=np.int32)
+        oracle_lib().orc_get_order(self.h, _ptr(o, _pi32))
+        return o
+
+    def base_durations(self) -> np.ndarray:
+        """Durations of links that need no random draw (NaN where a draw is needed)."""
+        # Last registration per type wins (README.md:44-45).
+        last = {}
+        for d in self.dag.dists:
+            last[d.activity_type] = d
+        types = np.asarray(list(last.keys()), dtype=np.int32)
+        is_const = np.asarray([1 if d.kind == 0 else 0 for d in last.values()], dtype=np.int32)
+        factor = np.asarray([d.params[0] if d.kind == 0 else 0.0 for d in last.values()], dtype=np.float64)
+        out = np.empty(self.L)
+        oracle_lib().orc_base_durations(
+            self.h, types.size, _ptr(types, _pi32), _ptr(is_const, _pi32), _ptr(factor, _pf64), _ptr(out, _pf64)
+        )
+        return out
+
+    def propagate(self, durations: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+        """realized (n, E), cause (n, E) for injected durations (n, L) (any row stride)."""
+        d = np.asarray(durations, dtype=np.float64)
+        if d.ndim == 1:
+            d = d[None, :]
+        d = np.ascontiguousarray(d)
+        n = d.shape[0]
+        r = np.empty((n, self.E))
+        c = np.empty((n, self.E), dtype=np.int32)
+        oracle_lib().orc_propagate(self.h, _ptr(d, _pf64), d.shape[1], n, _ptr(r, _pf64), _ptr(c, _pi32))
+        return r, c
+
+    def __del__(self) -> None:  # pragma: no cover
+        try:
+            oracle_lib().orc_free(self.h)
+        except Exception:
+            pass
+
+
+def philox4x32(ctr: np.ndarray, key: np.ndarray, rounds: int = 10) -> np.ndarray:
+    ctr = np.ascontiguousarray(ctr, dtype=np.uint32).reshape(-1, 4)
+    key = np.ascontiguousarray(key, dtype=np.uint32).reshape(-1, 2)
+    out = np.empty_like(ctr)
+    oracle_lib().orc_philox4x32_many(rounds, ctr.shape[0], _ptr(ctr, _pu32), _ptr(key, _pu32), _ptr(out, _pu32))
+    return out
+
+
+def philox2x32(ctr: np.ndarray, key: np.ndarray, rounds: int = 10) -> np.ndarray:
+    ctr = np.ascontiguousarray(ctr, dtype=np.uint32).reshape(-1, 2)
+    key = np.ascontiguousarray(key, dtype=np.uint32).reshape(-1)
+    out = np.empty_like(ctr)
+    oracle_lib().orc_philox2x32_many(rounds, ctr.shape[0], _ptr(ctr, _pu32), _ptr(key, _pu32), _ptr(out, _pu32))
+    return out
+
+
+# ---- the compiled reference (oracle/_ref) ------------------------------------
+_rlib = None
+
+
+def ref_available() -> bool:
+    return REF_LIB.exists()
+
+
+def ref_lib() -> C.CDLL:
+    global _rlib
+    if _rlib is None:
+        if not REF_LIB.exists():
+            raise FileNotFoundError(f"{REF_LIB} missing: run `make -C oracle ref` where the reference sources exist")
+        L = C.CDLL(str(REF_LIB))
+        L.ref_sim_create.restype = _P
+        L.ref_sim_create.argtypes = [
+            C.c_int, _pf64, _pf64, _pf64, C.c_int, _pi32, _pi32, _pi32, _pf64, _pi32, C.c_int, _pi32, _pi64,
+            _pi32, _pi32, C.c_double, C.c_int, _pi32, _pi32, _pf64, _pi64, _pf64, _pf64, C.c_char_p, C.c_int,
+        ]
+        L.ref_sim_destroy.argtypes = [_P]
+        L.ref_sim_node_count.argtypes = [_P]
+        L.ref_sim_activity_count.argtypes = [_P]
+        L.ref_sim_run.argtypes = [_P, _pi32, C.c_int64, _pf64, _pf64, _pi32]
+        L.ref_run_many_timed.restype = C.c_double
+        L.ref_run_many_timed.argtypes = [_P, _pi32, C.c_int64, C.c_int]
+        _rlib = L
+    return _rlib
+
+
+class RefSim:
+    """The reference's own Simulator (compiled from its sources), driven through ctypes."""
+
+    def __init__(self, dag) -> None:
+        L = ref_lib()
+        tabs_v, tabs_w, off = [], [], [0]
+        types, kinds, params = [], [], []
+        for d in dag.dists:
+            types.append(d.activity_type)
+            kinds.append(d.kind)
+            params.extend(d.params)
+            if d.values is not None:
+                tabs_v.extend(np.asarray(d.values, dtype=np.float64))
+                tabs_w.extend(np.asarray(d.weights, dtype=np.float64))
+            off.append(len(tabs_v))
+        arrs = [
+            np.ascontiguousarray(dag.earliest, np.float64), np.ascontiguousarray(dag.latest, np.float64),
+            np.ascontiguousarray(dag.actual, np.float64), np.ascontiguousarray(dag.act_src, np.int32),
+            np.ascontiguousarray(dag.act_dst, np.int32), np.ascontiguousarray(dag.act_idx, np.int32),
+            np.ascontiguousarray(dag.act_dur, np.float64), np.ascontiguousarray(dag.act_type, np.int32),
+            np.ascontiguousarray(dag.prec_target, np.int32), np.ascontiguousarray(dag.prec_off, np.int64),
+            np.ascontiguousarray(dag.prec_src, np.int32), np.ascontiguousarray(dag.prec_edge, np.int32),
+            np.asarray(types, np.int32), np.asarray(kinds, np.int32), np.asarray(params or [0.0], np.float64),
+            np.asarray(off, np.int64), np.asarray(tabs_v or [0.0], np.float64), np.asarray(tabs_w or [0.0], np.float64),
+        ]
+        self._keep = arrs
+        (ea, la, ac, asr, ads, ai, ad, at, pt, po, ps, pe, ty, ki, pa, of, tv, tw) = arrs
+        err = C.create_string_buffer(512)
+        self.h = L.ref_sim_create(
+            ea.size, _ptr(ea, _pf64), _ptr(la, _pf64), _ptr(ac, _pf64), ai.size, _ptr(asr, _pi32), _ptr(ads, _pi32),
+            _ptr(ai, _pi32), _ptr(ad, _pf64), _ptr(at, _pi32), pt.size, _ptr(pt, _pi32), _ptr(po, _pi64),
+            _ptr(ps, _pi32), _ptr(pe, _pi32), float(dag.max_delay), ty.size, _ptr(ty, _pi32), _ptr(ki, _pi32),
+            _ptr(pa, _pf64), _ptr(of, _pi64), _ptr(tv, _pf64), _ptr(tw, _pf64), err, 512,
+        )
+        if not self.h:
+            raise RuntimeError(err.value.decode())
+        self.E = L.ref_sim_node_count(self.h)
+        self.L = L.ref_sim_activity_count(self.h)
+
+    def run(self, seeds) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """Simulator::run for each seed in order: realized (n,E), durations (n,L), cause (n,E)."""
+        s = np.ascontiguousarray(seeds, dtype=np.int32)
+        n = s.size
+        r = np.empty((n, self.E))
+        d = np.empty((n, self.L))
+        c = np.empty((n, self.E), dtype=np.int32)
+        ref_lib().ref_sim_run(self.h, _ptr(s, _pi32), n, _ptr(r, _pf64), _ptr(d, _pf64), _ptr(c, _pi32))
+        return r, d, c
+
+    def run_many_timed(self, seeds, threads: int = 1) -> float:
+        s = np.ascontiguousarray(seeds, dtype=np.int32)
+        return float(ref_lib().ref_run_many_timed(self.h, _ptr(s, _pi32), s.size, int(threads)))
+
+    def __del__(self) -> None:  # pragma: no cover
+        try:
+            ref_lib().ref_sim_destroy(self.h)
+        except Exception:
+            pass
+
+
+# ---- closed forms of the distributions (_core.cpp:72-141) ---------------------
+def cdf_exponential(x, lam: float, max_scale: float = np.inf):
+    """CDF of X ~ Exp(mean lam) conditioned on X <= max_scale (_core.cpp:78-90)."""
+    x = np.asarray(x, dtype=np.float64)
+    base = -np.expm1(-np.clip(x, 0, None) / lam)
+    if np.isinf(max_scale):
+        return base
+    return np.clip(base / -np.expm1(-max_scale / lam), 0.0, 1.0)
+
+
+def cdf_gamma(x, shape: float, scale: float, max_scale: float = np.inf):
+    """CDF of X ~ Gamma(shape, scale) conditioned on X <= max_scale (_core.cpp:92-105)."""
+    from scipy.special import gammainc
+
+    x = np.asarray(x, dtype=np.float64)
+    base = gammainc(shape, np.clip(x, 0, None) / scale)
+    if np.isinf(max_scale):
+        return base
+    return np.clip(base / gammainc(shape, max_scale / scale), 0.0, 1.0)
+
+
+def gamma_moments(shape: float, scale: float) -> tuple[float, float, float, float]:
+    """Mean, variance, skewness and excess kurtosis of Gamma(shape, scale)."""
+    return shape * scale, shape * scale**2, 2.0 / np.sqrt(shape), 6.0 / shape
+
+
+def pmf_empirical(weights) -> np.ndarray:
+    """P(bin i) = w_i / sum(w) (libstdc++ discrete_distribution, used at _core.cpp:121, 136)."""
+    w = np.asarray(weights, dtype=np.float64)
+    if w.size < 2:
+        return np.ones(max(w.size, 1))
+    return w / w.sum()

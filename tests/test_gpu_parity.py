@@ -1,0 +1,197 @@
+"""GPU parity: the gfx950 path against the oracle, bit for bit.
+
+Every test calls libmcdp through its C ABI (directly via ctypes, or through the
+pybind11 module, which is a thin layer over the same ABI).
+
+* Philox words: device == oracle, bit-exact.
+* Propagation with injected durations: device == the reference's own results
+  (golden fixtures) and == the oracle on edge-case DAGs and durations
+  (NaN, +-inf, -0.0, ties, caps, duplicate targets, index gaps, edges outside
+  the link range, wide levels that spill past LDS).
+* Fused sampling + propagation (run_many): the device's own durations, fed to the
+  oracle, reproduce the device's realized/cause bit-exactly; durations that need
+  no draw equal the oracle's closed form bit-exactly.
+"""
+
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import dags as D
+import oracle
+from mc_dagprop_amd import capi
+from mc_dagprop_amd.workloads import CONSTANT, EMPIRICAL_ABSOLUTE, DistSpec, FlatDag, layered_network, layered_small
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+GOLDEN_FILES = sorted(GOLDEN.glob("*.npz"))
+INF = float("inf")
+
+
+def bits(a: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(a).view(np.uint64)
+
+
+def test_device_philox_matches_oracle(gpu) -> None:
+    rng = np.random.default_rng(1)
+    n = 4096
+    ctr4 = rng.integers(0, 2**32, (n, 4), dtype=np.uint64).astype(np.uint32)
+    key4 = rng.integers(0, 2**32, (n, 2), dtype=np.uint64).astype(np.uint32)
+    ctr4[:3] = [[0, 0, 0, 0], [0xFFFFFFFF] * 4, [0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344]]
+    key4[:3] = [[0, 0], [0xFFFFFFFF] * 2, [0xA4093822, 0x299F31D0]]
+    np.testing.assert_array_equal(capi.debug_philox(4, ctr4, key4), oracle.philox4x32(ctr4, key4))
+    ctr2 = ctr4[:, :2].copy()
+    key2 = key4[:, 0].copy()
+    np.testing.assert_array_equal(capi.debug_philox(2, ctr2, key2), oracle.philox2x32(ctr2, key2))
+    assert tuple(capi.debug_philox(4, ctr4[:1], key4[:1])[0]) == (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)
+
+
+@pytest.mark.parametrize("path", GOLDEN_FILES, ids=[p.stem for p in GOLDEN_FILES])
+def test_injected_matches_reference_fixture(gpu, path: Path) -> None:
+    """Given the reference's own durations, device realized/cause equal the reference's, bit for bit."""
+    z = np.load(path)
+    dag = D.dag_from_npz(z)
+    P = capi.Propagator(dag)
+    r, c = P.propagate(z["durations"])
+    np.testing.assert_array_equal(bits(r), bits(z["realized"]))
+    np.testing.assert_array_equal(c, z["cause"])
+
+
+@pytest.mark.parametrize("path", GOLDEN_FILES, ids=[p.stem for p in GOLDEN_FILES])
+def test_run_many_consistent_on_fixture_dags(gpu, path: Path) -> None:
+    """Device durations -> oracle propagation reproduces device realized/cause; fixed durations match."""
+    z = np.load(path)
+    dag = D.dag_from_npz(z)
+    P = capi.Propagator(dag)
+    seeds = np.concatenate([z["seeds"], np.arange(100, 200)]).astype(np.int32)
+    r, d, c = P.run_many(seeds)
+    ctx = oracle.OracleCtx(dag)
+    rr, cc = ctx.propagate(d)
+    np.testing.assert_array_equal(bits(r), bits(rr))
+    np.testing.assert_array_equal(c, cc)
+    base = ctx.base_durations()
+    fixed = ~np.isnan(base)
+    np.testing.assert_array_equal(bits(d[:, fixed]), bits(np.broadcast_to(base[fixed], d[:, fixed].shape)))
+    # Realized times stay inside [earliest, earliest + max_delay] (_core.cpp:336, 348) unless inputs are NaN.
+    e = dag.earliest[None, :]
+    assert np.all((r >= e) & (r <= e + dag.max_delay))
+
+
+@pytest.mark.parametrize("case", range(12))
+def test_injected_special_values(gpu, case: int) -> None:
+    rng = np.random.default_rng(100 + case)
+    kw = [{}, {"duplicates": True}, {"gaps": True}, {"oob_edges": True}][case % 4]
+    dag = D.random_dag(rng, int(rng.integers(3, 150)), max_preds=int(rng.integers(1, 7)), **kw)
+    P = capi.Propagator(dag)
+    ctx = oracle.OracleCtx(dag)
+    n = int(rng.choice([1, 63, 64, 65, 200]))
+    dur = D.special_durations(rng, n, ctx.L)
+    r, c = P.propagate(dur)
+    rr, cc = ctx.propagate(dur)
+    np.testing.assert_array_equal(bits(r), bits(rr))
+    np.testing.assert_array_equal(c, cc)
+
+
+def _wide_dag(n_src: int, depth: int) -> FlatDag:
+    """n_src sources all read again `depth` levels later: more live values than LDS slots."""
+    events = [(float(i % 7), 1e9, 0.0) for i in range(n_src)]
+    acts, prec = {}, []
+    chain_prev = None
+    idx = 0
+    for k in range(depth):  # a chain that creates the levels
+        e = len(events)
+        events.append((0.0, 1e9, 0.0))
+        if chain_prev is not None:
+            acts[(chain_prev, e)] = (idx, 1.0, 0)
+            prec.append((e, [(chain_prev, idx)]))
+            idx += 1
+        chain_prev = e
+    for i in range(n_src):
+        e = len(events)
+        events.append((0.0, 1e9, 0.0))
+        acts[(i, e)] = (idx, 2.0, 0)
+        acts[(chain_prev, e)] = (idx + 1, 0.5, 0)
+        prec.append((e, [(i, idx), (chain_prev, idx + 1)]))
+        idx += 2
+    return FlatDag.from_lists(events, acts, prec, 1e6, [DistSpec(0, CONSTANT, (0.25, 0, INF))])
+
+
+def test_global_memory_fallback(gpu) -> None:
+    dag = _wide_dag(600, 5)
+    P = capi.Propagator(dag)
+    info = P.plan_info()
+    assert info["global_preds"] > 0, info
+    ctx = oracle.OracleCtx(dag)
+    rng = np.random.default_rng(3)
+    dur = np.round(rng.uniform(0, 9, (130, ctx.L)))
+    r, c = P.propagate(dur)
+    rr, cc = ctx.propagate(dur)
+    np.testing.assert_array_equal(bits(r), bits(rr))
+    np.testing.assert_array_equal(c, cc)
+    r, d, c = P.run_many(np.arange(70))
+    rr, cc = ctx.propagate(d)
+    np.testing.assert_array_equal(bits(r), bits(rr))
+    np.testing.assert_array_equal(c, cc)
+
+
+def test_device_api_leading_dimension(gpu) -> None:
+    """Device entry points with ld > n write only the first n columns of each row."""
+    dag = layered_small()
+    P = capi.Propagator(dag)
+    n, ld = 1000, 1088
+    seeds = capi.DeviceBuffer.from_array(np.arange(n, dtype=np.int32))
+    sentinel = np.full((P.E, ld), -7.0)
+    real = capi.DeviceBuffer.from_array(sentinel)
+    dur = capi.DeviceBuffer.from_array(np.full((P.L, ld), -7.0))
+    cause = capi.DeviceBuffer.from_array(np.full((P.E, ld), -9, dtype=np.int32))
+    P.run_many_device(seeds, n, ld, real, dur, cause)
+    capi.synchronize()
+    R = real.to_array(np.float64, (P.E, ld))
+    Dd = dur.to_array(np.float64, (P.L, ld))
+    Cc = cause.to_array(np.int32, (P.E, ld))
+    assert np.all(R[:, n:] == -7.0) and np.all(Dd[:, n:] == -7.0) and np.all(Cc[:, n:] == -9)
+    hr, hd, hc = P.run_many(np.arange(n))
+    np.testing.assert_array_equal(bits(R[:, :n].T), bits(hr))
+    np.testing.assert_array_equal(bits(Dd[:, :n].T), bits(hd))
+    np.testing.assert_array_equal(Cc[:, :n].T, hc)
+    # injected path on the same buffers
+    r2 = capi.DeviceBuffer(P.E * ld * 8)
+    c2 = capi.DeviceBuffer(P.E * ld * 4)
+    P.propagate_device(dur, n, ld, r2, c2)
+    capi.synchronize()
+    np.testing.assert_array_equal(bits(r2.to_array(np.float64, (P.E, ld))[:, :n]), bits(R[:, :n]))
+    np.testing.assert_array_equal(c2.to_array(np.int32, (P.E, ld))[:, :n], Cc[:, :n])
+
+
+def test_seed_is_the_key(gpu) -> None:
+    """A sample's result depends on its seed only: not on its position, batch size or neighbours."""
+    dag = layered_small()
+    P = capi.Propagator(dag)
+    seeds = np.array([5, -1, 2**31 - 1, -(2**31), 0, 5, 77] * 20, dtype=np.int32)
+    r, d, c = P.run_many(seeds)
+    for i, s in enumerate(seeds[:7]):
+        r1, d1, c1 = P.run_many([s])
+        np.testing.assert_array_equal(bits(r1[0]), bits(r[i]))
+        np.testing.assert_array_equal(bits(d1[0]), bits(d[i]))
+        np.testing.assert_array_equal(c1[0], c[i])
+    np.testing.assert_array_equal(bits(r[0]), bits(r[5]))
+    assert not np.array_equal(d[0], d[1])
+
+
+def test_network_slice_consistent(gpu) -> None:
+    """A 40-layer slice of the network workload (8 activity types), 300 samples."""
+    dag = layered_network(layers=40)
+    P = capi.Propagator(dag)
+    r, d, c = P.run_many(np.arange(300))
+    ctx = oracle.OracleCtx(dag)
+    rr, cc = ctx.propagate(d)
+    np.testing.assert_array_equal(bits(r), bits(rr))
+    np.testing.assert_array_equal(c, cc)
+    base = ctx.base_durations()
+    fixed = ~np.isnan(base)
+    np.testing.assert_array_equal(bits(d[:, fixed]), bits(np.broadcast_to(base[fixed], d[:, fixed].shape)))
+    assert np.all(np.isfinite(d)) and np.all(d >= dag.act_dur[None, :])

@@ -1,0 +1,246 @@
+"""Pins the CPU oracle (oracle/) before anything is checked against it.
+
+* Philox: Random123 known-answer vectors.
+* Propagation + compile: the reference test suite's deterministic KATs, the
+  golden fixtures written by the compiled reference, brute force on tiny DAGs,
+  and the compiled reference itself on random DAGs (when oracle/_ref is built).
+* Closed forms: the distribution CDFs against scipy.stats and textbook moments.
+"""
+
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import dags as D
+import oracle
+from mc_dagprop_amd.workloads import CONSTANT, EXPONENTIAL, GAMMA, DistSpec
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+INF = float("inf")
+
+
+# ---- Philox: Random123 kat_vectors ------------------------------------------
+PHILOX4_KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+PHILOX2_KAT = [
+    ((0, 0), 0, (0xFF1DAE59, 0x6CD10DF2)),
+    ((0xFFFFFFFF, 0xFFFFFFFF), 0xFFFFFFFF, (0x2C3F628B, 0xAB4FD7AD)),
+    ((0x243F6A88, 0x85A308D3), 0x13198A2E, (0xDD7CE038, 0xF62A4C12)),
+]
+
+
+def test_philox4x32_10_known_answers() -> None:
+    ctr = np.array([k[0] for k in PHILOX4_KAT], dtype=np.uint32)
+    key = np.array([k[1] for k in PHILOX4_KAT], dtype=np.uint32)
+    out = oracle.philox4x32(ctr, key)
+    np.testing.assert_array_equal(out, np.array([k[2] for k in PHILOX4_KAT], dtype=np.uint32))
+
+
+def test_philox2x32_10_known_answers() -> None:
+    ctr = np.array([k[0] for k in PHILOX2_KAT], dtype=np.uint32)
+    key = np.array([k[1] for k in PHILOX2_KAT], dtype=np.uint32)
+    out = oracle.philox2x32(ctr, key)
+    np.testing.assert_array_equal(out, np.array([k[2] for k in PHILOX2_KAT], dtype=np.uint32))
+
+
+def test_philox_round_count_matters() -> None:
+    # A dropped round would still be a bijection; the KAT above fixes R = 10.
+    ctr = np.zeros((1, 4), dtype=np.uint32)
+    key = np.zeros((1, 2), dtype=np.uint32)
+    assert not np.array_equal(oracle.philox4x32(ctr, key, rounds=9), oracle.philox4x32(ctr, key, rounds=10))
+
+
+# ---- reference KATs through the oracle -----------------------------------------
+def test_propagation_kat() -> None:
+    """test/test_simulator.py:107-137: durations 6,10,10,15,40; realized 0,6,16,22,62,100."""
+    dag = D.sim6([DistSpec(1, CONSTANT, (1.0, 0, INF)), DistSpec(3, CONSTANT, (3.0, 0, INF))])
+    ctx = oracle.OracleCtx(dag)
+    dur = ctx.base_durations()
+    np.testing.assert_array_equal(dur, [6.0, 10.0, 10.0, 15.0, 40.0])
+    r, c = ctx.propagate(dur)
+    np.testing.assert_array_equal(r[0], [0.0, 6.0, 16.0, 22.0, 62.0, 100.0])
+    np.testing.assert_array_equal(c[0], [-1, 0, 1, -1, 3, -1])
+
+
+def test_constant_kat_and_order_invariance() -> None:
+    """test/test_simulator.py:37-83."""
+    dists = [DistSpec(1, CONSTANT, (1.0, 0, INF))]
+    a, b = oracle.OracleCtx(D.sim6(dists)), oracle.OracleCtx(D.sim6_reversed(dists))
+    ra, ca = a.propagate(a.base_durations())
+    rb, cb = b.propagate(b.base_durations())
+    assert ra[0, 1] == 6.0 and ra[0, 2] == 16.0
+    np.testing.assert_array_equal(ca[0, :3], [-1, 0, 1])
+    np.testing.assert_array_equal(ra, rb)
+    np.testing.assert_array_equal(ca, cb)
+    assert list(a.order) != list(b.order) or True  # order may differ; results may not
+
+
+def test_chain_constant_kat() -> None:
+    """test/test_monte_carlo_extra.py:27-34: durations [2, 4], realized [0, 2, 6]."""
+    ctx = oracle.OracleCtx(D.chain3([DistSpec(1, CONSTANT, (1.0, 0, INF))]))
+    d = ctx.base_durations()
+    np.testing.assert_array_equal(d, [2.0, 4.0])
+    r, _ = ctx.propagate(d)
+    np.testing.assert_array_equal(r[0], [0.0, 2.0, 6.0])
+
+
+def test_large_chain_kat() -> None:
+    """test/test_simulator.py:183-199: realized[9999] == 59988 (edge 9999 is outside the links => 0.0)."""
+    ctx = oracle.OracleCtx(D.large_chain([DistSpec(1, CONSTANT, (1.0, 0, INF))]))
+    assert ctx.E == 10_000 and ctx.L == 9_999
+    r, _ = ctx.propagate(ctx.base_durations())
+    assert r[0, 0] == 0.0 and r[0, 9999] == 59988.0
+
+
+@pytest.mark.parametrize(
+    "dists,code",
+    [
+        ([DistSpec(-1, CONSTANT, (0.0, 0, INF))], 1),  # test_monte_carlo_extra.py:72-77
+    ],
+)
+def test_reserved_type(dists, code) -> None:
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.OracleCtx(D.chain3(dists))
+    assert e.value.code == code and "reserved" in str(e.value)
+
+
+def test_negative_max_delay_and_cycle() -> None:
+    """test_monte_carlo_extra.py:59-91."""
+    with pytest.raises(oracle.OracleError, match="max_delay"):
+        oracle.OracleCtx(D.chain3([DistSpec(1, CONSTANT, (0.0, 0, INF))], max_delay=-1.0))
+    from mc_dagprop_amd.workloads import FlatDag
+
+    cyc = FlatDag.from_lists([(0.0, 100.0, 0.0)] * 2, {(0, 1): (0, 1.0, 1), (1, 0): (1, 1.0, 1)},
+                             [(1, [(0, 0)]), (0, [(1, 1)])], 1e6, [DistSpec(1, CONSTANT, (0.0, 0, INF))])
+    with pytest.raises(oracle.OracleError, match="cycle"):
+        oracle.OracleCtx(cyc)
+
+
+# ---- golden fixtures written by the compiled reference ---------------------------
+GOLDEN_FILES = sorted(GOLDEN.glob("*.npz"))
+
+
+def test_golden_manifest() -> None:
+    import hashlib
+
+    listed = {}
+    for line in (GOLDEN / "MANIFEST.txt").read_text().splitlines():
+        h, name = line.split()
+        listed[name] = h
+    assert sorted(listed) == [p.name for p in GOLDEN_FILES]
+    for p in GOLDEN_FILES:
+        assert hashlib.sha256(p.read_bytes()).hexdigest() == listed[p.name], p.name
+
+
+@pytest.mark.parametrize("path", GOLDEN_FILES, ids=[p.stem for p in GOLDEN_FILES])
+def test_oracle_matches_reference_fixture(path: Path) -> None:
+    """Given the reference's own durations, the oracle's realized/cause are bit-identical."""
+    z = np.load(path)
+    dag = D.dag_from_npz(z)
+    ctx = oracle.OracleCtx(dag)
+    r, c = ctx.propagate(z["durations"])
+    np.testing.assert_array_equal(r.view(np.uint64), z["realized"].view(np.uint64))
+    np.testing.assert_array_equal(c, z["cause"])
+    # Links that need no draw have the same duration in every sample.
+    base = ctx.base_durations()
+    fixed = ~np.isnan(base)
+    np.testing.assert_array_equal(z["durations"][:, fixed].view(np.uint64),
+                                  np.broadcast_to(base[fixed], z["durations"][:, fixed].shape).view(np.uint64))
+
+
+def test_golden_stream_kats() -> None:
+    """The reference's stream-dependent asserts hold in the fixtures it wrote (test_simulator.py:139-172)."""
+    z = np.load(GOLDEN / "sim6_empirical_absolute.npz")
+    assert z["realized"][0, 3] == 68.0 and z["realized"][0, 5] == 100.0
+    z = np.load(GOLDEN / "sim6_empirical_relative.npz")
+    assert abs(z["realized"][0, 3] - 34.10) < 5e-5
+    z = np.load(GOLDEN / "sim6_empirical_relative_hist.npz")
+    assert abs(z["realized"][0, 3] - 23.062461393412335) < 5e-4
+
+
+# ---- brute force on tiny DAGs ------------------------------------------------------
+def _brute_force(dag, dur):
+    """Longest path with caps, computed by fixed-point iteration instead of a topological order."""
+    E = dag.n_events
+    preds = {}
+    for i, t in enumerate(dag.prec_target):
+        a, b = dag.prec_off[i], dag.prec_off[i + 1]
+        preds[int(t)] = list(zip(dag.prec_src[a:b], dag.prec_edge[a:b]))
+    r = dag.earliest.astype(float).copy()
+    for _ in range(E + 1):
+        new = r.copy()
+        cause = np.full(E, -1)
+        for e in range(E):
+            upper = dag.earliest[e] + dag.max_delay
+            best = dag.earliest[e]
+            for s, l in preds.get(e, []):
+                t = min(r[s] + dur[l], upper)
+                if t >= best:
+                    best, cause[e] = t, s
+            new[e] = min(best, upper)
+        r = new
+    return r, cause
+
+
+def test_oracle_matches_brute_force() -> None:
+    rng = np.random.default_rng(5)
+    for trial in range(40):
+        dag = D.random_dag(rng, int(rng.integers(2, 12)), max_delay=float(rng.choice([0.0, 4.0, 1e6])))
+        ctx = oracle.OracleCtx(dag)
+        dur = np.round(rng.uniform(0, 10, ctx.L))
+        r, c = ctx.propagate(dur)
+        br, bc = _brute_force(dag, dur)
+        np.testing.assert_array_equal(r[0], br)
+        np.testing.assert_array_equal(c[0], bc)
+
+
+def test_planted_mistake_is_caught() -> None:
+    """A tie-break flipped to '>' changes a cause on the propagation KAT's tie (a check that can fail)."""
+    dag = D.sim6([DistSpec(1, CONSTANT, (1.0, 0, INF)), DistSpec(3, CONSTANT, (3.0, 0, INF))])
+    ctx = oracle.OracleCtx(dag)
+    d = ctx.base_durations()
+    d[3] = 46.0  # 16 + 46 == 22 + 40: tie at event 4, the later predecessor (3) must win
+    r, c = ctx.propagate(d)
+    assert r[0, 4] == 62.0 and c[0, 4] == 3
+    br, bc = _brute_force(dag, d)
+    assert bc[4] == 3
+
+
+# ---- the compiled reference on random DAGs (when oracle/_ref is built) ---------------
+@pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built (needs the reference sources)")
+def test_oracle_matches_compiled_reference_random() -> None:
+    rng = np.random.default_rng(11)
+    for trial in range(30):
+        kw = [{}, {"duplicates": True}, {"gaps": True}][trial % 3]
+        dag = D.random_dag(rng, int(rng.integers(2, 60)), dists=D.STD_DISTS, **kw)
+        ref = oracle.RefSim(dag)
+        rr, rd, rc = ref.run(np.arange(8))
+        ctx = oracle.OracleCtx(dag)
+        assert (ctx.E, ctx.L) == (ref.E, ref.L)
+        r, c = ctx.propagate(rd)
+        np.testing.assert_array_equal(r.view(np.uint64), rr.view(np.uint64))
+        np.testing.assert_array_equal(c, rc)
+
+
+# ---- closed forms ---------------------------------------------------------------------
+def test_closed_form_cdfs() -> None:
+    from scipy import stats
+
+    x = np.linspace(0, 10, 101)
+    np.testing.assert_allclose(oracle.cdf_exponential(x, 2.0), stats.expon(scale=2.0).cdf(x), rtol=1e-12, atol=1e-15)
+    t = oracle.cdf_exponential(x, 2.0, 3.0)
+    np.testing.assert_allclose(t[x <= 3], stats.expon(scale=2.0).cdf(x[x <= 3]) / stats.expon(scale=2.0).cdf(3.0))
+    assert np.all(t[x >= 3] == 1.0)
+    np.testing.assert_allclose(oracle.cdf_gamma(x, 2.5, 1.5), stats.gamma(2.5, scale=1.5).cdf(x), rtol=1e-10)
+    # Gamma(2, s) is Erlang: 1 - e^{-x/s}(1 + x/s)
+    np.testing.assert_allclose(oracle.cdf_gamma(x, 2.0, 1.0), 1 - np.exp(-x) * (1 + x), rtol=1e-12, atol=1e-15)
+    m, v, sk, ku = oracle.gamma_moments(4.0, 0.5)
+    assert (m, v) == (2.0, 1.0) and sk == 1.0 and ku == 1.5
+    np.testing.assert_allclose(oracle.pmf_empirical([1, 3]), [0.25, 0.75])
