@@ -1,0 +1,209 @@
+"""Benchmark of the Monte Carlo DAG propagation path on one MI355X.
+
+One step = one run_many over the whole sample batch on the device: delay
+sampling + max-plus propagation for every (sample, activity), writing the full
+realized / durations / cause_event outputs to HBM (inputs already resident).
+
+Default workload (the network-scale configuration): 200 layers x 100 events =
+20,000 events, 1-4 predecessors each (50,005 activities), 8 activity types,
+100,000 samples, FP64.  The metric is activity-samples/s = activity_count x
+samples / seconds per step.
+
+Also reported:
+  roofline      the walk kernel's algorithmic HBM bytes per launch
+                (samples x (8 L + 12 E): durations f64, realized f64, cause i32)
+                over its average launch time from HIP events on its stream,
+                against MI355X's 8 TB/s; `traffic` is the rocprofv3-measured HBM
+                bytes per launch when profiles/walk_network_traffic.txt exists;
+  cpu_baseline  the reference engine (compiled from its sources into oracle/_ref)
+                running its run_many loop on the host cores over a bounded
+                sample of the same workload.
+  check         device durations of a few samples fed to the CPU oracle
+                reproduce the device's realized / cause bit-exactly.
+
+    python bench.py [--steps K] [--warmup W] [--samples S] [--workload network|small]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def read_traffic(path: Path):
+    """HBM bytes per launch from a committed rocprofv3 summary (key: value lines), or None."""
+    if not path.exists():
+        return None
+    m = re.search(r"^hbm_bytes_per_launch:\s*([0-9.eE+]+)", path.read_text(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(dag, L: int, threads: int, per_thread: int):
+    import oracle
+
+    if not oracle.ref_available():
+        return None
+    ref = oracle.RefSim(dag)
+    ref.run_many_timed(np.arange(threads), threads)  # warm up
+    n = threads * per_thread
+    seeds = np.arange(n, dtype=np.int32)
+    t = ref.run_many_timed(seeds, threads)
+    return {
+        "value": L * n / t,
+        "unit": "activity-samples/s",
+        "cores": threads,
+        "kind": "reference",
+        "sample": f"reference run_many loop (Simulator::run per seed, SimResult materialised), {n} seeds "
+        f"split over {threads} threads, one Simulator each; {t:.2f} s wall, same DAG and distributions",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--samples", type=int, default=100_000)
+    ap.add_argument("--workload", choices=["network", "small"], default="network")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
+    ap.add_argument("--check-samples", type=int, default=64)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (samples would shard "
+              "across GPUs with the context replicated; see the sharding note in DESIGN.md).", file=sys.stderr)
+        return 2
+
+    from mc_dagprop_amd import capi
+    from mc_dagprop_amd.workloads import layered_network, layered_small
+
+    if capi.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        return 1
+
+    dag = layered_network() if args.workload == "network" else layered_small()
+    prop = capi.Propagator(dag)
+    E, L, S = prop.E, prop.L, args.samples
+    log(f"workload {args.workload}: E={E} L={L} S={S} plan={prop.plan_info()}")
+
+    seeds = capi.DeviceBuffer.from_array(np.arange(S, dtype=np.int32))
+    realized = capi.DeviceBuffer(8 * E * S)
+    durations = capi.DeviceBuffer(8 * L * S)
+    cause = capi.DeviceBuffer(4 * E * S)
+    stream = capi.C.c_void_p(0)
+    capi.check(capi.lib().mcdp_stream_create(capi.C.byref(stream)))
+
+    def step() -> None:
+        prop.run_many_device(seeds, S, S, realized, durations, cause, stream)
+
+    for _ in range(args.warmup):
+        step()
+    capi.synchronize(stream)
+    log(f"warmup done ({args.warmup} steps)")
+
+    timers = [capi.Timer() for _ in range(args.steps)]
+    capi.synchronize(stream)
+    t0 = time.perf_counter()
+    for tm in timers:
+        tm.start(stream)
+        step()
+        tm.stop(stream)
+    capi.synchronize(stream)
+    wall = time.perf_counter() - t0
+    kernel_ms = [tm.elapsed_ms() for tm in timers]
+    ms_per_step = 1000.0 * wall / args.steps
+    avg_kernel_ms = float(np.mean(kernel_ms))
+    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step wall, kernel avg {avg_kernel_ms:.3f} ms "
+        f"(min {min(kernel_ms):.3f}, max {max(kernel_ms):.3f})")
+
+    # Correctness at full size: a spread of samples, device durations -> oracle.
+    check = None
+    if args.check_samples > 0:
+        import oracle
+
+        w = max(1, min(args.check_samples // 2, S // 2 if S > 1 else 1))
+        blocks = sorted({0, S - w})  # first samples and the last (tail) tile
+        R = np.concatenate([realized.columns(np.float64, E, S, c0, w) for c0 in blocks])
+        Dd = np.concatenate([durations.columns(np.float64, L, S, c0, w) for c0 in blocks])
+        Cc = np.concatenate([cause.columns(np.int32, E, S, c0, w) for c0 in blocks])
+        rr, cc = oracle.OracleCtx(dag).propagate(Dd)
+        ok = bool(np.array_equal(np.ascontiguousarray(R).view(np.uint64), rr.view(np.uint64))
+                  and np.array_equal(Cc, cc))
+        check = {"samples_checked": int(R.shape[0]), "bit_exact": ok}
+        log(f"check: {check}")
+        if not ok:
+            print("device results differ from the oracle", file=sys.stderr)
+            return 3
+
+    value = L * S / (ms_per_step / 1000.0)
+    alg_bytes = float(S) * (8 * L + 12 * E)
+    achieved = alg_bytes / (avg_kernel_ms / 1000.0) / 1e9
+    traffic = read_traffic(ROOT / "profiles" / f"walk_{args.workload}_traffic.txt")
+    baseline = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        baseline = cpu_baseline(dag, L, threads, 256 if args.workload == "network" else 2048)
+        if baseline:
+            log(f"cpu baseline: {baseline['value']:.4g} activity-samples/s on {threads} threads")
+
+    out = {
+        "metric": "activity-samples/sec GPU vs host run_many (bit-exact); p50/p99 solve latency",
+        "value": value,
+        "unit": "activity-samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (deterministic SplitMix64 layered DAG; seeds 0..S-1)",
+        "config": {
+            "workload": (
+                f"layered_network: 200 layers x 100 events ({E} events, {L} activities, 1-4 preds, 8 activity "
+                f"types), {S} samples, full realized/durations/cause outputs in HBM"
+                if args.workload == "network"
+                else f"layered_small: 20 layers x 10 events ({E} events, {L} activities), {S} samples"
+            ),
+            "events": E,
+            "activities": L,
+            "samples": S,
+            "outputs": "realized f64 [E][S], durations f64 [L][S], cause i32 [E][S]",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": alg_bytes,
+            "kernel_ms": avg_kernel_ms,
+        },
+        "cpu_baseline": baseline,
+        "speedup_vs_cpu_baseline": (value / baseline["value"]) if baseline else None,
+        "check": check,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -128,6 +128,11 @@ int mcdp_free(void *ptr);
 int mcdp_memcpy_htod(void *dst_dev, const void *src_host, size_t bytes);
 int mcdp_memcpy_dtoh(void *dst_host, const void *src_dev, size_t bytes);
 int mcdp_memset(void *dst_dev, int value, size_t bytes);
+/* Copies `height` rows of `width` bytes from a device array with row pitch
+ * `src_pitch` to a host array with row pitch `dst_pitch` (e.g. a block of
+ * sample columns out of an event-major output). */
+int mcdp_memcpy2d_dtoh(void *dst_host, size_t dst_pitch, const void *src_dev, size_t src_pitch, size_t width,
+                       size_t height);
 int mcdp_stream_create(void **stream);
 int mcdp_stream_destroy(void *stream);
 int mcdp_stream_synchronize(void *stream);

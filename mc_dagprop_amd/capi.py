@@ -52,6 +52,7 @@ _SIGNATURES = {
     "mcdp_memcpy_htod": (C.c_int, [_P, _P, C.c_size_t]),
     "mcdp_memcpy_dtoh": (C.c_int, [_P, _P, C.c_size_t]),
     "mcdp_memset": (C.c_int, [_P, C.c_int, C.c_size_t]),
+    "mcdp_memcpy2d_dtoh": (C.c_int, [_P, C.c_size_t, _P, C.c_size_t, C.c_size_t, C.c_size_t]),
     "mcdp_stream_create": (C.c_int, [C.POINTER(_P)]),
     "mcdp_stream_destroy": (C.c_int, [_P]),
     "mcdp_stream_synchronize": (C.c_int, [_P]),
@@ -131,6 +132,15 @@ class DeviceBuffer:
         if out.nbytes:
             check(lib().mcdp_memcpy_dtoh(out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes))
         return out
+
+    def columns(self, dtype, rows: int, ld: int, c0: int, width: int) -> np.ndarray:
+        """Columns [c0, c0 + width) of a [rows][ld] device array, as a (width, rows) host array."""
+        item = np.dtype(dtype).itemsize
+        out = np.empty((rows, width), dtype=dtype)
+        if out.nbytes:
+            check(lib().mcdp_memcpy2d_dtoh(out.ctypes.data_as(C.c_void_p), width * item,
+                                           C.c_void_p(self.ptr.value + c0 * item), ld * item, width * item, rows))
+        return out.T
 
     def close(self) -> None:
         if self.ptr and self.ptr.value:

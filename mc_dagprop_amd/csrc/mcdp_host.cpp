@@ -886,6 +886,11 @@ int mcdp_memset(void *dst, int value, size_t bytes) {
     HIP_TRY(hipMemset(dst, value, bytes), "hipMemset");
     return MCDP_OK;
 }
+int mcdp_memcpy2d_dtoh(void *dst, size_t dst_pitch, const void *src, size_t src_pitch, size_t width,
+                       size_t height) {
+    HIP_TRY(hipMemcpy2D(dst, dst_pitch, src, src_pitch, width, height, hipMemcpyDeviceToHost), "hipMemcpy2D DtoH");
+    return MCDP_OK;
+}
 int mcdp_stream_create(void **stream) {
     if (!stream) return fail(MCDP_E_INVALID, "stream is null");
     hipStream_t s = nullptr;
