@@ -83,6 +83,9 @@ def main() -> int:
     ap.add_argument("--workload", choices=["network", "small"], default="network")
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--check-samples", type=int, default=64)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (read + write) --steps times, to calibrate "
+                    "rocprofv3 FETCH_SIZE / WRITE_SIZE against known bytes")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -97,7 +100,23 @@ def main() -> int:
         print("no HIP device visible", file=sys.stderr)
         return 1
 
-    dag = layered_network() if args.workload == "network" else layered_small()
+    if args.copy_calibration > 0:
+        n = args.copy_calibration * (1 << 20) // 8
+        src = capi.DeviceBuffer(8 * n)
+        dst = capi.DeviceBuffer(8 * n)
+        capi.check(capi.lib().mcdp_memset(src.ptr, 0, 8 * n))
+        tm = capi.Timer()
+        tm.start()
+        for _ in range(args.steps):
+            capi.check(capi.lib().mcdp_debug_copy(dst.ptr, src.ptr, n, None))
+        tm.stop()
+        capi.synchronize()
+        ms = tm.elapsed_ms() / args.steps
+        print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "ms": ms,
+                          "GB/s": 16 * n / ms / 1e6}), flush=True)
+        return 0
+
+    dag =layered_network() if args.workload == "network" else layered_small()
     prop = capi.Propagator(dag)
     E, L, S = prop.E, prop.L, args.samples
     log(f"workload {args.workload}: E={E} L={L} S={S} plan={prop.plan_info()}")

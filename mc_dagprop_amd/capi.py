@@ -61,6 +61,7 @@ _SIGNATURES = {
     "mcdp_event_record": (C.c_int, [_P, _P]),
     "mcdp_event_elapsed_ms": (C.c_int, [_P, _P, C.POINTER(C.c_float)]),
     "mcdp_debug_philox": (C.c_int, [C.c_int, _i64, _pu32, _pu32, _pu32]),
+    "mcdp_debug_copy": (C.c_int, [_P, _P, _i64, _P]),
 }
 
 _lib = None

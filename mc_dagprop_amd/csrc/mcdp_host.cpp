@@ -928,6 +928,12 @@ int mcdp_event_elapsed_ms(void *start, void *stop, float *ms) {
     return MCDP_OK;
 }
 
+int mcdp_debug_copy(double *dst, const double *src, int64_t n, void *stream) {
+    if (n < 0 || (n > 0 && (!dst || !src))) return fail(MCDP_E_INVALID, "invalid arguments");
+    HIP_TRY(launch_copy(dst, src, n, static_cast<hipStream_t>(stream)), "copy kernel launch");
+    return MCDP_OK;
+}
+
 int mcdp_debug_philox(int variant, int64_t n, const uint32_t *ctr, const uint32_t *key, uint32_t *out) {
     if (variant != 2 && variant != 4) return fail(MCDP_E_INVALID, "variant must be 2 or 4");
     if (n < 0 || (n > 0 && (!ctr || !key || !out))) return fail(MCDP_E_INVALID, "invalid arguments");

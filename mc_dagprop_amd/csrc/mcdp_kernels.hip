@@ -28,6 +28,13 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) 
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int W = p.waves;
     const int64_t ld = p.ld;
+    // The plan never changes during a launch: reading it through the constant
+    // address space lets wave-uniform record loads become scalar loads.
+    const auto step_off = as_const_space(p.step_off);
+    const auto evs = as_const_space(p.ev);
+    const auto prs = as_const_space(p.pr);
+    const auto dists = as_const_space(p.dist);
+    const auto alias = as_const_space(p.alias);
 
     for (int64_t tile = blockIdx.x; tile < p.n_tiles; tile += gridDim.x) {
         const int64_t s = tile * kLanes + lane;
@@ -36,15 +43,15 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) 
         if (!kInjected) key = seed_key(active ? p.seeds[s] : 0);
 
         for (int t = 0; t < p.n_steps; ++t) {
-            const int b = p.step_off[t * W + wave];
-            const int e = p.step_off[t * W + wave + 1];
+            const int b = step_off[t * W + wave];
+            const int e = step_off[t * W + wave + 1];
             double res[kMaxEventsPerWave];
             for (int i = b; i < e; ++i) {
-                const EventRec er = p.ev[i];
+                const EventRec er = evs[i];
                 double best = er.earliest;
                 int32_t who = -1;
                 for (int k = er.pred_begin; k < er.pred_begin + er.pred_count; ++k) {
-                    const PredRec pr = p.pr[k];
+                    const PredRec pr = prs[k];
                     double rs;
                     if (pr.src_slot >= 0) {
                         rs = lds[pr.src_slot * kLanes + lane];
@@ -56,13 +63,13 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) 
                     double dur = 0.0;
                     if (pr.info & kInfoEdgeValid) {
                         if (kInjected) {
-                            dur = active ? p.dur_in[(int64_t)pr.edge * ld + s] : 0.0;
+                            dur = active ? __builtin_nontemporal_load(p.dur_in + (int64_t)pr.edge * ld + s) : 0.0;
                         } else {
                             const uint32_t di = pr.info & kInfoDistMask;
                             dur = (di == kNoDist) ? pr.dmin
-                                                  : draw_duration(p.dist[di], p.alias, (uint32_t)pr.edge, pr.dmin, key);
+                                                  : draw_duration(dists[di], alias, (uint32_t)pr.edge, pr.dmin, key);
                             if ((pr.info & kInfoOwner) && p.durations && active)
-                                p.durations[(int64_t)pr.edge * ld + s] = dur;
+                                __builtin_nontemporal_store(dur, p.durations + (int64_t)pr.edge * ld + s);
                         }
                     }
                     double tt = rs + dur;
@@ -74,8 +81,8 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) 
                 }
                 const double r = (er.upper < best) ? er.upper : best;
                 if (active) {
-                    if (p.realized) p.realized[(int64_t)er.event * ld + s] = r;
-                    if (p.cause) p.cause[(int64_t)er.event * ld + s] = who;
+                    if (p.realized) __builtin_nontemporal_store(r, p.realized + (int64_t)er.event * ld + s);
+                    if (p.cause) __builtin_nontemporal_store(who, p.cause + (int64_t)er.event * ld + s);
                 }
                 res[i - b] = r;
             }
@@ -83,7 +90,7 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) 
             // and every commit is visible before the next step reads it.
             __syncthreads();
             for (int i = b; i < e; ++i) {
-                const int slot = p.ev[i].slot;
+                const int slot = evs[i].slot;
                 if (slot >= 0) lds[slot * kLanes + lane] = res[i - b];
             }
             __syncthreads();
@@ -91,12 +98,13 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) 
 
         // Activities no predecessor entry references still get a duration row.
         if (!kInjected && p.durations) {
+            const auto orph = as_const_space(p.orph);
             for (int j = wave; j < p.n_orphans; j += W) {
-                const OrphanRec o = p.orph[j];
+                const OrphanRec o = orph[j];
                 const uint32_t di = o.info & kInfoDistMask;
                 const double dur =
-                    (di == kNoDist) ? o.dmin : draw_duration(p.dist[di], p.alias, (uint32_t)o.edge, o.dmin, key);
-                if (active) p.durations[(int64_t)o.edge * ld + s] = dur;
+                    (di == kNoDist) ? o.dmin : draw_duration(dists[di], alias, (uint32_t)o.edge, o.dmin, key);
+                if (active) __builtin_nontemporal_store(dur, p.durations + (int64_t)o.edge * ld + s);
             }
         }
     }
@@ -118,6 +126,18 @@ __global__ void philox_debug_kernel(int variant, int64_t n, const uint32_t *__re
         out[2 * i] = w.x;
         out[2 * i + 1] = w.y;
     }
+}
+
+__global__ void __launch_bounds__(256) copy_kernel(double *__restrict__ dst, const double *__restrict__ src,
+                                                   int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy(double *dst, const double *src, int64_t n, hipStream_t stream) {
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 256 * 64);
+    if (blocks > 0) hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, dst, src, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_walk(const WalkParams &p, bool injected, int grid, size_t lds_bytes, hipStream_t stream) {

@@ -20,6 +20,22 @@
 
 namespace mcdp {
 
+// Read-only views in the constant address space (4): loads through them are
+// known not to alias any store of the kernel, so wave-uniform ones are issued
+// as scalar loads.  (hipcc's host pass parses kernel bodies too; there the
+// view is a plain pointer.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using const_ptr = const __attribute__((address_space(4))) T *;
+#else
+template <typename T>
+using const_ptr = const T *;
+#endif
+template <typename T>
+__device__ __forceinline__ const_ptr<T> as_const_space(const T *p) {
+    return (const_ptr<T>)p;
+}
+
 // ---- Philox -----------------------------------------------------------------
 // Multipliers and Weyl key increments as published with Random123.
 constexpr uint32_t kPhiloxM4a = 0xD2511F53u, kPhiloxM4b = 0xCD9E8D57u;
@@ -113,7 +129,7 @@ __device__ __forceinline__ bool mt_attempt(const Words4 &w, double d, double c, 
 // (wave-uniform).  Returns base + extra rounded exactly as the reference
 // (_core.cpp:75, 88, 103, 125, 140, 328): extra = X * d (one rounding), then
 // d + extra (one rounding); this file is compiled with -ffp-contract=off.
-__device__ __forceinline__ double draw_duration(const DistRec &D, const AliasRec *__restrict__ alias, uint32_t edge,
+__device__ __forceinline__ double draw_duration(const DistRec &D, const_ptr<AliasRec> alias, uint32_t edge,
                                                 double dmin, SeedKey key) {
     switch (D.kind) {
         case kConstant: {
@@ -147,7 +163,7 @@ __device__ __forceinline__ double draw_duration(const DistRec &D, const AliasRec
             int i = (int)xn;
             i = (i >= D.n) ? D.n - 1 : i;
             const double frac = xn - (double)i;
-            const AliasRec *rec = alias + D.table_off + i;
+            const const_ptr<AliasRec> rec = alias + D.table_off + i;
             const double v = (frac < rec->prob) ? rec->v_self : rec->v_alias;
             if (D.kind == kEmpiricalAbs) return dmin + v;
             const double extra = v * dmin;
