@@ -163,8 +163,9 @@ int compile_exponential(double lam, double m, DistRec &D) {
     } else {
         D.kind = kExpTrunc;
         D.p[0] = lam;
-        D.p[1] = -std::expm1(-m / lam);
+        D.p[1] = std::exp(-m / lam);     // q
         D.p[2] = m;
+        D.p[3] = -std::expm1(-m / lam);  // 1 - q
     }
     return MCDP_OK;
 }

@@ -63,7 +63,7 @@ struct OrphanRec {
 enum DistKind : int32_t {
     kConstant = 0,         // extra = d * p0
     kExpInf = 1,           // X = -p0 * log(u), u in (0,1]                       (mean p0)
-    kExpTrunc = 2,         // X = -p0 * log1p(-v * p1), v in [0,1), clamp at p2  (p1 = 1 - e^{-m/p0})
+    kExpTrunc = 2,         // X = -p0 * log(p1 + p3 u), u in (0,1], clamp at p2  (p1 = e^{-m/p0}, p3 = 1 - p1)
     kGammaErlang2 = 3,     // X = -p0 * log(u1 * u2)                              (shape 2, scale p0)
     kGammaHalf = 4,        // X = p0 * (-log u1) * cos^2(2 pi u2)                 (shape 1/2, scale p0)
     kGammaMT = 5,          // Marsaglia-Tsang, shape p1 >= 1 (or p1 + 1 with boost), scale p0

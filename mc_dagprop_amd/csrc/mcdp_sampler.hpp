@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "mcdp_math.hpp"
 #include "mcdp_plan.hpp"
 
 namespace mcdp {
@@ -114,18 +115,18 @@ __device__ __forceinline__ bool mt_attempt(const Words4 &w, double d, double c, 
     const double u1 = u01_open0(w.x, w.y);
     const double u2 = u01_mid32(w.z);
     const double u3 = u01_mid32(w.w);
-    const double z = sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
+    const double z = sqrt(-2.0 * log_fast(u1)) * cos2pi_fast(u2);
     double v = 1.0 + c * z;
     if (!(v > 0.0)) return false;
     v = v * v * v;
     const double z2 = z * z;
     x = d * v;
     if (u3 < 1.0 - 0.0331 * (z2 * z2)) return true;
-    return log(u3) < 0.5 * z2 + d * (1.0 - v + log(v));
+    return log_fast(u3) < 0.5 * z2 + d * (1.0 - v + log_fast(v));
 }
 
-// Draws the extra delay of one activity for one sample.  `edge` is the
-// activity index, `dmin` its minimal duration, `D` its compiled distribution
+// Draws the duration of one activity for one sample.  `edge` is the activity
+// index, `dmin` its minimal duration, `D` its compiled distribution
 // (wave-uniform).  Returns base + extra rounded exactly as the reference
 // (_core.cpp:75, 88, 103, 125, 140, 328): extra = X * d (one rounding), then
 // d + extra (one rounding); this file is compiled with -ffp-contract=off.
@@ -141,16 +142,18 @@ __device__ __forceinline__ double draw_duration(const DistRec &D, const_ptr<Alia
             const double extra = D.p[0] * dmin;
             return dmin + extra;
         }
-        case kExpInf: {
+        case kExpInf: {  // inversion: X = -mean log U, U in (0, 1]
             const Words2 w = philox2x32_10(edge, kTagMain, key.lo);
-            const double X = -D.p[0] * log(u01_open0(w.x, w.y));
+            const double X = -D.p[0] * log_fast(u01_open0(w.x, w.y));
             const double extra = X * dmin;
             return dmin + extra;
         }
         case kExpTrunc: {
-            // Inverse of the truncated CDF (1 - e^{-x/l}) / (1 - e^{-m/l}).
+            // Inversion of the CDF truncated at m: U' = q + (1 - q) U is uniform on
+            // (q, 1] with q = e^{-m/mean}, and X = -mean log U' lies in [0, m].
             const Words2 w = philox2x32_10(edge, kTagMain, key.lo);
-            double X = -D.p[0] * log1p(-u01(w.x, w.y) * D.p[1]);
+            const double u = __builtin_fma(u01_open0(w.x, w.y), D.p[3], D.p[1]);
+            double X = -D.p[0] * log_fast(u);
             X = (X > D.p[2]) ? D.p[2] : X;
             const double extra = X * dmin;
             return dmin + extra;
@@ -182,11 +185,11 @@ __device__ __forceinline__ double draw_duration(const DistRec &D, const_ptr<Alia
         bool ok = true;
         switch (D.kind) {
             case kGammaErlang2:  // sum of two unit exponentials
-                X = -D.p[0] * log(u01_open0(w.x, w.y) * u01_open0(w.z, w.w));
+                X = -D.p[0] * log_fast(u01_open0(w.x, w.y) * u01_open0(w.z, w.w));
                 break;
-            case kGammaHalf: {  // Gamma(1/2) = N(0,1)^2 / 2
-                const double cs = cospi(2.0 * u01_mid32(w.z));
-                X = D.p[0] * (-log(u01_open0(w.x, w.y)) * (cs * cs));
+            case kGammaHalf: {  // Gamma(1/2) = N(0,1)^2 / 2 = Exp(1) cos^2(2 pi U)
+                const double cs = cos2pi_fast(u01_mid32(w.z));
+                X = D.p[0] * (-log_fast(u01_open0(w.x, w.y)) * (cs * cs));
                 break;
             }
             case kGammaMT: {
@@ -195,7 +198,7 @@ __device__ __forceinline__ double draw_duration(const DistRec &D, const_ptr<Alia
                 if (ok) {
                     if (D.boost) {  // shape < 1: Gamma(k) = Gamma(k + 1) U^{1/k}
                         const Words2 b = philox2x32_10(edge, (j << 8) | kTagBoost, key.lo);
-                        y *= exp(log(u01_open0(b.x, b.y)) * D.p[4]);
+                        y *= exp_fast(log_fast(u01_open0(b.x, b.y)) * D.p[4]);
                     }
                     X = D.p[0] * y;
                 }
@@ -203,8 +206,8 @@ __device__ __forceinline__ double draw_duration(const DistRec &D, const_ptr<Alia
             }
             default: {  // kGammaPow: density x^{k-1} e^{-x} on [0, m'] by
                         // proposal m' U^{1/k}, accepted with probability e^{-x}
-                const double y = D.p[1] * exp(log(u01_open0(w.x, w.y)) * D.p[2]);
-                ok = u01_open0(w.z, w.w) <= exp(-y);
+                const double y = D.p[1] * exp_fast(log_fast(u01_open0(w.x, w.y)) * D.p[2]);
+                ok = u01_open0(w.z, w.w) <= exp_fast(-y);
                 X = (y * D.p[0] > m) ? m : y * D.p[0];
                 break;
             }
