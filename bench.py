@@ -83,6 +83,9 @@ def main() -> int:
     ap.add_argument("--workload", choices=["network", "small"], default="network")
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--check-samples", type=int, default=64)
+    ap.add_argument("--mode", choices=["run_many", "propagate"], default="run_many",
+                    help="run_many: sample + propagate (the metric); propagate: propagation only, durations "
+                    "injected from a prior run_many (reads durations instead of sampling them)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write) --steps times, to calibrate "
                     "rocprofv3 FETCH_SIZE / WRITE_SIZE against known bytes")
@@ -130,6 +133,12 @@ def main() -> int:
 
     def step() -> None:
         prop.run_many_device(seeds, S, S, realized, durations, cause, stream)
+
+    if args.mode == "propagate":
+        step()  # durations to inject
+
+        def step() -> None:  # noqa: F811
+            prop.propagate_device(durations, S, S, realized, cause, stream)
 
     for _ in range(args.warmup):
         step()
@@ -189,6 +198,7 @@ def main() -> int:
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_per_step,
+        "mode": args.mode,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

@@ -97,8 +97,8 @@ int32_t mcdp_propagator_activity_count(const mcdp_propagator *prop);
 /* Simulator::run_many (_core.cpp:355-361) on device buffers, asynchronous on
  * `stream` (a hipStream_t, NULL = default stream).  seeds_dev: int32[n] device;
  * outputs event-major with leading dimension ld >= n.  durations_dev or
- * cause_dev may be NULL to skip that output; realized_dev may be NULL only if
- * the context keeps every predecessor in LDS (otherwise MCDP_E_INVALID). */
+ * cause_dev may be NULL to skip that output; realized_dev is required (the
+ * kernel reads its own rows back within the launch). */
 int mcdp_run_many_device(mcdp_propagator *prop, const int32_t *seeds_dev, int64_t n, int64_t ld,
                          double *realized_dev, double *durations_dev, int32_t *cause_dev, void *stream);
 

@@ -51,6 +51,11 @@ int hip_fail(hipError_t e, const char *what) {
 // API-level distribution kinds (the reference's DistVar alternatives, :143).
 enum ApiKind { kApiConstant, kApiExponential, kApiGamma, kApiEmpAbs, kApiEmpRel };
 
+// A compiled distribution (kind, flags, parameters): records of edges of its
+// activity type carry kind, flags and p0 inline and its table index.
+using DistRec = DistParams;
+constexpr uint32_t kNoDistIndex = 0xFFFFFFFFu;  // the link's type has no distribution
+
 struct DistSpec {
     ApiKind kind;
     double a = 0.0, b = 0.0, c = 0.0;
@@ -192,6 +197,7 @@ int compile_gamma(double k, double th, double m, DistRec &D) {
         if (p_rej < kMinAcceptance) return fail(MCDP_E_INVALID, "gamma max_scale truncation too severe to sample");
         D.p[0] = th;
         D.p[3] = m;
+        D.flags = std::isinf(m) ? 0u : kInfoTruncated;
         if (k == 2.0) {
             D.kind = kGammaErlang2;
         } else if (k == 0.5) {
@@ -199,7 +205,7 @@ int compile_gamma(double k, double th, double m, DistRec &D) {
         } else {
             const double kp = k < 1.0 ? k + 1.0 : k;
             D.kind = kGammaMT;
-            D.boost = k < 1.0 ? 1 : 0;
+            D.flags |= k < 1.0 ? kInfoBoost : 0u;
             D.p[1] = kp - 1.0 / 3.0;
             D.p[2] = 1.0 / std::sqrt(9.0 * D.p[1]);
             D.p[4] = 1.0 / k;
@@ -207,6 +213,7 @@ int compile_gamma(double k, double th, double m, DistRec &D) {
     } else {
         if (p_pow < kMinAcceptance) return fail(MCDP_E_INVALID, "gamma max_scale truncation too severe to sample");
         D.kind = kGammaPow;
+        D.flags = kInfoTruncated;
         D.p[0] = th;
         D.p[1] = mp;
         D.p[2] = 1.0 / k;
@@ -231,15 +238,18 @@ int compile_empirical(const DistSpec &s, DistRec &D, std::vector<AliasRec> &tabl
     std::vector<double> prob;
     std::vector<int> alias;
     build_alias(s.weights, prob, alias);
+    if (table.size() + n > (size_t)std::numeric_limits<int32_t>::max()) return fail(MCDP_E_INVALID, "empirical tables too large");
     D.kind = abs ? kEmpiricalAbs : kEmpiricalRel;
-    D.n = (int32_t)n;
-    D.table_off = (int32_t)table.size();
+    // p0 carries (n, first alias record) as two uint32 (low, high).
+    const uint64_t packed = (uint64_t)(uint32_t)n | ((uint64_t)(uint32_t)table.size() << 32);
+    std::memcpy(&D.p[0], &packed, sizeof(double));
     for (size_t i = 0; i < n; ++i) table.push_back(AliasRec{prob[i], s.values[i], s.values[(size_t)alias[i]], 0.0});
     return MCDP_OK;
 }
 
 int compile_dist(const DistSpec &s, DistRec &D, std::vector<AliasRec> &table) {
     D = DistRec{};
+    D.kind = kNone;
     switch (s.kind) {
         case kApiConstant:
             D.kind = kConstant;
@@ -251,22 +261,25 @@ int compile_dist(const DistSpec &s, DistRec &D, std::vector<AliasRec> &table) {
     }
 }
 
-// Relative cost of one draw (VALU work), for balancing waves within a step.
+// Relative cost of one draw (VALU instructions, measured per method), for
+// balancing waves within a step.
 int draw_cost(const DistRec *D) {
-    if (!D) return 1;
+    if (!D) return 2;
     switch (D->kind) {
         case kConstant:
         case kEmpiricalAbs1:
-        case kEmpiricalRel1: return 2;
+        case kEmpiricalRel1: return 5;
         case kEmpiricalAbs:
-        case kEmpiricalRel: return 12;
-        case kExpInf:
-        case kExpTrunc: return 16;
-        case kGammaErlang2: return 18;
-        case kGammaHalf: return 24;
-        default: return 40;
+        case kEmpiricalRel: return 45;
+        case kExpInf: return 75;
+        case kExpTrunc: return 80;
+        case kGammaErlang2: return (D->flags & kInfoTruncated) ? 160 : 110;
+        case kGammaHalf: return (D->flags & kInfoTruncated) ? 180 : 130;
+        default: return 250;
     }
 }
+constexpr int kPredCost = 20;   // propagation work per predecessor record
+constexpr int kEventCost = 15;  // per event: header, final cap, stores
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
@@ -282,22 +295,36 @@ struct DeviceCopy {
     WalkParams base{};
 };
 
+// One predecessor entry as the reference evaluates it.
+struct PredEntry {
+    int32_t src;
+    int32_t edge;
+    bool stale;  // read before src is evaluated in the reference's order
+};
+
 struct mcdp_propagator {
     int32_t E = 0, L = 0;
-    // host plan
-    std::vector<int32_t> step_off;
-    std::vector<EventRec> ev;
-    std::vector<PredRec> pr;
-    std::vector<OrphanRec> orph;
+    double max_delay = 0.0;
+    // topology (schedule-independent)
+    std::vector<double> earliest;              // [E]
+    std::vector<std::vector<int32_t>> levels;  // events by level, in Kahn order
+    std::vector<int64_t> pred_off;             // [E+1] into preds (winning entry, input order)
+    std::vector<PredEntry> preds;
+    std::vector<double> dmin;                  // [L]
+    std::vector<uint32_t> ldist;               // [L] distribution index or kNoDistIndex
     std::vector<DistRec> dist;
     std::vector<AliasRec> alias;
-    int32_t n_steps = 0, waves = 1, slots = 0, max_per_wave = 0;
-    int64_t n_global = 0, n_stale = 0;
+    int64_t n_stale = 0;
+    // schedule (for `waves` waves per workgroup, at most `k` events per wave and step)
+    std::vector<int32_t> step_off, ev_off, slot_list;
+    std::vector<Record> recs, orph;
+    int32_t n_steps = 0, waves = 1, slots = 0, max_per_wave = 0, k = kMaxEventsPerWave;
+    int64_t n_global = 0;
     // device copies, one per device used
     std::mutex mu;
     std::vector<DeviceCopy> dev;
 
-    ~mcdp_propagator() {
+    void drop_device_copies() {
         for (auto &d : dev) {
             if (d.blob) {
                 int cur = -1;
@@ -307,12 +334,189 @@ struct mcdp_propagator {
                 if (cur >= 0) (void)hipSetDevice(cur);
             }
         }
+        dev.clear();
     }
+    ~mcdp_propagator() { drop_device_copies(); }
 };
 
 namespace {
 
 constexpr size_t kLdsBudgetBytes = 64 * 1024;  // LDS per workgroup for predecessor slots
+
+// A predecessor record for edge l: its minimal duration, sampling kind and p0
+// inline; the distribution index for the kinds that read more parameters.
+Record edge_record(const mcdp_propagator &P, int32_t l) {
+    Record r{};
+    r.i2 = l;
+    if (l >= 0 && l < P.L) {
+        r.a = P.dmin[(size_t)l];
+        r.info = kInfoEdgeValid;
+        const uint32_t di = P.ldist[(size_t)l];
+        if (di == kNoDistIndex) {
+            r.info |= kNone;
+        } else {
+            const DistRec &D = P.dist[di];
+            r.info |= D.kind | D.flags | (di << kInfoDistShift);
+            r.b = D.p[0];
+        }
+    } else {
+        r.info = kNone;  // undefined in the reference: contributes 0.0
+    }
+    return r;
+}
+
+// Steps, wave assignment, LDS slots and record streams for W waves per
+// workgroup and at most K events per wave and step.
+int build_schedule(mcdp_propagator &P, int32_t W, int32_t K) {
+    const int32_t E = P.E, L = P.L;
+    P.waves = W;
+    P.k = K;
+    P.step_off.assign(1, 0);
+    P.ev_off.assign(1, 0);
+    P.slot_list.clear();
+    P.recs.clear();
+    P.orph.clear();
+    P.max_per_wave = 0;
+    P.n_global = 0;
+
+    auto event_cost = [&](int32_t e) {
+        int c = kEventCost;
+        for (int64_t k = P.pred_off[(size_t)e]; k < P.pred_off[(size_t)e + 1]; ++k) {
+            const int32_t l = P.preds[(size_t)k].edge;
+            const DistRec *D = (l >= 0 && l < L && P.ldist[(size_t)l] != kNoDistIndex) ? &P.dist[P.ldist[(size_t)l]] : nullptr;
+            c += kPredCost + draw_cost(D);
+        }
+        return c;
+    };
+
+    // (a) steps: each level in as few steps as W x K events allow, split evenly;
+    // within a step, events go to the least-loaded wave (longest first).
+    std::vector<int32_t> ev_step((size_t)E, -1);
+    std::vector<int32_t> flat_events;  // stream order
+    int32_t step = 0;
+    for (const auto &lev : P.levels) {
+        if (lev.empty()) continue;
+        const size_t cap = (size_t)W * (size_t)K;
+        const size_t nsteps = (lev.size() + cap - 1) / cap;
+        for (size_t c = 0; c < nsteps; ++c) {
+            const size_t c0 = lev.size() * c / nsteps, c1 = lev.size() * (c + 1) / nsteps;
+            std::vector<std::pair<int, int32_t>> items;
+            for (size_t i = c0; i < c1; ++i) items.emplace_back(event_cost(lev[i]), lev[i]);
+            std::stable_sort(items.begin(), items.end(), [](auto &a, auto &b) { return a.first > b.first; });
+            std::vector<std::vector<int32_t>> per((size_t)W);
+            std::vector<int64_t> load((size_t)W, 0);
+            for (auto &it : items) {
+                int best = -1;
+                for (int w = 0; w < W; ++w)
+                    if ((int)per[(size_t)w].size() < K && (best < 0 || load[(size_t)w] < load[(size_t)best])) best = w;
+                per[(size_t)best].push_back(it.second);
+                load[(size_t)best] += it.first;
+            }
+            for (int w = 0; w < W; ++w) {
+                for (int32_t e : per[(size_t)w]) {
+                    flat_events.push_back(e);
+                    ev_step[(size_t)e] = step;
+                }
+                P.max_per_wave = std::max<int32_t>(P.max_per_wave, (int32_t)per[(size_t)w].size());
+                P.ev_off.push_back((int32_t)flat_events.size());
+            }
+            ++step;
+        }
+    }
+    P.n_steps = step;
+
+    // (b) LDS slots: an event read by a later step occupies a slot from the
+    // commit of its step to the end of the last step that reads it.
+    std::vector<int32_t> last_use((size_t)E, -1);
+    for (int32_t e = 0; e < E; ++e)
+        for (int64_t k = P.pred_off[(size_t)e]; k < P.pred_off[(size_t)e + 1]; ++k) {
+            const PredEntry &pe = P.preds[(size_t)k];
+            if (!pe.stale) last_use[(size_t)pe.src] = std::max(last_use[(size_t)pe.src], ev_step[(size_t)e]);
+        }
+    const int32_t slot_cap = (int32_t)(kLdsBudgetBytes / (kLanes * sizeof(double)));
+    std::vector<int32_t> slot_of((size_t)E, -1);
+    {
+        std::vector<std::vector<int32_t>> release((size_t)std::max(P.n_steps, 1));
+        std::vector<int32_t> free_slots;
+        int32_t used = 0;
+        for (int32_t t = 0; t < P.n_steps; ++t) {
+            for (int32_t e : release[(size_t)t]) free_slots.push_back(slot_of[(size_t)e]);
+            const int32_t b = P.ev_off[(size_t)t * W], en = P.ev_off[(size_t)(t + 1) * W];
+            for (int32_t i = b; i < en; ++i) {
+                const int32_t e = flat_events[(size_t)i];
+                if (last_use[(size_t)e] < 0) continue;
+                int32_t s = -1;
+                if (!free_slots.empty()) {
+                    s = free_slots.back();
+                    free_slots.pop_back();
+                } else if (used < slot_cap) {
+                    s = used++;
+                }
+                slot_of[(size_t)e] = s;
+                if (s >= 0) release[(size_t)last_use[(size_t)e]].push_back(e);
+            }
+        }
+        P.slots = used;
+    }
+
+    // (c) record streams: for each (step, wave), each event's header then its
+    // predecessors in input order.  The first predecessor record of an edge
+    // writes its duration row.
+    std::vector<uint8_t> referenced((size_t)L, 0);
+    P.recs.reserve(flat_events.size() + P.preds.size() + 1);
+    P.step_off.assign(1, 0);
+    for (size_t g = 0; g + 1 < P.ev_off.size(); ++g) {
+        for (int32_t i = P.ev_off[g]; i < P.ev_off[g + 1]; ++i) {
+            const int32_t e = flat_events[(size_t)i];
+            P.slot_list.push_back(slot_of[(size_t)e]);
+            Record h{};
+            h.a = P.earliest[(size_t)e];
+            h.b = P.earliest[(size_t)e] + P.max_delay;
+            h.i0 = e;
+            h.i1 = slot_of[(size_t)e];
+            h.i2 = (int32_t)(P.pred_off[(size_t)e + 1] - P.pred_off[(size_t)e]);
+            h.info = kInfoHeader;
+            P.recs.push_back(h);
+            for (int64_t k = P.pred_off[(size_t)e]; k < P.pred_off[(size_t)e + 1]; ++k) {
+                const PredEntry &pe = P.preds[(size_t)k];
+                Record r = edge_record(P, pe.edge);
+                r.i0 = pe.src;
+                if (pe.stale) {
+                    r.i1 = kSrcStale;
+                } else {
+                    r.i1 = slot_of[(size_t)pe.src] >= 0 ? slot_of[(size_t)pe.src] : kSrcGlobal;
+                    if (r.i1 == kSrcGlobal) ++P.n_global;
+                }
+                if (pe.edge >= 0 && pe.edge < L && !referenced[(size_t)pe.edge]) {
+                    referenced[(size_t)pe.edge] = 1;
+                    r.info |= kInfoOwner;
+                }
+                P.recs.push_back(r);
+            }
+        }
+        P.step_off.push_back((int32_t)P.recs.size());
+    }
+    P.recs.push_back(Record{});  // padding: the kernel prefetches one record ahead
+    if (P.recs.size() > (size_t)std::numeric_limits<int32_t>::max()) return fail(MCDP_E_INVALID, "plan too large");
+    for (int32_t l = 0; l < L; ++l)
+        if (!referenced[(size_t)l]) P.orph.push_back(edge_record(P, l));
+    return MCDP_OK;
+}
+
+// Default (waves, events per wave) for a DAG: enough waves to spread the widest
+// level (up to 8), and the smallest compiled bound that holds it in one step.
+void default_shape(const mcdp_propagator &P, int32_t &W, int32_t &K) {
+    size_t max_width = 0;
+    for (const auto &v : P.levels) max_width = std::max(max_width, v.size());
+    W = 1;
+    while (W < 8 && (size_t)W * 4 < max_width) W *= 2;
+    K = kMaxEventsPerWave;
+    for (int k : kEventsPerWaveChoices)
+        if ((size_t)k * (size_t)W >= max_width) {
+            K = k;
+            break;
+        }
+}
 
 int compile_propagator(mcdp_propagator &P, int32_t n_events, const double *earliest, int32_t n_acts,
                        const int32_t *act_idx, const double *act_dur, const int32_t *act_type, int32_t n_entries,
@@ -323,6 +527,7 @@ int compile_propagator(mcdp_propagator &P, int32_t n_events, const double *earli
     if (max_delay < 0.0) return fail(MCDP_E_MAX_DELAY, "max_delay must be non-negative");
     if (n_events < 0 || n_acts < 0 || n_entries < 0) return fail(MCDP_E_INVALID, "negative count");
     const int32_t E = n_events;
+    P.max_delay = max_delay;
 
     // (2) distributions.
     std::map<int32_t, int32_t> type_to_dist;
@@ -333,7 +538,7 @@ int compile_propagator(mcdp_propagator &P, int32_t n_events, const double *earli
         type_to_dist[kv.first] = (int32_t)P.dist.size();
         P.dist.push_back(D);
     }
-    if (P.dist.size() >= kNoDist) return fail(MCDP_E_INVALID, "too many distributions");
+    if (P.dist.size() > (size_t)kInfoDistMask) return fail(MCDP_E_INVALID, "too many distributions");
 
     // (3) link table.
     int64_t max_idx = -1;
@@ -342,21 +547,22 @@ int compile_propagator(mcdp_propagator &P, int32_t n_events, const double *earli
         max_idx = std::max<int64_t>(max_idx, act_idx[i]);
     }
     const int32_t L = (int32_t)(max_idx + 1);
-    std::vector<double> dmin((size_t)L, 0.0);
+    P.dmin.assign((size_t)L, 0.0);
     std::vector<int32_t> ltype((size_t)L, -1);
     std::vector<uint8_t> seen((size_t)L, 0);
     for (int32_t i = 0; i < n_acts; ++i) {
         const int32_t l = act_idx[i];
-        if (seen[(size_t)l] && (std::memcmp(&dmin[(size_t)l], &act_dur[i], sizeof(double)) != 0 || ltype[(size_t)l] != act_type[i]))
+        if (seen[(size_t)l] &&
+            (std::memcmp(&P.dmin[(size_t)l], &act_dur[i], sizeof(double)) != 0 || ltype[(size_t)l] != act_type[i]))
             return fail(MCDP_E_INVALID, "two activities share index " + std::to_string(l) + " with different attributes");
         seen[(size_t)l] = 1;
-        dmin[(size_t)l] = act_dur[i];
+        P.dmin[(size_t)l] = act_dur[i];
         ltype[(size_t)l] = act_type[i];
     }
-    std::vector<uint32_t> ldist((size_t)L, kNoDist);
+    P.ldist.assign((size_t)L, kNoDistIndex);
     for (int32_t l = 0; l < L; ++l) {
         auto it = type_to_dist.find(ltype[(size_t)l]);
-        if (it != type_to_dist.end()) ldist[(size_t)l] = (uint32_t)it->second;
+        if (it != type_to_dist.end()) P.ldist[(size_t)l] = (uint32_t)it->second;
     }
 
     // (4) topology.
@@ -399,166 +605,48 @@ int compile_propagator(mcdp_propagator &P, int32_t n_events, const double *earli
     }
     if ((int32_t)order.size() != E) return fail(MCDP_E_CYCLE, "Invalid DAG: cycle detected in precedence list");
 
-    // (5) levels.  A predecessor evaluated after its consumer in the
-    // reference's order is read before it is written there, i.e. at its
-    // earliest time ("stale"); every other predecessor is a true dependency.
+    // (5) predecessor lists and levels.  A predecessor evaluated after its
+    // consumer in the reference's order is read before it is written there,
+    // i.e. at its earliest time ("stale"); every other predecessor is a true
+    // dependency.
     std::vector<int32_t> pos((size_t)E), level((size_t)E, 0);
     for (int32_t i = 0; i < E; ++i) pos[(size_t)order[(size_t)i]] = i;
-    int32_t n_levels = E > 0 ? 1 : 0;
-    for (int32_t e : order) {
+    P.pred_off.assign((size_t)E + 1, 0);
+    for (int32_t e = 0; e < E; ++e) {
         const int32_t w = win[(size_t)e];
-        if (w < 0) continue;
-        int32_t lv = 0;
-        for (int64_t k = entry_off[w]; k < entry_off[w + 1]; ++k) {
-            const int32_t s = pred_event[k];
-            if (pos[(size_t)s] < pos[(size_t)e]) lv = std::max(lv, level[(size_t)s] + 1);
-        }
-        level[(size_t)e] = lv;
-        n_levels = std::max(n_levels, lv + 1);
+        P.pred_off[(size_t)e + 1] = P.pred_off[(size_t)e] + (w >= 0 ? entry_off[w + 1] - entry_off[w] : 0);
     }
-
-    // (6) steps and wave assignment.
-    std::vector<std::vector<int32_t>> by_level((size_t)n_levels);
-    for (int32_t e : order) by_level[(size_t)level[(size_t)e]].push_back(e);
-    size_t max_width = 0;
-    for (auto &v : by_level) max_width = std::max(max_width, v.size());
-    int32_t W = 1;
-    while (W < 8 && (size_t)W * 4 < max_width) W *= 2;
-    const int32_t K = kMaxEventsPerWave;
-    P.waves = W;
-
-    auto event_cost = [&](int32_t e) {
-        int c = 1;
-        const int32_t w = win[(size_t)e];
-        if (w < 0) return c;
-        for (int64_t k = entry_off[w]; k < entry_off[w + 1]; ++k) {
-            const int32_t l = pred_act[k];
-            const DistRec *D = (l >= 0 && l < L && ldist[(size_t)l] != kNoDist) ? &P.dist[ldist[(size_t)l]] : nullptr;
-            c += 1 + draw_cost(D);
-        }
-        return c;
-    };
-
-    std::vector<int32_t> ev_step((size_t)E, -1);
-    std::vector<std::vector<int32_t>> step_waves;  // flattened: per step, per wave, events
-    P.step_off.assign(1, 0);
-    std::vector<int32_t> flat_events;
-    int32_t step = 0;
-    for (auto &lev : by_level) {
-        for (size_t c0 = 0; c0 < lev.size(); c0 += (size_t)W * K) {
-            const size_t c1 = std::min(lev.size(), c0 + (size_t)W * K);
-            std::vector<std::pair<int, int32_t>> items;
-            for (size_t i = c0; i < c1; ++i) items.emplace_back(event_cost(lev[i]), lev[i]);
-            std::stable_sort(items.begin(), items.end(), [](auto &a, auto &b) { return a.first > b.first; });
-            std::vector<std::vector<int32_t>> per((size_t)W);
-            std::vector<int64_t> load((size_t)W, 0);
-            for (auto &it : items) {
-                int best = -1;
-                for (int w = 0; w < W; ++w)
-                    if ((int)per[(size_t)w].size() < K && (best < 0 || load[(size_t)w] < load[(size_t)best])) best = w;
-                per[(size_t)best].push_back(it.second);
-                load[(size_t)best] += it.first;
-            }
-            for (int w = 0; w < W; ++w) {
-                for (int32_t e : per[(size_t)w]) {
-                    flat_events.push_back(e);
-                    ev_step[(size_t)e] = step;
-                }
-                P.max_per_wave = std::max<int32_t>(P.max_per_wave, (int32_t)per[(size_t)w].size());
-                P.step_off.push_back((int32_t)flat_events.size());
-            }
-            ++step;
-        }
-    }
-    P.n_steps = step;
-
-    // (7) LDS slots: an event read by a later step occupies a slot from the
-    // commit of its step to the end of the last step that reads it.
-    std::vector<int32_t> last_use((size_t)E, -1);
+    P.preds.resize((size_t)P.pred_off[(size_t)E]);
     for (int32_t e = 0; e < E; ++e) {
         const int32_t w = win[(size_t)e];
         if (w < 0) continue;
-        for (int64_t k = entry_off[w]; k < entry_off[w + 1]; ++k) {
+        int64_t o = P.pred_off[(size_t)e];
+        for (int64_t k = entry_off[w]; k < entry_off[w + 1]; ++k, ++o) {
             const int32_t s = pred_event[k];
-            if (pos[(size_t)s] < pos[(size_t)e]) last_use[(size_t)s] = std::max(last_use[(size_t)s], ev_step[(size_t)e]);
+            const bool stale = pos[(size_t)s] >= pos[(size_t)e];
+            P.preds[(size_t)o] = PredEntry{s, pred_act[k], stale};
+            if (stale) ++P.n_stale;
         }
     }
-    const int32_t slot_cap = (int32_t)(kLdsBudgetBytes / (kLanes * sizeof(double)));
-    std::vector<int32_t> slot_of((size_t)E, -1);
-    {
-        std::vector<std::vector<int32_t>> release((size_t)std::max(P.n_steps, 1));
-        std::vector<int32_t> free_slots;
-        int32_t used = 0;
-        for (int32_t t = 0; t < P.n_steps; ++t) {
-            for (int32_t e : release[(size_t)t]) free_slots.push_back(slot_of[(size_t)e]);
-            const int32_t b = P.step_off[(size_t)t * W], en = P.step_off[(size_t)(t + 1) * W];
-            for (int32_t i = b; i < en; ++i) {
-                const int32_t e = flat_events[(size_t)i];
-                if (last_use[(size_t)e] < 0) continue;
-                int32_t s = -1;
-                if (!free_slots.empty()) {
-                    s = free_slots.back();
-                    free_slots.pop_back();
-                } else if (used < slot_cap) {
-                    s = used++;
-                }
-                slot_of[(size_t)e] = s;
-                if (s >= 0) release[(size_t)last_use[(size_t)e]].push_back(e);
-            }
-        }
-        P.slots = used;
+    int32_t n_levels = E > 0 ? 1 : 0;
+    for (int32_t e : order) {
+        int32_t lv = 0;
+        for (int64_t k = P.pred_off[(size_t)e]; k < P.pred_off[(size_t)e + 1]; ++k)
+            if (!P.preds[(size_t)k].stale) lv = std::max(lv, level[(size_t)P.preds[(size_t)k].src] + 1);
+        level[(size_t)e] = lv;
+        n_levels = std::max(n_levels, lv + 1);
     }
-
-    // (8) records.
-    P.ev.reserve(flat_events.size());
-    std::vector<uint8_t> referenced((size_t)L, 0);
-    for (int32_t e : flat_events) {
-        EventRec er{};
-        er.earliest = earliest[e];
-        er.upper = earliest[e] + max_delay;
-        er.event = e;
-        er.slot = slot_of[(size_t)e];
-        er.pred_begin = (int32_t)P.pr.size();
-        const int32_t w = win[(size_t)e];
-        if (w >= 0) {
-            for (int64_t k = entry_off[w]; k < entry_off[w + 1]; ++k) {
-                PredRec r{};
-                const int32_t s = pred_event[k], l = pred_act[k];
-                r.src_event = s;
-                r.src_value = earliest[s];
-                if (pos[(size_t)s] < pos[(size_t)e]) {
-                    r.src_slot = slot_of[(size_t)s] >= 0 ? slot_of[(size_t)s] : kSrcGlobal;
-                    if (r.src_slot == kSrcGlobal) ++P.n_global;
-                } else {
-                    r.src_slot = kSrcStale;
-                    ++P.n_stale;
-                }
-                r.edge = l;
-                if (l >= 0 && l < L) {
-                    r.info = ldist[(size_t)l] | kInfoEdgeValid;
-                    r.dmin = dmin[(size_t)l];
-                    if (!referenced[(size_t)l]) {
-                        referenced[(size_t)l] = 1;
-                        r.info |= kInfoOwner;
-                    }
-                } else {
-                    r.info = kNoDist;  // undefined in the reference: contributes 0.0
-                    r.dmin = 0.0;
-                }
-                P.pr.push_back(r);
-            }
-        }
-        er.pred_count = (int32_t)P.pr.size() - er.pred_begin;
-        P.ev.push_back(er);
-    }
-    for (int32_t l = 0; l < L; ++l)
-        if (!referenced[(size_t)l]) P.orph.push_back(OrphanRec{l, ldist[(size_t)l], dmin[(size_t)l]});
-    if (P.pr.size() > (size_t)std::numeric_limits<int32_t>::max())
-        return fail(MCDP_E_INVALID, "too many predecessor entries");
+    P.levels.assign((size_t)n_levels, {});
+    for (int32_t e : order) P.levels[(size_t)level[(size_t)e]].push_back(e);
 
     P.E = E;
     P.L = L;
-    return MCDP_OK;
+    P.earliest.assign(earliest, earliest + E);
+
+    // (6) default schedule.
+    int32_t W, K;
+    default_shape(P, W, K);
+    return build_schedule(P, W, K);
 }
 
 // Uploads the plan of `P` to the current device (once per device).
@@ -575,29 +663,32 @@ int ensure_device(mcdp_propagator &P, DeviceCopy *&out) {
         }
     const size_t a = 256;
     size_t off = 0;
-    const size_t o_step = off;
-    off = align_up(off + P.step_off.size() * sizeof(int32_t), a);
-    const size_t o_ev = off;
-    off = align_up(off + P.ev.size() * sizeof(EventRec), a);
-    const size_t o_pr = off;
-    off = align_up(off + P.pr.size() * sizeof(PredRec), a);
-    const size_t o_or = off;
-    off = align_up(off + P.orph.size() * sizeof(OrphanRec), a);
-    const size_t o_di = off;
-    off = align_up(off + P.dist.size() * sizeof(DistRec), a);
-    const size_t o_al = off;
-    off = align_up(off + P.alias.size() * sizeof(AliasRec), a);
+    auto reserve = [&](size_t bytes) {
+        const size_t o = off;
+        off = align_up(off + bytes, a);
+        return o;
+    };
+    const size_t o_step = reserve(P.step_off.size() * sizeof(int32_t));
+    const size_t o_recs = reserve(P.recs.size() * sizeof(Record));
+    const size_t o_evoff = reserve(P.ev_off.size() * sizeof(int32_t));
+    const size_t o_slots = reserve(P.slot_list.size() * sizeof(int32_t));
+    const size_t o_orph = reserve(P.orph.size() * sizeof(Record));
+    const size_t o_dist = reserve(P.dist.size() * sizeof(DistParams));
+    const size_t o_alias = reserve(P.alias.size() * sizeof(AliasRec));
+    const size_t o_earl = reserve(P.earliest.size() * sizeof(double));
     const size_t total = std::max<size_t>(off, a);
     std::vector<unsigned char> host(total, 0);
     auto put = [&](size_t o, const void *src, size_t bytes) {
         if (bytes) std::memcpy(host.data() + o, src, bytes);
     };
     put(o_step, P.step_off.data(), P.step_off.size() * sizeof(int32_t));
-    put(o_ev, P.ev.data(), P.ev.size() * sizeof(EventRec));
-    put(o_pr, P.pr.data(), P.pr.size() * sizeof(PredRec));
-    put(o_or, P.orph.data(), P.orph.size() * sizeof(OrphanRec));
-    put(o_di, P.dist.data(), P.dist.size() * sizeof(DistRec));
-    put(o_al, P.alias.data(), P.alias.size() * sizeof(AliasRec));
+    put(o_recs, P.recs.data(), P.recs.size() * sizeof(Record));
+    put(o_evoff, P.ev_off.data(), P.ev_off.size() * sizeof(int32_t));
+    put(o_slots, P.slot_list.data(), P.slot_list.size() * sizeof(int32_t));
+    put(o_orph, P.orph.data(), P.orph.size() * sizeof(Record));
+    put(o_dist, P.dist.data(), P.dist.size() * sizeof(DistParams));
+    put(o_alias, P.alias.data(), P.alias.size() * sizeof(AliasRec));
+    put(o_earl, P.earliest.data(), P.earliest.size() * sizeof(double));
     void *blob = nullptr;
     hipError_t e = hipMalloc(&blob, total);
     if (e != hipSuccess) return fail(MCDP_E_NOMEM, std::string("hipMalloc (plan): ") + hipGetErrorString(e));
@@ -611,17 +702,34 @@ int ensure_device(mcdp_propagator &P, DeviceCopy *&out) {
     d.blob = blob;
     auto *b = static_cast<unsigned char *>(blob);
     d.base.step_off = reinterpret_cast<const int32_t *>(b + o_step);
-    d.base.ev = reinterpret_cast<const EventRec *>(b + o_ev);
-    d.base.pr = reinterpret_cast<const PredRec *>(b + o_pr);
-    d.base.orph = reinterpret_cast<const OrphanRec *>(b + o_or);
-    d.base.dist = reinterpret_cast<const DistRec *>(b + o_di);
-    d.base.alias = reinterpret_cast<const AliasRec *>(b + o_al);
+    d.base.recs = reinterpret_cast<const Record *>(b + o_recs);
+    d.base.ev_off = reinterpret_cast<const int32_t *>(b + o_evoff);
+    d.base.slot_list = reinterpret_cast<const int32_t *>(b + o_slots);
+    d.base.orph = reinterpret_cast<const Record *>(b + o_orph);
+    d.base.dists = reinterpret_cast<const DistParams *>(b + o_dist);
+    d.base.alias = reinterpret_cast<const AliasRec *>(b + o_alias);
+    d.base.earliest = reinterpret_cast<const double *>(b + o_earl);
     d.base.n_steps = P.n_steps;
     d.base.n_orphans = (int32_t)P.orph.size();
     d.base.waves = P.waves;
+    d.base.n_alias = (int32_t)P.alias.size();
+    d.base.slots = P.slots;
     P.dev.push_back(d);
     out = &P.dev.back();
     return MCDP_OK;
+}
+// The fast kernel variant needs every read predecessor in LDS (or stale) and
+// the alias tables small enough to stage in LDS.
+WalkVariant walk_variant(const mcdp_propagator &P, bool injected) {
+    const bool alias_fits = P.alias.size() * sizeof(AliasRec) <= kAliasLdsMaxBytes;
+    const bool fast = P.n_global == 0 && (injected || alias_fits);
+    return WalkVariant{injected, fast ? kPathsFast : kPathsGeneral, P.k};
+}
+
+size_t walk_lds_bytes(const mcdp_propagator &P, WalkVariant v) {
+    size_t bytes = (size_t)P.slots * kLanes * sizeof(double);
+    if (!v.injected && v.paths == kPathsFast) bytes += P.alias.size() * sizeof(AliasRec);
+    return bytes;
 }
 
 int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double *dur_in, int64_t n, int64_t ld,
@@ -630,8 +738,7 @@ int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double
     if (n > 0 && ld < n) return fail(MCDP_E_INVALID, "ld must be >= n");
     if (injected && !dur_in && P.L > 0 && n > 0) return fail(MCDP_E_INVALID, "durations input is null");
     if (!injected && !seeds && n > 0) return fail(MCDP_E_INVALID, "seeds is null");
-    if (!realized && P.n_global > 0 && n > 0)
-        return fail(MCDP_E_INVALID, "this context reads predecessors from the realized output; realized must not be null");
+    if (!realized && P.E > 0 && n > 0) return fail(MCDP_E_INVALID, "realized must not be null");
     std::lock_guard<std::mutex> lock(P.mu);
     DeviceCopy *d = nullptr;
     int rc = ensure_device(P, d);
@@ -646,13 +753,14 @@ int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double
     p.realized = realized;
     p.durations = injected ? nullptr : durations;
     p.cause = cause;
-    const size_t lds = (size_t)P.slots * kLanes * sizeof(double);
+    const WalkVariant v = walk_variant(P, injected);
+    const size_t lds = walk_lds_bytes(P, v);
     int cus = 0, bpc = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device), "hipDeviceGetAttribute");
-    HIP_TRY(walk_occupancy(injected, P.waves, lds, &bpc), "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    HIP_TRY(walk_occupancy(v, P.waves, lds, &bpc), "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     if (bpc < 1) return fail(MCDP_E_DEVICE, "walk kernel cannot be resident with " + std::to_string(lds) + " bytes of LDS");
     const int64_t grid = std::min<int64_t>(p.n_tiles, (int64_t)cus * bpc);
-    HIP_TRY(launch_walk(p, injected, (int)grid, lds, stream), "walk kernel launch");
+    HIP_TRY(launch_walk(p, v, (int)grid, lds, stream), "walk kernel launch");
     return MCDP_OK;
 }
 

@@ -3,8 +3,10 @@
 // walk_kernel fuses the two hot loops of Simulator::run (_core.cpp:323-329
 // delay sampling, :332-350 max-plus propagation) for 64 samples per
 // workgroup, one sample per lane:
-//   * plan records (events, predecessors, distributions) are wave-uniform and
-//     arrive through scalar loads; the distribution switch is a scalar branch;
+//   * plan records (events, predecessors with their distribution inline) are
+//     wave-uniform and arrive through scalar loads, the next one prefetched
+//     while the current one is sampled; the distribution switch is a scalar
+//     branch;
 //   * realized times of predecessors that later steps read live in LDS
 //     (slot-major, 64 doubles per slot, so every access is one conflict-free
 //     ds_read_b64 / ds_write_b64 per lane);
@@ -16,12 +18,14 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "mcdp_kernels.hpp"
 #include "mcdp_sampler.hpp"
 
 namespace mcdp {
 
-template <bool kInjected>
+template <bool kInjected, int kPaths, int K>
 __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) {
     extern __shared__ double lds[];
     const int lane = threadIdx.x & (kLanes - 1);
@@ -31,10 +35,24 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) 
     // The plan never changes during a launch: reading it through the constant
     // address space lets wave-uniform record loads become scalar loads.
     const auto step_off = as_const_space(p.step_off);
-    const auto evs = as_const_space(p.ev);
-    const auto prs = as_const_space(p.pr);
-    const auto dists = as_const_space(p.dist);
-    const auto alias = as_const_space(p.alias);
+    const auto recs = as_const_space(p.recs);
+    const auto ev_off = as_const_space(p.ev_off);
+    const auto slot_list = as_const_space(p.slot_list);
+    const auto dists = as_const_space(p.dists);
+    const auto earliest = as_const_space(p.earliest);
+
+    // Fast variant: stage the alias tables in LDS after the slots, so an
+    // empirical draw is a ds_read rather than a vector memory load.
+    AliasRec *const lds_alias = reinterpret_cast<AliasRec *>(lds + (size_t)p.slots * kLanes);
+    if (!kInjected && kPaths == kPathsFast) {
+        const auto src = as_const_space(p.alias);
+        for (int i = threadIdx.x; i < p.n_alias; i += blockDim.x) lds_alias[i] = src[i];
+        __syncthreads();
+    }
+    auto draw = [&](const Record &r, SeedKey key) -> double {
+        if (kPaths == kPathsFast) return draw_duration(r.info, r.a, r.b, (uint32_t)r.i2, key, lds_alias, dists);
+        return draw_duration(r.info, r.a, r.b, (uint32_t)r.i2, key, as_const_space(p.alias), dists);
+    };
 
     for (int64_t tile = blockIdx.x; tile < p.n_tiles; tile += gridDim.x) {
         const int64_t s = tile * kLanes + lane;
@@ -43,55 +61,71 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) 
         if (!kInjected) key = seed_key(active ? p.seeds[s] : 0);
 
         for (int t = 0; t < p.n_steps; ++t) {
-            const int b = step_off[t * W + wave];
-            const int e = step_off[t * W + wave + 1];
-            double res[kMaxEventsPerWave];
-            for (int i = b; i < e; ++i) {
-                const EventRec er = evs[i];
-                double best = er.earliest;
-                int32_t who = -1;
-                for (int k = er.pred_begin; k < er.pred_begin + er.pred_count; ++k) {
-                    const PredRec pr = prs[k];
-                    double rs;
-                    if (pr.src_slot >= 0) {
-                        rs = lds[pr.src_slot * kLanes + lane];
-                    } else if (pr.src_slot == kSrcGlobal) {
-                        rs = active ? p.realized[(int64_t)pr.src_event * ld + s] : 0.0;
+            const int r0 = step_off[t * W + wave];
+            const int r1 = step_off[t * W + wave + 1];
+            double res[K];
+            int j = 0;
+            if (r0 < r1) {
+                // One stream of records: an event header, then its predecessors
+                // in input order.  The stream ends in a padding record, so the
+                // one-ahead prefetch never leaves the array.
+                Record cur = recs[r0];
+                double best = 0.0, upper = 0.0;
+                int32_t who = -1, event = 0, remaining = -1;
+                for (int r = r0; r < r1; ++r) {
+                    const Record nxt = recs[r + 1];
+                    if (cur.info & kInfoHeader) {
+                        best = cur.a;
+                        upper = cur.b;
+                        event = cur.i0;
+                        who = -1;
+                        remaining = cur.i2;
                     } else {
-                        rs = pr.src_value;
-                    }
-                    double dur = 0.0;
-                    if (pr.info & kInfoEdgeValid) {
-                        if (kInjected) {
-                            dur = active ? __builtin_nontemporal_load(p.dur_in + (int64_t)pr.edge * ld + s) : 0.0;
-                        } else {
-                            const uint32_t di = pr.info & kInfoDistMask;
-                            dur = (di == kNoDist) ? pr.dmin
-                                                  : draw_duration(dists[di], alias, (uint32_t)pr.edge, pr.dmin, key);
-                            if ((pr.info & kInfoOwner) && p.durations && active)
-                                __builtin_nontemporal_store(dur, p.durations + (int64_t)pr.edge * ld + s);
+                        double dur = 0.0;
+                        if (cur.info & kInfoEdgeValid) {
+                            if (kInjected) {
+                                dur = active ? __builtin_nontemporal_load(p.dur_in + (int64_t)cur.i2 * ld + s) : 0.0;
+                            } else {
+                                dur = draw(cur, key);
+                                if ((cur.info & kInfoOwner) && p.durations && active)
+                                    __builtin_nontemporal_store(dur, p.durations + (int64_t)cur.i2 * ld + s);
+                            }
                         }
+                        double rs;
+                        if (cur.i1 >= 0) {
+                            rs = lds[cur.i1 * kLanes + lane];
+                        } else if (kPaths == kPathsGeneral && cur.i1 == kSrcGlobal) {
+                            rs = active ? p.realized[(int64_t)cur.i0 * ld + s] : 0.0;
+                        } else {
+                            rs = earliest[cur.i0];
+                        }
+                        double tt = rs + dur;
+                        tt = (upper < tt) ? upper : tt;
+                        if (tt >= best) {
+                            best = tt;
+                            who = cur.i0;
+                        }
+                        --remaining;
                     }
-                    double tt = rs + dur;
-                    tt = (er.upper < tt) ? er.upper : tt;
-                    if (tt >= best) {
-                        best = tt;
-                        who = pr.src_event;
+                    if (remaining == 0) {  // the event's last predecessor (or a header with none)
+                        const double rr = (upper < best) ? upper : best;
+                        if (active) {
+                            __builtin_nontemporal_store(rr, p.realized + (int64_t)event * ld + s);
+                            if (p.cause) __builtin_nontemporal_store(who, p.cause + (int64_t)event * ld + s);
+                        }
+                        res[j++] = rr;
+                        remaining = -1;
                     }
+                    cur = nxt;
                 }
-                const double r = (er.upper < best) ? er.upper : best;
-                if (active) {
-                    if (p.realized) __builtin_nontemporal_store(r, p.realized + (int64_t)er.event * ld + s);
-                    if (p.cause) __builtin_nontemporal_store(who, p.cause + (int64_t)er.event * ld + s);
-                }
-                res[i - b] = r;
             }
             // Every read of this step is done before any slot is overwritten,
             // and every commit is visible before the next step reads it.
             __syncthreads();
-            for (int i = b; i < e; ++i) {
-                const int slot = evs[i].slot;
-                if (slot >= 0) lds[slot * kLanes + lane] = res[i - b];
+            const int e0 = ev_off[t * W + wave];
+            for (int i = 0; i < j; ++i) {
+                const int slot = slot_list[e0 + i];
+                if (slot >= 0) lds[slot * kLanes + lane] = res[i];
             }
             __syncthreads();
         }
@@ -99,12 +133,10 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) 
         // Activities no predecessor entry references still get a duration row.
         if (!kInjected && p.durations) {
             const auto orph = as_const_space(p.orph);
-            for (int j = wave; j < p.n_orphans; j += W) {
-                const OrphanRec o = orph[j];
-                const uint32_t di = o.info & kInfoDistMask;
-                const double dur =
-                    (di == kNoDist) ? o.dmin : draw_duration(dists[di], alias, (uint32_t)o.edge, o.dmin, key);
-                if (active) __builtin_nontemporal_store(dur, p.durations + (int64_t)o.edge * ld + s);
+            for (int i = wave; i < p.n_orphans; i += W) {
+                const Record o = orph[i];
+                const double dur = draw(o, key);
+                if (active) __builtin_nontemporal_store(dur, p.durations + (int64_t)o.i2 * ld + s);
             }
         }
     }
@@ -140,19 +172,42 @@ hipError_t launch_copy(double *dst, const double *src, int64_t n, hipStream_t st
     return hipGetLastError();
 }
 
-hipError_t launch_walk(const WalkParams &p, bool injected, int grid, size_t lds_bytes, hipStream_t stream) {
-    const dim3 block(p.waves * kLanes);
-    if (injected)
-        hipLaunchKernelGGL(walk_kernel<true>, dim3(grid), block, lds_bytes, stream, p);
-    else
-        hipLaunchKernelGGL(walk_kernel<false>, dim3(grid), block, lds_bytes, stream, p);
-    return hipGetLastError();
+template <bool kInjected, int kPaths>
+const void *walk_symbol(int k) {
+    switch (k) {
+        case 4: return (const void *)walk_kernel<kInjected, kPaths, 4>;
+        case 8: return (const void *)walk_kernel<kInjected, kPaths, 8>;
+        case 12: return (const void *)walk_kernel<kInjected, kPaths, 12>;
+        case 16: return (const void *)walk_kernel<kInjected, kPaths, 16>;
+        default: return nullptr;
+    }
 }
 
-hipError_t walk_occupancy(bool injected, int waves, size_t lds_bytes, int *blocks_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        blocks_per_cu, injected ? (const void *)walk_kernel<true> : (const void *)walk_kernel<false>,
-        waves * kLanes, lds_bytes);
+const void *walk_symbol(WalkVariant v) {
+    if (v.injected) return v.paths == kPathsFast ? walk_symbol<true, kPathsFast>(v.k) : walk_symbol<true, kPathsGeneral>(v.k);
+    return v.paths == kPathsFast ? walk_symbol<false, kPathsFast>(v.k) : walk_symbol<false, kPathsGeneral>(v.k);
+}
+
+hipError_t launch_walk(const WalkParams &p, WalkVariant v, int grid, size_t lds_bytes, hipStream_t stream) {
+    const void *fn = walk_symbol(v);
+    if (!fn) return hipErrorInvalidValue;
+    if (lds_bytes > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+        if (e != hipSuccess) return e;
+    }
+    WalkParams args = p;
+    void *kargs[] = {&args};
+    return hipLaunchKernel(fn, dim3(grid), dim3(p.waves * kLanes), kargs, lds_bytes, stream);
+}
+
+hipError_t walk_occupancy(WalkVariant v, int waves, size_t lds_bytes, int *blocks_per_cu) {
+    const void *fn = walk_symbol(v);
+    if (!fn) return hipErrorInvalidValue;
+    if (lds_bytes > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+        if (e != hipSuccess) return e;
+    }
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, waves * kLanes, lds_bytes);
 }
 
 hipError_t launch_philox_debug(int variant, int64_t n, const uint32_t *ctr, const uint32_t *key, uint32_t *out,

@@ -8,32 +8,56 @@
 
 namespace mcdp {
 
-constexpr int kMaxWaves = 16;           // waves per workgroup (1024 threads)
-constexpr int kMaxEventsPerWave = 16;   // events one wave evaluates in one step
+constexpr int kMaxWaves = 16;  // waves per workgroup (1024 threads)
+
+// Events one wave evaluates in one step: a compile-time bound, because each
+// wave keeps its step's results in registers until the commit (beyond 16 the
+// compiler moves that array to scratch memory).  The host builds the schedule
+// for one of these bounds.
+constexpr int kEventsPerWaveChoices[] = {4, 8, 12, 16};
+constexpr int kMaxEventsPerWave = 16;
+
+// Alias tables up to this size are staged in LDS once per workgroup.
+constexpr size_t kAliasLdsMaxBytes = 16 * 1024;
+
+// Kernel variants: the fast one assumes every predecessor the plan reads is
+// in LDS (or stale) and the alias tables fit in LDS, so the predecessor loop
+// issues no vector memory load (a vector load there would also wait for all
+// of the wave's outstanding output stores: gfx9 counts both in vmcnt).
+enum WalkPaths : int { kPathsFast = 0, kPathsGeneral = 1 };
 
 struct WalkParams {
-    const int32_t *step_off;  // [n_steps * waves + 1]
-    const EventRec *ev;
-    const PredRec *pr;
-    const OrphanRec *orph;
-    const DistRec *dist;
+    const int32_t *step_off;   // [n_steps * waves + 1]: record range of (step, wave)
+    const Record *recs;        // record streams, ending with one padding record
+    const int32_t *ev_off;     // [n_steps * waves + 1]: event range of (step, wave) in slot_list
+    const int32_t *slot_list;  // LDS slot of each event in stream order (-1: none)
+    const Record *orph;        // activities no predecessor entry references
+    const DistParams *dists;   // compiled distributions (p1..p4 of the rarer kinds)
     const AliasRec *alias;
+    const double *earliest;    // [E]: realized value of a stale predecessor
     int32_t n_steps;
     int32_t n_orphans;
     int32_t waves;
-    int32_t pad;
+    int32_t n_alias;          // AliasRecs (staged to LDS after the slots in the fast variant)
+    int32_t slots;            // LDS predecessor slots
     const int32_t *seeds;   // sampled mode: int32[n]
     const double *dur_in;   // injected mode: [L][ld]
     int64_t n;              // samples
     int64_t ld;             // leading dimension of every per-sample array
     int64_t n_tiles;        // ceil(n / 64)
-    double *realized;       // [E][ld] or null
+    double *realized;       // [E][ld] (read within the launch when a plan spills slots)
     double *durations;      // [L][ld] or null (sampled mode)
     int32_t *cause;         // [E][ld] or null
 };
 
-hipError_t launch_walk(const WalkParams &p, bool injected, int grid, size_t lds_bytes, hipStream_t stream);
-hipError_t walk_occupancy(bool injected, int waves, size_t lds_bytes, int *blocks_per_cu);
+struct WalkVariant {
+    bool injected;  // durations read from dur_in instead of sampled
+    int paths;      // WalkPaths
+    int k;          // one of kEventsPerWaveChoices
+};
+
+hipError_t launch_walk(const WalkParams &p, WalkVariant v, int grid, size_t lds_bytes, hipStream_t stream);
+hipError_t walk_occupancy(WalkVariant v, int waves, size_t lds_bytes, int *blocks_per_cu);
 hipError_t launch_copy(double *dst, const double *src, int64_t n, hipStream_t stream);
 hipError_t launch_philox_debug(int variant, int64_t n, const uint32_t *ctr, const uint32_t *key, uint32_t *out,
                                hipStream_t stream);

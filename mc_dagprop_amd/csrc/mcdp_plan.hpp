@@ -8,6 +8,11 @@
 // workgroup owns 64 samples (one per lane), so all control flow and all plan
 // records are wave-uniform and come through scalar loads; only the per-sample
 // values (realized times, durations, random draws) live in vector registers.
+//
+// Each (step, wave) owns one contiguous stream of 32-byte records: an event
+// header followed by that event's predecessor records, for each of the wave's
+// events of the step.  A wave walks its stream with one scalar load per
+// record, the next record in flight while the current one is sampled.
 #pragma once
 #include <stdint.h>
 
@@ -15,76 +20,68 @@ namespace mcdp {
 
 constexpr int kLanes = 64;  // wave64: one sample per lane
 
-// One event evaluated by one wave in one step.  32 bytes, 8-byte aligned so a
-// wave fetches it with a single scalar load.
-struct EventRec {
-    double earliest;   // EventTimestamp.earliest (initial realized value)
-    double upper;      // earliest + max_delay, rounded once on the host
-    int32_t event;     // event id: output row of realized / cause
-    int32_t slot;      // LDS slot receiving this event's realized, or -1
-    int32_t pred_begin;  // first PredRec
-    int32_t pred_count;  // number of PredRecs (input order of the winning entry)
-};
-
 // Where a predecessor's realized time is read from.
 enum : int32_t {
     kSrcGlobal = -1,  // from the realized output row (written in an earlier step)
     kSrcStale = -2,   // the reference reads this predecessor before it is evaluated:
-                      // use its earliest time (held in src_value)
-};
-
-// Flags packed in PredRec::info above the 24-bit distribution index.
-enum : uint32_t {
-    kInfoDistMask = 0x00FFFFFFu,  // distribution index; kNoDist = no distribution
-    kNoDist = 0x00FFFFFFu,
-    kInfoOwner = 1u << 30,        // this record writes durations[edge]
-    kInfoEdgeValid = 1u << 29,    // edge in [0, link_count); otherwise duration 0.0
-};
-
-// One predecessor entry (src, edge) of an event.  32 bytes.
-struct PredRec {
-    int32_t src_event;  // predecessor event id (the cause_event value)
-    int32_t src_slot;   // LDS slot (>= 0), kSrcGlobal or kSrcStale
-    int32_t edge;       // activity index (link) -- durations row and RNG counter
-    uint32_t info;      // distribution index | flags
-    double dmin;        // Activity.minimal_duration of the edge (0.0 for gaps)
-    double src_value;   // earliest time of src (used when src_slot == kSrcStale)
-};
-
-// An activity that no predecessor entry references still has a duration row
-// (the reference samples every link, _core.cpp:323-329).
-struct OrphanRec {
-    int32_t edge;
-    uint32_t info;  // distribution index (kNoDist if none)
-    double dmin;
+                      // use its earliest time (read from the earliest array)
 };
 
 // Sampling methods.  Each registered distribution compiles to one of these.
-enum DistKind : int32_t {
-    kConstant = 0,         // extra = d * p0
-    kExpInf = 1,           // X = -p0 * log(u), u in (0,1]                       (mean p0)
-    kExpTrunc = 2,         // X = -p0 * log(p1 + p3 u), u in (0,1], clamp at p2  (p1 = e^{-m/p0}, p3 = 1 - p1)
-    kGammaErlang2 = 3,     // X = -p0 * log(u1 * u2)                              (shape 2, scale p0)
-    kGammaHalf = 4,        // X = p0 * (-log u1) * cos^2(2 pi u2)                 (shape 1/2, scale p0)
-    kGammaMT = 5,          // Marsaglia-Tsang, shape p1 >= 1 (or p1 + 1 with boost), scale p0
-    kGammaPow = 6,         // truncated gamma on [0, m]: X = m' U^{1/k}, accept w.p. e^{-X}
-    kEmpiricalAbs = 7,     // extra = v[alias draw]
-    kEmpiricalRel = 8,     // extra = f[alias draw] * d
-    kEmpiricalAbs1 = 9,    // single bin: extra = p0 (no draw)
-    kEmpiricalRel1 = 10,   // single bin: extra = p0 * d (no draw)
+// Parameters: p0 travels in the record; p1..p4 in the distribution table.
+enum DistKind : uint32_t {
+    kConstant = 0,        // extra = d * p0
+    kExpInf = 1,          // X = -p0 log u, u in (0,1]                           (mean p0)
+    kExpTrunc = 2,        // X = -p0 log(p1 + p3 u), u in (0,1], clamp at p2  (p1 = e^{-m/p0}, p3 = 1 - p1)
+    kGammaErlang2 = 3,    // X = -p0 log(u1 u2)                                  (shape 2, scale p0)
+    kGammaHalf = 4,       // X = p0 (-log u1) cos^2(2 pi u2)                     (shape 1/2, scale p0)
+    kGammaMT = 5,         // Marsaglia-Tsang: scale p0, d = p1, c = p2, 1/k = p4 (boost flag)
+    kGammaPow = 6,        // truncated on [0, m]: X = scale m' U^{1/k}, accept w.p. e^{-m' U^{1/k}}
+    kEmpiricalAbs = 7,    // extra = v[alias draw]; p0 holds (n, table_off) as two uint32
+    kEmpiricalRel = 8,    // extra = f[alias draw] * d
+    kEmpiricalAbs1 = 9,   // single bin: extra = p0 (no draw)
+    kEmpiricalRel1 = 10,  // single bin: extra = p0 * d (no draw)
+    kNone = 0xFF,         // no distribution: duration = minimal_duration
 };
 
-// A compiled distribution.  64 bytes.  Parameter meaning depends on kind:
-//   kGammaMT:  p0 = scale, p1 = d = k' - 1/3, p2 = c = 1/sqrt(9 d), p3 = max_scale,
-//              p4 = 1/k (boost exponent, used when boost != 0)
-//   kGammaPow: p0 = scale, p1 = m' = max_scale/scale, p2 = 1/k, p3 = max_scale
-//   kGammaErlang2 / kGammaHalf: p0 = scale, p3 = max_scale
-struct DistRec {
-    int32_t kind;
-    int32_t n;          // empirical: number of bins
-    int32_t table_off;  // empirical: first AliasRec
-    int32_t boost;      // kGammaMT: 1 when shape < 1 (sample shape+1, multiply by U^{1/k})
-    double p[6];
+// Record::info.
+enum : uint32_t {
+    kInfoKindMask = 0xFFu,       // DistKind (predecessor records)
+    kInfoBoost = 1u << 8,        // gamma shape < 1: Gamma(k + 1) U^{1/k}
+    kInfoTruncated = 1u << 9,    // gamma family: rejection on max_scale (table p3)
+    kInfoEdgeValid = 1u << 10,   // edge in [0, link_count); otherwise duration 0.0
+    kInfoOwner = 1u << 11,       // this record writes durations[edge]
+    kInfoDistShift = 12,         // bits 12..30: distribution table index
+    kInfoDistMask = 0x7FFFFu,
+    kInfoHeader = 1u << 31,      // an event header, not a predecessor
+};
+
+// A 32-byte plan record, read with one s_load_dwordx8.  Two shapes share it:
+//   event header (info & kInfoHeader):
+//     a = earliest, b = upper (earliest + max_delay, rounded once on the host),
+//     i0 = event id, i1 = LDS slot receiving its realized time (or -1),
+//     i2 = number of predecessor records that follow;
+//   predecessor (src, edge) of the event above, in the entry's input order:
+//     a = minimal_duration of the edge, b = p0 of its distribution,
+//     i0 = src event id (the cause_event value), i1 = src LDS slot or
+//     kSrcGlobal / kSrcStale, i2 = edge (activity index: durations row and
+//     RNG counter).
+struct Record {
+    double a;
+    double b;
+    int32_t i0;
+    int32_t i1;
+    int32_t i2;
+    uint32_t info;
+};
+static_assert(sizeof(Record) == 32, "Record is one s_load_dwordx8");
+
+// A compiled distribution: kind, flags and parameters.  Records carry p[0];
+// kinds that need more read this table (a scalar load, off the common path).
+struct DistParams {
+    uint32_t kind;   // DistKind
+    uint32_t flags;  // kInfoBoost | kInfoTruncated
+    double p[5];
 };
 
 // One bin of a Vose alias table with both candidate values resolved, so a

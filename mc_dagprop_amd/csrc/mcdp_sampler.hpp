@@ -125,96 +125,109 @@ __device__ __forceinline__ bool mt_attempt(const Words4 &w, double d, double c, 
     return log_fast(u3) < 0.5 * z2 + d * (1.0 - v + log_fast(v));
 }
 
-// Draws the duration of one activity for one sample.  `edge` is the activity
-// index, `dmin` its minimal duration, `D` its compiled distribution
-// (wave-uniform).  Returns base + extra rounded exactly as the reference
-// (_core.cpp:75, 88, 103, 125, 140, 328): extra = X * d (one rounding), then
-// d + extra (one rounding); this file is compiled with -ffp-contract=off.
-__device__ __forceinline__ double draw_duration(const DistRec &D, const_ptr<AliasRec> alias, uint32_t edge,
-                                                double dmin, SeedKey key) {
-    switch (D.kind) {
-        case kConstant: {
-            const double extra = dmin * D.p[0];
-            return dmin + extra;
-        }
-        case kEmpiricalAbs1: return dmin + D.p[0];
-        case kEmpiricalRel1: {
-            const double extra = D.p[0] * dmin;
-            return dmin + extra;
-        }
-        case kExpInf: {  // inversion: X = -mean log U, U in (0, 1]
-            const Words2 w = philox2x32_10(edge, kTagMain, key.lo);
-            const double X = -D.p[0] * log_fast(u01_open0(w.x, w.y));
-            const double extra = X * dmin;
-            return dmin + extra;
-        }
-        case kExpTrunc: {
-            // Inversion of the CDF truncated at m: U' = q + (1 - q) U is uniform on
-            // (q, 1] with q = e^{-m/mean}, and X = -mean log U' lies in [0, m].
-            const Words2 w = philox2x32_10(edge, kTagMain, key.lo);
-            const double u = __builtin_fma(u01_open0(w.x, w.y), D.p[3], D.p[1]);
-            double X = -D.p[0] * log_fast(u);
-            X = (X > D.p[2]) ? D.p[2] : X;
-            const double extra = X * dmin;
-            return dmin + extra;
-        }
-        case kEmpiricalAbs:
-        case kEmpiricalRel: {
-            // Vose alias table: one uniform picks the bin and the coin.
-            const Words2 w = philox2x32_10(edge, kTagMain, key.lo);
-            const double xn = u01(w.x, w.y) * (double)D.n;
-            int i = (int)xn;
-            i = (i >= D.n) ? D.n - 1 : i;
-            const double frac = xn - (double)i;
-            const const_ptr<AliasRec> rec = alias + D.table_off + i;
-            const double v = (frac < rec->prob) ? rec->v_self : rec->v_alias;
-            if (D.kind == kEmpiricalAbs) return dmin + v;
-            const double extra = v * dmin;
-            return dmin + extra;
-        }
-        default: break;
-    }
-    // Gamma family: rejection on the truncation bound max_scale (D.p[3]),
-    // with the attempt number in the counter so every lane's draw is fixed
-    // by (seed, edge) alone.
+// Gamma family with rejection: on the truncation bound max_scale (and, for
+// Marsaglia-Tsang and the power proposal, on the method's own test), with the
+// attempt number in the counter so every lane's draw is fixed by (seed, edge)
+// alone.  Returns X (scale included).
+__device__ __forceinline__ double gamma_rejection(uint32_t kind, uint32_t flags, const DistParams &D, uint32_t edge,
+                                               SeedKey key) {
     const double m = D.p[3];
     double X = 0.0;
     uint32_t j = 0;
     for (;;) {
         const Words4 w = philox4x32_10(edge, j, kTagMain, 0u, key.lo, key.hi);
         bool ok = true;
-        switch (D.kind) {
-            case kGammaErlang2:  // sum of two unit exponentials
-                X = -D.p[0] * log_fast(u01_open0(w.x, w.y) * u01_open0(w.z, w.w));
-                break;
-            case kGammaHalf: {  // Gamma(1/2) = N(0,1)^2 / 2 = Exp(1) cos^2(2 pi U)
-                const double cs = cos2pi_fast(u01_mid32(w.z));
-                X = D.p[0] * (-log_fast(u01_open0(w.x, w.y)) * (cs * cs));
-                break;
-            }
-            case kGammaMT: {
-                double y;
-                ok = mt_attempt(w, D.p[1], D.p[2], y);
-                if (ok) {
-                    if (D.boost) {  // shape < 1: Gamma(k) = Gamma(k + 1) U^{1/k}
-                        const Words2 b = philox2x32_10(edge, (j << 8) | kTagBoost, key.lo);
-                        y *= exp_fast(log_fast(u01_open0(b.x, b.y)) * D.p[4]);
-                    }
-                    X = D.p[0] * y;
+        if (kind == kGammaErlang2) {  // sum of two unit exponentials
+            X = -D.p[0] * log_fast(u01_open0(w.x, w.y) * u01_open0(w.z, w.w));
+        } else if (kind == kGammaHalf) {  // Gamma(1/2) = N(0,1)^2 / 2 = Exp(1) cos^2(2 pi U)
+            const double cs = cos2pi_fast(u01_mid32(w.z));
+            X = D.p[0] * (-log_fast(u01_open0(w.x, w.y)) * (cs * cs));
+        } else if (kind == kGammaMT) {
+            double y;
+            ok = mt_attempt(w, D.p[1], D.p[2], y);
+            if (ok) {
+                if (flags & kInfoBoost) {  // shape < 1: Gamma(k) = Gamma(k + 1) U^{1/k}
+                    const Words2 b = philox2x32_10(edge, (j << 8) | kTagBoost, key.lo);
+                    y *= exp_fast(log_fast(u01_open0(b.x, b.y)) * D.p[4]);
                 }
-                break;
+                X = D.p[0] * y;
             }
-            default: {  // kGammaPow: density x^{k-1} e^{-x} on [0, m'] by
-                        // proposal m' U^{1/k}, accepted with probability e^{-x}
-                const double y = D.p[1] * exp_fast(log_fast(u01_open0(w.x, w.y)) * D.p[2]);
-                ok = u01_open0(w.z, w.w) <= exp_fast(-y);
-                X = (y * D.p[0] > m) ? m : y * D.p[0];
-                break;
-            }
+        } else {  // kGammaPow: density x^{k-1} e^{-x} on [0, m'] by proposal
+                  // m' U^{1/k}, accepted with probability e^{-x}
+            const double y = D.p[1] * exp_fast(log_fast(u01_open0(w.x, w.y)) * D.p[2]);
+            ok = u01_open0(w.z, w.w) <= exp_fast(-y);
+            X = (y * D.p[0] > m) ? m : y * D.p[0];
         }
         if (ok && !(X > m)) break;
         if (++j >= kMaxAttempts) {
             X = m;
+            break;
+        }
+    }
+    return X;
+}
+
+// Draws the duration of one activity for one sample.  `info`, `dmin` and `p0`
+// come from the activity's wave-uniform record, `edge` is its index; kinds that
+// need more parameters read them from `dists`.  Returns base + extra rounded
+// exactly as the reference (_core.cpp:75, 88, 103, 125, 140, 328): extra =
+// X * d (one rounding), then d + extra (one rounding); this file is compiled
+// with -ffp-contract=off.  `alias` points at the alias records, in LDS or in
+// the constant address space.
+template <typename AliasPtr>
+__device__ __forceinline__ double draw_duration(uint32_t info, double dmin, double p0, uint32_t edge, SeedKey key,
+                                                AliasPtr alias, const_ptr<DistParams> dists) {
+    const uint32_t kind = info & kInfoKindMask;
+    double X;
+    switch (kind) {
+        case kNone: return dmin;
+        case kEmpiricalAbs1: return dmin + p0;
+        case kEmpiricalAbs:
+        case kEmpiricalRel: {
+            // Vose alias table: one uniform picks the bin and the coin.
+            const uint64_t packed = (uint64_t)__double_as_longlong(p0);  // (n, table_off)
+            const int n = (int)(uint32_t)packed;
+            const int table_off = (int)(uint32_t)(packed >> 32);
+            const Words2 w = philox2x32_10(edge, kTagMain, key.lo);
+            const double xn = u01(w.x, w.y) * (double)n;
+            int i = (int)xn;
+            i = (i >= n) ? n - 1 : i;
+            const double frac = xn - (double)i;
+            const AliasRec rec = alias[table_off + i];
+            const double v = (frac < rec.prob) ? rec.v_self : rec.v_alias;
+            if (kind == kEmpiricalAbs) return dmin + v;
+            X = v;
+            break;
+        }
+        case kConstant:
+        case kEmpiricalRel1: X = p0; break;
+        case kExpInf: {  // inversion: X = -mean log U, U in (0, 1]
+            const Words2 w = philox2x32_10(edge, kTagMain, key.lo);
+            X = -p0 * log_fast(u01_open0(w.x, w.y));
+            break;
+        }
+        case kExpTrunc: {
+            // Inversion of the CDF truncated at m: U' = q + (1 - q) U is uniform on
+            // (q, 1] with q = e^{-m/mean}, and X = -mean log U' lies in [0, m].
+            const DistParams D = dists[(info >> kInfoDistShift) & kInfoDistMask];
+            const Words2 w = philox2x32_10(edge, kTagMain, key.lo);
+            const double u = __builtin_fma(u01_open0(w.x, w.y), D.p[3], D.p[1]);
+            X = -p0 * log_fast(u);
+            X = (X > D.p[2]) ? D.p[2] : X;
+            break;
+        }
+        default: {
+            if (!(info & kInfoTruncated) && kind == kGammaErlang2) {  // sum of two unit exponentials
+                const Words4 w = philox4x32_10(edge, 0u, kTagMain, 0u, key.lo, key.hi);
+                X = -p0 * log_fast(u01_open0(w.x, w.y) * u01_open0(w.z, w.w));
+            } else if (!(info & kInfoTruncated) && kind == kGammaHalf) {  // Exp(1) cos^2(2 pi U)
+                const Words4 w = philox4x32_10(edge, 0u, kTagMain, 0u, key.lo, key.hi);
+                const double cs = cos2pi_fast(u01_mid32(w.z));
+                X = p0 * (-log_fast(u01_open0(w.x, w.y)) * (cs * cs));
+            } else {
+                const DistParams D = dists[(info >> kInfoDistShift) & kInfoDistMask];
+                X = gamma_rejection(kind, info, D, edge, key);
+            }
             break;
         }
     }
