@@ -86,6 +86,11 @@ def main() -> int:
     ap.add_argument("--mode", choices=["run_many", "propagate"], default="run_many",
                     help="run_many: sample + propagate (the metric); propagate: propagation only, durations "
                     "injected from a prior run_many (reads durations instead of sampling them)")
+    ap.add_argument("--shape", default="", metavar="W,K",
+                    help="fix the schedule shape (waves per workgroup, events per wave and step) instead of the "
+                    "occupancy-based choice at the first launch")
+    ap.add_argument("--autotune", type=int, default=0, metavar="N",
+                    help="time every schedule shape on N samples before the run and keep the fastest")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write) --steps times, to calibrate "
                     "rocprofv3 FETCH_SIZE / WRITE_SIZE against known bytes")
@@ -119,10 +124,15 @@ def main() -> int:
                           "GB/s": 16 * n / ms / 1e6}), flush=True)
         return 0
 
-    dag =layered_network() if args.workload == "network" else layered_small()
+    dag = layered_network() if args.workload == "network" else layered_small()
     prop = capi.Propagator(dag)
     E, L, S = prop.E, prop.L, args.samples
-    log(f"workload {args.workload}: E={E} L={L} S={S} plan={prop.plan_info()}")
+    if args.shape:
+        prop.configure(*map(int, args.shape.split(",")))
+    if args.autotune > 0:
+        w, k, ms = prop.autotune(args.autotune)
+        log(f"autotune on {args.autotune} samples: waves={w} events_per_wave={k} ({ms:.3f} ms)")
+    log(f"workload {args.workload}: E={E} L={L} S={S}")
 
     seeds = capi.DeviceBuffer.from_array(np.arange(S, dtype=np.int32))
     realized = capi.DeviceBuffer(8 * E * S)
@@ -143,7 +153,8 @@ def main() -> int:
     for _ in range(args.warmup):
         step()
     capi.synchronize(stream)
-    log(f"warmup done ({args.warmup} steps)")
+    plan = prop.plan_info()
+    log(f"warmup done ({args.warmup} steps); plan {plan}")
 
     timers = [capi.Timer() for _ in range(args.steps)]
     capi.synchronize(stream)
@@ -215,6 +226,7 @@ def main() -> int:
             "activities": L,
             "samples": S,
             "outputs": "realized f64 [E][S], durations f64 [L][S], cause i32 [E][S]",
+            "schedule": {"waves": plan["waves"], "events_per_wave": plan["events_per_wave"], "steps": plan["steps"]},
         },
         "roofline": {
             "bound": "hbm",

@@ -320,6 +320,7 @@ struct mcdp_propagator {
     std::vector<Record> recs, orph;
     int32_t n_steps = 0, waves = 1, slots = 0, max_per_wave = 0, k = kMaxEventsPerWave;
     int64_t n_global = 0;
+    bool shape_fixed = false;  // set by configure / autotune / the first launch
     // device copies, one per device used
     std::mutex mu;
     std::vector<DeviceCopy> dev;
@@ -330,6 +331,7 @@ struct mcdp_propagator {
                 int cur = -1;
                 (void)hipGetDevice(&cur);
                 (void)hipSetDevice(d.device);
+                (void)hipDeviceSynchronize();  // launches in flight on any stream may still read the plan
                 (void)hipFree(d.blob);
                 if (cur >= 0) (void)hipSetDevice(cur);
             }
@@ -732,6 +734,51 @@ size_t walk_lds_bytes(const mcdp_propagator &P, WalkVariant v) {
     return bytes;
 }
 
+// Schedule shapes worth trying: every (waves, events per wave) pair whose
+// step holds the widest level (or that is the largest shape available).
+std::vector<std::pair<int32_t, int32_t>> shape_candidates(const mcdp_propagator &P) {
+    size_t max_width = 1;
+    for (const auto &v : P.levels) max_width = std::max(max_width, v.size());
+    const size_t need = std::min<size_t>(max_width, (size_t)kMaxWaves * kMaxEventsPerWave);
+    static const int32_t kWaveChoices[] = {1, 2, 4, 6, 8, 10, 12, 14, 16};
+    std::vector<std::pair<int32_t, int32_t>> out;
+    for (int k : kEventsPerWaveChoices)
+        for (int32_t w : kWaveChoices)
+            if ((size_t)w * (size_t)k >= need) out.emplace_back(w, k);
+    return out;
+}
+
+// Picks the shape with the most busy waves per CU for the sampling kernel on
+// the current device: resident workgroups per CU (from the runtime's
+// occupancy for the candidate's LDS) times the mean number of waves that have
+// events in a step.  Ties: fewer steps, then fewer waves per workgroup.
+// Candidate schedules are built on the host; P is left with the winner.
+int choose_shape(mcdp_propagator &P) {
+    int32_t best_w = P.waves, best_k = P.k;
+    double best_busy = -1.0;
+    int64_t best_steps = 0;
+    for (const auto &c : shape_candidates(P)) {
+        int rc = build_schedule(P, c.first, c.second);
+        if (rc) return rc;
+        const WalkVariant v = walk_variant(P, false);
+        int bpc = 0;
+        if (walk_occupancy(v, P.waves, walk_lds_bytes(P, v), &bpc) != hipSuccess || bpc < 1) continue;
+        int64_t busy_wave_steps = 0;
+        for (size_t g = 0; g + 1 < P.ev_off.size(); ++g) busy_wave_steps += P.ev_off[g + 1] > P.ev_off[g] ? 1 : 0;
+        const double busy = (double)bpc * (double)busy_wave_steps / (double)std::max(P.n_steps, 1);
+        const bool better = busy > best_busy * 1.0001 ||
+                            (busy >= best_busy * 0.9999 &&
+                             (P.n_steps < best_steps || (P.n_steps == best_steps && c.first < best_w)));
+        if (better) {
+            best_busy = busy;
+            best_steps = P.n_steps;
+            best_w = c.first;
+            best_k = c.second;
+        }
+    }
+    return build_schedule(P, best_w, best_k);
+}
+
 int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double *dur_in, int64_t n, int64_t ld,
            double *realized, double *durations, int32_t *cause, hipStream_t stream) {
     if (n < 0) return fail(MCDP_E_INVALID, "n must be non-negative");
@@ -740,10 +787,18 @@ int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double
     if (!injected && !seeds && n > 0) return fail(MCDP_E_INVALID, "seeds is null");
     if (!realized && P.E > 0 && n > 0) return fail(MCDP_E_INVALID, "realized must not be null");
     std::lock_guard<std::mutex> lock(P.mu);
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return fail(MCDP_E_DEVICE, "no HIP device available: the Monte Carlo engine runs on the GPU only");
+    if (!P.shape_fixed) {  // first launch: fit the schedule to this device
+        int rc = choose_shape(P);
+        if (rc) return rc;
+        P.shape_fixed = true;
+    }
     DeviceCopy *d = nullptr;
     int rc = ensure_device(P, d);
     if (rc) return rc;
-    if (n == 0 || P.E == 0 && (injected || P.orph.empty())) return MCDP_OK;
+    if (n == 0 || (P.E == 0 && (injected || P.orph.empty()))) return MCDP_OK;
     WalkParams p = d->base;
     p.seeds = seeds;
     p.dur_in = dur_in;
@@ -954,15 +1009,102 @@ int mcdp_propagate_host(mcdp_propagator *prop, const double *durations, int64_t 
 
 int mcdp_propagator_plan_info(const mcdp_propagator *prop, int64_t *out, int n_out) {
     if (!prop || !out) return fail(MCDP_E_INVALID, "null argument");
-    const int64_t v[8] = {prop->n_steps,
+    const int64_t v[9] = {prop->n_steps,
                           prop->waves,
                           prop->slots,
                           (int64_t)prop->slots * kLanes * (int64_t)sizeof(double),
                           prop->n_global,
                           prop->n_stale,
                           (int64_t)prop->orph.size(),
-                          prop->max_per_wave};
-    for (int i = 0; i < n_out && i < 8; ++i) out[i] = v[i];
+                          prop->max_per_wave,
+                          prop->k};
+    for (int i = 0; i < n_out && i < 9; ++i) out[i] = v[i];
+    return MCDP_OK;
+}
+
+int mcdp_propagator_configure(mcdp_propagator *prop, int32_t waves, int32_t events_per_wave) {
+    if (!prop) return fail(MCDP_E_INVALID, "propagator is null");
+    if (waves < 1 || waves > kMaxWaves) return fail(MCDP_E_INVALID, "waves must be in [1, 16]");
+    bool k_ok = false;
+    for (int k : kEventsPerWaveChoices) k_ok |= k == events_per_wave;
+    if (!k_ok) return fail(MCDP_E_INVALID, "events_per_wave must be 4, 8, 12 or 16");
+    std::lock_guard<std::mutex> lock(prop->mu);
+    int rc = build_schedule(*prop, waves, events_per_wave);
+    if (rc) return rc;
+    prop->drop_device_copies();
+    prop->shape_fixed = true;
+    return MCDP_OK;
+}
+
+int mcdp_propagator_autotune(mcdp_propagator *prop, int64_t n_samples, int32_t *waves, int32_t *events_per_wave,
+                             float *ms) {
+    if (!prop) return fail(MCDP_E_INVALID, "propagator is null");
+    if (n_samples < 1) return fail(MCDP_E_INVALID, "n_samples must be positive");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(MCDP_E_DEVICE, "no HIP device available");
+    const int64_t n = n_samples, E = prop->E, L = prop->L;
+    std::vector<int32_t> hseeds((size_t)n);
+    for (int64_t i = 0; i < n; ++i) hseeds[(size_t)i] = (int32_t)i;
+    void *buf = nullptr;
+    const size_t bytes = align_up((size_t)n * 4, 256) + (size_t)n * (size_t)(12 * E + 8 * L);
+    hipError_t e = hipMalloc(&buf, bytes);
+    if (e != hipSuccess) return fail(MCDP_E_NOMEM, std::string("hipMalloc (autotune): ") + hipGetErrorString(e));
+    auto *b = static_cast<unsigned char *>(buf);
+    int32_t *d_seeds = reinterpret_cast<int32_t *>(b);
+    double *d_real = reinterpret_cast<double *>(b + align_up((size_t)n * 4, 256));
+    double *d_dur = d_real + (size_t)(E * n);
+    int32_t *d_cause = reinterpret_cast<int32_t *>(d_dur + (size_t)(L * n));
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    int rc = MCDP_OK;
+    float best_ms = -1.0f;
+    int32_t best_w = prop->waves, best_k = prop->k;
+    std::vector<std::pair<int32_t, int32_t>> cands;
+    {
+        std::lock_guard<std::mutex> lock(prop->mu);
+        cands = shape_candidates(*prop);
+    }
+    e = hipMemcpy(d_seeds, hseeds.data(), (size_t)n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipEventCreate(&t0);
+    if (e == hipSuccess) e = hipEventCreate(&t1);
+    if (e != hipSuccess) rc = hip_fail(e, "autotune setup");
+    for (size_t c = 0; rc == MCDP_OK && c < cands.size(); ++c) {
+        rc = mcdp_propagator_configure(prop, cands[c].first, cands[c].second);
+        if (rc) break;
+        {  // skip shapes that cannot be resident
+            std::lock_guard<std::mutex> lock(prop->mu);
+            const WalkVariant v = walk_variant(*prop, false);
+            int bpc = 0;
+            if (walk_occupancy(v, prop->waves, walk_lds_bytes(*prop, v), &bpc) != hipSuccess || bpc < 1) continue;
+        }
+        rc = launch(*prop, false, d_seeds, nullptr, n, n, d_real, d_dur, d_cause, nullptr);  // warm-up + upload
+        if (rc) break;
+        if ((e = hipEventRecord(t0, nullptr)) != hipSuccess) {
+            rc = hip_fail(e, "hipEventRecord");
+            break;
+        }
+        rc = launch(*prop, false, d_seeds, nullptr, n, n, d_real, d_dur, d_cause, nullptr);
+        if (rc) break;
+        float t = 0.0f;
+        if ((e = hipEventRecord(t1, nullptr)) != hipSuccess || (e = hipEventSynchronize(t1)) != hipSuccess ||
+            (e = hipEventElapsedTime(&t, t0, t1)) != hipSuccess) {
+            rc = hip_fail(e, "autotune timing");
+            break;
+        }
+        if (best_ms < 0.0f || t < best_ms) {
+            best_ms = t;
+            best_w = cands[c].first;
+            best_k = cands[c].second;
+        }
+    }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    (void)hipFree(buf);
+    if (rc) return rc;
+    rc = mcdp_propagator_configure(prop, best_w, best_k);
+    if (rc) return rc;
+    if (waves) *waves = best_w;
+    if (events_per_wave) *events_per_wave = best_k;
+    if (ms) *ms = best_ms;
     return MCDP_OK;
 }
 

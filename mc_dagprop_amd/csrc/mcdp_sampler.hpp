@@ -5,8 +5,8 @@
 // exponential / gamma / discrete distributions (_core.cpp:78-141).  A GPU lane
 // cannot share a sequential stream, so each draw here is a pure function of
 // (seed, activity, attempt): Philox (Salmon, Moraes, Dror, Shaw, "Parallel
-// random numbers: as easy as 1, 2, 3", SC'11) keyed by the run's seed, with
-// the activity index and the attempt number as counter.  A duration therefore
+// random numbers: as easy as 1, 2, 3", SC'11) keyed by the activity index,
+// with the run's seed and the attempt number as counter.  A duration therefore
 // does not depend on which lane, tile, batch or launch computes it, and
 // run_many(seeds)[i] equals run(seeds[i]) exactly.
 //
@@ -106,6 +106,19 @@ __device__ __forceinline__ SeedKey seed_key(int32_t seed) {
     return SeedKey{(uint32_t)seed, seed < 0 ? 0xFFFFFFFFu : 0u};
 }
 
+// The random words of one draw.  Philox is keyed by the activity index (the
+// same for every lane of a wave, so the round keys live in scalar registers)
+// and counts over the run's seed: activity a is stream a, the seed is the
+// position in it.
+//   two words:  Philox-2x32-10(ctr = (seed_lo, tag), key = a)
+//   four words: Philox-4x32-10(ctr = (seed_lo, seed_hi, attempt, 0), key = (a, 0))
+__device__ __forceinline__ Words2 draw_words2(uint32_t edge, SeedKey s, uint32_t tag) {
+    return philox2x32_10(s.lo, tag, edge);
+}
+__device__ __forceinline__ Words4 draw_words4(uint32_t edge, SeedKey s, uint32_t attempt) {
+    return philox4x32_10(s.lo, s.hi, attempt, 0u, edge, 0u);
+}
+
 // ---- per-kind draws of the multiplier X (or the absolute extra) -------------
 
 // One Marsaglia-Tsang attempt for shape k' = d + 1/3 >= 1, scale 1
@@ -135,7 +148,7 @@ __device__ __forceinline__ double gamma_rejection(uint32_t kind, uint32_t flags,
     double X = 0.0;
     uint32_t j = 0;
     for (;;) {
-        const Words4 w = philox4x32_10(edge, j, kTagMain, 0u, key.lo, key.hi);
+        const Words4 w = draw_words4(edge, key, j);
         bool ok = true;
         if (kind == kGammaErlang2) {  // sum of two unit exponentials
             X = -D.p[0] * log_fast(u01_open0(w.x, w.y) * u01_open0(w.z, w.w));
@@ -147,7 +160,7 @@ __device__ __forceinline__ double gamma_rejection(uint32_t kind, uint32_t flags,
             ok = mt_attempt(w, D.p[1], D.p[2], y);
             if (ok) {
                 if (flags & kInfoBoost) {  // shape < 1: Gamma(k) = Gamma(k + 1) U^{1/k}
-                    const Words2 b = philox2x32_10(edge, (j << 8) | kTagBoost, key.lo);
+                    const Words2 b = draw_words2(edge, key, (j << 8) | kTagBoost);
                     y *= exp_fast(log_fast(u01_open0(b.x, b.y)) * D.p[4]);
                 }
                 X = D.p[0] * y;
@@ -188,7 +201,7 @@ __device__ __forceinline__ double draw_duration(uint32_t info, double dmin, doub
             const uint64_t packed = (uint64_t)__double_as_longlong(p0);  // (n, table_off)
             const int n = (int)(uint32_t)packed;
             const int table_off = (int)(uint32_t)(packed >> 32);
-            const Words2 w = philox2x32_10(edge, kTagMain, key.lo);
+            const Words2 w = draw_words2(edge, key, kTagMain);
             const double xn = u01(w.x, w.y) * (double)n;
             int i = (int)xn;
             i = (i >= n) ? n - 1 : i;
@@ -202,7 +215,7 @@ __device__ __forceinline__ double draw_duration(uint32_t info, double dmin, doub
         case kConstant:
         case kEmpiricalRel1: X = p0; break;
         case kExpInf: {  // inversion: X = -mean log U, U in (0, 1]
-            const Words2 w = philox2x32_10(edge, kTagMain, key.lo);
+            const Words2 w = draw_words2(edge, key, kTagMain);
             X = -p0 * log_fast(u01_open0(w.x, w.y));
             break;
         }
@@ -210,7 +223,7 @@ __device__ __forceinline__ double draw_duration(uint32_t info, double dmin, doub
             // Inversion of the CDF truncated at m: U' = q + (1 - q) U is uniform on
             // (q, 1] with q = e^{-m/mean}, and X = -mean log U' lies in [0, m].
             const DistParams D = dists[(info >> kInfoDistShift) & kInfoDistMask];
-            const Words2 w = philox2x32_10(edge, kTagMain, key.lo);
+            const Words2 w = draw_words2(edge, key, kTagMain);
             const double u = __builtin_fma(u01_open0(w.x, w.y), D.p[3], D.p[1]);
             X = -p0 * log_fast(u);
             X = (X > D.p[2]) ? D.p[2] : X;
@@ -218,10 +231,10 @@ __device__ __forceinline__ double draw_duration(uint32_t info, double dmin, doub
         }
         default: {
             if (!(info & kInfoTruncated) && kind == kGammaErlang2) {  // sum of two unit exponentials
-                const Words4 w = philox4x32_10(edge, 0u, kTagMain, 0u, key.lo, key.hi);
+                const Words4 w = draw_words4(edge, key, 0u);
                 X = -p0 * log_fast(u01_open0(w.x, w.y) * u01_open0(w.z, w.w));
             } else if (!(info & kInfoTruncated) && kind == kGammaHalf) {  // Exp(1) cos^2(2 pi U)
-                const Words4 w = philox4x32_10(edge, 0u, kTagMain, 0u, key.lo, key.hi);
+                const Words4 w = draw_words4(edge, key, 0u);
                 const double cs = cos2pi_fast(u01_mid32(w.z));
                 X = p0 * (-log_fast(u01_open0(w.x, w.y)) * (cs * cs));
             } else {

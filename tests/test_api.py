@@ -1,0 +1,338 @@
+"""The reference's Python API, served by mc_dagprop_amd (port of test/test_simulator.py and
+test/test_monte_carlo_extra.py), plus the C ABI surface.
+
+CPU tests: the C-ABI library loads and exports every symbol include/mcdp.h declares; the pybind
+module exposes the reference's classes and signatures; construction-time validation raises the
+reference's exceptions; running without a GPU fails loudly (no CPU fallback).
+
+GPU tests: the reference's own test cases through the same API.  Asserts that depend on the
+reference's Xoshiro256++/libstdc++ random stream (test_simulator.py:139-172: realized[3] == 68.0
+at seed 7, ...) cannot hold for the device's counter-based stream; those become assertions on the
+set of values the law allows, and the exact reference values are pinned in the oracle tests
+against the fixtures the compiled reference wrote.
+"""
+
+from __future__ import annotations
+
+import inspect
+import re
+from concurrent.futures import ThreadPoolExecutor
+from itertools import chain
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import mc_dagprop_amd as mc
+from mc_dagprop_amd import Activity, DagContext, Event, EventTimestamp, GenericDelayGenerator, Simulator, capi
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "mcdp.h"
+
+
+# ---- C ABI surface ----------------------------------------------------------------
+def test_header_symbols_exported() -> None:
+    """libmcdp.so loads without a GPU and exports every function include/mcdp.h declares."""
+    names = capi.header_symbols()
+    assert len(names) >= 30
+    lib = capi.lib()
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) <= set(capi._SIGNATURES), set(names) - set(capi._SIGNATURES)
+
+
+def test_abi_version_matches_header() -> None:
+    m = re.search(r"#define MCDP_ABI_VERSION (\d+)", HEADER.read_text())
+    assert m and capi.lib().mcdp_abi_version() == int(m.group(1))
+
+
+def test_status_codes_match_header() -> None:
+    text = HEADER.read_text()
+    for name, value in [("MCDP_OK", 0), ("MCDP_E_INVALID", 1), ("MCDP_E_RESERVED_TYPE", 2),
+                        ("MCDP_E_MAX_DELAY", 3), ("MCDP_E_CYCLE", 4), ("MCDP_E_DEVICE", 5), ("MCDP_E_NOMEM", 6)]:
+        assert re.search(rf"\b{name} = {value}\b", text), name
+
+
+def test_null_arguments_rejected() -> None:
+    lib = capi.lib()
+    assert lib.mcdp_propagator_plan_info(None, None, 0) == 1
+    assert b"null" in lib.mcdp_last_error()
+    assert lib.mcdp_propagator_configure(None, 4, 8) == 1
+
+
+# ---- Python API surface (src/mc_dagprop/monte_carlo/_core.pyi) ---------------------------
+def test_public_names() -> None:
+    for name in ["GenericDelayGenerator", "DagContext", "SimResult", "Event", "Activity", "Simulator",
+                 "MonteCarloPropagator", "EventTimestamp"]:
+        assert hasattr(mc, name), name
+    assert mc.Simulator is mc.MonteCarloPropagator
+    from mc_dagprop_amd.core import Activity as A2, DagContext as C2
+    from mc_dagprop_amd.monte_carlo import Simulator as S2
+
+    assert A2 is Activity and C2 is DagContext and S2 is Simulator
+
+
+def test_dataclass_fields() -> None:
+    ts = EventTimestamp(1.0, 2.0, 3.0)
+    assert (ts.earliest, ts.latest, ts.actual) == (1.0, 2.0, 3.0)
+    ev = Event("x", ts)
+    assert ev.event_id == "x" and ev.timestamp.latest == 2.0
+    a = Activity(idx=3, minimal_duration=1.5, activity_type=7)
+    assert (a.idx, a.minimal_duration, a.activity_type) == (3, 1.5, 7)
+    ctx = DagContext(events=[ev], activities={(0, 0): a}, precedence_list=[], max_delay=5.0)
+    assert ctx.max_delay == 5.0 and len(ctx.events) == 1 and (0, 0) in ctx.activities
+
+
+def test_method_names() -> None:
+    for m in ["set_seed", "add_constant", "add_exponential", "add_gamma", "add_empirical_absolute",
+              "add_empirical_relative"]:
+        assert callable(getattr(GenericDelayGenerator, m)), m
+    for m in ["run", "run_many", "node_count", "activity_count"]:
+        assert callable(getattr(Simulator, m)), m
+    doc = Simulator.run_many.__doc__ or ""
+    assert "seeds" in doc
+
+
+def _ctx3(max_delay: float = 1e6) -> DagContext:
+    """test_monte_carlo_extra.py:8-19."""
+    events = [Event(str(i), EventTimestamp(0.0, 100.0, 0.0)) for i in range(3)]
+    acts = {(0, 1): Activity(idx=0, minimal_duration=1.0, activity_type=1),
+            (1, 2): Activity(idx=1, minimal_duration=2.0, activity_type=1)}
+    return DagContext(events=events, activities=acts, precedence_list=[(1, [(0, 0)]), (2, [(1, 1)])],
+                      max_delay=max_delay)
+
+
+def _ctx6() -> DagContext:
+    """test_simulator.py:12-36."""
+    earliest = [0.0, 5.0, 10.0, 22.0, 20.0, 100.0]
+    events = [Event(str(i), EventTimestamp(t, 100.0, 0.0)) for i, t in enumerate(earliest)]
+    acts = {(0, 1): Activity(idx=0, minimal_duration=3.0, activity_type=1),
+            (1, 2): Activity(idx=1, minimal_duration=5.0, activity_type=1),
+            (1, 3): Activity(idx=2, minimal_duration=5.0, activity_type=1),
+            (2, 4): Activity(idx=3, minimal_duration=15.0, activity_type=2),
+            (3, 4): Activity(idx=4, minimal_duration=10.0, activity_type=3)}
+    prec = [(1, [(0, 0)]), (2, [(1, 1)]), (3, [(1, 2)]), (4, [(2, 3), (3, 4)])]
+    return DagContext(events=events, activities=acts, precedence_list=prec, max_delay=1e6)
+
+
+def test_cycle_detection() -> None:
+    """test_monte_carlo_extra.py:60-72."""
+    events = [Event("0", EventTimestamp(0.0, 100.0, 0.0)), Event("1", EventTimestamp(0.0, 100.0, 0.0))]
+    acts = {(0, 1): Activity(idx=0, minimal_duration=1.0, activity_type=1),
+            (1, 0): Activity(idx=1, minimal_duration=1.0, activity_type=1)}
+    ctx = DagContext(events=events, activities=acts, precedence_list=[(1, [(0, 0)]), (0, [(1, 1)])], max_delay=1e6)
+    gen = GenericDelayGenerator()
+    gen.add_constant(1, 0.0)
+    with pytest.raises(RuntimeError, match="[Cc]ycle"):
+        Simulator(ctx, gen)
+
+
+def test_reserved_activity_type() -> None:
+    """test_monte_carlo_extra.py:74-79."""
+    gen = GenericDelayGenerator()
+    gen.add_constant(-1, 0.0)
+    with pytest.raises(RuntimeError):
+        Simulator(_ctx3(), gen)
+
+
+def test_negative_max_delay_rejected() -> None:
+    """test_monte_carlo_extra.py:81-92."""
+    gen = GenericDelayGenerator()
+    gen.add_constant(1, 0.0)
+    with pytest.raises(RuntimeError):
+        Simulator(_ctx3(max_delay=-1.0), gen)
+
+
+def test_empirical_length_mismatch() -> None:
+    gen = GenericDelayGenerator()
+    with pytest.raises((RuntimeError, ValueError)):
+        gen.add_empirical_absolute(1, [1.0, 2.0], [1.0])
+    with pytest.raises((RuntimeError, ValueError)):
+        gen.add_empirical_relative(1, [1.0], [1.0, 2.0])
+
+
+def test_counts_without_gpu() -> None:
+    gen = GenericDelayGenerator()
+    gen.add_constant(1, 1.0)
+    sim = Simulator(_ctx6(), gen)
+    assert sim.node_count() == 6 and sim.activity_count() == 5
+
+
+def test_no_cpu_fallback() -> None:
+    """Without a HIP device the engine raises instead of computing on the host."""
+    if capi.device_count() > 0:
+        pytest.skip("a GPU is present")
+    gen = GenericDelayGenerator()
+    gen.add_constant(1, 1.0)
+    sim = Simulator(_ctx6(), gen)
+    with pytest.raises(RuntimeError, match="GPU|device"):
+        sim.run(seed=1)
+    with pytest.raises(RuntimeError, match="GPU|device"):
+        sim.run_many([1, 2])
+
+
+def test_plan_is_host_side() -> None:
+    """The schedule is compiled on the host: plan facts and shape changes need no device."""
+    gen = GenericDelayGenerator()
+    gen.add_constant(1, 1.0)
+    sim = Simulator(_ctx6(), gen)
+    info = sim.plan_info()
+    assert info["steps"] >= 1 and info["waves"] >= 1
+    sim.configure(3, 4)
+    info = sim.plan_info()
+    assert info["waves"] == 3 and info["events_per_wave"] == 4
+    with pytest.raises(RuntimeError):
+        sim.configure(0, 4)
+    with pytest.raises(RuntimeError):
+        sim.configure(2, 5)
+
+
+# ---- the reference's cases on the GPU -------------------------------------------------
+@pytest.mark.gpu
+def test_constant_via_generic(gpu) -> None:
+    """test_simulator.py:38-65."""
+    gen = GenericDelayGenerator()
+    gen.add_constant(activity_type=1, factor=1.0)
+    sim = Simulator(_ctx6(), gen)
+    res = sim.run(seed=7)
+    batch = sim.run_many([1, 2, 3])
+    assert len(batch) == 3 and all(isinstance(b, type(res)) for b in batch)
+    assert res.realized[1] == pytest.approx(6.0) and res.realized[2] == pytest.approx(16.0)
+    assert len(res.durations) == 5 and len(res.realized) == 6 and len(res.cause_event) == 6
+    assert list(res.cause_event[:3]) == [-1, 0, 1]
+
+
+@pytest.mark.gpu
+def test_unsorted_precedence_same_result(gpu) -> None:
+    """test_simulator.py:67-86."""
+    ctx = _ctx6()
+    unsorted = DagContext(events=ctx.events, activities=ctx.activities,
+                          precedence_list=list(reversed(ctx.precedence_list)), max_delay=1e6)
+    gen = GenericDelayGenerator()
+    gen.add_constant(activity_type=1, factor=1.0)
+    a, b = Simulator(ctx, gen).run(seed=7), Simulator(unsorted, gen).run(seed=7)
+    np.testing.assert_array_equal(a.realized, b.realized)
+    np.testing.assert_array_equal(a.durations, b.durations)
+    np.testing.assert_array_equal(a.cause_event, b.cause_event)
+
+
+@pytest.mark.gpu
+def test_exponential_via_generic(gpu) -> None:
+    """test_simulator.py:88-107."""
+    gen = GenericDelayGenerator()
+    gen.add_exponential(1, 1000.0, max_scale=1.0)
+    sim = Simulator(_ctx6(), gen)
+    for res in sim.run_many(tuple(range(3))):
+        r = list(res.realized)
+        assert r[0] == 0.0 and r[1] >= 5.0 and r[2] >= r[1] and r[2] <= r[1] + 5.0 * 10.0 + 1e-6
+        assert len(res.durations) == 5
+
+
+@pytest.mark.gpu
+def test_propagation(gpu) -> None:
+    """test_simulator.py:109-137: exact values (constant factors)."""
+    gen = GenericDelayGenerator()
+    gen.add_constant(activity_type=1, factor=1.0)
+    gen.add_constant(activity_type=3, factor=3.0)
+    res = Simulator(_ctx6(), gen).run_many(tuple(range(5)))[3]
+    assert list(res.durations) == [6.0, 10.0, 10.0, 15.0, 40.0]
+    assert list(res.realized) == [0.0, 6.0, 16.0, 22.0, 62.0, 100.0]
+    assert list(res.cause_event) == [-1, 0, 1, -1, 3, -1]
+
+
+@pytest.mark.gpu
+def test_empirical_absolute(gpu) -> None:
+    """test_simulator.py:139-145.  The reference's seed-7 value (68.0) is one of the law's outcomes:
+    realized[3] = max(22, realized[1] + 5 + v), realized[1] = max(5, 3 + v')."""
+    gen = GenericDelayGenerator()
+    gen.add_empirical_absolute(activity_type=1, values=[10, 20, 40, 50], weights=[0.1, 0.2, 0.3, 0.4])
+    sim = Simulator(_ctx6(), gen)
+    allowed = {max(22.0, max(5.0, 3.0 + v1) + 5.0 + v2) for v1 in (10, 20, 40, 50) for v2 in (10, 20, 40, 50)}
+    assert 68.0 in allowed
+    seen = set()
+    for res in sim.run_many(range(400)):
+        assert res.realized[3] in allowed and res.realized[5] == 100.0
+        seen.add(res.realized[3])
+    assert len(seen) >= 6
+
+
+@pytest.mark.gpu
+def test_empirical_relative(gpu) -> None:
+    """test_simulator.py:147-154: realized[3] = 3 + 3 f1 + 5 + 5 f2 (reference at seed 7: 34.10)."""
+    factors = [1.2, 1.3, 1.35, 4.5]
+    gen = GenericDelayGenerator()
+    gen.add_empirical_relative(activity_type=1, factors=factors, weights=[0.1, 0.2, 0.3, 0.4])
+    sim = Simulator(_ctx6(), gen)
+    allowed = np.array([max(22.0, (3.0 + f1 * 3.0) + (5.0 + f2 * 5.0)) for f1 in factors for f2 in factors])
+    assert np.min(np.abs(allowed - 34.10)) < 1e-4
+    for res in sim.run_many(range(200)):
+        assert np.min(np.abs(allowed - res.realized[3])) < 1e-9 and res.realized[5] == 100.0
+
+
+@pytest.mark.gpu
+def test_run_many_matches_individual_runs(gpu) -> None:
+    """test_monte_carlo_extra.py:95-111, bit for bit."""
+    gen = GenericDelayGenerator()
+    gen.add_exponential(activity_type=1, lambda_=1.0, max_scale=0.5)
+    sim = Simulator(_ctx3(), gen)
+    seeds = list(range(5))
+    for b, s in zip(sim.run_many(seeds), [sim.run(x) for x in seeds]):
+        np.testing.assert_array_equal(b.realized, s.realized)
+        np.testing.assert_array_equal(b.durations, s.durations)
+        np.testing.assert_array_equal(b.cause_event, s.cause_event)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["constant", "exponential", "gamma"])
+def test_distribution_bounds(gpu, kind: str) -> None:
+    """test_monte_carlo_extra.py:22-56."""
+    gen = GenericDelayGenerator()
+    if kind == "constant":
+        gen.add_constant(activity_type=1, factor=1.0)
+    elif kind == "exponential":
+        gen.add_exponential(activity_type=1, lambda_=2.0, max_scale=0.5)
+    else:
+        gen.add_gamma(activity_type=1, shape=2.0, scale=1.0, max_scale=0.5)
+    sim = Simulator(_ctx3(), gen)
+    for res in sim.run_many(range(256)):
+        d, r = np.asarray(res.durations), np.asarray(res.realized)
+        if kind == "constant":
+            np.testing.assert_array_equal(d, [2.0, 4.0])
+            np.testing.assert_array_equal(r, [0.0, 2.0, 6.0])
+        else:
+            assert 1.0 <= d[0] <= 1.5 and 2.0 <= d[1] <= 3.0 and r[2] >= r[1]
+
+
+@pytest.mark.gpu
+def test_large_scale_simulation(gpu) -> None:
+    """test_simulator.py:175-200: a 10,000-event chain."""
+    n = 10_000
+    events = [Event(str(i), EventTimestamp(float(i), 100.0 + i, 0.0)) for i in range(n)]
+    acts = {(i, i + 1): Activity(idx=i, minimal_duration=3.0, activity_type=1) for i in range(n - 1)}
+    ctx = DagContext(events=events, activities=acts, precedence_list=[(i, [(i - 1, i)]) for i in range(1, n)],
+                     max_delay=1e6)
+    gen = GenericDelayGenerator()
+    gen.add_constant(activity_type=1, factor=1.0)
+    sim = Simulator(ctx, gen)
+    res = sim.run(seed=7)
+    assert len(res.realized) == n and len(res.durations) == n - 1 and len(res.cause_event) == n
+    assert res.realized[0] == 0.0 and res.realized[n - 1] == pytest.approx(59988.0)
+    # test_simulator.py:202-218: concurrent run_many from several threads
+    sims = [Simulator(ctx, gen) for _ in range(4)]
+    seeds = [[i + j for i in range(250)] for j in range(4)]
+    with ThreadPoolExecutor(max_workers=4) as pool:
+        results = list(chain.from_iterable(pool.map(lambda a: a[0].run_many(a[1]), zip(sims, seeds))))
+    assert len(results) == 1000
+    for r in results:
+        assert len(r.realized) == n and r.realized[n - 1] == res.realized[n - 1]
+
+
+@pytest.mark.gpu
+def test_signature_matches_reference_pyi(gpu) -> None:
+    """run(seed) / run_many(seeds) keyword names (src/mc_dagprop/monte_carlo/_core.pyi)."""
+    gen = GenericDelayGenerator()
+    gen.add_constant(activity_type=1, factor=1.0)
+    sim = Simulator(_ctx3(), gen)
+    a = sim.run(seed=3)
+    b = sim.run_many(seeds=[3])[0]
+    np.testing.assert_array_equal(a.realized, b.realized)
+    assert inspect.isclass(Simulator)

@@ -170,13 +170,15 @@ def test_draws_independent_across_activities_and_seeds(gpu) -> None:
 
 
 def test_exponential_stream_formula(gpu) -> None:
-    """Documented stream: X = -mean log(((w0 >> 11) 2^32 + w1 + 1) 2^-53), (w0, w1) = Philox-2x32-10(ctr=(a, 0), key=seed)."""
+    """Documented stream: X = -mean log(((w0 >> 11) 2^32 + w1 + 1) 2^-53),
+    (w0, w1) = Philox-2x32-10(ctr = (seed, 0), key = a) for activity a."""
     lam = 3.0
     seeds = np.arange(-40, 40, dtype=np.int32)
     x = draws(DistSpec(1, EXPONENTIAL, (lam, INF, INF)), seeds=seeds).reshape(seeds.size, N_ACTS)
     a = np.tile(np.arange(N_ACTS, dtype=np.uint32), seeds.size)
-    ctr = np.stack([a, np.zeros_like(a)], axis=1)
-    key = np.repeat(seeds.astype(np.uint32), N_ACTS)
+    s = np.repeat(seeds.astype(np.uint32), N_ACTS)
+    ctr = np.stack([s, np.zeros_like(s)], axis=1)
+    key = a
     w = oracle.philox2x32(ctr, key)
     m = (w[:, 0] >> np.uint32(11)).astype(np.float64) * 2.0**32 + w[:, 1].astype(np.float64)
     expect = (1.0 + (-lam * np.log((m + 1.0) * 2.0**-53))) - 1.0

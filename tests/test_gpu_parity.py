@@ -195,3 +195,38 @@ def test_network_slice_consistent(gpu) -> None:
     fixed = ~np.isnan(base)
     np.testing.assert_array_equal(bits(d[:, fixed]), bits(np.broadcast_to(base[fixed], d[:, fixed].shape)))
     assert np.all(np.isfinite(d)) and np.all(d >= dag.act_dur[None, :])
+
+
+@pytest.mark.parametrize("shape", [(1, 4), (2, 16), (5, 8), (16, 12), (16, 16)])
+def test_results_independent_of_schedule_shape(gpu, shape) -> None:
+    """Every schedule shape (waves per workgroup, events per wave and step) gives the same bits."""
+    dag = layered_network(layers=12)
+    seeds = np.arange(-50, 150)
+    ref = capi.Propagator(dag)
+    r0, d0, c0 = ref.run_many(seeds)
+    P = capi.Propagator(dag)
+    P.configure(*shape)
+    info = P.plan_info()
+    assert (info["waves"], info["events_per_wave"]) == shape
+    r, d, c = P.run_many(seeds)
+    np.testing.assert_array_equal(bits(r), bits(r0))
+    np.testing.assert_array_equal(bits(d), bits(d0))
+    np.testing.assert_array_equal(c, c0)
+    rr, cc = P.propagate(d0)
+    np.testing.assert_array_equal(bits(rr), bits(r0))
+    np.testing.assert_array_equal(cc, c0)
+
+
+def test_autotune_keeps_results(gpu) -> None:
+    dag = layered_network(layers=12)
+    seeds = np.arange(300)
+    P = capi.Propagator(dag)
+    r0, d0, c0 = P.run_many(seeds)
+    w, k, ms = P.autotune(4096)
+    assert 1 <= w <= 16 and k in (4, 8, 12, 16) and ms > 0
+    info = P.plan_info()
+    assert (info["waves"], info["events_per_wave"]) == (w, k)
+    r, d, c = P.run_many(seeds)
+    np.testing.assert_array_equal(bits(r), bits(r0))
+    np.testing.assert_array_equal(bits(d), bits(d0))
+    np.testing.assert_array_equal(c, c0)
