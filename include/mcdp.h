@@ -160,6 +160,10 @@ int mcdp_event_elapsed_ms(void *start, void *stop, float *ms);
  * n host-supplied counters/keys and copies the words back to the host.
  * variant 4: ctr[4n], key[2n], out[4n]; variant 2: ctr[2n], key[n], out[2n]. */
 int mcdp_debug_philox(int variant, int64_t n, const uint32_t *ctr, const uint32_t *key, uint32_t *out);
+/* The samplers' device elementary functions on n host inputs, results copied
+ * back: fn 0 = natural log (x > 0), 1 = cos(2 pi x) (x in [0, 1]), 2 = e^x
+ * (x <= 0). */
+int mcdp_debug_math(int fn, int64_t n, const double *x, double *out);
 /* Plain device copy of n doubles (one read and one write per element): the
  * calibration kernel for HBM byte counters. */
 int mcdp_debug_copy(double *dst_dev, const double *src_dev, int64_t n, void *stream);

@@ -67,6 +67,7 @@ _SIGNATURES = {
     "mcdp_event_elapsed_ms": (C.c_int, [_P, _P, C.POINTER(C.c_float)]),
     "mcdp_debug_philox": (C.c_int, [C.c_int, _i64, _pu32, _pu32, _pu32]),
     "mcdp_debug_copy": (C.c_int, [_P, _P, _i64, _P]),
+    "mcdp_debug_math": (C.c_int, [C.c_int, _i64, _pf64, _pf64]),
 }
 
 _lib = None
@@ -299,6 +300,15 @@ def debug_philox(variant: int, ctr: np.ndarray, key: np.ndarray) -> np.ndarray:
     n = ctr.shape[0]
     out = np.empty_like(ctr)
     check(lib().mcdp_debug_philox(variant, n, _ptr(ctr, _pu32), _ptr(key, _pu32), _ptr(out, _pu32)))
+    return out
+
+
+def debug_math(fn: str, x: np.ndarray) -> np.ndarray:
+    """The device samplers' log / cos2pi / exp on host inputs."""
+    code = {"log": 0, "cos2pi": 1, "exp": 2}[fn]
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty_like(x)
+    check(lib().mcdp_debug_math(code, x.size, _ptr(x, _pf64), _ptr(out, _pf64)))
     return out
 
 

@@ -1206,4 +1206,21 @@ int mcdp_debug_philox(int variant, int64_t n, const uint32_t *ctr, const uint32_
     return MCDP_OK;
 }
 
+int mcdp_debug_math(int fn, int64_t n, const double *x, double *out) {
+    if (fn < 0 || fn > 2) return fail(MCDP_E_INVALID, "fn must be 0 (log), 1 (cos2pi) or 2 (exp)");
+    if (n < 0 || (n > 0 && (!x || !out))) return fail(MCDP_E_INVALID, "invalid arguments");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(MCDP_E_DEVICE, "no HIP device available");
+    if (n == 0) return MCDP_OK;
+    double *d = nullptr;
+    HIP_TRY(hipMalloc(&d, sizeof(double) * 2 * (size_t)n), "hipMalloc");
+    hipError_t e = hipMemcpy(d, x, sizeof(double) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_math_debug(fn, n, d, d + n, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(out, d + n, sizeof(double) * n, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "math debug");
+    return MCDP_OK;
+}
+
 }  // extern "C"

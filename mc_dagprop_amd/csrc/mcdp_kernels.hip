@@ -160,6 +160,15 @@ __global__ void philox_debug_kernel(int variant, int64_t n, const uint32_t *__re
     }
 }
 
+// The samplers' elementary functions on n inputs: 0 log_fast, 1 cos2pi_fast,
+// 2 exp_fast.
+__global__ void math_debug_kernel(int fn, int64_t n, const double *__restrict__ x, double *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = x[i];
+    out[i] = fn == 0 ? log_fast(v) : fn == 1 ? cos2pi_fast(v) : exp_fast(v);
+}
+
 __global__ void __launch_bounds__(256) copy_kernel(double *__restrict__ dst, const double *__restrict__ src,
                                                    int64_t n) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -215,6 +224,13 @@ hipError_t launch_philox_debug(int variant, int64_t n, const uint32_t *ctr, cons
     const int block = 256;
     const int grid = (int)((n + block - 1) / block);
     if (grid > 0) hipLaunchKernelGGL(philox_debug_kernel, dim3(grid), dim3(block), 0, stream, variant, n, ctr, key, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_math_debug(int fn, int64_t n, const double *x, double *out, hipStream_t stream) {
+    const int block = 256;
+    const int grid = (int)((n + block - 1) / block);
+    if (grid > 0) hipLaunchKernelGGL(math_debug_kernel, dim3(grid), dim3(block), 0, stream, fn, n, x, out);
     return hipGetLastError();
 }
 

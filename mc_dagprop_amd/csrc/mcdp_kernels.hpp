@@ -59,6 +59,7 @@ struct WalkVariant {
 hipError_t launch_walk(const WalkParams &p, WalkVariant v, int grid, size_t lds_bytes, hipStream_t stream);
 hipError_t walk_occupancy(WalkVariant v, int waves, size_t lds_bytes, int *blocks_per_cu);
 hipError_t launch_copy(double *dst, const double *src, int64_t n, hipStream_t stream);
+hipError_t launch_math_debug(int fn, int64_t n, const double *x, double *out, hipStream_t stream);
 hipError_t launch_philox_debug(int variant, int64_t n, const uint32_t *ctr, const uint32_t *key, uint32_t *out,
                                hipStream_t stream);
 

@@ -9,6 +9,21 @@
 
 namespace mcdp {
 
+// p * z + c for a polynomial coefficient c.  The compiler's accumulate form
+// (v_fmac_f64) wants c in vector registers and rebuilds each 64-bit constant
+// with two v_mov_b32 per use; here c is bound to a scalar register pair, so a
+// Horner step is one v_fma_f64 (the pair is set up by scalar moves, which
+// issue beside the vector pipeline).
+__device__ __forceinline__ double fma_k(double p, double z, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(p), "v"(z), "s"(c));
+    return r;
+#else
+    return __builtin_fma(p, z, c);
+#endif
+}
+
 // Quotient a / b to ~1 ulp: hardware reciprocal estimate refined by two
 // Newton-Raphson steps, then one residual correction of the quotient.
 __device__ __forceinline__ double div_fast(double a, double b) {
@@ -36,15 +51,15 @@ __device__ __forceinline__ double log_fast(double x) {
     const double s = div_fast(f, m + 1.0);
     const double z = s * s;
     double p = 1.0 / 21.0;
-    p = __builtin_fma(p, z, 1.0 / 19.0);
-    p = __builtin_fma(p, z, 1.0 / 17.0);
-    p = __builtin_fma(p, z, 1.0 / 15.0);
-    p = __builtin_fma(p, z, 1.0 / 13.0);
-    p = __builtin_fma(p, z, 1.0 / 11.0);
-    p = __builtin_fma(p, z, 1.0 / 9.0);
-    p = __builtin_fma(p, z, 1.0 / 7.0);
-    p = __builtin_fma(p, z, 1.0 / 5.0);
-    p = __builtin_fma(p, z, 1.0 / 3.0);
+    p = fma_k(p, z, 1.0 / 19.0);
+    p = fma_k(p, z, 1.0 / 17.0);
+    p = fma_k(p, z, 1.0 / 15.0);
+    p = fma_k(p, z, 1.0 / 13.0);
+    p = fma_k(p, z, 1.0 / 11.0);
+    p = fma_k(p, z, 1.0 / 9.0);
+    p = fma_k(p, z, 1.0 / 7.0);
+    p = fma_k(p, z, 1.0 / 5.0);
+    p = fma_k(p, z, 1.0 / 3.0);
     // log m = 2s + 2s z p; the correction term is < 1% of 2s.
     const double two_s = s + s;
     const double log_m = __builtin_fma(two_s * z, p, two_s);
@@ -54,7 +69,7 @@ __device__ __forceinline__ double log_fast(double x) {
     return __builtin_fma(kd, ln2_hi, __builtin_fma(kd, ln2_lo, log_m));
 }
 
-// cos(2 pi u) for u in [0, 1], to within ~1e-16 absolute.
+// cos(2 pi u) for u in [0, 1], to within ~3e-16 absolute.
 // Reduce to w in [0, 1/4] by the symmetries of cosine (period 1, even,
 // cos(pi - a) = -cos a), then the Taylor polynomial of cos in a = 2 pi w,
 // |a| <= pi/2, to degree 22 (remainder (pi/2)^24 / 24! < 1e-22).  Near
@@ -70,16 +85,16 @@ __device__ __forceinline__ double cos2pi_fast(double u) {
     const double a2 = a * a;
     // cos a = sum (-1)^n a^{2n} / (2n)!
     double c = -1.0 / 1.1240007277776077e21;            // -1/22!
-    c = __builtin_fma(c, a2, 1.0 / 2432902008176640000.0);  //  1/20!
-    c = __builtin_fma(c, a2, -1.0 / 6402373705728000.0);    // -1/18!
-    c = __builtin_fma(c, a2, 1.0 / 20922789888000.0);   //  1/16!
-    c = __builtin_fma(c, a2, -1.0 / 87178291200.0);     // -1/14!
-    c = __builtin_fma(c, a2, 1.0 / 479001600.0);        //  1/12!
-    c = __builtin_fma(c, a2, -1.0 / 3628800.0);         // -1/10!
-    c = __builtin_fma(c, a2, 1.0 / 40320.0);            //  1/8!
-    c = __builtin_fma(c, a2, -1.0 / 720.0);             // -1/6!
-    c = __builtin_fma(c, a2, 1.0 / 24.0);               //  1/4!
-    c = __builtin_fma(c, a2, -0.5);                     // -1/2!
+    c = fma_k(c, a2, 1.0 / 2432902008176640000.0);  //  1/20!
+    c = fma_k(c, a2, -1.0 / 6402373705728000.0);    // -1/18!
+    c = fma_k(c, a2, 1.0 / 20922789888000.0);   //  1/16!
+    c = fma_k(c, a2, -1.0 / 87178291200.0);     // -1/14!
+    c = fma_k(c, a2, 1.0 / 479001600.0);        //  1/12!
+    c = fma_k(c, a2, -1.0 / 3628800.0);         // -1/10!
+    c = fma_k(c, a2, 1.0 / 40320.0);            //  1/8!
+    c = fma_k(c, a2, -1.0 / 720.0);             // -1/6!
+    c = fma_k(c, a2, 1.0 / 24.0);               //  1/4!
+    c = __builtin_fma(c, a2, -0.5);                    // -1/2!
     c = __builtin_fma(c, a2, 1.0);
     return neg ? -c : c;
 }
@@ -94,16 +109,16 @@ __device__ __forceinline__ double exp_fast(double x) {
     const double kd = __builtin_rint(x * inv_ln2);
     const double r = __builtin_fma(-kd, ln2_lo, __builtin_fma(-kd, ln2_hi, x));
     double p = 1.0 / 6227020800.0;                 // 1/13!
-    p = __builtin_fma(p, r, 1.0 / 479001600.0);    // 1/12!
-    p = __builtin_fma(p, r, 1.0 / 39916800.0);     // 1/11!
-    p = __builtin_fma(p, r, 1.0 / 3628800.0);      // 1/10!
-    p = __builtin_fma(p, r, 1.0 / 362880.0);       // 1/9!
-    p = __builtin_fma(p, r, 1.0 / 40320.0);        // 1/8!
-    p = __builtin_fma(p, r, 1.0 / 5040.0);         // 1/7!
-    p = __builtin_fma(p, r, 1.0 / 720.0);          // 1/6!
-    p = __builtin_fma(p, r, 1.0 / 120.0);          // 1/5!
-    p = __builtin_fma(p, r, 1.0 / 24.0);           // 1/4!
-    p = __builtin_fma(p, r, 1.0 / 6.0);            // 1/3!
+    p = fma_k(p, r, 1.0 / 479001600.0);    // 1/12!
+    p = fma_k(p, r, 1.0 / 39916800.0);     // 1/11!
+    p = fma_k(p, r, 1.0 / 3628800.0);      // 1/10!
+    p = fma_k(p, r, 1.0 / 362880.0);       // 1/9!
+    p = fma_k(p, r, 1.0 / 40320.0);        // 1/8!
+    p = fma_k(p, r, 1.0 / 5040.0);         // 1/7!
+    p = fma_k(p, r, 1.0 / 720.0);          // 1/6!
+    p = fma_k(p, r, 1.0 / 120.0);          // 1/5!
+    p = fma_k(p, r, 1.0 / 24.0);           // 1/4!
+    p = fma_k(p, r, 1.0 / 6.0);            // 1/3!
     p = __builtin_fma(p, r, 0.5);
     p = __builtin_fma(p, r, 1.0);
     p = __builtin_fma(p, r, 1.0);
