@@ -86,9 +86,9 @@ def main() -> int:
     ap.add_argument("--mode", choices=["run_many", "propagate"], default="run_many",
                     help="run_many: sample + propagate (the metric); propagate: propagation only, durations "
                     "injected from a prior run_many (reads durations instead of sampling them)")
-    ap.add_argument("--shape", default="", metavar="W,K",
-                    help="fix the schedule shape (waves per workgroup, events per wave and step) instead of the "
-                    "occupancy-based choice at the first launch")
+    ap.add_argument("--shape", default="", metavar="W,K[,S]",
+                    help="fix the schedule shape (waves per workgroup, events per wave and step, samples per lane) "
+                    "instead of the choice made at the first launch")
     ap.add_argument("--autotune", type=int, default=0, metavar="N",
                     help="time every schedule shape on N samples before the run and keep the fastest")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -130,8 +130,8 @@ def main() -> int:
     if args.shape:
         prop.configure(*map(int, args.shape.split(",")))
     if args.autotune > 0:
-        w, k, ms = prop.autotune(args.autotune)
-        log(f"autotune on {args.autotune} samples: waves={w} events_per_wave={k} ({ms:.3f} ms)")
+        w, k, spl, ms = prop.autotune(args.autotune)
+        log(f"autotune on {args.autotune} samples: waves={w} events_per_wave={k} samples_per_lane={spl} ({ms:.3f} ms)")
     log(f"workload {args.workload}: E={E} L={L} S={S}")
 
     seeds = capi.DeviceBuffer.from_array(np.arange(S, dtype=np.int32))
@@ -226,7 +226,8 @@ def main() -> int:
             "activities": L,
             "samples": S,
             "outputs": "realized f64 [E][S], durations f64 [L][S], cause i32 [E][S]",
-            "schedule": {"waves": plan["waves"], "events_per_wave": plan["events_per_wave"], "steps": plan["steps"]},
+            "schedule": {"waves": plan["waves"], "events_per_wave": plan["events_per_wave"],
+                         "samples_per_lane": plan["samples_per_lane"], "steps": plan["steps"]},
         },
         "roofline": {
             "bound": "hbm",

@@ -115,24 +115,27 @@ int mcdp_propagate_device(mcdp_propagator *prop, const double *durations_dev, in
 int mcdp_propagate_host(mcdp_propagator *prop, const double *durations, int64_t n, double *realized,
                         int32_t *cause);
 
-/* Plan facts for tests and tuning: out[0..8] = {steps, waves per workgroup,
- * LDS slots, LDS bytes per workgroup, predecessors read from global memory,
- * stale predecessors, orphan activities, max events per wave and step, events
- * per wave bound}.  (No counterpart in the reference.) */
+/* Plan facts for tests and tuning: out[0..9] = {steps, waves per workgroup,
+ * LDS slots, LDS bytes of the slots per workgroup, predecessors read from
+ * global memory, stale predecessors, orphan activities, max events per wave
+ * and step, events per wave bound, samples per lane}.  (No counterpart in the
+ * reference.) */
 int mcdp_propagator_plan_info(const mcdp_propagator *prop, int64_t *out, int n_out);
 
-/* Fixes the schedule shape: `waves` waves per workgroup (1..16) and at most
- * `events_per_wave` events per wave and step (4, 8, 12 or 16).  Without it
- * the first run picks the shape with the most busy waves per CU on the
- * current device.  Results do not depend on the shape.  (No counterpart in
- * the reference: a tuning knob of the device schedule.) */
-int mcdp_propagator_configure(mcdp_propagator *prop, int32_t waves, int32_t events_per_wave);
+/* Fixes the schedule shape: `waves` waves per workgroup (1..16), at most
+ * `events_per_wave` events per wave and step (4, 8, 12 or 16) and
+ * `samples_per_lane` samples per lane (1, or 2 with events_per_wave <= 8).
+ * Without it the first run picks a shape for the current device.  Results do
+ * not depend on the shape.  (No counterpart in the reference: a tuning knob of
+ * the device schedule.) */
+int mcdp_propagator_configure(mcdp_propagator *prop, int32_t waves, int32_t events_per_wave,
+                              int32_t samples_per_lane);
 /* Times every candidate shape on the current device with n_samples samples
  * (seeds 0..n-1, full outputs in temporary device memory) and keeps the
  * fastest; reports it and its time per launch.  (No counterpart in the
  * reference.) */
 int mcdp_propagator_autotune(mcdp_propagator *prop, int64_t n_samples, int32_t *waves, int32_t *events_per_wave,
-                             float *ms);
+                             int32_t *samples_per_lane, float *ms);
 
 /* ---- Device plumbing for callers without their own HIP allocator -------- */
 /* (no counterpart in the reference, which has no device) */

@@ -46,10 +46,10 @@ _SIGNATURES = {
     "mcdp_propagate_device": (C.c_int, [_P, _P, _i64, _i64, _P, _P, _P]),
     "mcdp_propagate_host": (C.c_int, [_P, _pf64, _i64, _pf64, _pi32]),
     "mcdp_propagator_plan_info": (C.c_int, [_P, _pi64, C.c_int]),
-    "mcdp_propagator_configure": (C.c_int, [_P, _i32, _i32]),
+    "mcdp_propagator_configure": (C.c_int, [_P, _i32, _i32, _i32]),
     "mcdp_propagator_autotune": (
         C.c_int,
-        [_P, _i64, C.POINTER(_i32), C.POINTER(_i32), C.POINTER(C.c_float)],
+        [_P, _i64, C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_i32), C.POINTER(C.c_float)],
     ),
     "mcdp_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "mcdp_malloc": (C.c_int, [C.POINTER(_P), C.c_size_t]),
@@ -238,22 +238,23 @@ class Propagator:
         self.L = lib().mcdp_propagator_activity_count(self.h)
 
     PLAN_FIELDS = ["steps", "waves", "lds_slots", "lds_bytes", "global_preds", "stale_preds", "orphans",
-                   "max_events_per_wave", "events_per_wave"]
+                   "max_events_per_wave", "events_per_wave", "samples_per_lane"]
 
     def plan_info(self) -> dict:
         v = np.zeros(len(self.PLAN_FIELDS), dtype=np.int64)
         check(lib().mcdp_propagator_plan_info(self.h, _ptr(v, _pi64), v.size))
         return dict(zip(self.PLAN_FIELDS, map(int, v)))
 
-    def configure(self, waves: int, events_per_wave: int) -> None:
+    def configure(self, waves: int, events_per_wave: int, samples_per_lane: int = 1) -> None:
         """Fixes the schedule shape (results do not depend on it)."""
-        check(lib().mcdp_propagator_configure(self.h, waves, events_per_wave))
+        check(lib().mcdp_propagator_configure(self.h, waves, events_per_wave, samples_per_lane))
 
-    def autotune(self, n_samples: int) -> tuple[int, int, float]:
-        """Times every candidate shape with n_samples samples; keeps and returns (waves, events_per_wave, ms)."""
-        w, k, ms = _i32(0), _i32(0), C.c_float(0)
-        check(lib().mcdp_propagator_autotune(self.h, n_samples, C.byref(w), C.byref(k), C.byref(ms)))
-        return w.value, k.value, float(ms.value)
+    def autotune(self, n_samples: int) -> tuple[int, int, int, float]:
+        """Times every candidate shape with n_samples samples; keeps and returns
+        (waves, events_per_wave, samples_per_lane, ms)."""
+        w, k, spl, ms = _i32(0), _i32(0), _i32(0), C.c_float(0)
+        check(lib().mcdp_propagator_autotune(self.h, n_samples, C.byref(w), C.byref(k), C.byref(spl), C.byref(ms)))
+        return w.value, k.value, spl.value, float(ms.value)
 
     def run_many(self, seeds) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
         """Host run_many: (realized (n,E), durations (n,L), cause (n,E)) as transposed views."""

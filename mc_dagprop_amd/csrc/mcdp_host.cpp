@@ -319,6 +319,7 @@ struct mcdp_propagator {
     std::vector<int32_t> step_off, ev_off, slot_list;
     std::vector<Record> recs, orph;
     int32_t n_steps = 0, waves = 1, slots = 0, max_per_wave = 0, k = kMaxEventsPerWave;
+    int32_t spl = 1;  // samples per lane of the walk kernel (schedule-independent)
     int64_t n_global = 0;
     bool shape_fixed = false;  // set by configure / autotune / the first launch
     // device copies, one per device used
@@ -725,40 +726,59 @@ int ensure_device(mcdp_propagator &P, DeviceCopy *&out) {
 WalkVariant walk_variant(const mcdp_propagator &P, bool injected) {
     const bool alias_fits = P.alias.size() * sizeof(AliasRec) <= kAliasLdsMaxBytes;
     const bool fast = P.n_global == 0 && (injected || alias_fits);
-    return WalkVariant{injected, fast ? kPathsFast : kPathsGeneral, P.k};
+    return WalkVariant{injected, fast ? kPathsFast : kPathsGeneral, P.k, P.spl};
 }
 
 size_t walk_lds_bytes(const mcdp_propagator &P, WalkVariant v) {
-    size_t bytes = (size_t)P.slots * kLanes * sizeof(double);
+    size_t bytes = (size_t)P.slots * kLanes * (size_t)P.spl * sizeof(double);
     if (!v.injected && v.paths == kPathsFast) bytes += P.alias.size() * sizeof(AliasRec);
     return bytes;
 }
 
-// Schedule shapes worth trying: every (waves, events per wave) pair whose
-// step holds the widest level (or that is the largest shape available).
-std::vector<std::pair<int32_t, int32_t>> shape_candidates(const mcdp_propagator &P) {
+// A schedule shape: waves per workgroup, events per wave and step, samples
+// per lane.
+struct Shape {
+    int32_t waves, k, spl;
+};
+
+bool shape_valid(int32_t waves, int32_t k, int32_t spl) {
+    if (waves < 1 || waves > kMaxWaves) return false;
+    if (spl != 1 && spl != 2) return false;
+    bool k_ok = false;
+    for (int c : kEventsPerWaveChoices) k_ok |= c == k;
+    return k_ok && (spl == 1 || k <= kMaxEventsPerWaveSpl2);
+}
+
+// Shapes worth trying: every valid shape whose step holds the widest level
+// (or that is the largest shape available).
+std::vector<Shape> shape_candidates(const mcdp_propagator &P) {
     size_t max_width = 1;
     for (const auto &v : P.levels) max_width = std::max(max_width, v.size());
-    const size_t need = std::min<size_t>(max_width, (size_t)kMaxWaves * kMaxEventsPerWave);
     static const int32_t kWaveChoices[] = {1, 2, 4, 6, 8, 10, 12, 14, 16};
-    std::vector<std::pair<int32_t, int32_t>> out;
-    for (int k : kEventsPerWaveChoices)
-        for (int32_t w : kWaveChoices)
-            if ((size_t)w * (size_t)k >= need) out.emplace_back(w, k);
+    std::vector<Shape> out;
+    for (int spl : kSplChoices) {
+        const int kmax = spl == 1 ? kMaxEventsPerWave : kMaxEventsPerWaveSpl2;
+        const size_t need = std::min<size_t>(max_width, (size_t)kMaxWaves * (size_t)kmax);
+        for (int k : kEventsPerWaveChoices)
+            for (int32_t w : kWaveChoices)
+                if (shape_valid(w, k, spl) && (size_t)w * (size_t)k >= need) out.push_back(Shape{w, k, spl});
+    }
     return out;
 }
 
-// Picks the shape with the most busy waves per CU for the sampling kernel on
-// the current device: resident workgroups per CU (from the runtime's
-// occupancy for the candidate's LDS) times the mean number of waves that have
-// events in a step.  Ties: fewer steps, then fewer waves per workgroup.
-// Candidate schedules are built on the host; P is left with the winner.
+// Picks the one-sample-per-lane shape with the most busy waves per CU on the
+// current device: resident workgroups per CU (from the runtime's occupancy
+// for the candidate's LDS) times the mean number of waves that have events in
+// a step.  Ties: fewer steps, then fewer waves per workgroup.  Candidate
+// schedules are built on the host; P is left with the winner.
 int choose_shape(mcdp_propagator &P) {
-    int32_t best_w = P.waves, best_k = P.k;
+    Shape best{P.waves, P.k, 1};
     double best_busy = -1.0;
     int64_t best_steps = 0;
-    for (const auto &c : shape_candidates(P)) {
-        int rc = build_schedule(P, c.first, c.second);
+    P.spl = 1;
+    for (const Shape &c : shape_candidates(P)) {
+        if (c.spl != 1) continue;
+        int rc = build_schedule(P, c.waves, c.k);
         if (rc) return rc;
         const WalkVariant v = walk_variant(P, false);
         int bpc = 0;
@@ -768,15 +788,14 @@ int choose_shape(mcdp_propagator &P) {
         const double busy = (double)bpc * (double)busy_wave_steps / (double)std::max(P.n_steps, 1);
         const bool better = busy > best_busy * 1.0001 ||
                             (busy >= best_busy * 0.9999 &&
-                             (P.n_steps < best_steps || (P.n_steps == best_steps && c.first < best_w)));
+                             (P.n_steps < best_steps || (P.n_steps == best_steps && c.waves < best.waves)));
         if (better) {
             best_busy = busy;
             best_steps = P.n_steps;
-            best_w = c.first;
-            best_k = c.second;
+            best = c;
         }
     }
-    return build_schedule(P, best_w, best_k);
+    return build_schedule(P, best.waves, best.k);
 }
 
 int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double *dur_in, int64_t n, int64_t ld,
@@ -804,7 +823,8 @@ int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double
     p.dur_in = dur_in;
     p.n = n;
     p.ld = ld;
-    p.n_tiles = (n + kLanes - 1) / kLanes;
+    const int64_t tile = (int64_t)kLanes * P.spl;
+    p.n_tiles = (n + tile - 1) / tile;
     p.realized = realized;
     p.durations = injected ? nullptr : durations;
     p.cause = cause;
@@ -1009,35 +1029,38 @@ int mcdp_propagate_host(mcdp_propagator *prop, const double *durations, int64_t 
 
 int mcdp_propagator_plan_info(const mcdp_propagator *prop, int64_t *out, int n_out) {
     if (!prop || !out) return fail(MCDP_E_INVALID, "null argument");
-    const int64_t v[9] = {prop->n_steps,
+    const int64_t v[10] = {prop->n_steps,
                           prop->waves,
                           prop->slots,
-                          (int64_t)prop->slots * kLanes * (int64_t)sizeof(double),
+                          (int64_t)prop->slots * kLanes * prop->spl * (int64_t)sizeof(double),
                           prop->n_global,
                           prop->n_stale,
                           (int64_t)prop->orph.size(),
                           prop->max_per_wave,
-                          prop->k};
-    for (int i = 0; i < n_out && i < 9; ++i) out[i] = v[i];
+                          prop->k,
+                          prop->spl};
+    for (int i = 0; i < n_out && i < 10; ++i) out[i] = v[i];
     return MCDP_OK;
 }
 
-int mcdp_propagator_configure(mcdp_propagator *prop, int32_t waves, int32_t events_per_wave) {
+int mcdp_propagator_configure(mcdp_propagator *prop, int32_t waves, int32_t events_per_wave,
+                              int32_t samples_per_lane) {
     if (!prop) return fail(MCDP_E_INVALID, "propagator is null");
-    if (waves < 1 || waves > kMaxWaves) return fail(MCDP_E_INVALID, "waves must be in [1, 16]");
-    bool k_ok = false;
-    for (int k : kEventsPerWaveChoices) k_ok |= k == events_per_wave;
-    if (!k_ok) return fail(MCDP_E_INVALID, "events_per_wave must be 4, 8, 12 or 16");
+    if (!shape_valid(waves, events_per_wave, samples_per_lane))
+        return fail(MCDP_E_INVALID,
+                    "invalid shape: waves in [1, 16], events_per_wave in {4, 8, 12, 16}, samples_per_lane 1 or 2 "
+                    "(with events_per_wave <= 8)");
     std::lock_guard<std::mutex> lock(prop->mu);
     int rc = build_schedule(*prop, waves, events_per_wave);
     if (rc) return rc;
+    prop->spl = samples_per_lane;
     prop->drop_device_copies();
     prop->shape_fixed = true;
     return MCDP_OK;
 }
 
 int mcdp_propagator_autotune(mcdp_propagator *prop, int64_t n_samples, int32_t *waves, int32_t *events_per_wave,
-                             float *ms) {
+                             int32_t *samples_per_lane, float *ms) {
     if (!prop) return fail(MCDP_E_INVALID, "propagator is null");
     if (n_samples < 1) return fail(MCDP_E_INVALID, "n_samples must be positive");
     int count = 0;
@@ -1057,8 +1080,8 @@ int mcdp_propagator_autotune(mcdp_propagator *prop, int64_t n_samples, int32_t *
     hipEvent_t t0 = nullptr, t1 = nullptr;
     int rc = MCDP_OK;
     float best_ms = -1.0f;
-    int32_t best_w = prop->waves, best_k = prop->k;
-    std::vector<std::pair<int32_t, int32_t>> cands;
+    Shape best{prop->waves, prop->k, prop->spl};
+    std::vector<Shape> cands;
     {
         std::lock_guard<std::mutex> lock(prop->mu);
         cands = shape_candidates(*prop);
@@ -1068,7 +1091,7 @@ int mcdp_propagator_autotune(mcdp_propagator *prop, int64_t n_samples, int32_t *
     if (e == hipSuccess) e = hipEventCreate(&t1);
     if (e != hipSuccess) rc = hip_fail(e, "autotune setup");
     for (size_t c = 0; rc == MCDP_OK && c < cands.size(); ++c) {
-        rc = mcdp_propagator_configure(prop, cands[c].first, cands[c].second);
+        rc = mcdp_propagator_configure(prop, cands[c].waves, cands[c].k, cands[c].spl);
         if (rc) break;
         {  // skip shapes that cannot be resident
             std::lock_guard<std::mutex> lock(prop->mu);
@@ -1092,18 +1115,18 @@ int mcdp_propagator_autotune(mcdp_propagator *prop, int64_t n_samples, int32_t *
         }
         if (best_ms < 0.0f || t < best_ms) {
             best_ms = t;
-            best_w = cands[c].first;
-            best_k = cands[c].second;
+            best = cands[c];
         }
     }
     if (t0) (void)hipEventDestroy(t0);
     if (t1) (void)hipEventDestroy(t1);
     (void)hipFree(buf);
     if (rc) return rc;
-    rc = mcdp_propagator_configure(prop, best_w, best_k);
+    rc = mcdp_propagator_configure(prop, best.waves, best.k, best.spl);
     if (rc) return rc;
-    if (waves) *waves = best_w;
-    if (events_per_wave) *events_per_wave = best_k;
+    if (waves) *waves = best.waves;
+    if (events_per_wave) *events_per_wave = best.k;
+    if (samples_per_lane) *samples_per_lane = best.spl;
     if (ms) *ms = best_ms;
     return MCDP_OK;
 }

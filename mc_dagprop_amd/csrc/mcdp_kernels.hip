@@ -1,16 +1,22 @@
 // gfx950 kernels of the Monte Carlo DAG propagation path.
 //
 // walk_kernel fuses the two hot loops of Simulator::run (_core.cpp:323-329
-// delay sampling, :332-350 max-plus propagation) for 64 samples per
-// workgroup, one sample per lane:
-//   * plan records (events, predecessors with their distribution inline) are
-//     wave-uniform and arrive through scalar loads, the next one prefetched
-//     while the current one is sampled; the distribution switch is a scalar
-//     branch;
-//   * realized times of predecessors that later steps read live in LDS
-//     (slot-major, 64 doubles per slot, so every access is one conflict-free
-//     ds_read_b64 / ds_write_b64 per lane);
-//   * outputs are sample-contiguous rows, so each wave stores 512-byte runs.
+// delay sampling, :332-350 max-plus propagation) for a tile of 64 x kSpl
+// samples per workgroup: lane l of every wave carries samples
+// tile * 64 kSpl + i * 64 + l, i < kSpl.
+//   * plan records (event headers; predecessors with their distribution
+//     inline) are wave-uniform and arrive through scalar loads, the next one
+//     prefetched while the current one is processed; the distribution switch
+//     is a scalar branch taken once for the lane's kSpl samples, whose draws
+//     then run as independent instruction chains;
+//   * the W waves of a workgroup split each step's events (a step is part of
+//     one topological level); results stay in registers until a barrier, then
+//     go to LDS slots that later steps read;
+//   * LDS is slot-major, a slot holding 64 x kSpl doubles with a lane's
+//     samples adjacent, so a lane reads or writes a slot with one ds_read_b64
+//     or ds_read_b128;
+//   * outputs are sample-contiguous rows, so each store instruction of a wave
+//     writes a 512-byte run (nontemporal: nothing reads them back).
 // The arithmetic of propagation is FP64 add and compare-select only, in the
 // reference's order: t = realized[src] + dur; t = (upper < t) ? upper : t;
 // if (t >= best) {best = t; cause = src}; realized = (upper < best) ? upper : best.
@@ -25,7 +31,28 @@
 
 namespace mcdp {
 
-template <bool kInjected, int kPaths, int K>
+// A lane's kSpl samples of one LDS slot.
+template <int kSpl>
+__device__ __forceinline__ void lds_load(const double *lds, int slot, int lane, double (&v)[kSpl]) {
+    if constexpr (kSpl == 1) {
+        v[0] = lds[slot * kLanes + lane];
+    } else {
+        static_assert(kSpl == 2, "one or two samples per lane");
+        const double2 t = reinterpret_cast<const double2 *>(lds)[slot * kLanes + lane];
+        v[0] = t.x;
+        v[1] = t.y;
+    }
+}
+template <int kSpl>
+__device__ __forceinline__ void lds_store(double *lds, int slot, int lane, const double (&v)[kSpl]) {
+    if constexpr (kSpl == 1) {
+        lds[slot * kLanes + lane] = v[0];
+    } else {
+        reinterpret_cast<double2 *>(lds)[slot * kLanes + lane] = double2{v[0], v[1]};
+    }
+}
+
+template <bool kInjected, int kPaths, int K, int kSpl>
 __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) {
     extern __shared__ double lds[];
     const int lane = threadIdx.x & (kLanes - 1);
@@ -43,77 +70,112 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) 
 
     // Fast variant: stage the alias tables in LDS after the slots, so an
     // empirical draw is a ds_read rather than a vector memory load.
-    AliasRec *const lds_alias = reinterpret_cast<AliasRec *>(lds + (size_t)p.slots * kLanes);
+    AliasRec *const lds_alias = reinterpret_cast<AliasRec *>(lds + (size_t)p.slots * kLanes * kSpl);
     if (!kInjected && kPaths == kPathsFast) {
         const auto src = as_const_space(p.alias);
         for (int i = threadIdx.x; i < p.n_alias; i += blockDim.x) lds_alias[i] = src[i];
         __syncthreads();
     }
-    auto draw = [&](const Record &r, SeedKey key) -> double {
-        if (kPaths == kPathsFast) return draw_duration(r.info, r.a, r.b, (uint32_t)r.i2, key, lds_alias, dists);
-        return draw_duration(r.info, r.a, r.b, (uint32_t)r.i2, key, as_const_space(p.alias), dists);
+    auto draw = [&](const Record &r, const uint32_t(&seed)[kSpl], double(&dur)[kSpl]) {
+        if (kPaths == kPathsFast)
+            draw_durations<kSpl>(r.info, r.a, r.b, (uint32_t)r.i2, seed, lds_alias, dists, dur);
+        else
+            draw_durations<kSpl>(r.info, r.a, r.b, (uint32_t)r.i2, seed, as_const_space(p.alias), dists, dur);
     };
 
     for (int64_t tile = blockIdx.x; tile < p.n_tiles; tile += gridDim.x) {
-        const int64_t s = tile * kLanes + lane;
-        const bool active = s < p.n;
-        SeedKey key{0u, 0u};
-        if (!kInjected) key = seed_key(active ? p.seeds[s] : 0);
+        int64_t s[kSpl];
+        bool active[kSpl];
+        uint32_t seed[kSpl];
+#pragma unroll
+        for (int i = 0; i < kSpl; ++i) {
+            s[i] = tile * (kLanes * kSpl) + i * kLanes + lane;
+            active[i] = s[i] < p.n;
+            seed[i] = (!kInjected && active[i]) ? (uint32_t)p.seeds[s[i]] : 0u;
+        }
 
         for (int t = 0; t < p.n_steps; ++t) {
             const int r0 = step_off[t * W + wave];
             const int r1 = step_off[t * W + wave + 1];
-            double res[K];
+            double res[K][kSpl];
             int j = 0;
             if (r0 < r1) {
                 // One stream of records: an event header, then its predecessors
                 // in input order.  The stream ends in a padding record, so the
                 // one-ahead prefetch never leaves the array.
                 Record cur = recs[r0];
-                double best = 0.0, upper = 0.0;
-                int32_t who = -1, event = 0, remaining = -1;
+                double best[kSpl], upper = 0.0;
+                int32_t who[kSpl], event = 0, remaining = -1;
+#pragma unroll
+                for (int i = 0; i < kSpl; ++i) {
+                    best[i] = 0.0;
+                    who[i] = -1;
+                }
                 for (int r = r0; r < r1; ++r) {
                     const Record nxt = recs[r + 1];
                     if (cur.info & kInfoHeader) {
-                        best = cur.a;
                         upper = cur.b;
                         event = cur.i0;
-                        who = -1;
                         remaining = cur.i2;
+#pragma unroll
+                        for (int i = 0; i < kSpl; ++i) {
+                            best[i] = cur.a;
+                            who[i] = -1;
+                        }
                     } else {
-                        double dur = 0.0;
+                        double dur[kSpl];
+#pragma unroll
+                        for (int i = 0; i < kSpl; ++i) dur[i] = 0.0;
                         if (cur.info & kInfoEdgeValid) {
+                            const int64_t row = (int64_t)cur.i2 * ld;
                             if (kInjected) {
-                                dur = active ? __builtin_nontemporal_load(p.dur_in + (int64_t)cur.i2 * ld + s) : 0.0;
+#pragma unroll
+                                for (int i = 0; i < kSpl; ++i)
+                                    dur[i] = active[i] ? __builtin_nontemporal_load(p.dur_in + row + s[i]) : 0.0;
                             } else {
-                                dur = draw(cur, key);
-                                if ((cur.info & kInfoOwner) && p.durations && active)
-                                    __builtin_nontemporal_store(dur, p.durations + (int64_t)cur.i2 * ld + s);
+                                draw(cur, seed, dur);
+                                if ((cur.info & kInfoOwner) && p.durations) {
+#pragma unroll
+                                    for (int i = 0; i < kSpl; ++i)
+                                        if (active[i]) __builtin_nontemporal_store(dur[i], p.durations + row + s[i]);
+                                }
                             }
                         }
-                        double rs;
+                        double rs[kSpl];
                         if (cur.i1 >= 0) {
-                            rs = lds[cur.i1 * kLanes + lane];
+                            lds_load<kSpl>(lds, cur.i1, lane, rs);
                         } else if (kPaths == kPathsGeneral && cur.i1 == kSrcGlobal) {
-                            rs = active ? p.realized[(int64_t)cur.i0 * ld + s] : 0.0;
+                            const int64_t row = (int64_t)cur.i0 * ld;
+#pragma unroll
+                            for (int i = 0; i < kSpl; ++i) rs[i] = active[i] ? p.realized[row + s[i]] : 0.0;
                         } else {
-                            rs = earliest[cur.i0];
+                            const double e = earliest[cur.i0];
+#pragma unroll
+                            for (int i = 0; i < kSpl; ++i) rs[i] = e;
                         }
-                        double tt = rs + dur;
-                        tt = (upper < tt) ? upper : tt;
-                        if (tt >= best) {
-                            best = tt;
-                            who = cur.i0;
+#pragma unroll
+                        for (int i = 0; i < kSpl; ++i) {
+                            double tt = rs[i] + dur[i];
+                            tt = (upper < tt) ? upper : tt;
+                            if (tt >= best[i]) {
+                                best[i] = tt;
+                                who[i] = cur.i0;
+                            }
                         }
                         --remaining;
                     }
                     if (remaining == 0) {  // the event's last predecessor (or a header with none)
-                        const double rr = (upper < best) ? upper : best;
-                        if (active) {
-                            __builtin_nontemporal_store(rr, p.realized + (int64_t)event * ld + s);
-                            if (p.cause) __builtin_nontemporal_store(who, p.cause + (int64_t)event * ld + s);
+                        const int64_t row = (int64_t)event * ld;
+#pragma unroll
+                        for (int i = 0; i < kSpl; ++i) {
+                            const double rr = (upper < best[i]) ? upper : best[i];
+                            if (active[i]) {
+                                __builtin_nontemporal_store(rr, p.realized + row + s[i]);
+                                if (p.cause) __builtin_nontemporal_store(who[i], p.cause + row + s[i]);
+                            }
+                            res[j][i] = rr;
                         }
-                        res[j++] = rr;
+                        ++j;
                         remaining = -1;
                     }
                     cur = nxt;
@@ -125,7 +187,7 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) 
             const int e0 = ev_off[t * W + wave];
             for (int i = 0; i < j; ++i) {
                 const int slot = slot_list[e0 + i];
-                if (slot >= 0) lds[slot * kLanes + lane] = res[i];
+                if (slot >= 0) lds_store<kSpl>(lds, slot, lane, res[i]);
             }
             __syncthreads();
         }
@@ -133,10 +195,14 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) 
         // Activities no predecessor entry references still get a duration row.
         if (!kInjected && p.durations) {
             const auto orph = as_const_space(p.orph);
-            for (int i = wave; i < p.n_orphans; i += W) {
-                const Record o = orph[i];
-                const double dur = draw(o, key);
-                if (active) __builtin_nontemporal_store(dur, p.durations + (int64_t)o.i2 * ld + s);
+            for (int o = wave; o < p.n_orphans; o += W) {
+                const Record rec = orph[o];
+                double dur[kSpl];
+                draw(rec, seed, dur);
+                const int64_t row = (int64_t)rec.i2 * ld;
+#pragma unroll
+                for (int i = 0; i < kSpl; ++i)
+                    if (active[i]) __builtin_nontemporal_store(dur[i], p.durations + row + s[i]);
             }
         }
     }
@@ -181,20 +247,28 @@ hipError_t launch_copy(double *dst, const double *src, int64_t n, hipStream_t st
     return hipGetLastError();
 }
 
-template <bool kInjected, int kPaths>
+template <bool kInjected, int kPaths, int kSpl>
 const void *walk_symbol(int k) {
     switch (k) {
-        case 4: return (const void *)walk_kernel<kInjected, kPaths, 4>;
-        case 8: return (const void *)walk_kernel<kInjected, kPaths, 8>;
-        case 12: return (const void *)walk_kernel<kInjected, kPaths, 12>;
-        case 16: return (const void *)walk_kernel<kInjected, kPaths, 16>;
+        case 4: return (const void *)walk_kernel<kInjected, kPaths, 4, kSpl>;
+        case 8: return (const void *)walk_kernel<kInjected, kPaths, 8, kSpl>;
+        case 12: return kSpl == 1 ? (const void *)walk_kernel<kInjected, kPaths, 12, 1> : nullptr;
+        case 16: return kSpl == 1 ? (const void *)walk_kernel<kInjected, kPaths, 16, 1> : nullptr;
         default: return nullptr;
     }
 }
 
+template <bool kInjected, int kPaths>
+const void *walk_symbol(int k, int spl) {
+    return spl == 2 ? walk_symbol<kInjected, kPaths, 2>(k) : spl == 1 ? walk_symbol<kInjected, kPaths, 1>(k) : nullptr;
+}
+
 const void *walk_symbol(WalkVariant v) {
-    if (v.injected) return v.paths == kPathsFast ? walk_symbol<true, kPathsFast>(v.k) : walk_symbol<true, kPathsGeneral>(v.k);
-    return v.paths == kPathsFast ? walk_symbol<false, kPathsFast>(v.k) : walk_symbol<false, kPathsGeneral>(v.k);
+    if (v.injected)
+        return v.paths == kPathsFast ? walk_symbol<true, kPathsFast>(v.k, v.spl)
+                                     : walk_symbol<true, kPathsGeneral>(v.k, v.spl);
+    return v.paths == kPathsFast ? walk_symbol<false, kPathsFast>(v.k, v.spl)
+                                 : walk_symbol<false, kPathsGeneral>(v.k, v.spl);
 }
 
 hipError_t launch_walk(const WalkParams &p, WalkVariant v, int grid, size_t lds_bytes, hipStream_t stream) {

@@ -17,6 +17,14 @@ constexpr int kMaxWaves = 16;  // waves per workgroup (1024 threads)
 constexpr int kEventsPerWaveChoices[] = {4, 8, 12, 16};
 constexpr int kMaxEventsPerWave = 16;
 
+// Samples per lane: a workgroup's tile is 64 x spl samples.  Two samples per
+// lane halve the scalar work (record loads, distribution dispatch) per sample
+// and give each wave two independent draw chains; they double the LDS of a
+// slot and the registers of the step results, so they pair with 4 or 8
+// events per wave.
+constexpr int kSplChoices[] = {1, 2};
+constexpr int kMaxEventsPerWaveSpl2 = 8;
+
 // Alias tables up to this size are staged in LDS once per workgroup.
 constexpr size_t kAliasLdsMaxBytes = 16 * 1024;
 
@@ -44,7 +52,7 @@ struct WalkParams {
     const double *dur_in;   // injected mode: [L][ld]
     int64_t n;              // samples
     int64_t ld;             // leading dimension of every per-sample array
-    int64_t n_tiles;        // ceil(n / 64)
+    int64_t n_tiles;        // ceil(n / (64 spl))
     double *realized;       // [E][ld] (read within the launch when a plan spills slots)
     double *durations;      // [L][ld] or null (sampled mode)
     int32_t *cause;         // [E][ld] or null
@@ -54,6 +62,7 @@ struct WalkVariant {
     bool injected;  // durations read from dur_in instead of sampled
     int paths;      // WalkPaths
     int k;          // one of kEventsPerWaveChoices
+    int spl;        // samples per lane: one of kSplChoices (spl 2: k <= kMaxEventsPerWaveSpl2)
 };
 
 hipError_t launch_walk(const WalkParams &p, WalkVariant v, int grid, size_t lds_bytes, hipStream_t stream);

@@ -97,26 +97,19 @@ __device__ __forceinline__ double u01(uint32_t hi, uint32_t lo) { return bits53(
 // the acceptance test of Marsaglia-Tsang.
 __device__ __forceinline__ double u01_mid32(uint32_t w) { return ((double)w + 0.5) * 0x1p-32; }
 
-// Key of a run: the reference seeds from the C int seed converted to uint64
-// (sign extension, _custom_rng.hpp:570); the 64-bit key is that value.
-struct SeedKey {
-    uint32_t lo, hi;
-};
-__device__ __forceinline__ SeedKey seed_key(int32_t seed) {
-    return SeedKey{(uint32_t)seed, seed < 0 ? 0xFFFFFFFFu : 0u};
-}
-
 // The random words of one draw.  Philox is keyed by the activity index (the
 // same for every lane of a wave, so the round keys live in scalar registers)
 // and counts over the run's seed: activity a is stream a, the seed is the
-// position in it.
-//   two words:  Philox-2x32-10(ctr = (seed_lo, tag), key = a)
-//   four words: Philox-4x32-10(ctr = (seed_lo, seed_hi, attempt, 0), key = (a, 0))
-__device__ __forceinline__ Words2 draw_words2(uint32_t edge, SeedKey s, uint32_t tag) {
-    return philox2x32_10(s.lo, tag, edge);
+// position in it, a tag word separates the draws one activity may need.  The
+// seed is the C int seed's low 32 bits; the reference widens it by sign
+// extension (_custom_rng.hpp:570), so the high word carries nothing more.
+//   two words:  Philox-2x32-10(ctr = (seed, tag), key = a)
+//   four words: Philox-4x32-10(ctr = (seed, tag, attempt, 0), key = (a, 0))
+__device__ __forceinline__ Words2 draw_words2(uint32_t edge, uint32_t seed, uint32_t tag) {
+    return philox2x32_10(seed, tag, edge);
 }
-__device__ __forceinline__ Words4 draw_words4(uint32_t edge, SeedKey s, uint32_t attempt) {
-    return philox4x32_10(s.lo, s.hi, attempt, 0u, edge, 0u);
+__device__ __forceinline__ Words4 draw_words4(uint32_t edge, uint32_t seed, uint32_t attempt, uint32_t tag) {
+    return philox4x32_10(seed, tag, attempt, 0u, edge, 0u);
 }
 
 // ---- per-kind draws of the multiplier X (or the absolute extra) -------------
@@ -143,12 +136,12 @@ __device__ __forceinline__ bool mt_attempt(const Words4 &w, double d, double c, 
 // attempt number in the counter so every lane's draw is fixed by (seed, edge)
 // alone.  Returns X (scale included).
 __device__ __forceinline__ double gamma_rejection(uint32_t kind, uint32_t flags, const DistParams &D, uint32_t edge,
-                                               SeedKey key) {
+                                                  uint32_t seed) {
     const double m = D.p[3];
     double X = 0.0;
     uint32_t j = 0;
     for (;;) {
-        const Words4 w = draw_words4(edge, key, j);
+        const Words4 w = draw_words4(edge, seed, j, kTagMain);
         bool ok = true;
         if (kind == kGammaErlang2) {  // sum of two unit exponentials
             X = -D.p[0] * log_fast(u01_open0(w.x, w.y) * u01_open0(w.z, w.w));
@@ -160,7 +153,7 @@ __device__ __forceinline__ double gamma_rejection(uint32_t kind, uint32_t flags,
             ok = mt_attempt(w, D.p[1], D.p[2], y);
             if (ok) {
                 if (flags & kInfoBoost) {  // shape < 1: Gamma(k) = Gamma(k + 1) U^{1/k}
-                    const Words2 b = draw_words2(edge, key, (j << 8) | kTagBoost);
+                    const Words2 b = draw_words2(edge, seed, (j << 8) | kTagBoost);
                     y *= exp_fast(log_fast(u01_open0(b.x, b.y)) * D.p[4]);
                 }
                 X = D.p[0] * y;
@@ -180,72 +173,104 @@ __device__ __forceinline__ double gamma_rejection(uint32_t kind, uint32_t flags,
     return X;
 }
 
-// Draws the duration of one activity for one sample.  `info`, `dmin` and `p0`
-// come from the activity's wave-uniform record, `edge` is its index; kinds that
-// need more parameters read them from `dists`.  Returns base + extra rounded
-// exactly as the reference (_core.cpp:75, 88, 103, 125, 140, 328): extra =
-// X * d (one rounding), then d + extra (one rounding); this file is compiled
-// with -ffp-contract=off.  `alias` points at the alias records, in LDS or in
-// the constant address space.
-template <typename AliasPtr>
-__device__ __forceinline__ double draw_duration(uint32_t info, double dmin, double p0, uint32_t edge, SeedKey key,
-                                                AliasPtr alias, const_ptr<DistParams> dists) {
+// Draws the duration of one activity for N samples (a lane's samples): the
+// distribution switch is taken once and each case runs the N independent
+// draws as straight-line code, so their dependency chains interleave.
+// `info`, `dmin` and `p0` come from the activity's wave-uniform record, `edge`
+// is its index; kinds that need more parameters read them from `dists`.
+// Each result is base + extra rounded exactly as the reference (_core.cpp:75,
+// 88, 103, 125, 140, 328): extra = X * d (one rounding), then d + extra (one
+// rounding); this file is compiled with -ffp-contract=off.  `alias` points at
+// the alias records, in LDS or in the constant address space.
+template <int N, typename AliasPtr>
+__device__ __forceinline__ void draw_durations(uint32_t info, double dmin, double p0, uint32_t edge,
+                                               const uint32_t (&seed)[N], AliasPtr alias,
+                                               const_ptr<DistParams> dists, double (&dur)[N]) {
     const uint32_t kind = info & kInfoKindMask;
-    double X;
+    double X[N];
     switch (kind) {
-        case kNone: return dmin;
-        case kEmpiricalAbs1: return dmin + p0;
+        case kNone:
+#pragma unroll
+            for (int i = 0; i < N; ++i) dur[i] = dmin;
+            return;
+        case kEmpiricalAbs1:
+#pragma unroll
+            for (int i = 0; i < N; ++i) dur[i] = dmin + p0;
+            return;
         case kEmpiricalAbs:
         case kEmpiricalRel: {
             // Vose alias table: one uniform picks the bin and the coin.
             const uint64_t packed = (uint64_t)__double_as_longlong(p0);  // (n, table_off)
             const int n = (int)(uint32_t)packed;
             const int table_off = (int)(uint32_t)(packed >> 32);
-            const Words2 w = draw_words2(edge, key, kTagMain);
-            const double xn = u01(w.x, w.y) * (double)n;
-            int i = (int)xn;
-            i = (i >= n) ? n - 1 : i;
-            const double frac = xn - (double)i;
-            const AliasRec rec = alias[table_off + i];
-            const double v = (frac < rec.prob) ? rec.v_self : rec.v_alias;
-            if (kind == kEmpiricalAbs) return dmin + v;
-            X = v;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const Words2 w = draw_words2(edge, seed[i], kTagMain);
+                const double xn = u01(w.x, w.y) * (double)n;
+                int b = (int)xn;
+                b = (b >= n) ? n - 1 : b;
+                const double frac = xn - (double)b;
+                const AliasRec rec = alias[table_off + b];
+                X[i] = (frac < rec.prob) ? rec.v_self : rec.v_alias;
+            }
+            if (kind == kEmpiricalAbs) {
+#pragma unroll
+                for (int i = 0; i < N; ++i) dur[i] = dmin + X[i];
+                return;
+            }
             break;
         }
         case kConstant:
-        case kEmpiricalRel1: X = p0; break;
-        case kExpInf: {  // inversion: X = -mean log U, U in (0, 1]
-            const Words2 w = draw_words2(edge, key, kTagMain);
-            X = -p0 * log_fast(u01_open0(w.x, w.y));
+        case kEmpiricalRel1:
+#pragma unroll
+            for (int i = 0; i < N; ++i) X[i] = p0;
             break;
-        }
+        case kExpInf:  // inversion: X = -mean log U, U in (0, 1]
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const Words2 w = draw_words2(edge, seed[i], kTagMain);
+                X[i] = -p0 * log_fast(u01_open0(w.x, w.y));
+            }
+            break;
         case kExpTrunc: {
             // Inversion of the CDF truncated at m: U' = q + (1 - q) U is uniform on
             // (q, 1] with q = e^{-m/mean}, and X = -mean log U' lies in [0, m].
             const DistParams D = dists[(info >> kInfoDistShift) & kInfoDistMask];
-            const Words2 w = draw_words2(edge, key, kTagMain);
-            const double u = __builtin_fma(u01_open0(w.x, w.y), D.p[3], D.p[1]);
-            X = -p0 * log_fast(u);
-            X = (X > D.p[2]) ? D.p[2] : X;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const Words2 w = draw_words2(edge, seed[i], kTagMain);
+                const double u = __builtin_fma(u01_open0(w.x, w.y), D.p[3], D.p[1]);
+                const double x = -p0 * log_fast(u);
+                X[i] = (x > D.p[2]) ? D.p[2] : x;
+            }
             break;
         }
         default: {
             if (!(info & kInfoTruncated) && kind == kGammaErlang2) {  // sum of two unit exponentials
-                const Words4 w = draw_words4(edge, key, 0u);
-                X = -p0 * log_fast(u01_open0(w.x, w.y) * u01_open0(w.z, w.w));
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    const Words4 w = draw_words4(edge, seed[i], 0u, kTagMain);
+                    X[i] = -p0 * log_fast(u01_open0(w.x, w.y) * u01_open0(w.z, w.w));
+                }
             } else if (!(info & kInfoTruncated) && kind == kGammaHalf) {  // Exp(1) cos^2(2 pi U)
-                const Words4 w = draw_words4(edge, key, 0u);
-                const double cs = cos2pi_fast(u01_mid32(w.z));
-                X = p0 * (-log_fast(u01_open0(w.x, w.y)) * (cs * cs));
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    const Words4 w = draw_words4(edge, seed[i], 0u, kTagMain);
+                    const double cs = cos2pi_fast(u01_mid32(w.z));
+                    X[i] = p0 * (-log_fast(u01_open0(w.x, w.y)) * (cs * cs));
+                }
             } else {
                 const DistParams D = dists[(info >> kInfoDistShift) & kInfoDistMask];
-                X = gamma_rejection(kind, info, D, edge, key);
+                for (int i = 0; i < N; ++i) X[i] = gamma_rejection(kind, info, D, edge, seed[i]);
             }
             break;
         }
     }
-    const double extra = X * dmin;
-    return dmin + extra;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const double extra = X[i] * dmin;
+        dur[i] = dmin + extra;
+    }
 }
 
 }  // namespace mcdp

@@ -375,36 +375,36 @@ PYBIND11_MODULE(_core, m) {
         .def(
             "plan_info",
             [](const Propagator &p) {
-                int64_t v[9];
-                check(mcdp_propagator_plan_info(p.p, v, 9));
+                int64_t v[10];
+                check(mcdp_propagator_plan_info(p.p, v, 10));
                 py::dict d;
-                const char *names[9] = {"steps",        "waves",       "lds_slots",
-                                        "lds_bytes",    "global_preds", "stale_preds",
-                                        "orphans",      "max_events_per_wave", "events_per_wave"};
-                for (int i = 0; i < 9; ++i) d[names[i]] = v[i];
+                const char *names[10] = {"steps",   "waves",           "lds_slots",           "lds_bytes",
+                                         "global_preds", "stale_preds", "orphans",             "max_events_per_wave",
+                                         "events_per_wave", "samples_per_lane"};
+                for (int i = 0; i < 10; ++i) d[names[i]] = v[i];
                 return d;
             },
             "Facts about the compiled device plan")
         .def(
             "configure",
-            [](Propagator &p, int waves, int events_per_wave) {
-                check(mcdp_propagator_configure(p.p, waves, events_per_wave));
+            [](Propagator &p, int waves, int events_per_wave, int samples_per_lane) {
+                check(mcdp_propagator_configure(p.p, waves, events_per_wave, samples_per_lane));
             },
-            py::arg("waves"), py::arg("events_per_wave"),
-            "Fixes the device schedule shape (waves per workgroup, events per wave and step); "
-            "results do not depend on it")
+            py::arg("waves"), py::arg("events_per_wave"), py::arg("samples_per_lane") = 1,
+            "Fixes the device schedule shape (waves per workgroup, events per wave and step, samples per "
+            "lane); results do not depend on it")
         .def(
             "autotune",
             [](Propagator &p, int64_t n_samples) {
-                int32_t w = 0, k = 0;
+                int32_t w = 0, k = 0, spl = 0;
                 float ms = 0.f;
                 {
                     py::gil_scoped_release nogil;
-                    check(mcdp_propagator_autotune(p.p, n_samples, &w, &k, &ms));
+                    check(mcdp_propagator_autotune(p.p, n_samples, &w, &k, &spl, &ms));
                 }
-                return py::make_tuple(w, k, ms);
+                return py::make_tuple(w, k, spl, ms);
             },
             py::arg("n_samples"),
             "Times each candidate schedule shape on n_samples samples and keeps the fastest; "
-            "returns (waves, events_per_wave, ms)");
+            "returns (waves, events_per_wave, samples_per_lane, ms)");
 }

@@ -56,7 +56,7 @@ def test_null_arguments_rejected() -> None:
     lib = capi.lib()
     assert lib.mcdp_propagator_plan_info(None, None, 0) == 1
     assert b"null" in lib.mcdp_last_error()
-    assert lib.mcdp_propagator_configure(None, 4, 8) == 1
+    assert lib.mcdp_propagator_configure(None, 4, 8, 1) == 1
 
 
 # ---- Python API surface (src/mc_dagprop/monte_carlo/_core.pyi) ---------------------------
@@ -184,6 +184,11 @@ def test_plan_is_host_side() -> None:
         sim.configure(0, 4)
     with pytest.raises(RuntimeError):
         sim.configure(2, 5)
+    with pytest.raises(RuntimeError):
+        sim.configure(2, 16, 2)
+    sim.configure(2, 8, 2)
+    info = sim.plan_info()
+    assert info["samples_per_lane"] == 2 and info["lds_bytes"] == info["lds_slots"] * 1024
 
 
 # ---- the reference's cases on the GPU -------------------------------------------------

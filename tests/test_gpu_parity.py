@@ -197,9 +197,13 @@ def test_network_slice_consistent(gpu) -> None:
     assert np.all(np.isfinite(d)) and np.all(d >= dag.act_dur[None, :])
 
 
-@pytest.mark.parametrize("shape", [(1, 4), (2, 16), (5, 8), (16, 12), (16, 16)])
+SHAPES = [(1, 4, 1), (2, 16, 1), (5, 8, 1), (16, 12, 1), (16, 16, 1), (1, 4, 2), (3, 8, 2), (7, 4, 2), (16, 8, 2)]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=[f"w{w}k{k}s{s}" for w, k, s in SHAPES])
 def test_results_independent_of_schedule_shape(gpu, shape) -> None:
-    """Every schedule shape (waves per workgroup, events per wave and step) gives the same bits."""
+    """Every schedule shape (waves per workgroup, events per wave and step, samples per lane) gives
+    the same bits; 200 samples leave a partial last tile in both tile sizes (64 and 128)."""
     dag = layered_network(layers=12)
     seeds = np.arange(-50, 150)
     ref = capi.Propagator(dag)
@@ -207,7 +211,7 @@ def test_results_independent_of_schedule_shape(gpu, shape) -> None:
     P = capi.Propagator(dag)
     P.configure(*shape)
     info = P.plan_info()
-    assert (info["waves"], info["events_per_wave"]) == shape
+    assert (info["waves"], info["events_per_wave"], info["samples_per_lane"]) == shape
     r, d, c = P.run_many(seeds)
     np.testing.assert_array_equal(bits(r), bits(r0))
     np.testing.assert_array_equal(bits(d), bits(d0))
@@ -222,11 +226,45 @@ def test_autotune_keeps_results(gpu) -> None:
     seeds = np.arange(300)
     P = capi.Propagator(dag)
     r0, d0, c0 = P.run_many(seeds)
-    w, k, ms = P.autotune(4096)
-    assert 1 <= w <= 16 and k in (4, 8, 12, 16) and ms > 0
+    w, k, spl, ms = P.autotune(4096)
+    assert 1 <= w <= 16 and k in (4, 8, 12, 16) and spl in (1, 2) and ms > 0
     info = P.plan_info()
-    assert (info["waves"], info["events_per_wave"]) == (w, k)
+    assert (info["waves"], info["events_per_wave"], info["samples_per_lane"]) == (w, k, spl)
     r, d, c = P.run_many(seeds)
     np.testing.assert_array_equal(bits(r), bits(r0))
     np.testing.assert_array_equal(bits(d), bits(d0))
     np.testing.assert_array_equal(c, c0)
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 127, 128, 129, 300])
+def test_two_samples_per_lane_edges(gpu, n: int) -> None:
+    """Tiles of 128 samples: sizes around the tile and half-tile boundaries, device API with ld > n,
+    injected durations, all against the one-sample-per-lane kernel."""
+    dag = layered_small()
+    A = capi.Propagator(dag)
+    A.configure(8, 16, 1)
+    B = capi.Propagator(dag)
+    B.configure(8, 8, 2)
+    seeds = np.arange(1000, 1000 + n)
+    ra, da, ca = A.run_many(seeds)
+    rb, db, cb = B.run_many(seeds)
+    np.testing.assert_array_equal(bits(ra), bits(rb))
+    np.testing.assert_array_equal(bits(da), bits(db))
+    np.testing.assert_array_equal(ca, cb)
+    rr, cc = B.propagate(da)
+    np.testing.assert_array_equal(bits(rr), bits(ra))
+    np.testing.assert_array_equal(cc, ca)
+    ld = n + 3  # odd leading dimension
+    sd = capi.DeviceBuffer.from_array(seeds.astype(np.int32))
+    real = capi.DeviceBuffer.from_array(np.full((B.E, ld), -7.0))
+    dur = capi.DeviceBuffer.from_array(np.full((B.L, ld), -7.0))
+    cause = capi.DeviceBuffer.from_array(np.full((B.E, ld), -9, dtype=np.int32))
+    B.run_many_device(sd, n, ld, real, dur, cause)
+    capi.synchronize()
+    R = real.to_array(np.float64, (B.E, ld))
+    Dd = dur.to_array(np.float64, (B.L, ld))
+    Cc = cause.to_array(np.int32, (B.E, ld))
+    assert np.all(R[:, n:] == -7.0) and np.all(Dd[:, n:] == -7.0) and np.all(Cc[:, n:] == -9)
+    np.testing.assert_array_equal(bits(R[:, :n].T), bits(ra))
+    np.testing.assert_array_equal(bits(Dd[:, :n].T), bits(da))
+    np.testing.assert_array_equal(Cc[:, :n].T, ca)
