@@ -54,7 +54,10 @@ def read_traffic(path: Path):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(dag, L: int, threads: int, per_thread: int):
+def cpu_baseline(dag, L: int, threads: int, per_thread: int, min_seconds: float = 10.0, max_chunks: int = 12):
+    """The reference's run_many loop on host threads: chunks of threads x per_thread seeds (each chunk's
+    SimResults are freed before the next, so host memory stays bounded), repeated until at least
+    min_seconds of work; the rate is total activity-samples over total time."""
     import oracle
 
     if not oracle.ref_available():
@@ -62,15 +65,20 @@ def cpu_baseline(dag, L: int, threads: int, per_thread: int):
     ref = oracle.RefSim(dag)
     ref.run_many_timed(np.arange(threads), threads)  # warm up
     n = threads * per_thread
-    seeds = np.arange(n, dtype=np.int32)
-    t = ref.run_many_timed(seeds, threads)
+    total_t, total_n, chunks = 0.0, 0, 0
+    while chunks < max_chunks and (chunks == 0 or total_t < min_seconds):
+        seeds = np.arange(total_n, total_n + n, dtype=np.int32)
+        total_t += ref.run_many_timed(seeds, threads)
+        total_n += n
+        chunks += 1
     return {
-        "value": L * n / t,
+        "value": L * total_n / total_t,
         "unit": "activity-samples/s",
         "cores": threads,
         "kind": "reference",
-        "sample": f"reference run_many loop (Simulator::run per seed, SimResult materialised), {n} seeds "
-        f"split over {threads} threads, one Simulator each; {t:.2f} s wall, same DAG and distributions",
+        "sample": f"reference run_many loop (Simulator::run per seed, SimResult materialised) over seeds "
+        f"0..{total_n - 1} in {chunks} chunks of {n} seeds, each split over {threads} threads with one Simulator "
+        f"each; {total_t:.2f} s of wall time, same DAG and distributions",
     }
 
 
@@ -89,8 +97,9 @@ def main() -> int:
     ap.add_argument("--shape", default="", metavar="W,K[,S]",
                     help="fix the schedule shape (waves per workgroup, events per wave and step, samples per lane) "
                     "instead of the choice made at the first launch")
-    ap.add_argument("--autotune", type=int, default=0, metavar="N",
-                    help="time every schedule shape on N samples before the run and keep the fastest")
+    ap.add_argument("--autotune", type=int, default=-1, metavar="N",
+                    help="before the run, time every schedule shape on N samples (default: the workload's "
+                    "sample count) and keep the fastest; 0 keeps the shape chosen at the first launch")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write) --steps times, to calibrate "
                     "rocprofv3 FETCH_SIZE / WRITE_SIZE against known bytes")
@@ -98,7 +107,7 @@ def main() -> int:
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (samples would shard "
-              "across GPUs with the context replicated; see the sharding note in DESIGN.md).", file=sys.stderr)
+              "across GPUs with the compiled context replicated on each).", file=sys.stderr)
         return 2
 
     from mc_dagprop_amd import capi
@@ -129,8 +138,9 @@ def main() -> int:
     E, L, S = prop.E, prop.L, args.samples
     if args.shape:
         prop.configure(*map(int, args.shape.split(",")))
-    if args.autotune > 0:
-        w, k, spl, ms = prop.autotune(args.autotune)
+    elif args.autotune != 0:
+        args.autotune = S if args.autotune < 0 else args.autotune
+        w, k, spl, ms = prop.autotune(args.autotune)  # setup, outside the timed steps
         log(f"autotune on {args.autotune} samples: waves={w} events_per_wave={k} samples_per_lane={spl} ({ms:.3f} ms)")
     log(f"workload {args.workload}: E={E} L={L} S={S}")
 
@@ -197,7 +207,7 @@ def main() -> int:
     baseline = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        baseline = cpu_baseline(dag, L, threads, 256 if args.workload == "network" else 2048)
+        baseline = cpu_baseline(dag, L, threads, 1024 if args.workload == "network" else 65536)
         if baseline:
             log(f"cpu baseline: {baseline['value']:.4g} activity-samples/s on {threads} threads")
 
