@@ -730,7 +730,7 @@ WalkVariant walk_variant(const mcdp_propagator &P, bool injected) {
 }
 
 size_t walk_lds_bytes(const mcdp_propagator &P, WalkVariant v) {
-    size_t bytes = (size_t)P.slots * kLanes * (size_t)P.spl * sizeof(double);
+    size_t bytes = (size_t)P.slots * kLanes * (size_t)v.spl * sizeof(double);
     if (!v.injected && v.paths == kPathsFast) bytes += P.alias.size() * sizeof(AliasRec);
     return bytes;
 }
@@ -766,35 +766,42 @@ std::vector<Shape> shape_candidates(const mcdp_propagator &P) {
     return out;
 }
 
-// Picks the one-sample-per-lane shape with the most busy waves per CU on the
-// current device: resident workgroups per CU (from the runtime's occupancy
-// for the candidate's LDS) times the mean number of waves that have events in
-// a step.  Ties: fewer steps, then fewer waves per workgroup.  Candidate
-// schedules are built on the host; P is left with the winner.
+// Picks a shape for the current device without running anything: the most
+// samples resident per CU (workgroups per CU x 64 x samples per lane), then
+// two samples per lane (less scalar work per sample), then the least work on
+// the busiest wave of each step (events x samples per lane, summed over
+// steps), then more waves.  Candidate schedules are built on the host; P is
+// left with the winner.
 int choose_shape(mcdp_propagator &P) {
     Shape best{P.waves, P.k, 1};
-    double best_busy = -1.0;
-    int64_t best_steps = 0;
-    P.spl = 1;
+    int64_t best_res = -1, best_load = 0;
     for (const Shape &c : shape_candidates(P)) {
-        if (c.spl != 1) continue;
         int rc = build_schedule(P, c.waves, c.k);
         if (rc) return rc;
+        P.spl = c.spl;
         const WalkVariant v = walk_variant(P, false);
         int bpc = 0;
         if (walk_occupancy(v, P.waves, walk_lds_bytes(P, v), &bpc) != hipSuccess || bpc < 1) continue;
-        int64_t busy_wave_steps = 0;
-        for (size_t g = 0; g + 1 < P.ev_off.size(); ++g) busy_wave_steps += P.ev_off[g + 1] > P.ev_off[g] ? 1 : 0;
-        const double busy = (double)bpc * (double)busy_wave_steps / (double)std::max(P.n_steps, 1);
-        const bool better = busy > best_busy * 1.0001 ||
-                            (busy >= best_busy * 0.9999 &&
-                             (P.n_steps < best_steps || (P.n_steps == best_steps && c.waves < best.waves)));
+        const int64_t res = (int64_t)bpc * kLanes * c.spl;
+        int64_t load = 0;
+        for (int32_t t = 0; t < P.n_steps; ++t) {
+            int32_t m = 0;
+            for (int32_t w = 0; w < P.waves; ++w)
+                m = std::max(m, P.ev_off[(size_t)t * P.waves + w + 1] - P.ev_off[(size_t)t * P.waves + w]);
+            load += (int64_t)m * c.spl;
+        }
+        const bool better =
+            res > best_res ||
+            (res == best_res &&
+             (c.spl > best.spl ||
+              (c.spl == best.spl && (load < best_load || (load == best_load && c.waves > best.waves)))));
         if (better) {
-            best_busy = busy;
-            best_steps = P.n_steps;
+            best_res = res;
+            best_load = load;
             best = c;
         }
     }
+    P.spl = best.spl;
     return build_schedule(P, best.waves, best.k);
 }
 
@@ -821,10 +828,7 @@ int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double
     WalkParams p = d->base;
     p.seeds = seeds;
     p.dur_in = dur_in;
-    p.n = n;
     p.ld = ld;
-    const int64_t tile = (int64_t)kLanes * P.spl;
-    p.n_tiles = (n + tile - 1) / tile;
     p.realized = realized;
     p.durations = injected ? nullptr : durations;
     p.cause = cause;
@@ -834,8 +838,46 @@ int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device), "hipDeviceGetAttribute");
     HIP_TRY(walk_occupancy(v, P.waves, lds, &bpc), "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     if (bpc < 1) return fail(MCDP_E_DEVICE, "walk kernel cannot be resident with " + std::to_string(lds) + " bytes of LDS");
-    const int64_t grid = std::min<int64_t>(p.n_tiles, (int64_t)cus * bpc);
-    HIP_TRY(launch_walk(p, v, (int)grid, lds, stream), "walk kernel launch");
+    const int64_t resident = (int64_t)cus * bpc;
+    const int64_t tile = (int64_t)kLanes * v.spl;
+
+    // A tile's run time hardly depends on how many of its lanes are live, so a
+    // last round of only a few tiles costs almost a full round.  With two
+    // samples per lane, the samples past the last full round go to the
+    // one-sample-per-lane kernel of the same schedule when they fit in one of
+    // its rounds: its tiles finish sooner.  Results do not depend on the split.
+    int64_t n_main = n, n_tail = 0;
+    WalkVariant v1 = v;
+    v1.spl = 1;
+    const size_t lds1 = walk_lds_bytes(P, v1);
+    int64_t resident1 = 0;
+    if (v.spl == 2) {
+        const int64_t full_rounds = ((n + tile - 1) / tile) / resident;
+        const int64_t rest = n - full_rounds * resident * tile;
+        if (full_rounds >= 1 && rest > 0) {
+            int bpc1 = 0;
+            HIP_TRY(walk_occupancy(v1, P.waves, lds1, &bpc1), "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+            resident1 = (int64_t)cus * bpc1;
+            if (bpc1 >= 1 && (rest + kLanes - 1) / kLanes <= resident1) {
+                n_main = n - rest;
+                n_tail = rest;
+            }
+        }
+    }
+    p.n = n_main;
+    p.n_tiles = (n_main + tile - 1) / tile;
+    HIP_TRY(launch_walk(p, v, (int)std::min<int64_t>(p.n_tiles, resident), lds, stream), "walk kernel launch");
+    if (n_tail > 0) {
+        WalkParams q = p;
+        q.seeds = seeds ? seeds + n_main : nullptr;
+        q.dur_in = dur_in ? dur_in + n_main : nullptr;
+        q.realized = realized + n_main;
+        q.durations = p.durations ? p.durations + n_main : nullptr;
+        q.cause = cause ? cause + n_main : nullptr;
+        q.n = n_tail;
+        q.n_tiles = (n_tail + kLanes - 1) / kLanes;
+        HIP_TRY(launch_walk(q, v1, (int)std::min<int64_t>(q.n_tiles, resident1), lds1, stream), "walk kernel launch");
+    }
     return MCDP_OK;
 }
 

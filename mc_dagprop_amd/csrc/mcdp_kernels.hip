@@ -283,6 +283,11 @@ hipError_t launch_walk(const WalkParams &p, WalkVariant v, int grid, size_t lds_
     return hipLaunchKernel(fn, dim3(grid), dim3(p.waves * kLanes), kargs, lds_bytes, stream);
 }
 
+// Workgroups of the walk kernel resident per CU.  The runtime's figure counts
+// waves per CU; a workgroup's waves are spread over the CU's 4 SIMDs, ceil(W/4)
+// on the fullest one, so a second workgroup fits only if every SIMD has that
+// many wave slots left under the kernel's VGPR budget (512 per lane, in
+// granules of 8, at most 8 waves per SIMD).
 hipError_t walk_occupancy(WalkVariant v, int waves, size_t lds_bytes, int *blocks_per_cu) {
     const void *fn = walk_symbol(v);
     if (!fn) return hipErrorInvalidValue;
@@ -290,7 +295,18 @@ hipError_t walk_occupancy(WalkVariant v, int waves, size_t lds_bytes, int *block
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
         if (e != hipSuccess) return e;
     }
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, waves * kLanes, lds_bytes);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, waves * kLanes, lds_bytes);
+    if (e != hipSuccess) return e;
+    hipFuncAttributes attr;
+    e = hipFuncGetAttributes(&attr, fn);
+    if (e != hipSuccess) return e;
+    if (attr.numRegs > 0) {
+        const int granules = (attr.numRegs + 7) / 8;
+        const int per_simd = std::min(8, 512 / (granules * 8));
+        const int per_wg = (waves + 3) / 4;
+        *blocks_per_cu = std::min(*blocks_per_cu, per_simd / per_wg);
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_philox_debug(int variant, int64_t n, const uint32_t *ctr, const uint32_t *key, uint32_t *out,

@@ -268,3 +268,28 @@ def test_two_samples_per_lane_edges(gpu, n: int) -> None:
     np.testing.assert_array_equal(bits(R[:, :n].T), bits(ra))
     np.testing.assert_array_equal(bits(Dd[:, :n].T), bits(da))
     np.testing.assert_array_equal(Cc[:, :n].T, ca)
+
+
+def test_tail_split_matches(gpu) -> None:
+    """Two samples per lane with a short last round: the samples past the last full round run in the
+    one-sample-per-lane kernel (a second launch on the same stream); every sample matches the
+    one-sample-per-lane run and the oracle."""
+    dag = layered_small()
+    n = 3 * 256 * 128 + 77  # three full rounds of 256 resident tiles of 128, plus 77 samples
+    seeds = np.arange(n, dtype=np.int32) - 1000
+    A = capi.Propagator(dag)
+    A.configure(16, 8, 1)
+    B = capi.Propagator(dag)
+    B.configure(16, 8, 2)
+    ra, da, ca = A.run_many(seeds)
+    rb, db, cb = B.run_many(seeds)
+    np.testing.assert_array_equal(bits(rb), bits(ra))
+    np.testing.assert_array_equal(bits(db), bits(da))
+    np.testing.assert_array_equal(cb, ca)
+    rr, cc = B.propagate(da)
+    np.testing.assert_array_equal(bits(rr), bits(ra))
+    np.testing.assert_array_equal(cc, ca)
+    pick = np.r_[0:64, n - 200:n]
+    r0, c0 = oracle.OracleCtx(dag).propagate(db[pick])
+    np.testing.assert_array_equal(bits(r0), bits(rb[pick]))
+    np.testing.assert_array_equal(c0, cb[pick])
