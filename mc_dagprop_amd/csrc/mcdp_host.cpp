@@ -344,7 +344,10 @@ struct mcdp_propagator {
 
 namespace {
 
-constexpr size_t kLdsBudgetBytes = 64 * 1024;  // LDS per workgroup for predecessor slots
+constexpr size_t kLdsBudgetBytes = 64 * 1024;  // LDS per workgroup for predecessor slots (one sample per lane)
+// Sample indices and leading dimensions are 32-bit in the kernel; tiles of up
+// to 128 samples must not overflow them.
+constexpr int64_t kMaxSamplesPerLaunch = ((int64_t)1 << 31) - 256;
 
 // A predecessor record for edge l: its minimal duration, sampling kind and p0
 // inline; the distribution index for the kinds that read more parameters.
@@ -478,7 +481,7 @@ int build_schedule(mcdp_propagator &P, int32_t W, int32_t K) {
             h.i0 = e;
             h.i1 = slot_of[(size_t)e];
             h.i2 = (int32_t)(P.pred_off[(size_t)e + 1] - P.pred_off[(size_t)e]);
-            h.info = kInfoHeader;
+            h.info = kInfoHeader | (h.i2 == 0 ? kInfoLast : 0u);
             P.recs.push_back(h);
             for (int64_t k = P.pred_off[(size_t)e]; k < P.pred_off[(size_t)e + 1]; ++k) {
                 const PredEntry &pe = P.preds[(size_t)k];
@@ -494,6 +497,7 @@ int build_schedule(mcdp_propagator &P, int32_t W, int32_t K) {
                     referenced[(size_t)pe.edge] = 1;
                     r.info |= kInfoOwner;
                 }
+                if (k + 1 == P.pred_off[(size_t)e + 1]) r.info |= kInfoLast;
                 P.recs.push_back(r);
             }
         }
@@ -809,6 +813,8 @@ int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double
            double *realized, double *durations, int32_t *cause, hipStream_t stream) {
     if (n < 0) return fail(MCDP_E_INVALID, "n must be non-negative");
     if (n > 0 && ld < n) return fail(MCDP_E_INVALID, "ld must be >= n");
+    if (n > kMaxSamplesPerLaunch || ld > std::numeric_limits<int32_t>::max())
+        return fail(MCDP_E_INVALID, "n and ld must be below 2^31 - 256 per call");
     if (injected && !dur_in && P.L > 0 && n > 0) return fail(MCDP_E_INVALID, "durations input is null");
     if (!injected && !seeds && n > 0) return fail(MCDP_E_INVALID, "seeds is null");
     if (!realized && P.E > 0 && n > 0) return fail(MCDP_E_INVALID, "realized must not be null");
@@ -828,7 +834,7 @@ int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double
     WalkParams p = d->base;
     p.seeds = seeds;
     p.dur_in = dur_in;
-    p.ld = ld;
+    p.ld = (int32_t)ld;
     p.realized = realized;
     p.durations = injected ? nullptr : durations;
     p.cause = cause;
@@ -864,8 +870,8 @@ int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double
             }
         }
     }
-    p.n = n_main;
-    p.n_tiles = (n_main + tile - 1) / tile;
+    p.n = (int32_t)n_main;
+    p.n_tiles = (int32_t)((n_main + tile - 1) / tile);
     HIP_TRY(launch_walk(p, v, (int)std::min<int64_t>(p.n_tiles, resident), lds, stream), "walk kernel launch");
     if (n_tail > 0) {
         WalkParams q = p;
@@ -874,8 +880,8 @@ int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double
         q.realized = realized + n_main;
         q.durations = p.durations ? p.durations + n_main : nullptr;
         q.cause = cause ? cause + n_main : nullptr;
-        q.n = n_tail;
-        q.n_tiles = (n_tail + kLanes - 1) / kLanes;
+        q.n = (int32_t)n_tail;
+        q.n_tiles = (int32_t)((n_tail + kLanes - 1) / kLanes);
         HIP_TRY(launch_walk(q, v1, (int)std::min<int64_t>(q.n_tiles, resident1), lds1, stream), "walk kernel launch");
     }
     return MCDP_OK;

@@ -52,13 +52,17 @@ __device__ __forceinline__ void lds_store(double *lds, int slot, int lane, const
     }
 }
 
+// Element offset of (row, column 0) in an array with leading dimension ld:
+// a 32 x 32 -> 64-bit product (rows and ld are non-negative and < 2^31).
+__device__ __forceinline__ uint64_t row_offset(int32_t row, uint32_t ld) { return (uint64_t)(uint32_t)row * ld; }
+
 template <bool kInjected, int kPaths, int K, int kSpl>
 __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) {
     extern __shared__ double lds[];
     const int lane = threadIdx.x & (kLanes - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int W = p.waves;
-    const int64_t ld = p.ld;
+    const uint32_t ld = (uint32_t)p.ld;
     // The plan never changes during a launch: reading it through the constant
     // address space lets wave-uniform record loads become scalar loads.
     const auto step_off = as_const_space(p.step_off);
@@ -83,15 +87,17 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) 
             draw_durations<kSpl>(r.info, r.a, r.b, (uint32_t)r.i2, seed, as_const_space(p.alias), dists, dur);
     };
 
-    for (int64_t tile = blockIdx.x; tile < p.n_tiles; tile += gridDim.x) {
-        int64_t s[kSpl];
-        bool active[kSpl];
+    for (int32_t tile = blockIdx.x; tile < p.n_tiles; tile += gridDim.x) {
+        // Lanes past the last sample (last tile only) take over sample n - 1:
+        // same seed or injected row, so they compute its values bit for bit and
+        // their stores rewrite its bytes unchanged.  No store needs a lane mask.
+        uint32_t s[kSpl];
         uint32_t seed[kSpl];
 #pragma unroll
         for (int i = 0; i < kSpl; ++i) {
-            s[i] = tile * (kLanes * kSpl) + i * kLanes + lane;
-            active[i] = s[i] < p.n;
-            seed[i] = (!kInjected && active[i]) ? (uint32_t)p.seeds[s[i]] : 0u;
+            const int32_t si = tile * (kLanes * kSpl) + i * kLanes + lane;
+            s[i] = si < p.n ? si : p.n - 1;
+            seed[i] = kInjected ? 0u : (uint32_t)p.seeds[s[i]];
         }
 
         for (int t = 0; t < p.n_steps; ++t) {
@@ -105,7 +111,7 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) 
                 // one-ahead prefetch never leaves the array.
                 Record cur = recs[r0];
                 double best[kSpl], upper = 0.0;
-                int32_t who[kSpl], event = 0, remaining = -1;
+                int32_t who[kSpl], event = 0;
 #pragma unroll
                 for (int i = 0; i < kSpl; ++i) {
                     best[i] = 0.0;
@@ -116,7 +122,6 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) 
                     if (cur.info & kInfoHeader) {
                         upper = cur.b;
                         event = cur.i0;
-                        remaining = cur.i2;
 #pragma unroll
                         for (int i = 0; i < kSpl; ++i) {
                             best[i] = cur.a;
@@ -127,17 +132,17 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) 
 #pragma unroll
                         for (int i = 0; i < kSpl; ++i) dur[i] = 0.0;
                         if (cur.info & kInfoEdgeValid) {
-                            const int64_t row = (int64_t)cur.i2 * ld;
+                            const uint64_t row = row_offset(cur.i2, ld);
                             if (kInjected) {
 #pragma unroll
                                 for (int i = 0; i < kSpl; ++i)
-                                    dur[i] = active[i] ? __builtin_nontemporal_load(p.dur_in + row + s[i]) : 0.0;
+                                    dur[i] = __builtin_nontemporal_load(p.dur_in + row + s[i]);
                             } else {
                                 draw(cur, seed, dur);
                                 if ((cur.info & kInfoOwner) && p.durations) {
 #pragma unroll
                                     for (int i = 0; i < kSpl; ++i)
-                                        if (active[i]) __builtin_nontemporal_store(dur[i], p.durations + row + s[i]);
+                                        __builtin_nontemporal_store(dur[i], p.durations + row + s[i]);
                                 }
                             }
                         }
@@ -145,9 +150,9 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) 
                         if (cur.i1 >= 0) {
                             lds_load<kSpl>(lds, cur.i1, lane, rs);
                         } else if (kPaths == kPathsGeneral && cur.i1 == kSrcGlobal) {
-                            const int64_t row = (int64_t)cur.i0 * ld;
+                            const uint64_t row = row_offset(cur.i0, ld);
 #pragma unroll
-                            for (int i = 0; i < kSpl; ++i) rs[i] = active[i] ? p.realized[row + s[i]] : 0.0;
+                            for (int i = 0; i < kSpl; ++i) rs[i] = p.realized[row + s[i]];
                         } else {
                             const double e = earliest[cur.i0];
 #pragma unroll
@@ -162,21 +167,17 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) 
                                 who[i] = cur.i0;
                             }
                         }
-                        --remaining;
                     }
-                    if (remaining == 0) {  // the event's last predecessor (or a header with none)
-                        const int64_t row = (int64_t)event * ld;
+                    if (cur.info & kInfoLast) {  // the event's last predecessor (or a header with none)
+                        const uint64_t row = row_offset(event, ld);
 #pragma unroll
                         for (int i = 0; i < kSpl; ++i) {
                             const double rr = (upper < best[i]) ? upper : best[i];
-                            if (active[i]) {
-                                __builtin_nontemporal_store(rr, p.realized + row + s[i]);
-                                if (p.cause) __builtin_nontemporal_store(who[i], p.cause + row + s[i]);
-                            }
+                            __builtin_nontemporal_store(rr, p.realized + row + s[i]);
+                            if (p.cause) __builtin_nontemporal_store(who[i], p.cause + row + s[i]);
                             res[j][i] = rr;
                         }
                         ++j;
-                        remaining = -1;
                     }
                     cur = nxt;
                 }
@@ -199,10 +200,10 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) 
                 const Record rec = orph[o];
                 double dur[kSpl];
                 draw(rec, seed, dur);
-                const int64_t row = (int64_t)rec.i2 * ld;
+                const uint64_t row = row_offset(rec.i2, ld);
 #pragma unroll
                 for (int i = 0; i < kSpl; ++i)
-                    if (active[i]) __builtin_nontemporal_store(dur[i], p.durations + row + s[i]);
+                    __builtin_nontemporal_store(dur[i], p.durations + row + s[i]);
             }
         }
     }

@@ -50,9 +50,9 @@ struct WalkParams {
     int32_t slots;            // LDS predecessor slots
     const int32_t *seeds;   // sampled mode: int32[n]
     const double *dur_in;   // injected mode: [L][ld]
-    int64_t n;              // samples
-    int64_t ld;             // leading dimension of every per-sample array
-    int64_t n_tiles;        // ceil(n / (64 spl))
+    int32_t n;              // samples (< 2^31)
+    int32_t ld;             // leading dimension of every per-sample array (< 2^31)
+    int32_t n_tiles;        // ceil(n / (64 spl))
     double *realized;       // [E][ld] (read within the launch when a plan spills slots)
     double *durations;      // [L][ld] or null (sampled mode)
     int32_t *cause;         // [E][ld] or null

@@ -51,8 +51,9 @@ enum : uint32_t {
     kInfoTruncated = 1u << 9,    // gamma family: rejection on max_scale (table p3)
     kInfoEdgeValid = 1u << 10,   // edge in [0, link_count); otherwise duration 0.0
     kInfoOwner = 1u << 11,       // this record writes durations[edge]
-    kInfoDistShift = 12,         // bits 12..30: distribution table index
-    kInfoDistMask = 0x7FFFFu,
+    kInfoLast = 1u << 12,        // the event's last record: its realized time is final
+    kInfoDistShift = 13,         // bits 13..30: distribution table index
+    kInfoDistMask = 0x3FFFFu,
     kInfoHeader = 1u << 31,      // an event header, not a predecessor
 };
 
@@ -60,12 +61,12 @@ enum : uint32_t {
 //   event header (info & kInfoHeader):
 //     a = earliest, b = upper (earliest + max_delay, rounded once on the host),
 //     i0 = event id, i1 = LDS slot receiving its realized time (or -1),
-//     i2 = number of predecessor records that follow;
+//     i2 = number of predecessor records that follow (kInfoLast if none);
 //   predecessor (src, edge) of the event above, in the entry's input order:
 //     a = minimal_duration of the edge, b = p0 of its distribution,
 //     i0 = src event id (the cause_event value), i1 = src LDS slot or
 //     kSrcGlobal / kSrcStale, i2 = edge (activity index: durations row and
-//     RNG counter).
+//     RNG counter); the event's last predecessor carries kInfoLast.
 struct Record {
     double a;
     double b;
