@@ -137,6 +137,38 @@ int mcdp_propagator_configure(mcdp_propagator *prop, int32_t waves, int32_t even
 int mcdp_propagator_autotune(mcdp_propagator *prop, int64_t n_samples, int32_t *waves, int32_t *events_per_wave,
                              int32_t *samples_per_lane, float *ms);
 
+/* ---- Per-event statistics (no counterpart in the reference) -------------- */
+/* The reference returns one SimResult per seed and leaves statistics to its
+ * callers (demo/monte_carlo.py:45-63 counts realized values per event on the
+ * host).  These reduce event-major results on the device instead.  All
+ * results are per event e < node_count():
+ *   mean[e]                   exactly rounded sum of realized[e, :n], / n
+ *   quantiles[e * n_q + j]    numpy.quantile(realized[e, :n], q[j]) (method
+ *                             "linear"; NaN if the row holds a NaN), n_q <= 8,
+ *                             0 <= q[j] <= 1
+ *   cause_counts[off[e] + e]          samples with cause_event -1
+ *   cause_counts[off[e] + e + 1 + k]  samples whose cause is the source of the
+ *                             k-th predecessor of e (for a source listed twice,
+ *                             its first position), off = pred_offsets
+ * Any output pointer may be NULL to skip it. */
+
+/* off[0..node_count()]: start of each event's predecessor list (input order of
+ * its winning precedence entry); cause_counts holds off[E] + E counters. */
+int mcdp_propagator_pred_offsets(const mcdp_propagator *prop, int64_t *off);
+
+/* Statistics of device arrays realized_dev / cause_dev (event-major, leading
+ * dimension ld >= n), asynchronous on `stream`; q is a host array; outputs are
+ * device arrays. */
+int mcdp_event_stats_device(mcdp_propagator *prop, const double *realized_dev, const int32_t *cause_dev, int64_t n,
+                            int64_t ld, int32_t n_q, const double *q, double *mean_dev, double *quantiles_dev,
+                            int64_t *cause_counts_dev, void *stream);
+
+/* run_many over `seeds` on the device followed by the statistics, without
+ * moving per-sample results to the host; outputs are host arrays.
+ * Synchronous. */
+int mcdp_run_stats_host(mcdp_propagator *prop, const int32_t *seeds, int64_t n, int32_t n_q, const double *q,
+                        double *mean, double *quantiles, int64_t *cause_counts);
+
 /* ---- Device plumbing for callers without their own HIP allocator -------- */
 /* (no counterpart in the reference, which has no device) */
 int mcdp_device_count(int *count);

@@ -1,7 +1,8 @@
 """Builds the native parts of mc_dagprop_amd in place.
 
-* ``libmcdp.so``: the gfx950 kernels (``csrc/mcdp_kernels.hip``, hipcc) and the
-  host compiler + C ABI (``csrc/mcdp_host.cpp``), declared in ``include/mcdp.h``.
+* ``libmcdp.so``: the gfx950 kernels (``csrc/mcdp_kernels.hip``, ``csrc/mcdp_stats.hip``; hipcc) and
+  the host compiler + C ABI (``csrc/mcdp_host.cpp``, ``csrc/mcdp_stats.cpp``; g++), declared in
+  ``include/mcdp.h``.
 * ``monte_carlo/_core*.so``: the pybind11 module over that C ABI.
 
 Kernel flags:
@@ -72,15 +73,15 @@ def build(verbose: bool = False) -> None:
     """Compiles libmcdp.so and the _core extension (always rebuilds)."""
     BUILD.mkdir(exist_ok=True)
     hipcc = str(ROCM / "bin" / "hipcc")
-    k_obj, h_obj = BUILD / "mcdp_kernels.o", BUILD / "mcdp_host.o"
-    steps = [
-        [hipcc, *KERNEL_FLAGS, "-c", CSRC / "mcdp_kernels.hip", "-o", k_obj],
-        ["g++", *HOST_FLAGS, "-c", CSRC / "mcdp_host.cpp", "-o", h_obj],
-    ]
-    with ThreadPoolExecutor(max_workers=2) as pool:
+    kernels = ["mcdp_kernels.hip", "mcdp_stats.hip"]
+    hosts = ["mcdp_host.cpp", "mcdp_stats.cpp"]
+    objs = [BUILD / (Path(f).stem + ("_dev.o" if f.endswith(".hip") else "_host.o")) for f in kernels + hosts]
+    steps = [[hipcc, *KERNEL_FLAGS, "-c", CSRC / f, "-o", o] for f, o in zip(kernels, objs)]
+    steps += [["g++", *HOST_FLAGS, "-c", CSRC / f, "-o", o] for f, o in zip(hosts, objs[len(kernels):])]
+    with ThreadPoolExecutor(max_workers=len(steps)) as pool:
         list(pool.map(lambda c: _run(c, verbose), steps))
     tmp_lib = LIB.with_suffix(".so.tmp")
-    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", k_obj, h_obj, "-o", tmp_lib], verbose)
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp_lib], verbose)
     os.replace(tmp_lib, LIB)
     tmp_ext = EXT.with_name(EXT.name + ".tmp")
     _run(

@@ -68,6 +68,9 @@ _SIGNATURES = {
     "mcdp_debug_philox": (C.c_int, [C.c_int, _i64, _pu32, _pu32, _pu32]),
     "mcdp_debug_copy": (C.c_int, [_P, _P, _i64, _P]),
     "mcdp_debug_math": (C.c_int, [C.c_int, _i64, _pf64, _pf64]),
+    "mcdp_propagator_pred_offsets": (C.c_int, [_P, _pi64]),
+    "mcdp_event_stats_device": (C.c_int, [_P, _P, _P, _i64, _i64, _i32, _pf64, _P, _P, _P, _P]),
+    "mcdp_run_stats_host": (C.c_int, [_P, _pi32, _i64, _i32, _pf64, _pf64, _pf64, _pi64]),
 }
 
 _lib = None
@@ -248,6 +251,43 @@ class Propagator:
     def configure(self, waves: int, events_per_wave: int, samples_per_lane: int = 1) -> None:
         """Fixes the schedule shape (results do not depend on it)."""
         check(lib().mcdp_propagator_configure(self.h, waves, events_per_wave, samples_per_lane))
+
+    def pred_offsets(self) -> np.ndarray:
+        """off[0..E]: start of each event's predecessor list; cause counters of event e are
+        counts[off[e] + e : off[e + 1] + e + 1] (cause -1 first, then each predecessor position)."""
+        off = np.zeros(self.E + 1, dtype=np.int64)
+        check(lib().mcdp_propagator_pred_offsets(self.h, _ptr(off, _pi64)))
+        return off
+
+    def run_stats(self, seeds, q=(0.5, 0.9, 0.99)):
+        """run_many on the device, then per-event statistics; returns (mean [E], quantiles [E, len(q)],
+        cause_counts [off[E] + E])."""
+        seeds = np.ascontiguousarray(seeds, dtype=np.int32)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        off = self.pred_offsets()
+        mean = np.empty(self.E)
+        quant = np.empty((self.E, q.size))
+        counts = np.empty(int(off[-1]) + self.E, dtype=np.int64)
+        check(lib().mcdp_run_stats_host(self.h, _ptr(seeds, _pi32), seeds.size, q.size, _ptr(q, _pf64),
+                                        _ptr(mean, _pf64), _ptr(quant, _pf64), _ptr(counts, _pi64)))
+        return mean, quant, counts
+
+    def event_stats_device(self, realized: "DeviceBuffer", cause: "DeviceBuffer", n: int, ld: int, q=(0.5,),
+                           stream=None):
+        """Statistics of event-major device arrays; results copied to the host."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        off = self.pred_offsets()
+        d_mean = DeviceBuffer(8 * max(self.E, 1))
+        d_quant = DeviceBuffer(8 * max(self.E * q.size, 1))
+        d_cnt = DeviceBuffer(8 * max(int(off[-1]) + self.E, 1))
+        check(lib().mcdp_event_stats_device(self.h, realized.ptr, cause.ptr if cause is not None else None, n, ld,
+                                            q.size, _ptr(q, _pf64), d_mean.ptr, d_quant.ptr if q.size else None,
+                                            d_cnt.ptr if cause is not None else None, stream))
+        synchronize(stream)
+        mean = d_mean.to_array(np.float64, (self.E,))
+        quant = d_quant.to_array(np.float64, (self.E, q.size)) if q.size else np.empty((self.E, 0))
+        counts = d_cnt.to_array(np.int64, (int(off[-1]) + self.E,)) if cause is not None else None
+        return mean, quant, counts
 
     def autotune(self, n_samples: int) -> tuple[int, int, int, float]:
         """Times every candidate shape with n_samples samples; keeps and returns

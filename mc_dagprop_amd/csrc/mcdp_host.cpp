@@ -24,37 +24,30 @@
 #include <vector>
 
 #include "../../include/mcdp.h"
+#include "mcdp_internal.hpp"
 #include "mcdp_kernels.hpp"
 #include "mcdp_plan.hpp"
 
 using namespace mcdp;
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
 
+namespace mcdp {
 int fail(int code, const std::string &msg) {
     g_last_error = msg;
     return code;
 }
-
 int hip_fail(hipError_t e, const char *what) {
     return fail(MCDP_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
+}  // namespace mcdp
 
-#define HIP_TRY(expr, what)                          \
-    do {                                             \
-        hipError_t _e = (expr);                      \
-        if (_e != hipSuccess) return hip_fail(_e, what); \
-    } while (0)
+namespace {
 
 // API-level distribution kinds (the reference's DistVar alternatives, :143).
 enum ApiKind { kApiConstant, kApiExponential, kApiGamma, kApiEmpAbs, kApiEmpRel };
-
-// A compiled distribution (kind, flags, parameters): records of edges of its
-// activity type carry kind, flags and p0 inline and its table index.
-using DistRec = DistParams;
-constexpr uint32_t kNoDistIndex = 0xFFFFFFFFu;  // the link's type has no distribution
 
 struct DistSpec {
     ApiKind kind;
@@ -289,58 +282,6 @@ struct mcdp_generator {
     std::map<int32_t, DistSpec> dists;  // last add_* per type wins
 };
 
-struct DeviceCopy {
-    int device = -1;
-    void *blob = nullptr;
-    WalkParams base{};
-};
-
-// One predecessor entry as the reference evaluates it.
-struct PredEntry {
-    int32_t src;
-    int32_t edge;
-    bool stale;  // read before src is evaluated in the reference's order
-};
-
-struct mcdp_propagator {
-    int32_t E = 0, L = 0;
-    double max_delay = 0.0;
-    // topology (schedule-independent)
-    std::vector<double> earliest;              // [E]
-    std::vector<std::vector<int32_t>> levels;  // events by level, in Kahn order
-    std::vector<int64_t> pred_off;             // [E+1] into preds (winning entry, input order)
-    std::vector<PredEntry> preds;
-    std::vector<double> dmin;                  // [L]
-    std::vector<uint32_t> ldist;               // [L] distribution index or kNoDistIndex
-    std::vector<DistRec> dist;
-    std::vector<AliasRec> alias;
-    int64_t n_stale = 0;
-    // schedule (for `waves` waves per workgroup, at most `k` events per wave and step)
-    std::vector<int32_t> step_off, ev_off, slot_list;
-    std::vector<Record> recs, orph;
-    int32_t n_steps = 0, waves = 1, slots = 0, max_per_wave = 0, k = kMaxEventsPerWave;
-    int32_t spl = 1;  // samples per lane of the walk kernel (schedule-independent)
-    int64_t n_global = 0;
-    bool shape_fixed = false;  // set by configure / autotune / the first launch
-    // device copies, one per device used
-    std::mutex mu;
-    std::vector<DeviceCopy> dev;
-
-    void drop_device_copies() {
-        for (auto &d : dev) {
-            if (d.blob) {
-                int cur = -1;
-                (void)hipGetDevice(&cur);
-                (void)hipSetDevice(d.device);
-                (void)hipDeviceSynchronize();  // launches in flight on any stream may still read the plan
-                (void)hipFree(d.blob);
-                if (cur >= 0) (void)hipSetDevice(cur);
-            }
-        }
-        dev.clear();
-    }
-    ~mcdp_propagator() { drop_device_copies(); }
-};
 
 namespace {
 
@@ -657,7 +598,7 @@ int compile_propagator(mcdp_propagator &P, int32_t n_events, const double *earli
 }
 
 // Uploads the plan of `P` to the current device (once per device).
-int ensure_device(mcdp_propagator &P, DeviceCopy *&out) {
+int ensure_device_impl(mcdp_propagator &P, DeviceCopy *&out) {
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
         return fail(MCDP_E_DEVICE, "no HIP device available: the Monte Carlo engine runs on the GPU only");
@@ -828,7 +769,7 @@ int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double
         P.shape_fixed = true;
     }
     DeviceCopy *d = nullptr;
-    int rc = ensure_device(P, d);
+    int rc = ensure_device_impl(P, d);
     if (rc) return rc;
     if (n == 0 || (P.E == 0 && (injected || P.orph.empty()))) return MCDP_OK;
     WalkParams p = d->base;
@@ -958,6 +899,8 @@ int run_host(mcdp_propagator &P, bool injected, const int32_t *seeds, const doub
 }
 
 }  // namespace
+
+int mcdp::ensure_device(mcdp_propagator &P, DeviceCopy *&out) { return ensure_device_impl(P, out); }
 
 // ============================================================================
 // C ABI

@@ -1,0 +1,99 @@
+// Host-side internals of libmcdp shared by its translation units: the
+// propagator object behind the C handle, its per-device copies, and the error
+// helpers that set mcdp_last_error().
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mcdp.h"
+#include "mcdp_kernels.hpp"
+#include "mcdp_plan.hpp"
+
+namespace mcdp {
+
+// Sets the calling thread's last error message; returns `code`.
+int fail(int code, const std::string &msg);
+// fail(MCDP_E_DEVICE, "<what>: <HIP error string>").
+int hip_fail(hipError_t e, const char *what);
+
+#define HIP_TRY(expr, what)                                        \
+    do {                                                           \
+        hipError_t _e = (expr);                                    \
+        if (_e != hipSuccess) return ::mcdp::hip_fail(_e, what);   \
+    } while (0)
+
+// A compiled distribution (kind, flags, parameters): records of edges of its
+// activity type carry kind, flags and p0 inline and its table index.
+using DistRec = DistParams;
+constexpr uint32_t kNoDistIndex = 0xFFFFFFFFu;  // the link's type has no distribution
+
+// The plan (and, once statistics are asked for, the predecessor table) on one
+// device.
+struct DeviceCopy {
+    int device = -1;
+    void *blob = nullptr;
+    WalkParams base{};
+    void *topo = nullptr;                // statistics: predecessor table, uploaded on first use
+    const int32_t *pred_off = nullptr;   // [E + 1]
+    const int32_t *pred_key = nullptr;   // [P] per event: its distinct sources, ascending
+    const int32_t *pred_pos = nullptr;   // [P] first position of pred_key in the event's list
+    const int32_t *pred_nkey = nullptr;  // [E] distinct sources per event
+};
+
+// One predecessor entry as the reference evaluates it.
+struct PredEntry {
+    int32_t src;
+    int32_t edge;
+    bool stale;  // read before src is evaluated in the reference's order
+};
+
+}  // namespace mcdp
+
+struct mcdp_propagator {
+    int32_t E = 0, L = 0;
+    double max_delay = 0.0;
+    // topology (schedule-independent)
+    std::vector<double> earliest;              // [E]
+    std::vector<std::vector<int32_t>> levels;  // events by level, in Kahn order
+    std::vector<int64_t> pred_off;             // [E+1] into preds (winning entry, input order)
+    std::vector<mcdp::PredEntry> preds;
+    std::vector<double> dmin;                  // [L]
+    std::vector<uint32_t> ldist;               // [L] distribution index or kNoDistIndex
+    std::vector<mcdp::DistRec> dist;
+    std::vector<mcdp::AliasRec> alias;
+    int64_t n_stale = 0;
+    // schedule (for `waves` waves per workgroup, at most `k` events per wave and step)
+    std::vector<int32_t> step_off, ev_off, slot_list;
+    std::vector<mcdp::Record> recs, orph;
+    int32_t n_steps = 0, waves = 1, slots = 0, max_per_wave = 0, k = mcdp::kMaxEventsPerWave;
+    int32_t spl = 1;  // samples per lane of the walk kernel (schedule-independent)
+    int64_t n_global = 0;
+    bool shape_fixed = false;  // set by configure / autotune / the first launch
+    // device copies, one per device used
+    std::mutex mu;
+    std::vector<mcdp::DeviceCopy> dev;
+
+    void drop_device_copies() {
+        for (auto &d : dev) {
+            int cur = -1;
+            (void)hipGetDevice(&cur);
+            (void)hipSetDevice(d.device);
+            (void)hipDeviceSynchronize();  // launches in flight on any stream may still read the plan
+            if (d.blob) (void)hipFree(d.blob);
+            if (d.topo) (void)hipFree(d.topo);
+            if (cur >= 0) (void)hipSetDevice(cur);
+        }
+        dev.clear();
+    }
+    ~mcdp_propagator() { drop_device_copies(); }
+};
+
+namespace mcdp {
+// The propagator's plan on the current device (uploaded on first use).  The
+// caller holds P.mu.
+int ensure_device(mcdp_propagator &P, DeviceCopy *&out);
+}  // namespace mcdp

@@ -1,0 +1,201 @@
+// Per-event statistics of Monte Carlo results (C ABI in include/mcdp.h):
+// the consumer side of run_many that the reference leaves to its callers
+// (demo/monte_carlo.py:45-63 gathers every SimResult on the host and counts
+// per event).  Here the statistics are reduced on the device, next to the
+// event-major outputs, and only E-sized results leave it.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/mcdp.h"
+#include "mcdp_internal.hpp"
+#include "mcdp_stats.hpp"
+
+using namespace mcdp;
+
+namespace {
+
+// The predecessor table the cause counts need, on the device of `d`: per
+// event, its distinct sources in ascending order with the first position of
+// each in the event's predecessor list (a binary search maps a cause to its
+// position).
+int ensure_topo(const mcdp_propagator &P, DeviceCopy &d) {
+    if (d.topo) return MCDP_OK;
+    const int64_t np = (int64_t)P.preds.size();
+    if (np > std::numeric_limits<int32_t>::max() - P.E - 1)
+        return fail(MCDP_E_INVALID, "too many predecessor entries for cause counts");
+    std::vector<int32_t> off((size_t)P.E + 1), key((size_t)np), pos((size_t)np), nkey((size_t)P.E);
+    std::vector<std::pair<int32_t, int32_t>> tmp;
+    for (int32_t e = 0; e < P.E; ++e) {
+        const int64_t a = P.pred_off[(size_t)e], b = P.pred_off[(size_t)e + 1];
+        off[(size_t)e] = (int32_t)a;
+        tmp.clear();
+        for (int64_t k = a; k < b; ++k) tmp.emplace_back(P.preds[(size_t)k].src, (int32_t)(k - a));
+        std::stable_sort(tmp.begin(), tmp.end(),
+                         [](const std::pair<int32_t, int32_t> &x, const std::pair<int32_t, int32_t> &y) {
+                             return x.first < y.first;
+                         });
+        int32_t m = 0;
+        for (size_t i = 0; i < tmp.size(); ++i) {
+            if (i > 0 && tmp[i].first == tmp[i - 1].first) continue;  // keep the first position
+            key[(size_t)a + m] = tmp[i].first;
+            pos[(size_t)a + m] = tmp[i].second;
+            ++m;
+        }
+        nkey[(size_t)e] = m;
+    }
+    off[(size_t)P.E] = (int32_t)np;
+    const size_t bytes = sizeof(int32_t) * (off.size() + key.size() + pos.size() + nkey.size());
+    void *blob = nullptr;
+    hipError_t e = hipMalloc(&blob, std::max<size_t>(bytes, 4));
+    if (e != hipSuccess) return fail(MCDP_E_NOMEM, std::string("hipMalloc (predecessor table): ") + hipGetErrorString(e));
+    auto *p = static_cast<int32_t *>(blob);
+    int32_t *d_off = p, *d_key = d_off + off.size(), *d_pos = d_key + key.size(), *d_nkey = d_pos + pos.size();
+    e = hipMemcpy(d_off, off.data(), sizeof(int32_t) * off.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && np) e = hipMemcpy(d_key, key.data(), sizeof(int32_t) * key.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && np) e = hipMemcpy(d_pos, pos.data(), sizeof(int32_t) * pos.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && P.E) e = hipMemcpy(d_nkey, nkey.data(), sizeof(int32_t) * nkey.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(blob);
+        return hip_fail(e, "upload predecessor table");
+    }
+    d.topo = blob;
+    d.pred_off = d_off;
+    d.pred_key = d_key;
+    d.pred_pos = d_pos;
+    d.pred_nkey = d_nkey;
+    return MCDP_OK;
+}
+
+// numpy.quantile(..., method="linear") index arithmetic for n values
+// (numpy/lib/_function_base_impl.py: _QuantileMethods["linear"], _get_indexes,
+// _get_gamma), in double as numpy evaluates it.
+int quantile_args(int64_t n, int32_t n_q, const double *q, QuantileArgs &qa) {
+    if (n_q < 0 || n_q > kMaxQuantiles) return fail(MCDP_E_INVALID, "between 0 and 8 quantiles per call");
+    if (n_q > 0 && !q) return fail(MCDP_E_INVALID, "quantiles is null");
+    qa = QuantileArgs{};
+    qa.n_q = n_q;
+    auto rank_index = [&](int64_t r) {
+        for (int i = 0; i < qa.n_ranks; ++i)
+            if (qa.rank[i] == r) return i;
+        qa.rank[qa.n_ranks] = (int32_t)r;
+        return qa.n_ranks++;
+    };
+    for (int j = 0; j < n_q; ++j) {
+        if (!(q[j] >= 0.0 && q[j] <= 1.0)) return fail(MCDP_E_INVALID, "quantiles must be in the range [0, 1]");
+        const double v = (double)(n - 1) * q[j];
+        double prev_index;  // numpy's previous_indexes (-1: the last value)
+        int64_t lo, hi;
+        if (v >= (double)(n - 1)) {
+            prev_index = -1.0;
+            lo = hi = n - 1;
+        } else if (v < 0.0) {
+            prev_index = 0.0;
+            lo = hi = 0;
+        } else {
+            prev_index = std::floor(v);
+            lo = (int64_t)prev_index;
+            hi = lo + 1;
+        }
+        qa.lo[j] = rank_index(lo);
+        qa.hi[j] = rank_index(hi);
+        qa.gamma[j] = v - prev_index;
+    }
+    return MCDP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mcdp_propagator_pred_offsets(const mcdp_propagator *prop, int64_t *out) {
+    if (!prop || !out) return fail(MCDP_E_INVALID, "null argument");
+    std::copy(prop->pred_off.begin(), prop->pred_off.end(), out);
+    return MCDP_OK;
+}
+
+int mcdp_event_stats_device(mcdp_propagator *prop, const double *realized_dev, const int32_t *cause_dev, int64_t n,
+                            int64_t ld, int32_t n_q, const double *q, double *mean_dev, double *quantiles_dev,
+                            int64_t *cause_counts_dev, void *stream) {
+    if (!prop) return fail(MCDP_E_INVALID, "propagator is null");
+    if (n < 1) return fail(MCDP_E_INVALID, "statistics need at least one sample");
+    if (ld < n) return fail(MCDP_E_INVALID, "ld must be >= n");
+    if (n > std::numeric_limits<int32_t>::max() || ld > std::numeric_limits<int32_t>::max())
+        return fail(MCDP_E_INVALID, "n and ld must be below 2^31");
+    if ((mean_dev || quantiles_dev) && !realized_dev) return fail(MCDP_E_INVALID, "realized is null");
+    if (cause_counts_dev && !cause_dev) return fail(MCDP_E_INVALID, "cause is null");
+    QuantileArgs qa;
+    int rc = quantile_args(n, quantiles_dev ? n_q : 0, q, qa);
+    if (rc) return rc;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return fail(MCDP_E_DEVICE, "no HIP device available: statistics run on the GPU only");
+    std::lock_guard<std::mutex> lock(prop->mu);
+    const auto s = static_cast<hipStream_t>(stream);
+    const int32_t E = prop->E;
+    if (mean_dev) HIP_TRY(launch_event_mean(realized_dev, (int32_t)n, (int32_t)ld, E, mean_dev, s), "mean kernel");
+    if (quantiles_dev && qa.n_q > 0)
+        HIP_TRY(launch_event_quantiles(realized_dev, (int32_t)n, (int32_t)ld, E, qa, quantiles_dev, s),
+                "quantile kernel");
+    if (cause_counts_dev) {
+        DeviceCopy *d = nullptr;
+        rc = ensure_device(*prop, d);
+        if (rc) return rc;
+        rc = ensure_topo(*prop, *d);
+        if (rc) return rc;
+        HIP_TRY(launch_cause_counts(cause_dev, (int32_t)n, (int32_t)ld, E, d->pred_off, d->pred_key, d->pred_pos,
+                                    d->pred_nkey, cause_counts_dev, s),
+                "cause count kernel");
+    }
+    return MCDP_OK;
+}
+
+int mcdp_run_stats_host(mcdp_propagator *prop, const int32_t *seeds, int64_t n, int32_t n_q, const double *q,
+                        double *mean, double *quantiles, int64_t *cause_counts) {
+    if (!prop) return fail(MCDP_E_INVALID, "propagator is null");
+    if (n < 1 || !seeds) return fail(MCDP_E_INVALID, "statistics need at least one seed");
+    const int64_t E = prop->E, P = (int64_t)prop->preds.size();
+    if (E == 0) return MCDP_OK;
+    QuantileArgs qa;
+    int rc = quantile_args(n, quantiles ? n_q : 0, q, qa);
+    if (rc) return rc;
+    const size_t o_seeds = 0, o_real = 256 * (((size_t)n * 4 + 255) / 256);
+    const size_t o_cause = o_real + (size_t)(E * n) * 8;
+    const size_t o_mean = o_cause + 256 * (((size_t)(E * n) * 4 + 255) / 256);
+    const size_t o_q = o_mean + (size_t)E * 8;
+    const size_t o_cnt = o_q + (size_t)E * (size_t)std::max(n_q, 0) * 8;
+    const size_t bytes = o_cnt + (size_t)(E + P) * 8;
+    void *buf = nullptr;
+    hipError_t e = hipMalloc(&buf, bytes);
+    if (e != hipSuccess) return fail(MCDP_E_NOMEM, std::string("hipMalloc (statistics): ") + hipGetErrorString(e));
+    auto *b = static_cast<unsigned char *>(buf);
+    auto *d_seeds = reinterpret_cast<int32_t *>(b + o_seeds);
+    auto *d_real = reinterpret_cast<double *>(b + o_real);
+    auto *d_cause = reinterpret_cast<int32_t *>(b + o_cause);
+    auto *d_mean = reinterpret_cast<double *>(b + o_mean);
+    auto *d_q = reinterpret_cast<double *>(b + o_q);
+    auto *d_cnt = reinterpret_cast<int64_t *>(b + o_cnt);
+    e = hipMemcpy(d_seeds, seeds, (size_t)n * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy");
+    if (!rc) rc = mcdp_run_many_device(prop, d_seeds, n, n, d_real, nullptr, cause_counts ? d_cause : nullptr, nullptr);
+    if (!rc)
+        rc = mcdp_event_stats_device(prop, d_real, d_cause, n, n, n_q, q, mean ? d_mean : nullptr,
+                                     quantiles ? d_q : nullptr, cause_counts ? d_cnt : nullptr, nullptr);
+    if (!rc && (e = hipDeviceSynchronize()) != hipSuccess) rc = hip_fail(e, "statistics");
+    if (!rc && mean && (e = hipMemcpy(mean, d_mean, (size_t)E * 8, hipMemcpyDeviceToHost)) != hipSuccess)
+        rc = hip_fail(e, "hipMemcpy");
+    if (!rc && quantiles && n_q > 0 &&
+        (e = hipMemcpy(quantiles, d_q, (size_t)E * (size_t)n_q * 8, hipMemcpyDeviceToHost)) != hipSuccess)
+        rc = hip_fail(e, "hipMemcpy");
+    if (!rc && cause_counts &&
+        (e = hipMemcpy(cause_counts, d_cnt, (size_t)(E + P) * 8, hipMemcpyDeviceToHost)) != hipSuccess)
+        rc = hip_fail(e, "hipMemcpy");
+    (void)hipFree(buf);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,33 @@
+// Launch interface of the per-event statistics kernels (mcdp_stats.hip).
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+namespace mcdp {
+
+constexpr int kMaxQuantiles = 8;
+constexpr int kMaxRanks = 2 * kMaxQuantiles;
+// Events with more predecessors than this count their causes with global
+// atomics instead of LDS counters.
+constexpr int kCauseLdsCounters = 8192;
+
+// Order statistics the quantiles need and how each quantile combines two of
+// them (numpy's "linear" method; computed on the host in double).
+struct QuantileArgs {
+    int32_t n_q;
+    int32_t n_ranks;
+    int32_t rank[kMaxRanks];   // distinct 0-based ranks to select
+    int32_t lo[kMaxQuantiles]; // index into rank[] of floor((n - 1) q), clamped
+    int32_t hi[kMaxQuantiles]; // index into rank[] of the next order statistic
+    double gamma[kMaxQuantiles];
+};
+
+hipError_t launch_event_mean(const double *realized, int32_t n, int32_t ld, int32_t E, double *mean,
+                             hipStream_t stream);
+hipError_t launch_event_quantiles(const double *realized, int32_t n, int32_t ld, int32_t E, const QuantileArgs &qa,
+                                  double *out, hipStream_t stream);
+hipError_t launch_cause_counts(const int32_t *cause, int32_t n, int32_t ld, int32_t E, const int32_t *pred_off,
+                               const int32_t *pred_key, const int32_t *pred_pos, const int32_t *pred_nkey,
+                               int64_t *counts, hipStream_t stream);
+
+}  // namespace mcdp

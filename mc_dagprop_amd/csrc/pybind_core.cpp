@@ -386,6 +386,39 @@ PYBIND11_MODULE(_core, m) {
             },
             "Facts about the compiled device plan")
         .def(
+            "run_stats",
+            [](Propagator &p, const std::vector<int> &seeds, const std::vector<double> &q) {
+                const int64_t E = p.node_count();
+                std::vector<int64_t> off((size_t)E + 1);
+                check(mcdp_propagator_pred_offsets(p.p, off.data()));
+                py::array_t<double> mean(E);
+                py::array_t<double> quant(std::vector<py::ssize_t>{(py::ssize_t)E, (py::ssize_t)q.size()});
+                py::array_t<int64_t> counts(off[(size_t)E] + E);
+                py::array_t<int64_t> coff(E + 1);
+                auto co = coff.mutable_unchecked<1>();
+                for (int64_t e = 0; e <= E; ++e) co(e) = off[(size_t)e] + e;
+                std::vector<int32_t> s(seeds.begin(), seeds.end());
+                double *pm = mean.mutable_data(), *pq = quant.mutable_data();
+                int64_t *pc = counts.mutable_data();
+                int rc;
+                {
+                    py::gil_scoped_release nogil;
+                    rc = mcdp_run_stats_host(p.p, s.data(), (int64_t)s.size(), (int32_t)q.size(), q.data(), pm, pq, pc);
+                }
+                check(rc);
+                py::dict d;
+                d["mean"] = mean;
+                d["quantiles"] = quant;
+                d["cause_counts"] = counts;
+                d["cause_offsets"] = coff;
+                return d;
+            },
+            py::arg("seeds"), py::arg("quantiles") = std::vector<double>{0.5, 0.9, 0.99},
+            "run_many on the GPU reduced to per-event statistics: mean (n,), quantiles (n, len(quantiles)) "
+            "as numpy.quantile's default method, and cause_counts, where event e's counters "
+            "cause_counts[cause_offsets[e]:cause_offsets[e+1]] count cause -1 and then each predecessor "
+            "position")
+        .def(
             "configure",
             [](Propagator &p, int waves, int events_per_wave, int samples_per_lane) {
                 check(mcdp_propagator_configure(p.p, waves, events_per_wave, samples_per_lane));

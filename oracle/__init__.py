@@ -280,3 +280,61 @@ def pmf_empirical(weights) -> np.ndarray:
     if w.size < 2:
         return np.ones(max(w.size, 1))
     return w / w.sum()
+
+
+# ---- per-event statistics ---------------------------------------------------------
+def pred_sources(dag) -> list[list[int]]:
+    """Each event's predecessor sources in input order, from the last precedence entry naming it
+    (_core.cpp:232-285: a target's last entry supplies its list)."""
+    preds: list[list[int]] = [[] for _ in range(dag.n_events)]
+    for i, t in enumerate(dag.prec_target):
+        a, b = int(dag.prec_off[i]), int(dag.prec_off[i + 1])
+        preds[int(t)] = [int(x) for x in dag.prec_src[a:b]]
+    return preds
+
+
+def event_stats(realized: np.ndarray, cause: np.ndarray | None, preds: list[list[int]], q) -> tuple:
+    """Per-event statistics of sample-major results realized[s, e] / cause[s, e], as a consumer of
+    run_many computes them on the host (demo/monte_carlo.py:45-63 gathers realized per event):
+
+    * mean[e] = math.fsum(realized[:, e]) / n, the exactly rounded sum; rows whose float sum is not
+      finite (inf, NaN) use that sum;
+    * quantiles[e, j] = numpy.quantile(realized[:, e], q[j]) (method "linear");
+    * counts: per event, [#cause == -1, #cause == source of predecessor 0, ...], a source listed twice
+      counted at its first position.
+    """
+    import math
+
+    realized = np.asarray(realized, dtype=np.float64)
+    n, E = realized.shape
+    mean = np.empty(E)
+    for e in range(E):
+        col = realized[:, e]
+        plain = float(np.sum(col))
+        if math.isfinite(plain):
+            try:
+                mean[e] = math.fsum(col.tolist()) / n
+            except OverflowError:
+                mean[e] = plain / n
+        else:
+            mean[e] = plain / n
+    q = np.asarray(q, dtype=np.float64)
+    quant = np.quantile(realized, q, axis=0).T.copy() if q.size else np.empty((E, 0))
+    counts = None
+    if cause is not None:
+        cause = np.asarray(cause)
+        out = []
+        for e in range(E):
+            first = {}
+            for k, src in enumerate(preds[e]):
+                first.setdefault(src, k)
+            c = np.zeros(1 + len(preds[e]), dtype=np.int64)
+            vals, cnt = np.unique(cause[:, e], return_counts=True)
+            for v, m in zip(vals.tolist(), cnt.tolist()):
+                if v == -1:
+                    c[0] += m
+                elif v in first:
+                    c[1 + first[v]] += m
+            out.append(c)
+        counts = np.concatenate(out) if out else np.zeros(0, dtype=np.int64)
+    return mean, quant, counts

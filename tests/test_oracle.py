@@ -244,3 +244,22 @@ def test_closed_form_cdfs() -> None:
     m, v, sk, ku = oracle.gamma_moments(4.0, 0.5)
     assert (m, v) == (2.0, 1.0) and sk == 1.0 and ku == 1.5
     np.testing.assert_allclose(oracle.pmf_empirical([1, 3]), [0.25, 0.75])
+
+
+# ---- per-event statistics oracle -----------------------------------------------------
+def test_event_stats_oracle_known_values() -> None:
+    """Hand-computed statistics: mean is the exactly rounded sum / n, quantiles follow the linear
+    (Hyndman-Fan 7) definition, causes are counted at a source's first predecessor position."""
+    realized = np.array([[1.0, 10.0], [2.0, 10.0], [3.0, 0.1], [4.0, 0.2]])
+    cause = np.array([[-1, 0], [-1, 0], [-1, -1], [-1, 0]], dtype=np.int32)
+    preds = [[], [0, 0]]  # event 1 lists source 0 twice
+    mean, quant, counts = oracle.event_stats(realized, cause, preds, [0.0, 0.5, 0.9, 1.0])
+    assert mean[0] == 2.5
+    assert mean[1] == (10.0 + 10.0 + 0.1 + 0.2) / 4  # 20.3 / 4, exact rounding of the sum
+    # sorted event 0: 1 2 3 4 -> q=0.5: index 1.5 -> 2.5; q=0.9: index 2.7 -> 3 + 0.7 = 3.7
+    np.testing.assert_allclose(quant[0], [1.0, 2.5, 3.7, 4.0], rtol=0, atol=1e-15)
+    np.testing.assert_array_equal(counts, [4, 1, 3, 0])  # event 0: [-1]; event 1: [-1, pos 0, pos 1]
+    # the exactly rounded mean differs from naive summation where that loses bits
+    big = np.array([[1e16], [1.0], [-1e16], [1.0]])
+    m, _, _ = oracle.event_stats(big, None, [[]], [])
+    assert m[0] == 0.5 and float(np.sum(big[:, 0])) / 4 != 0.5
