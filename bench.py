@@ -200,6 +200,33 @@ def main() -> int:
             print("device results differ from the oracle", file=sys.stderr)
             return 3
 
+    # Per-event statistics of the same results on the device (mean, three quantiles, cause counts),
+    # reported beside the metric: reads realized (8 E S bytes) and cause (4 E S bytes).
+    stats = None
+    if args.mode == "run_many":
+        qs = np.array([0.5, 0.9, 0.99])
+        off = prop.pred_offsets()
+        d_mean = capi.DeviceBuffer(8 * E)
+        d_q = capi.DeviceBuffer(8 * E * qs.size)
+        d_cnt = capi.DeviceBuffer(8 * (int(off[-1]) + E))
+
+        def stats_call() -> None:
+            capi.check(capi.lib().mcdp_event_stats_device(prop.h, realized.ptr, cause.ptr, S, S, qs.size,
+                                                          qs.ctypes.data_as(capi._pf64), d_mean.ptr, d_q.ptr,
+                                                          d_cnt.ptr, stream))
+
+        stats_call()
+        capi.synchronize(stream)
+        tm = capi.Timer()
+        tm.start(stream)
+        stats_call()
+        tm.stop(stream)
+        capi.synchronize(stream)
+        sms = tm.elapsed_ms()
+        sbytes = 12.0 * E * S
+        stats = {"ms": sms, "quantiles": qs.tolist(), "bytes_read": sbytes, "GB/s": sbytes / sms / 1e6}
+        log(f"event statistics: {sms:.3f} ms ({sbytes / sms / 1e6:.0f} GB/s of realized + cause)")
+
     value = L * S / (ms_per_step / 1000.0)
     alg_bytes = float(S) * (8 * L + 12 * E)
     achieved = alg_bytes / (avg_kernel_ms / 1000.0) / 1e9
@@ -251,6 +278,7 @@ def main() -> int:
         },
         "cpu_baseline": baseline,
         "speedup_vs_cpu_baseline": (value / baseline["value"]) if baseline else None,
+        "event_stats": stats,
         "check": check,
     }
     print(json.dumps(out), flush=True)

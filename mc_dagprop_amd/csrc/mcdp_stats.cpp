@@ -137,10 +137,11 @@ int mcdp_event_stats_device(mcdp_propagator *prop, const double *realized_dev, c
     std::lock_guard<std::mutex> lock(prop->mu);
     const auto s = static_cast<hipStream_t>(stream);
     const int32_t E = prop->E;
-    if (mean_dev) HIP_TRY(launch_event_mean(realized_dev, (int32_t)n, (int32_t)ld, E, mean_dev, s), "mean kernel");
     if (quantiles_dev && qa.n_q > 0)
-        HIP_TRY(launch_event_quantiles(realized_dev, (int32_t)n, (int32_t)ld, E, qa, quantiles_dev, s),
+        HIP_TRY(launch_event_quantiles(realized_dev, (int32_t)n, (int32_t)ld, E, qa, quantiles_dev, mean_dev, s),
                 "quantile kernel");
+    else if (mean_dev)
+        HIP_TRY(launch_event_mean(realized_dev, (int32_t)n, (int32_t)ld, E, mean_dev, s), "mean kernel");
     if (cause_counts_dev) {
         DeviceCopy *d = nullptr;
         rc = ensure_device(*prop, d);

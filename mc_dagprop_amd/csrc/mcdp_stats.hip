@@ -4,8 +4,10 @@
 //   * mean: the sum of the row, exactly rounded (double-double accumulation,
 //     a fixed reduction order), divided by n;
 //   * quantiles: numpy's default method ("linear", Hyndman & Fan 7) from
-//     exact order statistics found by radix selection on order-preserving
-//     64-bit keys (8 passes of 8 bits; the row stays in L2 between passes);
+//     exact order statistics: a first pass finds the row's extremes and their
+//     multiplicities (ranks inside those masses are done), then radix
+//     selection on order-preserving 64-bit keys over the key bytes in which
+//     the extremes differ (at most 8 passes of 8 bits);
 //   * cause counts: how many samples have cause -1 and how many have each
 //     distinct predecessor source, counted in LDS.
 #include <hip/hip_runtime.h>
@@ -100,76 +102,180 @@ __device__ double lerp_numpy(double a, double b, double t) {
     return (t >= 0.5) ? b - diff * (1.0 - t) : a + diff * t;
 }
 
+// Adds this wave's matching lanes to hist[digit]: lanes that share the first
+// matching lane's digit (the common case for rows with many equal values, e.g.
+// saturated at a cap) in one atomic, the others one atomic each.
+__device__ __forceinline__ void wave_hist_add(uint32_t *hist, bool m, uint32_t digit, int lane) {
+    const uint64_t mm = __ballot(m);
+    if (mm == 0) return;
+    const int leader = __ffsll((long long)mm) - 1;
+    const uint32_t d0 = __shfl(digit, leader);
+    const uint64_t same = __ballot(m && digit == d0);
+    if (lane == leader) atomicAdd(&hist[d0], (uint32_t)__popcll(same));
+    if (m && digit != d0) atomicAdd(&hist[digit], 1u);
+}
+
+// Extremes of a row for the quantile kernel: smallest and largest key with
+// their multiplicities, and the NaN count.
+struct Extremes {
+    uint64_t mn, mx;
+    uint32_t cmn, cmx, nans;
+};
+
+__device__ __forceinline__ Extremes merge_extremes(Extremes a, const Extremes &b) {
+    if (b.mn < a.mn) {
+        a.mn = b.mn;
+        a.cmn = b.cmn;
+    } else if (b.mn == a.mn) {
+        a.cmn += b.cmn;
+    }
+    if (b.mx > a.mx) {
+        a.mx = b.mx;
+        a.cmx = b.cmx;
+    } else if (b.mx == a.mx) {
+        a.cmx += b.cmx;
+    }
+    a.nans += b.nans;
+    return a;
+}
+
+// Quantiles of each row; with `mean` non-null, its first pass also sums the
+// row (the mean without a separate read).
 __global__ void __launch_bounds__(kBlock) quantile_kernel(const double *__restrict__ realized, int32_t n, uint32_t ld,
-                                                          int32_t E, QuantileArgs qa, double *__restrict__ out) {
+                                                          int32_t E, QuantileArgs qa, double *__restrict__ out,
+                                                          double *__restrict__ mean) {
     __shared__ uint32_t hist[kMaxRanks][256];
-    __shared__ uint64_t prefix[kMaxRanks];
-    __shared__ int32_t krem[kMaxRanks];
-    __shared__ uint32_t nans;
+    __shared__ Extremes ext[kBlock];
+    __shared__ DD part[kBlock];
+    __shared__ double naive[kBlock];
+    __shared__ uint64_t prefix[kMaxRanks];  // per rank: its key's digits found so far
+    __shared__ int32_t krem[kMaxRanks];     // per rank: its rank among keys with that prefix
+    __shared__ int32_t done[kMaxRanks];     // per rank: key known
+    __shared__ uint64_t gprefix[kMaxRanks]; // ranks whose prefixes agree share one histogram
+    __shared__ int32_t group_of[kMaxRanks];
+    __shared__ int32_t n_groups;
+    __shared__ int32_t first_pass;
     const int R = qa.n_ranks;
+    const int lane = threadIdx.x & 63;
     for (int32_t e = blockIdx.x; e < E; e += gridDim.x) {
         const double *row = realized + (uint64_t)(uint32_t)e * ld;
-        if ((int)threadIdx.x < R) {
-            prefix[threadIdx.x] = 0;
-            krem[threadIdx.x] = qa.rank[threadIdx.x];
+        // Pass 0: extremes.  Ranks inside the mass of the smallest or the
+        // largest value (a row saturated at its cap puts its high quantiles
+        // there) are known at once; the others need only the key bytes in
+        // which the smallest and largest keys differ.
+        Extremes x{~0ull, 0ull, 0u, 0u, 0u};
+        DD acc{0.0, 0.0};
+        double plain = 0.0;
+        for (int32_t s = threadIdx.x; s < n; s += kBlock) {
+            const double v = row[s];
+            if (mean) {
+                acc = dd_add(acc, v);
+                plain += v;
+            }
+            if (v != v) {
+                ++x.nans;
+                continue;
+            }
+            const uint64_t key = order_key(v);
+            x = merge_extremes(x, Extremes{key, key, 1u, 1u, 0u});
         }
-        if (threadIdx.x == 0) nans = 0;
-        for (int pass = 0; pass < 8; ++pass) {
-            const int shift = 56 - 8 * pass;
-            for (int i = threadIdx.x; i < R * 256; i += kBlock) hist[i >> 8][i & 255] = 0;
+        ext[threadIdx.x] = x;
+        part[threadIdx.x] = acc;
+        naive[threadIdx.x] = plain;
+        __syncthreads();
+        for (int w = kBlock / 2; w > 0; w >>= 1) {  // fixed pairing: deterministic
+            if ((int)threadIdx.x < w) {
+                ext[threadIdx.x] = merge_extremes(ext[threadIdx.x], ext[threadIdx.x + w]);
+                part[threadIdx.x] = dd_add(part[threadIdx.x], part[threadIdx.x + w]);
+                naive[threadIdx.x] += naive[threadIdx.x + w];
+            }
             __syncthreads();
-            // digits of each rank's key found so far (above this pass's digit)
+        }
+        const Extremes X = ext[0];
+        if (mean && threadIdx.x == 0)
+            mean[e] = (__builtin_isfinite(naive[0]) ? part[0].hi : naive[0]) / (double)n;
+        const int32_t valid = n - (int32_t)X.nans;
+        if ((int)threadIdx.x < R) {
+            const int r = threadIdx.x;
+            const int32_t k = qa.rank[r];
+            done[r] = 1;
+            if (X.nans || k >= valid - (int32_t)X.cmx) {
+                prefix[r] = X.mx;
+            } else if (k < (int32_t)X.cmn) {
+                prefix[r] = X.mn;
+            } else {
+                done[r] = 0;
+                prefix[r] = 0;
+                krem[r] = k;
+            }
+        }
+        if (threadIdx.x == 0) {
+            // whole key bytes shared by every key of the row (mn < mx here if
+            // any rank is left)
+            const uint64_t diff = X.mn ^ X.mx;
+            first_pass = diff ? __builtin_clzll(diff) / 8 : 8;
+        }
+        __syncthreads();
+        const int p0 = first_pass;
+        if ((int)threadIdx.x < R && !done[threadIdx.x] && p0 > 0)
+            prefix[threadIdx.x] = X.mn & ~(~0ull >> (8 * p0));
+        for (int pass = p0; pass < 8; ++pass) {
+            const int shift = 56 - 8 * pass;
+            __syncthreads();
+            if (threadIdx.x == 0) {  // distinct prefixes among the open ranks
+                int g = 0;
+                for (int r = 0; r < R; ++r) {
+                    group_of[r] = -1;
+                    if (done[r]) continue;
+                    int found = -1;
+                    for (int h = 0; h < g; ++h)
+                        if (gprefix[h] == prefix[r]) found = h;
+                    if (found < 0) {
+                        gprefix[g] = prefix[r];
+                        found = g++;
+                    }
+                    group_of[r] = found;
+                }
+                n_groups = g;
+            }
+            __syncthreads();
+            const int G = n_groups;
+            if (G == 0) break;
+            for (int i = threadIdx.x; i < G * 256; i += kBlock) hist[i >> 8][i & 255] = 0;
             uint64_t pre[kMaxRanks];
 #pragma unroll
-            for (int r = 0; r < kMaxRanks; ++r) pre[r] = (r < R && pass > 0) ? prefix[r] >> (shift + 8) : 0;
-            uint32_t my_nans = 0;
-            const int lane = threadIdx.x & 63;
+            for (int g = 0; g < kMaxRanks; ++g) pre[g] = (g < G && pass > 0) ? gprefix[g] >> (shift + 8) : 0;
+            __syncthreads();
             for (int32_t s = threadIdx.x; s < n; s += kBlock) {
                 const double v = row[s];
                 const bool ok = v == v;
-                my_nans += (!ok && pass == 0) ? 1u : 0u;
                 const uint64_t key = order_key(v);
                 const uint32_t digit = (uint32_t)(key >> shift) & 255u;
                 const uint64_t high = pass > 0 ? key >> (shift + 8) : 0;
-                const uint64_t active = __ballot(1);
-                const uint32_t d0 = __builtin_amdgcn_readfirstlane(digit);
-                const bool same_digit = __all(digit == d0);
 #pragma unroll
-                for (int r = 0; r < kMaxRanks; ++r) {
-                    if (r < R) {
-                        const bool m = ok && high == pre[r];
-                        const uint64_t mm = __ballot(m);
-                        if (mm == active && same_digit) {
-                            // the whole wave adds to one bin (rows saturated at a
-                            // cap hold many equal values): one atomic, not 64
-                            if (lane == __ffsll((long long)active) - 1)
-                                atomicAdd(&hist[r][d0], (uint32_t)__popcll(mm));
-                        } else if (m) {
-                            atomicAdd(&hist[r][digit], 1u);
-                        }
-                    }
-                }
+                for (int g = 0; g < kMaxRanks; ++g)
+                    if (g < G) wave_hist_add(hist[g], ok && high == pre[g], digit, lane);
             }
-            if (pass == 0 && my_nans) atomicAdd(&nans, my_nans);
             __syncthreads();
-            if ((int)threadIdx.x < R) {
+            if ((int)threadIdx.x < R && !done[threadIdx.x]) {
                 const int r = threadIdx.x;
+                const uint32_t *h = hist[group_of[r]];
                 int32_t k = krem[r];
                 uint32_t d = 0;
                 for (; d < 255; ++d) {
-                    const int32_t c = (int32_t)hist[r][d];
+                    const int32_t c = (int32_t)h[d];
                     if (k < c) break;
                     k -= c;
                 }
                 krem[r] = k;
                 prefix[r] |= (uint64_t)d << shift;
             }
-            __syncthreads();
         }
+        __syncthreads();
         if (threadIdx.x == 0) {
             for (int j = 0; j < qa.n_q; ++j) {
                 double res;
-                if (nans) {
+                if (X.nans) {
                     res = __builtin_nan("");
                 } else {
                     const double a = key_value(prefix[qa.lo[j]]);
@@ -179,7 +285,7 @@ __global__ void __launch_bounds__(kBlock) quantile_kernel(const double *__restri
                 out[(uint64_t)(uint32_t)e * (uint32_t)qa.n_q + j] = res;
             }
         }
-        __syncthreads();
+        __syncthreads();  // prefix[] and ext[] are read above before the next event resets them
     }
 }
 
@@ -203,10 +309,17 @@ __global__ void __launch_bounds__(kBlock) cause_kernel(const int32_t *__restrict
                                                        const int32_t *__restrict__ pred_nkey,
                                                        int64_t *__restrict__ counts) {
     __shared__ uint32_t c_lds[kCauseLdsCounters];
+    __shared__ int32_t k_lds[kCauseLdsKeys], p_lds[kCauseLdsKeys];
     for (int32_t e = blockIdx.x; e < E; e += gridDim.x) {
         const int32_t p0 = pred_off[e], np = pred_off[e + 1] - p0, nk = pred_nkey[e];
-        const int32_t *keys = pred_key + p0;
-        const int32_t *pos = pred_pos + p0;
+        // the event's distinct sources and positions, in LDS when they fit
+        const bool keys_in_lds = nk <= kCauseLdsKeys;
+        for (int i = threadIdx.x; keys_in_lds && i < nk; i += kBlock) {
+            k_lds[i] = pred_key[p0 + i];
+            p_lds[i] = pred_pos[p0 + i];
+        }
+        const int32_t *keys = keys_in_lds ? k_lds : pred_key + p0;
+        const int32_t *pos = keys_in_lds ? p_lds : pred_pos + p0;
         int64_t *out = counts + (int64_t)p0 + e;  // [1 + np]: cause -1, then each predecessor position
         const bool in_lds = np + 1 <= kCauseLdsCounters;
         if (in_lds) {
@@ -225,11 +338,21 @@ __global__ void __launch_bounds__(kBlock) cause_kernel(const int32_t *__restrict
                 const int32_t k = find_source(keys, nk, c);
                 if (k >= 0) slot = 1 + pos[k];
             }
-            if (slot < 0) continue;  // not a predecessor of e: counted nowhere
-            if (in_lds)
-                atomicAdd(&c_lds[slot], 1u);
-            else
+            // slot < 0: not a predecessor of e, counted nowhere
+            if (in_lds) {
+                // a wave's causes take few distinct values: one atomic per value
+                bool pending = slot >= 0;
+                while (__ballot(pending)) {
+                    const uint64_t mm = __ballot(pending);
+                    const int leader = __ffsll((long long)mm) - 1;
+                    const int32_t s0 = __shfl(slot, leader);
+                    const uint64_t same = __ballot(pending && slot == s0);
+                    if ((int)(threadIdx.x & 63) == leader) atomicAdd(&c_lds[s0], (uint32_t)__popcll(same));
+                    pending = pending && slot != s0;
+                }
+            } else if (slot >= 0) {
                 atomicAdd((unsigned long long *)&out[slot], 1ull);
+            }
         }
         __syncthreads();
         if (in_lds)
@@ -247,10 +370,10 @@ hipError_t launch_event_mean(const double *realized, int32_t n, int32_t ld, int3
 }
 
 hipError_t launch_event_quantiles(const double *realized, int32_t n, int32_t ld, int32_t E, const QuantileArgs &qa,
-                                  double *out, hipStream_t stream) {
+                                  double *out, double *mean, hipStream_t stream) {
     if (E > 0)
         hipLaunchKernelGGL(quantile_kernel, dim3((unsigned)E), dim3(kBlock), 0, stream, realized, n, (uint32_t)ld, E, qa,
-                           out);
+                           out, mean);
     return hipGetLastError();
 }
 

@@ -9,7 +9,10 @@ constexpr int kMaxQuantiles = 8;
 constexpr int kMaxRanks = 2 * kMaxQuantiles;
 // Events with more predecessors than this count their causes with global
 // atomics instead of LDS counters.
-constexpr int kCauseLdsCounters = 8192;
+constexpr int kCauseLdsCounters = 4096;
+// Events with more distinct predecessor sources than this search them in
+// global memory instead of LDS.
+constexpr int kCauseLdsKeys = 1024;
 
 // Order statistics the quantiles need and how each quantile combines two of
 // them (numpy's "linear" method; computed on the host in double).
@@ -24,8 +27,9 @@ struct QuantileArgs {
 
 hipError_t launch_event_mean(const double *realized, int32_t n, int32_t ld, int32_t E, double *mean,
                              hipStream_t stream);
+// `mean` (nullable): the row means, computed in the quantile kernel's first pass.
 hipError_t launch_event_quantiles(const double *realized, int32_t n, int32_t ld, int32_t E, const QuantileArgs &qa,
-                                  double *out, hipStream_t stream);
+                                  double *out, double *mean, hipStream_t stream);
 hipError_t launch_cause_counts(const int32_t *cause, int32_t n, int32_t ld, int32_t E, const int32_t *pred_off,
                                const int32_t *pred_key, const int32_t *pred_pos, const int32_t *pred_nkey,
                                int64_t *counts, hipStream_t stream);
