@@ -63,6 +63,8 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) 
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int W = p.waves;
     const uint32_t ld = (uint32_t)p.ld;
+    // records that write their duration row (none when no durations output)
+    const uint32_t owner_bit = p.durations ? (uint32_t)kInfoOwner : 0u;
     // The plan never changes during a launch: reading it through the constant
     // address space lets wave-uniform record loads become scalar loads.
     const auto step_off = as_const_space(p.step_off);
@@ -129,21 +131,24 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) 
                         }
                     } else {
                         double dur[kSpl];
+                        if (kInjected) {
 #pragma unroll
-                        for (int i = 0; i < kSpl; ++i) dur[i] = 0.0;
-                        if (cur.info & kInfoEdgeValid) {
-                            const uint64_t row = row_offset(cur.i2, ld);
-                            if (kInjected) {
+                            for (int i = 0; i < kSpl; ++i) dur[i] = 0.0;
+                            if (cur.info & kInfoEdgeValid) {
+                                const uint64_t row = row_offset(cur.i2, ld);
 #pragma unroll
                                 for (int i = 0; i < kSpl; ++i)
                                     dur[i] = __builtin_nontemporal_load(p.dur_in + row + s[i]);
-                            } else {
-                                draw(cur, seed, dur);
-                                if ((cur.info & kInfoOwner) && p.durations) {
+                            }
+                        } else {
+                            // an edge outside the link table is kind kNone with
+                            // minimal duration 0.0, and owns no row
+                            draw(cur, seed, dur);
+                            if (cur.info & owner_bit) {
+                                const uint64_t row = row_offset(cur.i2, ld);
 #pragma unroll
-                                    for (int i = 0; i < kSpl; ++i)
-                                        __builtin_nontemporal_store(dur[i], p.durations + row + s[i]);
-                                }
+                                for (int i = 0; i < kSpl; ++i)
+                                    __builtin_nontemporal_store(dur[i], p.durations + row + s[i]);
                             }
                         }
                         double rs[kSpl];

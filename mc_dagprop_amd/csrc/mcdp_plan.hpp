@@ -30,26 +30,30 @@ enum : int32_t {
 // Sampling methods.  Each registered distribution compiles to one of these.
 // Parameters: p0 travels in the record; p1..p4 in the distribution table.
 enum DistKind : uint32_t {
-    kConstant = 0,        // extra = d * p0
-    kExpInf = 1,          // X = -p0 log u, u in (0,1]                           (mean p0)
-    kExpTrunc = 2,        // X = -p0 log(p1 + p3 u), u in (0,1], clamp at p2  (p1 = e^{-m/p0}, p3 = 1 - p1)
-    kGammaErlang2 = 3,    // X = -p0 log(u1 u2)                                  (shape 2, scale p0)
-    kGammaHalf = 4,       // X = p0 (-log u1) cos^2(2 pi u2)                     (shape 1/2, scale p0)
-    kGammaMT = 5,         // Marsaglia-Tsang: scale p0, d = p1, c = p2, 1/k = p4 (boost flag)
-    kGammaPow = 6,        // truncated on [0, m]: X = scale m' U^{1/k}, accept w.p. e^{-m' U^{1/k}}
-    kEmpiricalAbs = 7,    // extra = v[alias draw]; p0 holds (n, table_off) as two uint32
-    kEmpiricalRel = 8,    // extra = f[alias draw] * d
-    kEmpiricalAbs1 = 9,   // single bin: extra = p0 (no draw)
-    kEmpiricalRel1 = 10,  // single bin: extra = p0 * d (no draw)
-    kNone = 0xFF,         // no distribution: duration = minimal_duration
+    // no draw
+    kNone = 0,            // no distribution: duration = minimal_duration
+    kConstant = 1,        // extra = d * p0
+    kEmpiricalAbs1 = 2,   // single bin: extra = p0
+    kEmpiricalRel1 = 3,   // single bin: extra = p0 * d
+    // one alias-table draw
+    kEmpiricalAbs = 4,    // extra = v[alias draw]; p0 holds (n, table_off) as two uint32
+    kEmpiricalRel = 5,    // extra = f[alias draw] * d
+    // exponential inversion
+    kExpInf = 6,          // X = -p0 log u, u in (0,1]                           (mean p0)
+    kExpTrunc = 7,        // X = -p0 log(p1 + p3 u), u in (0,1], clamp at p2  (p1 = e^{-m/p0}, p3 = 1 - p1)
+    // gamma family
+    kGammaErlang2 = 8,    // X = -p0 log(u1 u2)                                  (shape 2, scale p0)
+    kGammaHalf = 9,       // X = p0 (-log u1) cos^2(2 pi u2)                     (shape 1/2, scale p0)
+    kGammaMT = 10,        // Marsaglia-Tsang: scale p0, d = p1, c = p2, 1/k = p4 (boost flag)
+    kGammaPow = 11,       // truncated on [0, m]: X = scale m' U^{1/k}, accept w.p. e^{-m' U^{1/k}}
 };
 
 // Record::info.
 enum : uint32_t {
-    kInfoKindMask = 0xFFu,       // DistKind (predecessor records)
+    kInfoKindMask = 0xFFu,       // DistKind (predecessor records; kNone in padding records)
     kInfoBoost = 1u << 8,        // gamma shape < 1: Gamma(k + 1) U^{1/k}
     kInfoTruncated = 1u << 9,    // gamma family: rejection on max_scale (table p3)
-    kInfoEdgeValid = 1u << 10,   // edge in [0, link_count); otherwise duration 0.0
+    kInfoEdgeValid = 1u << 10,   // edge in [0, link_count); otherwise duration 0.0 (kind kNone, a = 0)
     kInfoOwner = 1u << 11,       // this record writes durations[edge]
     kInfoLast = 1u << 12,        // the event's last record: its realized time is final
     kInfoDistShift = 13,         // bits 13..30: distribution table index
