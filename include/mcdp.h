@@ -40,6 +40,7 @@ typedef enum mcdp_status {
 
 typedef struct mcdp_generator mcdp_generator;
 typedef struct mcdp_propagator mcdp_propagator;
+typedef struct mcdp_batch mcdp_batch;
 
 /* Message of the last failing call on this thread ("" if none). */
 const char *mcdp_last_error(void);
@@ -136,6 +137,29 @@ int mcdp_propagator_configure(mcdp_propagator *prop, int32_t waves, int32_t even
  * reference.) */
 int mcdp_propagator_autotune(mcdp_propagator *prop, int64_t n_samples, int32_t *waves, int32_t *events_per_wave,
                              int32_t *samples_per_lane, float *ms);
+
+/* ---- Batched run_many over several DAGs (no counterpart in the reference) - */
+/* The reference runs one DAG per Simulator; callers run several DAGs from
+ * several threads (test/test_simulator.py:201-215).  A batch runs them in one
+ * persistent launch: workgroups take (DAG, sample tile) items from one queue,
+ * largest DAGs first, so small DAGs and uneven sample counts fill the device.
+ * Results are bit-identical to each propagator's own mcdp_run_many_device. */
+
+/* Makes a batch of n_props propagators (they stay owned by the caller and must
+ * outlive the batch).  Every propagator is configured to the batch's schedule
+ * shape (waves, events_per_wave, samples_per_lane); all three 0 choose
+ * (16, 8, 2).  Reconfiguring one of them afterwards makes runs fail. */
+int mcdp_batch_create(mcdp_propagator *const *props, int32_t n_props, int32_t waves, int32_t events_per_wave,
+                      int32_t samples_per_lane, mcdp_batch **out);
+void mcdp_batch_destroy(mcdp_batch *batch);
+/* run_many of every propagator p of the batch on its own device buffers:
+ * seeds_dev[p] (n[p] seeds), realized_dev[p] / durations_dev[p] /
+ * cause_dev[p] event-major with leading dimension ld[p] >= n[p]
+ * (durations_dev / cause_dev may be NULL, or hold NULL entries, to skip those
+ * outputs).  Launches on `stream` and returns when the batch is done. */
+int mcdp_batch_run_device(mcdp_batch *batch, const int32_t *const *seeds_dev, const int64_t *n, const int64_t *ld,
+                          double *const *realized_dev, double *const *durations_dev, int32_t *const *cause_dev,
+                          void *stream);
 
 /* ---- Per-event statistics (no counterpart in the reference) -------------- */
 /* The reference returns one SimResult per seed and leaves statistics to its

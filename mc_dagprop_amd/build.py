@@ -1,7 +1,8 @@
 """Builds the native parts of mc_dagprop_amd in place.
 
 * ``libmcdp.so``: the gfx950 kernels (``csrc/mcdp_kernels.hip``, ``csrc/mcdp_stats.hip``; hipcc) and
-  the host compiler + C ABI (``csrc/mcdp_host.cpp``, ``csrc/mcdp_stats.cpp``; g++), declared in
+  the host compiler + C ABI (``csrc/mcdp_host.cpp``, ``csrc/mcdp_stats.cpp``, ``csrc/mcdp_batch.cpp``; g++),
+  declared in
   ``include/mcdp.h``.
 * ``monte_carlo/_core*.so``: the pybind11 module over that C ABI.
 
@@ -74,7 +75,7 @@ def build(verbose: bool = False) -> None:
     BUILD.mkdir(exist_ok=True)
     hipcc = str(ROCM / "bin" / "hipcc")
     kernels = ["mcdp_kernels.hip", "mcdp_stats.hip"]
-    hosts = ["mcdp_host.cpp", "mcdp_stats.cpp"]
+    hosts = ["mcdp_host.cpp", "mcdp_stats.cpp", "mcdp_batch.cpp"]
     objs = [BUILD / (Path(f).stem + ("_dev.o" if f.endswith(".hip") else "_host.o")) for f in kernels + hosts]
     steps = [[hipcc, *KERNEL_FLAGS, "-c", CSRC / f, "-o", o] for f, o in zip(kernels, objs)]
     steps += [["g++", *HOST_FLAGS, "-c", CSRC / f, "-o", o] for f, o in zip(hosts, objs[len(kernels):])]

@@ -71,6 +71,9 @@ _SIGNATURES = {
     "mcdp_propagator_pred_offsets": (C.c_int, [_P, _pi64]),
     "mcdp_event_stats_device": (C.c_int, [_P, _P, _P, _i64, _i64, _i32, _pf64, _P, _P, _P, _P]),
     "mcdp_run_stats_host": (C.c_int, [_P, _pi32, _i64, _i32, _pf64, _pf64, _pf64, _pi64]),
+    "mcdp_batch_create": (C.c_int, [C.POINTER(_P), _i32, _i32, _i32, _i32, C.POINTER(_P)]),
+    "mcdp_batch_destroy": (None, [_P]),
+    "mcdp_batch_run_device": (C.c_int, [_P, C.POINTER(_P), _pi64, _pi64, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), _P]),
 }
 
 _lib = None
@@ -351,6 +354,52 @@ def debug_math(fn: str, x: np.ndarray) -> np.ndarray:
     out = np.empty_like(x)
     check(lib().mcdp_debug_math(code, x.size, _ptr(x, _pf64), _ptr(out, _pf64)))
     return out
+
+
+class Batch:
+    """Several propagators run in one persistent launch (mcdp_batch_*)."""
+
+    def __init__(self, props: list, waves: int = 0, events_per_wave: int = 0, samples_per_lane: int = 0) -> None:
+        self.props = list(props)  # keeps the propagators alive
+        arr = (_P * len(self.props))(*[p.h for p in self.props])
+        self.h = C.c_void_p(0)
+        check(lib().mcdp_batch_create(arr, len(self.props), waves, events_per_wave, samples_per_lane,
+                                      C.byref(self.h)))
+
+    def run_many_device(self, seeds, n, ld, realized, durations=None, cause=None, stream=None) -> None:
+        """Device buffers per propagator (lists of DeviceBuffer or None)."""
+        k = len(self.props)
+
+        def ptrs(bufs):
+            if bufs is None:
+                return None
+            return (_P * k)(*[(b.ptr if b is not None else None) for b in bufs])
+
+        nn = np.ascontiguousarray(n, dtype=np.int64)
+        ll = np.ascontiguousarray(ld, dtype=np.int64)
+        check(lib().mcdp_batch_run_device(self.h, ptrs(seeds), _ptr(nn, _pi64), _ptr(ll, _pi64), ptrs(realized),
+                                          ptrs(durations), ptrs(cause), stream))
+
+    def run_many(self, seed_lists) -> list:
+        """Host convenience: [(realized (n, E), durations (n, L), cause (n, E))] per propagator."""
+        bufs = []
+        for P, sd in zip(self.props, seed_lists):
+            sd = np.ascontiguousarray(sd, dtype=np.int32)
+            n = sd.size
+            bufs.append((n, DeviceBuffer.from_array(sd), DeviceBuffer(8 * P.E * max(n, 1)),
+                         DeviceBuffer(8 * P.L * max(n, 1)), DeviceBuffer(4 * P.E * max(n, 1))))
+        self.run_many_device([b[1] for b in bufs], [b[0] for b in bufs], [max(b[0], 1) for b in bufs],
+                             [b[2] for b in bufs], [b[3] for b in bufs], [b[4] for b in bufs])
+        out = []
+        for P, (n, _, r, d, c) in zip(self.props, bufs):
+            out.append((r.to_array(np.float64, (P.E, n)).T, d.to_array(np.float64, (P.L, n)).T,
+                        c.to_array(np.int32, (P.E, n)).T))
+        return out
+
+    def __del__(self) -> None:  # pragma: no cover
+        if getattr(self, "h", None) and self.h.value and _lib is not None:
+            _lib.mcdp_batch_destroy(self.h)
+            self.h = C.c_void_p(0)
 
 
 class Timer:

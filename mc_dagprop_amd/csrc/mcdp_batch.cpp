@@ -1,0 +1,177 @@
+// Batched run_many over several compiled DAGs in one persistent launch
+// (C ABI in include/mcdp.h).  The reference runs one DAG per Simulator and
+// leaves parallelism across DAGs to threads (test/test_simulator.py:201-215);
+// here workgroups take (DAG, sample tile) items from one queue, largest DAGs
+// first, so small DAGs and uneven sample counts still fill the device.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <limits>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/mcdp.h"
+#include "mcdp_internal.hpp"
+
+using namespace mcdp;
+
+struct mcdp_batch {
+    std::vector<mcdp_propagator *> props;
+    int32_t waves = 16, k = 8, spl = 2;
+    void *dev = nullptr;  // WalkParams[plans], item_dag[], item_tile[], counter
+    size_t dev_bytes = 0;
+    int device = -1;
+    ~mcdp_batch() {
+        if (dev) {
+            (void)hipDeviceSynchronize();
+            (void)hipFree(dev);
+        }
+    }
+};
+
+namespace {
+size_t align_to(size_t x, size_t a) { return (x + a - 1) / a * a; }
+}  // namespace
+
+extern "C" {
+
+int mcdp_batch_create(mcdp_propagator *const *props, int32_t n_props, int32_t waves, int32_t events_per_wave,
+                      int32_t samples_per_lane, mcdp_batch **out) {
+    if (!out) return fail(MCDP_E_INVALID, "out is null");
+    *out = nullptr;
+    if (n_props < 1 || !props) return fail(MCDP_E_INVALID, "a batch needs at least one propagator");
+    for (int32_t i = 0; i < n_props; ++i)
+        if (!props[i]) return fail(MCDP_E_INVALID, "propagator is null");
+    if (waves == 0 && events_per_wave == 0 && samples_per_lane == 0) {
+        waves = 16;
+        events_per_wave = 8;
+        samples_per_lane = 2;
+    }
+    if (samples_per_lane != 1 && samples_per_lane != 2)
+        return fail(MCDP_E_INVALID, "samples_per_lane must be 1 or 2");
+    for (int32_t i = 0; i < n_props; ++i) {  // one schedule shape for every plan of the launch
+        int rc = configure_shape(props[i], waves, events_per_wave, samples_per_lane);
+        if (rc) return rc;
+    }
+    auto *b = new (std::nothrow) mcdp_batch;
+    if (!b) return fail(MCDP_E_NOMEM, "out of host memory");
+    b->props.assign(props, props + n_props);
+    b->waves = waves;
+    b->k = events_per_wave;
+    b->spl = samples_per_lane;
+    *out = b;
+    return MCDP_OK;
+}
+
+void mcdp_batch_destroy(mcdp_batch *b) { delete b; }
+
+int mcdp_batch_run_device(mcdp_batch *b, const int32_t *const *seeds_dev, const int64_t *n, const int64_t *ld,
+                          double *const *realized_dev, double *const *durations_dev, int32_t *const *cause_dev,
+                          void *stream) {
+    if (!b) return fail(MCDP_E_INVALID, "batch is null");
+    if (!seeds_dev || !n || !ld || !realized_dev) return fail(MCDP_E_INVALID, "null argument");
+    const size_t D = b->props.size();
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return fail(MCDP_E_DEVICE, "no HIP device available: the Monte Carlo engine runs on the GPU only");
+    int device = 0;
+    HIP_TRY(hipGetDevice(&device), "hipGetDevice");
+
+    // Per-plan launch parameters; the fast variant only if every plan allows it.
+    std::vector<WalkParams> params(D);
+    std::vector<int64_t> tiles(D), cost(D);
+    bool all_fast = true;
+    size_t lds = 0;
+    const int64_t tile = (int64_t)kLanes * b->spl;
+    for (size_t d = 0; d < D; ++d) {
+        mcdp_propagator &P = *b->props[d];
+        if (n[d] < 0 || (n[d] > 0 && ld[d] < n[d])) return fail(MCDP_E_INVALID, "need 0 <= n <= ld for every plan");
+        if (n[d] > std::numeric_limits<int32_t>::max() - 256 || ld[d] > std::numeric_limits<int32_t>::max())
+            return fail(MCDP_E_INVALID, "n and ld must be below 2^31 - 256 per plan");
+        if (n[d] > 0 && (!seeds_dev[d] || (P.E > 0 && !realized_dev[d])))
+            return fail(MCDP_E_INVALID, "seeds and realized are required for every plan with samples");
+        std::lock_guard<std::mutex> lock(P.mu);
+        if (P.waves != b->waves || P.k != b->k || P.spl != b->spl)
+            return fail(MCDP_E_INVALID, "a propagator of the batch was reconfigured after the batch was made");
+        DeviceCopy *dc = nullptr;
+        int rc = ensure_device(P, dc);
+        if (rc) return rc;
+        WalkParams p = dc->base;
+        p.seeds = seeds_dev[d];
+        p.dur_in = nullptr;
+        p.n = (int32_t)n[d];
+        p.ld = (int32_t)std::max<int64_t>(ld[d], 1);
+        p.realized = realized_dev[d];
+        p.durations = durations_dev ? durations_dev[d] : nullptr;
+        p.cause = cause_dev ? cause_dev[d] : nullptr;
+        const bool has_work = n[d] > 0 && (P.E > 0 || (!P.orph.empty() && p.durations));
+        tiles[d] = has_work ? (n[d] + tile - 1) / tile : 0;
+        p.n_tiles = (int32_t)tiles[d];
+        params[d] = p;
+        const WalkVariant v = plan_variant(P, false);
+        all_fast = all_fast && v.paths == kPathsFast;
+        cost[d] = (int64_t)P.recs.size() + (int64_t)P.orph.size();
+    }
+    WalkVariant v{false, all_fast ? kPathsFast : kPathsGeneral, b->k, b->spl};
+    for (size_t d = 0; d < D; ++d) lds = std::max(lds, plan_lds_bytes(*b->props[d], v));
+
+    // Work items: every tile of every plan, plans with the most records first.
+    std::vector<size_t> order(D);
+    std::iota(order.begin(), order.end(), (size_t)0);
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+    std::vector<int32_t> item_dag, item_tile;
+    for (size_t d : order)
+        for (int64_t t = 0; t < tiles[d]; ++t) {
+            item_dag.push_back((int32_t)d);
+            item_tile.push_back((int32_t)t);
+        }
+    if (item_dag.size() > (size_t)std::numeric_limits<int32_t>::max() / 2)
+        return fail(MCDP_E_INVALID, "too many tiles in one batch");
+    const int32_t n_items = (int32_t)item_dag.size();
+    if (n_items == 0) return MCDP_OK;
+
+    const size_t o_par = 0, o_dag = align_to(sizeof(WalkParams) * D, 256);
+    const size_t o_tile = o_dag + align_to(sizeof(int32_t) * item_dag.size(), 256);
+    const size_t o_cnt = o_tile + align_to(sizeof(int32_t) * item_tile.size(), 256);
+    const size_t bytes = o_cnt + 256;
+    if (!b->dev || b->dev_bytes < bytes || b->device != device) {
+        if (b->dev) {
+            (void)hipDeviceSynchronize();
+            (void)hipFree(b->dev);
+            b->dev = nullptr;
+        }
+        hipError_t e = hipMalloc(&b->dev, bytes);
+        if (e != hipSuccess) return fail(MCDP_E_NOMEM, std::string("hipMalloc (batch): ") + hipGetErrorString(e));
+        b->dev_bytes = bytes;
+        b->device = device;
+    }
+    auto *base = static_cast<unsigned char *>(b->dev);
+    const auto s = static_cast<hipStream_t>(stream);
+    // Earlier launches on this stream may still read the descriptors: the
+    // copies are ordered on the same stream, from host memory that outlives
+    // them (synchronised below before returning).
+    HIP_TRY(hipMemcpyAsync(base + o_par, params.data(), sizeof(WalkParams) * D, hipMemcpyHostToDevice, s), "copy batch");
+    HIP_TRY(hipMemcpyAsync(base + o_dag, item_dag.data(), sizeof(int32_t) * item_dag.size(), hipMemcpyHostToDevice, s),
+            "copy batch");
+    HIP_TRY(hipMemcpyAsync(base + o_tile, item_tile.data(), sizeof(int32_t) * item_tile.size(), hipMemcpyHostToDevice,
+                           s),
+            "copy batch");
+    WalkBatchParams bp;
+    bp.params = reinterpret_cast<const WalkParams *>(base + o_par);
+    bp.item_dag = reinterpret_cast<const int32_t *>(base + o_dag);
+    bp.item_tile = reinterpret_cast<const int32_t *>(base + o_tile);
+    bp.n_items = n_items;
+    bp.counter = reinterpret_cast<int32_t *>(base + o_cnt);
+    int cus = 0, bpc = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute");
+    HIP_TRY(walk_batch_occupancy(v, b->waves, lds, &bpc), "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    if (bpc < 1) return fail(MCDP_E_DEVICE, "batch kernel cannot be resident with " + std::to_string(lds) + " bytes of LDS");
+    const int grid = (int)std::min<int64_t>(n_items, (int64_t)cus * bpc);
+    HIP_TRY(launch_walk_batch(bp, v, b->waves, grid, lds, s), "batch kernel launch");
+    // The host vectors above back the asynchronous copies.
+    HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+    return MCDP_OK;
+}
+
+}  // extern "C"

@@ -901,6 +901,11 @@ int run_host(mcdp_propagator &P, bool injected, const int32_t *seeds, const doub
 }  // namespace
 
 int mcdp::ensure_device(mcdp_propagator &P, DeviceCopy *&out) { return ensure_device_impl(P, out); }
+mcdp::WalkVariant mcdp::plan_variant(const mcdp_propagator &P, bool injected) { return walk_variant(P, injected); }
+size_t mcdp::plan_lds_bytes(const mcdp_propagator &P, WalkVariant v) { return walk_lds_bytes(P, v); }
+int mcdp::configure_shape(mcdp_propagator *prop, int32_t waves, int32_t k, int32_t spl) {
+    return mcdp_propagator_configure(prop, waves, k, spl);
+}
 
 // ============================================================================
 // C ABI

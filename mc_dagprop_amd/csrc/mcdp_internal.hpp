@@ -96,4 +96,9 @@ namespace mcdp {
 // The propagator's plan on the current device (uploaded on first use).  The
 // caller holds P.mu.
 int ensure_device(mcdp_propagator &P, DeviceCopy *&out);
+// Walk-kernel variant (fast or general paths) and its LDS bytes for P's plan.
+WalkVariant plan_variant(const mcdp_propagator &P, bool injected);
+size_t plan_lds_bytes(const mcdp_propagator &P, WalkVariant v);
+// mcdp_propagator_configure.
+int configure_shape(mcdp_propagator *prop, int32_t waves, int32_t k, int32_t spl);
 }  // namespace mcdp

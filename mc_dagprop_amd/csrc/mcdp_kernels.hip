@@ -56,9 +56,27 @@ __device__ __forceinline__ void lds_store(double *lds, int slot, int lane, const
 // a 32 x 32 -> 64-bit product (rows and ld are non-negative and < 2^31).
 __device__ __forceinline__ uint64_t row_offset(int32_t row, uint32_t ld) { return (uint64_t)(uint32_t)row * ld; }
 
+// Alias tables staged in LDS after the slots (fast sampled variant), so an
+// empirical draw is a ds_read rather than a vector memory load.
+template <bool kInjected, int kPaths, int kSpl>
+__device__ __forceinline__ AliasRec *lds_alias_of(const WalkParams &p, double *lds) {
+    return reinterpret_cast<AliasRec *>(lds + (size_t)p.slots * kLanes * kSpl);
+}
+template <bool kInjected, int kPaths, int kSpl>
+__device__ __forceinline__ void stage_alias(const WalkParams &p, double *lds) {
+    if (!kInjected && kPaths == kPathsFast) {
+        AliasRec *const dst = lds_alias_of<kInjected, kPaths, kSpl>(p, lds);
+        const auto src = as_const_space(p.alias);
+        for (int i = threadIdx.x; i < p.n_alias; i += blockDim.x) dst[i] = src[i];
+        __syncthreads();
+    }
+}
+
+// One tile of 64 x kSpl samples of one plan: every step, then the orphan
+// activities.  Every wave of the workgroup calls it; it ends with all waves
+// past the last step's barrier.
 template <bool kInjected, int kPaths, int K, int kSpl>
-__global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) {
-    extern __shared__ double lds[];
+__device__ __forceinline__ void walk_tile(const WalkParams &p, int32_t tile, double *lds) {
     const int lane = threadIdx.x & (kLanes - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int W = p.waves;
@@ -74,14 +92,7 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) 
     const auto dists = as_const_space(p.dists);
     const auto earliest = as_const_space(p.earliest);
 
-    // Fast variant: stage the alias tables in LDS after the slots, so an
-    // empirical draw is a ds_read rather than a vector memory load.
-    AliasRec *const lds_alias = reinterpret_cast<AliasRec *>(lds + (size_t)p.slots * kLanes * kSpl);
-    if (!kInjected && kPaths == kPathsFast) {
-        const auto src = as_const_space(p.alias);
-        for (int i = threadIdx.x; i < p.n_alias; i += blockDim.x) lds_alias[i] = src[i];
-        __syncthreads();
-    }
+    AliasRec *const lds_alias = lds_alias_of<kInjected, kPaths, kSpl>(p, lds);
     auto draw = [&](const Record &r, const uint32_t(&seed)[kSpl], double(&dur)[kSpl]) {
         if (kPaths == kPathsFast)
             draw_durations<kSpl>(r.info, r.a, r.b, (uint32_t)r.i2, seed, lds_alias, dists, dur);
@@ -89,130 +100,164 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) 
             draw_durations<kSpl>(r.info, r.a, r.b, (uint32_t)r.i2, seed, as_const_space(p.alias), dists, dur);
     };
 
-    for (int32_t tile = blockIdx.x; tile < p.n_tiles; tile += gridDim.x) {
-        // Lanes past the last sample (last tile only) take over sample n - 1:
-        // same seed or injected row, so they compute its values bit for bit and
-        // their stores rewrite its bytes unchanged.  No store needs a lane mask.
-        uint32_t s[kSpl];
-        uint32_t seed[kSpl];
+    // Lanes past the last sample (last tile only) take over sample n - 1:
+    // same seed or injected row, so they compute its values bit for bit and
+    // their stores rewrite its bytes unchanged.  No store needs a lane mask.
+    uint32_t s[kSpl];
+    uint32_t seed[kSpl];
 #pragma unroll
-        for (int i = 0; i < kSpl; ++i) {
-            const int32_t si = tile * (kLanes * kSpl) + i * kLanes + lane;
-            s[i] = si < p.n ? si : p.n - 1;
-            seed[i] = kInjected ? 0u : (uint32_t)p.seeds[s[i]];
-        }
+    for (int i = 0; i < kSpl; ++i) {
+        const int32_t si = tile * (kLanes * kSpl) + i * kLanes + lane;
+        s[i] = si < p.n ? si : p.n - 1;
+        seed[i] = kInjected ? 0u : (uint32_t)p.seeds[s[i]];
+    }
 
-        for (int t = 0; t < p.n_steps; ++t) {
-            const int r0 = step_off[t * W + wave];
-            const int r1 = step_off[t * W + wave + 1];
-            double res[K][kSpl];
-            int j = 0;
-            if (r0 < r1) {
-                // One stream of records: an event header, then its predecessors
-                // in input order.  The stream ends in a padding record, so the
-                // one-ahead prefetch never leaves the array.
-                Record cur = recs[r0];
-                double best[kSpl], upper = 0.0;
-                int32_t who[kSpl], event = 0;
+    for (int t = 0; t < p.n_steps; ++t) {
+        const int r0 = step_off[t * W + wave];
+        const int r1 = step_off[t * W + wave + 1];
+        double res[K][kSpl];
+        int j = 0;
+        if (r0 < r1) {
+            // One stream of records: an event header, then its predecessors
+            // in input order.  The stream ends in a padding record, so the
+            // one-ahead prefetch never leaves the array.
+            Record cur = recs[r0];
+            double best[kSpl], upper = 0.0;
+            int32_t who[kSpl], event = 0;
 #pragma unroll
-                for (int i = 0; i < kSpl; ++i) {
-                    best[i] = 0.0;
-                    who[i] = -1;
-                }
-                for (int r = r0; r < r1; ++r) {
-                    const Record nxt = recs[r + 1];
-                    if (cur.info & kInfoHeader) {
-                        upper = cur.b;
-                        event = cur.i0;
+            for (int i = 0; i < kSpl; ++i) {
+                best[i] = 0.0;
+                who[i] = -1;
+            }
+            for (int r = r0; r < r1; ++r) {
+                const Record nxt = recs[r + 1];
+                if (cur.info & kInfoHeader) {
+                    upper = cur.b;
+                    event = cur.i0;
 #pragma unroll
-                        for (int i = 0; i < kSpl; ++i) {
-                            best[i] = cur.a;
-                            who[i] = -1;
+                    for (int i = 0; i < kSpl; ++i) {
+                        best[i] = cur.a;
+                        who[i] = -1;
+                    }
+                } else {
+                    double dur[kSpl];
+                    if (kInjected) {
+#pragma unroll
+                        for (int i = 0; i < kSpl; ++i) dur[i] = 0.0;
+                        if (cur.info & kInfoEdgeValid) {
+                            const uint64_t row = row_offset(cur.i2, ld);
+#pragma unroll
+                            for (int i = 0; i < kSpl; ++i)
+                                dur[i] = __builtin_nontemporal_load(p.dur_in + row + s[i]);
                         }
                     } else {
-                        double dur[kSpl];
-                        if (kInjected) {
+                        // an edge outside the link table is kind kNone with
+                        // minimal duration 0.0, and owns no row
+                        draw(cur, seed, dur);
+                        if (cur.info & owner_bit) {
+                            const uint64_t row = row_offset(cur.i2, ld);
 #pragma unroll
-                            for (int i = 0; i < kSpl; ++i) dur[i] = 0.0;
-                            if (cur.info & kInfoEdgeValid) {
-                                const uint64_t row = row_offset(cur.i2, ld);
-#pragma unroll
-                                for (int i = 0; i < kSpl; ++i)
-                                    dur[i] = __builtin_nontemporal_load(p.dur_in + row + s[i]);
-                            }
-                        } else {
-                            // an edge outside the link table is kind kNone with
-                            // minimal duration 0.0, and owns no row
-                            draw(cur, seed, dur);
-                            if (cur.info & owner_bit) {
-                                const uint64_t row = row_offset(cur.i2, ld);
-#pragma unroll
-                                for (int i = 0; i < kSpl; ++i)
-                                    __builtin_nontemporal_store(dur[i], p.durations + row + s[i]);
-                            }
-                        }
-                        double rs[kSpl];
-                        if (cur.i1 >= 0) {
-                            lds_load<kSpl>(lds, cur.i1, lane, rs);
-                        } else if (kPaths == kPathsGeneral && cur.i1 == kSrcGlobal) {
-                            const uint64_t row = row_offset(cur.i0, ld);
-#pragma unroll
-                            for (int i = 0; i < kSpl; ++i) rs[i] = p.realized[row + s[i]];
-                        } else {
-                            const double e = earliest[cur.i0];
-#pragma unroll
-                            for (int i = 0; i < kSpl; ++i) rs[i] = e;
-                        }
-#pragma unroll
-                        for (int i = 0; i < kSpl; ++i) {
-                            double tt = rs[i] + dur[i];
-                            tt = (upper < tt) ? upper : tt;
-                            if (tt >= best[i]) {
-                                best[i] = tt;
-                                who[i] = cur.i0;
-                            }
+                            for (int i = 0; i < kSpl; ++i)
+                                __builtin_nontemporal_store(dur[i], p.durations + row + s[i]);
                         }
                     }
-                    if (cur.info & kInfoLast) {  // the event's last predecessor (or a header with none)
-                        const uint64_t row = row_offset(event, ld);
+                    double rs[kSpl];
+                    if (cur.i1 >= 0) {
+                        lds_load<kSpl>(lds, cur.i1, lane, rs);
+                    } else if (kPaths == kPathsGeneral && cur.i1 == kSrcGlobal) {
+                        const uint64_t row = row_offset(cur.i0, ld);
 #pragma unroll
-                        for (int i = 0; i < kSpl; ++i) {
-                            const double rr = (upper < best[i]) ? upper : best[i];
-                            __builtin_nontemporal_store(rr, p.realized + row + s[i]);
-                            if (p.cause) __builtin_nontemporal_store(who[i], p.cause + row + s[i]);
-                            res[j][i] = rr;
-                        }
-                        ++j;
+                        for (int i = 0; i < kSpl; ++i) rs[i] = p.realized[row + s[i]];
+                    } else {
+                        const double e = earliest[cur.i0];
+#pragma unroll
+                        for (int i = 0; i < kSpl; ++i) rs[i] = e;
                     }
-                    cur = nxt;
+#pragma unroll
+                    for (int i = 0; i < kSpl; ++i) {
+                        double tt = rs[i] + dur[i];
+                        tt = (upper < tt) ? upper : tt;
+                        if (tt >= best[i]) {
+                            best[i] = tt;
+                            who[i] = cur.i0;
+                        }
+                    }
                 }
-            }
-            // Every read of this step is done before any slot is overwritten,
-            // and every commit is visible before the next step reads it.
-            __syncthreads();
-            const int e0 = ev_off[t * W + wave];
-            for (int i = 0; i < j; ++i) {
-                const int slot = slot_list[e0 + i];
-                if (slot >= 0) lds_store<kSpl>(lds, slot, lane, res[i]);
-            }
-            __syncthreads();
-        }
-
-        // Activities no predecessor entry references still get a duration row.
-        if (!kInjected && p.durations) {
-            const auto orph = as_const_space(p.orph);
-            for (int o = wave; o < p.n_orphans; o += W) {
-                const Record rec = orph[o];
-                double dur[kSpl];
-                draw(rec, seed, dur);
-                const uint64_t row = row_offset(rec.i2, ld);
+                if (cur.info & kInfoLast) {  // the event's last predecessor (or a header with none)
+                    const uint64_t row = row_offset(event, ld);
 #pragma unroll
-                for (int i = 0; i < kSpl; ++i)
-                    __builtin_nontemporal_store(dur[i], p.durations + row + s[i]);
+                    for (int i = 0; i < kSpl; ++i) {
+                        const double rr = (upper < best[i]) ? upper : best[i];
+                        __builtin_nontemporal_store(rr, p.realized + row + s[i]);
+                        if (p.cause) __builtin_nontemporal_store(who[i], p.cause + row + s[i]);
+                        res[j][i] = rr;
+                    }
+                    ++j;
+                }
+                cur = nxt;
             }
+        }
+        // Every read of this step is done before any slot is overwritten,
+        // and every commit is visible before the next step reads it.
+        __syncthreads();
+        const int e0 = ev_off[t * W + wave];
+        for (int i = 0; i < j; ++i) {
+            const int slot = slot_list[e0 + i];
+            if (slot >= 0) lds_store<kSpl>(lds, slot, lane, res[i]);
+        }
+        __syncthreads();
+    }
+
+    // Activities no predecessor entry references still get a duration row.
+    if (!kInjected && p.durations) {
+        const auto orph = as_const_space(p.orph);
+        for (int o = wave; o < p.n_orphans; o += W) {
+            const Record rec = orph[o];
+            double dur[kSpl];
+            draw(rec, seed, dur);
+            const uint64_t row = row_offset(rec.i2, ld);
+#pragma unroll
+            for (int i = 0; i < kSpl; ++i)
+                __builtin_nontemporal_store(dur[i], p.durations + row + s[i]);
         }
     }
 }
+
+template <bool kInjected, int kPaths, int K, int kSpl>
+__global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) {
+    extern __shared__ double lds[];
+    stage_alias<kInjected, kPaths, kSpl>(p, lds);
+    for (int32_t tile = blockIdx.x; tile < p.n_tiles; tile += gridDim.x) walk_tile<kInjected, kPaths, K, kSpl>(p, tile, lds);
+}
+
+// Several plans in one launch: workgroups take (plan, tile) items from a
+// queue (largest plans first, as ordered by the host) until it is empty;
+// every workgroup leaves through the same check of the queue head.
+template <bool kInjected, int kPaths, int K, int kSpl>
+__global__ void __launch_bounds__(kMaxWaves * kLanes) walk_batch_kernel(WalkBatchParams b) {
+    extern __shared__ double lds[];
+    __shared__ int32_t next_item;
+    int32_t staged = -1;
+    const auto item_dag = as_const_space(b.item_dag);
+    const auto item_tile = as_const_space(b.item_tile);
+    const auto params = as_const_space(b.params);
+    for (;;) {
+        __syncthreads();  // the previous tile is done with LDS and next_item
+        if (threadIdx.x == 0) next_item = atomicAdd(b.counter, 1);
+        __syncthreads();
+        const int32_t item = __builtin_amdgcn_readfirstlane(next_item);
+        if (item >= b.n_items) break;
+        const int32_t d = item_dag[item];
+        const WalkParams p = params[d];
+        if (d != staged) {
+            stage_alias<kInjected, kPaths, kSpl>(p, lds);
+            staged = d;
+        }
+        walk_tile<kInjected, kPaths, K, kSpl>(p, item_tile[item], lds);
+    }
+}
+
+__global__ void reset_counter_kernel(int32_t *counter) { *counter = 0; }
 
 __global__ void philox_debug_kernel(int variant, int64_t n, const uint32_t *__restrict__ ctr,
                                     const uint32_t *__restrict__ key, uint32_t *__restrict__ out) {
@@ -275,6 +320,65 @@ const void *walk_symbol(WalkVariant v) {
                                      : walk_symbol<true, kPathsGeneral>(v.k, v.spl);
     return v.paths == kPathsFast ? walk_symbol<false, kPathsFast>(v.k, v.spl)
                                  : walk_symbol<false, kPathsGeneral>(v.k, v.spl);
+}
+
+template <bool kInjected, int kPaths, int kSpl>
+const void *walk_batch_symbol(int k) {
+    switch (k) {
+        case 4: return (const void *)walk_batch_kernel<kInjected, kPaths, 4, kSpl>;
+        case 8: return (const void *)walk_batch_kernel<kInjected, kPaths, 8, kSpl>;
+        case 12: return kSpl == 1 ? (const void *)walk_batch_kernel<kInjected, kPaths, 12, 1> : nullptr;
+        case 16: return kSpl == 1 ? (const void *)walk_batch_kernel<kInjected, kPaths, 16, 1> : nullptr;
+        default: return nullptr;
+    }
+}
+
+// Batches run sampled plans (the injected variant has no batch form).
+const void *walk_batch_symbol(WalkVariant v) {
+    if (v.injected) return nullptr;
+    if (v.spl == 1)
+        return v.paths == kPathsFast ? walk_batch_symbol<false, kPathsFast, 1>(v.k)
+                                     : walk_batch_symbol<false, kPathsGeneral, 1>(v.k);
+    if (v.spl == 2)
+        return v.paths == kPathsFast ? walk_batch_symbol<false, kPathsFast, 2>(v.k)
+                                     : walk_batch_symbol<false, kPathsGeneral, 2>(v.k);
+    return nullptr;
+}
+
+hipError_t walk_batch_occupancy(WalkVariant v, int waves, size_t lds_bytes, int *blocks_per_cu) {
+    const void *fn = walk_batch_symbol(v);
+    if (!fn) return hipErrorInvalidValue;
+    if (lds_bytes > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+        if (e != hipSuccess) return e;
+    }
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, waves * kLanes, lds_bytes);
+    if (e != hipSuccess) return e;
+    hipFuncAttributes attr;
+    e = hipFuncGetAttributes(&attr, fn);
+    if (e != hipSuccess) return e;
+    if (attr.numRegs > 0) {
+        const int granules = (attr.numRegs + 7) / 8;
+        const int per_simd = std::min(8, 512 / (granules * 8));
+        *blocks_per_cu = std::min(*blocks_per_cu, per_simd / ((waves + 3) / 4));
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_walk_batch(const WalkBatchParams &b, WalkVariant v, int waves, int grid, size_t lds_bytes,
+                             hipStream_t stream) {
+    const void *fn = walk_batch_symbol(v);
+    if (!fn) return hipErrorInvalidValue;
+    if (lds_bytes > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, b.counter);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    WalkBatchParams args = b;
+    void *kargs[] = {&args};
+    return hipLaunchKernel(fn, dim3(grid), dim3(waves * kLanes), kargs, lds_bytes, stream);
 }
 
 hipError_t launch_walk(const WalkParams &p, WalkVariant v, int grid, size_t lds_bytes, hipStream_t stream) {

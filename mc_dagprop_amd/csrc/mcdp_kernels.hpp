@@ -58,6 +58,15 @@ struct WalkParams {
     int32_t *cause;         // [E][ld] or null
 };
 
+// A batch: several plans' WalkParams and a queue of (plan, tile) items.
+struct WalkBatchParams {
+    const WalkParams *params;  // [plans]
+    const int32_t *item_dag;   // [n_items] plan of each item
+    const int32_t *item_tile;  // [n_items] tile of each item
+    int32_t n_items;
+    int32_t *counter;          // queue head, zeroed before the launch
+};
+
 struct WalkVariant {
     bool injected;  // durations read from dur_in instead of sampled
     int paths;      // WalkPaths
@@ -67,6 +76,9 @@ struct WalkVariant {
 
 hipError_t launch_walk(const WalkParams &p, WalkVariant v, int grid, size_t lds_bytes, hipStream_t stream);
 hipError_t walk_occupancy(WalkVariant v, int waves, size_t lds_bytes, int *blocks_per_cu);
+hipError_t launch_walk_batch(const WalkBatchParams &b, WalkVariant v, int waves, int grid, size_t lds_bytes,
+                             hipStream_t stream);
+hipError_t walk_batch_occupancy(WalkVariant v, int waves, size_t lds_bytes, int *blocks_per_cu);
 hipError_t launch_copy(double *dst, const double *src, int64_t n, hipStream_t stream);
 hipError_t launch_math_debug(int fn, int64_t n, const double *x, double *out, hipStream_t stream);
 hipError_t launch_philox_debug(int variant, int64_t n, const uint32_t *ctr, const uint32_t *key, uint32_t *out,

@@ -1,0 +1,92 @@
+"""Batched run_many: several DAGs in one persistent launch give, for every DAG and sample, the bits
+each DAG's own run_many gives (the sampler is keyed by seed and activity, the recurrence is the
+reference's), whatever the batch's shape, sample counts, leading dimensions or missing outputs."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import dags as D
+from mc_dagprop_amd import capi
+from mc_dagprop_amd.workloads import layered_network, layered_small
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(a).view(np.uint64)
+
+
+def _dags():
+    rng = np.random.default_rng(11)
+    return [
+        layered_small(),
+        layered_network(layers=12),
+        D.random_dag(rng, 120, max_preds=5, duplicates=True, oob_edges=True, dists=D.STD_DISTS),
+        D.random_dag(rng, 7, max_preds=2, dists=D.STD_DISTS),
+    ]
+
+
+@pytest.mark.parametrize("shape", [(0, 0, 0), (8, 16, 1), (5, 4, 2)])
+def test_batch_matches_single_runs(gpu, shape) -> None:
+    dags = _dags()
+    ns = [1000, 300, 129, 1]
+    seeds = [np.arange(n, dtype=np.int32) * (k + 3) - 500 for k, n in enumerate(ns)]
+    singles = [capi.Propagator(d).run_many(s) for d, s in zip(dags, seeds)]
+    B = capi.Batch([capi.Propagator(d) for d in dags], *shape)
+    got = B.run_many(seeds)
+    for (r, d, c), (r0, d0, c0) in zip(got, singles):
+        np.testing.assert_array_equal(bits(r), bits(r0))
+        np.testing.assert_array_equal(bits(d), bits(d0))
+        np.testing.assert_array_equal(c, c0)
+
+
+def test_batch_general_path_and_empty_plan(gpu) -> None:
+    """A plan that spills predecessors to global memory puts the whole batch on the general
+    variant; a plan with no samples contributes no work."""
+    from test_gpu_parity import _wide_dag
+
+    dags = [_wide_dag(600, 5), layered_small(), layered_small()]
+    props = [capi.Propagator(d) for d in dags]
+    B = capi.Batch(props)
+    assert props[0].plan_info()["global_preds"] > 0
+    seeds = [np.arange(70), np.arange(200, 330), np.arange(0)]
+    got = B.run_many(seeds)
+    for d, s, (r, dd, c) in zip(dags, seeds, got):
+        if s.size == 0:
+            assert r.shape[0] == 0
+            continue
+        r0, d0, c0 = capi.Propagator(d).run_many(s)
+        np.testing.assert_array_equal(bits(r), bits(r0))
+        np.testing.assert_array_equal(bits(dd), bits(d0))
+        np.testing.assert_array_equal(c, c0)
+
+
+def test_batch_device_leading_dimension_and_missing_outputs(gpu) -> None:
+    dags = [layered_small(), layered_network(layers=5)]
+    props = [capi.Propagator(d) for d in dags]
+    B = capi.Batch(props)
+    ns, lds = [777, 200], [800, 203]
+    sd = [capi.DeviceBuffer.from_array(np.arange(n, dtype=np.int32)) for n in ns]
+    real = [capi.DeviceBuffer.from_array(np.full((P.E, ld), -7.0)) for P, ld in zip(props, lds)]
+    cause = [capi.DeviceBuffer.from_array(np.full((P.E, ld), -9, dtype=np.int32)) for P, ld in zip(props, lds)]
+    B.run_many_device(sd, ns, lds, real, None, [cause[0], None])
+    for k, (P, n, ld) in enumerate(zip(props, ns, lds)):
+        R = real[k].to_array(np.float64, (P.E, ld))
+        Cc = cause[k].to_array(np.int32, (P.E, ld))
+        r0, _, c0 = capi.Propagator(dags[k]).run_many(np.arange(n))
+        assert np.all(R[:, n:] == -7.0)
+        np.testing.assert_array_equal(bits(R[:, :n].T), bits(r0))
+        if k == 0:
+            np.testing.assert_array_equal(Cc[:, :n].T, c0)
+        else:
+            assert np.all(Cc == -9)  # cause output skipped
+
+
+def test_reconfigured_member_is_rejected(gpu) -> None:
+    props = [capi.Propagator(layered_small())]
+    B = capi.Batch(props)
+    props[0].configure(4, 8, 1)
+    with pytest.raises(capi.McdpError, match="reconfigured"):
+        B.run_many([np.arange(10)])
