@@ -193,6 +193,49 @@ int mcdp_event_stats_device(mcdp_propagator *prop, const double *realized_dev, c
 int mcdp_run_stats_host(mcdp_propagator *prop, const int32_t *seeds, int64_t n, int32_t n_q, const double *q,
                         double *mean, double *quantiles, int64_t *cause_counts);
 
+/* ---- Pathwise gradients (no counterpart in the reference) ---------------- */
+/* The objective per sample s is
+ *     J_s = sum_e weights[e] * max(0, realized[e][s] - earliest[e]),
+ * the weighted lateness of every event.  Its pathwise derivative follows the
+ * predecessor entry that decided each realized time in the propagation loop
+ * (_core.cpp:332-350): an event whose time is its earliest time (cause -1) or
+ * its cap earliest + max_delay passes its adjoint to its own earliest time;
+ * otherwise to the winning source (or to the source's earliest time when the
+ * loop read it before evaluating it) and, through the entry's duration, to
+ * the link's minimal duration d: d duration / d d = 1 for links whose type
+ * has no distribution or an empirical absolute one, 1 + X for the others
+ * (duration = d + X d), evaluated as duration * (1 / d), or by drawing X
+ * again when d = 0.  Holds almost everywhere: on a tie the later entry wins,
+ * as in the forward loop. */
+
+/* Replaces the minimal duration of every link (dmin[link_count()], indexed
+ * by Activity.idx; gaps included) and the earliest time of every event
+ * (earliest[node_count()]).  Either may be NULL to keep it.  The schedule is
+ * kept; the next launch uploads the new times.  Must not overlap a launch of
+ * this propagator. */
+int mcdp_propagator_set_times(mcdp_propagator *prop, const double *dmin, const double *earliest);
+/* The current times, in the layout of mcdp_propagator_set_times. */
+int mcdp_propagator_get_times(const mcdp_propagator *prop, double *dmin, double *earliest);
+
+/* The reverse walk over results of mcdp_run_many_device with the same seeds
+ * (event-major, leading dimension ld >= n; durations required).  Writes SUMS
+ * over the n samples: objective_dev[0] = sum_s J_s, grad_dmin_dev[l] =
+ * sum_s dJ_s/d dmin[l], grad_earliest_dev[e] = sum_s dJ_s/d earliest[e]; with
+ * accumulate != 0 it adds them to the outputs' current values.  Device
+ * pointers; asynchronous on `stream`.  The propagator's gradient workspace
+ * serves one call at a time: calls that overlap on different streams race. */
+int mcdp_reverse_device(mcdp_propagator *prop, const int32_t *seeds_dev, const double *weights_dev,
+                        const double *realized_dev, const double *durations_dev, const int32_t *cause_dev, int64_t n,
+                        int64_t ld, int accumulate, double *objective_dev, double *grad_dmin_dev,
+                        double *grad_earliest_dev, void *stream);
+/* Host convenience: run_many then the reverse walk over chunks of at most
+ * `chunk` samples (0: as many as fit a bounded workspace), so per-sample
+ * results never need to exist for all seeds at once.  Writes MEANS over the n
+ * seeds: objective[0], grad_dmin[link_count()], grad_earliest[node_count()].
+ * The result does not depend on the chunk size. */
+int mcdp_gradient_host(mcdp_propagator *prop, const int32_t *seeds, int64_t n, const double *weights, int64_t chunk,
+                       double *objective, double *grad_dmin, double *grad_earliest);
+
 /* ---- Device plumbing for callers without their own HIP allocator -------- */
 /* (no counterpart in the reference, which has no device) */
 int mcdp_device_count(int *count);

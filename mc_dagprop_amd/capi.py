@@ -74,6 +74,10 @@ _SIGNATURES = {
     "mcdp_batch_create": (C.c_int, [C.POINTER(_P), _i32, _i32, _i32, _i32, C.POINTER(_P)]),
     "mcdp_batch_destroy": (None, [_P]),
     "mcdp_batch_run_device": (C.c_int, [_P, C.POINTER(_P), _pi64, _pi64, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), _P]),
+    "mcdp_propagator_set_times": (C.c_int, [_P, _pf64, _pf64]),
+    "mcdp_propagator_get_times": (C.c_int, [_P, _pf64, _pf64]),
+    "mcdp_reverse_device": (C.c_int, [_P, _P, _P, _P, _P, _P, _i64, _i64, C.c_int, _P, _P, _P, _P]),
+    "mcdp_gradient_host": (C.c_int, [_P, _pi32, _i64, _pf64, _i64, _pf64, _pf64, _pf64]),
 }
 
 _lib = None
@@ -291,6 +295,48 @@ class Propagator:
         quant = d_quant.to_array(np.float64, (self.E, q.size)) if q.size else np.empty((self.E, 0))
         counts = d_cnt.to_array(np.int64, (int(off[-1]) + self.E,)) if cause is not None else None
         return mean, quant, counts
+
+    def set_times(self, dmin=None, earliest=None) -> None:
+        """Replaces the minimal duration of every link ([L], by Activity.idx) and/or the earliest time of
+        every event ([E]); the schedule is kept."""
+        d = None if dmin is None else np.ascontiguousarray(dmin, dtype=np.float64)
+        e = None if earliest is None else np.ascontiguousarray(earliest, dtype=np.float64)
+        if d is not None and d.shape != (self.L,):
+            raise ValueError(f"dmin must have shape ({self.L},)")
+        if e is not None and e.shape != (self.E,):
+            raise ValueError(f"earliest must have shape ({self.E},)")
+        check(lib().mcdp_propagator_set_times(self.h, None if d is None else _ptr(d, _pf64),
+                                              None if e is None else _ptr(e, _pf64)))
+
+    def get_times(self) -> tuple[np.ndarray, np.ndarray]:
+        """(minimal duration per link [L], earliest per event [E])."""
+        d = np.empty(self.L)
+        e = np.empty(self.E)
+        check(lib().mcdp_propagator_get_times(self.h, _ptr(d, _pf64), _ptr(e, _pf64)))
+        return d, e
+
+    def gradient(self, seeds, weights, chunk: int = 0) -> tuple[float, np.ndarray, np.ndarray]:
+        """Means over the seeds of J = sum_e w_e max(0, realized_e - earliest_e) and of its pathwise
+        derivatives: (J, dJ/d minimal_duration [L], dJ/d earliest [E])."""
+        s = np.ascontiguousarray(seeds, dtype=np.int32)
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        if w.shape != (self.E,):
+            raise ValueError(f"weights must have shape ({self.E},)")
+        obj = np.zeros(1)
+        gd = np.empty(self.L)
+        ge = np.empty(self.E)
+        check(lib().mcdp_gradient_host(self.h, _ptr(s, _pi32), s.size, _ptr(w, _pf64), chunk, _ptr(obj, _pf64),
+                                       _ptr(gd, _pf64), _ptr(ge, _pf64)))
+        return float(obj[0]), gd, ge
+
+    def reverse_device(self, seeds, weights, realized, durations, cause, n: int, ld: int, out_obj, out_dmin,
+                       out_earliest, accumulate: bool = False, stream=None) -> None:
+        """The reverse walk over device results of run_many_device; outputs are device buffers of sums."""
+        def p(b):
+            return b.ptr if b is not None else None
+
+        check(lib().mcdp_reverse_device(self.h, p(seeds), p(weights), p(realized), p(durations), p(cause), n, ld,
+                                        int(accumulate), p(out_obj), p(out_dmin), p(out_earliest), stream))
 
     def autotune(self, n_samples: int) -> tuple[int, int, int, float]:
         """Times every candidate shape with n_samples samples; keeps and returns

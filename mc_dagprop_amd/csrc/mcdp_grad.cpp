@@ -1,0 +1,478 @@
+// Pathwise gradients of the expected weighted lateness (C ABI in
+// include/mcdp.h): the reverse plan, its upload, and the entry points.  The
+// reference has no derivative of its propagation loop (_core.cpp:332-350);
+// this is the loop's adjoint along each event's winning entry, computed over
+// the outputs of run_many.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/mcdp.h"
+#include "mcdp_internal.hpp"
+
+using namespace mcdp;
+
+namespace {
+
+constexpr size_t kRevLdsBudget = 96 * 1024;          // LDS per workgroup for slots
+constexpr size_t kSlotBytes = (size_t)kLanes * 12;   // adjoint + winner position per lane
+constexpr size_t kPartialBudget = (size_t)1 << 30;   // bytes of partial rows per launch
+constexpr size_t kGradWorkspace = (size_t)32 << 30;  // per-sample buffers of mcdp_gradient_host
+
+size_t align_to(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// d duration / d minimal duration for link l (see mcdp.h).
+uint32_t mul_class(const mcdp_propagator &P, int32_t l, const Record &rec) {
+    if (l < 0 || l >= P.L) return kMulNone;
+    switch (rec.info & kInfoKindMask) {
+        case kNone:
+        case kEmpiricalAbs1:
+        case kEmpiricalAbs:
+            return kMulOne;
+        default:
+            return P.dmin[(size_t)l] != 0.0 ? kMulRead : kMulDraw;
+    }
+}
+
+// Waves per workgroup: enough to hold the widest level in one step.
+int32_t reverse_waves(const mcdp_propagator &P) {
+    size_t width = 0;
+    for (const auto &v : P.levels) width = std::max(width, v.size());
+    int32_t W = 1;
+    while (W < kMaxWaves && (size_t)W * kRevEventsPerWave < width) W *= 2;
+    return W;
+}
+
+int build_reverse(const mcdp_propagator &P, RevPlan &R) {
+    const int32_t E = P.E, L = P.L;
+    const int32_t W = reverse_waves(P);
+    const int32_t K = kRevEventsPerWave;
+    R = RevPlan{};
+    R.waves = W;
+
+    // successor lists: (target, position) for every entry that reads its source fresh
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> succ((size_t)E);
+    for (int32_t e = 0; e < E; ++e) {
+        const int64_t a = P.pred_off[(size_t)e], b = P.pred_off[(size_t)e + 1];
+        if (b - a > (int64_t)kRevNpredMask) return fail(MCDP_E_INVALID, "too many predecessors for the gradient");
+        for (int64_t k = a; k < b; ++k) {
+            const PredEntry &pe = P.preds[(size_t)k];
+            if (pe.stale) continue;
+            succ[(size_t)pe.src].emplace_back(e, (int32_t)(k - a));
+        }
+    }
+    auto cost = [&](int32_t e) {
+        return 2 + (int64_t)succ[(size_t)e].size() + 3 * (P.pred_off[(size_t)e + 1] - P.pred_off[(size_t)e]);
+    };
+
+    // (a) steps: levels in reverse order, each in as few steps as W x K events allow
+    std::vector<int32_t> ev_step((size_t)E, -1);
+    R.ev_off.assign(1, 0);
+    int32_t step = 0;
+    for (auto it = P.levels.rbegin(); it != P.levels.rend(); ++it) {
+        const auto &lev = *it;
+        if (lev.empty()) continue;
+        const size_t cap = (size_t)W * K;
+        const size_t nsteps = (lev.size() + cap - 1) / cap;
+        for (size_t c = 0; c < nsteps; ++c) {
+            const size_t c0 = lev.size() * c / nsteps, c1 = lev.size() * (c + 1) / nsteps;
+            std::vector<std::pair<int64_t, int32_t>> items;
+            for (size_t i = c0; i < c1; ++i) items.emplace_back(cost(lev[i]), lev[i]);
+            std::stable_sort(items.begin(), items.end(), [](auto &x, auto &y) { return x.first > y.first; });
+            std::vector<std::vector<int32_t>> per((size_t)W);
+            std::vector<int64_t> load((size_t)W, 0);
+            for (auto &itm : items) {
+                int best = -1;
+                for (int w = 0; w < W; ++w)
+                    if ((int)per[(size_t)w].size() < K && (best < 0 || load[(size_t)w] < load[(size_t)best])) best = w;
+                per[(size_t)best].push_back(itm.second);
+                load[(size_t)best] += itm.first;
+            }
+            for (int w = 0; w < W; ++w) {
+                for (int32_t e : per[(size_t)w]) {
+                    R.ev_list.push_back(e);
+                    ev_step[(size_t)e] = step;
+                }
+                R.ev_off.push_back((int32_t)R.ev_list.size());
+            }
+            ++step;
+        }
+    }
+    R.n_steps = step;
+
+    // (b) slots: an event whose adjoint some predecessor gathers holds a slot
+    // from its step to the last step of those predecessors (later in reverse).
+    std::vector<int32_t> last_use((size_t)E, -1);
+    for (int32_t e = 0; e < E; ++e)
+        for (int64_t k = P.pred_off[(size_t)e]; k < P.pred_off[(size_t)e + 1]; ++k) {
+            const PredEntry &pe = P.preds[(size_t)k];
+            if (!pe.stale) last_use[(size_t)e] = std::max(last_use[(size_t)e], ev_step[(size_t)pe.src]);
+        }
+    std::vector<int32_t> slot_of((size_t)E, -1);
+    {
+        std::vector<std::vector<int32_t>> release((size_t)std::max(R.n_steps, 1));
+        std::vector<int32_t> free_slots;
+        int32_t used = 0;
+        for (int32_t t = 0; t < R.n_steps; ++t) {
+            for (int32_t e : release[(size_t)t]) free_slots.push_back(slot_of[(size_t)e]);
+            // lowest free slots first, so the busiest slots stay in LDS
+            std::sort(free_slots.begin(), free_slots.end(), std::greater<int32_t>());
+            for (int32_t i = R.ev_off[(size_t)t * W]; i < R.ev_off[(size_t)(t + 1) * W]; ++i) {
+                const int32_t e = R.ev_list[(size_t)i];
+                if (last_use[(size_t)e] < 0) continue;
+                int32_t s;
+                if (!free_slots.empty()) {
+                    s = free_slots.back();
+                    free_slots.pop_back();
+                } else {
+                    s = used++;
+                }
+                slot_of[(size_t)e] = s;
+                release[(size_t)last_use[(size_t)e]].push_back(e);
+            }
+        }
+        R.slots = used;
+    }
+    for (int32_t e : R.ev_list) R.ev_slot.push_back(slot_of[(size_t)e]);
+
+    // (c) partial-row columns: link l's first fresh entry writes column l,
+    // events write L + e; other entries of a link, and stale entries (two
+    // columns: link, then the source's earliest time), get extra columns.
+    std::vector<int32_t> col((size_t)P.preds.size(), -1);
+    std::vector<std::vector<int32_t>> link_cols((size_t)L), event_cols((size_t)E);
+    std::vector<uint8_t> claimed((size_t)L, 0);
+    int32_t next = L + E;
+    for (int32_t e = 0; e < E; ++e)
+        for (int64_t k = P.pred_off[(size_t)e]; k < P.pred_off[(size_t)e + 1]; ++k) {
+            const PredEntry &pe = P.preds[(size_t)k];
+            const bool valid = pe.edge >= 0 && pe.edge < L;
+            if (pe.stale) {
+                col[(size_t)k] = next;
+                if (valid) link_cols[(size_t)pe.edge].push_back(next);
+                event_cols[(size_t)pe.src].push_back(next + 1);
+                next += 2;
+            } else if (valid) {
+                if (!claimed[(size_t)pe.edge]) {
+                    claimed[(size_t)pe.edge] = 1;
+                    col[(size_t)k] = pe.edge;
+                    link_cols[(size_t)pe.edge].insert(link_cols[(size_t)pe.edge].begin(), pe.edge);
+                } else {
+                    col[(size_t)k] = next;
+                    link_cols[(size_t)pe.edge].push_back(next++);
+                }
+            }
+            if (next > std::numeric_limits<int32_t>::max() - kMaxWaves)
+                return fail(MCDP_E_INVALID, "too many gradient columns");
+        }
+    R.obj_col = next;
+    R.n_cols = next + W;
+    R.map_off.assign(1, 0);
+    for (int32_t l = 0; l < L; ++l) {
+        for (int32_t c : link_cols[(size_t)l]) R.map_col.push_back(c);
+        R.map_off.push_back((int32_t)R.map_col.size());
+    }
+    for (int32_t e = 0; e < E; ++e) {
+        R.map_col.push_back(L + e);
+        for (int32_t c : event_cols[(size_t)e]) R.map_col.push_back(c);
+        R.map_off.push_back((int32_t)R.map_col.size());
+    }
+    for (int32_t w = 0; w < W; ++w) R.map_col.push_back(R.obj_col + w);
+    R.map_off.push_back((int32_t)R.map_col.size());
+
+    // (d) records per (step, wave): header, successors, predecessors
+    R.step_off.assign(1, 0);
+    for (size_t g = 0; g + 1 < R.ev_off.size(); ++g) {
+        for (int32_t i = R.ev_off[g]; i < R.ev_off[g + 1]; ++i) {
+            const int32_t e = R.ev_list[(size_t)i];
+            const int64_t a = P.pred_off[(size_t)e], b = P.pred_off[(size_t)e + 1];
+            Record h{};
+            h.a = P.earliest[(size_t)e];
+            h.b = P.earliest[(size_t)e] + P.max_delay;
+            h.i0 = e;
+            h.i1 = slot_of[(size_t)e];
+            h.i2 = (int32_t)succ[(size_t)e].size();
+            h.info = kInfoHeader | (uint32_t)(b - a);
+            R.recs.push_back(h);
+            for (const auto &sp : succ[(size_t)e]) {
+                Record r{};
+                r.i0 = sp.first;
+                r.i1 = slot_of[(size_t)sp.first];
+                r.i2 = sp.second;
+                R.recs.push_back(r);
+            }
+            std::vector<int32_t> srcs;
+            for (int64_t k = a; k < b; ++k) srcs.push_back(P.preds[(size_t)k].src);
+            std::sort(srcs.begin(), srcs.end());
+            for (int64_t k = a; k < b; ++k) {
+                const PredEntry &pe = P.preds[(size_t)k];
+                const auto range = std::equal_range(srcs.begin(), srcs.end(), pe.src);
+                const int64_t dups = range.second - range.first;
+                Record r = link_record(P, pe.edge);
+                const uint32_t cls = mul_class(P, pe.edge, r);
+                r.a = cls == kMulRead ? 1.0 / P.dmin[(size_t)pe.edge] : 0.0;
+                r.i0 = pe.src;
+                r.i1 = col[(size_t)k];
+                r.i2 = pe.edge;
+                r.info = (r.info & ~(uint32_t)(kRevClassMask | kRevDup | kRevStale | kInfoOwner | kInfoLast)) |
+                         (cls << kRevClassShift) | (dups > 1 ? kRevDup : 0u) | (pe.stale ? kRevStale : 0u);
+                if (cls == kMulDraw) R.needs_seeds = true;
+                R.recs.push_back(r);
+            }
+        }
+        R.step_off.push_back((int32_t)R.recs.size());
+    }
+    if (R.recs.size() > (size_t)std::numeric_limits<int32_t>::max()) return fail(MCDP_E_INVALID, "plan too large");
+    return MCDP_OK;
+}
+
+// The reverse plan on the device of `d` (built and uploaded on first use).
+int ensure_reverse(mcdp_propagator &P, DeviceCopy &d) {
+    if (!P.rev) {
+        auto R = std::make_shared<RevPlan>();
+        int rc = build_reverse(P, *R);
+        if (rc) return rc;
+        P.rev = R;
+    }
+    if (d.rev) return MCDP_OK;
+    const RevPlan &R = *P.rev;
+    size_t off = 0;
+    auto reserve = [&](size_t bytes) {
+        const size_t o = off;
+        off = align_to(off + bytes, 256);
+        return o;
+    };
+    const size_t o_step = reserve(R.step_off.size() * 4), o_recs = reserve(R.recs.size() * sizeof(Record));
+    const size_t o_evoff = reserve(R.ev_off.size() * 4), o_evl = reserve(R.ev_list.size() * 4);
+    const size_t o_evs = reserve(R.ev_slot.size() * 4), o_moff = reserve(R.map_off.size() * 4);
+    const size_t o_mcol = reserve(R.map_col.size() * 4);
+    const size_t total = std::max<size_t>(off, 256);
+    std::vector<unsigned char> host(total, 0);
+    auto put = [&](size_t o, const void *src, size_t bytes) {
+        if (bytes) std::memcpy(host.data() + o, src, bytes);
+    };
+    put(o_step, R.step_off.data(), R.step_off.size() * 4);
+    put(o_recs, R.recs.data(), R.recs.size() * sizeof(Record));
+    put(o_evoff, R.ev_off.data(), R.ev_off.size() * 4);
+    put(o_evl, R.ev_list.data(), R.ev_list.size() * 4);
+    put(o_evs, R.ev_slot.data(), R.ev_slot.size() * 4);
+    put(o_moff, R.map_off.data(), R.map_off.size() * 4);
+    put(o_mcol, R.map_col.data(), R.map_col.size() * 4);
+    void *blob = nullptr;
+    hipError_t e = hipMalloc(&blob, total);
+    if (e != hipSuccess) return fail(MCDP_E_NOMEM, std::string("hipMalloc (reverse plan): ") + hipGetErrorString(e));
+    e = hipMemcpy(blob, host.data(), total, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(blob);
+        return hip_fail(e, "hipMemcpy (reverse plan)");
+    }
+    auto *b = static_cast<unsigned char *>(blob);
+    d.rev = blob;
+    RevParams &p = d.rev_base;
+    p = RevParams{};
+    p.step_off = reinterpret_cast<const int32_t *>(b + o_step);
+    p.recs = reinterpret_cast<const Record *>(b + o_recs);
+    p.ev_off = reinterpret_cast<const int32_t *>(b + o_evoff);
+    p.ev_list = reinterpret_cast<const int32_t *>(b + o_evl);
+    p.ev_slot = reinterpret_cast<const int32_t *>(b + o_evs);
+    p.dists = d.base.dists;
+    p.alias = d.base.alias;
+    p.earliest = d.base.earliest;
+    p.n_steps = R.n_steps;
+    p.waves = R.waves;
+    p.lds_slots = std::min<int32_t>(R.slots, (int32_t)(kRevLdsBudget / kSlotBytes));
+    p.global_slots = R.slots - p.lds_slots;
+    p.L = P.L;
+    p.n_cols = R.n_cols;
+    p.obj_col = R.obj_col;
+    d.rev_map_off = reinterpret_cast<const int32_t *>(b + o_moff);
+    d.rev_map_col = reinterpret_cast<const int32_t *>(b + o_mcol);
+    return MCDP_OK;
+}
+
+// Grows the propagator's gradient workspace on `d` to `bytes`.
+int ensure_workspace(DeviceCopy &d, size_t bytes) {
+    if (d.rev_work && d.rev_work_bytes >= bytes) return MCDP_OK;
+    if (d.rev_work) {
+        (void)hipDeviceSynchronize();  // earlier launches may still use it
+        (void)hipFree(d.rev_work);
+        d.rev_work = nullptr;
+        d.rev_work_bytes = 0;
+    }
+    hipError_t e = hipMalloc(&d.rev_work, bytes);
+    if (e != hipSuccess) {
+        d.rev_work = nullptr;
+        return fail(MCDP_E_NOMEM, std::string("hipMalloc (gradient workspace): ") + hipGetErrorString(e));
+    }
+    d.rev_work_bytes = bytes;
+    return MCDP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mcdp_propagator_set_times(mcdp_propagator *prop, const double *dmin, const double *earliest) {
+    if (!prop) return fail(MCDP_E_INVALID, "propagator is null");
+    std::lock_guard<std::mutex> lock(prop->mu);
+    mcdp_propagator &P = *prop;
+    if (dmin) std::copy(dmin, dmin + P.L, P.dmin.begin());
+    if (earliest) std::copy(earliest, earliest + P.E, P.earliest.begin());
+    for (Record &r : P.recs) {
+        if (r.info & kInfoHeader) {
+            r.a = P.earliest[(size_t)r.i0];
+            r.b = P.earliest[(size_t)r.i0] + P.max_delay;
+        } else if (r.info & kInfoEdgeValid) {
+            r.a = P.dmin[(size_t)r.i2];
+        }
+    }
+    for (Record &r : P.orph) r.a = P.dmin[(size_t)r.i2];
+    P.rev.reset();
+    P.drop_device_copies();
+    return MCDP_OK;
+}
+
+int mcdp_propagator_get_times(const mcdp_propagator *prop, double *dmin, double *earliest) {
+    if (!prop) return fail(MCDP_E_INVALID, "propagator is null");
+    if (dmin) std::copy(prop->dmin.begin(), prop->dmin.end(), dmin);
+    if (earliest) std::copy(prop->earliest.begin(), prop->earliest.end(), earliest);
+    return MCDP_OK;
+}
+
+int mcdp_reverse_device(mcdp_propagator *prop, const int32_t *seeds_dev, const double *weights_dev,
+                        const double *realized_dev, const double *durations_dev, const int32_t *cause_dev, int64_t n,
+                        int64_t ld, int accumulate, double *objective_dev, double *grad_dmin_dev,
+                        double *grad_earliest_dev, void *stream) {
+    if (!prop) return fail(MCDP_E_INVALID, "propagator is null");
+    if (n < 0 || (n > 0 && ld < n)) return fail(MCDP_E_INVALID, "need 0 <= n <= ld");
+    if (n > std::numeric_limits<int32_t>::max() - 256 || ld > std::numeric_limits<int32_t>::max())
+        return fail(MCDP_E_INVALID, "n and ld must be below 2^31 - 256 per call");
+    if (!objective_dev || (!grad_dmin_dev && prop->L > 0) || (!grad_earliest_dev && prop->E > 0))
+        return fail(MCDP_E_INVALID, "gradient outputs are null");
+    if (n > 0 && prop->E > 0 && (!weights_dev || !realized_dev || !cause_dev || (!durations_dev && prop->L > 0)))
+        return fail(MCDP_E_INVALID, "weights, realized, durations and cause are required");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return fail(MCDP_E_DEVICE, "no HIP device available: gradients run on the GPU only");
+    std::lock_guard<std::mutex> lock(prop->mu);
+    mcdp_propagator &P = *prop;
+    DeviceCopy *d = nullptr;
+    int rc = ensure_device(P, d);
+    if (rc) return rc;
+    rc = ensure_reverse(P, *d);
+    if (rc) return rc;
+    if (n > 0 && P.rev->needs_seeds && !seeds_dev)
+        return fail(MCDP_E_INVALID, "seeds are required: a relative link has minimal duration 0");
+    const auto s = static_cast<hipStream_t>(stream);
+    const RevParams &base = d->rev_base;
+    const int64_t tiles = (n + kLanes - 1) / kLanes;
+    const size_t row_bytes = (size_t)base.n_cols * sizeof(double);
+    const int64_t max_tiles = std::max<int64_t>(1, (int64_t)(kPartialBudget / row_bytes));
+    const size_t lds = (size_t)base.lds_slots * kSlotBytes;
+    int cus = 0, bpc = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device), "hipDeviceGetAttribute");
+    HIP_TRY(reverse_occupancy(base.waves, lds, &bpc), "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    if (bpc < 1) return fail(MCDP_E_DEVICE, "reverse kernel cannot be resident with " + std::to_string(lds) + " bytes of LDS");
+    const int64_t resident = (int64_t)cus * bpc;
+    const int64_t chunk_tiles = std::min<int64_t>(std::max<int64_t>(tiles, 1), max_tiles);
+    const int64_t grid_max = std::min<int64_t>(chunk_tiles, resident);
+    const size_t o_gslots = align_to(row_bytes * (size_t)chunk_tiles, 256);
+    const size_t gslot_bytes = (size_t)grid_max * base.global_slots * kLanes * 2 * sizeof(double);
+    rc = ensure_workspace(*d, o_gslots + std::max<size_t>(gslot_bytes, 256));
+    if (rc) return rc;
+    auto *work = static_cast<unsigned char *>(d->rev_work);
+    RevParams p = base;
+    p.seeds = seeds_dev;
+    p.weights = weights_dev;
+    p.realized = realized_dev;
+    p.durations = durations_dev;
+    p.cause = cause_dev;
+    p.n = (int32_t)n;
+    p.ld = (int32_t)std::max<int64_t>(ld, 1);
+    p.partial = reinterpret_cast<double *>(work);
+    p.gslots = reinterpret_cast<double *>(work + o_gslots);
+    if (tiles == 0 || P.E == 0) {  // no samples: sums are 0 (or unchanged)
+        HIP_TRY(launch_reduce_columns(p.partial, 0, p.n_cols, d->rev_map_off, d->rev_map_col, P.L, P.E, accumulate,
+                                      grad_dmin_dev, grad_earliest_dev, objective_dev, s),
+                "reduce kernel launch");
+        return MCDP_OK;
+    }
+    for (int64_t t0 = 0; t0 < tiles; t0 += chunk_tiles) {
+        p.tile0 = (int32_t)t0;
+        p.n_tiles = (int32_t)std::min<int64_t>(chunk_tiles, tiles - t0);
+        const int grid = (int)std::min<int64_t>(p.n_tiles, resident);
+        HIP_TRY(launch_reverse(p, grid, lds, s), "reverse kernel launch");
+        HIP_TRY(launch_reduce_columns(p.partial, p.n_tiles, p.n_cols, d->rev_map_off, d->rev_map_col, P.L, P.E,
+                                      accumulate || t0 > 0, grad_dmin_dev, grad_earliest_dev, objective_dev, s),
+                "reduce kernel launch");
+    }
+    return MCDP_OK;
+}
+
+int mcdp_gradient_host(mcdp_propagator *prop, const int32_t *seeds, int64_t n, const double *weights, int64_t chunk,
+                       double *objective, double *grad_dmin, double *grad_earliest) {
+    if (!prop) return fail(MCDP_E_INVALID, "propagator is null");
+    if (n < 1 || !seeds) return fail(MCDP_E_INVALID, "a gradient needs at least one seed");
+    if (!weights || !objective || (!grad_dmin && prop->L > 0) || (!grad_earliest && prop->E > 0))
+        return fail(MCDP_E_INVALID, "null argument");
+    if (chunk < 0) return fail(MCDP_E_INVALID, "chunk must be >= 0");
+    const int64_t E = prop->E, L = prop->L;
+    const int64_t per_sample = 12 * E + 8 * L + 4;
+    if (chunk == 0) chunk = std::max<int64_t>(kLanes, (int64_t)(kGradWorkspace / (size_t)std::max<int64_t>(per_sample, 1)));
+    chunk = std::min<int64_t>(chunk, std::numeric_limits<int32_t>::max() - 256);
+    chunk = std::max<int64_t>(kLanes, chunk / kLanes * kLanes);  // whole tiles: sums do not depend on the chunking
+    chunk = std::min<int64_t>(chunk, (n + kLanes - 1) / kLanes * kLanes);
+    const size_t o_seeds = 0, o_real = align_to((size_t)chunk * 4, 256);
+    const size_t o_dur = o_real + align_to((size_t)(E * chunk) * 8, 256);
+    const size_t o_cause = o_dur + align_to((size_t)(L * chunk) * 8, 256);
+    const size_t o_w = o_cause + align_to((size_t)(E * chunk) * 4, 256);
+    const size_t o_out = o_w + align_to((size_t)E * 8, 256);
+    const size_t bytes = o_out + (size_t)(1 + L + E) * 8;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return fail(MCDP_E_DEVICE, "no HIP device available: gradients run on the GPU only");
+    void *buf = nullptr;
+    hipError_t e = hipMalloc(&buf, bytes);
+    if (e != hipSuccess) return fail(MCDP_E_NOMEM, std::string("hipMalloc (gradient): ") + hipGetErrorString(e));
+    auto *b = static_cast<unsigned char *>(buf);
+    auto *d_seeds = reinterpret_cast<int32_t *>(b + o_seeds);
+    auto *d_real = reinterpret_cast<double *>(b + o_real);
+    auto *d_dur = reinterpret_cast<double *>(b + o_dur);
+    auto *d_cause = reinterpret_cast<int32_t *>(b + o_cause);
+    auto *d_w = reinterpret_cast<double *>(b + o_w);
+    auto *d_obj = reinterpret_cast<double *>(b + o_out);
+    double *d_gd = d_obj + 1, *d_ge = d_obj + 1 + L;
+    int rc = MCDP_OK;
+    if (E > 0 && (e = hipMemcpy(d_w, weights, (size_t)E * 8, hipMemcpyHostToDevice)) != hipSuccess)
+        rc = hip_fail(e, "hipMemcpy");
+    for (int64_t c0 = 0; c0 < n && !rc; c0 += chunk) {
+        const int64_t nc = std::min<int64_t>(chunk, n - c0);
+        if ((e = hipMemcpy(d_seeds, seeds + c0, (size_t)nc * 4, hipMemcpyHostToDevice)) != hipSuccess) {
+            rc = hip_fail(e, "hipMemcpy");
+            break;
+        }
+        rc = mcdp_run_many_device(prop, d_seeds, nc, nc, d_real, d_dur, d_cause, nullptr);
+        if (!rc)
+            rc = mcdp_reverse_device(prop, d_seeds, d_w, d_real, d_dur, d_cause, nc, nc, c0 > 0, d_obj, d_gd, d_ge,
+                                     nullptr);
+        // the next chunk's seeds overwrite this chunk's
+        if (!rc && (e = hipDeviceSynchronize()) != hipSuccess) rc = hip_fail(e, "gradient");
+    }
+    std::vector<double> out((size_t)(1 + L + E));
+    if (!rc && (e = hipMemcpy(out.data(), d_obj, out.size() * 8, hipMemcpyDeviceToHost)) != hipSuccess)
+        rc = hip_fail(e, "hipMemcpy");
+    (void)hipFree(buf);
+    if (rc) return rc;
+    const double dn = (double)n;
+    objective[0] = out[0] / dn;
+    for (int64_t l = 0; l < L; ++l) grad_dmin[l] = out[(size_t)(1 + l)] / dn;
+    for (int64_t i = 0; i < E; ++i) grad_earliest[i] = out[(size_t)(1 + L + i)] / dn;
+    return MCDP_OK;
+}
+
+}  // extern "C"

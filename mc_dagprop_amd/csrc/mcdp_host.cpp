@@ -906,6 +906,7 @@ size_t mcdp::plan_lds_bytes(const mcdp_propagator &P, WalkVariant v) { return wa
 int mcdp::configure_shape(mcdp_propagator *prop, int32_t waves, int32_t k, int32_t spl) {
     return mcdp_propagator_configure(prop, waves, k, spl);
 }
+mcdp::Record mcdp::link_record(const mcdp_propagator &P, int32_t l) { return edge_record(P, l); }
 
 // ============================================================================
 // C ABI

@@ -5,11 +5,13 @@
 #include <hip/hip_runtime_api.h>
 #include <stdint.h>
 
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/mcdp.h"
+#include "mcdp_grad.hpp"
 #include "mcdp_kernels.hpp"
 #include "mcdp_plan.hpp"
 
@@ -42,6 +44,23 @@ struct DeviceCopy {
     const int32_t *pred_key = nullptr;   // [P] per event: its distinct sources, ascending
     const int32_t *pred_pos = nullptr;   // [P] first position of pred_key in the event's list
     const int32_t *pred_nkey = nullptr;  // [E] distinct sources per event
+    void *rev = nullptr;                 // gradients: reverse plan, uploaded on first use
+    RevParams rev_base{};
+    const int32_t *rev_map_off = nullptr;  // [L + E + 2] columns of each output
+    const int32_t *rev_map_col = nullptr;
+    void *rev_work = nullptr;            // partial rows and global slots
+    size_t rev_work_bytes = 0;
+};
+
+// The reverse walk's plan (mcdp_grad.cpp): the forward levels in reverse
+// order, split into steps for `waves` waves of at most kRevEventsPerWave
+// events, with the records, slots and partial-row columns of each event.
+struct RevPlan {
+    int32_t waves = 1, n_steps = 0, slots = 0, n_cols = 0, obj_col = 0;
+    bool needs_seeds = false;  // some entry draws its multiplier again
+    std::vector<int32_t> step_off, ev_off, ev_list, ev_slot;
+    std::vector<Record> recs;
+    std::vector<int32_t> map_off, map_col;  // outputs: L links, E events, the objective
 };
 
 // One predecessor entry as the reference evaluates it.
@@ -73,6 +92,7 @@ struct mcdp_propagator {
     int32_t spl = 1;  // samples per lane of the walk kernel (schedule-independent)
     int64_t n_global = 0;
     bool shape_fixed = false;  // set by configure / autotune / the first launch
+    std::shared_ptr<mcdp::RevPlan> rev;  // built on the first gradient call
     // device copies, one per device used
     std::mutex mu;
     std::vector<mcdp::DeviceCopy> dev;
@@ -85,6 +105,8 @@ struct mcdp_propagator {
             (void)hipDeviceSynchronize();  // launches in flight on any stream may still read the plan
             if (d.blob) (void)hipFree(d.blob);
             if (d.topo) (void)hipFree(d.topo);
+            if (d.rev) (void)hipFree(d.rev);
+            if (d.rev_work) (void)hipFree(d.rev_work);
             if (cur >= 0) (void)hipSetDevice(cur);
         }
         dev.clear();
@@ -101,4 +123,7 @@ WalkVariant plan_variant(const mcdp_propagator &P, bool injected);
 size_t plan_lds_bytes(const mcdp_propagator &P, WalkVariant v);
 // mcdp_propagator_configure.
 int configure_shape(mcdp_propagator *prop, int32_t waves, int32_t k, int32_t spl);
+// The forward plan's record of link l (kind, flags, distribution index,
+// minimal duration, p0); an edge outside the link table is kind kNone.
+Record link_record(const mcdp_propagator &P, int32_t l);
 }  // namespace mcdp

@@ -13,6 +13,9 @@ follow and what pins them):
   own test KATs.
 * :func:`philox4x32` / :func:`philox2x32` -- Philox-4x32-10 / -2x32-10, pinned by
   the Random123 known-answer vectors.
+* :meth:`OracleCtx.gradient` -- ``orc_gradient``: the pathwise gradient of the
+  expected weighted lateness, pinned by finite differences of the oracle's own
+  propagation and by closed forms.
 * :class:`RefSim` -- the reference engine itself, compiled from its sources by
   ``oracle/Makefile`` into ``oracle/_ref/`` (the stream-exact durations of
   Xoshiro256++ + libstdc++ come only from here).
@@ -62,6 +65,10 @@ def oracle_lib() -> C.CDLL:
         L.orc_propagate.argtypes = [_P, _pf64, C.c_int64, C.c_int64, _pf64, _pi32]
         L.orc_philox4x32_many.argtypes = [C.c_int, C.c_int64, _pu32, _pu32, _pu32]
         L.orc_philox2x32_many.argtypes = [C.c_int, C.c_int64, _pu32, _pu32, _pu32]
+        L.orc_get_links.argtypes = [_P, _pf64, _pi32]
+        L.orc_gradient.restype = C.c_int
+        L.orc_gradient.argtypes = [_P, _pf64, _pi32, _pf64, C.c_int64, _pf64, C.c_int64, _pf64, _pi32, _pf64, _pf64,
+                                   _pf64, C.c_char_p, C.c_int]
         _olib = L
     return _olib
 
@@ -135,6 +142,44 @@ class OracleCtx:
         c = np.empty((n, self.E), dtype=np.int32)
         oracle_lib().orc_propagate(self.h, _ptr(d, _pf64), d.shape[1], n, _ptr(r, _pf64), _ptr(c, _pi32))
         return r, c
+
+    def links(self) -> tuple[np.ndarray, np.ndarray]:
+        """The compiled link table: (minimal duration [L], activity type [L])."""
+        d = np.empty(self.L)
+        t = np.empty(self.L, dtype=np.int32)
+        oracle_lib().orc_get_links(self.h, _ptr(d, _pf64), _ptr(t, _pi32))
+        return d, t
+
+    def link_relative(self) -> np.ndarray:
+        """1 for links whose duration is d + X*d (constant, exponential, gamma, empirical relative), 0 for
+        d + X (empirical absolute) or d (no distribution)."""
+        last = {}
+        for d in self.dag.dists:
+            last[d.activity_type] = d.kind
+        _, types = self.links()
+        rel = {t for t, k in last.items() if k != 3}  # 3: empirical absolute
+        return np.asarray([1 if int(t) in rel else 0 for t in types], dtype=np.int32)
+
+    def gradient(self, realized, cause, durations, weights, mult1=None):
+        """Sums over samples of (J_s, dJ_s/d minimal_duration [L], dJ_s/d earliest [E]), with
+        J_s = sum_e w_e max(0, realized_e - earliest_e); inputs sample-major (n, E) / (n, L)."""
+        r = np.ascontiguousarray(realized, dtype=np.float64)
+        c = np.ascontiguousarray(cause, dtype=np.int32)
+        d = np.ascontiguousarray(durations, dtype=np.float64)
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        m1 = None if mult1 is None else np.ascontiguousarray(mult1, dtype=np.float64)
+        rel = self.link_relative()
+        n = r.shape[0]
+        obj = np.zeros(1)
+        gd = np.empty(self.L)
+        ge = np.empty(self.E)
+        err = C.create_string_buffer(256)
+        rc = oracle_lib().orc_gradient(self.h, _ptr(r, _pf64), _ptr(c, _pi32), _ptr(d, _pf64), d.shape[1] if n else 0,
+                                       None if m1 is None else _ptr(m1, _pf64), n, _ptr(w, _pf64), _ptr(rel, _pi32),
+                                       _ptr(obj, _pf64), _ptr(gd, _pf64), _ptr(ge, _pf64), err, 256)
+        if rc:
+            raise OracleError(rc, err.value.decode())
+        return float(obj[0]), gd, ge
 
     def __del__(self) -> None:  # pragma: no cover
         try:

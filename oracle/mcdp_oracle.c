@@ -22,6 +22,13 @@
  *   orc_philox4x32 / orc_philox2x32  Philox-4x32-R and Philox-2x32-R
  *                   (Salmon, Moraes, Dror, Shaw, SC'11).  Pinned by the
  *                   Random123 known-answer vectors.
+ *   orc_gradient    pathwise gradient of the expected weighted lateness
+ *                   with respect to minimal durations and earliest times
+ *                   (no counterpart in the reference: the derivative of
+ *                   _core.cpp:332-350).  Pinned by central finite
+ *                   differences of orc_propagate itself on random DAGs with
+ *                   caps, stale reads, repeated sources and invalid edges,
+ *                   and by closed forms on chains and diamonds.
  *
  * Behaviour the reference leaves undefined is given the definition the GPU
  * library uses (each one is listed where it applies):
@@ -329,4 +336,125 @@ void orc_philox4x32_many(int rounds, int64_t n, const uint32_t *ctr, const uint3
 }
 void orc_philox2x32_many(int rounds, int64_t n, const uint32_t *ctr, const uint32_t *key, uint32_t *out) {
     for (int64_t i = 0; i < n; ++i) orc_philox2x32(rounds, ctr + 2 * i, key[i], out + 2 * i);
+}
+
+/* The link table as compiled: minimal duration and activity type per link. */
+void orc_get_links(const orc_ctx *c, double *dmin, int *type) {
+    memcpy(dmin, c->dmin, sizeof(double) * (size_t)c->L);
+    memcpy(type, c->type, sizeof(int) * (size_t)c->L);
+}
+
+static int same_bits(double a, double b) { return memcmp(&a, &b, sizeof(double)) == 0; }
+
+/*
+ * Pathwise gradient of the expected weighted lateness, for n samples.
+ *
+ * There is no counterpart in the reference; this restates the derivative of
+ * its propagation loop (_core.cpp:332-350) along the path that decided each
+ * realized time.  Per sample s the objective is
+ *     J_s = sum_e w[e] * max(0, realized[e] - earliest[e]),
+ * and realized[e] is a max-plus expression of earliest times and durations.
+ * Its derivative, where it exists, follows the recomputed winner of each
+ * event (the last predecessor entry whose capped candidate reaches the
+ * maximum, as the forward loop's `t >= latest` selects):
+ *   - no entry beats earliest[e] (cause -1), or realized[e] equals the cap
+ *     earliest[e] + max_delay: d realized[e] = d earliest[e];
+ *   - otherwise realized[e] = base + duration, where base is realized[src]
+ *     (or earliest[src] for an entry the forward loop read before src was
+ *     evaluated: the reference's stale read) and d duration / d minimal
+ *     duration is 1 for links whose type has no distribution or an empirical
+ *     absolute one (duration = d + X), and 1 + X for the other types
+ *     (duration = d + X*d), computed as duration * (1/d).  When d is 0 the
+ *     multiplier comes from mult1 (the duration the same draw gives at d = 1).
+ * Adjoints flow in reverse evaluation order.  The function also checks that
+ * realized and cause are what the forward loop gives for these durations
+ * (error 6 otherwise).
+ *
+ * Inputs are sample-major, as orc_propagate writes them: realized, cause
+ * [n][E]; durations and mult1 [n][ld_dur].  link_rel[l] is 1 for the
+ * duration = d + X*d types.  Outputs are SUMS over the n samples.
+ */
+int orc_gradient(const orc_ctx *c, const double *realized, const int *cause, const double *durations,
+                 int64_t ld_dur, const double *mult1, int64_t n, const double *weights, const int *link_rel,
+                 double *obj, double *grad_dmin, double *grad_earliest, char *err, int errlen) {
+    const int E = c->E, L = c->L;
+    int *pos = (int *)malloc(sizeof(int) * (size_t)(E > 0 ? E : 1));
+    double *g = (double *)malloc(sizeof(double) * (size_t)(E > 0 ? E : 1));
+    if (!pos || !g) {
+        free(pos);
+        free(g);
+        set_err(err, errlen, "out of memory");
+        return 5;
+    }
+    for (int i = 0; i < E; ++i) pos[c->order[i]] = i;
+    *obj = 0.0;
+    for (int l = 0; l < L; ++l) grad_dmin[l] = 0.0;
+    for (int e = 0; e < E; ++e) grad_earliest[e] = 0.0;
+    int rc = 0;
+    for (int64_t s = 0; s < n && !rc; ++s) {
+        const double *R = realized + s * (int64_t)E;
+        const int *Cs = cause + s * (int64_t)E;
+        const double *D = durations + s * ld_dur;
+        for (int e = 0; e < E; ++e) g[e] = 0.0;
+        for (int oi = E - 1; oi >= 0; --oi) {
+            const int e = c->order[oi];
+            const double r = R[e], ea = c->earliest[e];
+            const double upper = ea + c->max_delay;
+            const int active = r > ea;
+            const double gt = g[e] + (active ? weights[e] : 0.0);
+            if (active) {
+                *obj += weights[e] * (r - ea);
+                grad_earliest[e] -= weights[e];
+            }
+            /* the forward loop's winner, recomputed */
+            double latest = ea;
+            int64_t wk = -1;
+            for (int64_t k = c->off[e]; k < c->off[e + 1]; ++k) {
+                const int src = c->src[k], edge = c->edge[k];
+                const double base = (pos[src] > pos[e]) ? c->earliest[src] : R[src];
+                const double d = (edge >= 0 && edge < L) ? D[edge] : 0.0;
+                double t = base + d;
+                t = (upper < t) ? upper : t;
+                if (t >= latest) {
+                    latest = t;
+                    wk = k;
+                }
+            }
+            const double rr = (upper < latest) ? upper : latest;
+            if ((wk >= 0 ? c->src[wk] : -1) != Cs[e] || !same_bits(rr, r)) {
+                set_err(err, errlen, "realized/cause do not follow from these durations");
+                rc = 6;
+                break;
+            }
+            if (wk < 0 || r == upper) {
+                grad_earliest[e] += gt;
+                continue;
+            }
+            const int src = c->src[wk], edge = c->edge[wk];
+            if (pos[src] > pos[e])
+                grad_earliest[src] += gt;
+            else
+                g[src] += gt;
+            if (edge >= 0 && edge < L) {
+                double m = 1.0;
+                if (link_rel[edge]) {
+                    if (c->dmin[edge] != 0.0) {
+                        const double inv = 1.0 / c->dmin[edge];
+                        m = D[edge] * inv;
+                    } else {
+                        if (!mult1) {
+                            set_err(err, errlen, "a relative link with minimal duration 0 needs mult1");
+                            rc = 7;
+                            break;
+                        }
+                        m = mult1[s * ld_dur + edge];
+                    }
+                }
+                grad_dmin[edge] += gt * m;
+            }
+        }
+    }
+    free(pos);
+    free(g);
+    return rc;
 }

@@ -180,6 +180,33 @@ STD_DISTS = [
 ]
 
 
+GRAD_DISTS = STD_DISTS + [DistSpec(4, EMPIRICAL_ABSOLUTE, values=np.array([0.0, 2.0, 5.0]), weights=np.array([1.0, 1.0, 2.0]))]
+
+
+def continuous_dag(seed: int, n_events: int, *, max_delay: float, shared_links: bool) -> FlatDag:
+    """random_dag's structure (repeated targets, gaps, out-of-range edges) with continuous times, plus entries
+    that repeat a source with another link and entries that read a link another entry reads too."""
+    rng = np.random.default_rng(seed)
+    d = random_dag(rng, n_events, duplicates=True, gaps=True, oob_edges=True, n_types=5, max_delay=max_delay,
+                   dists=GRAD_DISTS)
+    d.earliest = rng.uniform(0.0, 40.0, d.earliest.size)
+    d.act_dur = rng.uniform(1.0, 20.0, d.act_dur.size)
+    if shared_links and d.act_idx.size:
+        src, edge = list(d.prec_src), list(d.prec_edge)
+        off = list(d.prec_off)
+        for j in range(len(d.prec_target)):
+            a, b = off[j], off[j + 1]
+            if b > a and rng.random() < 0.3:  # repeat this entry's first source with some other link
+                ins = b
+                src.insert(ins, src[a])
+                edge.insert(ins, int(rng.choice(d.act_idx)))
+                off = off[: j + 1] + [o + 1 for o in off[j + 1:]]
+        d.prec_src = np.asarray(src, dtype=np.int32)
+        d.prec_edge = np.asarray(edge, dtype=np.int32)
+        d.prec_off = np.asarray(off, dtype=np.int64)
+    return d
+
+
 # ---- fixture (de)serialisation ------------------------------------------------
 _ARRAYS = ("earliest", "latest", "actual", "act_src", "act_dst", "act_idx", "act_dur", "act_type", "prec_target",
            "prec_off", "prec_src", "prec_edge")
