@@ -21,7 +21,11 @@ Also reported:
   check         device durations of a few samples fed to the CPU oracle
                 reproduce the device's realized / cause bit-exactly.
 
-    python bench.py [--steps K] [--warmup W] [--samples S] [--workload network|small]
+    python bench.py [--steps K] [--warmup W] [--samples S] [--workload network|small|batch|grad]
+
+--workload grad times the pathwise-gradient step (run_many + the reverse walk) on the
+network DAG with 65,536 samples, with the reverse walk's roofline and a check of its
+gradient against the FP64 CPU oracle.
 """
 
 from __future__ import annotations
@@ -154,15 +158,147 @@ def bench_batch(args) -> int:
     return 0 if ok else 3
 
 
+def bench_grad(args) -> int:
+    """Pathwise gradient of the expected weighted lateness on the network DAG: one step is run_many over
+    the samples (full outputs in HBM) followed by the reverse walk over them (per-link and per-event sums)."""
+    from mc_dagprop_amd import capi
+    from mc_dagprop_amd.workloads import EMPIRICAL_ABSOLUTE, SplitMix64, layered_network
+
+    dag = layered_network()
+    prop = capi.Propagator(dag)
+    E, L = prop.E, prop.L
+    S = args.samples if args.samples_given else 65536
+    rng = SplitMix64(20260004)
+    w_h = np.asarray([rng.uniform(0.5, 2.0) for _ in range(E)])
+    if args.shape:
+        prop.configure(*map(int, args.shape.split(",")))
+    elif args.autotune != 0:
+        w_, k_, spl_, ms_ = prop.autotune(S if args.autotune < 0 else args.autotune)
+        log(f"autotune: waves={w_} events_per_wave={k_} samples_per_lane={spl_} ({ms_:.3f} ms)")
+    seeds = capi.DeviceBuffer.from_array(np.arange(S, dtype=np.int32))
+    weights = capi.DeviceBuffer.from_array(w_h)
+    realized, durations, cause = capi.DeviceBuffer(8 * E * S), capi.DeviceBuffer(8 * L * S), capi.DeviceBuffer(4 * E * S)
+    obj, gd, ge = capi.DeviceBuffer(8), capi.DeviceBuffer(8 * L), capi.DeviceBuffer(8 * E)
+    stream = capi.C.c_void_p(0)
+    capi.check(capi.lib().mcdp_stream_create(capi.C.byref(stream)))
+
+    def forward() -> None:
+        prop.run_many_device(seeds, S, S, realized, durations, cause, stream)
+
+    def reverse() -> None:
+        prop.reverse_device(seeds, weights, realized, durations, cause, S, S, obj, gd, ge, stream=stream)
+
+    for _ in range(args.warmup):
+        forward()
+        reverse()
+    capi.synchronize(stream)
+    tf = [(capi.Timer(), capi.Timer()) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for a, b in tf:
+        a.start(stream)
+        forward()
+        a.stop(stream)
+        b.start(stream)
+        reverse()
+        b.stop(stream)
+    capi.synchronize(stream)
+    wall = time.perf_counter() - t0
+    fwd_ms = float(np.mean([a.elapsed_ms() for a, _ in tf]))
+    rev_ms = float(np.mean([b.elapsed_ms() for _, b in tf]))
+    ms_per_step = 1000.0 * wall / args.steps
+    J = obj.to_array(np.float64, (1,))[0] / S
+    log(f"gradient: forward {fwd_ms:.3f} ms, reverse {rev_ms:.3f} ms, {ms_per_step:.3f} ms/step wall; J = {J:.6f}")
+
+    # entries whose derivative reads a duration row (relative kinds, nonzero minimal duration)
+    rel_types = {d.activity_type for d in dag.dists if d.kind != EMPIRICAL_ABSOLUTE}
+    link_type = np.full(L, -1, dtype=np.int64)
+    link_dmin = np.zeros(L)
+    link_type[dag.act_idx] = dag.act_type
+    link_dmin[dag.act_idx] = dag.act_dur
+    edges = np.asarray(dag.prec_edge)
+    valid = (edges >= 0) & (edges < L)
+    reads = int(np.sum([t in rel_types for t in link_type[edges[valid]]] & (link_dmin[edges[valid]] != 0.0)))
+    alg_bytes = float(S) * (12 * E + 8 * reads)
+    achieved = alg_bytes / (rev_ms / 1000.0) / 1e9
+
+    check = None
+    baseline = None
+    if args.check_samples > 0 or not args.no_cpu:
+        import oracle
+
+        ctx = oracle.OracleCtx(dag)
+        m = max(64, args.check_samples * 4)
+        seeds_c = np.arange(m, dtype=np.int32)
+        r, d, c = prop.run_many(seeds_c)
+        t1 = time.perf_counter()
+        o_obj, o_gd, o_ge = ctx.gradient(r, c, d, w_h)
+        t_cpu = time.perf_counter() - t1
+        g_obj, g_gd, g_ge = prop.gradient(seeds_c, w_h)
+        scale = max(1.0, float(np.max(np.abs(o_gd / m))), float(np.max(np.abs(o_ge / m))))
+        err = max(abs(g_obj - o_obj / m) / max(1.0, abs(o_obj / m)), float(np.max(np.abs(g_gd - o_gd / m))) / scale,
+                  float(np.max(np.abs(g_ge - o_ge / m))) / scale)
+        check = {"samples_checked": m, "max_rel_err_vs_fp64_cpu": err, "ok": bool(err < 1e-10)}
+        log(f"check: {check}")
+        if not args.no_cpu:
+            reps = max(1, int(np.ceil(10.0 / max(t_cpu, 1e-3))))
+            reps = min(reps, 200)
+            t1 = time.perf_counter()
+            for _ in range(reps):
+                ctx.gradient(r, c, d, w_h)
+            t_cpu = (time.perf_counter() - t1) / reps
+            baseline = {"value": L * m / t_cpu, "unit": "activity-samples/s (reverse walk only)", "cores": 1,
+                        "kind": "port",
+                        "sample": f"the oracle's FP64 reverse walk (C, one thread) over the device's run_many outputs "
+                                  f"for seeds 0..{m - 1}, repeated {reps} times ({reps * t_cpu:.1f} s)"}
+            log(f"cpu reverse walk: {baseline['value']:.4g} activity-samples/s on 1 thread")
+        if not check["ok"]:
+            print("device gradient differs from the oracle", file=sys.stderr)
+            return 3
+
+    out = {
+        "metric": "pathwise gradient (run_many + reverse walk): activity-samples/s",
+        "value": L * S / (ms_per_step / 1000.0),
+        "unit": "activity-samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "mode": "gradient",
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (deterministic SplitMix64 layered DAG; seeds 0..S-1; weights U[0.5, 2])",
+        "config": {"workload": f"layered_network: 200 layers x 100 events ({E} events, {L} activities), {S} samples; "
+                               "objective E[sum_e w_e max(0, realized_e - earliest_e)], gradient with respect to every "
+                               "minimal_duration and earliest time",
+                   "events": E, "activities": L, "samples": S},
+        "forward_ms": fwd_ms,
+        "reverse_ms": rev_ms,
+        "objective": J,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": read_traffic(ROOT / "profiles" / "reverse_network_traffic.txt"),
+                     "algorithmic_bytes_per_launch": alg_bytes, "kernel_ms": rev_ms,
+                     "kernel": "reverse walk (reverse_kernel + reduce_columns_kernel)"},
+        "cpu_baseline": baseline,
+        "speedup_vs_cpu_baseline": (L * S / (rev_ms / 1000.0)) / baseline["value"] if baseline else None,
+        "check": check,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--samples", type=int, default=100_000)
-    ap.add_argument("--workload", choices=["network", "small", "batch"], default="network",
+    ap.add_argument("--samples", type=int, default=None, help="samples per step (100,000; 65,536 for grad)")
+    ap.add_argument("--workload", choices=["network", "small", "batch", "grad"], default="network",
                     help="batch: 16 layered DAGs of 1,000-10,000 events (log-uniform) with 4,096-32,768 samples "
-                    "each (log-uniform), run in one batched launch and, for comparison, one launch per DAG")
+                    "each (log-uniform), run in one batched launch and, for comparison, one launch per DAG; "
+                    "grad: run_many + the reverse walk (pathwise gradient) on the network DAG, 65,536 samples")
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--check-samples", type=int, default=64)
     ap.add_argument("--mode", choices=["run_many", "propagate"], default="run_many",
@@ -178,6 +314,9 @@ def main() -> int:
                     help="only run a plain device copy of MIB MiB (read + write) --steps times, to calibrate "
                     "rocprofv3 FETCH_SIZE / WRITE_SIZE against known bytes")
     args = ap.parse_args()
+    args.samples_given = args.samples is not None
+    if args.samples is None:
+        args.samples = 100_000
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (samples would shard "
@@ -209,6 +348,8 @@ def main() -> int:
 
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "grad":
+        return bench_grad(args)
 
     dag = layered_network() if args.workload == "network" else layered_small()
     prop = capi.Propagator(dag)

@@ -228,11 +228,15 @@ int mcdp_reverse_device(mcdp_propagator *prop, const int32_t *seeds_dev, const d
                         const double *realized_dev, const double *durations_dev, const int32_t *cause_dev, int64_t n,
                         int64_t ld, int accumulate, double *objective_dev, double *grad_dmin_dev,
                         double *grad_earliest_dev, void *stream);
+/* Facts about the reverse walk's plan: out[0..5] = {steps, waves per
+ * workgroup, slots, slots in LDS, partial-row columns, samples per lane}. */
+int mcdp_propagator_gradient_info(mcdp_propagator *prop, int64_t *out, int n_out);
 /* Host convenience: run_many then the reverse walk over chunks of at most
- * `chunk` samples (0: as many as fit a bounded workspace), so per-sample
- * results never need to exist for all seeds at once.  Writes MEANS over the n
- * seeds: objective[0], grad_dmin[link_count()], grad_earliest[node_count()].
- * The result does not depend on the chunk size. */
+ * `chunk` samples (0: as many as fit a bounded workspace; rounded to whole
+ * tiles of the reverse kernel, 64 or 128 samples), so per-sample results never
+ * need to exist for all seeds at once.  Writes MEANS over the n seeds:
+ * objective[0], grad_dmin[link_count()], grad_earliest[node_count()].  The
+ * result does not depend on the chunk size. */
 int mcdp_gradient_host(mcdp_propagator *prop, const int32_t *seeds, int64_t n, const double *weights, int64_t chunk,
                        double *objective, double *grad_dmin, double *grad_earliest);
 
@@ -269,6 +273,10 @@ int mcdp_debug_math(int fn, int64_t n, const double *x, double *out);
 /* Plain device copy of n doubles (one read and one write per element): the
  * calibration kernel for HBM byte counters. */
 int mcdp_debug_copy(double *dst_dev, const double *src_dev, int64_t n, void *stream);
+/* Caps the tiles of 64 samples one reverse-kernel launch of this propagator
+ * covers (0: the default, bounded by a 1 GiB workspace), so tests can run a
+ * mcdp_reverse_device call as several launches. */
+int mcdp_debug_reverse_tiles(mcdp_propagator *prop, int64_t max_tiles);
 
 #ifdef __cplusplus
 }

@@ -78,6 +78,8 @@ _SIGNATURES = {
     "mcdp_propagator_get_times": (C.c_int, [_P, _pf64, _pf64]),
     "mcdp_reverse_device": (C.c_int, [_P, _P, _P, _P, _P, _P, _i64, _i64, C.c_int, _P, _P, _P, _P]),
     "mcdp_gradient_host": (C.c_int, [_P, _pi32, _i64, _pf64, _i64, _pf64, _pf64, _pf64]),
+    "mcdp_debug_reverse_tiles": (C.c_int, [_P, _i64]),
+    "mcdp_propagator_gradient_info": (C.c_int, [_P, _pi64, C.c_int]),
 }
 
 _lib = None
@@ -314,6 +316,14 @@ class Propagator:
         e = np.empty(self.E)
         check(lib().mcdp_propagator_get_times(self.h, _ptr(d, _pf64), _ptr(e, _pf64)))
         return d, e
+
+    GRADIENT_FIELDS = ["steps", "waves", "slots", "lds_slots", "columns", "samples_per_lane"]
+
+    def gradient_info(self) -> dict:
+        """Facts about the reverse walk's plan (built on the host)."""
+        v = np.zeros(len(self.GRADIENT_FIELDS), dtype=np.int64)
+        check(lib().mcdp_propagator_gradient_info(self.h, _ptr(v, _pi64), v.size))
+        return dict(zip(self.GRADIENT_FIELDS, map(int, v)))
 
     def gradient(self, seeds, weights, chunk: int = 0) -> tuple[float, np.ndarray, np.ndarray]:
         """Means over the seeds of J = sum_e w_e max(0, realized_e - earliest_e) and of its pathwise

@@ -20,8 +20,8 @@ using namespace mcdp;
 
 namespace {
 
-constexpr size_t kRevLdsBudget = 96 * 1024;          // LDS per workgroup for slots
-constexpr size_t kSlotBytes = (size_t)kLanes * 12;   // adjoint + winner position per lane
+constexpr size_t kRevLdsBudget = 150 * 1024;         // LDS per workgroup for slots
+constexpr size_t kSlotBytes = (size_t)kLanes * 12;   // adjoint + winner position per sample of a wave
 constexpr size_t kPartialBudget = (size_t)1 << 30;   // bytes of partial rows per launch
 constexpr size_t kGradWorkspace = (size_t)32 << 30;  // per-sample buffers of mcdp_gradient_host
 
@@ -139,6 +139,8 @@ int build_reverse(const mcdp_propagator &P, RevPlan &R) {
         }
         R.slots = used;
     }
+    // two samples per lane when the window of slots fits in LDS that way
+    R.spl = (size_t)R.slots * kSlotBytes * 2 <= kRevLdsBudget ? 2 : 1;
     for (int32_t e : R.ev_list) R.ev_slot.push_back(slot_of[(size_t)e]);
 
     // (c) partial-row columns: link l's first fresh entry writes column l,
@@ -185,7 +187,8 @@ int build_reverse(const mcdp_propagator &P, RevPlan &R) {
     for (int32_t w = 0; w < W; ++w) R.map_col.push_back(R.obj_col + w);
     R.map_off.push_back((int32_t)R.map_col.size());
 
-    // (d) records per (step, wave): header, successors, predecessors
+    // (d) records per (step, wave): header, then predecessors; successor pairs apart
+    std::vector<int32_t> unit_row((size_t)L, -1);
     R.step_off.assign(1, 0);
     for (size_t g = 0; g + 1 < R.ev_off.size(); ++g) {
         for (int32_t i = R.ev_off[g]; i < R.ev_off[g + 1]; ++i) {
@@ -198,14 +201,8 @@ int build_reverse(const mcdp_propagator &P, RevPlan &R) {
             h.i1 = slot_of[(size_t)e];
             h.i2 = (int32_t)succ[(size_t)e].size();
             h.info = kInfoHeader | (uint32_t)(b - a);
+            R.ev_rec.push_back((int32_t)R.recs.size());
             R.recs.push_back(h);
-            for (const auto &sp : succ[(size_t)e]) {
-                Record r{};
-                r.i0 = sp.first;
-                r.i1 = slot_of[(size_t)sp.first];
-                r.i2 = sp.second;
-                R.recs.push_back(r);
-            }
             std::vector<int32_t> srcs;
             for (int64_t k = a; k < b; ++k) srcs.push_back(P.preds[(size_t)k].src);
             std::sort(srcs.begin(), srcs.end());
@@ -213,16 +210,30 @@ int build_reverse(const mcdp_propagator &P, RevPlan &R) {
                 const PredEntry &pe = P.preds[(size_t)k];
                 const auto range = std::equal_range(srcs.begin(), srcs.end(), pe.src);
                 const int64_t dups = range.second - range.first;
-                Record r = link_record(P, pe.edge);
-                const uint32_t cls = mul_class(P, pe.edge, r);
-                r.a = cls == kMulRead ? 1.0 / P.dmin[(size_t)pe.edge] : 0.0;
+                const Record fwd = link_record(P, pe.edge);
+                const uint32_t cls = mul_class(P, pe.edge, fwd);
+                Record r = fwd;
+                r.a = cls == kMulRead ? 1.0 / P.dmin[(size_t)pe.edge] : 1.0;
                 r.i0 = pe.src;
                 r.i1 = col[(size_t)k];
                 r.i2 = pe.edge;
                 r.info = (r.info & ~(uint32_t)(kRevClassMask | kRevDup | kRevStale | kInfoOwner | kInfoLast)) |
                          (cls << kRevClassShift) | (dups > 1 ? kRevDup : 0u) | (pe.stale ? kRevStale : 0u);
-                if (cls == kMulDraw) R.needs_seeds = true;
+                if (cls == kMulDraw) {  // its multiplier comes from a row of unit durations
+                    int32_t &u = unit_row[(size_t)pe.edge];
+                    if (u < 0) {
+                        if (R.unit_links.size() > kInfoDistMask) return fail(MCDP_E_INVALID, "too many links with minimal duration 0");
+                        u = (int32_t)R.unit_links.size();
+                        R.unit_links.push_back(fwd);
+                    }
+                    r.info = (r.info & ~(kInfoDistMask << kInfoDistShift)) | ((uint32_t)u << kInfoDistShift);
+                }
                 R.recs.push_back(r);
+            }
+            R.ev_succ.push_back((int32_t)(R.succ.size() / 2));
+            for (const auto &sp : succ[(size_t)e]) {
+                R.succ.push_back(slot_of[(size_t)sp.first]);
+                R.succ.push_back(sp.second);
             }
         }
         R.step_off.push_back((int32_t)R.recs.size());
@@ -250,6 +261,9 @@ int ensure_reverse(mcdp_propagator &P, DeviceCopy &d) {
     const size_t o_step = reserve(R.step_off.size() * 4), o_recs = reserve(R.recs.size() * sizeof(Record));
     const size_t o_evoff = reserve(R.ev_off.size() * 4), o_evl = reserve(R.ev_list.size() * 4);
     const size_t o_evs = reserve(R.ev_slot.size() * 4), o_moff = reserve(R.map_off.size() * 4);
+    const size_t o_evr = reserve(R.ev_rec.size() * 4);
+    const size_t o_unit = reserve(R.unit_links.size() * sizeof(Record));
+    const size_t o_evsucc = reserve(R.ev_succ.size() * 4), o_succ = reserve(R.succ.size() * 4 + 32);
     const size_t o_mcol = reserve(R.map_col.size() * 4);
     const size_t total = std::max<size_t>(off, 256);
     std::vector<unsigned char> host(total, 0);
@@ -261,6 +275,10 @@ int ensure_reverse(mcdp_propagator &P, DeviceCopy &d) {
     put(o_evoff, R.ev_off.data(), R.ev_off.size() * 4);
     put(o_evl, R.ev_list.data(), R.ev_list.size() * 4);
     put(o_evs, R.ev_slot.data(), R.ev_slot.size() * 4);
+    put(o_evr, R.ev_rec.data(), R.ev_rec.size() * 4);
+    put(o_unit, R.unit_links.data(), R.unit_links.size() * sizeof(Record));
+    put(o_evsucc, R.ev_succ.data(), R.ev_succ.size() * 4);
+    put(o_succ, R.succ.data(), R.succ.size() * 4);
     put(o_moff, R.map_off.data(), R.map_off.size() * 4);
     put(o_mcol, R.map_col.data(), R.map_col.size() * 4);
     void *blob = nullptr;
@@ -280,18 +298,22 @@ int ensure_reverse(mcdp_propagator &P, DeviceCopy &d) {
     p.ev_off = reinterpret_cast<const int32_t *>(b + o_evoff);
     p.ev_list = reinterpret_cast<const int32_t *>(b + o_evl);
     p.ev_slot = reinterpret_cast<const int32_t *>(b + o_evs);
+    p.ev_rec = reinterpret_cast<const int32_t *>(b + o_evr);
+    p.ev_succ = reinterpret_cast<const int32_t *>(b + o_evsucc);
+    p.succ = reinterpret_cast<const int2 *>(b + o_succ);
     p.dists = d.base.dists;
     p.alias = d.base.alias;
     p.earliest = d.base.earliest;
     p.n_steps = R.n_steps;
     p.waves = R.waves;
-    p.lds_slots = std::min<int32_t>(R.slots, (int32_t)(kRevLdsBudget / kSlotBytes));
+    p.lds_slots = std::min<int32_t>(R.slots, (int32_t)(kRevLdsBudget / (kSlotBytes * R.spl)));
     p.global_slots = R.slots - p.lds_slots;
     p.L = P.L;
     p.n_cols = R.n_cols;
     p.obj_col = R.obj_col;
     d.rev_map_off = reinterpret_cast<const int32_t *>(b + o_moff);
     d.rev_map_col = reinterpret_cast<const int32_t *>(b + o_mcol);
+    d.rev_unit_links = reinterpret_cast<const Record *>(b + o_unit);
     return MCDP_OK;
 }
 
@@ -366,28 +388,36 @@ int mcdp_reverse_device(mcdp_propagator *prop, const int32_t *seeds_dev, const d
     if (rc) return rc;
     rc = ensure_reverse(P, *d);
     if (rc) return rc;
-    if (n > 0 && P.rev->needs_seeds && !seeds_dev)
+    const int64_t n_unit = (int64_t)P.rev->unit_links.size();
+    if (n > 0 && n_unit > 0 && !seeds_dev)
         return fail(MCDP_E_INVALID, "seeds are required: a relative link has minimal duration 0");
     const auto s = static_cast<hipStream_t>(stream);
     const RevParams &base = d->rev_base;
-    const int64_t tiles = (n + kLanes - 1) / kLanes;
+    const int spl = P.rev->spl;
+    const int64_t tile = (int64_t)kLanes * spl;
+    const int64_t tiles = (n + tile - 1) / tile;
     const size_t row_bytes = (size_t)base.n_cols * sizeof(double);
-    const int64_t max_tiles = std::max<int64_t>(1, (int64_t)(kPartialBudget / row_bytes));
-    const size_t lds = (size_t)base.lds_slots * kSlotBytes;
+    int64_t max_tiles = std::max<int64_t>(1, (int64_t)(kPartialBudget / row_bytes));
+    if (P.rev_max_tiles > 0) max_tiles = std::min(max_tiles, P.rev_max_tiles);
+    const size_t lds = (size_t)base.lds_slots * kSlotBytes * spl;
     int cus = 0, bpc = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device), "hipDeviceGetAttribute");
-    HIP_TRY(reverse_occupancy(base.waves, lds, &bpc), "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    HIP_TRY(reverse_occupancy(base.waves, lds, base.global_slots > 0, spl, &bpc),
+            "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     if (bpc < 1) return fail(MCDP_E_DEVICE, "reverse kernel cannot be resident with " + std::to_string(lds) + " bytes of LDS");
     const int64_t resident = (int64_t)cus * bpc;
-    const int64_t chunk_tiles = std::min<int64_t>(std::max<int64_t>(tiles, 1), max_tiles);
+    int64_t chunk_tiles = std::min<int64_t>(std::max<int64_t>(tiles, 1), max_tiles);
+    if (n_unit > 0)  // unit-duration rows for the launch's samples stay within the same budget
+        chunk_tiles = std::min<int64_t>(chunk_tiles, std::max<int64_t>(1, (int64_t)(kPartialBudget / ((size_t)n_unit * tile * 8))));
     const int64_t grid_max = std::min<int64_t>(chunk_tiles, resident);
     const size_t o_gslots = align_to(row_bytes * (size_t)chunk_tiles, 256);
-    const size_t gslot_bytes = (size_t)grid_max * base.global_slots * kLanes * 2 * sizeof(double);
-    rc = ensure_workspace(*d, o_gslots + std::max<size_t>(gslot_bytes, 256));
+    const size_t gslot_bytes = (size_t)grid_max * base.global_slots * tile * 2 * sizeof(double);
+    const size_t o_unit = o_gslots + align_to(std::max<size_t>(gslot_bytes, 256), 256);
+    const int32_t ld_unit = (int32_t)(chunk_tiles * tile);
+    rc = ensure_workspace(*d, o_unit + std::max<size_t>((size_t)n_unit * (size_t)ld_unit * 8, 256));
     if (rc) return rc;
     auto *work = static_cast<unsigned char *>(d->rev_work);
     RevParams p = base;
-    p.seeds = seeds_dev;
     p.weights = weights_dev;
     p.realized = realized_dev;
     p.durations = durations_dev;
@@ -396,6 +426,9 @@ int mcdp_reverse_device(mcdp_propagator *prop, const int32_t *seeds_dev, const d
     p.ld = (int32_t)std::max<int64_t>(ld, 1);
     p.partial = reinterpret_cast<double *>(work);
     p.gslots = reinterpret_cast<double *>(work + o_gslots);
+    double *unit = reinterpret_cast<double *>(work + o_unit);
+    p.unit = unit;
+    p.ld_unit = ld_unit;
     if (tiles == 0 || P.E == 0) {  // no samples: sums are 0 (or unchanged)
         HIP_TRY(launch_reduce_columns(p.partial, 0, p.n_cols, d->rev_map_off, d->rev_map_col, P.L, P.E, accumulate,
                                       grad_dmin_dev, grad_earliest_dev, objective_dev, s),
@@ -406,11 +439,38 @@ int mcdp_reverse_device(mcdp_propagator *prop, const int32_t *seeds_dev, const d
         p.tile0 = (int32_t)t0;
         p.n_tiles = (int32_t)std::min<int64_t>(chunk_tiles, tiles - t0);
         const int grid = (int)std::min<int64_t>(p.n_tiles, resident);
-        HIP_TRY(launch_reverse(p, grid, lds, s), "reverse kernel launch");
+        HIP_TRY(launch_unit_draws(d->rev_unit_links, (int32_t)n_unit, seeds_dev, (int32_t)(t0 * tile),
+                                  (int32_t)(p.n_tiles * tile), (int32_t)n, base.dists, base.alias, unit, ld_unit, s),
+                "unit draw kernel launch");
+        HIP_TRY(launch_reverse(p, spl, grid, lds, s), "reverse kernel launch");
         HIP_TRY(launch_reduce_columns(p.partial, p.n_tiles, p.n_cols, d->rev_map_off, d->rev_map_col, P.L, P.E,
                                       accumulate || t0 > 0, grad_dmin_dev, grad_earliest_dev, objective_dev, s),
                 "reduce kernel launch");
     }
+    return MCDP_OK;
+}
+
+int mcdp_propagator_gradient_info(mcdp_propagator *prop, int64_t *out, int n_out) {
+    if (!prop || !out) return fail(MCDP_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lock(prop->mu);
+    if (!prop->rev) {
+        auto R = std::make_shared<RevPlan>();
+        int rc = build_reverse(*prop, *R);
+        if (rc) return rc;
+        prop->rev = R;
+    }
+    const RevPlan &R = *prop->rev;
+    const int64_t v[6] = {R.n_steps, R.waves, R.slots,
+                          std::min<int64_t>(R.slots, (int64_t)(kRevLdsBudget / (kSlotBytes * R.spl))), R.n_cols, R.spl};
+    for (int i = 0; i < n_out && i < 6; ++i) out[i] = v[i];
+    return MCDP_OK;
+}
+
+int mcdp_debug_reverse_tiles(mcdp_propagator *prop, int64_t max_tiles) {
+    if (!prop) return fail(MCDP_E_INVALID, "propagator is null");
+    if (max_tiles < 0) return fail(MCDP_E_INVALID, "max_tiles must be >= 0");
+    std::lock_guard<std::mutex> lock(prop->mu);
+    prop->rev_max_tiles = max_tiles;
     return MCDP_OK;
 }
 
@@ -423,10 +483,21 @@ int mcdp_gradient_host(mcdp_propagator *prop, const int32_t *seeds, int64_t n, c
     if (chunk < 0) return fail(MCDP_E_INVALID, "chunk must be >= 0");
     const int64_t E = prop->E, L = prop->L;
     const int64_t per_sample = 12 * E + 8 * L + 4;
-    if (chunk == 0) chunk = std::max<int64_t>(kLanes, (int64_t)(kGradWorkspace / (size_t)std::max<int64_t>(per_sample, 1)));
+    int64_t tile = kLanes;
+    {
+        std::lock_guard<std::mutex> lock(prop->mu);
+        if (!prop->rev) {
+            auto R = std::make_shared<RevPlan>();
+            int rc = build_reverse(*prop, *R);
+            if (rc) return rc;
+            prop->rev = R;
+        }
+        tile = (int64_t)kLanes * prop->rev->spl;
+    }
+    if (chunk == 0) chunk = std::max<int64_t>(tile, (int64_t)(kGradWorkspace / (size_t)std::max<int64_t>(per_sample, 1)));
     chunk = std::min<int64_t>(chunk, std::numeric_limits<int32_t>::max() - 256);
-    chunk = std::max<int64_t>(kLanes, chunk / kLanes * kLanes);  // whole tiles: sums do not depend on the chunking
-    chunk = std::min<int64_t>(chunk, (n + kLanes - 1) / kLanes * kLanes);
+    chunk = std::max<int64_t>(tile, chunk / tile * tile);  // whole tiles: sums do not depend on the chunking
+    chunk = std::min<int64_t>(chunk, (n + tile - 1) / tile * tile);
     const size_t o_seeds = 0, o_real = align_to((size_t)chunk * 4, 256);
     const size_t o_dur = o_real + align_to((size_t)(E * chunk) * 8, 256);
     const size_t o_cause = o_dur + align_to((size_t)(L * chunk) * 8, 256);

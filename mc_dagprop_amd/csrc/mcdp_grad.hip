@@ -1,7 +1,8 @@
 // gfx950 kernels of the reverse (pathwise-gradient) walk.
 //
-// reverse_kernel: one workgroup per tile of 64 samples (lane l carries sample
-// tile * 64 + l), W waves splitting each step's events as in the forward walk,
+// reverse_kernel: one workgroup per tile of 64 spl samples (lane l carries
+// samples tile * 64 spl + i * 64 + l, i < spl), W waves splitting each step's
+// events as in the forward walk,
 // the steps in reverse topological order.  Per event e and sample:
 //   g      = sum over e's successor records (t, k) of adj[t] if win[t] == k
 //   active = realized[e] > earliest[e]
@@ -15,6 +16,8 @@
 //   adj[e] = gt, win[e] = winner (-1 if none, capped or stale)  -> e's slot
 // Sums over the tile's lanes are wave reductions (DPP, a fixed order), written
 // once per (tile, column) to the partial matrix; reduce_columns sums the tiles.
+// An event's per-sample rows (realized, cause, durations) are loaded while the
+// previous event is processed.
 // The arithmetic is FP64 add, multiply and compare-select; this file is
 // compiled without FMA contraction.
 #include <hip/hip_runtime.h>
@@ -61,8 +64,10 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 __device__ __forceinline__ bool any_lane(bool b) { return __builtin_amdgcn_ballot_w64(b) != 0; }
 
-// A slot's adjoint and winner position for this lane: LDS for the first
-// lds_slots slots, the workgroup's global region for the rest.
+// A slot's adjoints and winner positions for this lane's kSpl samples
+// (adjacent: one ds_read_b64 / b128 per plane).  kGlobal: slots past n_lds
+// live in the workgroup's global region; otherwise every slot is in LDS.
+template <bool kGlobal, int kSpl>
 struct Slots {
     double *g_lds;
     int32_t *w_lds;
@@ -70,177 +75,380 @@ struct Slots {
     int32_t *w_glb;
     int32_t n_lds;
     int lane;
-    __device__ __forceinline__ void read(int32_t slot, double &g, int32_t &w) const {
-        if (slot < n_lds) {
-            g = g_lds[slot * kLanes + lane];
-            w = w_lds[slot * kLanes + lane];
+    __device__ __forceinline__ void read(int32_t slot, double (&g)[kSpl], int32_t (&w)[kSpl]) const {
+        const bool in_lds = !kGlobal || slot < n_lds;
+        const int32_t i = ((in_lds ? slot : slot - n_lds) * kLanes + lane) * kSpl;
+        if (in_lds) {
+            if constexpr (kSpl == 1) {
+                g[0] = g_lds[i];
+                w[0] = w_lds[i];
+            } else {
+                const double2 gg = *reinterpret_cast<const double2 *>(g_lds + i);
+                const int2 ww = *reinterpret_cast<const int2 *>(w_lds + i);
+                g[0] = gg.x, g[1] = gg.y, w[0] = ww.x, w[1] = ww.y;
+            }
         } else {
-            const int32_t s = slot - n_lds;
-            g = g_glb[s * kLanes + lane];
-            w = w_glb[s * kLanes + lane];
+#pragma unroll
+            for (int k = 0; k < kSpl; ++k) {
+                g[k] = __builtin_nontemporal_load(g_glb + i + k);
+                w[k] = __builtin_nontemporal_load(w_glb + i + k);
+            }
         }
     }
-    __device__ __forceinline__ void write(int32_t slot, double g, int32_t w) const {
-        if (slot < n_lds) {
-            g_lds[slot * kLanes + lane] = g;
-            w_lds[slot * kLanes + lane] = w;
+    __device__ __forceinline__ void write(int32_t slot, const double (&g)[kSpl], const int32_t (&w)[kSpl]) const {
+        const bool in_lds = !kGlobal || slot < n_lds;
+        const int32_t i = ((in_lds ? slot : slot - n_lds) * kLanes + lane) * kSpl;
+        if (in_lds) {
+            if constexpr (kSpl == 1) {
+                g_lds[i] = g[0];
+                w_lds[i] = w[0];
+            } else {
+                *reinterpret_cast<double2 *>(g_lds + i) = double2{g[0], g[1]};
+                *reinterpret_cast<int2 *>(w_lds + i) = int2{w[0], w[1]};
+            }
         } else {
-            const int32_t s = slot - n_lds;
-            g_glb[s * kLanes + lane] = g;
-            w_glb[s * kLanes + lane] = w;
+#pragma unroll
+            for (int k = 0; k < kSpl; ++k) {
+                g_glb[i + k] = g[k];
+                w_glb[i + k] = w[k];
+            }
         }
     }
 };
 
+// Column sums of one wave (wave-uniform values), staged one per lane and
+// written with one store instruction per 64 of them after the step's loads:
+// a store issued between loads would make the next load's wait include it.
+struct ColumnStage {
+    int32_t col = 0;
+    double val = 0.0;
+    int n = 0;  // wave-uniform
+    __device__ __forceinline__ void flush(double *prow, int lane) {
+        if (lane < n) prow[col] = val;
+        n = 0;
+    }
+    __device__ __forceinline__ void put(int32_t c, double v, double *prow, int lane) {
+        const bool mine = lane == n;
+        col = mine ? c : col;
+        val = mine ? v : val;
+        if (++n == kLanes) flush(prow, lane);
+    }
+};
+
+constexpr int kPre = 3;        // multiplier rows of an event's first predecessors, loaded one event ahead
+constexpr int kFastPreds = 4;  // predecessor records read per group
+
+// Lane sum of a per-sample value, then the wave sum: a fixed order.
+template <int kSpl>
+__device__ __forceinline__ double tile_sum(const double (&v)[kSpl]) {
+    double x = v[0];
+#pragma unroll
+    for (int i = 1; i < kSpl; ++i) x += v[i];
+    return wave_sum(x);
+}
+
+template <int kSpl>
+__device__ __forceinline__ bool any_of(const bool (&b)[kSpl]) {
+    bool x = b[0];
+#pragma unroll
+    for (int i = 1; i < kSpl; ++i) x = x || b[i];
+    return any_lane(x);
+}
+
+template <bool kGlobal, int kSpl>
 __global__ void __launch_bounds__(kMaxWaves * kLanes) reverse_kernel(RevParams p) {
     extern __shared__ double lds[];
+    constexpr int K = kRevEventsPerWave;
     const int lane = threadIdx.x & (kLanes - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int W = p.waves;
     const uint32_t ld = (uint32_t)p.ld;
-    const auto step_off = as_const_space(p.step_off);
     const auto recs = as_const_space(p.recs);
+    const auto succ = as_const_space(p.succ);
     const auto ev_off = as_const_space(p.ev_off);
     const auto ev_list = as_const_space(p.ev_list);
+    const auto ev_rec = as_const_space(p.ev_rec);
+    const auto ev_succ = as_const_space(p.ev_succ);
     const auto ev_slot = as_const_space(p.ev_slot);
     const auto weights = as_const_space(p.weights);
     const auto earliest = as_const_space(p.earliest);
 
-    Slots sl;
+    Slots<kGlobal, kSpl> sl;
     sl.g_lds = lds;
-    sl.w_lds = reinterpret_cast<int32_t *>(lds + (size_t)p.lds_slots * kLanes);
-    sl.g_glb = p.gslots + (size_t)blockIdx.x * p.global_slots * kLanes * 2;
-    sl.w_glb = reinterpret_cast<int32_t *>(sl.g_glb + (size_t)p.global_slots * kLanes);
+    sl.w_lds = reinterpret_cast<int32_t *>(lds + (size_t)p.lds_slots * kLanes * kSpl);
+    sl.g_glb = p.gslots + (size_t)blockIdx.x * p.global_slots * kLanes * kSpl * 2;
+    sl.w_glb = reinterpret_cast<int32_t *>(sl.g_glb + (size_t)p.global_slots * kLanes * kSpl);
     sl.n_lds = p.lds_slots;
     sl.lane = lane;
 
     for (int32_t tile = blockIdx.x; tile < p.n_tiles; tile += gridDim.x) {
-        // Lanes past the last sample repeat sample n - 1 with weight 0: every
-        // sum they enter gets +0 from them.
-        const int32_t s_raw = (p.tile0 + tile) * kLanes + lane;
-        const bool live = s_raw < p.n;
-        const uint32_t s = live ? (uint32_t)s_raw : (uint32_t)(p.n - 1);
+        // Lane l carries samples (tile0 + tile) * 64 kSpl + i * 64 + l.  Lanes
+        // past the last sample repeat sample n - 1 with weight 0: every sum
+        // they enter gets +0 from them.
+        bool live[kSpl];
+        uint32_t s[kSpl], s_unit[kSpl];
+#pragma unroll
+        for (int i = 0; i < kSpl; ++i) {
+            const int32_t s_raw = (p.tile0 + tile) * (kLanes * kSpl) + i * kLanes + lane;
+            live[i] = s_raw < p.n;
+            s[i] = live[i] ? (uint32_t)s_raw : (uint32_t)(p.n - 1);
+            // unit-duration rows cover this launch's tiles only
+            s_unit[i] = s[i] - (uint32_t)p.tile0 * (kLanes * kSpl);
+        }
         double *const prow = p.partial + (size_t)tile * p.n_cols;
-        double obj = 0.0;
+        double obj[kSpl];
+#pragma unroll
+        for (int i = 0; i < kSpl; ++i) obj[i] = 0.0;
+        ColumnStage st;
+        // A predecessor's multiplier row: its duration row (times 1 / d) or its
+        // row of unit durations (times 1).
+        auto mult_values = [&](const Record &pr, double (&out)[kSpl]) {
+            const bool unit = (pr.info & kRevClassMask) == (kMulDraw << kRevClassShift);
+            const double *base = unit ? p.unit + row_offset((int32_t)((pr.info >> kInfoDistShift) & kInfoDistMask), p.ld_unit)
+                                      : p.durations + row_offset(pr.i2, ld);
+#pragma unroll
+            for (int i = 0; i < kSpl; ++i) out[i] = base[unit ? s_unit[i] : s[i]];
+        };
+        auto reads_row = [](const Record &pr) {
+            const uint32_t cls = (pr.info & kRevClassMask) >> kRevClassShift;
+            return cls == kMulRead || cls == kMulDraw;
+        };
+
+        // An event's per-sample rows (realized, cause, the multiplier rows of
+        // its first kPre predecessors), loaded one event ahead: the next event
+        // of the step, or the first event of the next step.
+        struct EvRows {
+            double r[kSpl];
+            int32_t c[kSpl];
+            double d[kPre][kSpl];
+        };
+        auto load_event = [&](int idx, EvRows &x) {
+            const uint64_t row = row_offset(ev_list[idx], ld);
+#pragma unroll
+            for (int i = 0; i < kSpl; ++i) {
+                x.r[i] = p.realized[row + s[i]];
+                x.c[i] = p.cause[row + s[i]];
+            }
+            const int h = ev_rec[idx];
+            const int n_pred = (int)(recs[h].info & kRevNpredMask);
+#pragma unroll
+            for (int k = 0; k < kPre; ++k) {
+#pragma unroll
+                for (int i = 0; i < kSpl; ++i) x.d[k][i] = 0.0;
+                if (k < n_pred) {
+                    const Record pr = recs[h + 1 + k];
+                    if (reads_row(pr)) mult_values(pr, x.d[k]);
+                }
+            }
+        };
+        EvRows cur{}, nxt{};
+        int loaded = -1;  // index (in ev_list) of the event whose rows are in `cur`
 
         for (int t = 0; t < p.n_steps; ++t) {
             const int e0 = ev_off[t * W + wave];
             const int ne = ev_off[t * W + wave + 1] - e0;
-            // This wave's events of the step: realized and cause loads issued together.
-            double rr[kRevEventsPerWave];
-            int32_t cc[kRevEventsPerWave];
-#pragma unroll
-            for (int j = 0; j < kRevEventsPerWave; ++j) {
-                rr[j] = 0.0;
-                cc[j] = -1;
-                if (j < ne) {
-                    const uint64_t row = row_offset(ev_list[e0 + j], ld);
-                    rr[j] = p.realized[row + s];
-                    cc[j] = p.cause[row + s];
+            if (ne > 0 && loaded != e0) load_event(e0, cur);
+            double gres[K][kSpl];
+            int32_t wres[K][kSpl];
+#pragma unroll 1
+            for (int j = 0; j < ne; ++j) {
+                // the next event's rows
+                int next = -1;
+                if (j + 1 < ne) {
+                    next = e0 + j + 1;
+                } else if (t + 1 < p.n_steps) {
+                    const int f0 = ev_off[(t + 1) * W + wave];
+                    if (ev_off[(t + 1) * W + wave + 1] > f0) next = f0;
                 }
-            }
-            double gres[kRevEventsPerWave];
-            int32_t wres[kRevEventsPerWave];
-            int h = step_off[t * W + wave];
-#pragma unroll
-            for (int j = 0; j < kRevEventsPerWave; ++j) {
-                gres[j] = 0.0;
-                wres[j] = -1;
-                if (j >= ne) continue;
+                if (next >= 0) load_event(next, nxt);
+
+                const int h = ev_rec[e0 + j];
                 const Record hdr = recs[h];
                 const int32_t ev = hdr.i0;
-                const int32_t n_succ = hdr.i2;
                 const int32_t n_pred = (int32_t)(hdr.info & kRevNpredMask);
+                const int32_t n_succ = hdr.i2;
+                const int pb = h + 1;
                 const double we = weights[ev];
-                // adjoint from the successors
-                double g = 0.0;
-                for (int q = h + 1; q <= h + n_succ; ++q) {
-                    const Record sr = recs[q];
-                    double gv;
-                    int32_t wv;
-                    sl.read(sr.i1, gv, wv);
-                    g += (wv == sr.i2) ? gv : 0.0;
+                // adjoint from the successors, four entries per round
+                double g[kSpl];
+#pragma unroll
+                for (int i = 0; i < kSpl; ++i) g[i] = 0.0;
+                {
+                    const int q0 = ev_succ[e0 + j];
+                    int q = 0;
+                    for (; q + 4 <= n_succ; q += 4) {
+                        int2 sr[4];
+                        double gv[4][kSpl];
+                        int32_t wv[4][kSpl];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) sr[u] = succ[q0 + q + u];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) sl.read(sr[u].x, gv[u], wv[u]);
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+#pragma unroll
+                            for (int i = 0; i < kSpl; ++i) g[i] += (wv[u][i] == sr[u].y) ? gv[u][i] : 0.0;
+                    }
+                    for (; q < n_succ; ++q) {
+                        const int2 sr = succ[q0 + q];
+                        double gv[kSpl];
+                        int32_t wv[kSpl];
+                        sl.read(sr.x, gv, wv);
+#pragma unroll
+                        for (int i = 0; i < kSpl; ++i) g[i] += (wv[i] == sr.y) ? gv[i] : 0.0;
+                    }
                 }
-                const double r = rr[j];
-                const int32_t c = cc[j];
-                const bool active = live && (r > hdr.a);
-                const double gt = g + (active ? we : 0.0);
-                obj += active ? we * (r - hdr.a) : 0.0;
-                const bool capped = (c < 0) || (r == hdr.b);
-                double ge = active ? -we : 0.0;
-                ge += capped ? gt : 0.0;
-                // the winning entry
-                const int pb = h + 1 + n_succ;
-                int32_t win = -1;
-                if (any_lane(live && !capped)) {
-                    for (int k = 0; k < n_pred; ++k) {
-                        const Record pr = recs[pb + k];
-                        bool m = live && !capped && (c == pr.i0);
-                        if ((pr.info & kRevDup) && any_lane(m)) {
-                            // a repeated source: the entry whose candidate is the maximum
-                            const double base = (pr.info & kRevStale) ? earliest[pr.i0]
-                                                                      : p.realized[row_offset(pr.i0, ld) + s];
-                            const double d = (pr.info & kInfoEdgeValid) ? p.durations[row_offset(pr.i2, ld) + s] : 0.0;
+                double gt[kSpl], ge[kSpl];
+                bool capped[kSpl], open[kSpl], ge_nz[kSpl];
+#pragma unroll
+                for (int i = 0; i < kSpl; ++i) {
+                    const double r = cur.r[i];
+                    const bool active = live[i] && (r > hdr.a);
+                    capped[i] = (cur.c[i] < 0) || (r == hdr.b);
+                    open[i] = live[i] && !capped[i];
+                    gt[i] = g[i] + (active ? we : 0.0);
+                    obj[i] += active ? we * (r - hdr.a) : 0.0;
+                    ge[i] = active ? -we : 0.0;
+                    ge[i] += capped[i] ? gt[i] : 0.0;
+                    ge_nz[i] = ge[i] != 0.0;
+                }
+                const bool any_open = any_of<kSpl>(open);
+
+                // is entry q the winner of sample i: its source is the cause and,
+                // for a repeated source, its capped candidate is the maximum
+                auto wins = [&](const Record &q, bool (&m)[kSpl]) {
+#pragma unroll
+                    for (int i = 0; i < kSpl; ++i) m[i] = open[i] && (cur.c[i] == q.i0);
+                    if ((q.info & kRevDup) && any_of<kSpl>(m)) {
+#pragma unroll
+                        for (int i = 0; i < kSpl; ++i) {
+                            const double base =
+                                (q.info & kRevStale) ? earliest[q.i0] : p.realized[row_offset(q.i0, ld) + s[i]];
+                            const double d =
+                                (q.info & kInfoEdgeValid) ? p.durations[row_offset(q.i2, ld) + s[i]] : 0.0;
                             double tt = base + d;
                             tt = (hdr.b < tt) ? hdr.b : tt;
-                            m = m && (tt == r);
+                            m[i] = m[i] && (tt == cur.r[i]);
                         }
-                        win = m ? k : win;
                     }
-                }
-                // derivatives of the entries; the winner passes the adjoint on
-                int32_t wslot = -1;
-                for (int k = 0; k < n_pred; ++k) {
-                    const Record pr = recs[pb + k];
-                    const bool mine = win == k;
-                    const bool stale = (pr.info & kRevStale) != 0;
-                    if (mine && !stale) wslot = k;
-                    const uint32_t cls = (pr.info & kRevClassMask) >> kRevClassShift;
-                    if (cls == kMulNone && !stale) continue;
+                };
+                // the derivative columns of entry k; the winner passes the adjoint on
+                int32_t wslot[kSpl];
+#pragma unroll
+                for (int i = 0; i < kSpl; ++i) wslot[i] = -1;
+                auto entry = [&](const Record &q, int k, const int32_t (&win)[kSpl], const double (&dk)[kSpl]) {
+                    const bool stale = (q.info & kRevStale) != 0;
+                    bool mine[kSpl];
+#pragma unroll
+                    for (int i = 0; i < kSpl; ++i) {
+                        mine[i] = win[i] == k;
+                        if (mine[i] && !stale) wslot[i] = k;
+                    }
+                    const uint32_t cls = (q.info & kRevClassMask) >> kRevClassShift;
+                    if (cls == kMulNone && !stale) return;
                     double dm = 0.0, ds = 0.0;
-                    if (any_lane(mine)) {
+                    if (any_of<kSpl>(mine)) {
                         if (cls != kMulNone) {
-                            double mult = 1.0;
-                            if (cls == kMulRead) {
-                                mult = p.durations[row_offset(pr.i2, ld) + s] * pr.a;
-                            } else if (cls == kMulDraw) {
-                                const uint32_t seed[1] = {(uint32_t)p.seeds[s]};
-                                double unit[1];
-                                draw_durations<1>(pr.info & ~(kRevClassMask | kRevDup | kRevStale), 1.0, pr.b,
-                                                  (uint32_t)pr.i2, seed, as_const_space(p.alias),
-                                                  as_const_space(p.dists), unit);
-                                mult = unit[0];
-                            }
-                            dm = wave_sum(mine ? gt * mult : 0.0);
+                            double v[kSpl];
+#pragma unroll
+                            for (int i = 0; i < kSpl; ++i) v[i] = mine[i] ? gt[i] * (cls == kMulOne ? 1.0 : dk[i] * q.a) : 0.0;
+                            dm = tile_sum<kSpl>(v);
                         }
-                        if (stale) ds = wave_sum(mine ? gt : 0.0);
+                        if (stale) {
+                            double v[kSpl];
+#pragma unroll
+                            for (int i = 0; i < kSpl; ++i) v[i] = mine[i] ? gt[i] : 0.0;
+                            ds = tile_sum<kSpl>(v);
+                        }
                     }
-                    if (lane == 0) {
-                        prow[pr.i1] = dm;
-                        if (stale) prow[pr.i1 + 1] = ds;
+                    st.put(q.i1, dm, prow, lane);
+                    if (stale) st.put(q.i1 + 1, ds, prow, lane);
+                };
+
+                // predecessors in groups of four: the winner first, then the columns
+                int32_t win[kSpl];
+#pragma unroll
+                for (int i = 0; i < kSpl; ++i) win[i] = -1;
+                if (any_open) {
+                    for (int k0 = 0; k0 < n_pred; k0 += kFastPreds) {
+                        Record q[kFastPreds];
+#pragma unroll
+                        for (int u = 0; u < kFastPreds; ++u)
+                            if (k0 + u < n_pred) q[u] = recs[pb + k0 + u];
+#pragma unroll
+                        for (int u = 0; u < kFastPreds; ++u) {
+                            if (k0 + u < n_pred) {
+                                bool m[kSpl];
+                                wins(q[u], m);
+#pragma unroll
+                                for (int i = 0; i < kSpl; ++i) win[i] = m[i] ? k0 + u : win[i];
+                            }
+                        }
                     }
                 }
-                const double se = any_lane(ge != 0.0) ? wave_sum(ge) : 0.0;
-                if (lane == 0) prow[p.L + ev] = se;
-                gres[j] = gt;
-                wres[j] = wslot;
-                h = pb + n_pred;
+                for (int k0 = 0; k0 < n_pred; k0 += kFastPreds) {
+                    Record q[kFastPreds];
+#pragma unroll
+                    for (int u = 0; u < kFastPreds; ++u)
+                        if (k0 + u < n_pred) q[u] = recs[pb + k0 + u];
+#pragma unroll
+                    for (int u = 0; u < kFastPreds; ++u) {
+                        if (k0 + u < n_pred) {
+                            double dk[kSpl];
+                            if (k0 == 0 && u < kPre) {
+#pragma unroll
+                                for (int i = 0; i < kSpl; ++i) dk[i] = cur.d[u < kPre ? u : 0][i];
+                            } else if (reads_row(q[u])) {
+                                mult_values(q[u], dk);
+                            } else {
+#pragma unroll
+                                for (int i = 0; i < kSpl; ++i) dk[i] = 0.0;
+                            }
+                            entry(q[u], k0 + u, win, dk);
+                        }
+                    }
+                }
+                st.put(p.L + ev, any_of<kSpl>(ge_nz) ? tile_sum<kSpl>(ge) : 0.0, prow, lane);
+#pragma unroll
+                for (int i = 0; i < kSpl; ++i) {
+                    gres[j][i] = gt[i];
+                    wres[j][i] = wslot[i];
+                }
+                cur = nxt;
+                loaded = next;
             }
+            st.flush(prow, lane);
             // Every slot read of this step is done before any is overwritten,
             // and every commit is visible before the next step reads it.
             __syncthreads();
-#pragma unroll
-            for (int j = 0; j < kRevEventsPerWave; ++j) {
-                if (j < ne) {
-                    const int32_t slot = ev_slot[e0 + j];
-                    if (slot >= 0) sl.write(slot, gres[j], wres[j]);
-                }
+#pragma unroll 1
+            for (int j = 0; j < ne; ++j) {
+                const int32_t slot = ev_slot[e0 + j];
+                if (slot >= 0) sl.write(slot, gres[j], wres[j]);
             }
             __syncthreads();
         }
-        const double so = wave_sum(obj);
-        if (lane == 0) prow[p.obj_col + wave] = so;
+        st.put(p.obj_col + wave, tile_sum<kSpl>(obj), prow, lane);
+        st.flush(prow, lane);
     }
+}
+
+// Unit durations of the links that need them (relative kinds with minimal
+// duration 0): out[q][i] = the duration link q's draw gives sample
+// first + i (clamped to n - 1) at minimal duration 1, i.e. 1 + X rounded once.
+__global__ void __launch_bounds__(256) unit_draw_kernel(const Record *__restrict__ links, int32_t n_links,
+                                                        const int32_t *__restrict__ seeds, int32_t first, int32_t count,
+                                                        int32_t n, const DistParams *dists, const AliasRec *alias,
+                                                        double *__restrict__ out, uint32_t ld_out) {
+    const int32_t q = blockIdx.y;
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_links || i >= count) return;
+    const int32_t si = first + i < n ? first + i : n - 1;
+    const Record rec = as_const_space(links)[q];
+    const uint32_t seed[1] = {(uint32_t)seeds[si]};
+    double unit[1];
+    draw_durations<1>(rec.info, 1.0, rec.b, (uint32_t)rec.i2, seed, as_const_space(alias), as_const_space(dists), unit);
+    out[row_offset(q, ld_out) + i] = unit[0];
 }
 
 __global__ void __launch_bounds__(256) reduce_columns_kernel(const double *__restrict__ partial, int32_t n_tiles,
@@ -253,22 +461,74 @@ __global__ void __launch_bounds__(256) reduce_columns_kernel(const double *__res
     double *const out = o < L ? grad_dmin + o : (o < L + E ? grad_earliest + (o - L) : objective);
     const int32_t c0 = map_off[o], c1 = map_off[o + 1];
     double acc = accumulate ? *out : 0.0;
-    for (int32_t t = 0; t < n_tiles; ++t) {
-        const double *row = partial + (size_t)t * n_cols;
-        for (int32_t c = c0; c < c1; ++c) acc += row[map_col[c]];
+    if (c1 == c0 + 1) {  // one column (most outputs): 16 tiles' loads in flight, added in tile order
+        const double *col = partial + map_col[c0];
+        int32_t t = 0;
+        for (; t + 16 <= n_tiles; t += 16) {
+            double v[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) v[i] = col[(size_t)(t + i) * n_cols];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc += v[i];
+        }
+        for (; t < n_tiles; ++t) acc += col[(size_t)t * n_cols];
+    } else {
+        for (int32_t t = 0; t < n_tiles; ++t) {
+            const double *row = partial + (size_t)t * n_cols;
+            for (int32_t c = c0; c < c1; ++c) acc += row[map_col[c]];
+        }
     }
     *out = acc;
 }
 
 }  // namespace
 
-hipError_t launch_reverse(const RevParams &p, int grid, size_t lds_bytes, hipStream_t stream) {
-    hipLaunchKernelGGL(reverse_kernel, dim3((unsigned)grid), dim3((unsigned)(p.waves * kLanes)), lds_bytes, stream, p);
+template <int kSpl>
+const void *reverse_symbol(bool global_slots) {
+    return global_slots ? reinterpret_cast<const void *>(&reverse_kernel<true, kSpl>)
+                        : reinterpret_cast<const void *>(&reverse_kernel<false, kSpl>);
+}
+
+const void *reverse_symbol(bool global_slots, int spl) {
+    return spl == 2 ? reverse_symbol<2>(global_slots) : reverse_symbol<1>(global_slots);
+}
+
+hipError_t launch_reverse(const RevParams &p, int spl, int grid, size_t lds_bytes, hipStream_t stream) {
+    const bool gl = p.global_slots > 0;
+    const dim3 block((unsigned)(p.waves * kLanes));
+    if (spl == 2) {
+        if (gl)
+            hipLaunchKernelGGL((reverse_kernel<true, 2>), dim3((unsigned)grid), block, lds_bytes, stream, p);
+        else
+            hipLaunchKernelGGL((reverse_kernel<false, 2>), dim3((unsigned)grid), block, lds_bytes, stream, p);
+    } else {
+        if (gl)
+            hipLaunchKernelGGL((reverse_kernel<true, 1>), dim3((unsigned)grid), block, lds_bytes, stream, p);
+        else
+            hipLaunchKernelGGL((reverse_kernel<false, 1>), dim3((unsigned)grid), block, lds_bytes, stream, p);
+    }
     return hipGetLastError();
 }
 
-hipError_t reverse_occupancy(int waves, size_t lds_bytes, int *blocks_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, reverse_kernel, waves * kLanes, lds_bytes);
+hipError_t launch_unit_draws(const Record *links, int32_t n_links, const int32_t *seeds, int32_t first, int32_t count,
+                             int32_t n, const DistParams *dists, const AliasRec *alias, double *out, int32_t ld_out,
+                             hipStream_t stream) {
+    if (n_links <= 0 || count <= 0) return hipSuccess;
+    constexpr int32_t kMaxRows = 65535;  // grid y dimension per launch
+    for (int32_t q0 = 0; q0 < n_links; q0 += kMaxRows) {
+        const int32_t m = std::min(kMaxRows, n_links - q0);
+        hipLaunchKernelGGL(unit_draw_kernel, dim3((unsigned)((count + 255) / 256), (unsigned)m), dim3(256), 0, stream,
+                           links + q0, m, seeds, first, count, n, dists, alias, out + (size_t)q0 * (uint32_t)ld_out,
+                           (uint32_t)ld_out);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t reverse_occupancy(int waves, size_t lds_bytes, bool global_slots, int spl, int *blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, reverse_symbol(global_slots, spl),
+                                                        waves * kLanes, lds_bytes);
 }
 
 hipError_t launch_reduce_columns(const double *partial, int32_t n_tiles, int32_t n_cols, const int32_t *map_off,

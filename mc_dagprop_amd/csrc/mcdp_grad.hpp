@@ -23,18 +23,18 @@ namespace mcdp {
 // Reverse-walk records (the 32-byte Record of the forward plan).
 //   header:      a = earliest, b = earliest + max_delay, i0 = event,
 //                i1 = its slot (-1: no predecessor reads its adjoint),
-//                i2 = number of successor records that follow,
-//                info = kInfoHeader | number of predecessor records after those;
-//   successor:   i0 = a target event t that lists this event as a predecessor,
-//                i1 = t's slot, i2 = the position of this event in t's list
-//                (one record per position: a source listed twice has two);
+//                i2 = number of successor entries,
+//                info = kInfoHeader | number of predecessor records that follow;
+//   successors:  not records but (slot, position) pairs in `succ`, from
+//                ev_succ: per target event t that lists this event as a
+//                predecessor, t's slot and this event's position in t's list
+//                (one pair per position: a source listed twice has two);
 //   predecessor: in the event's input order; i0 = source, i1 = column of the
 //                minimal-duration derivative (-1: none; for a stale entry the
 //                first of two columns, the second for the source's earliest
-//                time), i2 = edge (durations row and draw key), a = 1 / d
-//                (kMulRead), b = p0 of the distribution (kMulDraw), info =
-//                the forward record's kind, flags and distribution index with
-//                the bits below.
+//                time), i2 = edge (durations row), a = 1 / d (kMulRead) or 1
+//                (kMulDraw), info = the forward record's kind, flags and
+//                distribution index (kMulDraw: unit row) with the bits below.
 enum : uint32_t {
     kRevClassShift = 4,
     kRevClassMask = 3u << kRevClassShift,  // d duration / d minimal duration:
@@ -46,7 +46,8 @@ enum RevMul : uint32_t {
     kMulNone = 0,  // edge outside the link table: no minimal duration
     kMulOne = 1,   // duration = d or d + X
     kMulRead = 2,  // duration = d + X d, d != 0: duration * (1 / d)
-    kMulDraw = 3,  // duration = d + X d, d == 0: 1 + X, drawn again from the seed
+    kMulDraw = 3,  // duration = d + X d, d == 0: 1 + X, drawn again from the seed into a row of
+                   // unit durations (record bits 13..30: the row, a = 1)
 };
 
 constexpr int kRevEventsPerWave = 8;  // events of one wave and step (compile-time bound)
@@ -56,33 +57,43 @@ struct RevParams {
     const Record *recs;
     const int32_t *ev_off;    // [n_steps * waves + 1]: range of (step, wave) in ev_list
     const int32_t *ev_list;   // events in stream order
+    const int32_t *ev_rec;    // their header records
+    const int32_t *ev_succ;   // their first successor entry
+    const int2 *succ;         // successor entries: (target's slot, position in the target's list)
     const int32_t *ev_slot;   // their slots (-1: none)
     const DistParams *dists;  // the forward plan's distributions and alias tables (kMulDraw)
     const AliasRec *alias;
     const double *earliest;   // [E] (repeated sources read stale predecessors' earliest time)
     int32_t n_steps;
     int32_t waves;
-    int32_t lds_slots;        // slots in LDS: 64 adjoints (double) + 64 winner positions (int32) each
+    int32_t lds_slots;        // slots in LDS: 64 spl adjoints (double) + 64 spl winner positions (int32) each
     int32_t global_slots;     // further slots in global memory, per workgroup
     int32_t L;                // column of event e's earliest-time derivative: L + e
     int32_t n_cols;           // columns of a partial row
     int32_t obj_col;          // first of `waves` objective columns
-    int32_t tile0;            // first tile of this launch
-    const int32_t *seeds;     // [n] (kMulDraw entries only)
+    int32_t tile0;            // first tile (of 64 spl samples) of this launch
     const double *weights;    // [E]
     const double *realized;   // [E][ld]
     const double *durations;  // [L][ld]
     const int32_t *cause;     // [E][ld]
+    const double *unit;       // [unit links][ld_unit]: unit durations for this launch's tiles (kMulDraw)
+    int32_t ld_unit;
     int32_t n;                // samples of the whole call
     int32_t ld;
     int32_t n_tiles;          // tiles of this launch
     double *partial;          // [n_tiles][n_cols]
-    double *gslots;           // per workgroup: [global_slots][64] adjoints, then [global_slots][64] int32
-                              // positions, in 2 * global_slots * 64 doubles
+    double *gslots;           // per workgroup: [global_slots][64 spl] adjoints, then [global_slots][64 spl]
+                              // int32 positions, in 2 * global_slots * 64 spl doubles
 };
 
-hipError_t launch_reverse(const RevParams &p, int grid, size_t lds_bytes, hipStream_t stream);
-hipError_t reverse_occupancy(int waves, size_t lds_bytes, int *blocks_per_cu);
+// spl: samples per lane (1 or 2), so a tile is 64 spl samples.
+hipError_t launch_reverse(const RevParams &p, int spl, int grid, size_t lds_bytes, hipStream_t stream);
+// Unit durations of `n_links` links (their forward records) for samples
+// first .. first + count - 1 (clamped to n - 1): out[q * ld_out + i].
+hipError_t launch_unit_draws(const Record *links, int32_t n_links, const int32_t *seeds, int32_t first, int32_t count,
+                             int32_t n, const DistParams *dists, const AliasRec *alias, double *out, int32_t ld_out,
+                             hipStream_t stream);
+hipError_t reverse_occupancy(int waves, size_t lds_bytes, bool global_slots, int spl, int *blocks_per_cu);
 // out_X[o] (+)= sum over tiles t, in order, of sum over map_col[map_off[o] ..
 // map_off[o + 1]) of partial[t][col]; outputs o < L go to grad_dmin, the next
 // E to grad_earliest, the last one to objective.

@@ -48,6 +48,7 @@ struct DeviceCopy {
     RevParams rev_base{};
     const int32_t *rev_map_off = nullptr;  // [L + E + 2] columns of each output
     const int32_t *rev_map_col = nullptr;
+    const Record *rev_unit_links = nullptr;
     void *rev_work = nullptr;            // partial rows and global slots
     size_t rev_work_bytes = 0;
 };
@@ -57,10 +58,12 @@ struct DeviceCopy {
 // events, with the records, slots and partial-row columns of each event.
 struct RevPlan {
     int32_t waves = 1, n_steps = 0, slots = 0, n_cols = 0, obj_col = 0;
-    bool needs_seeds = false;  // some entry draws its multiplier again
-    std::vector<int32_t> step_off, ev_off, ev_list, ev_slot;
+    int32_t spl = 1;  // samples per lane of the reverse kernel
+    std::vector<int32_t> step_off, ev_off, ev_list, ev_rec, ev_succ, ev_slot;
+    std::vector<int32_t> succ;  // (target slot, position) pairs
     std::vector<Record> recs;
     std::vector<int32_t> map_off, map_col;  // outputs: L links, E events, the objective
+    std::vector<Record> unit_links;         // forward records of the links drawn again (kMulDraw)
 };
 
 // One predecessor entry as the reference evaluates it.
@@ -93,6 +96,7 @@ struct mcdp_propagator {
     int64_t n_global = 0;
     bool shape_fixed = false;  // set by configure / autotune / the first launch
     std::shared_ptr<mcdp::RevPlan> rev;  // built on the first gradient call
+    int64_t rev_max_tiles = 0;           // test hook: tiles per reverse launch (0: default)
     // device copies, one per device used
     std::mutex mu;
     std::vector<mcdp::DeviceCopy> dev;

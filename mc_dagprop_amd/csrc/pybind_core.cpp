@@ -12,6 +12,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <optional>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -418,6 +419,46 @@ PYBIND11_MODULE(_core, m) {
             "as numpy.quantile's default method, and cause_counts, where event e's counters "
             "cause_counts[cause_offsets[e]:cause_offsets[e+1]] count cause -1 and then each predecessor "
             "position")
+        .def(
+            "set_times",
+            [](Propagator &p, std::optional<std::vector<double>> dmin, std::optional<std::vector<double>> earliest) {
+                if (dmin && (int64_t)dmin->size() != p.activity_count())
+                    throw std::runtime_error("minimal_durations must have activity_count() entries");
+                if (earliest && (int64_t)earliest->size() != p.node_count())
+                    throw std::runtime_error("earliest must have node_count() entries");
+                check(mcdp_propagator_set_times(p.p, dmin ? dmin->data() : nullptr,
+                                                earliest ? earliest->data() : nullptr));
+            },
+            py::arg("minimal_durations") = py::none(), py::arg("earliest") = py::none(),
+            "Replaces the minimal duration of every link (by Activity.idx) and/or the earliest time of every "
+            "event; the compiled schedule is kept")
+        .def(
+            "get_times",
+            [](const Propagator &p) {
+                py::array_t<double> d(p.activity_count()), e(p.node_count());
+                check(mcdp_propagator_get_times(p.p, d.mutable_data(), e.mutable_data()));
+                return py::make_tuple(d, e);
+            },
+            "(minimal duration per link, earliest time per event)")
+        .def(
+            "gradient",
+            [](Propagator &p, const std::vector<int> &seeds, const std::vector<double> &weights, int64_t chunk) {
+                if ((int64_t)weights.size() != p.node_count())
+                    throw std::runtime_error("weights must have node_count() entries");
+                std::vector<int32_t> s(seeds.begin(), seeds.end());
+                py::array_t<double> gd(p.activity_count()), ge(p.node_count());
+                double obj = 0.0, *pd = gd.mutable_data(), *pe = ge.mutable_data();
+                int rc;
+                {
+                    py::gil_scoped_release nogil;
+                    rc = mcdp_gradient_host(p.p, s.data(), (int64_t)s.size(), weights.data(), chunk, &obj, pd, pe);
+                }
+                check(rc);
+                return py::make_tuple(obj, gd, ge);
+            },
+            py::arg("seeds"), py::arg("weights"), py::arg("chunk") = 0,
+            "Mean over the seeds of J = sum_e w_e max(0, realized_e - earliest_e) and its pathwise gradient: "
+            "(J, dJ/d minimal_duration per link, dJ/d earliest per event)")
         .def(
             "configure",
             [](Propagator &p, int waves, int events_per_wave, int samples_per_lane) {

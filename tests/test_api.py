@@ -168,6 +168,27 @@ def test_no_cpu_fallback() -> None:
         sim.run(seed=1)
     with pytest.raises(RuntimeError, match="GPU|device"):
         sim.run_many([1, 2])
+    with pytest.raises(RuntimeError, match="GPU|device"):
+        sim.gradient([1, 2], [1.0] * 6)
+
+
+def test_set_times_is_host_side() -> None:
+    """Replacing minimal durations (per link) and earliest times needs no device; wrong lengths raise."""
+    gen = GenericDelayGenerator()
+    gen.add_constant(1, 1.0)
+    sim = Simulator(_ctx6(), gen)
+    d, e = sim.get_times()
+    np.testing.assert_array_equal(d, [3.0, 5.0, 5.0, 15.0, 10.0])
+    np.testing.assert_array_equal(e, [0.0, 5.0, 10.0, 22.0, 20.0, 100.0])
+    sim.set_times(minimal_durations=[1.0, 2.0, 3.0, 4.0, 5.0])
+    sim.set_times(earliest=[0.5] * 6)
+    d, e = sim.get_times()
+    np.testing.assert_array_equal(d, [1.0, 2.0, 3.0, 4.0, 5.0])
+    np.testing.assert_array_equal(e, [0.5] * 6)
+    with pytest.raises(RuntimeError):
+        sim.set_times(minimal_durations=[1.0])
+    with pytest.raises(RuntimeError):
+        sim.set_times(earliest=[1.0] * 7)
 
 
 def test_plan_is_host_side() -> None:

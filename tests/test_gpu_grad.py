@@ -100,6 +100,25 @@ def test_chunking_and_repeats_are_bit_exact(gpu) -> None:
         np.testing.assert_array_equal(bits(got[2]), bits(ref[2]))
 
 
+def test_several_reverse_launches_per_call(gpu) -> None:
+    """A reverse call split into launches of 3 tiles (tile offsets, unit rows per launch) gives the sums of one
+    launch bit for bit, also with unit-duration links."""
+    dag = layered_small()
+    prop = capi.Propagator(dag)
+    dmin, e = prop.get_times()
+    dmin[::4] = 0.0
+    prop.set_times(dmin, e - 300.0)
+    w = np.random.default_rng(8).uniform(0.5, 2.0, prop.E)
+    seeds = np.arange(1000, dtype=np.int32)
+    ref = prop.gradient(seeds, w)
+    capi.check(capi.lib().mcdp_debug_reverse_tiles(prop.h, 3))
+    got = prop.gradient(seeds, w)
+    capi.check(capi.lib().mcdp_debug_reverse_tiles(prop.h, 0))
+    assert bits([got[0]]) == bits([ref[0]])
+    np.testing.assert_array_equal(bits(got[1]), bits(ref[1]))
+    np.testing.assert_array_equal(bits(got[2]), bits(ref[2]))
+
+
 def test_set_times_matches_a_fresh_compile(gpu) -> None:
     dag = D.continuous_dag(5, 60, max_delay=20.0, shared_links=True)
     prop = capi.Propagator(dag)
@@ -219,3 +238,31 @@ def test_common_random_number_finite_differences(gpu) -> None:
         minus.set_times(None, em)
         fd = fd_objective_difference(plus, minus, seeds, w, ep, em) / ((ep[e] - em[e]) * n)
         assert fd == pytest.approx(ge[e], rel=2e-3, abs=1e-6), f"event {e}: fd {fd} vs {ge[e]}"
+
+
+def test_autograd_function_on_the_simulator(gpu) -> None:
+    """torch.autograd over the drop-in Simulator: J and both gradients equal the C ABI's, bit for bit."""
+    import torch
+
+    import mc_dagprop_amd as api
+    from mc_dagprop_amd.grad import expected_lateness
+
+    dag = layered_small()
+    ctx, gen = dag.to_api(api)
+    sim = api.Simulator(ctx, gen)
+    d0, e0 = sim.get_times()
+    dmin = torch.tensor(d0, dtype=torch.float64, requires_grad=True)
+    earliest = torch.tensor(e0, dtype=torch.float64, requires_grad=True)
+    w = np.random.default_rng(1).uniform(0.5, 2.0, len(e0))
+    seeds = list(range(2000))
+    J = expected_lateness(sim, dmin, earliest, seeds, w)
+    (2.0 * J).backward()
+    obj, gd, ge = capi.Propagator(dag).gradient(seeds, w)
+    assert J.item() == obj
+    np.testing.assert_array_equal(bits(dmin.grad.numpy()), bits(2.0 * gd))
+    np.testing.assert_array_equal(bits(earliest.grad.numpy()), bits(2.0 * ge))
+    # a step of gradient descent on the earliest times lowers the sampled lateness (common random numbers)
+    with torch.no_grad():
+        e1 = earliest - 0.01 * earliest.grad.sign()
+    J1 = expected_lateness(sim, dmin.detach(), e1, seeds, w)
+    assert J1.item() < J.item()
