@@ -21,11 +21,13 @@ Also reported:
   check         device durations of a few samples fed to the CPU oracle
                 reproduce the device's realized / cause bit-exactly.
 
-    python bench.py [--steps K] [--warmup W] [--samples S] [--workload network|small|batch|grad]
+    python bench.py [--steps K] [--warmup W] [--samples S] [--workload network|small|batch|grad|serve]
 
 --workload grad times the pathwise-gradient step (run_many + the reverse walk) on the
 network DAG with 65,536 samples, with the reverse walk's roofline and a check of its
-gradient against the FP64 CPU oracle.
+gradient against the FP64 CPU oracle.  --workload serve runs a Poisson stream of
+statistics requests over 200 resident contexts through the serving loop and reports
+p50/p99 latency beside the aggregate throughput.
 """
 
 from __future__ import annotations
@@ -153,6 +155,111 @@ def bench_batch(args) -> int:
         "one_launch_per_dag_ms": ms_solo,
         "speedup_vs_one_launch_per_dag": ms_solo / ms_batch,
         "check": {"bit_exact": ok},
+    }
+    print(json.dumps(out), flush=True)
+    return 0 if ok else 3
+
+
+def bench_serve(args) -> int:
+    """A Poisson stream of statistics requests over resident contexts, served with footprint admission and
+    continuous admission (mcdp_server_*): p50/p99 latency from submission to results on the host, and the
+    aggregate activity-samples/s over the stream."""
+    import threading
+
+    from mc_dagprop_amd import capi
+    from mc_dagprop_amd.workloads import layered_vectorized, network_dists
+
+    rng = np.random.default_rng(20260003)
+    n_ctx = args.contexts
+    dists = network_dists()
+    events = np.exp(rng.uniform(np.log(1000.0), np.log(200000.0), n_ctx))
+    layers = np.maximum(10, np.round(events / 100.0)).astype(int)
+    t_build = time.perf_counter()
+    props = [capi.Propagator(layered_vectorized(int(m), 100, 4, dists, 8, seed=7000 + i)) for i, m in enumerate(layers)]
+    budget = int(args.budget_gb * (1 << 30))
+    cap = budget // 4  # one request's share of the pool, at most
+    srv = capi.Server(budget, max_running=8, n_streams=4)
+    ids = [srv.add_context(p) for p in props]
+    log(f"serve: {n_ctx} contexts ({sum(p.E for p in props)} events) compiled and resident in "
+        f"{time.perf_counter() - t_build:.1f} s; pool {args.budget_gb:.0f} GiB, at most {cap / (1 << 30):.0f} GiB "
+        "per request")
+    q = (0.5, 0.9, 0.99)
+    # warm-up (not timed): one small request per context size class
+    for c in range(0, n_ctx, max(1, n_ctx // 8)):
+        srv.wait(srv.submit(ids[c], np.arange(256, dtype=np.int32), q))
+
+    N, rate = args.requests, args.rate
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, N))
+    ctx_of = rng.integers(0, n_ctx, N)
+    n_of = np.exp(rng.uniform(np.log(16384.0), np.log(262144.0), N)).astype(np.int64)
+    clipped = 0
+    for i in range(N):
+        E = props[ctx_of[i]].E
+        n_max = max(64, (cap - 64 * (1 << 20)) // (12 * E))
+        if n_of[i] > n_max:
+            n_of[i] = n_max
+            clipped += 1
+    seeds_of = [np.arange(1_000_000 * i, 1_000_000 * i + n_of[i], dtype=np.int32) for i in range(N)]
+    work = float(sum(props[ctx_of[i]].L * n_of[i] for i in range(N)))
+    rids = [None] * N
+    lat = np.zeros(N)
+    keep = {}
+    check_idx = set(int(i) for i in rng.choice(np.nonzero(n_of <= 40000)[0], min(3, int(np.sum(n_of <= 40000))),
+                                                 replace=False))
+    t0 = time.perf_counter()
+
+    def submitter() -> None:
+        for i in range(N):
+            delay = t0 + arrivals[i] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            rids[i] = srv.submit(ids[ctx_of[i]], seeds_of[i], q)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    for i in range(N):
+        while rids[i] is None:
+            time.sleep(0.0005)
+        mean, quant, counts, lat[i] = srv.wait(rids[i])
+        if i in check_idx:
+            keep[i] = (mean, quant, counts)
+    wall = time.perf_counter() - t0
+    th.join()
+    st = srv.stats()
+    srv.close()
+    # each checked request, run alone, gives the same statistics bit for bit
+    ok = True
+    for i, (mean, quant, counts) in keep.items():
+        m2, q2, c2 = props[ctx_of[i]].run_stats(seeds_of[i], q)
+        ok = ok and np.array_equal(mean.view(np.uint64), m2.view(np.uint64)) and np.array_equal(counts, c2) and bool(
+            np.all((quant.view(np.uint64) == q2.view(np.uint64)) | ((quant == 0) & (q2 == 0))))
+    p50, p99 = np.percentile(lat, [50, 99])
+    log(f"serve: {N} requests at {rate:g}/s over {wall:.2f} s: p50 {p50:.1f} ms, p99 {p99:.1f} ms, "
+        f"{work / wall:.3e} activity-samples/s; {clipped} requests clipped to the per-request memory share; "
+        f"at most {st['max_running']} running at once; checks bit-exact: {ok}")
+    out = {
+        "metric": "activity-samples/sec GPU vs host run_many (bit-exact); p50/p99 solve latency",
+        "value": work / wall,
+        "unit": "activity-samples/s",
+        "n_gpus": 1,
+        "steps": N,
+        "warmup": 0,
+        "ms_per_step": 1000.0 * wall / N,
+        "mode": "serve",
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (layered DAGs from numpy PCG64; seeds disjoint per request)",
+        "config": {"workload": f"Poisson requests at {rate:g}/s over {n_ctx} resident contexts (1k-200k events, "
+                               "log-uniform; width 100, 1-4 preds, 8 activity types); 16,384-262,144 samples per "
+                               "request (log-uniform, clipped to a quarter of the memory pool); per-event mean, "
+                               "quantiles 0.5/0.9/0.99 and cause counts",
+                   "contexts": n_ctx, "requests": N, "rate_per_s": rate, "pool_gib": args.budget_gb,
+                   "clipped_requests": clipped},
+        "latency_ms": {"p50": float(p50), "p99": float(p99), "mean": float(lat.mean()), "max": float(lat.max())},
+        "server": st,
+        "check": {"requests_checked": len(keep), "bit_exact_alone_vs_mix": ok},
     }
     print(json.dumps(out), flush=True)
     return 0 if ok else 3
@@ -295,11 +402,15 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--samples", type=int, default=None, help="samples per step (100,000; 65,536 for grad)")
-    ap.add_argument("--workload", choices=["network", "small", "batch", "grad"], default="network",
+    ap.add_argument("--workload", choices=["network", "small", "batch", "grad", "serve"], default="network",
                     help="batch: 16 layered DAGs of 1,000-10,000 events (log-uniform) with 4,096-32,768 samples "
                     "each (log-uniform), run in one batched launch and, for comparison, one launch per DAG; "
                     "grad: run_many + the reverse walk (pathwise gradient) on the network DAG, 65,536 samples")
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
+    ap.add_argument("--contexts", type=int, default=200, help="serve: resident contexts")
+    ap.add_argument("--requests", type=int, default=300, help="serve: requests in the stream")
+    ap.add_argument("--rate", type=float, default=12.0, help="serve: Poisson arrival rate (requests/s)")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: device memory pool (GiB)")
     ap.add_argument("--check-samples", type=int, default=64)
     ap.add_argument("--mode", choices=["run_many", "propagate"], default="run_many",
                     help="run_many: sample + propagate (the metric); propagate: propagation only, durations "
@@ -350,6 +461,8 @@ def main() -> int:
         return bench_batch(args)
     if args.workload == "grad":
         return bench_grad(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     dag = layered_network() if args.workload == "network" else layered_small()
     prop = capi.Propagator(dag)

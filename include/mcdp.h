@@ -240,6 +240,38 @@ int mcdp_propagator_gradient_info(mcdp_propagator *prop, int64_t *out, int n_out
 int mcdp_gradient_host(mcdp_propagator *prop, const int32_t *seeds, int64_t n, const double *weights, int64_t chunk,
                        double *objective, double *grad_dmin, double *grad_earliest);
 
+/* ---- Serving (no counterpart in the reference) --------------------------- */
+/* A stream of statistics requests over resident contexts.  The reference
+ * answers one run_many call at a time on the calling thread
+ * (_core.cpp:355-361) and leaves statistics to its callers.  A server owns one
+ * device memory pool of `memory_budget` bytes and a scheduler thread.  Each
+ * request reserves its footprint (seeds, realized and cause rows, statistics:
+ * about 12 * node_count() bytes per sample) from the pool; requests start in
+ * arrival order as soon as the pool and one of `max_running` slots allow,
+ * while others run, on `n_streams` streams (work of different requests
+ * overlaps on the device).  A request's results equal mcdp_run_stats_host on
+ * the same seeds.  All entry points are thread-safe. */
+typedef struct mcdp_server mcdp_server;
+int mcdp_server_create(int64_t memory_budget, int32_t max_running, int32_t n_streams, mcdp_server **out);
+/* Waits for running requests; fails queued ones; frees the pool. */
+void mcdp_server_destroy(mcdp_server *srv);
+/* Makes a context resident (plan and predecessor table uploaded now) and
+ * returns its id.  The propagator must outlive the server. */
+int mcdp_server_add_context(mcdp_server *srv, mcdp_propagator *prop, int32_t *ctx_id);
+/* Queues run_many over seeds[0..n) of context ctx_id and the statistics of
+ * mcdp_run_stats_host (n_q <= 8 quantiles); returns at once.  Fails with
+ * MCDP_E_NOMEM if the request could never fit the budget. */
+int mcdp_server_submit(mcdp_server *srv, int32_t ctx_id, const int32_t *seeds, int64_t n, int32_t n_q,
+                       const double *q, int64_t *request_id);
+/* Blocks until the request is done and writes its results (layouts of
+ * mcdp_run_stats_host; any may be NULL) and its latency from submission to
+ * completion; the request is then forgotten. */
+int mcdp_server_wait(mcdp_server *srv, int64_t request_id, double *mean, double *quantiles, int64_t *cause_counts,
+                     double *latency_ms);
+/* out[0..7] = {submitted, completed, failed, queued, running, most running at
+ * once, pool bytes in use, peak pool bytes in use}. */
+int mcdp_server_stats(mcdp_server *srv, int64_t *out, int n_out);
+
 /* ---- Device plumbing for callers without their own HIP allocator -------- */
 /* (no counterpart in the reference, which has no device) */
 int mcdp_device_count(int *count);

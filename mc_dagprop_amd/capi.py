@@ -80,6 +80,12 @@ _SIGNATURES = {
     "mcdp_gradient_host": (C.c_int, [_P, _pi32, _i64, _pf64, _i64, _pf64, _pf64, _pf64]),
     "mcdp_debug_reverse_tiles": (C.c_int, [_P, _i64]),
     "mcdp_propagator_gradient_info": (C.c_int, [_P, _pi64, C.c_int]),
+    "mcdp_server_create": (C.c_int, [_i64, _i32, _i32, C.POINTER(_P)]),
+    "mcdp_server_destroy": (None, [_P]),
+    "mcdp_server_add_context": (C.c_int, [_P, _P, C.POINTER(_i32)]),
+    "mcdp_server_submit": (C.c_int, [_P, _i32, _pi32, _i64, _i32, _pf64, C.POINTER(_i64)]),
+    "mcdp_server_wait": (C.c_int, [_P, _i64, _pf64, _pf64, _pi64, C.POINTER(_f64)]),
+    "mcdp_server_stats": (C.c_int, [_P, _pi64, C.c_int]),
 }
 
 _lib = None
@@ -456,6 +462,61 @@ class Batch:
         if getattr(self, "h", None) and self.h.value and _lib is not None:
             _lib.mcdp_batch_destroy(self.h)
             self.h = C.c_void_p(0)
+
+
+class Server:
+    """Statistics requests over resident contexts, admitted by device-memory footprint (mcdp_server_*)."""
+
+    STAT_FIELDS = ["submitted", "completed", "failed", "queued", "running", "max_running", "bytes_in_use",
+                   "peak_bytes"]
+
+    def __init__(self, memory_budget: int, max_running: int = 8, n_streams: int = 4) -> None:
+        self.h = C.c_void_p(0)
+        check(lib().mcdp_server_create(int(memory_budget), max_running, n_streams, C.byref(self.h)))
+        self._ctx: list[Propagator] = []
+        self._req: dict[int, tuple[int, int, int]] = {}
+
+    def add_context(self, prop: Propagator) -> int:
+        cid = _i32(0)
+        check(lib().mcdp_server_add_context(self.h, prop.h, C.byref(cid)))
+        self._ctx.append(prop)
+        return cid.value
+
+    def submit(self, ctx: int, seeds, q=(0.5, 0.9, 0.99)) -> int:
+        s = np.ascontiguousarray(seeds, dtype=np.int32)
+        qq = np.ascontiguousarray(q, dtype=np.float64)
+        rid = _i64(0)
+        check(lib().mcdp_server_submit(self.h, ctx, _ptr(s, _pi32), s.size, qq.size, _ptr(qq, _pf64), C.byref(rid)))
+        p = self._ctx[ctx]
+        self._req[rid.value] = (p.E, int(p.pred_offsets()[-1]), qq.size)
+        return rid.value
+
+    def wait(self, rid: int):
+        """(mean [E], quantiles [E, n_q], cause_counts [off[E] + E], latency_ms)."""
+        E, P, nq = self._req.pop(rid)
+        mean = np.empty(E)
+        quant = np.empty((E, nq))
+        counts = np.empty(E + P, dtype=np.int64)
+        lat = _f64(0)
+        check(lib().mcdp_server_wait(self.h, rid, _ptr(mean, _pf64), _ptr(quant, _pf64), _ptr(counts, _pi64),
+                                     C.byref(lat)))
+        return mean, quant, counts, lat.value
+
+    def stats(self) -> dict:
+        v = np.zeros(len(self.STAT_FIELDS), dtype=np.int64)
+        check(lib().mcdp_server_stats(self.h, _ptr(v, _pi64), v.size))
+        return dict(zip(self.STAT_FIELDS, map(int, v)))
+
+    def close(self) -> None:
+        if self.h:
+            lib().mcdp_server_destroy(self.h)
+            self.h = C.c_void_p(0)
+
+    def __del__(self) -> None:  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Timer:

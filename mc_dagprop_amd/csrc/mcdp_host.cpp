@@ -900,7 +900,19 @@ int run_host(mcdp_propagator &P, bool injected, const int32_t *seeds, const doub
 
 }  // namespace
 
-int mcdp::ensure_device(mcdp_propagator &P, DeviceCopy *&out) { return ensure_device_impl(P, out); }
+int mcdp::ensure_device(mcdp_propagator &P, DeviceCopy *&out) {
+    // The uploaded plan must be the one the walk kernel will run: fix the
+    // schedule shape first, as the first launch would.
+    if (!P.shape_fixed) {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+            return fail(MCDP_E_DEVICE, "no HIP device available: the Monte Carlo engine runs on the GPU only");
+        int rc = choose_shape(P);
+        if (rc) return rc;
+        P.shape_fixed = true;
+    }
+    return ensure_device_impl(P, out);
+}
 mcdp::WalkVariant mcdp::plan_variant(const mcdp_propagator &P, bool injected) { return walk_variant(P, injected); }
 size_t mcdp::plan_lds_bytes(const mcdp_propagator &P, WalkVariant v) { return walk_lds_bytes(P, v); }
 int mcdp::configure_shape(mcdp_propagator *prop, int32_t waves, int32_t k, int32_t spl) {

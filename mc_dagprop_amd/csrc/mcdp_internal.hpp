@@ -127,6 +127,9 @@ WalkVariant plan_variant(const mcdp_propagator &P, bool injected);
 size_t plan_lds_bytes(const mcdp_propagator &P, WalkVariant v);
 // mcdp_propagator_configure.
 int configure_shape(mcdp_propagator *prop, int32_t waves, int32_t k, int32_t spl);
+// Uploads P's plan and the statistics' predecessor table to the current
+// device (what a resident context needs before serving requests).
+int prepare_stats(mcdp_propagator &P);
 // The forward plan's record of link l (kind, flags, distribution index,
 // minimal duration, p0); an edge outside the link table is kind kNone.
 Record link_record(const mcdp_propagator &P, int32_t l);

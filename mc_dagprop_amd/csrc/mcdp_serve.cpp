@@ -1,0 +1,402 @@
+// Serving: per-event statistics for a stream of requests over resident
+// compiled contexts (C ABI in include/mcdp.h).  The reference answers one
+// run_many call at a time on the calling thread (_core.cpp:355-361) and leaves
+// statistics to its callers; here a scheduler thread owns the device:
+//   * contexts are uploaded once (plan + predecessor table) and stay in HBM;
+//   * every request reserves its footprint (seeds, realized and cause rows,
+//     statistics) from one device pool of a fixed budget; requests are
+//     admitted in arrival order as soon as the pool and a running slot allow,
+//     also while others are on the device (no batch boundary);
+//   * an admitted request runs run_many (no durations output) and the
+//     statistics kernels on one of a few streams, so requests overlap on the
+//     device, and copies its results into pinned host memory;
+//   * completion is seen through an event, which returns the memory at once.
+// A request's results are those of mcdp_run_stats_host on the same seeds, bit
+// for bit: the kernels do not depend on what else runs.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <limits>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/mcdp.h"
+#include "mcdp_internal.hpp"
+
+using namespace mcdp;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Best-fit allocator over [0, size) with coalescing free blocks.
+class Pool {
+   public:
+    explicit Pool(size_t size) : size_(size) {
+        if (size) free_[0] = size;
+    }
+    bool alloc(size_t bytes, size_t &off) {
+        bytes = align_up(std::max<size_t>(bytes, 1), 256);
+        auto best = free_.end();
+        for (auto it = free_.begin(); it != free_.end(); ++it)
+            if (it->second >= bytes && (best == free_.end() || it->second < best->second)) best = it;
+        if (best == free_.end()) return false;
+        off = best->first;
+        const size_t rest = best->second - bytes;
+        free_.erase(best);
+        if (rest) free_[off + bytes] = rest;
+        used_ += bytes;
+        peak_ = std::max(peak_, used_);
+        return true;
+    }
+    void release(size_t off, size_t bytes) {
+        bytes = align_up(std::max<size_t>(bytes, 1), 256);
+        used_ -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto next = std::next(it);
+        if (next != free_.end() && it->first + it->second == next->first) {
+            it->second += next->second;
+            free_.erase(next);
+        }
+        if (it != free_.begin()) {
+            auto prev = std::prev(it);
+            if (prev->first + prev->second == it->first) {
+                prev->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    size_t used() const { return used_; }
+    size_t peak() const { return peak_; }
+    size_t size() const { return size_; }
+
+   private:
+    size_t size_, used_ = 0, peak_ = 0;
+    std::map<size_t, size_t> free_;
+};
+
+enum class State { Queued, Running, Done, Failed };
+
+struct Request {
+    int64_t id = 0;
+    int32_t ctx = 0;
+    std::vector<int32_t> seeds;
+    std::vector<double> q;
+    int64_t E = 0, P = 0;
+    size_t bytes = 0, off = 0;
+    int stream = 0;
+    hipEvent_t done = nullptr;
+    double *h_out = nullptr;  // pinned: mean [E], quantiles [E * n_q], cause counts [E + P] (as int64)
+    Clock::time_point t_submit, t_start, t_end;
+    State state = State::Queued;
+    int rc = MCDP_OK;
+    std::string error;
+};
+
+// Device bytes of a request: seeds, realized and cause rows, statistics.
+size_t footprint(int64_t n, int64_t E, int64_t P, int64_t n_q) {
+    return align_up((size_t)n * 4, 256) + align_up((size_t)(E * n) * 8, 256) + align_up((size_t)(E * n) * 4, 256) +
+           align_up((size_t)E * 8, 256) + align_up((size_t)(E * n_q) * 8, 256) + align_up((size_t)(E + P) * 8, 256);
+}
+
+}  // namespace
+
+struct mcdp_server {
+    int device = 0;
+    void *pool_base = nullptr;
+    std::unique_ptr<Pool> pool;
+    int32_t max_running = 4;
+    std::vector<hipStream_t> streams;
+    std::vector<int> stream_load;
+    std::vector<mcdp_propagator *> contexts;
+
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request *> queued;
+    std::vector<Request *> running;
+    std::unordered_map<int64_t, std::unique_ptr<Request>> all;
+    int64_t next_id = 1;
+    int64_t n_submitted = 0, n_completed = 0, n_failed = 0, max_running_seen = 0;
+    bool stop = false;
+    std::thread worker;
+
+    void run();
+    void start(Request &r);
+};
+
+namespace {
+
+// Launches a request's work on its stream; the caller holds srv.mu.
+int launch_request(mcdp_server &srv, Request &r) {
+    mcdp_propagator *prop = srv.contexts[(size_t)r.ctx];
+    const int64_t n = (int64_t)r.seeds.size(), E = r.E, nq = (int64_t)r.q.size();
+    auto *base = static_cast<unsigned char *>(srv.pool_base) + r.off;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char *p = base + o;
+        o += align_up(bytes, 256);
+        return p;
+    };
+    auto *d_seeds = reinterpret_cast<int32_t *>(take((size_t)n * 4));
+    auto *d_real = reinterpret_cast<double *>(take((size_t)(E * n) * 8));
+    auto *d_cause = reinterpret_cast<int32_t *>(take((size_t)(E * n) * 4));
+    auto *d_mean = reinterpret_cast<double *>(take((size_t)E * 8));
+    auto *d_quant = reinterpret_cast<double *>(take((size_t)(E * nq) * 8));
+    auto *d_cnt = reinterpret_cast<int64_t *>(take((size_t)(E + r.P) * 8));
+    const hipStream_t s = srv.streams[(size_t)r.stream];
+    HIP_TRY(hipMemcpyAsync(d_seeds, r.seeds.data(), (size_t)n * 4, hipMemcpyHostToDevice, s), "copy seeds");
+    int rc = mcdp_run_many_device(prop, d_seeds, n, n, d_real, nullptr, d_cause, s);
+    if (rc) return rc;
+    rc = mcdp_event_stats_device(prop, d_real, d_cause, n, n, (int32_t)nq, r.q.data(), d_mean, nq ? d_quant : nullptr,
+                                 d_cnt, s);
+    if (rc) return rc;
+    double *h = r.h_out;
+    HIP_TRY(hipMemcpyAsync(h, d_mean, (size_t)E * 8, hipMemcpyDeviceToHost, s), "copy results");
+    if (nq) HIP_TRY(hipMemcpyAsync(h + E, d_quant, (size_t)(E * nq) * 8, hipMemcpyDeviceToHost, s), "copy results");
+    HIP_TRY(hipMemcpyAsync(h + E + E * nq, d_cnt, (size_t)(E + r.P) * 8, hipMemcpyDeviceToHost, s), "copy results");
+    HIP_TRY(hipEventCreateWithFlags(&r.done, hipEventDisableTiming), "hipEventCreate");
+    HIP_TRY(hipEventRecord(r.done, s), "hipEventRecord");
+    return MCDP_OK;
+}
+
+}  // namespace
+
+void mcdp_server::start(Request &r) {
+    int best = 0;
+    for (size_t i = 1; i < streams.size(); ++i)
+        if (stream_load[i] < stream_load[(size_t)best]) best = (int)i;
+    r.stream = best;
+    r.t_start = Clock::now();
+    r.rc = launch_request(*this, r);
+    if (r.rc) {
+        r.error = mcdp_last_error();
+        r.state = State::Failed;
+        // work already enqueued must finish before its memory is reused
+        (void)hipStreamSynchronize(streams[(size_t)r.stream]);
+        if (r.done) {
+            (void)hipEventDestroy(r.done);
+            r.done = nullptr;
+        }
+        pool->release(r.off, r.bytes);
+        r.t_end = Clock::now();
+        ++n_failed;
+        cv_done.notify_all();
+        return;
+    }
+    ++stream_load[(size_t)r.stream];
+    r.state = State::Running;
+    running.push_back(&r);
+    max_running_seen = std::max<int64_t>(max_running_seen, (int64_t)running.size());
+}
+
+void mcdp_server::run() {
+    (void)hipSetDevice(device);
+    std::unique_lock<std::mutex> lock(mu);
+    while (true) {
+        // completions
+        bool changed = false;
+        for (size_t i = 0; i < running.size();) {
+            Request &r = *running[i];
+            const hipError_t e = hipEventQuery(r.done);
+            if (e == hipErrorNotReady) {
+                ++i;
+                continue;
+            }
+            r.t_end = Clock::now();
+            if (e == hipSuccess) {
+                r.state = State::Done;
+                ++n_completed;
+            } else {
+                r.state = State::Failed;
+                r.rc = MCDP_E_DEVICE;
+                r.error = std::string("request failed on the device: ") + hipGetErrorString(e);
+                ++n_failed;
+            }
+            (void)hipEventDestroy(r.done);
+            r.done = nullptr;
+            pool->release(r.off, r.bytes);
+            --stream_load[(size_t)r.stream];
+            running.erase(running.begin() + (std::ptrdiff_t)i);
+            changed = true;
+        }
+        if (changed) cv_done.notify_all();
+        // admissions, in arrival order, while a slot and the memory allow
+        while (!queued.empty() && (int32_t)running.size() < max_running) {
+            Request &r = *queued.front();
+            size_t off = 0;
+            if (!pool->alloc(r.bytes, off)) break;
+            r.off = off;
+            queued.pop_front();
+            start(r);
+        }
+        if (stop && running.empty() && queued.empty()) break;
+        if (running.empty())
+            cv_work.wait(lock, [&] { return stop || !queued.empty(); });
+        else
+            cv_work.wait_for(lock, std::chrono::microseconds(50));
+    }
+}
+
+extern "C" {
+
+int mcdp_server_create(int64_t memory_budget, int32_t max_running, int32_t n_streams, mcdp_server **out) {
+    if (!out) return fail(MCDP_E_INVALID, "out is null");
+    *out = nullptr;
+    if (memory_budget <= 0) return fail(MCDP_E_INVALID, "memory_budget must be positive");
+    if (max_running < 1 || n_streams < 1 || n_streams > 16)
+        return fail(MCDP_E_INVALID, "need max_running >= 1 and 1 <= n_streams <= 16");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return fail(MCDP_E_DEVICE, "no HIP device available: serving runs on the GPU only");
+    auto srv = std::make_unique<mcdp_server>();
+    HIP_TRY(hipGetDevice(&srv->device), "hipGetDevice");
+    hipError_t e = hipMalloc(&srv->pool_base, (size_t)memory_budget);
+    if (e != hipSuccess)
+        return fail(MCDP_E_NOMEM, "hipMalloc (serving pool of " + std::to_string(memory_budget) +
+                                      " bytes): " + hipGetErrorString(e));
+    srv->pool = std::make_unique<Pool>((size_t)memory_budget);
+    srv->max_running = max_running;
+    for (int32_t i = 0; i < n_streams; ++i) {
+        hipStream_t s = nullptr;
+        e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            for (hipStream_t t : srv->streams) (void)hipStreamDestroy(t);
+            (void)hipFree(srv->pool_base);
+            return hip_fail(e, "hipStreamCreate");
+        }
+        srv->streams.push_back(s);
+        srv->stream_load.push_back(0);
+    }
+    mcdp_server *raw = srv.release();
+    raw->worker = std::thread([raw] { raw->run(); });
+    *out = raw;
+    return MCDP_OK;
+}
+
+void mcdp_server_destroy(mcdp_server *srv) {
+    if (!srv) return;
+    {
+        std::lock_guard<std::mutex> lock(srv->mu);
+        srv->stop = true;
+        for (Request *r : srv->queued) {  // never started: fail them
+            r->state = State::Failed;
+            r->rc = MCDP_E_INVALID;
+            r->error = "server destroyed before the request started";
+        }
+        srv->queued.clear();
+    }
+    srv->cv_work.notify_all();
+    if (srv->worker.joinable()) srv->worker.join();  // drains running requests
+    for (auto &kv : srv->all)
+        if (kv.second->h_out) (void)hipHostFree(kv.second->h_out);
+    for (hipStream_t s : srv->streams) (void)hipStreamDestroy(s);
+    (void)hipFree(srv->pool_base);
+    delete srv;
+}
+
+int mcdp_server_add_context(mcdp_server *srv, mcdp_propagator *prop, int32_t *ctx_id) {
+    if (!srv || !prop || !ctx_id) return fail(MCDP_E_INVALID, "null argument");
+    int rc = prepare_stats(*prop);  // plan and predecessor table resident from now on
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(srv->mu);
+    if (srv->contexts.size() >= (size_t)std::numeric_limits<int32_t>::max())
+        return fail(MCDP_E_INVALID, "too many contexts");
+    *ctx_id = (int32_t)srv->contexts.size();
+    srv->contexts.push_back(prop);
+    return MCDP_OK;
+}
+
+int mcdp_server_submit(mcdp_server *srv, int32_t ctx_id, const int32_t *seeds, int64_t n, int32_t n_q,
+                       const double *q, int64_t *request_id) {
+    if (!srv || !request_id) return fail(MCDP_E_INVALID, "null argument");
+    if (n < 1 || !seeds) return fail(MCDP_E_INVALID, "a request needs at least one seed");
+    if (n > std::numeric_limits<int32_t>::max() - 256) return fail(MCDP_E_INVALID, "at most 2^31 - 256 seeds per request");
+    if (n_q < 0 || n_q > 8 || (n_q > 0 && !q)) return fail(MCDP_E_INVALID, "between 0 and 8 quantiles per request");
+    for (int32_t j = 0; j < n_q; ++j)
+        if (!(q[j] >= 0.0 && q[j] <= 1.0)) return fail(MCDP_E_INVALID, "quantiles must be in the range [0, 1]");
+    auto r = std::make_unique<Request>();
+    mcdp_propagator *prop;
+    {
+        std::lock_guard<std::mutex> lock(srv->mu);
+        if (ctx_id < 0 || (size_t)ctx_id >= srv->contexts.size()) return fail(MCDP_E_INVALID, "unknown context");
+        prop = srv->contexts[(size_t)ctx_id];
+    }
+    r->ctx = ctx_id;
+    r->E = prop->E;
+    r->P = (int64_t)prop->preds.size();
+    r->seeds.assign(seeds, seeds + n);
+    r->q.assign(q, q + n_q);
+    r->bytes = footprint(n, r->E, r->P, n_q);
+    if (r->bytes > srv->pool->size())
+        return fail(MCDP_E_NOMEM, "request needs " + std::to_string(r->bytes) + " bytes of device memory; the server's "
+                                  "budget is " + std::to_string(srv->pool->size()));
+    const size_t host_doubles = (size_t)(r->E + r->E * n_q + r->E + r->P);
+    hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&r->h_out), std::max<size_t>(host_doubles, 1) * 8);
+    if (e != hipSuccess) return fail(MCDP_E_NOMEM, std::string("hipHostMalloc (results): ") + hipGetErrorString(e));
+    r->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lock(srv->mu);
+        if (srv->stop) {
+            (void)hipHostFree(r->h_out);
+            return fail(MCDP_E_INVALID, "server is shutting down");
+        }
+        r->id = srv->next_id++;
+        *request_id = r->id;
+        srv->queued.push_back(r.get());
+        ++srv->n_submitted;
+        srv->all.emplace(r->id, std::move(r));
+    }
+    srv->cv_work.notify_all();
+    return MCDP_OK;
+}
+
+int mcdp_server_wait(mcdp_server *srv, int64_t request_id, double *mean, double *quantiles, int64_t *cause_counts,
+                     double *latency_ms) {
+    if (!srv) return fail(MCDP_E_INVALID, "server is null");
+    std::unique_ptr<Request> r;
+    {
+        std::unique_lock<std::mutex> lock(srv->mu);
+        auto it = srv->all.find(request_id);
+        if (it == srv->all.end()) return fail(MCDP_E_INVALID, "unknown request (or already collected)");
+        Request *p = it->second.get();
+        srv->cv_done.wait(lock, [&] { return p->state == State::Done || p->state == State::Failed; });
+        r = std::move(it->second);
+        srv->all.erase(it);
+    }
+    int rc = r->rc;
+    if (r->state == State::Done) {
+        const int64_t E = r->E, nq = (int64_t)r->q.size();
+        if (mean) std::memcpy(mean, r->h_out, (size_t)E * 8);
+        if (quantiles && nq) std::memcpy(quantiles, r->h_out + E, (size_t)(E * nq) * 8);
+        if (cause_counts) std::memcpy(cause_counts, r->h_out + E + E * nq, (size_t)(E + r->P) * 8);
+        if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_end - r->t_submit).count();
+    } else {
+        rc = rc ? rc : MCDP_E_DEVICE;
+        (void)fail(rc, r->error.empty() ? "request failed" : r->error);
+    }
+    (void)hipHostFree(r->h_out);
+    return rc;
+}
+
+int mcdp_server_stats(mcdp_server *srv, int64_t *out, int n_out) {
+    if (!srv || !out) return fail(MCDP_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lock(srv->mu);
+    const int64_t v[8] = {srv->n_submitted,          srv->n_completed,         srv->n_failed,
+                          (int64_t)srv->queued.size(), (int64_t)srv->running.size(), srv->max_running_seen,
+                          (int64_t)srv->pool->used(), (int64_t)srv->pool->peak()};
+    for (int i = 0; i < n_out && i < 8; ++i) out[i] = v[i];
+    return MCDP_OK;
+}
+
+}  // extern "C"

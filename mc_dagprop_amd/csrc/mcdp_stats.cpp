@@ -110,6 +110,14 @@ int quantile_args(int64_t n, int32_t n_q, const double *q, QuantileArgs &qa) {
 
 }  // namespace
 
+int mcdp::prepare_stats(mcdp_propagator &P) {
+    std::lock_guard<std::mutex> lock(P.mu);
+    DeviceCopy *d = nullptr;
+    int rc = ensure_device(P, d);
+    if (rc) return rc;
+    return ensure_topo(P, *d);
+}
+
 extern "C" {
 
 int mcdp_propagator_pred_offsets(const mcdp_propagator *prop, int64_t *out) {

@@ -170,6 +170,8 @@ def test_no_cpu_fallback() -> None:
         sim.run_many([1, 2])
     with pytest.raises(RuntimeError, match="GPU|device"):
         sim.gradient([1, 2], [1.0] * 6)
+    with pytest.raises(capi.McdpError, match="GPU|device"):
+        capi.Server(1 << 20)
 
 
 def test_set_times_is_host_side() -> None:

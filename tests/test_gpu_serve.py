@@ -1,0 +1,136 @@
+"""The serving loop: requests over resident contexts, admitted by device-memory footprint.
+
+Each request's statistics must equal those of the same seeds run alone (``run_stats``), bit for bit,
+whatever else is queued or running; the pool's peak use stays within its budget; requests wait for
+memory instead of failing; requests that could never fit are refused at submission.
+"""
+
+from __future__ import annotations
+
+import threading
+
+import numpy as np
+import pytest
+
+import dags as D
+from mc_dagprop_amd import capi
+from mc_dagprop_amd.workloads import layered_network, layered_small
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def same_stats(got, want) -> None:
+    mean, quant, counts = got[:3]
+    w_mean, w_quant, w_counts = want
+    np.testing.assert_array_equal(bits(mean), bits(w_mean))
+    ok = (bits(quant) == bits(w_quant)) | ((quant == 0) & (w_quant == 0))
+    assert ok.all()
+    np.testing.assert_array_equal(counts, w_counts)
+
+
+def footprint(E: int, P: int, n: int, nq: int) -> int:
+    up = lambda x: (x + 255) // 256 * 256  # noqa: E731
+    return up(4 * n) + up(8 * E * n) + up(4 * E * n) + up(8 * E) + up(8 * E * nq) + up(8 * (E + P))
+
+
+def contexts():
+    dags = [layered_small(), layered_network(layers=10), D.continuous_dag(3, 60, max_delay=20.0, shared_links=True)]
+    return dags, [capi.Propagator(d) for d in dags]
+
+
+def test_results_in_the_mix_equal_alone(gpu) -> None:
+    dags, props = contexts()
+    budget = 48 << 20
+    srv = capi.Server(budget, max_running=4, n_streams=3)
+    ids = [srv.add_context(p) for p in props]
+    rng = np.random.default_rng(4)
+    reqs = []
+    for _ in range(30):
+        c = int(rng.integers(len(props)))
+        n = int(rng.integers(1, 3000))
+        seeds = rng.integers(0, 2**31 - 1, n).astype(np.int32)
+        q = sorted(rng.uniform(0, 1, int(rng.integers(0, 4))).tolist())
+        reqs.append((srv.submit(ids[c], seeds, q), c, seeds, q))
+    for rid, c, seeds, q in reqs:
+        got = srv.wait(rid)
+        assert got[3] > 0.0
+        same_stats(got, props[c].run_stats(seeds, q))
+    st = srv.stats()
+    assert st["submitted"] == 30 and st["completed"] == 30 and st["failed"] == 0
+    assert st["queued"] == 0 and st["running"] == 0 and st["bytes_in_use"] == 0
+    assert 0 < st["peak_bytes"] <= budget
+    assert st["max_running"] >= 2  # requests overlapped on the device
+    srv.close()
+
+
+def test_requests_wait_for_memory(gpu) -> None:
+    """A budget that holds one request at a time: all complete, one at a time, in order."""
+    dags, props = contexts()
+    p = props[1]
+    n = 2000
+    need = footprint(p.E, int(p.pred_offsets()[-1]), n, 3)
+    srv = capi.Server(need + need // 2, max_running=8, n_streams=4)
+    cid = srv.add_context(p)
+    seeds = [np.arange(i * n, (i + 1) * n, dtype=np.int32) for i in range(6)]
+    rids = [srv.submit(cid, s) for s in seeds]
+    for rid, s in zip(rids, seeds):
+        same_stats(srv.wait(rid), p.run_stats(s))
+    st = srv.stats()
+    assert st["completed"] == 6 and st["max_running"] == 1 and st["peak_bytes"] <= need + need // 2
+    srv.close()
+
+
+def test_oversized_request_is_refused(gpu) -> None:
+    _, props = contexts()
+    p = props[1]
+    srv = capi.Server(1 << 20, max_running=2, n_streams=1)
+    cid = srv.add_context(p)
+    with pytest.raises(capi.McdpError, match="budget"):
+        srv.submit(cid, np.arange(100_000, dtype=np.int32))
+    with pytest.raises(capi.McdpError):
+        srv.submit(cid + 5, np.arange(10, dtype=np.int32))
+    with pytest.raises(capi.McdpError):
+        srv.submit(cid, np.arange(10, dtype=np.int32), q=[1.5])
+    small = np.arange(10, dtype=np.int32)
+    same_stats(srv.wait(srv.submit(cid, small)), p.run_stats(small))
+    srv.close()
+
+
+def test_threads_submit_and_wait(gpu) -> None:
+    dags, props = contexts()
+    srv = capi.Server(64 << 20, max_running=6, n_streams=4)
+    ids = [srv.add_context(p) for p in props]
+    errors = []
+
+    def client(k: int) -> None:
+        try:
+            rng = np.random.default_rng(100 + k)
+            for _ in range(5):
+                c = int(rng.integers(len(props)))
+                seeds = rng.integers(0, 10**6, int(rng.integers(1, 1500))).astype(np.int32)
+                same_stats(srv.wait(srv.submit(ids[c], seeds)), props[c].run_stats(seeds))
+        except Exception as e:  # pragma: no cover - reported below
+            errors.append(e)
+
+    threads = [threading.Thread(target=client, args=(k,)) for k in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    assert srv.stats()["completed"] == 20
+    srv.close()
+
+
+def test_close_with_requests_in_flight(gpu) -> None:
+    """Closing drains running requests and drops queued ones without hanging."""
+    _, props = contexts()
+    srv = capi.Server(16 << 20, max_running=2, n_streams=2)
+    cid = srv.add_context(props[1])
+    for i in range(10):
+        srv.submit(cid, np.arange(i * 1000, (i + 1) * 1000, dtype=np.int32))
+    srv.close()

@@ -171,3 +171,18 @@ def test_stats_rejects_bad_quantiles(gpu) -> None:
         P.run_stats(np.arange(10), [1.5])
     with pytest.raises(capi.McdpError, match="8 quantiles"):
         P.run_stats(np.arange(10), np.linspace(0, 1, 9))
+
+
+def test_statistics_before_the_first_run_keep_the_plan_consistent(gpu) -> None:
+    """Statistics called before any run upload the plan; the later runs must still use the schedule they
+    launch with (bit-exact against a propagator that ran first)."""
+    dag = layered_small()
+    a, b = capi.Propagator(dag), capi.Propagator(dag)
+    E = a.E
+    n = 300
+    realized = capi.DeviceBuffer.from_array(np.zeros((E, n)))
+    cause = capi.DeviceBuffer.from_array(np.full((E, n), -1, dtype=np.int32))
+    a.event_stats_device(realized, cause, n, n, q=(0.5,))
+    seeds = np.arange(n, dtype=np.int32)
+    for x, y in zip(a.run_many(seeds), b.run_many(seeds)):
+        np.testing.assert_array_equal(np.ascontiguousarray(x).view(np.uint8), np.ascontiguousarray(y).view(np.uint8))
