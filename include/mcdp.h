@@ -233,7 +233,8 @@ int mcdp_reverse_device(mcdp_propagator *prop, const int32_t *seeds_dev, const d
 int mcdp_propagator_gradient_info(mcdp_propagator *prop, int64_t *out, int n_out);
 /* Host convenience: run_many then the reverse walk over chunks of at most
  * `chunk` samples (0: as many as fit a bounded workspace; rounded to whole
- * tiles of the reverse kernel, 64 or 128 samples), so per-sample results never
+ * groups of 16 tiles of the reverse kernel, 1,024 or 2,048 samples), so
+ * per-sample results never
  * need to exist for all seeds at once.  Writes MEANS over the n seeds:
  * objective[0], grad_dmin[link_count()], grad_earliest[node_count()].  The
  * result does not depend on the chunk size. */
@@ -305,9 +306,10 @@ int mcdp_debug_math(int fn, int64_t n, const double *x, double *out);
 /* Plain device copy of n doubles (one read and one write per element): the
  * calibration kernel for HBM byte counters. */
 int mcdp_debug_copy(double *dst_dev, const double *src_dev, int64_t n, void *stream);
-/* Caps the tiles of 64 samples one reverse-kernel launch of this propagator
- * covers (0: the default, bounded by a 1 GiB workspace), so tests can run a
- * mcdp_reverse_device call as several launches. */
+/* Caps the tiles one reverse-kernel launch of this propagator covers (0: the
+ * default, bounded by a 1 GiB workspace; rounded down to whole groups of 16
+ * tiles, at least one), so tests can run a mcdp_reverse_device call as several
+ * launches. */
 int mcdp_debug_reverse_tiles(mcdp_propagator *prop, int64_t max_tiles);
 
 #ifdef __cplusplus

@@ -139,58 +139,25 @@ int build_reverse(const mcdp_propagator &P, RevPlan &R) {
         }
         R.slots = used;
     }
-    // two samples per lane when the window of slots fits in LDS that way
-    R.spl = (size_t)R.slots * kSlotBytes * 2 <= kRevLdsBudget ? 2 : 1;
+    // two samples per lane when the window of slots fits in LDS that way (and
+    // winner positions fit the 16 bits a sample gets until the commit)
+    int64_t max_preds = 0;
+    for (int32_t e = 0; e < E; ++e) max_preds = std::max<int64_t>(max_preds, P.pred_off[(size_t)e + 1] - P.pred_off[(size_t)e]);
+    R.spl = ((size_t)R.slots * kSlotBytes * 2 <= kRevLdsBudget && max_preds <= 32767) ? 2 : 1;
     for (int32_t e : R.ev_list) R.ev_slot.push_back(slot_of[(size_t)e]);
 
-    // (c) partial-row columns: link l's first fresh entry writes column l,
-    // events write L + e; other entries of a link, and stale entries (two
-    // columns: link, then the source's earliest time), get extra columns.
-    std::vector<int32_t> col((size_t)P.preds.size(), -1);
-    std::vector<std::vector<int32_t>> link_cols((size_t)L), event_cols((size_t)E);
-    std::vector<uint8_t> claimed((size_t)L, 0);
-    int32_t next = L + E;
-    for (int32_t e = 0; e < E; ++e)
-        for (int64_t k = P.pred_off[(size_t)e]; k < P.pred_off[(size_t)e + 1]; ++k) {
-            const PredEntry &pe = P.preds[(size_t)k];
-            const bool valid = pe.edge >= 0 && pe.edge < L;
-            if (pe.stale) {
-                col[(size_t)k] = next;
-                if (valid) link_cols[(size_t)pe.edge].push_back(next);
-                event_cols[(size_t)pe.src].push_back(next + 1);
-                next += 2;
-            } else if (valid) {
-                if (!claimed[(size_t)pe.edge]) {
-                    claimed[(size_t)pe.edge] = 1;
-                    col[(size_t)k] = pe.edge;
-                    link_cols[(size_t)pe.edge].insert(link_cols[(size_t)pe.edge].begin(), pe.edge);
-                } else {
-                    col[(size_t)k] = next;
-                    link_cols[(size_t)pe.edge].push_back(next++);
-                }
-            }
-            if (next > std::numeric_limits<int32_t>::max() - kMaxWaves)
-                return fail(MCDP_E_INVALID, "too many gradient columns");
-        }
-    R.obj_col = next;
-    R.n_cols = next + W;
-    R.map_off.assign(1, 0);
-    for (int32_t l = 0; l < L; ++l) {
-        for (int32_t c : link_cols[(size_t)l]) R.map_col.push_back(c);
-        R.map_off.push_back((int32_t)R.map_col.size());
-    }
-    for (int32_t e = 0; e < E; ++e) {
-        R.map_col.push_back(L + e);
-        for (int32_t c : event_cols[(size_t)e]) R.map_col.push_back(c);
-        R.map_off.push_back((int32_t)R.map_col.size());
-    }
-    for (int32_t w = 0; w < W; ++w) R.map_col.push_back(R.obj_col + w);
-    R.map_off.push_back((int32_t)R.map_col.size());
-
-    // (d) records per (step, wave): header, then predecessors; successor pairs apart
+    // (c) records per (step, wave): header, then predecessors; successor pairs
+    // apart.  Partial-row columns are numbered in the order a wave writes them
+    // (per event: each predecessor entry with a minimal-duration column, plus
+    // the source's earliest-time column after a stale one, then the event's
+    // earliest-time column), so each wave's writes of a step are one run.
     std::vector<int32_t> unit_row((size_t)L, -1);
+    std::vector<std::vector<int32_t>> link_cols((size_t)L), event_cols((size_t)E);
+    int32_t next = 0;
     R.step_off.assign(1, 0);
+    R.col_off.clear();
     for (size_t g = 0; g + 1 < R.ev_off.size(); ++g) {
+        R.col_off.push_back(next);
         for (int32_t i = R.ev_off[g]; i < R.ev_off[g + 1]; ++i) {
             const int32_t e = R.ev_list[(size_t)i];
             const int64_t a = P.pred_off[(size_t)e], b = P.pred_off[(size_t)e + 1];
@@ -215,7 +182,12 @@ int build_reverse(const mcdp_propagator &P, RevPlan &R) {
                 Record r = fwd;
                 r.a = cls == kMulRead ? 1.0 / P.dmin[(size_t)pe.edge] : 1.0;
                 r.i0 = pe.src;
-                r.i1 = col[(size_t)k];
+                r.i1 = -1;
+                if (cls != kMulNone || pe.stale) {
+                    r.i1 = next++;
+                    if (cls != kMulNone) link_cols[(size_t)pe.edge].push_back(r.i1);
+                    if (pe.stale) event_cols[(size_t)pe.src].push_back(next++);
+                }
                 r.i2 = pe.edge;
                 r.info = (r.info & ~(uint32_t)(kRevClassMask | kRevDup | kRevStale | kInfoOwner | kInfoLast)) |
                          (cls << kRevClassShift) | (dups > 1 ? kRevDup : 0u) | (pe.stale ? kRevStale : 0u);
@@ -230,6 +202,9 @@ int build_reverse(const mcdp_propagator &P, RevPlan &R) {
                 }
                 R.recs.push_back(r);
             }
+            event_cols[(size_t)e].insert(event_cols[(size_t)e].begin(), next++);
+            if (next > std::numeric_limits<int32_t>::max() - kMaxWaves)
+                return fail(MCDP_E_INVALID, "too many gradient columns");
             R.ev_succ.push_back((int32_t)(R.succ.size() / 2));
             for (const auto &sp : succ[(size_t)e]) {
                 R.succ.push_back(slot_of[(size_t)sp.first]);
@@ -239,6 +214,20 @@ int build_reverse(const mcdp_propagator &P, RevPlan &R) {
         R.step_off.push_back((int32_t)R.recs.size());
     }
     if (R.recs.size() > (size_t)std::numeric_limits<int32_t>::max()) return fail(MCDP_E_INVALID, "plan too large");
+    // (d) the reduction's map: each output's columns, in writing order
+    R.obj_col = next;
+    R.n_cols = next + W;
+    R.map_off.assign(1, 0);
+    for (int32_t l = 0; l < L; ++l) {
+        for (int32_t c : link_cols[(size_t)l]) R.map_col.push_back(c);
+        R.map_off.push_back((int32_t)R.map_col.size());
+    }
+    for (int32_t e = 0; e < E; ++e) {
+        for (int32_t c : event_cols[(size_t)e]) R.map_col.push_back(c);
+        R.map_off.push_back((int32_t)R.map_col.size());
+    }
+    for (int32_t w = 0; w < W; ++w) R.map_col.push_back(R.obj_col + w);
+    R.map_off.push_back((int32_t)R.map_col.size());
     return MCDP_OK;
 }
 
@@ -264,6 +253,7 @@ int ensure_reverse(mcdp_propagator &P, DeviceCopy &d) {
     const size_t o_evr = reserve(R.ev_rec.size() * 4);
     const size_t o_unit = reserve(R.unit_links.size() * sizeof(Record));
     const size_t o_evsucc = reserve(R.ev_succ.size() * 4), o_succ = reserve(R.succ.size() * 4 + 32);
+    const size_t o_coloff = reserve(R.col_off.size() * 4);
     const size_t o_mcol = reserve(R.map_col.size() * 4);
     const size_t total = std::max<size_t>(off, 256);
     std::vector<unsigned char> host(total, 0);
@@ -279,6 +269,7 @@ int ensure_reverse(mcdp_propagator &P, DeviceCopy &d) {
     put(o_unit, R.unit_links.data(), R.unit_links.size() * sizeof(Record));
     put(o_evsucc, R.ev_succ.data(), R.ev_succ.size() * 4);
     put(o_succ, R.succ.data(), R.succ.size() * 4);
+    put(o_coloff, R.col_off.data(), R.col_off.size() * 4);
     put(o_moff, R.map_off.data(), R.map_off.size() * 4);
     put(o_mcol, R.map_col.data(), R.map_col.size() * 4);
     void *blob = nullptr;
@@ -301,6 +292,7 @@ int ensure_reverse(mcdp_propagator &P, DeviceCopy &d) {
     p.ev_rec = reinterpret_cast<const int32_t *>(b + o_evr);
     p.ev_succ = reinterpret_cast<const int32_t *>(b + o_evsucc);
     p.succ = reinterpret_cast<const int2 *>(b + o_succ);
+    p.col_off = reinterpret_cast<const int32_t *>(b + o_coloff);
     p.dists = d.base.dists;
     p.alias = d.base.alias;
     p.earliest = d.base.earliest;
@@ -397,8 +389,11 @@ int mcdp_reverse_device(mcdp_propagator *prop, const int32_t *seeds_dev, const d
     const int64_t tile = (int64_t)kLanes * spl;
     const int64_t tiles = (n + tile - 1) / tile;
     const size_t row_bytes = (size_t)base.n_cols * sizeof(double);
-    int64_t max_tiles = std::max<int64_t>(1, (int64_t)(kPartialBudget / row_bytes));
+    // launches cover whole groups of the column reduction, so sums do not depend on the split
+    int64_t max_tiles = std::max<int64_t>(kReduceGroup, (int64_t)(kPartialBudget / row_bytes));
     if (P.rev_max_tiles > 0) max_tiles = std::min(max_tiles, P.rev_max_tiles);
+    max_tiles = std::min<int64_t>(max_tiles, (int64_t)kReduceGroup * 65535);
+    max_tiles = std::max<int64_t>(kReduceGroup, max_tiles / kReduceGroup * kReduceGroup);
     const size_t lds = (size_t)base.lds_slots * kSlotBytes * spl;
     int cus = 0, bpc = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device), "hipDeviceGetAttribute");
@@ -407,14 +402,19 @@ int mcdp_reverse_device(mcdp_propagator *prop, const int32_t *seeds_dev, const d
     if (bpc < 1) return fail(MCDP_E_DEVICE, "reverse kernel cannot be resident with " + std::to_string(lds) + " bytes of LDS");
     const int64_t resident = (int64_t)cus * bpc;
     int64_t chunk_tiles = std::min<int64_t>(std::max<int64_t>(tiles, 1), max_tiles);
-    if (n_unit > 0)  // unit-duration rows for the launch's samples stay within the same budget
-        chunk_tiles = std::min<int64_t>(chunk_tiles, std::max<int64_t>(1, (int64_t)(kPartialBudget / ((size_t)n_unit * tile * 8))));
+    if (n_unit > 0) {  // unit-duration rows for the launch's samples stay within the same budget
+        const int64_t cap = (int64_t)(kPartialBudget / ((size_t)n_unit * tile * 8));
+        if (chunk_tiles > cap)
+            chunk_tiles = std::max<int64_t>(kReduceGroup, cap / kReduceGroup * kReduceGroup);
+    }
     const int64_t grid_max = std::min<int64_t>(chunk_tiles, resident);
     const size_t o_gslots = align_to(row_bytes * (size_t)chunk_tiles, 256);
     const size_t gslot_bytes = (size_t)grid_max * base.global_slots * tile * 2 * sizeof(double);
     const size_t o_unit = o_gslots + align_to(std::max<size_t>(gslot_bytes, 256), 256);
     const int32_t ld_unit = (int32_t)(chunk_tiles * tile);
-    rc = ensure_workspace(*d, o_unit + std::max<size_t>((size_t)n_unit * (size_t)ld_unit * 8, 256));
+    const size_t o_groups = o_unit + align_to(std::max<size_t>((size_t)n_unit * (size_t)ld_unit * 8, 256), 256);
+    const size_t group_bytes = row_bytes * (size_t)((chunk_tiles + kReduceGroup - 1) / kReduceGroup);
+    rc = ensure_workspace(*d, o_groups + std::max<size_t>(group_bytes, 256));
     if (rc) return rc;
     auto *work = static_cast<unsigned char *>(d->rev_work);
     RevParams p = base;
@@ -427,11 +427,12 @@ int mcdp_reverse_device(mcdp_propagator *prop, const int32_t *seeds_dev, const d
     p.partial = reinterpret_cast<double *>(work);
     p.gslots = reinterpret_cast<double *>(work + o_gslots);
     double *unit = reinterpret_cast<double *>(work + o_unit);
+    double *groups = reinterpret_cast<double *>(work + o_groups);
     p.unit = unit;
     p.ld_unit = ld_unit;
     if (tiles == 0 || P.E == 0) {  // no samples: sums are 0 (or unchanged)
-        HIP_TRY(launch_reduce_columns(p.partial, 0, p.n_cols, d->rev_map_off, d->rev_map_col, P.L, P.E, accumulate,
-                                      grad_dmin_dev, grad_earliest_dev, objective_dev, s),
+        HIP_TRY(launch_reduce_columns(p.partial, 0, p.n_cols, groups, d->rev_map_off, d->rev_map_col, P.L, P.E,
+                                      accumulate, grad_dmin_dev, grad_earliest_dev, objective_dev, s),
                 "reduce kernel launch");
         return MCDP_OK;
     }
@@ -443,7 +444,7 @@ int mcdp_reverse_device(mcdp_propagator *prop, const int32_t *seeds_dev, const d
                                   (int32_t)(p.n_tiles * tile), (int32_t)n, base.dists, base.alias, unit, ld_unit, s),
                 "unit draw kernel launch");
         HIP_TRY(launch_reverse(p, spl, grid, lds, s), "reverse kernel launch");
-        HIP_TRY(launch_reduce_columns(p.partial, p.n_tiles, p.n_cols, d->rev_map_off, d->rev_map_col, P.L, P.E,
+        HIP_TRY(launch_reduce_columns(p.partial, p.n_tiles, p.n_cols, groups, d->rev_map_off, d->rev_map_col, P.L, P.E,
                                       accumulate || t0 > 0, grad_dmin_dev, grad_earliest_dev, objective_dev, s),
                 "reduce kernel launch");
     }
@@ -496,8 +497,10 @@ int mcdp_gradient_host(mcdp_propagator *prop, const int32_t *seeds, int64_t n, c
     }
     if (chunk == 0) chunk = std::max<int64_t>(tile, (int64_t)(kGradWorkspace / (size_t)std::max<int64_t>(per_sample, 1)));
     chunk = std::min<int64_t>(chunk, std::numeric_limits<int32_t>::max() - 256);
-    chunk = std::max<int64_t>(tile, chunk / tile * tile);  // whole tiles: sums do not depend on the chunking
-    chunk = std::min<int64_t>(chunk, (n + tile - 1) / tile * tile);
+    // whole groups of the column reduction: sums do not depend on the chunking
+    const int64_t unit = tile * kReduceGroup;
+    chunk = std::max<int64_t>(unit, chunk / unit * unit);
+    chunk = std::min<int64_t>(chunk, (n + unit - 1) / unit * unit);
     const size_t o_seeds = 0, o_real = align_to((size_t)chunk * 4, 256);
     const size_t o_dur = o_real + align_to((size_t)(E * chunk) * 8, 256);
     const size_t o_cause = o_dur + align_to((size_t)(L * chunk) * 8, 256);

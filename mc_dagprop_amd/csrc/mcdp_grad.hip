@@ -135,6 +135,36 @@ struct ColumnStage {
     }
 };
 
+// Eight 32-bit values with a wave-uniform dynamic index, kept in registers:
+// the index selects a branch, not an address, so no private array is needed.
+struct Regs8 {
+    int32_t v0 = -1, v1 = -1, v2 = -1, v3 = -1, v4 = -1, v5 = -1, v6 = -1, v7 = -1;
+    __device__ __forceinline__ void set(int j, int32_t x) {
+        switch (j) {
+            case 0: v0 = x; break;
+            case 1: v1 = x; break;
+            case 2: v2 = x; break;
+            case 3: v3 = x; break;
+            case 4: v4 = x; break;
+            case 5: v5 = x; break;
+            case 6: v6 = x; break;
+            default: v7 = x; break;
+        }
+    }
+    __device__ __forceinline__ int32_t get(int j) const {
+        switch (j) {
+            case 0: return v0;
+            case 1: return v1;
+            case 2: return v2;
+            case 3: return v3;
+            case 4: return v4;
+            case 5: return v5;
+            case 6: return v6;
+            default: return v7;
+        }
+    }
+};
+
 constexpr int kPre = 3;        // multiplier rows of an event's first predecessors, loaded one event ahead
 constexpr int kFastPreds = 4;  // predecessor records read per group
 
@@ -170,6 +200,7 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) reverse_kernel(RevParams p
     const auto ev_rec = as_const_space(p.ev_rec);
     const auto ev_succ = as_const_space(p.ev_succ);
     const auto ev_slot = as_const_space(p.ev_slot);
+    const auto col_off = as_const_space(p.col_off);
     const auto weights = as_const_space(p.weights);
     const auto earliest = as_const_space(p.earliest);
 
@@ -248,8 +279,12 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) reverse_kernel(RevParams p
             const int e0 = ev_off[t * W + wave];
             const int ne = ev_off[t * W + wave + 1] - e0;
             if (ne > 0 && loaded != e0) load_event(e0, cur);
+            int32_t ev_col = col_off[t * W + wave];  // the next column this wave writes
+            // the step's results until the commit; the winner positions of a lane's
+            // samples share one word (positions < 2^15 with two samples per lane)
+            static_assert(K == 8, "Regs8 holds one step's winner positions");
             double gres[K][kSpl];
-            int32_t wres[K][kSpl];
+            Regs8 wres;
 #pragma unroll 1
             for (int j = 0; j < ne; ++j) {
                 // the next event's rows
@@ -363,6 +398,7 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) reverse_kernel(RevParams p
                     }
                     st.put(q.i1, dm, prow, lane);
                     if (stale) st.put(q.i1 + 1, ds, prow, lane);
+                    ev_col = q.i1 + (stale ? 2 : 1);
                 };
 
                 // predecessors in groups of four: the winner first, then the columns
@@ -408,12 +444,15 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) reverse_kernel(RevParams p
                         }
                     }
                 }
-                st.put(p.L + ev, any_of<kSpl>(ge_nz) ? tile_sum<kSpl>(ge) : 0.0, prow, lane);
+                // the event's earliest-time column follows its entries' columns
+                st.put(ev_col, any_of<kSpl>(ge_nz) ? tile_sum<kSpl>(ge) : 0.0, prow, lane);
+                ++ev_col;
 #pragma unroll
                 for (int i = 0; i < kSpl; ++i) {
                     gres[j][i] = gt[i];
-                    wres[j][i] = wslot[i];
                 }
+                wres.set(j, kSpl == 1 ? wslot[0]
+                                      : (int32_t)(((uint32_t)wslot[0] & 0xFFFFu) | ((uint32_t)wslot[kSpl - 1] << 16)));
                 cur = nxt;
                 loaded = next;
             }
@@ -424,7 +463,13 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) reverse_kernel(RevParams p
 #pragma unroll 1
             for (int j = 0; j < ne; ++j) {
                 const int32_t slot = ev_slot[e0 + j];
-                if (slot >= 0) sl.write(slot, gres[j], wres[j]);
+                if (slot >= 0) {
+                    int32_t w[kSpl];
+                    const int32_t x = wres.get(j);
+                    w[0] = kSpl == 1 ? x : (int32_t)(int16_t)(x & 0xFFFF);
+                    if constexpr (kSpl == 2) w[1] = x >> 16;
+                    sl.write(slot, gres[j], w);
+                }
             }
             __syncthreads();
         }
@@ -449,6 +494,27 @@ __global__ void __launch_bounds__(256) unit_draw_kernel(const Record *__restrict
     double unit[1];
     draw_durations<1>(rec.info, 1.0, rec.b, (uint32_t)rec.i2, seed, as_const_space(alias), as_const_space(dists), unit);
     out[row_offset(q, ld_out) + i] = unit[0];
+}
+
+// groups[g][c] = sum over tiles t of group g (kReduceGroup consecutive tiles),
+// in tile order, of partial[t][c]: the first of the two reduction stages,
+// with a group's loads in flight together.
+__global__ void __launch_bounds__(256) group_sum_kernel(const double *__restrict__ partial, int32_t n_tiles,
+                                                        int32_t n_cols, double *__restrict__ groups) {
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t g = blockIdx.y;
+    if (c >= n_cols) return;
+    const int32_t t0 = g * kReduceGroup;
+    const int32_t m = min(kReduceGroup, n_tiles - t0);
+    const double *col = partial + (size_t)t0 * n_cols + c;
+    double v[kReduceGroup];
+#pragma unroll
+    for (int i = 0; i < kReduceGroup; ++i) v[i] = i < m ? col[(size_t)i * n_cols] : 0.0;
+    double acc = v[0];
+#pragma unroll
+    for (int i = 1; i < kReduceGroup; ++i)
+        if (i < m) acc += v[i];
+    groups[(size_t)g * n_cols + c] = acc;
 }
 
 __global__ void __launch_bounds__(256) reduce_columns_kernel(const double *__restrict__ partial, int32_t n_tiles,
@@ -531,12 +597,19 @@ hipError_t reverse_occupancy(int waves, size_t lds_bytes, bool global_slots, int
                                                         waves * kLanes, lds_bytes);
 }
 
-hipError_t launch_reduce_columns(const double *partial, int32_t n_tiles, int32_t n_cols, const int32_t *map_off,
-                                 const int32_t *map_col, int32_t L, int32_t E, int accumulate, double *grad_dmin,
-                                 double *grad_earliest, double *objective, hipStream_t stream) {
+hipError_t launch_reduce_columns(const double *partial, int32_t n_tiles, int32_t n_cols, double *groups,
+                                 const int32_t *map_off, const int32_t *map_col, int32_t L, int32_t E, int accumulate,
+                                 double *grad_dmin, double *grad_earliest, double *objective, hipStream_t stream) {
+    const int32_t n_groups = (n_tiles + kReduceGroup - 1) / kReduceGroup;
+    if (n_groups > 0) {
+        hipLaunchKernelGGL(group_sum_kernel, dim3((unsigned)((n_cols + 255) / 256), (unsigned)n_groups), dim3(256), 0,
+                           stream, partial, n_tiles, n_cols, groups);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     const int32_t n_out = L + E + 1;
-    hipLaunchKernelGGL(reduce_columns_kernel, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0, stream, partial,
-                       n_tiles, n_cols, map_off, map_col, L, E, accumulate, grad_dmin, grad_earliest, objective);
+    hipLaunchKernelGGL(reduce_columns_kernel, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0, stream, groups,
+                       n_groups, n_cols, map_off, map_col, L, E, accumulate, grad_dmin, grad_earliest, objective);
     return hipGetLastError();
 }
 

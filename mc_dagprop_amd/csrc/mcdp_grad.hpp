@@ -32,7 +32,8 @@ namespace mcdp {
 //   predecessor: in the event's input order; i0 = source, i1 = column of the
 //                minimal-duration derivative (-1: none; for a stale entry the
 //                first of two columns, the second for the source's earliest
-//                time), i2 = edge (durations row), a = 1 / d (kMulRead) or 1
+//                time; columns are numbered in the order a wave writes them,
+//                from col_off of its step), i2 = edge (durations row), a = 1 / d (kMulRead) or 1
 //                (kMulDraw), info = the forward record's kind, flags and
 //                distribution index (kMulDraw: unit row) with the bits below.
 enum : uint32_t {
@@ -60,6 +61,7 @@ struct RevParams {
     const int32_t *ev_rec;    // their header records
     const int32_t *ev_succ;   // their first successor entry
     const int2 *succ;         // successor entries: (target's slot, position in the target's list)
+    const int32_t *col_off;   // [n_steps * waves]: first partial-row column (step, wave) writes
     const int32_t *ev_slot;   // their slots (-1: none)
     const DistParams *dists;  // the forward plan's distributions and alias tables (kMulDraw)
     const AliasRec *alias;
@@ -68,7 +70,7 @@ struct RevParams {
     int32_t waves;
     int32_t lds_slots;        // slots in LDS: 64 spl adjoints (double) + 64 spl winner positions (int32) each
     int32_t global_slots;     // further slots in global memory, per workgroup
-    int32_t L;                // column of event e's earliest-time derivative: L + e
+    int32_t L;                // links
     int32_t n_cols;           // columns of a partial row
     int32_t obj_col;          // first of `waves` objective columns
     int32_t tile0;            // first tile (of 64 spl samples) of this launch
@@ -94,11 +96,16 @@ hipError_t launch_unit_draws(const Record *links, int32_t n_links, const int32_t
                              int32_t n, const DistParams *dists, const AliasRec *alias, double *out, int32_t ld_out,
                              hipStream_t stream);
 hipError_t reverse_occupancy(int waves, size_t lds_bytes, bool global_slots, int spl, int *blocks_per_cu);
-// out_X[o] (+)= sum over tiles t, in order, of sum over map_col[map_off[o] ..
-// map_off[o + 1]) of partial[t][col]; outputs o < L go to grad_dmin, the next
-// E to grad_earliest, the last one to objective.
-hipError_t launch_reduce_columns(const double *partial, int32_t n_tiles, int32_t n_cols, const int32_t *map_off,
-                                 const int32_t *map_col, int32_t L, int32_t E, int accumulate, double *grad_dmin,
-                                 double *grad_earliest, double *objective, hipStream_t stream);
+// Tiles per group of the two-stage column reduction (grid dimension y of the
+// first stage: at most 65535 groups per launch).
+constexpr int kReduceGroup = 16;
+// groups[g][c] = sum of partial[t][c] over the kReduceGroup tiles of group g,
+// in order; then out_X[o] (+)= sum over groups g, in order, of sum over
+// map_col[map_off[o] .. map_off[o + 1]) of groups[g][col].  Outputs o < L go to
+// grad_dmin, the next E to grad_earliest, the last one to objective.  `groups`
+// holds ceil(n_tiles / kReduceGroup) rows of n_cols.
+hipError_t launch_reduce_columns(const double *partial, int32_t n_tiles, int32_t n_cols, double *groups,
+                                 const int32_t *map_off, const int32_t *map_col, int32_t L, int32_t E, int accumulate,
+                                 double *grad_dmin, double *grad_earliest, double *objective, hipStream_t stream);
 
 }  // namespace mcdp

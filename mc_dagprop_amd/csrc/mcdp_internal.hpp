@@ -60,7 +60,8 @@ struct RevPlan {
     int32_t waves = 1, n_steps = 0, slots = 0, n_cols = 0, obj_col = 0;
     int32_t spl = 1;  // samples per lane of the reverse kernel
     std::vector<int32_t> step_off, ev_off, ev_list, ev_rec, ev_succ, ev_slot;
-    std::vector<int32_t> succ;  // (target slot, position) pairs
+    std::vector<int32_t> succ;     // (target slot, position) pairs
+    std::vector<int32_t> col_off;  // first partial-row column of each (step, wave)
     std::vector<Record> recs;
     std::vector<int32_t> map_off, map_col;  // outputs: L links, E events, the objective
     std::vector<Record> unit_links;         // forward records of the links drawn again (kMulDraw)
