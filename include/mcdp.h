@@ -251,9 +251,16 @@ int mcdp_gradient_host(mcdp_propagator *prop, const int32_t *seeds, int64_t n, c
  * arrival order as soon as the pool and one of `max_running` slots allow,
  * while others run, on `n_streams` streams (work of different requests
  * overlaps on the device).  A request's results equal mcdp_run_stats_host on
- * the same seeds.  All entry points are thread-safe. */
+ * the same seeds (for chunked requests, see request_cap).  All entry points are
+ * thread-safe. */
 typedef struct mcdp_server mcdp_server;
-int mcdp_server_create(int64_t memory_budget, int32_t max_running, int32_t n_streams, mcdp_server **out);
+/* request_cap (0: memory_budget / 2) bounds one request's share of the pool:
+ * a request needing more runs in chunks of samples that fit it, with the
+ * exact mean and cause counts, and each quantile within max_delay / 4096 of
+ * the exact one (order statistics placed inside a 4,096-bin histogram over
+ * [earliest, earliest + max_delay]; needs a finite max_delay). */
+int mcdp_server_create(int64_t memory_budget, int64_t request_cap, int32_t max_running, int32_t n_streams,
+                       mcdp_server **out);
 /* Waits for running requests; fails queued ones; frees the pool. */
 void mcdp_server_destroy(mcdp_server *srv);
 /* Makes a context resident (plan and predecessor table uploaded now) and
@@ -261,7 +268,7 @@ void mcdp_server_destroy(mcdp_server *srv);
 int mcdp_server_add_context(mcdp_server *srv, mcdp_propagator *prop, int32_t *ctx_id);
 /* Queues run_many over seeds[0..n) of context ctx_id and the statistics of
  * mcdp_run_stats_host (n_q <= 8 quantiles); returns at once.  Fails with
- * MCDP_E_NOMEM if the request could never fit the budget. */
+ * MCDP_E_NOMEM if even chunks of 128 samples cannot fit the request cap. */
 int mcdp_server_submit(mcdp_server *srv, int32_t ctx_id, const int32_t *seeds, int64_t n, int32_t n_q,
                        const double *q, int64_t *request_id);
 /* Blocks until the request is done and writes its results (layouts of
@@ -269,8 +276,8 @@ int mcdp_server_submit(mcdp_server *srv, int32_t ctx_id, const int32_t *seeds, i
  * completion; the request is then forgotten. */
 int mcdp_server_wait(mcdp_server *srv, int64_t request_id, double *mean, double *quantiles, int64_t *cause_counts,
                      double *latency_ms);
-/* out[0..7] = {submitted, completed, failed, queued, running, most running at
- * once, pool bytes in use, peak pool bytes in use}. */
+/* out[0..8] = {submitted, completed, failed, queued, running, most running at
+ * once, pool bytes in use, peak pool bytes in use, requests run in chunks}. */
 int mcdp_server_stats(mcdp_server *srv, int64_t *out, int n_out);
 
 /* ---- Device plumbing for callers without their own HIP allocator -------- */

@@ -80,7 +80,7 @@ _SIGNATURES = {
     "mcdp_gradient_host": (C.c_int, [_P, _pi32, _i64, _pf64, _i64, _pf64, _pf64, _pf64]),
     "mcdp_debug_reverse_tiles": (C.c_int, [_P, _i64]),
     "mcdp_propagator_gradient_info": (C.c_int, [_P, _pi64, C.c_int]),
-    "mcdp_server_create": (C.c_int, [_i64, _i32, _i32, C.POINTER(_P)]),
+    "mcdp_server_create": (C.c_int, [_i64, _i64, _i32, _i32, C.POINTER(_P)]),
     "mcdp_server_destroy": (None, [_P]),
     "mcdp_server_add_context": (C.c_int, [_P, _P, C.POINTER(_i32)]),
     "mcdp_server_submit": (C.c_int, [_P, _i32, _pi32, _i64, _i32, _pf64, C.POINTER(_i64)]),
@@ -468,11 +468,12 @@ class Server:
     """Statistics requests over resident contexts, admitted by device-memory footprint (mcdp_server_*)."""
 
     STAT_FIELDS = ["submitted", "completed", "failed", "queued", "running", "max_running", "bytes_in_use",
-                   "peak_bytes"]
+                   "peak_bytes", "chunked"]
 
-    def __init__(self, memory_budget: int, max_running: int = 8, n_streams: int = 4) -> None:
+    def __init__(self, memory_budget: int, max_running: int = 8, n_streams: int = 4, request_cap: int = 0) -> None:
+        """request_cap: a request needing more device memory runs in chunks of samples (0: half the pool)."""
         self.h = C.c_void_p(0)
-        check(lib().mcdp_server_create(int(memory_budget), max_running, n_streams, C.byref(self.h)))
+        check(lib().mcdp_server_create(int(memory_budget), int(request_cap), max_running, n_streams, C.byref(self.h)))
         self._ctx: list[Propagator] = []
         self._req: dict[int, tuple[int, int, int]] = {}
 

@@ -14,6 +14,7 @@
 #include "mcdp_grad.hpp"
 #include "mcdp_kernels.hpp"
 #include "mcdp_plan.hpp"
+#include "mcdp_stats.hpp"
 
 namespace mcdp {
 
@@ -131,6 +132,14 @@ int configure_shape(mcdp_propagator *prop, int32_t waves, int32_t k, int32_t spl
 // Uploads P's plan and the statistics' predecessor table to the current
 // device (what a resident context needs before serving requests).
 int prepare_stats(mcdp_propagator &P);
+// Statistics of a run in sample chunks (serving): adds one chunk's rows
+// (realized[e * ld + s], cause likewise, s < m) to the running statistics and
+// cause counts (first: start them), then the means and quantiles of all n
+// samples; asynchronous on s.
+int chunk_stats_accumulate(mcdp_propagator &P, const double *realized, const int32_t *cause, int32_t m, int32_t ld,
+                           bool first, ChunkStat *st, uint32_t *hist, int64_t *counts, hipStream_t s);
+int chunk_stats_finish(mcdp_propagator &P, int64_t n, int32_t n_q, const double *q, const ChunkStat *st,
+                       const uint32_t *hist, double *mean, double *quantiles, hipStream_t s);
 // The forward plan's record of link l (kind, flags, distribution index,
 // minimal duration, p0); an edge outside the link table is kind kNone.
 Record link_record(const mcdp_propagator &P, int32_t l);

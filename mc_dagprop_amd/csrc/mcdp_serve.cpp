@@ -12,7 +12,9 @@
 //     device, and copies its results into pinned host memory;
 //   * completion is seen through an event, which returns the memory at once.
 // A request's results are those of mcdp_run_stats_host on the same seeds, bit
-// for bit: the kernels do not depend on what else runs.
+// for bit: the kernels do not depend on what else runs.  A request larger than
+// the per-request cap runs in chunks of samples: exact mean and cause counts,
+// quantiles within one histogram bin (max_delay / 4096) of the exact ones.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -94,6 +96,7 @@ struct Request {
     std::vector<int32_t> seeds;
     std::vector<double> q;
     int64_t E = 0, P = 0;
+    int64_t chunk = 0;  // > 0: run in chunks of this many samples (statistics over chunks)
     size_t bytes = 0, off = 0;
     int stream = 0;
     hipEvent_t done = nullptr;
@@ -104,10 +107,14 @@ struct Request {
     std::string error;
 };
 
-// Device bytes of a request: seeds, realized and cause rows, statistics.
-size_t footprint(int64_t n, int64_t E, int64_t P, int64_t n_q) {
-    return align_up((size_t)n * 4, 256) + align_up((size_t)(E * n) * 8, 256) + align_up((size_t)(E * n) * 4, 256) +
-           align_up((size_t)E * 8, 256) + align_up((size_t)(E * n_q) * 8, 256) + align_up((size_t)(E + P) * 8, 256);
+// Device bytes of a request: seeds, realized and cause rows for `rows`
+// samples at a time, statistics (and, run in chunks, their running state).
+size_t footprint(int64_t n, int64_t rows, int64_t E, int64_t P, int64_t n_q, bool chunked) {
+    size_t b = align_up((size_t)n * 4, 256) + align_up((size_t)(E * rows) * 8, 256) +
+               align_up((size_t)(E * rows) * 4, 256) + align_up((size_t)E * 8, 256) +
+               align_up((size_t)(E * n_q) * 8, 256) + align_up((size_t)(E + P) * 8, 256);
+    if (chunked) b += align_up((size_t)E * sizeof(ChunkStat), 256) + align_up((size_t)E * kMaxChunkBins * 4, 256);
+    return b;
 }
 
 }  // namespace
@@ -117,6 +124,7 @@ struct mcdp_server {
     void *pool_base = nullptr;
     std::unique_ptr<Pool> pool;
     int32_t max_running = 4;
+    size_t request_cap = 0;  // a request needing more runs in chunks
     std::vector<hipStream_t> streams;
     std::vector<int> stream_load;
     std::vector<mcdp_propagator *> contexts;
@@ -127,7 +135,7 @@ struct mcdp_server {
     std::vector<Request *> running;
     std::unordered_map<int64_t, std::unique_ptr<Request>> all;
     int64_t next_id = 1;
-    int64_t n_submitted = 0, n_completed = 0, n_failed = 0, max_running_seen = 0;
+    int64_t n_submitted = 0, n_completed = 0, n_failed = 0, max_running_seen = 0, n_chunked = 0;
     bool stop = false;
     std::thread worker;
 
@@ -141,6 +149,7 @@ namespace {
 int launch_request(mcdp_server &srv, Request &r) {
     mcdp_propagator *prop = srv.contexts[(size_t)r.ctx];
     const int64_t n = (int64_t)r.seeds.size(), E = r.E, nq = (int64_t)r.q.size();
+    const int64_t rows = r.chunk > 0 ? r.chunk : n;
     auto *base = static_cast<unsigned char *>(srv.pool_base) + r.off;
     size_t o = 0;
     auto take = [&](size_t bytes) {
@@ -149,17 +158,33 @@ int launch_request(mcdp_server &srv, Request &r) {
         return p;
     };
     auto *d_seeds = reinterpret_cast<int32_t *>(take((size_t)n * 4));
-    auto *d_real = reinterpret_cast<double *>(take((size_t)(E * n) * 8));
-    auto *d_cause = reinterpret_cast<int32_t *>(take((size_t)(E * n) * 4));
+    auto *d_real = reinterpret_cast<double *>(take((size_t)(E * rows) * 8));
+    auto *d_cause = reinterpret_cast<int32_t *>(take((size_t)(E * rows) * 4));
     auto *d_mean = reinterpret_cast<double *>(take((size_t)E * 8));
     auto *d_quant = reinterpret_cast<double *>(take((size_t)(E * nq) * 8));
     auto *d_cnt = reinterpret_cast<int64_t *>(take((size_t)(E + r.P) * 8));
     const hipStream_t s = srv.streams[(size_t)r.stream];
     HIP_TRY(hipMemcpyAsync(d_seeds, r.seeds.data(), (size_t)n * 4, hipMemcpyHostToDevice, s), "copy seeds");
-    int rc = mcdp_run_many_device(prop, d_seeds, n, n, d_real, nullptr, d_cause, s);
-    if (rc) return rc;
-    rc = mcdp_event_stats_device(prop, d_real, d_cause, n, n, (int32_t)nq, r.q.data(), d_mean, nq ? d_quant : nullptr,
-                                 d_cnt, s);
+    int rc;
+    if (r.chunk > 0) {
+        // run_many chunk by chunk into the same rows; statistics merged across chunks
+        auto *d_st = reinterpret_cast<ChunkStat *>(take((size_t)E * sizeof(ChunkStat)));
+        auto *d_hist = reinterpret_cast<uint32_t *>(take((size_t)E * kMaxChunkBins * 4));
+        for (int64_t c0 = 0; c0 < n; c0 += r.chunk) {
+            const int64_t m = std::min<int64_t>(r.chunk, n - c0);
+            rc = mcdp_run_many_device(prop, d_seeds + c0, m, r.chunk, d_real, nullptr, d_cause, s);
+            if (rc) return rc;
+            rc = chunk_stats_accumulate(*prop, d_real, d_cause, (int32_t)m, (int32_t)r.chunk, c0 == 0, d_st, d_hist,
+                                        d_cnt, s);
+            if (rc) return rc;
+        }
+        rc = chunk_stats_finish(*prop, n, (int32_t)nq, r.q.data(), d_st, d_hist, d_mean, d_quant, s);
+    } else {
+        rc = mcdp_run_many_device(prop, d_seeds, n, n, d_real, nullptr, d_cause, s);
+        if (rc) return rc;
+        rc = mcdp_event_stats_device(prop, d_real, d_cause, n, n, (int32_t)nq, r.q.data(), d_mean,
+                                     nq ? d_quant : nullptr, d_cnt, s);
+    }
     if (rc) return rc;
     double *h = r.h_out;
     HIP_TRY(hipMemcpyAsync(h, d_mean, (size_t)E * 8, hipMemcpyDeviceToHost, s), "copy results");
@@ -250,10 +275,13 @@ void mcdp_server::run() {
 
 extern "C" {
 
-int mcdp_server_create(int64_t memory_budget, int32_t max_running, int32_t n_streams, mcdp_server **out) {
+int mcdp_server_create(int64_t memory_budget, int64_t request_cap, int32_t max_running, int32_t n_streams,
+                       mcdp_server **out) {
     if (!out) return fail(MCDP_E_INVALID, "out is null");
     *out = nullptr;
     if (memory_budget <= 0) return fail(MCDP_E_INVALID, "memory_budget must be positive");
+    if (request_cap < 0 || request_cap > memory_budget)
+        return fail(MCDP_E_INVALID, "request_cap must be between 0 and memory_budget");
     if (max_running < 1 || n_streams < 1 || n_streams > 16)
         return fail(MCDP_E_INVALID, "need max_running >= 1 and 1 <= n_streams <= 16");
     int count = 0;
@@ -266,6 +294,7 @@ int mcdp_server_create(int64_t memory_budget, int32_t max_running, int32_t n_str
         return fail(MCDP_E_NOMEM, "hipMalloc (serving pool of " + std::to_string(memory_budget) +
                                       " bytes): " + hipGetErrorString(e));
     srv->pool = std::make_unique<Pool>((size_t)memory_budget);
+    srv->request_cap = request_cap ? (size_t)request_cap : (size_t)memory_budget / 2;
     srv->max_running = max_running;
     for (int32_t i = 0; i < n_streams; ++i) {
         hipStream_t s = nullptr;
@@ -337,10 +366,25 @@ int mcdp_server_submit(mcdp_server *srv, int32_t ctx_id, const int32_t *seeds, i
     r->P = (int64_t)prop->preds.size();
     r->seeds.assign(seeds, seeds + n);
     r->q.assign(q, q + n_q);
-    r->bytes = footprint(n, r->E, r->P, n_q);
-    if (r->bytes > srv->pool->size())
-        return fail(MCDP_E_NOMEM, "request needs " + std::to_string(r->bytes) + " bytes of device memory; the server's "
-                                  "budget is " + std::to_string(srv->pool->size()));
+    r->bytes = footprint(n, n, r->E, r->P, n_q, false);
+    if (r->bytes > srv->request_cap) {
+        // too large to hold all samples at once: chunks of samples, the largest that fit the cap
+        if (!(prop->max_delay < std::numeric_limits<double>::infinity()))
+            return fail(MCDP_E_NOMEM, "request needs " + std::to_string(r->bytes) + " bytes of device memory (cap " +
+                                          std::to_string(srv->request_cap) +
+                                          "); statistics over sample chunks need a finite max_delay");
+        const size_t fixed = footprint(n, 0, r->E, r->P, n_q, true);
+        const size_t per_sample = (size_t)r->E * 12;
+        int64_t m = fixed < srv->request_cap ? (int64_t)((srv->request_cap - fixed - 1024) / std::max<size_t>(per_sample, 1)) : 0;
+        m = m / 128 * 128;
+        if (m < 128)
+            return fail(MCDP_E_NOMEM, "request needs " + std::to_string(r->bytes) + " bytes of device memory; even "
+                                      "chunks of 128 samples exceed the request cap of " +
+                                      std::to_string(srv->request_cap) + " bytes (budget " +
+                                      std::to_string(srv->pool->size()) + ")");
+        r->chunk = std::min<int64_t>(m, n);
+        r->bytes = footprint(n, r->chunk, r->E, r->P, n_q, true);
+    }
     const size_t host_doubles = (size_t)(r->E + r->E * n_q + r->E + r->P);
     hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&r->h_out), std::max<size_t>(host_doubles, 1) * 8);
     if (e != hipSuccess) return fail(MCDP_E_NOMEM, std::string("hipHostMalloc (results): ") + hipGetErrorString(e));
@@ -355,6 +399,7 @@ int mcdp_server_submit(mcdp_server *srv, int32_t ctx_id, const int32_t *seeds, i
         *request_id = r->id;
         srv->queued.push_back(r.get());
         ++srv->n_submitted;
+        if (r->chunk > 0) ++srv->n_chunked;
         srv->all.emplace(r->id, std::move(r));
     }
     srv->cv_work.notify_all();
@@ -392,10 +437,10 @@ int mcdp_server_wait(mcdp_server *srv, int64_t request_id, double *mean, double 
 int mcdp_server_stats(mcdp_server *srv, int64_t *out, int n_out) {
     if (!srv || !out) return fail(MCDP_E_INVALID, "null argument");
     std::lock_guard<std::mutex> lock(srv->mu);
-    const int64_t v[8] = {srv->n_submitted,          srv->n_completed,         srv->n_failed,
+    const int64_t v[9] = {srv->n_submitted,          srv->n_completed,         srv->n_failed,
                           (int64_t)srv->queued.size(), (int64_t)srv->running.size(), srv->max_running_seen,
-                          (int64_t)srv->pool->used(), (int64_t)srv->pool->peak()};
-    for (int i = 0; i < n_out && i < 8; ++i) out[i] = v[i];
+                          (int64_t)srv->pool->used(), (int64_t)srv->pool->peak(), srv->n_chunked};
+    for (int i = 0; i < n_out && i < 9; ++i) out[i] = v[i];
     return MCDP_OK;
 }
 

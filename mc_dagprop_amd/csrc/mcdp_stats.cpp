@@ -110,6 +110,38 @@ int quantile_args(int64_t n, int32_t n_q, const double *q, QuantileArgs &qa) {
 
 }  // namespace
 
+int mcdp::chunk_stats_accumulate(mcdp_propagator &P, const double *realized, const int32_t *cause, int32_t m,
+                                 int32_t ld, bool first, ChunkStat *st, uint32_t *hist, int64_t *counts,
+                                 hipStream_t s) {
+    std::lock_guard<std::mutex> lock(P.mu);
+    DeviceCopy *d = nullptr;
+    int rc = ensure_device(P, d);
+    if (rc) return rc;
+    rc = ensure_topo(P, *d);
+    if (rc) return rc;
+    HIP_TRY(launch_chunk_accumulate(realized, m, ld, P.E, d->base.earliest, P.max_delay, kMaxChunkBins, first ? 1 : 0,
+                                    st, hist, s),
+            "chunk statistics kernel");
+    HIP_TRY(launch_cause_counts(cause, m, ld, P.E, d->pred_off, d->pred_key, d->pred_pos, d->pred_nkey, counts,
+                                first ? 0 : 1, s),
+            "cause count kernel");
+    return MCDP_OK;
+}
+
+int mcdp::chunk_stats_finish(mcdp_propagator &P, int64_t n, int32_t n_q, const double *q, const ChunkStat *st,
+                             const uint32_t *hist, double *mean, double *quantiles, hipStream_t s) {
+    QuantileArgs qa;
+    int rc = quantile_args(n, n_q, q, qa);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(P.mu);
+    DeviceCopy *d = nullptr;
+    rc = ensure_device(P, d);
+    if (rc) return rc;
+    HIP_TRY(launch_chunk_finish(n, P.E, qa, d->base.earliest, P.max_delay, kMaxChunkBins, st, hist, mean, quantiles, s),
+            "chunk statistics kernel");
+    return MCDP_OK;
+}
+
 int mcdp::prepare_stats(mcdp_propagator &P) {
     std::lock_guard<std::mutex> lock(P.mu);
     DeviceCopy *d = nullptr;
@@ -157,7 +189,7 @@ int mcdp_event_stats_device(mcdp_propagator *prop, const double *realized_dev, c
         rc = ensure_topo(*prop, *d);
         if (rc) return rc;
         HIP_TRY(launch_cause_counts(cause_dev, (int32_t)n, (int32_t)ld, E, d->pred_off, d->pred_key, d->pred_pos,
-                                    d->pred_nkey, cause_counts_dev, s),
+                                    d->pred_nkey, cause_counts_dev, 0, s),
                 "cause count kernel");
     }
     return MCDP_OK;

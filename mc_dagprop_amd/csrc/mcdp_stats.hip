@@ -307,7 +307,7 @@ __global__ void __launch_bounds__(kBlock) cause_kernel(const int32_t *__restrict
                                                        const int32_t *__restrict__ pred_key,
                                                        const int32_t *__restrict__ pred_pos,
                                                        const int32_t *__restrict__ pred_nkey,
-                                                       int64_t *__restrict__ counts) {
+                                                       int64_t *__restrict__ counts, int accumulate) {
     __shared__ uint32_t c_lds[kCauseLdsCounters];
     __shared__ int32_t k_lds[kCauseLdsKeys], p_lds[kCauseLdsKeys];
     for (int32_t e = blockIdx.x; e < E; e += gridDim.x) {
@@ -324,7 +324,7 @@ __global__ void __launch_bounds__(kBlock) cause_kernel(const int32_t *__restrict
         const bool in_lds = np + 1 <= kCauseLdsCounters;
         if (in_lds) {
             for (int i = threadIdx.x; i <= np; i += kBlock) c_lds[i] = 0;
-        } else {
+        } else if (!accumulate) {
             for (int i = threadIdx.x; i <= np; i += kBlock) out[i] = 0;
         }
         __syncthreads();
@@ -356,12 +356,166 @@ __global__ void __launch_bounds__(kBlock) cause_kernel(const int32_t *__restrict
         }
         __syncthreads();
         if (in_lds)
-            for (int i = threadIdx.x; i <= np; i += kBlock) out[i] = (int64_t)c_lds[i];
+            for (int i = threadIdx.x; i <= np; i += kBlock) out[i] = (accumulate ? out[i] : 0) + (int64_t)c_lds[i];
+        __syncthreads();
+    }
+}
+
+// ---- statistics over sample chunks ------------------------------------------
+// An event's running statistics over chunks of samples: the exactly rounded
+// sum (double-double, with the plain sum for non-finite rows) and the
+// extremes, merged in chunk order; and a histogram of `n_bins` bins over the
+// event's range [earliest, earliest + max_delay], where every realized time
+// lies.  Quantiles interpolate inside a bin, so each is within one bin width
+// of numpy's value; the mean and the extremes are exact.
+__global__ void __launch_bounds__(kBlock) chunk_accumulate_kernel(const double *__restrict__ realized, int32_t m,
+                                                                  uint32_t ld, int32_t E,
+                                                                  const double *__restrict__ earliest,
+                                                                  double inv_width, int32_t n_bins, int first,
+                                                                  ChunkStat *__restrict__ st,
+                                                                  uint32_t *__restrict__ hist) {
+    __shared__ uint32_t h[kMaxChunkBins];
+    __shared__ Extremes ext[kBlock];
+    __shared__ DD part[kBlock];
+    __shared__ double naive[kBlock];
+    const int lane = threadIdx.x & 63;
+    for (int32_t e = blockIdx.x; e < E; e += gridDim.x) {
+        const double *row = realized + (uint64_t)(uint32_t)e * ld;
+        const double lo = earliest[e];
+        for (int i = threadIdx.x; i < n_bins; i += kBlock) h[i] = 0;
+        __syncthreads();
+        Extremes x{~0ull, 0ull, 0u, 0u, 0u};
+        DD acc{0.0, 0.0};
+        double plain = 0.0;
+        for (int32_t s0 = 0; s0 < m; s0 += kBlock) {
+            const int32_t s = s0 + (int32_t)threadIdx.x;
+            const bool in = s < m;
+            const double v = in ? row[s] : 0.0;
+            const bool ok = in && v == v;
+            if (in) {
+                acc = dd_add(acc, v);
+                plain += v;
+                if (!ok) ++x.nans;
+            }
+            if (ok) {
+                const uint64_t key = order_key(v);
+                x = merge_extremes(x, Extremes{key, key, 1u, 1u, 0u});
+            }
+            double f = (v - lo) * inv_width;
+            f = f < 0.0 ? 0.0 : f;
+            int32_t b = f < (double)n_bins ? (int32_t)f : n_bins - 1;
+            wave_hist_add(h, ok, (uint32_t)b, lane);
+        }
+        ext[threadIdx.x] = x;
+        part[threadIdx.x] = acc;
+        naive[threadIdx.x] = plain;
+        __syncthreads();
+        for (int w = kBlock / 2; w > 0; w >>= 1) {  // fixed pairing: deterministic
+            if ((int)threadIdx.x < w) {
+                ext[threadIdx.x] = merge_extremes(ext[threadIdx.x], ext[threadIdx.x + w]);
+                part[threadIdx.x] = dd_add(part[threadIdx.x], part[threadIdx.x + w]);
+                naive[threadIdx.x] += naive[threadIdx.x + w];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            DD sum = part[0];
+            double nv = naive[0];
+            Extremes x0 = ext[0];
+            if (!first) {
+                const ChunkStat o = st[e];
+                sum = dd_add(DD{o.sum_hi, o.sum_lo}, sum);
+                nv = o.naive + nv;
+                x0 = merge_extremes(Extremes{o.mn, o.mx, o.cmn, o.cmx, o.nans}, x0);
+            }
+            ChunkStat c;
+            c.sum_hi = sum.hi;
+            c.sum_lo = sum.lo;
+            c.naive = nv;
+            c.mn = x0.mn;
+            c.mx = x0.mx;
+            c.cmn = x0.cmn;
+            c.cmx = x0.cmx;
+            c.nans = x0.nans;
+            c.pad = 0;
+            st[e] = c;
+        }
+        uint32_t *gh = hist + (uint64_t)(uint32_t)e * (uint32_t)n_bins;
+        for (int i = threadIdx.x; i < n_bins; i += kBlock) gh[i] = (first ? 0u : gh[i]) + h[i];
+        __syncthreads();
+    }
+}
+
+// Means and quantiles from the chunk statistics of n samples (numpy's
+// "linear" method on order statistics estimated inside their histogram bin).
+__global__ void __launch_bounds__(kBlock) chunk_finish_kernel(int64_t n, int32_t E, QuantileArgs qa,
+                                                              const double *__restrict__ earliest, double width,
+                                                              int32_t n_bins, const ChunkStat *__restrict__ st,
+                                                              const uint32_t *__restrict__ hist,
+                                                              double *__restrict__ mean, double *__restrict__ out) {
+    __shared__ uint64_t seg[kBlock];  // bins per thread: n_bins / kBlock (a power of two)
+    __shared__ double stat[kMaxRanks];
+    const int per = n_bins / kBlock;
+    for (int32_t e = blockIdx.x; e < E; e += gridDim.x) {
+        const ChunkStat c = st[e];
+        const uint32_t *h = hist + (uint64_t)(uint32_t)e * (uint32_t)n_bins;
+        uint64_t t = 0;
+        for (int i = 0; i < per; ++i) t += h[threadIdx.x * per + i];
+        seg[threadIdx.x] = t;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (mean) mean[e] = (__builtin_isfinite(c.naive) ? c.sum_hi : c.naive) / (double)n;
+            const double lo = earliest[e];
+            const int64_t valid = n - (int64_t)c.nans;
+            for (int r = 0; r < qa.n_ranks; ++r) {
+                const int64_t k = qa.rank[r];
+                double v;
+                if (c.nans) {
+                    v = __builtin_nan("");
+                } else if (k >= valid - (int64_t)c.cmx) {
+                    v = key_value(c.mx);
+                } else if (k < (int64_t)c.cmn) {
+                    v = key_value(c.mn);
+                } else {
+                    int64_t cum = 0;
+                    int g = 0;
+                    while (g < kBlock - 1 && cum + (int64_t)seg[g] <= k) cum += (int64_t)seg[g++];
+                    int b = g * per;
+                    while (b < (g + 1) * per - 1 && cum + (int64_t)h[b] <= k) cum += (int64_t)h[b++];
+                    const double inside = ((double)(k - cum) + 0.5) / (double)(h[b] ? h[b] : 1u);
+                    v = lo + ((double)b + inside) * width;
+                    const double vmin = key_value(c.mn), vmax = key_value(c.mx);
+                    v = v < vmin ? vmin : (v > vmax ? vmax : v);
+                }
+                stat[r] = v;
+            }
+            for (int j = 0; j < qa.n_q; ++j)
+                out[(uint64_t)(uint32_t)e * (uint32_t)qa.n_q + j] =
+                    c.nans ? __builtin_nan("") : lerp_numpy(stat[qa.lo[j]], stat[qa.hi[j]], qa.gamma[j]);
+        }
         __syncthreads();
     }
 }
 
 }  // namespace
+
+hipError_t launch_chunk_accumulate(const double *realized, int32_t m, int32_t ld, int32_t E, const double *earliest,
+                                   double max_delay, int32_t n_bins, int first, ChunkStat *st, uint32_t *hist,
+                                   hipStream_t stream) {
+    if (E > 0)
+        hipLaunchKernelGGL(chunk_accumulate_kernel, dim3((unsigned)E), dim3(kBlock), 0, stream, realized, m,
+                           (uint32_t)ld, E, earliest, (double)n_bins / max_delay, n_bins, first, st, hist);
+    return hipGetLastError();
+}
+
+hipError_t launch_chunk_finish(int64_t n, int32_t E, const QuantileArgs &qa, const double *earliest, double max_delay,
+                               int32_t n_bins, const ChunkStat *st, const uint32_t *hist, double *mean, double *out,
+                               hipStream_t stream) {
+    if (E > 0)
+        hipLaunchKernelGGL(chunk_finish_kernel, dim3((unsigned)E), dim3(kBlock), 0, stream, n, E, qa, earliest,
+                           max_delay / (double)n_bins, n_bins, st, hist, mean, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_event_mean(const double *realized, int32_t n, int32_t ld, int32_t E, double *mean,
                              hipStream_t stream) {
@@ -379,10 +533,10 @@ hipError_t launch_event_quantiles(const double *realized, int32_t n, int32_t ld,
 
 hipError_t launch_cause_counts(const int32_t *cause, int32_t n, int32_t ld, int32_t E, const int32_t *pred_off,
                                const int32_t *pred_key, const int32_t *pred_pos, const int32_t *pred_nkey,
-                               int64_t *counts, hipStream_t stream) {
+                               int64_t *counts, int accumulate, hipStream_t stream) {
     if (E > 0)
         hipLaunchKernelGGL(cause_kernel, dim3((unsigned)E), dim3(kBlock), 0, stream, cause, n, (uint32_t)ld, E, pred_off,
-                           pred_key, pred_pos, pred_nkey, counts);
+                           pred_key, pred_pos, pred_nkey, counts, accumulate);
     return hipGetLastError();
 }
 

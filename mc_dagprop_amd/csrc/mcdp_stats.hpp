@@ -30,8 +30,30 @@ hipError_t launch_event_mean(const double *realized, int32_t n, int32_t ld, int3
 // `mean` (nullable): the row means, computed in the quantile kernel's first pass.
 hipError_t launch_event_quantiles(const double *realized, int32_t n, int32_t ld, int32_t E, const QuantileArgs &qa,
                                   double *out, double *mean, hipStream_t stream);
+// accumulate != 0 adds to the counts already in `counts`.
 hipError_t launch_cause_counts(const int32_t *cause, int32_t n, int32_t ld, int32_t E, const int32_t *pred_off,
                                const int32_t *pred_key, const int32_t *pred_pos, const int32_t *pred_nkey,
-                               int64_t *counts, hipStream_t stream);
+                               int64_t *counts, int accumulate, hipStream_t stream);
+
+// Statistics of a request run in chunks of samples (serving).  Per event: the
+// double-double sum, the plain sum, the extremes with their multiplicities and
+// NaN count (ChunkStat), and a histogram of n_bins bins over
+// [earliest, earliest + max_delay].
+constexpr int kMaxChunkBins = 4096;  // a power of two and a multiple of the block size
+struct ChunkStat {
+    double sum_hi, sum_lo;
+    double naive;
+    uint64_t mn, mx;
+    uint32_t cmn, cmx, nans, pad;
+};
+// first != 0 starts the sums; each call adds chunk rows realized[e][0..m).
+hipError_t launch_chunk_accumulate(const double *realized, int32_t m, int32_t ld, int32_t E, const double *earliest,
+                                   double max_delay, int32_t n_bins, int first, ChunkStat *st, uint32_t *hist,
+                                   hipStream_t stream);
+// mean[e] (exactly rounded sum / n) and out[e * n_q + j]: numpy-linear quantiles
+// from order statistics placed inside their bin (within one bin width).
+hipError_t launch_chunk_finish(int64_t n, int32_t E, const QuantileArgs &qa, const double *earliest, double max_delay,
+                               int32_t n_bins, const ChunkStat *st, const uint32_t *hist, double *mean, double *out,
+                               hipStream_t stream);
 
 }  // namespace mcdp

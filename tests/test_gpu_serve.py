@@ -32,6 +32,9 @@ def same_stats(got, want) -> None:
     np.testing.assert_array_equal(counts, w_counts)
 
 
+BINS = 4096  # histogram bins of statistics over sample chunks
+
+
 def footprint(E: int, P: int, n: int, nq: int) -> int:
     up = lambda x: (x + 255) // 256 * 256  # noqa: E731
     return up(4 * n) + up(8 * E * n) + up(4 * E * n) + up(8 * E) + up(8 * E * nq) + up(8 * (E + P))
@@ -45,7 +48,7 @@ def contexts():
 def test_results_in_the_mix_equal_alone(gpu) -> None:
     dags, props = contexts()
     budget = 48 << 20
-    srv = capi.Server(budget, max_running=4, n_streams=3)
+    srv = capi.Server(budget, max_running=4, n_streams=3, request_cap=budget)
     ids = [srv.add_context(p) for p in props]
     rng = np.random.default_rng(4)
     reqs = []
@@ -73,7 +76,7 @@ def test_requests_wait_for_memory(gpu) -> None:
     p = props[1]
     n = 2000
     need = footprint(p.E, int(p.pred_offsets()[-1]), n, 3)
-    srv = capi.Server(need + need // 2, max_running=8, n_streams=4)
+    srv = capi.Server(need + need // 2, max_running=8, n_streams=4, request_cap=need + need // 2)
     cid = srv.add_context(p)
     seeds = [np.arange(i * n, (i + 1) * n, dtype=np.int32) for i in range(6)]
     rids = [srv.submit(cid, s) for s in seeds]
@@ -87,9 +90,9 @@ def test_requests_wait_for_memory(gpu) -> None:
 def test_oversized_request_is_refused(gpu) -> None:
     _, props = contexts()
     p = props[1]
-    srv = capi.Server(1 << 20, max_running=2, n_streams=1)
+    srv = capi.Server(1 << 20, max_running=2, n_streams=1, request_cap=1 << 20)
     cid = srv.add_context(p)
-    with pytest.raises(capi.McdpError, match="budget"):
+    with pytest.raises(capi.McdpError, match="cap"):
         srv.submit(cid, np.arange(100_000, dtype=np.int32))
     with pytest.raises(capi.McdpError):
         srv.submit(cid + 5, np.arange(10, dtype=np.int32))
@@ -102,7 +105,7 @@ def test_oversized_request_is_refused(gpu) -> None:
 
 def test_threads_submit_and_wait(gpu) -> None:
     dags, props = contexts()
-    srv = capi.Server(64 << 20, max_running=6, n_streams=4)
+    srv = capi.Server(64 << 20, max_running=6, n_streams=4, request_cap=64 << 20)
     ids = [srv.add_context(p) for p in props]
     errors = []
 
@@ -129,8 +132,53 @@ def test_threads_submit_and_wait(gpu) -> None:
 def test_close_with_requests_in_flight(gpu) -> None:
     """Closing drains running requests and drops queued ones without hanging."""
     _, props = contexts()
-    srv = capi.Server(16 << 20, max_running=2, n_streams=2)
+    srv = capi.Server(16 << 20, max_running=2, n_streams=2, request_cap=16 << 20)
     cid = srv.add_context(props[1])
     for i in range(10):
         srv.submit(cid, np.arange(i * 1000, (i + 1) * 1000, dtype=np.int32))
+    srv.close()
+
+
+def test_chunked_requests(gpu) -> None:
+    """Requests above the request cap run in chunks of samples: the mean (exactly rounded sum) and the cause
+    counts equal the one-piece statistics bit for bit; each quantile is within one histogram bin
+    (max_delay / 4096) of numpy's; chunked and one-piece requests share the device."""
+    dags, props = contexts()
+    big, small = props[1], props[0]
+    srv = capi.Server(96 << 20, max_running=4, n_streams=3, request_cap=24 << 20)
+    cb, cs = srv.add_context(big), srv.add_context(small)
+    rng = np.random.default_rng(12)
+    q = [0.0, 0.25, 0.5, 0.9, 0.99, 1.0]
+    reqs = []
+    for i in range(6):
+        seeds = rng.integers(0, 2**31 - 1, int(rng.integers(3000, 6000))).astype(np.int32)
+        reqs.append((srv.submit(cb, seeds, q), big, seeds, dags[1].max_delay))
+        s2 = rng.integers(0, 2**31 - 1, 500).astype(np.int32)
+        reqs.append((srv.submit(cs, s2, q), small, s2, None))
+    assert srv.stats()["chunked"] == 6
+    for rid, p, seeds, md in reqs:
+        mean, quant, counts, _ = srv.wait(rid)
+        w_mean, w_quant, w_counts = p.run_stats(seeds, q)
+        np.testing.assert_array_equal(bits(mean), bits(w_mean))
+        np.testing.assert_array_equal(counts, w_counts)
+        if md is None:
+            ok = (bits(quant) == bits(w_quant)) | ((quant == 0) & (w_quant == 0))
+            assert ok.all()
+        else:
+            tol = md / BINS * (1 + 1e-9) + 1e-9
+            assert np.max(np.abs(quant - w_quant)) <= tol
+            # the extremes (q = 0 and 1) are exact
+            np.testing.assert_array_equal(bits(quant[:, [0, -1]]), bits(w_quant[:, [0, -1]]))
+    st = srv.stats()
+    assert st["completed"] == 12 and st["peak_bytes"] <= 96 << 20 and st["max_running"] >= 2
+    srv.close()
+
+
+def test_chunked_request_needs_finite_max_delay(gpu) -> None:
+    dag = D.continuous_dag(3, 60, max_delay=float("inf"), shared_links=True)
+    p = capi.Propagator(dag)
+    srv = capi.Server(8 << 20, max_running=2, n_streams=1, request_cap=1 << 20)
+    cid = srv.add_context(p)
+    with pytest.raises(capi.McdpError, match="finite max_delay"):
+        srv.submit(cid, np.arange(20_000, dtype=np.int32))
     srv.close()
