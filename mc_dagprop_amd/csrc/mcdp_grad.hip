@@ -485,6 +485,9 @@ __global__ void __launch_bounds__(256) unit_draw_kernel(const Record *__restrict
                                                         const int32_t *__restrict__ seeds, int32_t first, int32_t count,
                                                         int32_t n, const DistParams *dists, const AliasRec *alias,
                                                         double *__restrict__ out, uint32_t ld_out) {
+    __shared__ LogTableLds log_lds;
+    stage_log_table(log_lds);
+    __syncthreads();
     const int32_t q = blockIdx.y;
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n_links || i >= count) return;
@@ -492,7 +495,8 @@ __global__ void __launch_bounds__(256) unit_draw_kernel(const Record *__restrict
     const Record rec = as_const_space(links)[q];
     const uint32_t seed[1] = {(uint32_t)seeds[si]};
     double unit[1];
-    draw_durations<1>(rec.info, 1.0, rec.b, (uint32_t)rec.i2, seed, as_const_space(alias), as_const_space(dists), unit);
+    draw_durations<1>(rec.info, 1.0, rec.b, (uint32_t)rec.i2, seed, as_const_space(alias), as_const_space(dists),
+                      log_lds.view(), unit);
     out[row_offset(q, ld_out) + i] = unit[0];
 }
 

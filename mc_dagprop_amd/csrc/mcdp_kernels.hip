@@ -64,10 +64,12 @@ __device__ __forceinline__ AliasRec *lds_alias_of(const WalkParams &p, double *l
 }
 template <bool kInjected, int kPaths, int kSpl>
 __device__ __forceinline__ void stage_alias(const WalkParams &p, double *lds) {
-    if (!kInjected && kPaths == kPathsFast) {
-        AliasRec *const dst = lds_alias_of<kInjected, kPaths, kSpl>(p, lds);
-        const auto src = as_const_space(p.alias);
-        for (int i = threadIdx.x; i < p.n_alias; i += blockDim.x) dst[i] = src[i];
+    if (!kInjected) {
+        if (kPaths == kPathsFast) {
+            AliasRec *const dst = lds_alias_of<kInjected, kPaths, kSpl>(p, lds);
+            const auto src = as_const_space(p.alias);
+            for (int i = threadIdx.x; i < p.n_alias; i += blockDim.x) dst[i] = src[i];
+        }
         __syncthreads();
     }
 }
@@ -76,7 +78,7 @@ __device__ __forceinline__ void stage_alias(const WalkParams &p, double *lds) {
 // activities.  Every wave of the workgroup calls it; it ends with all waves
 // past the last step's barrier.
 template <bool kInjected, int kPaths, int K, int kSpl>
-__device__ __forceinline__ void walk_tile(const WalkParams &p, int32_t tile, double *lds) {
+__device__ __forceinline__ void walk_tile(const WalkParams &p, int32_t tile, double *lds, const LogTable &lt) {
     const int lane = threadIdx.x & (kLanes - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int W = p.waves;
@@ -95,9 +97,9 @@ __device__ __forceinline__ void walk_tile(const WalkParams &p, int32_t tile, dou
     AliasRec *const lds_alias = lds_alias_of<kInjected, kPaths, kSpl>(p, lds);
     auto draw = [&](const Record &r, const uint32_t(&seed)[kSpl], double(&dur)[kSpl]) {
         if (kPaths == kPathsFast)
-            draw_durations<kSpl>(r.info, r.a, r.b, (uint32_t)r.i2, seed, lds_alias, dists, dur);
+            draw_durations<kSpl>(r.info, r.a, r.b, (uint32_t)r.i2, seed, lds_alias, dists, lt, dur);
         else
-            draw_durations<kSpl>(r.info, r.a, r.b, (uint32_t)r.i2, seed, as_const_space(p.alias), dists, dur);
+            draw_durations<kSpl>(r.info, r.a, r.b, (uint32_t)r.i2, seed, as_const_space(p.alias), dists, lt, dur);
     };
 
     // Lanes past the last sample (last tile only) take over sample n - 1:
@@ -226,8 +228,11 @@ __device__ __forceinline__ void walk_tile(const WalkParams &p, int32_t tile, dou
 template <bool kInjected, int kPaths, int K, int kSpl>
 __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) {
     extern __shared__ double lds[];
-    stage_alias<kInjected, kPaths, kSpl>(p, lds);
-    for (int32_t tile = blockIdx.x; tile < p.n_tiles; tile += gridDim.x) walk_tile<kInjected, kPaths, K, kSpl>(p, tile, lds);
+    __shared__ LogTableLds log_lds;
+    if (!kInjected) stage_log_table(log_lds);
+    stage_alias<kInjected, kPaths, kSpl>(p, lds);  // its barrier also publishes the log table
+    for (int32_t tile = blockIdx.x; tile < p.n_tiles; tile += gridDim.x)
+        walk_tile<kInjected, kPaths, K, kSpl>(p, tile, lds, log_lds.view());
 }
 
 // Several plans in one launch: workgroups take (plan, tile) items from a
@@ -237,6 +242,8 @@ template <bool kInjected, int kPaths, int K, int kSpl>
 __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_batch_kernel(WalkBatchParams b) {
     extern __shared__ double lds[];
     __shared__ int32_t next_item;
+    __shared__ LogTableLds log_lds;
+    stage_log_table(log_lds);  // published by the loop's first barrier
     int32_t staged = -1;
     const auto item_dag = as_const_space(b.item_dag);
     const auto item_tile = as_const_space(b.item_tile);
@@ -253,7 +260,7 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_batch_kernel(WalkBatc
             stage_alias<kInjected, kPaths, kSpl>(p, lds);
             staged = d;
         }
-        walk_tile<kInjected, kPaths, K, kSpl>(p, item_tile[item], lds);
+        walk_tile<kInjected, kPaths, K, kSpl>(p, item_tile[item], lds, log_lds.view());
     }
 }
 
@@ -277,13 +284,16 @@ __global__ void philox_debug_kernel(int variant, int64_t n, const uint32_t *__re
     }
 }
 
-// The samplers' elementary functions on n inputs: 0 log_fast, 1 cos2pi_fast,
+// The samplers' elementary functions on n inputs: 0 log_tab, 1 cos2pi_fast,
 // 2 exp_fast.
 __global__ void math_debug_kernel(int fn, int64_t n, const double *__restrict__ x, double *__restrict__ out) {
+    __shared__ LogTableLds log_lds;
+    stage_log_table(log_lds);
+    __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double v = x[i];
-    out[i] = fn == 0 ? log_fast(v) : fn == 1 ? cos2pi_fast(v) : exp_fast(v);
+    out[i] = fn == 0 ? log_tab(v, log_lds.view()) : fn == 1 ? cos2pi_fast(v) : exp_fast(v);
 }
 
 __global__ void __launch_bounds__(256) copy_kernel(double *__restrict__ dst, const double *__restrict__ src,

@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "mcdp_log_table.hpp"
+
 namespace mcdp {
 
 // p * z + c for a polynomial coefficient c.  The compiler's accumulate form
@@ -24,49 +26,62 @@ __device__ __forceinline__ double fma_k(double p, double z, double c) {
 #endif
 }
 
-// Quotient a / b to ~1 ulp: hardware reciprocal estimate refined by two
-// Newton-Raphson steps, then one residual correction of the quotient.
-__device__ __forceinline__ double div_fast(double a, double b) {
-    double r = __builtin_amdgcn_rcp(b);
-    double e = __builtin_fma(-b, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-b, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    const double q = a * r;
-    return __builtin_fma(__builtin_fma(-b, q, a), r, q);
+// The breakpoint table of log_tab, staged in LDS by every kernel that samples
+// (the lookups are per lane).  kLogInvF, kLogHi and kLogLo come from
+// mcdp_log_table.hpp, written by gen_log_table.py.
+struct LogTable {
+    const double *inv;    // [kLogTableSize]: 1/F_j rounded
+    const double2 *hilo;  // [kLogTableSize]: log F_j = hi + lo, hi a multiple of 2^-32
+};
+struct LogTableLds {
+    double inv[kLogTableSize];
+    double2 hilo[kLogTableSize];
+    __device__ LogTable view() const { return LogTable{inv, hilo}; }
+};
+// Every thread of the workgroup calls it; the caller's next barrier makes the
+// table visible to all waves.
+__device__ __forceinline__ void stage_log_table(LogTableLds &t) {
+    for (int i = threadIdx.x; i < kLogTableSize; i += blockDim.x) {
+        t.inv[i] = kLogInvF[i];
+        t.hilo[i] = double2{kLogHi[i], kLogLo[i]};
+    }
 }
 
-// Natural logarithm of a positive normal x.
-// x = 2^k m with m in [sqrt(1/2), sqrt(2)); with s = (m - 1) / (m + 1),
-// log m = 2 atanh s = 2 s (1 + s^2/3 + s^4/5 + ...), |s| <= 0.1716, so ten
-// terms of the series reach 2^-53; log x = k ln 2 + log m with ln 2 split in a
-// high part exact in k * ln2_hi and a low part.
-__device__ __forceinline__ double log_fast(double x) {
-    double m = __builtin_amdgcn_frexp_mant(x);  // [0.5, 1)
-    int k = __builtin_amdgcn_frexp_exp(x);
-    const bool lo = m < 0.70710678118654752440;
-    m = lo ? m + m : m;
-    k = lo ? k - 1 : k;
-    const double f = m - 1.0;  // exact (Sterbenz)
-    const double s = div_fast(f, m + 1.0);
-    const double z = s * s;
-    double p = 1.0 / 21.0;
-    p = fma_k(p, z, 1.0 / 19.0);
-    p = fma_k(p, z, 1.0 / 17.0);
-    p = fma_k(p, z, 1.0 / 15.0);
-    p = fma_k(p, z, 1.0 / 13.0);
-    p = fma_k(p, z, 1.0 / 11.0);
-    p = fma_k(p, z, 1.0 / 9.0);
-    p = fma_k(p, z, 1.0 / 7.0);
-    p = fma_k(p, z, 1.0 / 5.0);
-    p = fma_k(p, z, 1.0 / 3.0);
-    // log m = 2s + 2s z p; the correction term is < 1% of 2s.
-    const double two_s = s + s;
-    const double log_m = __builtin_fma(two_s * z, p, two_s);
+// Natural logarithm of a positive finite x, table-driven (P. T. P. Tang,
+// "Table-driven implementation of the logarithm function in IEEE
+// floating-point arithmetic", ACM TOMS 16(4), 1990):
+// x = 2^k m with m in [1/2, 1); F = 1/2 + j/256 the breakpoint nearest m;
+// log x = k ln2 + log F + log(1 + r) with r = (m - F)/F, |r| <= 2^-8.
+//   * m - F is exact (Sterbenz); r = (m - F) * (1/F) carries two roundings,
+//     i.e. ~2^-60 absolute;
+//   * log(1 + r) = r + r^2 P(r), Taylor to r^7 (remainder < 2^-67);
+//   * k ln2_hi + hi is exact (both multiples of 2^-32, sum below 2^10), and
+//     the small terms k ln2_lo + lo + log(1 + r) are added to it last.
+// At x = 1 (k = 1, F = 1/2, log F = -ln2_hi - ln2_lo) both parts cancel to 0;
+// near 1 from below F = 1 and the result is log(1 + r) alone, so the
+// relative error stays within an ulp there too.
+__device__ __forceinline__ double log_tab(double x, const LogTable &t) {
+    const double m = __builtin_amdgcn_frexp_mant(x);  // [1/2, 1)
+    const int k = __builtin_amdgcn_frexp_exp(x);
+    const double jd = __builtin_rint(__builtin_fma(m, 256.0, -128.0));  // exact argument; j in [0, 128]
+    const int j = (int)jd;
+    const double F = __builtin_fma(jd, 1.0 / 256.0, 0.5);  // exact
+    const double r = (m - F) * t.inv[j];
+    const double2 lf = t.hilo[j];
+    const double r2 = r * r;
+    double p = 1.0 / 7.0;
+    p = fma_k(p, r, -1.0 / 6.0);
+    p = fma_k(p, r, 1.0 / 5.0);
+    p = fma_k(p, r, -1.0 / 4.0);
+    p = fma_k(p, r, 1.0 / 3.0);
+    p = __builtin_fma(p, r, -0.5);
+    const double log1p_r = __builtin_fma(r2, p, r);
     const double kd = (double)k;
     constexpr double ln2_hi = 6.93147180369123816490e-01;  // 0x3fe62e42fee00000
     constexpr double ln2_lo = 1.90821492927058770002e-10;
-    return __builtin_fma(kd, ln2_hi, __builtin_fma(kd, ln2_lo, log_m));
+    const double hi = __builtin_fma(kd, ln2_hi, lf.x);
+    const double lo = __builtin_fma(kd, ln2_lo, lf.y) + log1p_r;
+    return hi + lo;
 }
 
 // cos(2 pi u) for u in [0, 1], to within ~3e-16 absolute.

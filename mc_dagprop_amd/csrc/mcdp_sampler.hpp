@@ -117,18 +117,18 @@ __device__ __forceinline__ Words4 draw_words4(uint32_t edge, uint32_t seed, uint
 // One Marsaglia-Tsang attempt for shape k' = d + 1/3 >= 1, scale 1
 // (Marsaglia & Tsang, ACM TOMS 26(3), 2000).  The normal comes from one
 // Box-Muller branch.  Returns false on rejection.
-__device__ __forceinline__ bool mt_attempt(const Words4 &w, double d, double c, double &x) {
+__device__ __forceinline__ bool mt_attempt(const Words4 &w, double d, double c, const LogTable &lt, double &x) {
     const double u1 = u01_open0(w.x, w.y);
     const double u2 = u01_mid32(w.z);
     const double u3 = u01_mid32(w.w);
-    const double z = sqrt(-2.0 * log_fast(u1)) * cos2pi_fast(u2);
+    const double z = sqrt(-2.0 * log_tab(u1, lt)) * cos2pi_fast(u2);
     double v = 1.0 + c * z;
     if (!(v > 0.0)) return false;
     v = v * v * v;
     const double z2 = z * z;
     x = d * v;
     if (u3 < 1.0 - 0.0331 * (z2 * z2)) return true;
-    return log_fast(u3) < 0.5 * z2 + d * (1.0 - v + log_fast(v));
+    return log_tab(u3, lt) < 0.5 * z2 + d * (1.0 - v + log_tab(v, lt));
 }
 
 // Gamma family with rejection: on the truncation bound max_scale (and, for
@@ -136,7 +136,7 @@ __device__ __forceinline__ bool mt_attempt(const Words4 &w, double d, double c, 
 // attempt number in the counter so every lane's draw is fixed by (seed, edge)
 // alone.  Returns X (scale included).
 __device__ __forceinline__ double gamma_rejection(uint32_t kind, uint32_t flags, const DistParams &D, uint32_t edge,
-                                                  uint32_t seed) {
+                                                  uint32_t seed, const LogTable &lt) {
     const double m = D.p[3];
     double X = 0.0;
     uint32_t j = 0;
@@ -144,23 +144,23 @@ __device__ __forceinline__ double gamma_rejection(uint32_t kind, uint32_t flags,
         const Words4 w = draw_words4(edge, seed, j, kTagMain);
         bool ok = true;
         if (kind == kGammaErlang2) {  // sum of two unit exponentials
-            X = -D.p[0] * log_fast(u01_open0(w.x, w.y) * u01_open0(w.z, w.w));
+            X = -D.p[0] * log_tab(u01_open0(w.x, w.y) * u01_open0(w.z, w.w), lt);
         } else if (kind == kGammaHalf) {  // Gamma(1/2) = N(0,1)^2 / 2 = Exp(1) cos^2(2 pi U)
             const double cs = cos2pi_fast(u01_mid32(w.z));
-            X = D.p[0] * (-log_fast(u01_open0(w.x, w.y)) * (cs * cs));
+            X = D.p[0] * (-log_tab(u01_open0(w.x, w.y), lt) * (cs * cs));
         } else if (kind == kGammaMT) {
             double y;
-            ok = mt_attempt(w, D.p[1], D.p[2], y);
+            ok = mt_attempt(w, D.p[1], D.p[2], lt, y);
             if (ok) {
                 if (flags & kInfoBoost) {  // shape < 1: Gamma(k) = Gamma(k + 1) U^{1/k}
                     const Words2 b = draw_words2(edge, seed, (j << 8) | kTagBoost);
-                    y *= exp_fast(log_fast(u01_open0(b.x, b.y)) * D.p[4]);
+                    y *= exp_fast(log_tab(u01_open0(b.x, b.y), lt) * D.p[4]);
                 }
                 X = D.p[0] * y;
             }
         } else {  // kGammaPow: density x^{k-1} e^{-x} on [0, m'] by proposal
                   // m' U^{1/k}, accepted with probability e^{-x}
-            const double y = D.p[1] * exp_fast(log_fast(u01_open0(w.x, w.y)) * D.p[2]);
+            const double y = D.p[1] * exp_fast(log_tab(u01_open0(w.x, w.y), lt) * D.p[2]);
             ok = u01_open0(w.z, w.w) <= exp_fast(-y);
             X = (y * D.p[0] > m) ? m : y * D.p[0];
         }
@@ -177,7 +177,8 @@ __device__ __forceinline__ double gamma_rejection(uint32_t kind, uint32_t flags,
 // distribution switch is taken once and each case runs the N independent
 // draws as straight-line code, so their dependency chains interleave.
 // `info`, `dmin` and `p0` come from the activity's wave-uniform record, `edge`
-// is its index; kinds that need more parameters read them from `dists`.
+// is its index; kinds that need more parameters read them from `dists`; `lt`
+// is the logarithm's breakpoint table in LDS.
 // Each result is base + extra rounded exactly as the reference (_core.cpp:75,
 // 88, 103, 125, 140, 328): extra = X * d (one rounding), then d + extra (one
 // rounding); this file is compiled with -ffp-contract=off.  `alias` points at
@@ -185,7 +186,7 @@ __device__ __forceinline__ double gamma_rejection(uint32_t kind, uint32_t flags,
 template <int N, typename AliasPtr>
 __device__ __forceinline__ void draw_durations(uint32_t info, double dmin, double p0, uint32_t edge,
                                                const uint32_t (&seed)[N], AliasPtr alias,
-                                               const_ptr<DistParams> dists, double (&dur)[N]) {
+                                               const_ptr<DistParams> dists, const LogTable &lt, double (&dur)[N]) {
     const uint32_t kind = info & kInfoKindMask;
     double X[N];
     switch (kind) {
@@ -229,7 +230,7 @@ __device__ __forceinline__ void draw_durations(uint32_t info, double dmin, doubl
 #pragma unroll
             for (int i = 0; i < N; ++i) {
                 const Words2 w = draw_words2(edge, seed[i], kTagMain);
-                X[i] = -p0 * log_fast(u01_open0(w.x, w.y));
+                X[i] = -p0 * log_tab(u01_open0(w.x, w.y), lt);
             }
             break;
         case kExpTrunc: {
@@ -240,7 +241,7 @@ __device__ __forceinline__ void draw_durations(uint32_t info, double dmin, doubl
             for (int i = 0; i < N; ++i) {
                 const Words2 w = draw_words2(edge, seed[i], kTagMain);
                 const double u = __builtin_fma(u01_open0(w.x, w.y), D.p[3], D.p[1]);
-                const double x = -p0 * log_fast(u);
+                const double x = -p0 * log_tab(u, lt);
                 X[i] = (x > D.p[2]) ? D.p[2] : x;
             }
             break;
@@ -250,18 +251,19 @@ __device__ __forceinline__ void draw_durations(uint32_t info, double dmin, doubl
 #pragma unroll
                 for (int i = 0; i < N; ++i) {
                     const Words4 w = draw_words4(edge, seed[i], 0u, kTagMain);
-                    X[i] = -p0 * log_fast(u01_open0(w.x, w.y) * u01_open0(w.z, w.w));
+                    X[i] = -p0 * log_tab(u01_open0(w.x, w.y) * u01_open0(w.z, w.w), lt);
                 }
             } else if (!(info & kInfoTruncated) && kind == kGammaHalf) {  // Exp(1) cos^2(2 pi U)
 #pragma unroll
                 for (int i = 0; i < N; ++i) {
                     const Words4 w = draw_words4(edge, seed[i], 0u, kTagMain);
                     const double cs = cos2pi_fast(u01_mid32(w.z));
-                    X[i] = p0 * (-log_fast(u01_open0(w.x, w.y)) * (cs * cs));
+                    X[i] = p0 * (-log_tab(u01_open0(w.x, w.y), lt) * (cs * cs));
                 }
             } else {
                 const DistParams D = dists[(info >> kInfoDistShift) & kInfoDistMask];
-                for (int i = 0; i < N; ++i) X[i] = gamma_rejection(kind, info, D, edge, seed[i]);
+#pragma unroll
+                for (int i = 0; i < N; ++i) X[i] = gamma_rejection(kind, info, D, edge, seed[i], lt);
             }
             break;
         }
