@@ -135,6 +135,9 @@ __device__ __forceinline__ void walk_tile(const WalkParams &p, int32_t tile, dou
                 const Record nxt = recs[r + 1];
                 if (cur.info & kInfoHeader) {
                     upper = cur.b;
+                    // one vector copy per event: the caps and the final clamp
+                    // select against it without rebuilding it from SGPRs
+                    asm volatile("" : "+v"(upper));
                     event = cur.i0;
 #pragma unroll
                     for (int i = 0; i < kSpl; ++i) {

@@ -89,8 +89,11 @@ __device__ __forceinline__ Words2 philox2x32_10(uint32_t c0, uint32_t c1, uint32
 __device__ __forceinline__ double bits53(uint32_t hi, uint32_t lo) {
     return __builtin_fma((double)(hi >> 11), 4294967296.0, (double)lo);
 }
-// (m + 1) 2^-53, in (0, 1]: safe argument for log.
-__device__ __forceinline__ double u01_open0(uint32_t hi, uint32_t lo) { return (bits53(hi, lo) + 1.0) * 0x1p-53; }
+// (m + 1) 2^-53, in (0, 1]: safe argument for log.  Both fmas are exact:
+// (lo + 1) 2^-53, then (hi >> 11) 2^-21 plus it, a multiple of 2^-53 <= 1.
+__device__ __forceinline__ double u01_open0(uint32_t hi, uint32_t lo) {
+    return __builtin_fma((double)(hi >> 11), 0x1p-21, __builtin_fma((double)lo, 0x1p-53, 0x1p-53));
+}
 // m 2^-53, in [0, 1).
 __device__ __forceinline__ double u01(uint32_t hi, uint32_t lo) { return bits53(hi, lo) * 0x1p-53; }
 // (w + 1/2) 2^-32, in (0, 1): 32-bit resolution, used for an angle and for
