@@ -116,11 +116,12 @@ int mcdp_propagate_device(mcdp_propagator *prop, const double *durations_dev, in
 int mcdp_propagate_host(mcdp_propagator *prop, const double *durations, int64_t n, double *realized,
                         int32_t *cause);
 
-/* Plan facts for tests and tuning: out[0..9] = {steps, waves per workgroup,
+/* Plan facts for tests and tuning: out[0..11] = {steps, waves per workgroup,
  * LDS slots, LDS bytes of the slots per workgroup, predecessors read from
  * global memory, stale predecessors, orphan activities, max events per wave
- * and step, events per wave bound, samples per lane}.  (No counterpart in the
- * reference.) */
+ * and step, events per wave bound, samples per lane, estimated latency of a
+ * partial last round as tiles, the same as wide steps (draw-cost units)}.
+ * (No counterpart in the reference.) */
 int mcdp_propagator_plan_info(const mcdp_propagator *prop, int64_t *out, int n_out);
 
 /* Fixes the schedule shape: `waves` waves per workgroup (1..16), at most
@@ -318,6 +319,12 @@ int mcdp_debug_copy(double *dst_dev, const double *src_dev, int64_t n, void *str
  * tiles, at least one), so tests can run a mcdp_reverse_device call as several
  * launches. */
 int mcdp_debug_reverse_tiles(mcdp_propagator *prop, int64_t max_tiles);
+/* Route of a launch's last partial round (the samples past its last full
+ * round of tiles): 0 = chosen from the schedule's latency estimates (the
+ * default), 1 = tiles of one sample per lane, 2 = wide steps (one launch per
+ * step, every event a wave per group of 64 samples).  Results do not depend on
+ * the route; tests use it to run both. */
+int mcdp_debug_tail_mode(mcdp_propagator *prop, int32_t mode);
 
 #ifdef __cplusplus
 }

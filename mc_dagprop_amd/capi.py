@@ -79,6 +79,7 @@ _SIGNATURES = {
     "mcdp_reverse_device": (C.c_int, [_P, _P, _P, _P, _P, _P, _i64, _i64, C.c_int, _P, _P, _P, _P]),
     "mcdp_gradient_host": (C.c_int, [_P, _pi32, _i64, _pf64, _i64, _pf64, _pf64, _pf64]),
     "mcdp_debug_reverse_tiles": (C.c_int, [_P, _i64]),
+    "mcdp_debug_tail_mode": (C.c_int, [_P, C.c_int32]),
     "mcdp_propagator_gradient_info": (C.c_int, [_P, _pi64, C.c_int]),
     "mcdp_server_create": (C.c_int, [_i64, _i64, _i32, _i32, C.POINTER(_P)]),
     "mcdp_server_destroy": (None, [_P]),
@@ -256,7 +257,7 @@ class Propagator:
         self.L = lib().mcdp_propagator_activity_count(self.h)
 
     PLAN_FIELDS = ["steps", "waves", "lds_slots", "lds_bytes", "global_preds", "stale_preds", "orphans",
-                   "max_events_per_wave", "events_per_wave", "samples_per_lane"]
+                   "max_events_per_wave", "events_per_wave", "samples_per_lane", "tail_tile_cost", "tail_wide_cost"]
 
     def plan_info(self) -> dict:
         v = np.zeros(len(self.PLAN_FIELDS), dtype=np.int64)

@@ -273,6 +273,14 @@ int draw_cost(const DistRec *D) {
 }
 constexpr int kPredCost = 20;   // propagation work per predecessor record
 constexpr int kEventCost = 15;  // per event: header, final cap, stores
+// A wide step's launch gap and predecessor loads, in the same units
+// (calibrated on the network workload: a 200-step tail costs about 12 ns per
+// unit either way).
+constexpr int kWideStepCost = 300;
+// Wide steps for a partial round of at most this many groups of 64 samples
+// (each step launches events x groups single-wave workgroups; the estimate
+// assumes about one of them per SIMD).
+constexpr int64_t kWideMaxGroups = 32;
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
@@ -321,10 +329,13 @@ int build_schedule(mcdp_propagator &P, int32_t W, int32_t K) {
     P.step_off.assign(1, 0);
     P.ev_off.assign(1, 0);
     P.slot_list.clear();
+    P.ev_rec.clear();
     P.recs.clear();
     P.orph.clear();
     P.max_per_wave = 0;
     P.n_global = 0;
+    P.tile_cost = 0;
+    P.wide_cost = 0;
 
     auto event_cost = [&](int32_t e) {
         int c = kEventCost;
@@ -423,6 +434,7 @@ int build_schedule(mcdp_propagator &P, int32_t W, int32_t K) {
             h.i1 = slot_of[(size_t)e];
             h.i2 = (int32_t)(P.pred_off[(size_t)e + 1] - P.pred_off[(size_t)e]);
             h.info = kInfoHeader | (h.i2 == 0 ? kInfoLast : 0u);
+            P.ev_rec.push_back((int32_t)P.recs.size());
             P.recs.push_back(h);
             for (int64_t k = P.pred_off[(size_t)e]; k < P.pred_off[(size_t)e + 1]; ++k) {
                 const PredEntry &pe = P.preds[(size_t)k];
@@ -444,8 +456,24 @@ int build_schedule(mcdp_propagator &P, int32_t W, int32_t K) {
         }
         P.step_off.push_back((int32_t)P.recs.size());
     }
-    P.recs.push_back(Record{});  // padding: the kernel prefetches one record ahead
+    // padding: the walk kernel prefetches one record ahead, a wide step loads
+    // kWideBatch predecessor records at a time
+    for (int i = 0; i < kWideBatch; ++i) P.recs.push_back(Record{});
     if (P.recs.size() > (size_t)std::numeric_limits<int32_t>::max()) return fail(MCDP_E_INVALID, "plan too large");
+    for (int32_t t = 0; t < P.n_steps; ++t) {
+        int64_t busiest = 0, costliest = 0;
+        for (int32_t w = 0; w < W; ++w) {
+            int64_t c = 0;
+            for (int32_t i = P.ev_off[(size_t)t * W + w]; i < P.ev_off[(size_t)t * W + w + 1]; ++i) {
+                const int64_t ce = event_cost(flat_events[(size_t)i]);
+                c += ce;
+                costliest = std::max(costliest, ce);
+            }
+            busiest = std::max(busiest, c);
+        }
+        P.tile_cost += busiest;
+        P.wide_cost += kWideStepCost + costliest;
+    }
     for (int32_t l = 0; l < L; ++l)
         if (!referenced[(size_t)l]) P.orph.push_back(edge_record(P, l));
     return MCDP_OK;
@@ -624,6 +652,7 @@ int ensure_device_impl(mcdp_propagator &P, DeviceCopy *&out) {
     const size_t o_dist = reserve(P.dist.size() * sizeof(DistParams));
     const size_t o_alias = reserve(P.alias.size() * sizeof(AliasRec));
     const size_t o_earl = reserve(P.earliest.size() * sizeof(double));
+    const size_t o_evrec = reserve(P.ev_rec.size() * sizeof(int32_t));
     const size_t total = std::max<size_t>(off, a);
     std::vector<unsigned char> host(total, 0);
     auto put = [&](size_t o, const void *src, size_t bytes) {
@@ -637,6 +666,7 @@ int ensure_device_impl(mcdp_propagator &P, DeviceCopy *&out) {
     put(o_dist, P.dist.data(), P.dist.size() * sizeof(DistParams));
     put(o_alias, P.alias.data(), P.alias.size() * sizeof(AliasRec));
     put(o_earl, P.earliest.data(), P.earliest.size() * sizeof(double));
+    put(o_evrec, P.ev_rec.data(), P.ev_rec.size() * sizeof(int32_t));
     void *blob = nullptr;
     hipError_t e = hipMalloc(&blob, total);
     if (e != hipSuccess) return fail(MCDP_E_NOMEM, std::string("hipMalloc (plan): ") + hipGetErrorString(e));
@@ -657,6 +687,7 @@ int ensure_device_impl(mcdp_propagator &P, DeviceCopy *&out) {
     d.base.dists = reinterpret_cast<const DistParams *>(b + o_dist);
     d.base.alias = reinterpret_cast<const AliasRec *>(b + o_alias);
     d.base.earliest = reinterpret_cast<const double *>(b + o_earl);
+    d.base.ev_rec = reinterpret_cast<const int32_t *>(b + o_evrec);
     d.base.n_steps = P.n_steps;
     d.base.n_orphans = (int32_t)P.orph.size();
     d.base.waves = P.waves;
@@ -789,25 +820,40 @@ int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double
     const int64_t tile = (int64_t)kLanes * v.spl;
 
     // A tile's run time hardly depends on how many of its lanes are live, so a
-    // last round of only a few tiles costs almost a full round.  With two
-    // samples per lane, the samples past the last full round go to the
-    // one-sample-per-lane kernel of the same schedule when they fit in one of
-    // its rounds: its tiles finish sooner.  Results do not depend on the split.
+    // last round of only a few tiles costs almost a full round.  The samples
+    // past the last full round take one of two cheaper routes:
+    //   * wide steps: one launch per step, each event of the step a wave of
+    //     its own for every group of 64 samples, predecessors read from the
+    //     output rows; the tail's latency is then about the steps' launch gaps
+    //     plus their costliest events, instead of a whole tile's walk;
+    //   * with two samples per lane, the one-sample-per-lane kernel of the
+    //     same schedule, whose tiles finish sooner.
+    // The estimates of both latencies (from the schedule) pick the route.
+    // Results do not depend on the split or the route.
     int64_t n_main = n, n_tail = 0;
+    bool wide = false;
     WalkVariant v1 = v;
     v1.spl = 1;
     const size_t lds1 = walk_lds_bytes(P, v1);
     int64_t resident1 = 0;
-    if (v.spl == 2) {
+    {
         const int64_t full_rounds = ((n + tile - 1) / tile) / resident;
         const int64_t rest = n - full_rounds * resident * tile;
+        const int64_t groups = (rest + kLanes - 1) / kLanes;
         if (full_rounds >= 1 && rest > 0) {
-            int bpc1 = 0;
-            HIP_TRY(walk_occupancy(v1, P.waves, lds1, &bpc1), "hipOccupancyMaxActiveBlocksPerMultiprocessor");
-            resident1 = (int64_t)cus * bpc1;
-            if (bpc1 >= 1 && (rest + kLanes - 1) / kLanes <= resident1) {
+            if (P.tail_mode == 2 ||
+                (P.tail_mode == 0 && groups <= kWideMaxGroups && P.wide_cost < P.tile_cost)) {
+                wide = true;
                 n_main = n - rest;
                 n_tail = rest;
+            } else if (v.spl == 2) {
+                int bpc1 = 0;
+                HIP_TRY(walk_occupancy(v1, P.waves, lds1, &bpc1), "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+                resident1 = (int64_t)cus * bpc1;
+                if (bpc1 >= 1 && groups <= resident1) {
+                    n_main = n - rest;
+                    n_tail = rest;
+                }
             }
         }
     }
@@ -823,7 +869,17 @@ int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double
         q.cause = cause ? cause + n_main : nullptr;
         q.n = (int32_t)n_tail;
         q.n_tiles = (int32_t)((n_tail + kLanes - 1) / kLanes);
-        HIP_TRY(launch_walk(q, v1, (int)std::min<int64_t>(q.n_tiles, resident1), lds1, stream), "walk kernel launch");
+        if (wide) {
+            const int32_t groups = q.n_tiles;
+            const int32_t W = P.waves;
+            for (int32_t t = 0; t < P.n_steps; ++t) {
+                const int32_t e0 = P.ev_off[(size_t)t * W], e1 = P.ev_off[(size_t)(t + 1) * W];
+                HIP_TRY(launch_walk_wide(q, injected, e0, e1 - e0, groups, stream), "wide step launch");
+            }
+            if (!injected) HIP_TRY(launch_walk_wide_orphans(q, groups, stream), "wide orphans launch");
+        } else {
+            HIP_TRY(launch_walk(q, v1, (int)std::min<int64_t>(q.n_tiles, resident1), lds1, stream), "walk kernel launch");
+        }
     }
     return MCDP_OK;
 }
@@ -1038,7 +1094,7 @@ int mcdp_propagate_host(mcdp_propagator *prop, const double *durations, int64_t 
 
 int mcdp_propagator_plan_info(const mcdp_propagator *prop, int64_t *out, int n_out) {
     if (!prop || !out) return fail(MCDP_E_INVALID, "null argument");
-    const int64_t v[10] = {prop->n_steps,
+    const int64_t v[12] = {prop->n_steps,
                           prop->waves,
                           prop->slots,
                           (int64_t)prop->slots * kLanes * prop->spl * (int64_t)sizeof(double),
@@ -1047,8 +1103,10 @@ int mcdp_propagator_plan_info(const mcdp_propagator *prop, int64_t *out, int n_o
                           (int64_t)prop->orph.size(),
                           prop->max_per_wave,
                           prop->k,
-                          prop->spl};
-    for (int i = 0; i < n_out && i < 10; ++i) out[i] = v[i];
+                          prop->spl,
+                          prop->tile_cost,
+                          prop->wide_cost};
+    for (int i = 0; i < n_out && i < 12; ++i) out[i] = v[i];
     return MCDP_OK;
 }
 
@@ -1137,6 +1195,14 @@ int mcdp_propagator_autotune(mcdp_propagator *prop, int64_t n_samples, int32_t *
     if (events_per_wave) *events_per_wave = best.k;
     if (samples_per_lane) *samples_per_lane = best.spl;
     if (ms) *ms = best_ms;
+    return MCDP_OK;
+}
+
+int mcdp_debug_tail_mode(mcdp_propagator *prop, int32_t mode) {
+    if (!prop) return fail(MCDP_E_INVALID, "propagator is null");
+    if (mode < 0 || mode > 2) return fail(MCDP_E_INVALID, "tail mode must be 0, 1 or 2");
+    std::lock_guard<std::mutex> lock(prop->mu);
+    prop->tail_mode = mode;
     return MCDP_OK;
 }
 

@@ -92,7 +92,13 @@ struct mcdp_propagator {
     int64_t n_stale = 0;
     // schedule (for `waves` waves per workgroup, at most `k` events per wave and step)
     std::vector<int32_t> step_off, ev_off, slot_list;
+    std::vector<int32_t> ev_rec;  // header record of each event, in slot_list order
     std::vector<mcdp::Record> recs, orph;
+    // Estimated latency of the last partial round, in draw-cost units: as
+    // tiles (sum over steps of the busiest wave's events) and as wide steps
+    // (sum over steps of the costliest event plus the launch gap).
+    int64_t tile_cost = 0, wide_cost = 0;
+    int32_t tail_mode = 0;  // test hook: 0 by the estimates, 1 tiles, 2 wide steps
     int32_t n_steps = 0, waves = 1, slots = 0, max_per_wave = 0, k = mcdp::kMaxEventsPerWave;
     int32_t spl = 1;  // samples per lane of the walk kernel (schedule-independent)
     int64_t n_global = 0;

@@ -267,6 +267,89 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_batch_kernel(WalkBatc
     }
 }
 
+// One step of the walk for a few samples, spread over the device instead of
+// over one workgroup's waves: workgroup (i, g) is one wave evaluating event
+// e0 + i of the schedule for lane group g (samples 64 g + lane).  A
+// predecessor's realized time comes from its output row, written by the
+// launch of an earlier step (or is its earliest time, when stale), so there
+// are no slots and no barriers; the stream orders the steps.  Per sample the
+// arithmetic is walk_tile's: the same records, draws and operation order.
+template <bool kInjected>
+__global__ void __launch_bounds__(kLanes) walk_wide_kernel(WalkParams p, int32_t e0) {
+    const int32_t si = (int32_t)blockIdx.y * kLanes + (int32_t)threadIdx.x;
+    const int32_t s = si < p.n ? si : p.n - 1;
+    const uint32_t ld = (uint32_t)p.ld;
+    const auto recs = as_const_space(p.recs);
+    const auto earliest = as_const_space(p.earliest);
+    const LogTable lt = global_log_table();
+    const uint32_t seed[1] = {kInjected ? 0u : (uint32_t)p.seeds[s]};
+    const int32_t r0 = as_const_space(p.ev_rec)[e0 + (int32_t)blockIdx.x];
+    // The header and the first batch of records load together.
+    const Record h = recs[r0];
+    Record c[kWideBatch];
+#pragma unroll
+    for (int u = 0; u < kWideBatch; ++u) c[u] = recs[r0 + 1 + u];
+    const double upper = h.b;
+    double best = h.a;
+    int32_t who = -1;
+    // Predecessors in batches: the batch's records (scalar loads), then its
+    // predecessor times and injected durations (vector loads), all in flight
+    // together; then the draws and the max-plus steps in input order.  The
+    // record array ends in kWideBatch padding records, so a batch never
+    // reads past it.
+    for (int32_t q0 = 0; q0 < h.i2; q0 += kWideBatch) {
+        double rs[kWideBatch], dur[kWideBatch];
+        if (q0 > 0) {
+#pragma unroll
+            for (int u = 0; u < kWideBatch; ++u) c[u] = recs[r0 + 1 + q0 + u];
+        }
+#pragma unroll
+        for (int u = 0; u < kWideBatch; ++u) {
+            rs[u] = 0.0;
+            dur[u] = 0.0;
+            if (q0 + u < h.i2) {
+                rs[u] = c[u].i1 == kSrcStale ? earliest[c[u].i0] : p.realized[row_offset(c[u].i0, ld) + s];
+                if (kInjected && (c[u].info & kInfoEdgeValid)) dur[u] = p.dur_in[row_offset(c[u].i2, ld) + s];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kWideBatch; ++u) {
+            if (q0 + u >= h.i2) break;
+            if (!kInjected) {
+                double d1[1];
+                draw_durations<1>(c[u].info, c[u].a, c[u].b, (uint32_t)c[u].i2, seed, as_const_space(p.alias),
+                                  as_const_space(p.dists), lt, d1);
+                dur[u] = d1[0];
+                if (p.durations && (c[u].info & kInfoOwner))
+                    __builtin_nontemporal_store(dur[u], p.durations + row_offset(c[u].i2, ld) + s);
+            }
+            double tt = rs[u] + dur[u];
+            tt = (upper < tt) ? upper : tt;
+            if (tt >= best) {
+                best = tt;
+                who = c[u].i0;
+            }
+        }
+    }
+    // realized is read by later steps: an ordinary store; cause is not
+    const uint64_t row = row_offset(h.i0, ld);
+    p.realized[row + s] = (upper < best) ? upper : best;
+    if (p.cause) __builtin_nontemporal_store(who, p.cause + row + s);
+}
+
+// Durations of the activities no predecessor entry references, for the wide
+// steps' samples: workgroup (o, g) draws orphan o for lane group g.
+__global__ void __launch_bounds__(kLanes) walk_wide_orphans_kernel(WalkParams p) {
+    const int32_t si = (int32_t)blockIdx.y * kLanes + (int32_t)threadIdx.x;
+    const int32_t s = si < p.n ? si : p.n - 1;
+    const Record rec = as_const_space(p.orph)[blockIdx.x];
+    const uint32_t seed[1] = {(uint32_t)p.seeds[s]};
+    double dur[1];
+    draw_durations<1>(rec.info, rec.a, rec.b, (uint32_t)rec.i2, seed, as_const_space(p.alias), as_const_space(p.dists),
+                      global_log_table(), dur);
+    p.durations[row_offset(rec.i2, (uint32_t)p.ld) + s] = dur[0];
+}
+
 __global__ void reset_counter_kernel(int32_t *counter) { *counter = 0; }
 
 __global__ void philox_debug_kernel(int variant, int64_t n, const uint32_t *__restrict__ ctr,
@@ -430,6 +513,25 @@ hipError_t walk_occupancy(WalkVariant v, int waves, size_t lds_bytes, int *block
         *blocks_per_cu = std::min(*blocks_per_cu, per_simd / per_wg);
     }
     return hipSuccess;
+}
+
+hipError_t launch_walk_wide(const WalkParams &p, bool injected, int32_t e0, int32_t n_events, int32_t groups,
+                            hipStream_t stream) {
+    if (n_events <= 0 || groups <= 0) return hipSuccess;
+    if (injected)
+        hipLaunchKernelGGL(walk_wide_kernel<true>, dim3((unsigned)n_events, (unsigned)groups), dim3(kLanes), 0, stream,
+                           p, e0);
+    else
+        hipLaunchKernelGGL(walk_wide_kernel<false>, dim3((unsigned)n_events, (unsigned)groups), dim3(kLanes), 0, stream,
+                           p, e0);
+    return hipGetLastError();
+}
+
+hipError_t launch_walk_wide_orphans(const WalkParams &p, int32_t groups, hipStream_t stream) {
+    if (p.n_orphans <= 0 || groups <= 0 || !p.durations || !p.seeds) return hipSuccess;
+    hipLaunchKernelGGL(walk_wide_orphans_kernel, dim3((unsigned)p.n_orphans, (unsigned)groups), dim3(kLanes), 0, stream,
+                       p);
+    return hipGetLastError();
 }
 
 hipError_t launch_philox_debug(int variant, int64_t n, const uint32_t *ctr, const uint32_t *key, uint32_t *out,

@@ -43,6 +43,7 @@ struct WalkParams {
     const DistParams *dists;   // compiled distributions (p1..p4 of the rarer kinds)
     const AliasRec *alias;
     const double *earliest;    // [E]: realized value of a stale predecessor
+    const int32_t *ev_rec;     // header record of each event, in slot_list order (wide steps)
     int32_t n_steps;
     int32_t n_orphans;
     int32_t waves;
@@ -79,6 +80,16 @@ hipError_t walk_occupancy(WalkVariant v, int waves, size_t lds_bytes, int *block
 hipError_t launch_walk_batch(const WalkBatchParams &b, WalkVariant v, int waves, int grid, size_t lds_bytes,
                              hipStream_t stream);
 hipError_t walk_batch_occupancy(WalkVariant v, int waves, size_t lds_bytes, int *blocks_per_cu);
+// Predecessor records a wide-step wave loads together; the record array ends
+// in this many padding records.
+constexpr int kWideBatch = 4;
+
+// One step of the walk spread over the device (few samples): events e0 ..
+// e0 + n_events - 1 of the schedule, `groups` groups of 64 samples; then the
+// orphan activities.  Steps are separated by stream order.
+hipError_t launch_walk_wide(const WalkParams &p, bool injected, int32_t e0, int32_t n_events, int32_t groups,
+                            hipStream_t stream);
+hipError_t launch_walk_wide_orphans(const WalkParams &p, int32_t groups, hipStream_t stream);
 hipError_t launch_copy(double *dst, const double *src, int64_t n, hipStream_t stream);
 hipError_t launch_math_debug(int fn, int64_t n, const double *x, double *out, hipStream_t stream);
 hipError_t launch_philox_debug(int variant, int64_t n, const uint32_t *ctr, const uint32_t *key, uint32_t *out,

@@ -26,13 +26,17 @@ __device__ __forceinline__ double fma_k(double p, double z, double c) {
 #endif
 }
 
-// The breakpoint table of log_tab, staged in LDS by every kernel that samples
-// (the lookups are per lane).  kLogInvF, kLogHi and kLogLo come from
-// mcdp_log_table.hpp, written by gen_log_table.py.
+// The breakpoint table of log_tab (the lookups are per lane): kLogInvF and
+// kLogHiLo from mcdp_log_table.hpp, written by gen_log_table.py.  Kernels
+// that sample many activities per workgroup stage it in LDS; the others read
+// it where it lies (global_log_table).
 struct LogTable {
     const double *inv;    // [kLogTableSize]: 1/F_j rounded
     const double2 *hilo;  // [kLogTableSize]: log F_j = hi + lo, hi a multiple of 2^-32
 };
+__device__ __forceinline__ LogTable global_log_table() {
+    return LogTable{kLogInvF, reinterpret_cast<const double2 *>(kLogHiLo)};
+}
 struct LogTableLds {
     double inv[kLogTableSize];
     double2 hilo[kLogTableSize];
@@ -43,7 +47,7 @@ struct LogTableLds {
 __device__ __forceinline__ void stage_log_table(LogTableLds &t) {
     for (int i = threadIdx.x; i < kLogTableSize; i += blockDim.x) {
         t.inv[i] = kLogInvF[i];
-        t.hilo[i] = double2{kLogHi[i], kLogLo[i]};
+        t.hilo[i] = double2{kLogHiLo[2 * i], kLogHiLo[2 * i + 1]};
     }
 }
 

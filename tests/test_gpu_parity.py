@@ -293,3 +293,44 @@ def test_tail_split_matches(gpu) -> None:
     r0, c0 = oracle.OracleCtx(dag).propagate(db[pick])
     np.testing.assert_array_equal(bits(r0), bits(rb[pick]))
     np.testing.assert_array_equal(c0, cb[pick])
+
+
+PARTIAL_ROUND_DAGS = {
+    "layered_small": lambda: layered_small(),
+    "edge_cases": lambda: D.continuous_dag(7, 60, max_delay=12.0, shared_links=True),
+}
+
+
+@pytest.mark.parametrize("name", sorted(PARTIAL_ROUND_DAGS))
+def test_partial_round_routes_match(gpu, name: str) -> None:
+    """The samples past the last full round of tiles run as one-sample-per-lane tiles (route 1) or as wide
+    steps, one launch per step with every event a wave of its own (route 2); both, and the automatic choice,
+    give the bits of a run with no split, and the oracle agrees on the last samples.  The edge-case DAG has
+    stale reads, repeated targets and sources, shared links, gaps and edges outside the link table."""
+    dag = PARTIAL_ROUND_DAGS[name]()
+    n = 256 * 128 + 77  # one full round of 256 resident tiles of 128 on a 256-CU device, plus 77 samples
+    seeds = np.arange(n, dtype=np.int32) * 3 - 5000
+    A = capi.Propagator(dag)
+    A.configure(16, 8, 1)
+    ra, da, ca = A.run_many(seeds)
+    for mode in (1, 2, 0):
+        B = capi.Propagator(dag)
+        B.configure(16, 8, 2)
+        capi.check(capi.lib().mcdp_debug_tail_mode(B.h, mode))
+        rb, db, cb = B.run_many(seeds)
+        np.testing.assert_array_equal(bits(rb), bits(ra), err_msg=f"route {mode}")
+        np.testing.assert_array_equal(bits(db), bits(da), err_msg=f"route {mode}")
+        np.testing.assert_array_equal(cb, ca, err_msg=f"route {mode}")
+        rr, cc = B.propagate(da)
+        np.testing.assert_array_equal(bits(rr), bits(ra), err_msg=f"route {mode} injected")
+        np.testing.assert_array_equal(cc, ca, err_msg=f"route {mode} injected")
+    pick = np.r_[0:64, n - 200:n]
+    r0, c0 = oracle.OracleCtx(dag).propagate(da[pick])
+    np.testing.assert_array_equal(bits(r0), bits(ra[pick]))
+    np.testing.assert_array_equal(c0, ca[pick])
+
+
+def test_tail_mode_rejects_unknown_routes(gpu) -> None:
+    P = capi.Propagator(layered_small())
+    assert capi.lib().mcdp_debug_tail_mode(P.h, 3) != 0
+    assert capi.lib().mcdp_debug_tail_mode(P.h, -1) != 0
