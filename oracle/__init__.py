@@ -364,7 +364,11 @@ def event_stats(realized: np.ndarray, cause: np.ndarray | None, preds: list[list
         else:
             mean[e] = plain / n
     q = np.asarray(q, dtype=np.float64)
-    quant = np.quantile(realized, q, axis=0).T.copy() if q.size else np.empty((E, 0))
+    # realized may hold +-inf (infinite durations or max_delay): numpy's linear interpolation
+    # between two equal infinities evaluates inf - inf and warns; the device follows the same
+    # interpolation formula, so both sides give the same value there
+    with np.errstate(invalid="ignore"):
+        quant = np.quantile(realized, q, axis=0).T.copy() if q.size else np.empty((E, 0))
     counts = None
     if cause is not None:
         cause = np.asarray(cause)
