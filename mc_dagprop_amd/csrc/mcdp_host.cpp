@@ -859,6 +859,20 @@ int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double
     }
     p.n = (int32_t)n_main;
     p.n_tiles = (int32_t)((n_main + tile - 1) / tile);
+    // The wide steps need no LDS and fewer registers than a tile leaves free
+    // on a SIMD, so they run on a second stream beside the tiles, forked from
+    // the caller's stream before the tiles and joined after.
+    hipStream_t tail_stream = stream;
+    if (wide) {
+        if (!d->side) {
+            HIP_TRY(hipStreamCreateWithFlags(&d->side, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+            HIP_TRY(hipEventCreateWithFlags(&d->fork, hipEventDisableTiming), "hipEventCreateWithFlags");
+            HIP_TRY(hipEventCreateWithFlags(&d->join, hipEventDisableTiming), "hipEventCreateWithFlags");
+        }
+        HIP_TRY(hipEventRecord(d->fork, stream), "hipEventRecord");
+        HIP_TRY(hipStreamWaitEvent(d->side, d->fork, 0), "hipStreamWaitEvent");
+        tail_stream = d->side;
+    }
     HIP_TRY(launch_walk(p, v, (int)std::min<int64_t>(p.n_tiles, resident), lds, stream), "walk kernel launch");
     if (n_tail > 0) {
         WalkParams q = p;
@@ -874,9 +888,11 @@ int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double
             const int32_t W = P.waves;
             for (int32_t t = 0; t < P.n_steps; ++t) {
                 const int32_t e0 = P.ev_off[(size_t)t * W], e1 = P.ev_off[(size_t)(t + 1) * W];
-                HIP_TRY(launch_walk_wide(q, injected, e0, e1 - e0, groups, stream), "wide step launch");
+                HIP_TRY(launch_walk_wide(q, injected, e0, e1 - e0, groups, tail_stream), "wide step launch");
             }
-            if (!injected) HIP_TRY(launch_walk_wide_orphans(q, groups, stream), "wide orphans launch");
+            if (!injected) HIP_TRY(launch_walk_wide_orphans(q, groups, tail_stream), "wide orphans launch");
+            HIP_TRY(hipEventRecord(d->join, tail_stream), "hipEventRecord");
+            HIP_TRY(hipStreamWaitEvent(stream, d->join, 0), "hipStreamWaitEvent");
         } else {
             HIP_TRY(launch_walk(q, v1, (int)std::min<int64_t>(q.n_tiles, resident1), lds1, stream), "walk kernel launch");
         }

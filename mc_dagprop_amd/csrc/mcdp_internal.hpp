@@ -52,6 +52,11 @@ struct DeviceCopy {
     const Record *rev_unit_links = nullptr;
     void *rev_work = nullptr;            // partial rows and global slots
     size_t rev_work_bytes = 0;
+    // A second stream for the wide steps of a last partial round, which run
+    // beside the tiles of the full rounds (created on first use), and the
+    // events that fork it from and join it to the caller's stream.
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 // The reverse walk's plan (mcdp_grad.cpp): the forward levels in reverse
@@ -119,6 +124,9 @@ struct mcdp_propagator {
             if (d.topo) (void)hipFree(d.topo);
             if (d.rev) (void)hipFree(d.rev);
             if (d.rev_work) (void)hipFree(d.rev_work);
+            if (d.side) (void)hipStreamDestroy(d.side);
+            if (d.fork) (void)hipEventDestroy(d.fork);
+            if (d.join) (void)hipEventDestroy(d.join);
             if (cur >= 0) (void)hipSetDevice(cur);
         }
         dev.clear();
