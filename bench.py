@@ -10,9 +10,10 @@ Default workload (the network-scale configuration): 200 layers x 100 events =
 samples / seconds per step.
 
 Also reported:
-  roofline      the walk kernel's algorithmic HBM bytes per launch
-                (samples x (8 L + 12 E): durations f64, realized f64, cause i32)
-                over its average launch time from HIP events on its stream,
+  roofline      the walk's algorithmic HBM bytes per launch (samples x (8 L +
+                12 E): durations f64, realized f64, cause i32) over the average
+                time of one run_many (the walk kernel, and beside it the wide
+                steps of a last partial round) from HIP events on its stream,
                 against MI355X's 8 TB/s; `traffic` is the rocprofv3-measured HBM
                 bytes per launch when profiles/walk_network_traffic.txt exists;
   cpu_baseline  the reference engine (compiled from its sources into oracle/_ref)
@@ -27,7 +28,9 @@ Also reported:
 network DAG with 65,536 samples, with the reverse walk's roofline and a check of its
 gradient against the FP64 CPU oracle.  --workload serve runs a Poisson stream of
 statistics requests over 200 resident contexts through the serving loop and reports
-p50/p99 latency beside the aggregate throughput.
+p50/p99 latency beside the aggregate throughput.  --workload batch runs 64 DAGs of
+1,000-50,000 events and 4,096-65,536 samples with per-event statistics reduced on the
+device, batched, against one DAG at a time.
 """
 
 from __future__ import annotations
@@ -89,52 +92,70 @@ def cpu_baseline(dag, L: int, threads: int, per_thread: int, min_seconds: float 
 
 
 def bench_batch(args) -> int:
-    """Several DAGs of uneven size and sample count: one batched launch per step against one launch
-    per DAG per step (each DAG with its own default schedule), full outputs in HBM."""
+    """Config 2: 64 layered DAGs, events log-uniform in [1,000, 50,000], samples log-uniform in [4,096,
+    65,536], each DAG with its own seed-generated distribution parameters, per-event mean and 50/90/99%
+    quantiles (and cause counts) reduced on the device (mcdp_batch_stats_host: DAGs in groups that fit the
+    memory budget, one batched launch per group).  Against one DAG at a time (mcdp_run_stats_host each)."""
     from mc_dagprop_amd import capi
-    from mc_dagprop_amd.workloads import SplitMix64, layered_network
+    from mc_dagprop_amd.workloads import SplitMix64, layered_vectorized, seeded_network_dists
 
     rng = SplitMix64(20260002)
     dags, ns = [], []
-    for _ in range(16):
-        events = int(round(1000.0 * 10.0 ** rng.uniform(0.0, 1.0), -2))
-        dags.append(layered_network(layers=events // 100))
-        ns.append(int(4096.0 * 8.0 ** rng.uniform(0.0, 1.0)))
+    for i in range(args.batch_dags):
+        events = int(round(1000.0 * 50.0 ** rng.uniform(0.0, 1.0), -2))
+        dags.append(layered_vectorized(events // 100, 100, 4, seeded_network_dists(i), 8, 20260002 + i))
+        ns.append(int(4096.0 * 16.0 ** rng.uniform(0.0, 1.0)))
     props = [capi.Propagator(d) for d in dags]
     solo = [capi.Propagator(d) for d in dags]
+    E_all = sum(P.E for P in props)
     work = sum(P.L * n for P, n in zip(props, ns))
-    bufs = [(capi.DeviceBuffer.from_array(np.arange(n, dtype=np.int32)), capi.DeviceBuffer(8 * P.E * n),
-             capi.DeviceBuffer(8 * P.L * n), capi.DeviceBuffer(4 * P.E * n)) for P, n in zip(props, ns)]
+    event_samples = sum(P.E * n for P, n in zip(props, ns))
+    seeds = [np.arange(n, dtype=np.int32) for n in ns]
+    q = (0.5, 0.9, 0.99)
+    budget = int(args.budget_gb * (1 << 30))
     batch = capi.Batch(props)
-    stream = capi.C.c_void_p(0)
-    capi.check(capi.lib().mcdp_stream_create(capi.C.byref(stream)))
 
-    def step_batch() -> None:
-        batch.run_many_device([b[0] for b in bufs], ns, ns, [b[1] for b in bufs], [b[2] for b in bufs],
-                              [b[3] for b in bufs], stream)
+    def step_batch():
+        return batch.stats(seeds, q, memory_budget=budget)
 
-    def step_solo() -> None:
-        for P, n, b in zip(solo, ns, bufs):
-            P.run_many_device(b[0], n, n, b[1], b[2], b[3], stream)
+    # one DAG at a time on buffers allocated once (the largest DAG's size): run_many, then the statistics
+    qa = np.ascontiguousarray(q, dtype=np.float64)
+    big = max(P.E * n for P, n in zip(solo, ns))
+    s_seeds = [capi.DeviceBuffer.from_array(sd) for sd in seeds]
+    s_real, s_cause = capi.DeviceBuffer(8 * big), capi.DeviceBuffer(4 * big)
+    e_max = max(P.E for P in solo)
+    offs = [P.pred_offsets() for P in solo]
+    c_max = max(int(o[-1]) + P.E for o, P in zip(offs, solo))
+    s_mean, s_quant, s_cnt = capi.DeviceBuffer(8 * e_max), capi.DeviceBuffer(8 * e_max * qa.size), \
+        capi.DeviceBuffer(8 * c_max)
 
-    def timed(fn) -> float:
+    def step_solo():
+        out = []
+        for P, sd, n, off in zip(solo, s_seeds, ns, offs):
+            P.run_many_device(sd, n, n, s_real, None, s_cause)
+            capi.check(capi.lib().mcdp_event_stats_device(P.h, s_real.ptr, s_cause.ptr, n, n, qa.size,
+                                                          qa.ctypes.data_as(capi._pf64), s_mean.ptr, s_quant.ptr,
+                                                          s_cnt.ptr, None))
+            capi.synchronize()
+            out.append((s_mean.to_array(np.float64, (P.E,)), s_quant.to_array(np.float64, (P.E, qa.size)),
+                        s_cnt.to_array(np.int64, (int(off[-1]) + P.E,))))
+        return out
+
+    def timed(fn):
         for _ in range(args.warmup):
             fn()
-        capi.synchronize(stream)
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            fn()
-        capi.synchronize(stream)
-        return 1000.0 * (time.perf_counter() - t0) / args.steps
+            out = fn()
+        return 1000.0 * (time.perf_counter() - t0) / args.steps, out
 
-    ms_batch = timed(step_batch)
-    ms_solo = timed(step_solo)
-    log(f"batch of {len(dags)} DAGs ({sum(P.E for P in props)} events, {work:.3e} activity-samples): "
-        f"{ms_batch:.3f} ms batched, {ms_solo:.3f} ms one launch per DAG")
-    # bit-exact spot check: first DAG's first samples against a single run
-    r0, d0, c0 = solo[0].run_many(np.arange(8))
-    R = bufs[0][1].columns(np.float64, props[0].E, ns[0], 0, 8)
-    ok = bool(np.array_equal(np.ascontiguousarray(R).view(np.uint64), r0.view(np.uint64)))
+    ms_batch, res_b = timed(step_batch)
+    ms_solo, res_s = timed(step_solo)
+    ok = all(np.array_equal(np.ascontiguousarray(mb).view(np.uint64), np.ascontiguousarray(ms).view(np.uint64))
+             and np.array_equal(np.ascontiguousarray(qb).view(np.uint64), np.ascontiguousarray(qs).view(np.uint64))
+             and np.array_equal(cb, cs) for (mb, qb, cb), (ms, qs, cs) in zip(res_b, res_s))
+    log(f"batch of {len(dags)} DAGs ({E_all} events, {work:.3e} activity-samples): {ms_batch:.1f} ms batched "
+        f"statistics, {ms_solo:.1f} ms one DAG at a time; results identical: {ok}")
     out = {
         "metric": "activity-samples/sec GPU vs host run_many (bit-exact); p50/p99 solve latency",
         "value": work / (ms_batch / 1000.0),
@@ -148,13 +169,17 @@ def bench_batch(args) -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (deterministic SplitMix64 layered DAGs; seeds 0..S-1 per DAG)",
-        "config": {"workload": "16 layered DAGs (width 100, 8 activity types), 1,000-10,000 events and 4,096-32,768 "
-                               "samples each (log-uniform), full outputs in HBM",
-                   "events": [P.E for P in props], "samples": ns},
-        "one_launch_per_dag_ms": ms_solo,
-        "speedup_vs_one_launch_per_dag": ms_solo / ms_batch,
-        "check": {"bit_exact": ok},
+        "data": "synthetic (layered DAGs from numpy PCG64, per-DAG distribution parameters from SplitMix64; "
+                "seeds 0..S-1 per DAG)",
+        "config": {"workload": f"{len(dags)} layered DAGs (width 100, 1-4 preds, 8 activity types with per-DAG "
+                               "parameters), 1,000-50,000 events and 4,096-65,536 samples each (log-uniform); "
+                               "per-event mean, 50/90/99% quantiles and cause counts reduced on the device, "
+                               f"DAGs grouped to fit {args.budget_gb:g} GiB",
+                   "dags": len(dags), "events": [P.E for P in props], "samples": ns,
+                   "event_samples": event_samples, "activity_samples": work},
+        "one_dag_at_a_time_ms": ms_solo,
+        "speedup_vs_one_dag_at_a_time": ms_solo / ms_batch,
+        "check": {"identical_to_one_dag_at_a_time": ok},
     }
     print(json.dumps(out), flush=True)
     return 0 if ok else 3
@@ -405,7 +430,9 @@ def main() -> int:
     ap.add_argument("--contexts", type=int, default=200, help="serve: resident contexts")
     ap.add_argument("--requests", type=int, default=300, help="serve: requests in the stream")
     ap.add_argument("--rate", type=float, default=12.0, help="serve: Poisson arrival rate (requests/s)")
-    ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: device memory pool (GiB)")
+    ap.add_argument("--budget-gb", type=float, default=200.0,
+                    help="serve: device memory pool; batch: device memory per group of DAGs (GiB)")
+    ap.add_argument("--batch-dags", type=int, default=64, help="batch: number of DAGs")
     ap.add_argument("--check-samples", type=int, default=64)
     ap.add_argument("--mode", choices=["run_many", "propagate"], default="run_many",
                     help="run_many: sample + propagate (the metric); propagate: propagation only, durations "

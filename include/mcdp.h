@@ -161,6 +161,21 @@ void mcdp_batch_destroy(mcdp_batch *batch);
 int mcdp_batch_run_device(mcdp_batch *batch, const int32_t *const *seeds_dev, const int64_t *n, const int64_t *ld,
                           double *const *realized_dev, double *const *durations_dev, int32_t *const *cause_dev,
                           void *stream);
+/* Per-event statistics (as mcdp_event_stats_device defines them) of every DAG
+ * p of the batch over its own seeds, without keeping per-sample results: the
+ * DAGs run in groups, in batch order, whose seeds, realized and cause rows and
+ * statistics fit `memory_budget` bytes of device memory; each group is one
+ * batched launch (the other DAGs get no samples) followed by the device
+ * statistics of its DAGs.  seeds[p] holds n[p] >= 1 host seeds; results go to
+ * host arrays mean[p] (E_p values), quantiles[p] (E_p * n_q; may be NULL when
+ * n_q == 0) and cause_counts[p] (off[E_p] + E_p; cause_counts or an entry may
+ * be NULL to skip them).  Durations are drawn but not stored.  Synchronous.
+ * The device workspace (at most memory_budget bytes) stays with the batch for
+ * its next calls until mcdp_batch_destroy.  Fails with MCDP_E_NOMEM if one DAG
+ * alone exceeds the budget. */
+int mcdp_batch_stats_host(mcdp_batch *batch, const int32_t *const *seeds, const int64_t *n, int32_t n_q,
+                          const double *q, int64_t memory_budget, double *const *mean, double *const *quantiles,
+                          int64_t *const *cause_counts);
 
 /* ---- Per-event statistics (no counterpart in the reference) -------------- */
 /* The reference returns one SimResult per seed and leaves statistics to its

@@ -74,6 +74,8 @@ _SIGNATURES = {
     "mcdp_batch_create": (C.c_int, [C.POINTER(_P), _i32, _i32, _i32, _i32, C.POINTER(_P)]),
     "mcdp_batch_destroy": (None, [_P]),
     "mcdp_batch_run_device": (C.c_int, [_P, C.POINTER(_P), _pi64, _pi64, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), _P]),
+    "mcdp_batch_stats_host": (C.c_int, [_P, C.POINTER(_P), _pi64, C.c_int32, _pf64, _i64, C.POINTER(_P), C.POINTER(_P),
+                                        C.POINTER(_P)]),
     "mcdp_propagator_set_times": (C.c_int, [_P, _pf64, _pf64]),
     "mcdp_propagator_get_times": (C.c_int, [_P, _pf64, _pf64]),
     "mcdp_reverse_device": (C.c_int, [_P, _P, _P, _P, _P, _P, _i64, _i64, C.c_int, _P, _P, _P, _P]),
@@ -458,6 +460,29 @@ class Batch:
             out.append((r.to_array(np.float64, (P.E, n)).T, d.to_array(np.float64, (P.L, n)).T,
                         c.to_array(np.int32, (P.E, n)).T))
         return out
+
+    def stats(self, seed_lists, q=(0.5, 0.9, 0.99), memory_budget: int = 64 << 30, cause_counts: bool = True) -> list:
+        """Per-event statistics of every DAG over its seeds (mcdp_batch_stats_host): DAGs in groups whose
+        device buffers fit memory_budget bytes, one batched launch per group.  [(mean (E,), quantiles
+        (E, n_q), counts or None)] per propagator."""
+        k = len(self.props)
+        seeds = [np.ascontiguousarray(sd, dtype=np.int32) for sd in seed_lists]
+        if len(seeds) != k:
+            raise ValueError(f"need {k} seed lists")
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        nn = np.array([sd.size for sd in seeds], dtype=np.int64)
+        means = [np.empty(P.E) for P in self.props]
+        quants = [np.empty((P.E, q.size)) for P in self.props]
+        counts = [np.empty(int(P.pred_offsets()[-1]) + P.E, dtype=np.int64) for P in self.props] if cause_counts \
+            else None
+
+        def ptrs(arrs):
+            return None if arrs is None else (_P * k)(*[a.ctypes.data_as(_P) for a in arrs])
+
+        check(lib().mcdp_batch_stats_host(self.h, ptrs(seeds), _ptr(nn, _pi64), q.size, _ptr(q, _pf64),
+                                          int(memory_budget), ptrs(means), ptrs(quants) if q.size else None,
+                                          ptrs(counts)))
+        return [(means[i], quants[i], counts[i] if counts is not None else None) for i in range(k)]
 
     def __del__(self) -> None:  # pragma: no cover
         if getattr(self, "h", None) and self.h.value and _lib is not None:

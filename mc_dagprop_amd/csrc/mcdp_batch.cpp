@@ -22,11 +22,15 @@ struct mcdp_batch {
     void *dev = nullptr;  // WalkParams[plans], item_dag[], item_tile[], counter
     size_t dev_bytes = 0;
     int device = -1;
+    // mcdp_batch_stats_host's workspace, kept for the next call (allocating
+    // hundreds of GB costs more than the launches that use them)
+    void *stats_work = nullptr;
+    size_t stats_bytes = 0;
+    int stats_device = -1;
     ~mcdp_batch() {
-        if (dev) {
-            (void)hipDeviceSynchronize();
-            (void)hipFree(dev);
-        }
+        if (dev || stats_work) (void)hipDeviceSynchronize();
+        if (dev) (void)hipFree(dev);
+        if (stats_work) (void)hipFree(stats_work);
     }
 };
 
@@ -172,6 +176,133 @@ int mcdp_batch_run_device(mcdp_batch *b, const int32_t *const *seeds_dev, const 
     // The host vectors above back the asynchronous copies.
     HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
     return MCDP_OK;
+}
+
+int mcdp_batch_stats_host(mcdp_batch *b, const int32_t *const *seeds, const int64_t *n, int32_t n_q, const double *q,
+                          int64_t memory_budget, double *const *mean, double *const *quantiles,
+                          int64_t *const *cause_counts) {
+    if (!b) return fail(MCDP_E_INVALID, "batch is null");
+    if (!seeds || !n || !mean) return fail(MCDP_E_INVALID, "null argument");
+    if (n_q < 0 || (n_q > 0 && (!q || !quantiles))) return fail(MCDP_E_INVALID, "quantiles need q and outputs");
+    if (memory_budget <= 0) return fail(MCDP_E_INVALID, "memory_budget must be positive");
+    const size_t D = b->props.size();
+    const size_t a = 256;
+    // Device bytes of one DAG's part of a group: seeds, realized and cause
+    // rows (ld = n), and its statistics.
+    std::vector<size_t> bytes(D), o_seed(D), o_real(D), o_cause(D), o_mean(D), o_quant(D), o_cnt(D);
+    std::vector<int64_t> n_counts(D, 0);
+    for (size_t d = 0; d < D; ++d) {
+        const mcdp_propagator &P = *b->props[d];
+        if (n[d] < 1) return fail(MCDP_E_INVALID, "statistics need at least one sample per DAG");
+        if (!seeds[d] || !mean[d] || (n_q > 0 && !quantiles[d])) return fail(MCDP_E_INVALID, "null argument");
+        const bool counts = cause_counts && cause_counts[d];
+        n_counts[d] = counts ? P.pred_off.back() + P.E : 0;
+        const size_t E = (size_t)P.E, m = (size_t)n[d];
+        size_t off = 0;
+        auto take = [&](size_t nbytes) {
+            const size_t o = off;
+            off += align_to(std::max<size_t>(nbytes, 1), a);
+            return o;
+        };
+        o_seed[d] = take(4 * m);
+        o_real[d] = take(8 * E * m);
+        o_cause[d] = take(counts ? 4 * E * m : 0);
+        o_mean[d] = take(8 * E);
+        o_quant[d] = take(8 * E * (size_t)n_q);
+        o_cnt[d] = take(8 * (size_t)n_counts[d]);
+        bytes[d] = off;
+        if (bytes[d] > (size_t)memory_budget)
+            return fail(MCDP_E_NOMEM, "one DAG's realized and cause rows exceed memory_budget");
+    }
+    // Groups in batch order, each as large as the budget allows.
+    std::vector<size_t> group_start{0};
+    size_t max_group = 0;
+    for (size_t d = 0, used = 0; d < D; ++d) {
+        if (used + bytes[d] > (size_t)memory_budget) {
+            group_start.push_back(d);
+            used = 0;
+        }
+        used += bytes[d];
+        max_group = std::max(max_group, used);
+    }
+    group_start.push_back(D);
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return fail(MCDP_E_DEVICE, "no HIP device available: the Monte Carlo engine runs on the GPU only");
+    int device = 0;
+    HIP_TRY(hipGetDevice(&device), "hipGetDevice");
+    if (!b->stats_work || b->stats_bytes < max_group || b->stats_device != device) {
+        if (b->stats_work) {
+            (void)hipDeviceSynchronize();
+            (void)hipFree(b->stats_work);
+            b->stats_work = nullptr;
+            b->stats_bytes = 0;
+        }
+        hipError_t e = hipMalloc(&b->stats_work, max_group);
+        if (e != hipSuccess) {
+            b->stats_work = nullptr;
+            return fail(MCDP_E_NOMEM, std::string("hipMalloc (batch statistics): ") + hipGetErrorString(e));
+        }
+        b->stats_bytes = max_group;
+        b->stats_device = device;
+    }
+    hipStream_t s = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamCreateWithFlags");
+    auto *base = static_cast<unsigned char *>(b->stats_work);
+    int rc = MCDP_OK;
+    std::vector<const int32_t *> sd(D, nullptr);
+    std::vector<int64_t> gn(D, 0), gld(D, 1);
+    std::vector<unsigned char *> part(D, nullptr);  // a DAG's part of the workspace in its group
+    std::vector<double *> real(D, nullptr);
+    std::vector<int32_t *> cause(D, nullptr);
+    for (size_t g = 0; rc == MCDP_OK && g + 1 < group_start.size(); ++g) {
+        std::fill(gn.begin(), gn.end(), 0);
+        std::fill(gld.begin(), gld.end(), 1);
+        size_t at = 0;
+        for (size_t d = group_start[g]; d < group_start[g + 1]; ++d) {
+            unsigned char *p = part[d] = base + at;
+            sd[d] = reinterpret_cast<const int32_t *>(p + o_seed[d]);
+            real[d] = reinterpret_cast<double *>(p + o_real[d]);
+            cause[d] = n_counts[d] ? reinterpret_cast<int32_t *>(p + o_cause[d]) : nullptr;
+            gn[d] = n[d];
+            gld[d] = n[d];
+            e = hipMemcpyAsync(p + o_seed[d], seeds[d], 4 * (size_t)n[d], hipMemcpyHostToDevice, s);
+            if (e != hipSuccess) {
+                rc = hip_fail(e, "copy seeds");
+                break;
+            }
+            at += bytes[d];
+        }
+        if (rc) break;
+        rc = mcdp_batch_run_device(b, sd.data(), gn.data(), gld.data(), real.data(), nullptr, cause.data(), s);
+        for (size_t d = group_start[g]; rc == MCDP_OK && d < group_start[g + 1]; ++d) {
+            unsigned char *p = part[d];
+            auto *dm = reinterpret_cast<double *>(p + o_mean[d]);
+            auto *dq = n_q > 0 ? reinterpret_cast<double *>(p + o_quant[d]) : nullptr;
+            auto *dc = n_counts[d] ? reinterpret_cast<int64_t *>(p + o_cnt[d]) : nullptr;
+            rc = mcdp_event_stats_device(b->props[d], real[d], cause[d], n[d], n[d], n_q, q, dm, dq, dc, s);
+            const size_t E = (size_t)b->props[d]->E;
+            if (rc == MCDP_OK && (e = hipMemcpyAsync(mean[d], dm, 8 * E, hipMemcpyDeviceToHost, s)) != hipSuccess)
+                rc = hip_fail(e, "copy means");
+            if (rc == MCDP_OK && dq &&
+                (e = hipMemcpyAsync(quantiles[d], dq, 8 * E * (size_t)n_q, hipMemcpyDeviceToHost, s)) != hipSuccess)
+                rc = hip_fail(e, "copy quantiles");
+            if (rc == MCDP_OK && dc &&
+                (e = hipMemcpyAsync(cause_counts[d], dc, 8 * (size_t)n_counts[d], hipMemcpyDeviceToHost, s)) !=
+                    hipSuccess)
+                rc = hip_fail(e, "copy cause counts");
+        }
+        if (rc == MCDP_OK && (e = hipStreamSynchronize(s)) != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize");
+        for (size_t d = group_start[g]; d < group_start[g + 1]; ++d) {
+            sd[d] = nullptr;
+            real[d] = nullptr;
+            cause[d] = nullptr;
+        }
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+    return rc;
 }
 
 }  // extern "C"

@@ -279,6 +279,28 @@ def network_dists() -> list[DistSpec]:
     ]
 
 
+def seeded_network_dists(seed: int) -> list[DistSpec]:
+    """The network workload's eight activity types with parameters drawn from `seed` (one DAG of a batch):
+    constant factor U[1, 2]; exponential scales U[15, 60] and U[60, 240]; gamma (2, U[15, 60]) and
+    (0.5, U[100, 400]); empirical absolute and relative 64-bin histograms on [0, 900] with Dirichlet(1)
+    weights; type 7 has no distribution."""
+    rng = SplitMix64(20260002 + 7919 * seed)
+    c = 1.0 + rng.uniform()
+    e1, e2 = 15.0 + 45.0 * rng.uniform(), 60.0 + 180.0 * rng.uniform()
+    g1, g2 = 15.0 + 45.0 * rng.uniform(), 100.0 + 300.0 * rng.uniform()
+    v_abs, w_abs = _histogram_tables(rng)
+    v_rel, w_rel = _histogram_tables(rng)
+    return [
+        DistSpec(0, CONSTANT, (c, 0.0, INF)),
+        DistSpec(1, EXPONENTIAL, (e1, INF, INF)),
+        DistSpec(2, EXPONENTIAL, (e2, INF, INF)),
+        DistSpec(3, GAMMA, (2.0, g1, INF)),
+        DistSpec(4, GAMMA, (0.5, g2, INF)),
+        DistSpec(5, EMPIRICAL_ABSOLUTE, values=v_abs, weights=w_abs),
+        DistSpec(6, EMPIRICAL_RELATIVE, values=v_rel, weights=w_rel),
+    ]
+
+
 def layered_small() -> FlatDag:
     """200 events / ~380 activities, 3 activity types (the CPU-sized workload)."""
     dists = [

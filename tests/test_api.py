@@ -214,6 +214,20 @@ def test_plan_is_host_side() -> None:
     assert info["samples_per_lane"] == 2 and info["lds_bytes"] == info["lds_slots"] * 1024
 
 
+def test_batch_stats_checks_arguments_before_the_device() -> None:
+    """The batch statistics reject a DAG whose rows exceed the memory budget, and bad statistics arguments,
+    before touching a device."""
+    from mc_dagprop_amd.workloads import layered_network
+
+    B = capi.Batch([capi.Propagator(layered_network(layers=12))])
+    with pytest.raises(capi.McdpError, match="memory_budget"):
+        B.stats([np.arange(5000, dtype=np.int32)], memory_budget=1 << 20)
+    with pytest.raises(capi.McdpError, match="at least one sample"):
+        B.stats([np.zeros(0, dtype=np.int32)])
+    with pytest.raises(ValueError):
+        B.stats([np.arange(3), np.arange(3)])
+
+
 # ---- the reference's cases on the GPU -------------------------------------------------
 @pytest.mark.gpu
 def test_constant_via_generic(gpu) -> None:

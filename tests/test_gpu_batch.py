@@ -90,3 +90,34 @@ def test_reconfigured_member_is_rejected(gpu) -> None:
     props[0].configure(4, 8, 1)
     with pytest.raises(capi.McdpError, match="reconfigured"):
         B.run_many([np.arange(10)])
+
+
+@pytest.mark.parametrize("budget_mb", [5, 4096])
+def test_batch_stats_match_single_runs(gpu, budget_mb: int) -> None:
+    """mcdp_batch_stats_host over DAGs of uneven size, in groups that fit the memory budget (5 MB: the
+    2.4 MB DAG alone, then the 4.3 MB one with the two small ones; 4 GB: one group), equals each DAG's own
+    run_many followed by the device statistics: means, quantiles and cause counts bit for bit."""
+    dags = _dags()
+    ns = [1000, 300, 129, 1]
+    seeds = [np.arange(n, dtype=np.int32) * (k + 5) + 77 for k, n in enumerate(ns)]
+    q = (0.0, 0.5, 0.9, 0.99, 1.0)
+    want = []
+    for d, s in zip(dags, seeds):
+        P = capi.Propagator(d)
+        n = s.size
+        sd = capi.DeviceBuffer.from_array(s)
+        real = capi.DeviceBuffer(8 * max(P.E * n, 1))
+        cause = capi.DeviceBuffer(4 * max(P.E * n, 1))
+        P.run_many_device(sd, n, n, real, None, cause)
+        want.append(P.event_stats_device(real, cause, n, n, q))
+    B = capi.Batch([capi.Propagator(d) for d in dags])
+    got = B.stats(seeds, q, memory_budget=budget_mb << 20)
+    for (m, qq, c), (m0, qq0, c0) in zip(got, want):
+        np.testing.assert_array_equal(bits(m), bits(m0))
+        np.testing.assert_array_equal(bits(qq), bits(qq0))
+        np.testing.assert_array_equal(c, c0)
+    means_only = B.stats(seeds, (), memory_budget=budget_mb << 20, cause_counts=False)
+    for (m, qq, c), (m0, _, _) in zip(means_only, want):
+        np.testing.assert_array_equal(bits(m), bits(m0))
+        assert qq.shape[1] == 0 and c is None
+
