@@ -161,6 +161,19 @@ void mcdp_batch_destroy(mcdp_batch *batch);
 int mcdp_batch_run_device(mcdp_batch *batch, const int32_t *const *seeds_dev, const int64_t *n, const int64_t *ld,
                           double *const *realized_dev, double *const *durations_dev, int32_t *const *cause_dev,
                           void *stream);
+/* mcdp_batch_run_device in two halves, for callers that replay a batch from a
+ * HIP graph.  prepare uploads the launch's descriptors (arguments as for
+ * run_device; synchronous, waits for the device first, may allocate).  launch
+ * enqueues the prepared launch on `stream`: one kernel that clears the work
+ * queue's head and the persistent batch kernel, with no allocation, copy or
+ * synchronisation, so it may be captured (mcdp_graph_capture_begin/end) and
+ * the graph replayed; every replay recomputes every output.  A graph holds the
+ * descriptors' device address: the next prepare of the batch changes what its
+ * replays compute, and may move the descriptors (then the graph must not be
+ * replayed again). */
+int mcdp_batch_prepare_device(mcdp_batch *batch, const int32_t *const *seeds_dev, const int64_t *n, const int64_t *ld,
+                              double *const *realized_dev, double *const *durations_dev, int32_t *const *cause_dev);
+int mcdp_batch_launch(mcdp_batch *batch, void *stream);
 /* Per-event statistics (as mcdp_event_stats_device defines them) of every DAG
  * p of the batch over its own seeds, without keeping per-sample results: the
  * DAGs run in groups, in batch order, whose seeds, realized and cause rows and
@@ -316,6 +329,13 @@ int mcdp_event_create(void **event);
 int mcdp_event_destroy(void *event);
 int mcdp_event_record(void *event, void *stream);
 int mcdp_event_elapsed_ms(void *start, void *stop, float *ms);
+/* Stream capture into a HIP graph: begin on `stream` (global capture mode),
+ * end returns an instantiated graph; launch replays it on a stream; destroy
+ * frees it. */
+int mcdp_graph_capture_begin(void *stream);
+int mcdp_graph_capture_end(void *stream, void **graph_exec);
+int mcdp_graph_launch(void *graph_exec, void *stream);
+int mcdp_graph_destroy(void *graph_exec);
 
 /* ---- Test hooks (no counterpart in the reference) ------------------------ */
 /* Runs the device Philox-4x32-10 (variant 4) or Philox-2x32-10 (variant 2) on

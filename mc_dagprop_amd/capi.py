@@ -74,6 +74,13 @@ _SIGNATURES = {
     "mcdp_batch_create": (C.c_int, [C.POINTER(_P), _i32, _i32, _i32, _i32, C.POINTER(_P)]),
     "mcdp_batch_destroy": (None, [_P]),
     "mcdp_batch_run_device": (C.c_int, [_P, C.POINTER(_P), _pi64, _pi64, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), _P]),
+    "mcdp_batch_prepare_device": (C.c_int, [_P, C.POINTER(_P), _pi64, _pi64, C.POINTER(_P), C.POINTER(_P),
+                                            C.POINTER(_P)]),
+    "mcdp_batch_launch": (C.c_int, [_P, _P]),
+    "mcdp_graph_capture_begin": (C.c_int, [_P]),
+    "mcdp_graph_capture_end": (C.c_int, [_P, C.POINTER(_P)]),
+    "mcdp_graph_launch": (C.c_int, [_P, _P]),
+    "mcdp_graph_destroy": (C.c_int, [_P]),
     "mcdp_batch_stats_host": (C.c_int, [_P, C.POINTER(_P), _pi64, C.c_int32, _pf64, _i64, C.POINTER(_P), C.POINTER(_P),
                                         C.POINTER(_P)]),
     "mcdp_propagator_set_times": (C.c_int, [_P, _pf64, _pf64]),
@@ -444,6 +451,24 @@ class Batch:
         ll = np.ascontiguousarray(ld, dtype=np.int64)
         check(lib().mcdp_batch_run_device(self.h, ptrs(seeds), _ptr(nn, _pi64), _ptr(ll, _pi64), ptrs(realized),
                                           ptrs(durations), ptrs(cause), stream))
+
+    def prepare_device(self, seeds, n, ld, realized, durations=None, cause=None) -> None:
+        """Uploads the launch descriptors for these device buffers (mcdp_batch_prepare_device)."""
+        k = len(self.props)
+
+        def ptrs(bufs):
+            if bufs is None:
+                return None
+            return (_P * k)(*[(b.ptr if b is not None else None) for b in bufs])
+
+        nn = np.ascontiguousarray(n, dtype=np.int64)
+        ll = np.ascontiguousarray(ld, dtype=np.int64)
+        check(lib().mcdp_batch_prepare_device(self.h, ptrs(seeds), _ptr(nn, _pi64), _ptr(ll, _pi64), ptrs(realized),
+                                              ptrs(durations), ptrs(cause)))
+
+    def launch(self, stream=None) -> None:
+        """Enqueues the prepared launch (capturable into a graph)."""
+        check(lib().mcdp_batch_launch(self.h, stream))
 
     def run_many(self, seed_lists) -> list:
         """Host convenience: [(realized (n, E), durations (n, L), cause (n, E))] per propagator."""

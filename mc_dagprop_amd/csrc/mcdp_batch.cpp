@@ -22,6 +22,12 @@ struct mcdp_batch {
     void *dev = nullptr;  // WalkParams[plans], item_dag[], item_tile[], counter
     size_t dev_bytes = 0;
     int device = -1;
+    // The prepared launch (mcdp_batch_prepare_device): descriptors in `dev`.
+    bool prepared = false;
+    WalkBatchParams bp{};
+    WalkVariant v{};
+    size_t lds = 0;
+    int grid = 0;
     // mcdp_batch_stats_host's workspace, kept for the next call (allocating
     // hundreds of GB costs more than the launches that use them)
     void *stats_work = nullptr;
@@ -70,10 +76,10 @@ int mcdp_batch_create(mcdp_propagator *const *props, int32_t n_props, int32_t wa
 
 void mcdp_batch_destroy(mcdp_batch *b) { delete b; }
 
-int mcdp_batch_run_device(mcdp_batch *b, const int32_t *const *seeds_dev, const int64_t *n, const int64_t *ld,
-                          double *const *realized_dev, double *const *durations_dev, int32_t *const *cause_dev,
-                          void *stream) {
+int mcdp_batch_prepare_device(mcdp_batch *b, const int32_t *const *seeds_dev, const int64_t *n, const int64_t *ld,
+                              double *const *realized_dev, double *const *durations_dev, int32_t *const *cause_dev) {
     if (!b) return fail(MCDP_E_INVALID, "batch is null");
+    b->prepared = false;
     if (!seeds_dev || !n || !ld || !realized_dev) return fail(MCDP_E_INVALID, "null argument");
     const size_t D = b->props.size();
     int count = 0;
@@ -133,7 +139,6 @@ int mcdp_batch_run_device(mcdp_batch *b, const int32_t *const *seeds_dev, const 
     if (item_dag.size() > (size_t)std::numeric_limits<int32_t>::max() / 2)
         return fail(MCDP_E_INVALID, "too many tiles in one batch");
     const int32_t n_items = (int32_t)item_dag.size();
-    if (n_items == 0) return MCDP_OK;
 
     const size_t o_par = 0, o_dag = align_to(sizeof(WalkParams) * D, 256);
     const size_t o_tile = o_dag + align_to(sizeof(int32_t) * item_dag.size(), 256);
@@ -151,16 +156,15 @@ int mcdp_batch_run_device(mcdp_batch *b, const int32_t *const *seeds_dev, const 
         b->device = device;
     }
     auto *base = static_cast<unsigned char *>(b->dev);
-    const auto s = static_cast<hipStream_t>(stream);
-    // Earlier launches on this stream may still read the descriptors: the
-    // copies are ordered on the same stream, from host memory that outlives
-    // them (synchronised below before returning).
-    HIP_TRY(hipMemcpyAsync(base + o_par, params.data(), sizeof(WalkParams) * D, hipMemcpyHostToDevice, s), "copy batch");
-    HIP_TRY(hipMemcpyAsync(base + o_dag, item_dag.data(), sizeof(int32_t) * item_dag.size(), hipMemcpyHostToDevice, s),
-            "copy batch");
-    HIP_TRY(hipMemcpyAsync(base + o_tile, item_tile.data(), sizeof(int32_t) * item_tile.size(), hipMemcpyHostToDevice,
-                           s),
-            "copy batch");
+    // Launches in flight may still read the descriptors being replaced.
+    HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    HIP_TRY(hipMemcpy(base + o_par, params.data(), sizeof(WalkParams) * D, hipMemcpyHostToDevice), "copy batch");
+    if (n_items > 0) {
+        HIP_TRY(hipMemcpy(base + o_dag, item_dag.data(), sizeof(int32_t) * item_dag.size(), hipMemcpyHostToDevice),
+                "copy batch");
+        HIP_TRY(hipMemcpy(base + o_tile, item_tile.data(), sizeof(int32_t) * item_tile.size(), hipMemcpyHostToDevice),
+                "copy batch");
+    }
     WalkBatchParams bp;
     bp.params = reinterpret_cast<const WalkParams *>(base + o_par);
     bp.item_dag = reinterpret_cast<const int32_t *>(base + o_dag);
@@ -171,10 +175,31 @@ int mcdp_batch_run_device(mcdp_batch *b, const int32_t *const *seeds_dev, const 
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute");
     HIP_TRY(walk_batch_occupancy(v, b->waves, lds, &bpc), "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     if (bpc < 1) return fail(MCDP_E_DEVICE, "batch kernel cannot be resident with " + std::to_string(lds) + " bytes of LDS");
-    const int grid = (int)std::min<int64_t>(n_items, (int64_t)cus * bpc);
-    HIP_TRY(launch_walk_batch(bp, v, b->waves, grid, lds, s), "batch kernel launch");
-    // The host vectors above back the asynchronous copies.
-    HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+    b->bp = bp;
+    b->v = v;
+    b->lds = lds;
+    b->grid = (int)std::min<int64_t>(n_items, (int64_t)cus * bpc);
+    b->prepared = true;
+    return MCDP_OK;
+}
+
+int mcdp_batch_launch(mcdp_batch *b, void *stream) {
+    if (!b) return fail(MCDP_E_INVALID, "batch is null");
+    if (!b->prepared) return fail(MCDP_E_INVALID, "no prepared launch: call mcdp_batch_prepare_device first");
+    if (b->bp.n_items == 0) return MCDP_OK;
+    HIP_TRY(launch_walk_batch(b->bp, b->v, b->waves, b->grid, b->lds, static_cast<hipStream_t>(stream)),
+            "batch kernel launch");
+    return MCDP_OK;
+}
+
+int mcdp_batch_run_device(mcdp_batch *b, const int32_t *const *seeds_dev, const int64_t *n, const int64_t *ld,
+                          double *const *realized_dev, double *const *durations_dev, int32_t *const *cause_dev,
+                          void *stream) {
+    int rc = mcdp_batch_prepare_device(b, seeds_dev, n, ld, realized_dev, durations_dev, cause_dev);
+    if (rc) return rc;
+    rc = mcdp_batch_launch(b, stream);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "hipStreamSynchronize");
     return MCDP_OK;
 }
 

@@ -1256,6 +1256,33 @@ int mcdp_memcpy2d_dtoh(void *dst, size_t dst_pitch, const void *src, size_t src_
     HIP_TRY(hipMemcpy2D(dst, dst_pitch, src, src_pitch, width, height, hipMemcpyDeviceToHost), "hipMemcpy2D DtoH");
     return MCDP_OK;
 }
+int mcdp_graph_capture_begin(void *stream) {
+    HIP_TRY(hipStreamBeginCapture(static_cast<hipStream_t>(stream), hipStreamCaptureModeGlobal),
+            "hipStreamBeginCapture");
+    return MCDP_OK;
+}
+int mcdp_graph_capture_end(void *stream, void **graph_exec) {
+    if (!graph_exec) return fail(MCDP_E_INVALID, "graph_exec is null");
+    *graph_exec = nullptr;
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamEndCapture(static_cast<hipStream_t>(stream), &g), "hipStreamEndCapture");
+    hipGraphExec_t x = nullptr;
+    hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    *graph_exec = x;
+    return MCDP_OK;
+}
+int mcdp_graph_launch(void *graph_exec, void *stream) {
+    if (!graph_exec) return fail(MCDP_E_INVALID, "graph_exec is null");
+    HIP_TRY(hipGraphLaunch(static_cast<hipGraphExec_t>(graph_exec), static_cast<hipStream_t>(stream)),
+            "hipGraphLaunch");
+    return MCDP_OK;
+}
+int mcdp_graph_destroy(void *graph_exec) {
+    if (graph_exec) HIP_TRY(hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec)), "hipGraphExecDestroy");
+    return MCDP_OK;
+}
 int mcdp_stream_create(void **stream) {
     if (!stream) return fail(MCDP_E_INVALID, "stream is null");
     hipStream_t s = nullptr;

@@ -121,3 +121,59 @@ def test_batch_stats_match_single_runs(gpu, budget_mb: int) -> None:
         np.testing.assert_array_equal(bits(m), bits(m0))
         assert qq.shape[1] == 0 and c is None
 
+
+
+def test_batch_launch_replays_from_a_graph(gpu) -> None:
+    """The prepared batch launch (queue-head reset kernel + persistent kernel, no memset, copy or sync)
+    captured into a HIP graph: three replays, each after the outputs were overwritten and after other eager
+    work on the device, reproduce the eager launch bit for bit."""
+    dags = _dags()
+    ns = [1000, 300, 129, 1]
+    seeds = [np.arange(n, dtype=np.int32) * (k + 2) + 9 for k, n in enumerate(ns)]
+    props = [capi.Propagator(d) for d in dags]
+    B = capi.Batch(props)
+    sd = [capi.DeviceBuffer.from_array(s) for s in seeds]
+    real = [capi.DeviceBuffer(8 * P.E * n) for P, n in zip(props, ns)]
+    dur = [capi.DeviceBuffer(8 * P.L * n) for P, n in zip(props, ns)]
+    cause = [capi.DeviceBuffer(4 * P.E * n) for P, n in zip(props, ns)]
+    outs = real + dur + cause
+
+    def read():
+        return ([r.to_array(np.float64, (P.E, n)) for r, P, n in zip(real, props, ns)],
+                [d.to_array(np.float64, (P.L, n)) for d, P, n in zip(dur, props, ns)],
+                [c.to_array(np.int32, (P.E, n)) for c, P, n in zip(cause, props, ns)])
+
+    B.prepare_device(sd, ns, ns, real, dur, cause)
+    stream = capi.C.c_void_p(0)
+    capi.check(capi.lib().mcdp_stream_create(capi.C.byref(stream)))
+    B.launch(stream)
+    capi.synchronize(stream)
+    want = read()
+    singles = [capi.Propagator(d).run_many(s) for d, s in zip(dags, seeds)]
+    for r, (r0, _, _) in zip(want[0], singles):
+        np.testing.assert_array_equal(bits(r.T), bits(r0))
+    graph = capi.C.c_void_p(0)
+    capi.check(capi.lib().mcdp_graph_capture_begin(stream))
+    B.launch(stream)
+    capi.check(capi.lib().mcdp_graph_capture_end(stream, capi.C.byref(graph)))
+    other = capi.Propagator(layered_small())
+    o_seeds = capi.DeviceBuffer.from_array(np.arange(500, dtype=np.int32))
+    o_real, o_dur, o_cause = (capi.DeviceBuffer(8 * other.E * 500), capi.DeviceBuffer(8 * other.L * 500),
+                              capi.DeviceBuffer(4 * other.E * 500))
+    try:
+        for rep in range(3):
+            for b in outs:
+                capi.check(capi.lib().mcdp_memset(b.ptr, 0xFF, b.nbytes))
+            other.run_many_device(o_seeds, 500, 500, o_real, o_dur, o_cause)
+            capi.synchronize()
+            capi.check(capi.lib().mcdp_graph_launch(graph, stream))
+            capi.synchronize(stream)
+            got = read()
+            for kind, g_list, w_list in zip(("realized", "durations", "cause"), got, want):
+                for g, w in zip(g_list, w_list):
+                    np.testing.assert_array_equal(bits(g) if g.dtype != np.int32 else g,
+                                                  bits(w) if w.dtype != np.int32 else w,
+                                                  err_msg=f"replay {rep} {kind}")
+    finally:
+        capi.check(capi.lib().mcdp_graph_destroy(graph))
+        capi.check(capi.lib().mcdp_stream_destroy(stream))
