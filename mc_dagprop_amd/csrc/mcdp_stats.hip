@@ -329,8 +329,7 @@ __global__ void __launch_bounds__(kBlock) cause_kernel(const int32_t *__restrict
         }
         __syncthreads();
         const int32_t *row = cause + (uint64_t)(uint32_t)e * ld;
-        for (int32_t s = threadIdx.x; s < n; s += kBlock) {
-            const int32_t c = row[s];
+        auto count = [&](int32_t c) {
             int32_t slot = -1;
             if (c == -1) {
                 slot = 0;
@@ -353,7 +352,21 @@ __global__ void __launch_bounds__(kBlock) cause_kernel(const int32_t *__restrict
             } else if (slot >= 0) {
                 atomicAdd((unsigned long long *)&out[slot], 1ull);
             }
+        };
+        // 16-byte loads over the row's aligned middle (four samples a lane),
+        // single samples before and after it
+        const int32_t head = min(n, (int32_t)(((16u - ((uint32_t)(uintptr_t)row & 15u)) & 15u) / 4u));
+        const int32_t n_vec = (n - head) / 4;
+        const int4 *vrow = reinterpret_cast<const int4 *>(row + head);
+        for (int32_t s = threadIdx.x; s < head; s += kBlock) count(row[s]);
+        for (int32_t v = threadIdx.x; v < n_vec; v += kBlock) {
+            const int4 c4 = vrow[v];
+            count(c4.x);
+            count(c4.y);
+            count(c4.z);
+            count(c4.w);
         }
+        for (int32_t s = head + 4 * n_vec + (int32_t)threadIdx.x; s < n; s += kBlock) count(row[s]);
         __syncthreads();
         if (in_lds)
             for (int i = threadIdx.x; i <= np; i += kBlock) out[i] = (accumulate ? out[i] : 0) + (int64_t)c_lds[i];
