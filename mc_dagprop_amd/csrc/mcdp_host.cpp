@@ -678,6 +678,17 @@ int ensure_device_impl(mcdp_propagator &P, DeviceCopy *&out) {
     DeviceCopy d;
     d.device = device;
     d.blob = blob;
+    // the side stream of wide steps and its fork / join events, made here so
+    // that a launch creates nothing (it may be under stream capture)
+    e = hipStreamCreateWithFlags(&d.side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&d.fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&d.join, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        if (d.side) (void)hipStreamDestroy(d.side);
+        if (d.fork) (void)hipEventDestroy(d.fork);
+        (void)hipFree(blob);
+        return hip_fail(e, "side stream");
+    }
     auto *b = static_cast<unsigned char *>(blob);
     d.base.step_off = reinterpret_cast<const int32_t *>(b + o_step);
     d.base.recs = reinterpret_cast<const Record *>(b + o_recs);
@@ -864,11 +875,6 @@ int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double
     // the caller's stream before the tiles and joined after.
     hipStream_t tail_stream = stream;
     if (wide) {
-        if (!d->side) {
-            HIP_TRY(hipStreamCreateWithFlags(&d->side, hipStreamNonBlocking), "hipStreamCreateWithFlags");
-            HIP_TRY(hipEventCreateWithFlags(&d->fork, hipEventDisableTiming), "hipEventCreateWithFlags");
-            HIP_TRY(hipEventCreateWithFlags(&d->join, hipEventDisableTiming), "hipEventCreateWithFlags");
-        }
         HIP_TRY(hipEventRecord(d->fork, stream), "hipEventRecord");
         HIP_TRY(hipStreamWaitEvent(d->side, d->fork, 0), "hipStreamWaitEvent");
         tail_stream = d->side;

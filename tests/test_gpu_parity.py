@@ -334,3 +334,45 @@ def test_tail_mode_rejects_unknown_routes(gpu) -> None:
     P = capi.Propagator(layered_small())
     assert capi.lib().mcdp_debug_tail_mode(P.h, 3) != 0
     assert capi.lib().mcdp_debug_tail_mode(P.h, -1) != 0
+
+
+def test_run_many_with_wide_steps_replays_from_a_graph(gpu) -> None:
+    """run_many_device whose last partial round runs as wide steps on the side stream (forked from and
+    joined to the caller's stream) captured into a HIP graph: three replays, each after the outputs were
+    overwritten and after other eager work, reproduce the eager run bit for bit."""
+    dag = layered_small()
+    n = 256 * 128 + 77
+    P = capi.Propagator(dag)
+    P.configure(16, 8, 2)
+    capi.check(capi.lib().mcdp_debug_tail_mode(P.h, 2))
+    sd = capi.DeviceBuffer.from_array(np.arange(n, dtype=np.int32) * 7 - 3)
+    real, dur, cause = capi.DeviceBuffer(8 * P.E * n), capi.DeviceBuffer(8 * P.L * n), capi.DeviceBuffer(4 * P.E * n)
+    stream = capi.C.c_void_p(0)
+    capi.check(capi.lib().mcdp_stream_create(capi.C.byref(stream)))
+
+    def read():
+        return (real.to_array(np.float64, (P.E, n)), dur.to_array(np.float64, (P.L, n)),
+                cause.to_array(np.int32, (P.E, n)))
+
+    P.run_many_device(sd, n, n, real, dur, cause, stream)
+    capi.synchronize(stream)
+    want = read()
+    graph = capi.C.c_void_p(0)
+    capi.check(capi.lib().mcdp_graph_capture_begin(stream))
+    P.run_many_device(sd, n, n, real, dur, cause, stream)
+    capi.check(capi.lib().mcdp_graph_capture_end(stream, capi.C.byref(graph)))
+    other = capi.Propagator(layered_network(layers=12))
+    try:
+        for rep in range(3):
+            for b in (real, dur, cause):
+                capi.check(capi.lib().mcdp_memset(b.ptr, 0xFF, b.nbytes))
+            other.run_many(np.arange(300))
+            capi.check(capi.lib().mcdp_graph_launch(graph, stream))
+            capi.synchronize(stream)
+            got = read()
+            np.testing.assert_array_equal(bits(got[0]), bits(want[0]), err_msg=f"replay {rep}")
+            np.testing.assert_array_equal(bits(got[1]), bits(want[1]), err_msg=f"replay {rep}")
+            np.testing.assert_array_equal(got[2], want[2], err_msg=f"replay {rep}")
+    finally:
+        capi.check(capi.lib().mcdp_graph_destroy(graph))
+        capi.check(capi.lib().mcdp_stream_destroy(stream))
