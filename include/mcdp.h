@@ -144,7 +144,9 @@ int mcdp_propagator_autotune(mcdp_propagator *prop, int64_t n_samples, int32_t *
  * several threads (test/test_simulator.py:201-215).  A batch runs them in one
  * persistent launch: workgroups take (DAG, sample tile) items from one queue,
  * largest DAGs first, so small DAGs and uneven sample counts fill the device.
- * Results are bit-identical to each propagator's own mcdp_run_many_device. */
+ * Results are bit-identical to each propagator's own mcdp_run_many_device.
+ * A batch holds device state of its own (launch descriptors, a statistics
+ * workspace): one batch must not be used from two threads at once. */
 
 /* Makes a batch of n_props propagators (they stay owned by the caller and must
  * outlive the batch).  Every propagator is configured to the batch's schedule
