@@ -376,3 +376,28 @@ def test_run_many_with_wide_steps_replays_from_a_graph(gpu) -> None:
     finally:
         capi.check(capi.lib().mcdp_graph_destroy(graph))
         capi.check(capi.lib().mcdp_stream_destroy(stream))
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_wide_steps_special_values(gpu, case: int) -> None:
+    """Injected durations with NaN, +-inf, -0.0, ties and caps through a run whose last partial round is
+    forced onto wide steps: every sample equals the oracle, the wide samples included."""
+    rng = np.random.default_rng(300 + case)
+    kw = [{}, {"duplicates": True}, {"gaps": True}, {"oob_edges": True}][case]
+    dag = D.random_dag(rng, int(rng.integers(40, 150)), max_preds=int(rng.integers(2, 7)), **kw)
+    P = capi.Propagator(dag)
+    P.configure(16, 8, 2)
+    capi.check(capi.lib().mcdp_debug_tail_mode(P.h, 2))
+    ctx = oracle.OracleCtx(dag)
+    n = 256 * 128 + 130  # one full round of 256 resident tiles of 128 on a 256-CU device, then 130 samples
+    dur = D.special_durations(rng, n, ctx.L)
+    r, c = P.propagate(dur)
+    pick = np.r_[0:64, n - 300:n]
+    rr, cc = ctx.propagate(dur[pick])
+    np.testing.assert_array_equal(bits(r[pick]), bits(rr))
+    np.testing.assert_array_equal(c[pick], cc)
+    Q = capi.Propagator(dag)
+    Q.configure(16, 8, 1)
+    r1, c1 = Q.propagate(dur)
+    np.testing.assert_array_equal(bits(r), bits(r1))
+    np.testing.assert_array_equal(c, c1)
