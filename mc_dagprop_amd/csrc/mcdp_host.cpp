@@ -274,13 +274,14 @@ int draw_cost(const DistRec *D) {
 constexpr int kPredCost = 20;   // propagation work per predecessor record
 constexpr int kEventCost = 15;  // per event: header, final cap, stores
 // A wide step's launch gap and predecessor loads, in the same units
-// (calibrated on the network workload: a 200-step tail costs about 12 ns per
-// unit either way).
+// (calibrated on the network workload: a 200-step tail run alone costs about
+// 12 ns per unit either way).
 constexpr int kWideStepCost = 300;
 // Wide steps for a partial round of at most this many groups of 64 samples
-// (each step launches events x groups single-wave workgroups; the estimate
-// assumes about one of them per SIMD).
-constexpr int64_t kWideMaxGroups = 32;
+// (each step launches events x groups single-wave workgroups; up to 128
+// groups they were measured to end inside the tiles' time on the network and
+// the small workloads).
+constexpr int64_t kWideMaxGroups = 128;
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
@@ -839,7 +840,10 @@ int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double
     //     plus their costliest events, instead of a whole tile's walk;
     //   * with two samples per lane, the one-sample-per-lane kernel of the
     //     same schedule, whose tiles finish sooner.
-    // The estimates of both latencies (from the schedule) pick the route.
+    // The wide steps run beside the full rounds' tiles (below), so they cost
+    // little as long as they end before the tiles do: they are taken when
+    // their estimated latency (from the schedule) is below that of the full
+    // rounds.
     // Results do not depend on the split or the route.
     int64_t n_main = n, n_tail = 0;
     bool wide = false;
@@ -853,7 +857,7 @@ int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double
         const int64_t groups = (rest + kLanes - 1) / kLanes;
         if (full_rounds >= 1 && rest > 0) {
             if (P.tail_mode == 2 ||
-                (P.tail_mode == 0 && groups <= kWideMaxGroups && P.wide_cost < P.tile_cost)) {
+                (P.tail_mode == 0 && groups <= kWideMaxGroups && P.wide_cost < full_rounds * P.tile_cost)) {
                 wide = true;
                 n_main = n - rest;
                 n_tail = rest;
