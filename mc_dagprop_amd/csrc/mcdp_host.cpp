@@ -855,7 +855,15 @@ int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double
         const int64_t full_rounds = ((n + tile - 1) / tile) / resident;
         const int64_t rest = n - full_rounds * resident * tile;
         const int64_t groups = (rest + kLanes - 1) / kLanes;
-        if (full_rounds >= 1 && rest > 0) {
+        if (full_rounds == 0 && rest > 0) {
+            // less than one round: the wide steps alone against one round of
+            // tiles that hardly fills the device
+            if (P.tail_mode == 2 || (P.tail_mode == 0 && groups <= kWideMaxGroups && P.wide_cost < P.tile_cost)) {
+                wide = true;
+                n_main = 0;
+                n_tail = n;
+            }
+        } else if (full_rounds >= 1 && rest > 0) {
             if (P.tail_mode == 2 ||
                 (P.tail_mode == 0 && groups <= kWideMaxGroups && P.wide_cost < full_rounds * P.tile_cost)) {
                 wide = true;
@@ -878,12 +886,13 @@ int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double
     // on a SIMD, so they run on a second stream beside the tiles, forked from
     // the caller's stream before the tiles and joined after.
     hipStream_t tail_stream = stream;
-    if (wide) {
+    if (wide && n_main > 0) {
         HIP_TRY(hipEventRecord(d->fork, stream), "hipEventRecord");
         HIP_TRY(hipStreamWaitEvent(d->side, d->fork, 0), "hipStreamWaitEvent");
         tail_stream = d->side;
     }
-    HIP_TRY(launch_walk(p, v, (int)std::min<int64_t>(p.n_tiles, resident), lds, stream), "walk kernel launch");
+    if (n_main > 0)
+        HIP_TRY(launch_walk(p, v, (int)std::min<int64_t>(p.n_tiles, resident), lds, stream), "walk kernel launch");
     if (n_tail > 0) {
         WalkParams q = p;
         q.seeds = seeds ? seeds + n_main : nullptr;
@@ -901,8 +910,10 @@ int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double
                 HIP_TRY(launch_walk_wide(q, injected, e0, e1 - e0, groups, tail_stream), "wide step launch");
             }
             if (!injected) HIP_TRY(launch_walk_wide_orphans(q, groups, tail_stream), "wide orphans launch");
-            HIP_TRY(hipEventRecord(d->join, tail_stream), "hipEventRecord");
-            HIP_TRY(hipStreamWaitEvent(stream, d->join, 0), "hipStreamWaitEvent");
+            if (tail_stream != stream) {
+                HIP_TRY(hipEventRecord(d->join, tail_stream), "hipEventRecord");
+                HIP_TRY(hipStreamWaitEvent(stream, d->join, 0), "hipStreamWaitEvent");
+            }
         } else {
             HIP_TRY(launch_walk(q, v1, (int)std::min<int64_t>(q.n_tiles, resident1), lds1, stream), "walk kernel launch");
         }

@@ -401,3 +401,32 @@ def test_wide_steps_special_values(gpu, case: int) -> None:
     r1, c1 = Q.propagate(dur)
     np.testing.assert_array_equal(bits(r), bits(r1))
     np.testing.assert_array_equal(c, c1)
+
+
+@pytest.mark.parametrize("name", sorted(PARTIAL_ROUND_DAGS))
+@pytest.mark.parametrize("n", [1, 77, 1000])
+def test_short_run_routes_match(gpu, name: str, n: int) -> None:
+    """Runs shorter than one round: one partial round of tiles (route 1) or wide steps alone (route 2), and
+    the automatic choice, give the same bits, sampled and injected; the oracle agrees."""
+    dag = PARTIAL_ROUND_DAGS[name]()
+    seeds = np.arange(n, dtype=np.int32) * 11 + 4
+    out = []
+    for mode in (1, 2, 0):
+        P = capi.Propagator(dag)
+        P.configure(16, 8, 2)
+        capi.check(capi.lib().mcdp_debug_tail_mode(P.h, mode))
+        out.append(P.run_many(seeds))
+    for r, d, c in out[1:]:
+        np.testing.assert_array_equal(bits(r), bits(out[0][0]))
+        np.testing.assert_array_equal(bits(d), bits(out[0][1]))
+        np.testing.assert_array_equal(c, out[0][2])
+    r0, d0, c0 = out[0]
+    P = capi.Propagator(dag)
+    P.configure(16, 8, 2)
+    capi.check(capi.lib().mcdp_debug_tail_mode(P.h, 2))
+    rr, cc = P.propagate(d0)
+    np.testing.assert_array_equal(bits(rr), bits(r0))
+    np.testing.assert_array_equal(cc, c0)
+    ro, co = oracle.OracleCtx(dag).propagate(d0[: min(n, 200)])
+    np.testing.assert_array_equal(bits(ro), bits(r0[: min(n, 200)]))
+    np.testing.assert_array_equal(co, c0[: min(n, 200)])
