@@ -271,9 +271,18 @@ int mcdp_batch_stats_host(mcdp_batch *b, const int32_t *const *seeds, const int6
         b->stats_bytes = max_group;
         b->stats_device = device;
     }
-    hipStream_t s = nullptr;
+    // the launch on one stream; the DAGs' statistics spread over a few, so
+    // that the small DAGs' kernels fill the device together
+    constexpr int kStatStreams = 4;
+    hipStream_t s = nullptr, ss[kStatStreams] = {};
     hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
-    if (e != hipSuccess) return hip_fail(e, "hipStreamCreateWithFlags");
+    for (int i = 0; e == hipSuccess && i < kStatStreams; ++i) e = hipStreamCreateWithFlags(&ss[i], hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        if (s) (void)hipStreamDestroy(s);
+        for (hipStream_t x : ss)
+            if (x) (void)hipStreamDestroy(x);
+        return hip_fail(e, "hipStreamCreateWithFlags");
+    }
     auto *base = static_cast<unsigned char *>(b->stats_work);
     int rc = MCDP_OK;
     std::vector<const int32_t *> sd(D, nullptr);
@@ -300,25 +309,29 @@ int mcdp_batch_stats_host(mcdp_batch *b, const int32_t *const *seeds, const int6
             at += bytes[d];
         }
         if (rc) break;
+        // returns with the launch done (its stream synchronised)
         rc = mcdp_batch_run_device(b, sd.data(), gn.data(), gld.data(), real.data(), nullptr, cause.data(), s);
         for (size_t d = group_start[g]; rc == MCDP_OK && d < group_start[g + 1]; ++d) {
+            const hipStream_t sx = ss[(d - group_start[g]) % kStatStreams];
             unsigned char *p = part[d];
             auto *dm = reinterpret_cast<double *>(p + o_mean[d]);
             auto *dq = n_q > 0 ? reinterpret_cast<double *>(p + o_quant[d]) : nullptr;
             auto *dc = n_counts[d] ? reinterpret_cast<int64_t *>(p + o_cnt[d]) : nullptr;
-            rc = mcdp_event_stats_device(b->props[d], real[d], cause[d], n[d], n[d], n_q, q, dm, dq, dc, s);
+            rc = mcdp_event_stats_device(b->props[d], real[d], cause[d], n[d], n[d], n_q, q, dm, dq, dc, sx);
             const size_t E = (size_t)b->props[d]->E;
-            if (rc == MCDP_OK && (e = hipMemcpyAsync(mean[d], dm, 8 * E, hipMemcpyDeviceToHost, s)) != hipSuccess)
+            if (rc == MCDP_OK && (e = hipMemcpyAsync(mean[d], dm, 8 * E, hipMemcpyDeviceToHost, sx)) != hipSuccess)
                 rc = hip_fail(e, "copy means");
             if (rc == MCDP_OK && dq &&
-                (e = hipMemcpyAsync(quantiles[d], dq, 8 * E * (size_t)n_q, hipMemcpyDeviceToHost, s)) != hipSuccess)
+                (e = hipMemcpyAsync(quantiles[d], dq, 8 * E * (size_t)n_q, hipMemcpyDeviceToHost, sx)) != hipSuccess)
                 rc = hip_fail(e, "copy quantiles");
             if (rc == MCDP_OK && dc &&
-                (e = hipMemcpyAsync(cause_counts[d], dc, 8 * (size_t)n_counts[d], hipMemcpyDeviceToHost, s)) !=
+                (e = hipMemcpyAsync(cause_counts[d], dc, 8 * (size_t)n_counts[d], hipMemcpyDeviceToHost, sx)) !=
                     hipSuccess)
                 rc = hip_fail(e, "copy cause counts");
         }
-        if (rc == MCDP_OK && (e = hipStreamSynchronize(s)) != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize");
+        // the next group reuses the workspace: every statistic of this one is done first
+        for (int i = 0; i < kStatStreams; ++i)
+            if ((e = hipStreamSynchronize(ss[i])) != hipSuccess && rc == MCDP_OK) rc = hip_fail(e, "hipStreamSynchronize");
         for (size_t d = group_start[g]; d < group_start[g + 1]; ++d) {
             sd[d] = nullptr;
             real[d] = nullptr;
@@ -327,6 +340,10 @@ int mcdp_batch_stats_host(mcdp_batch *b, const int32_t *const *seeds, const int6
     }
     (void)hipStreamSynchronize(s);
     (void)hipStreamDestroy(s);
+    for (hipStream_t x : ss) {
+        (void)hipStreamSynchronize(x);
+        (void)hipStreamDestroy(x);
+    }
     return rc;
 }
 
