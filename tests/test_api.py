@@ -214,6 +214,21 @@ def test_plan_is_host_side() -> None:
     assert info["samples_per_lane"] == 2 and info["lds_bytes"] == info["lds_slots"] * 1024
 
 
+def test_tail_route_estimates() -> None:
+    """The schedule's latency estimates of a last partial round (host-side): the 200-step network DAG, whose
+    levels are 100 events wide, favours wide steps; the 20-step small DAG favours its tiles."""
+    from mc_dagprop_amd.workloads import layered_network, layered_small
+
+    net = capi.Propagator(layered_network())
+    net.configure(16, 8, 2)
+    i = net.plan_info()
+    assert i["steps"] == 200 and 0 < i["tail_wide_cost"] < i["tail_tile_cost"]
+    small = capi.Propagator(layered_small())
+    small.configure(16, 8, 2)
+    i = small.plan_info()
+    assert i["steps"] == 20 and i["tail_wide_cost"] > i["tail_tile_cost"] > 0
+
+
 def test_batch_stats_checks_arguments_before_the_device() -> None:
     """The batch statistics reject a DAG whose rows exceed the memory budget, and bad statistics arguments,
     before touching a device."""
