@@ -9,7 +9,8 @@
 //     selection on order-preserving 64-bit keys over the key bytes in which
 //     the extremes differ (at most 8 passes of 8 bits);
 //   * cause counts: how many samples have cause -1 and how many have each
-//     distinct predecessor source, counted in LDS.
+//     distinct predecessor source, counted in registers (events with at most
+//     kCauseRegKeys distinct sources) or in LDS.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -51,8 +52,8 @@ __device__ __forceinline__ DD dd_add(DD x, DD y) {
     return r;
 }
 
-// Order-preserving key of a double: -inf < ... < -0 < +0 < ... < +inf (NaNs
-// are counted apart and never selected).
+// Order-preserving key of a double: -inf < ... < -0 < +0 < ... < +inf (NaN
+// keys lie beyond the infinities' and are never selected).
 __device__ __forceinline__ uint64_t order_key(double v) {
     const uint64_t u = (uint64_t)__double_as_longlong(v);
     return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
@@ -70,11 +71,20 @@ __global__ void __launch_bounds__(kBlock) mean_kernel(const double *__restrict__
         const double *row = realized + (uint64_t)(uint32_t)e * ld;
         DD acc{0.0, 0.0};
         double plain = 0.0;
-        for (int32_t s = threadIdx.x; s < n; s += kBlock) {
-            const double v = row[s];
+        // four loads per lane in flight; samples tid, tid + kBlock, ... in order
+        auto visit = [&](double v) {
             acc = dd_add(acc, v);
             plain += v;
+        };
+        int32_t s = threadIdx.x;
+        for (; s + 3 * kBlock < n; s += 4 * kBlock) {
+            const double v0 = row[s], v1 = row[s + kBlock], v2 = row[s + 2 * kBlock], v3 = row[s + 3 * kBlock];
+            visit(v0);
+            visit(v1);
+            visit(v2);
+            visit(v3);
         }
+        for (; s < n; s += kBlock) visit(row[s]);
         part[threadIdx.x] = acc;
         naive[threadIdx.x] = plain;
         __syncthreads();
@@ -139,8 +149,48 @@ __device__ __forceinline__ Extremes merge_extremes(Extremes a, const Extremes &b
     return a;
 }
 
-// Quantiles of each row; with `mean` non-null, its first pass also sums the
-// row (the mean without a separate read).
+// Keys of the infinities.  A NaN's key lies outside [kKeyNegInf, kKeyPosInf]
+// (above it for a positive sign bit, below for a negative one).
+constexpr uint64_t kKeyPosInf = 0xFFF0000000000000ull;
+constexpr uint64_t kKeyNegInf = 0x000FFFFFFFFFFFFFull;
+
+// The quantile kernel's first pass over a row: this thread's share of the key
+// extremes with their multiplicities and, with kSum, of the sum.  NaNs are not
+// set apart: their keys fall outside the infinities' (a row holds a NaN iff
+// its extremes do), and such a row's results are NaN whatever else it holds.
+// Four loads per lane are in flight per iteration; each thread still visits
+// its samples tid, tid + kBlock, ... in that order, so the sum's rounding does
+// not depend on the unrolling.
+template <bool kSum>
+__device__ __forceinline__ void extremes_pass(const double *__restrict__ row, int32_t n, Extremes &x, DD &acc,
+                                              double &plain) {
+    auto visit = [&](double v) {
+        if (kSum) {
+            acc = dd_add(acc, v);
+            plain += v;
+        }
+        const uint64_t key = order_key(v);
+        const bool lt = key < x.mn;
+        x.mn = lt ? key : x.mn;
+        x.cmn = (lt ? 0u : x.cmn) + (key == x.mn ? 1u : 0u);
+        const bool gt = key > x.mx;
+        x.mx = gt ? key : x.mx;
+        x.cmx = (gt ? 0u : x.cmx) + (key == x.mx ? 1u : 0u);
+    };
+    int32_t s = threadIdx.x;
+    for (; s + 3 * kBlock < n; s += 4 * kBlock) {
+        const double v0 = row[s], v1 = row[s + kBlock], v2 = row[s + 2 * kBlock], v3 = row[s + 3 * kBlock];
+        visit(v0);
+        visit(v1);
+        visit(v2);
+        visit(v3);
+    }
+    for (; s < n; s += kBlock) visit(row[s]);
+}
+
+// Quantiles of each row; with kMean, its first pass also sums the row (the
+// mean without a separate read).
+template <bool kMean>
 __global__ void __launch_bounds__(kBlock) quantile_kernel(const double *__restrict__ realized, int32_t n, uint32_t ld,
                                                           int32_t E, QuantileArgs qa, double *__restrict__ out,
                                                           double *__restrict__ mean) {
@@ -166,19 +216,7 @@ __global__ void __launch_bounds__(kBlock) quantile_kernel(const double *__restri
         Extremes x{~0ull, 0ull, 0u, 0u, 0u};
         DD acc{0.0, 0.0};
         double plain = 0.0;
-        for (int32_t s = threadIdx.x; s < n; s += kBlock) {
-            const double v = row[s];
-            if (mean) {
-                acc = dd_add(acc, v);
-                plain += v;
-            }
-            if (v != v) {
-                ++x.nans;
-                continue;
-            }
-            const uint64_t key = order_key(v);
-            x = merge_extremes(x, Extremes{key, key, 1u, 1u, 0u});
-        }
+        extremes_pass<kMean>(row, n, x, acc, plain);
         ext[threadIdx.x] = x;
         part[threadIdx.x] = acc;
         naive[threadIdx.x] = plain;
@@ -186,20 +224,22 @@ __global__ void __launch_bounds__(kBlock) quantile_kernel(const double *__restri
         for (int w = kBlock / 2; w > 0; w >>= 1) {  // fixed pairing: deterministic
             if ((int)threadIdx.x < w) {
                 ext[threadIdx.x] = merge_extremes(ext[threadIdx.x], ext[threadIdx.x + w]);
-                part[threadIdx.x] = dd_add(part[threadIdx.x], part[threadIdx.x + w]);
-                naive[threadIdx.x] += naive[threadIdx.x + w];
+                if (kMean) {
+                    part[threadIdx.x] = dd_add(part[threadIdx.x], part[threadIdx.x + w]);
+                    naive[threadIdx.x] += naive[threadIdx.x + w];
+                }
             }
             __syncthreads();
         }
         const Extremes X = ext[0];
-        if (mean && threadIdx.x == 0)
+        if (kMean && threadIdx.x == 0)
             mean[e] = (__builtin_isfinite(naive[0]) ? part[0].hi : naive[0]) / (double)n;
-        const int32_t valid = n - (int32_t)X.nans;
+        const bool has_nan = X.mx > kKeyPosInf || X.mn < kKeyNegInf;
         if ((int)threadIdx.x < R) {
             const int r = threadIdx.x;
             const int32_t k = qa.rank[r];
             done[r] = 1;
-            if (X.nans || k >= valid - (int32_t)X.cmx) {
+            if (has_nan || k >= n - (int32_t)X.cmx) {
                 prefix[r] = X.mx;
             } else if (k < (int32_t)X.cmn) {
                 prefix[r] = X.mn;
@@ -275,7 +315,7 @@ __global__ void __launch_bounds__(kBlock) quantile_kernel(const double *__restri
         if (threadIdx.x == 0) {
             for (int j = 0; j < qa.n_q; ++j) {
                 double res;
-                if (X.nans) {
+                if (has_nan) {
                     res = __builtin_nan("");
                 } else {
                     const double a = key_value(prefix[qa.lo[j]]);
@@ -286,6 +326,68 @@ __global__ void __launch_bounds__(kBlock) quantile_kernel(const double *__restri
             }
         }
         __syncthreads();  // prefix[] and ext[] are read above before the next event resets them
+    }
+}
+
+// Events with at most this many distinct sources count their row in registers.
+constexpr int kCauseRegKeys = 8;
+
+// First sample of a row of causes at a 16-byte boundary (at most n).
+__device__ __forceinline__ int32_t aligned_head(const int32_t *row, int32_t n) {
+    return min(n, (int32_t)(((16u - ((uint32_t)(uintptr_t)row & 15u)) & 15u) / 4u));
+}
+
+// Cause counts of one row for an event with NK distinct sources: per-lane
+// counters in registers (cnt[0] for cause -1, cnt[1 + k] for the source
+// keys[k]), one compare and add per counter and sample, then one LDS atomic
+// per counter and wave into c_lds[0] and c_lds[1 + pos[k]].  Causes that are
+// no predecessor of the event are counted nowhere.  Two 16-byte loads per
+// lane are in flight per iteration over the row's aligned middle.
+template <int NK>
+__device__ __forceinline__ void count_row_regs(const int32_t *__restrict__ row, int32_t n,
+                                               const int32_t *__restrict__ keys, const int32_t *__restrict__ pos,
+                                               uint32_t *c_lds) {
+    int32_t key[NK > 0 ? NK : 1];
+#pragma unroll
+    for (int k = 0; k < NK; ++k) key[k] = keys[k];
+    uint32_t cnt[NK + 1];
+#pragma unroll
+    for (int k = 0; k <= NK; ++k) cnt[k] = 0;
+    auto count = [&](int32_t c) {
+        cnt[0] += c == -1 ? 1u : 0u;
+#pragma unroll
+        for (int k = 0; k < NK; ++k) cnt[1 + k] += c == key[k] ? 1u : 0u;
+    };
+    const int32_t head = aligned_head(row, n);
+    const int32_t n_vec = (n - head) / 4;
+    const int4 *vrow = reinterpret_cast<const int4 *>(row + head);
+    for (int32_t s = threadIdx.x; s < head; s += kBlock) count(row[s]);
+    int32_t v = threadIdx.x;
+    for (; v + kBlock < n_vec; v += 2 * kBlock) {
+        const int4 a = vrow[v], b = vrow[v + kBlock];
+        count(a.x);
+        count(a.y);
+        count(a.z);
+        count(a.w);
+        count(b.x);
+        count(b.y);
+        count(b.z);
+        count(b.w);
+    }
+    for (; v < n_vec; v += kBlock) {
+        const int4 a = vrow[v];
+        count(a.x);
+        count(a.y);
+        count(a.z);
+        count(a.w);
+    }
+    for (int32_t s = head + 4 * n_vec + (int32_t)threadIdx.x; s < n; s += kBlock) count(row[s]);
+#pragma unroll
+    for (int k = 0; k <= NK; ++k) {
+        uint32_t t = cnt[k];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+        if ((threadIdx.x & 63) == 0 && t) atomicAdd(&c_lds[k == 0 ? 0 : 1 + pos[k - 1]], t);
     }
 }
 
@@ -329,6 +431,24 @@ __global__ void __launch_bounds__(kBlock) cause_kernel(const int32_t *__restrict
         }
         __syncthreads();
         const int32_t *row = cause + (uint64_t)(uint32_t)e * ld;
+        if (in_lds && nk <= kCauseRegKeys) {
+            const int32_t *k0 = pred_key + p0, *q0 = pred_pos + p0;
+            switch (nk) {
+            case 0: count_row_regs<0>(row, n, k0, q0, c_lds); break;
+            case 1: count_row_regs<1>(row, n, k0, q0, c_lds); break;
+            case 2: count_row_regs<2>(row, n, k0, q0, c_lds); break;
+            case 3: count_row_regs<3>(row, n, k0, q0, c_lds); break;
+            case 4: count_row_regs<4>(row, n, k0, q0, c_lds); break;
+            case 5: count_row_regs<5>(row, n, k0, q0, c_lds); break;
+            case 6: count_row_regs<6>(row, n, k0, q0, c_lds); break;
+            case 7: count_row_regs<7>(row, n, k0, q0, c_lds); break;
+            default: count_row_regs<8>(row, n, k0, q0, c_lds); break;
+            }
+            __syncthreads();
+            for (int i = threadIdx.x; i <= np; i += kBlock) out[i] = (accumulate ? out[i] : 0) + (int64_t)c_lds[i];
+            __syncthreads();
+            continue;
+        }
         auto count = [&](int32_t c) {
             int32_t slot = -1;
             if (c == -1) {
@@ -355,7 +475,7 @@ __global__ void __launch_bounds__(kBlock) cause_kernel(const int32_t *__restrict
         };
         // 16-byte loads over the row's aligned middle (four samples a lane),
         // single samples before and after it
-        const int32_t head = min(n, (int32_t)(((16u - ((uint32_t)(uintptr_t)row & 15u)) & 15u) / 4u));
+        const int32_t head = aligned_head(row, n);
         const int32_t n_vec = (n - head) / 4;
         const int4 *vrow = reinterpret_cast<const int4 *>(row + head);
         for (int32_t s = threadIdx.x; s < head; s += kBlock) count(row[s]);
@@ -539,8 +659,8 @@ hipError_t launch_event_mean(const double *realized, int32_t n, int32_t ld, int3
 hipError_t launch_event_quantiles(const double *realized, int32_t n, int32_t ld, int32_t E, const QuantileArgs &qa,
                                   double *out, double *mean, hipStream_t stream) {
     if (E > 0)
-        hipLaunchKernelGGL(quantile_kernel, dim3((unsigned)E), dim3(kBlock), 0, stream, realized, n, (uint32_t)ld, E, qa,
-                           out, mean);
+        hipLaunchKernelGGL(mean ? quantile_kernel<true> : quantile_kernel<false>, dim3((unsigned)E), dim3(kBlock), 0,
+                           stream, realized, n, (uint32_t)ld, E, qa, out, mean);
     return hipGetLastError();
 }
 

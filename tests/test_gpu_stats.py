@@ -95,6 +95,26 @@ def test_stats_synthetic_match_oracle(gpu, n: int) -> None:
     np.testing.assert_array_equal(counts, oc)
 
 
+@pytest.mark.parametrize("n,pad", [(3001, 5), (4096, 0), (77, 1)])
+def test_cause_counts_wide_events_and_strays(gpu, n: int, pad: int) -> None:
+    """Events with 1 to 14 predecessors (counted in registers up to 8 distinct sources, in LDS
+    beyond), causes that are no predecessor of their event (counted nowhere), rows at every
+    alignment (odd ld)."""
+    rng = np.random.default_rng(n + pad)
+    dag = D.random_dag(rng, 60, max_preds=14, duplicates=True)
+    P = capi.Propagator(dag)
+    preds = oracle.pred_sources(dag)
+    assert max(len(set(p)) for p in preds) > 8 and any(0 < len(set(p)) <= 8 for p in preds)
+    r, _ = synthetic(rng, n, P.E, preds)
+    c = np.empty((n, P.E), dtype=np.int32)
+    for e in range(P.E):
+        choices = np.array([-1] + list(preds[e]) + [P.E + 3, e, 2**31 - 1, -2], dtype=np.int32)
+        c[:, e] = rng.choice(choices, n)
+    _, _, counts = run_stats_device(P, r, c, [], pad=pad)
+    _, _, oc = oracle.event_stats(r, c, preds, [])
+    np.testing.assert_array_equal(counts, oc)
+
+
 def test_cause_counts_use_first_position_of_repeated_source(gpu) -> None:
     """Event 2 lists source 0 twice (through two activities): its causes count at position 0."""
     from mc_dagprop_amd.workloads import CONSTANT, DistSpec, FlatDag
