@@ -52,6 +52,21 @@ __device__ __forceinline__ DD dd_add(DD x, DD y) {
     return r;
 }
 
+constexpr int kWaves = kBlock / 64;
+static_assert(kWaves == 4, "block reductions merge four waves");
+
+// Butterfly sum over a wave's 64 lanes in a fixed order: the double-double
+// sum and the plain one.  Every step is commutative (TwoSum's error term is
+// exact, so a pair's lanes compute the same double-double), so all lanes end
+// with the same value.
+__device__ __forceinline__ void wave_sum(DD &acc, double &plain) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        acc = dd_add(acc, DD{__shfl_xor(acc.hi, o), __shfl_xor(acc.lo, o)});
+        plain += __shfl_xor(plain, o);
+    }
+}
+
 // Order-preserving key of a double: -inf < ... < -0 < +0 < ... < +inf (NaN
 // keys lie beyond the infinities' and are never selected).
 __device__ __forceinline__ uint64_t order_key(double v) {
@@ -65,8 +80,8 @@ __device__ __forceinline__ double key_value(uint64_t k) {
 
 __global__ void __launch_bounds__(kBlock) mean_kernel(const double *__restrict__ realized, int32_t n, uint32_t ld,
                                                       int32_t E, double *__restrict__ mean) {
-    __shared__ DD part[kBlock];
-    __shared__ double naive[kBlock];
+    __shared__ DD part[kWaves];
+    __shared__ double naive[kWaves];
     for (int32_t e = blockIdx.x; e < E; e += gridDim.x) {
         const double *row = realized + (uint64_t)(uint32_t)e * ld;
         DD acc{0.0, 0.0};
@@ -85,20 +100,18 @@ __global__ void __launch_bounds__(kBlock) mean_kernel(const double *__restrict__
             visit(v3);
         }
         for (; s < n; s += kBlock) visit(row[s]);
-        part[threadIdx.x] = acc;
-        naive[threadIdx.x] = plain;
-        __syncthreads();
-        for (int w = kBlock / 2; w > 0; w >>= 1) {  // fixed pairing: deterministic
-            if ((int)threadIdx.x < w) {
-                part[threadIdx.x] = dd_add(part[threadIdx.x], part[threadIdx.x + w]);
-                naive[threadIdx.x] += naive[threadIdx.x + w];
-            }
-            __syncthreads();
+        wave_sum(acc, plain);
+        if ((threadIdx.x & 63) == 0) {
+            part[threadIdx.x >> 6] = acc;
+            naive[threadIdx.x >> 6] = plain;
         }
+        __syncthreads();
         if (threadIdx.x == 0) {
-            // A non-finite sum (inf, NaN) carries no rounding to correct.
-            const double total = __builtin_isfinite(naive[0]) ? part[0].hi : naive[0];
-            mean[e] = total / (double)n;
+            // the waves' sums in a fixed order; a non-finite sum (inf, NaN)
+            // carries no rounding to correct
+            const DD sum = dd_add(dd_add(part[0], part[1]), dd_add(part[2], part[3]));
+            const double nv = (naive[0] + naive[1]) + (naive[2] + naive[3]);
+            mean[e] = (__builtin_isfinite(nv) ? sum.hi : nv) / (double)n;
         }
         __syncthreads();
     }
@@ -149,6 +162,16 @@ __device__ __forceinline__ Extremes merge_extremes(Extremes a, const Extremes &b
     return a;
 }
 
+// Butterfly merge of the extremes over a wave's 64 lanes (as wave_sum).
+__device__ __forceinline__ Extremes wave_merge_extremes(Extremes x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const Extremes y{__shfl_xor(x.mn, o), __shfl_xor(x.mx, o), __shfl_xor(x.cmn, o), __shfl_xor(x.cmx, o),
+                         __shfl_xor(x.nans, o)};
+        x = merge_extremes(x, y);
+    }
+    return x;
+}
 // Keys of the infinities.  A NaN's key lies outside [kKeyNegInf, kKeyPosInf]
 // (above it for a positive sign bit, below for a negative one).
 constexpr uint64_t kKeyPosInf = 0xFFF0000000000000ull;
@@ -195,9 +218,9 @@ __global__ void __launch_bounds__(kBlock) quantile_kernel(const double *__restri
                                                           int32_t E, QuantileArgs qa, double *__restrict__ out,
                                                           double *__restrict__ mean) {
     __shared__ uint32_t hist[kMaxRanks][256];
-    __shared__ Extremes ext[kBlock];
-    __shared__ DD part[kBlock];
-    __shared__ double naive[kBlock];
+    __shared__ Extremes ext[kWaves];
+    __shared__ DD part[kWaves];
+    __shared__ double naive[kWaves];
     __shared__ uint64_t prefix[kMaxRanks];  // per rank: its key's digits found so far
     __shared__ int32_t krem[kMaxRanks];     // per rank: its rank among keys with that prefix
     __shared__ int32_t done[kMaxRanks];     // per rank: key known
@@ -217,23 +240,20 @@ __global__ void __launch_bounds__(kBlock) quantile_kernel(const double *__restri
         DD acc{0.0, 0.0};
         double plain = 0.0;
         extremes_pass<kMean>(row, n, x, acc, plain);
-        ext[threadIdx.x] = x;
-        part[threadIdx.x] = acc;
-        naive[threadIdx.x] = plain;
-        __syncthreads();
-        for (int w = kBlock / 2; w > 0; w >>= 1) {  // fixed pairing: deterministic
-            if ((int)threadIdx.x < w) {
-                ext[threadIdx.x] = merge_extremes(ext[threadIdx.x], ext[threadIdx.x + w]);
-                if (kMean) {
-                    part[threadIdx.x] = dd_add(part[threadIdx.x], part[threadIdx.x + w]);
-                    naive[threadIdx.x] += naive[threadIdx.x + w];
-                }
-            }
-            __syncthreads();
+        x = wave_merge_extremes(x);
+        if (kMean) wave_sum(acc, plain);
+        if (lane == 0) {
+            ext[threadIdx.x >> 6] = x;
+            part[threadIdx.x >> 6] = acc;
+            naive[threadIdx.x >> 6] = plain;
         }
-        const Extremes X = ext[0];
-        if (kMean && threadIdx.x == 0)
-            mean[e] = (__builtin_isfinite(naive[0]) ? part[0].hi : naive[0]) / (double)n;
+        __syncthreads();
+        const Extremes X = merge_extremes(merge_extremes(ext[0], ext[1]), merge_extremes(ext[2], ext[3]));
+        if (kMean && threadIdx.x == 0) {
+            const DD sum = dd_add(dd_add(part[0], part[1]), dd_add(part[2], part[3]));
+            const double nv = (naive[0] + naive[1]) + (naive[2] + naive[3]);
+            mean[e] = (__builtin_isfinite(nv) ? sum.hi : nv) / (double)n;
+        }
         const bool has_nan = X.mx > kKeyPosInf || X.mn < kKeyNegInf;
         if ((int)threadIdx.x < R) {
             const int r = threadIdx.x;
@@ -282,9 +302,6 @@ __global__ void __launch_bounds__(kBlock) quantile_kernel(const double *__restri
             const int G = n_groups;
             if (G == 0) break;
             for (int i = threadIdx.x; i < G * 256; i += kBlock) hist[i >> 8][i & 255] = 0;
-            uint64_t pre[kMaxRanks];
-#pragma unroll
-            for (int g = 0; g < kMaxRanks; ++g) pre[g] = (g < G && pass > 0) ? gprefix[g] >> (shift + 8) : 0;
             __syncthreads();
             for (int32_t s = threadIdx.x; s < n; s += kBlock) {
                 const double v = row[s];
@@ -292,9 +309,8 @@ __global__ void __launch_bounds__(kBlock) quantile_kernel(const double *__restri
                 const uint64_t key = order_key(v);
                 const uint32_t digit = (uint32_t)(key >> shift) & 255u;
                 const uint64_t high = pass > 0 ? key >> (shift + 8) : 0;
-#pragma unroll
-                for (int g = 0; g < kMaxRanks; ++g)
-                    if (g < G) wave_hist_add(hist[g], ok && high == pre[g], digit, lane);
+                for (int g = 0; g < G; ++g)  // the groups' prefixes from LDS (broadcast reads)
+                    wave_hist_add(hist[g], ok && high == (pass > 0 ? gprefix[g] >> (shift + 8) : 0), digit, lane);
             }
             __syncthreads();
             if ((int)threadIdx.x < R && !done[threadIdx.x]) {
