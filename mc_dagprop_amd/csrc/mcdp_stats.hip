@@ -524,9 +524,9 @@ __global__ void __launch_bounds__(kBlock) chunk_accumulate_kernel(const double *
                                                                   ChunkStat *__restrict__ st,
                                                                   uint32_t *__restrict__ hist) {
     __shared__ uint32_t h[kMaxChunkBins];
-    __shared__ Extremes ext[kBlock];
-    __shared__ DD part[kBlock];
-    __shared__ double naive[kBlock];
+    __shared__ Extremes ext[kWaves];
+    __shared__ DD part[kWaves];
+    __shared__ double naive[kWaves];
     const int lane = threadIdx.x & 63;
     for (int32_t e = blockIdx.x; e < E; e += gridDim.x) {
         const double *row = realized + (uint64_t)(uint32_t)e * ld;
@@ -536,10 +536,9 @@ __global__ void __launch_bounds__(kBlock) chunk_accumulate_kernel(const double *
         Extremes x{~0ull, 0ull, 0u, 0u, 0u};
         DD acc{0.0, 0.0};
         double plain = 0.0;
-        for (int32_t s0 = 0; s0 < m; s0 += kBlock) {
-            const int32_t s = s0 + (int32_t)threadIdx.x;
-            const bool in = s < m;
-            const double v = in ? row[s] : 0.0;
+        // every lane takes part in each step (the histogram adds are wave-wide);
+        // four loads per lane in flight while a whole block of four fits
+        auto visit = [&](double v, bool in) {
             const bool ok = in && v == v;
             if (in) {
                 acc = dd_add(acc, v);
@@ -554,23 +553,33 @@ __global__ void __launch_bounds__(kBlock) chunk_accumulate_kernel(const double *
             f = f < 0.0 ? 0.0 : f;
             int32_t b = f < (double)n_bins ? (int32_t)f : n_bins - 1;
             wave_hist_add(h, ok, (uint32_t)b, lane);
+        };
+        int32_t s0 = 0;
+        for (; s0 + 4 * kBlock <= m; s0 += 4 * kBlock) {
+            const double *p = row + s0 + threadIdx.x;
+            const double v0 = p[0], v1 = p[kBlock], v2 = p[2 * kBlock], v3 = p[3 * kBlock];
+            visit(v0, true);
+            visit(v1, true);
+            visit(v2, true);
+            visit(v3, true);
         }
-        ext[threadIdx.x] = x;
-        part[threadIdx.x] = acc;
-        naive[threadIdx.x] = plain;
+        for (; s0 < m; s0 += kBlock) {
+            const int32_t s = s0 + (int32_t)threadIdx.x;
+            const bool in = s < m;
+            visit(in ? row[s] : 0.0, in);
+        }
+        x = wave_merge_extremes(x);
+        wave_sum(acc, plain);
+        if (lane == 0) {
+            ext[threadIdx.x >> 6] = x;
+            part[threadIdx.x >> 6] = acc;
+            naive[threadIdx.x >> 6] = plain;
+        }
         __syncthreads();
-        for (int w = kBlock / 2; w > 0; w >>= 1) {  // fixed pairing: deterministic
-            if ((int)threadIdx.x < w) {
-                ext[threadIdx.x] = merge_extremes(ext[threadIdx.x], ext[threadIdx.x + w]);
-                part[threadIdx.x] = dd_add(part[threadIdx.x], part[threadIdx.x + w]);
-                naive[threadIdx.x] += naive[threadIdx.x + w];
-            }
-            __syncthreads();
-        }
         if (threadIdx.x == 0) {
-            DD sum = part[0];
-            double nv = naive[0];
-            Extremes x0 = ext[0];
+            DD sum = dd_add(dd_add(part[0], part[1]), dd_add(part[2], part[3]));
+            double nv = (naive[0] + naive[1]) + (naive[2] + naive[3]);
+            Extremes x0 = merge_extremes(merge_extremes(ext[0], ext[1]), merge_extremes(ext[2], ext[3]));
             if (!first) {
                 const ChunkStat o = st[e];
                 sum = dd_add(DD{o.sum_hi, o.sum_lo}, sum);
