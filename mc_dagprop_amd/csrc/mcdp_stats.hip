@@ -172,43 +172,156 @@ __device__ __forceinline__ Extremes wave_merge_extremes(Extremes x) {
     }
     return x;
 }
-// Keys of the infinities.  A NaN's key lies outside [kKeyNegInf, kKeyPosInf]
-// (above it for a positive sign bit, below for a negative one).
-constexpr uint64_t kKeyPosInf = 0xFFF0000000000000ull;
-constexpr uint64_t kKeyNegInf = 0x000FFFFFFFFFFFFFull;
-
-// The quantile kernel's first pass over a row: this thread's share of the key
-// extremes with their multiplicities and, with kSum, of the sum.  NaNs are not
-// set apart: their keys fall outside the infinities' (a row holds a NaN iff
-// its extremes do), and such a row's results are NaN whatever else it holds.
-// Four loads per lane are in flight per iteration; each thread still visits
-// its samples tid, tid + kBlock, ... in that order, so the sum's rounding does
-// not depend on the unrolling.
+// The quantile kernel's first pass over a row: this thread's share of the
+// extremes with their multiplicities, of the plain sum and, with kSum, of the
+// double-double sum.  Extremes are taken over values: NaNs are left out (a
+// row holding one has a NaN plain sum, which the caller then checks), and -0
+// and +0 count as one value, keyed as -0 for a smallest value and as +0 for a
+// largest one, so that the key range brackets both zeros.  Four loads per
+// lane are in flight per iteration, and the four values are reduced to their
+// smallest and largest before they meet the running extremes.  Each thread
+// visits its samples tid, tid + kBlock, ... in that order, so the sums'
+// rounding does not depend on the unrolling.
 template <bool kSum>
 __device__ __forceinline__ void extremes_pass(const double *__restrict__ row, int32_t n, Extremes &x, DD &acc,
                                               double &plain) {
-    auto visit = [&](double v) {
-        if (kSum) {
-            acc = dd_add(acc, v);
-            plain += v;
-        }
-        const uint64_t key = order_key(v);
-        const bool lt = key < x.mn;
-        x.mn = lt ? key : x.mn;
-        x.cmn = (lt ? 0u : x.cmn) + (key == x.mn ? 1u : 0u);
-        const bool gt = key > x.mx;
-        x.mx = gt ? key : x.mx;
-        x.cmx = (gt ? 0u : x.cmx) + (key == x.mx ? 1u : 0u);
+    double mn = __builtin_inf(), mx = -__builtin_inf();
+    uint32_t cmn = 0, cmx = 0;
+    auto sum = [&](double v) {
+        if (kSum) acc = dd_add(acc, v);
+        plain += v;
     };
     int32_t s = threadIdx.x;
     for (; s + 3 * kBlock < n; s += 4 * kBlock) {
         const double v0 = row[s], v1 = row[s + kBlock], v2 = row[s + 2 * kBlock], v3 = row[s + 3 * kBlock];
-        visit(v0);
-        visit(v1);
-        visit(v2);
-        visit(v3);
+        sum(v0);
+        sum(v1);
+        sum(v2);
+        sum(v3);
+        // fmin / fmax pass over a NaN operand; four NaNs give a NaN, which the
+        // compares below never take
+        const double lo = __builtin_fmin(__builtin_fmin(v0, v1), __builtin_fmin(v2, v3));
+        const double hi = __builtin_fmax(__builtin_fmax(v0, v1), __builtin_fmax(v2, v3));
+        const bool lt = lo < mn;
+        mn = lt ? lo : mn;
+        cmn = (lt ? 0u : cmn) + (v0 == mn ? 1u : 0u) + (v1 == mn ? 1u : 0u) + (v2 == mn ? 1u : 0u) +
+              (v3 == mn ? 1u : 0u);
+        const bool gt = hi > mx;
+        mx = gt ? hi : mx;
+        cmx = (gt ? 0u : cmx) + (v0 == mx ? 1u : 0u) + (v1 == mx ? 1u : 0u) + (v2 == mx ? 1u : 0u) +
+              (v3 == mx ? 1u : 0u);
     }
-    for (; s < n; s += kBlock) visit(row[s]);
+    for (; s < n; s += kBlock) {
+        const double v = row[s];
+        sum(v);
+        const bool lt = v < mn;
+        mn = lt ? v : mn;
+        cmn = (lt ? 0u : cmn) + (v == mn ? 1u : 0u);
+        const bool gt = v > mx;
+        mx = gt ? v : mx;
+        cmx = (gt ? 0u : cmx) + (v == mx ? 1u : 0u);
+    }
+    x.mn = order_key(mn == 0.0 ? -0.0 : mn);
+    x.mx = order_key(mx == 0.0 ? 0.0 : mx);
+    x.cmn = cmn;
+    x.cmx = cmx;
+    x.nans = 0;
+}
+
+constexpr int kValueBins = kMaxRanks * 256;  // the radix histograms' LDS, reused
+constexpr int kGather = kValueBins / 2;      // keys of one bin gathered into the same LDS
+static_assert(kValueBins % kBlock == 0, "value bins split evenly over the block");
+
+// Rows with ranks outside the masses of their extremes, before any radix
+// pass: one pass counts the row in kValueBins bins of equal width over
+// [smallest, largest] value, bin floor((v - smallest) * kValueBins / range).
+// The bin is monotone in v, so the k-th smallest sample lies in the bin where
+// the running count passes k.  A second pass per distinct bin gathers that
+// bin's keys into LDS (at most kGather), where each rank in it is selected
+// exactly by counting, for every gathered key, the keys below and equal to it.
+// Ranks in bins holding more than kGather keys, and rows whose range is not a
+// positive finite double with a finite bin scale, are left to the radix
+// passes (done[r] stays 0).  Rows with NaNs never get here (every rank done).
+__device__ void value_bin_select(const double *__restrict__ row, int32_t n, const Extremes &X, int R, int lane,
+                                 uint32_t *vh, uint32_t *seg, uint64_t *prefix, const int32_t *krem, int32_t *done,
+                                 int32_t *vbin, int32_t *vloc, int32_t *vcnt, uint32_t &cand_n) {
+    bool open = false;
+    for (int r = 0; r < R; ++r) open |= done[r] == 0;
+    const double lo = key_value(X.mn), hi = key_value(X.mx);
+    const double span = hi - lo;
+    const double scale = (double)kValueBins / span;
+    if (!open || !(span > 0.0 && span < __builtin_inf() && scale < __builtin_inf()))
+        return;  // uniform: LDS and X agree in all threads
+    auto bin_of = [&](double v) {
+        const double f = (v - lo) * scale;
+        return f < (double)kValueBins ? (int32_t)f : kValueBins - 1;
+    };
+    // four loads per lane in flight
+    auto scan = [&](auto &&visit) {
+        int32_t s = threadIdx.x;
+        for (; s + 3 * kBlock < n; s += 4 * kBlock) {
+            double v[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[i] = row[s + i * kBlock];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) visit(v[i]);
+        }
+        for (; s < n; s += kBlock) visit(row[s]);
+    };
+    for (int i = threadIdx.x; i < kValueBins; i += kBlock) vh[i] = 0;
+    __syncthreads();
+    scan([&](double v) { wave_hist_add(vh, true, (uint32_t)bin_of(v), lane); });
+    __syncthreads();
+    constexpr int per = kValueBins / kBlock;
+    uint32_t t = 0;
+    for (int i = 0; i < per; ++i) t += vh[threadIdx.x * per + i];
+    seg[threadIdx.x] = t;
+    __syncthreads();
+    if ((int)threadIdx.x < R && !done[threadIdx.x]) {
+        const int r = threadIdx.x;
+        int32_t k = krem[r];  // its rank in the row (k < n)
+        int g = 0;
+        while (k >= (int32_t)seg[g]) k -= (int32_t)seg[g++];
+        int b = g * per;
+        while (k >= (int32_t)vh[b]) k -= (int32_t)vh[b++];
+        vbin[r] = b;
+        vloc[r] = k;
+        vcnt[r] = (int32_t)vh[b];
+    }
+    __syncthreads();
+    uint64_t *cand = reinterpret_cast<uint64_t *>(vh);
+    for (int r = 0; r < R; ++r) {
+        if (done[r] || vcnt[r] > kGather) continue;  // uniform (LDS, after a barrier)
+        const int32_t b = vbin[r];
+        if (threadIdx.x == 0) cand_n = 0;
+        __syncthreads();
+        scan([&](double v) {
+            const bool m = bin_of(v) == b;
+            const uint64_t mm = __ballot(m);
+            if (mm == 0) return;
+            const int leader = __ffsll((long long)mm) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&cand_n, (uint32_t)__popcll(mm));
+            base = __shfl(base, leader);
+            if (m) cand[base + (uint32_t)__popcll(mm & ((1ull << lane) - 1ull))] = order_key(v);
+        });
+        __syncthreads();
+        const int32_t c = (int32_t)cand_n;  // == vcnt[r]
+        for (int32_t i = threadIdx.x; i < c; i += kBlock) {
+            const uint64_t ki = cand[i];
+            int32_t below = 0, same = 0;
+            for (int32_t j = 0; j < c; ++j) {
+                const uint64_t kj = cand[j];
+                below += kj < ki ? 1 : 0;
+                same += kj == ki ? 1 : 0;
+            }
+            for (int q = 0; q < R; ++q)  // equal keys write the same value
+                if (!done[q] && vbin[q] == b && below <= vloc[q] && vloc[q] < below + same) prefix[q] = ki;
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < R && !done[threadIdx.x] && vbin[threadIdx.x] == b) done[threadIdx.x] = 1;
+        __syncthreads();
+    }
 }
 
 // Quantiles of each row; with kMean, its first pass also sums the row (the
@@ -228,6 +341,11 @@ __global__ void __launch_bounds__(kBlock) quantile_kernel(const double *__restri
     __shared__ int32_t group_of[kMaxRanks];
     __shared__ int32_t n_groups;
     __shared__ int32_t first_pass;
+    __shared__ uint32_t seg[kBlock];       // value bins: counts per run of kValueBins / kBlock bins
+    __shared__ int32_t vbin[kMaxRanks];    // per rank: its value bin, rank inside it, the bin's count
+    __shared__ int32_t vloc[kMaxRanks];
+    __shared__ int32_t vcnt[kMaxRanks];
+    __shared__ uint32_t cand_n;
     const int R = qa.n_ranks;
     const int lane = threadIdx.x & 63;
     for (int32_t e = blockIdx.x; e < E; e += gridDim.x) {
@@ -241,7 +359,12 @@ __global__ void __launch_bounds__(kBlock) quantile_kernel(const double *__restri
         double plain = 0.0;
         extremes_pass<kMean>(row, n, x, acc, plain);
         x = wave_merge_extremes(x);
-        if (kMean) wave_sum(acc, plain);
+        if (kMean) {
+            wave_sum(acc, plain);
+        } else {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) plain += __shfl_xor(plain, o);
+        }
         if (lane == 0) {
             ext[threadIdx.x >> 6] = x;
             part[threadIdx.x >> 6] = acc;
@@ -249,12 +372,19 @@ __global__ void __launch_bounds__(kBlock) quantile_kernel(const double *__restri
         }
         __syncthreads();
         const Extremes X = merge_extremes(merge_extremes(ext[0], ext[1]), merge_extremes(ext[2], ext[3]));
+        const double nv = (naive[0] + naive[1]) + (naive[2] + naive[3]);
         if (kMean && threadIdx.x == 0) {
             const DD sum = dd_add(dd_add(part[0], part[1]), dd_add(part[2], part[3]));
-            const double nv = (naive[0] + naive[1]) + (naive[2] + naive[3]);
             mean[e] = (__builtin_isfinite(nv) ? sum.hi : nv) / (double)n;
         }
-        const bool has_nan = X.mx > kKeyPosInf || X.mn < kKeyNegInf;
+        // A NaN sample makes the plain sum NaN, and so does +inf + -inf: only
+        // then is the row read again for NaNs (nv is the same in every thread).
+        bool has_nan = false;
+        if (nv != nv) {
+            int found = 0;
+            for (int32_t s = threadIdx.x; s < n; s += kBlock) found |= row[s] != row[s];
+            has_nan = __syncthreads_or(found) != 0;
+        }
         if ((int)threadIdx.x < R) {
             const int r = threadIdx.x;
             const int32_t k = qa.rank[r];
@@ -279,6 +409,8 @@ __global__ void __launch_bounds__(kBlock) quantile_kernel(const double *__restri
         const int p0 = first_pass;
         if ((int)threadIdx.x < R && !done[threadIdx.x] && p0 > 0)
             prefix[threadIdx.x] = X.mn & ~(~0ull >> (8 * p0));
+        __syncthreads();
+        value_bin_select(row, n, X, R, lane, &hist[0][0], seg, prefix, krem, done, vbin, vloc, vcnt, cand_n);
         for (int pass = p0; pass < 8; ++pass) {
             const int shift = 56 - 8 * pass;
             __syncthreads();
@@ -303,15 +435,26 @@ __global__ void __launch_bounds__(kBlock) quantile_kernel(const double *__restri
             if (G == 0) break;
             for (int i = threadIdx.x; i < G * 256; i += kBlock) hist[i >> 8][i & 255] = 0;
             __syncthreads();
-            for (int32_t s = threadIdx.x; s < n; s += kBlock) {
-                const double v = row[s];
+            auto visit = [&](double v) {
                 const bool ok = v == v;
                 const uint64_t key = order_key(v);
                 const uint32_t digit = (uint32_t)(key >> shift) & 255u;
                 const uint64_t high = pass > 0 ? key >> (shift + 8) : 0;
                 for (int g = 0; g < G; ++g)  // the groups' prefixes from LDS (broadcast reads)
                     wave_hist_add(hist[g], ok && high == (pass > 0 ? gprefix[g] >> (shift + 8) : 0), digit, lane);
+            };
+            // Few rows need these passes, and each runs in one workgroup while
+            // the others have finished: eight loads per lane in flight keep
+            // such a row from being bound by memory latency.
+            int32_t s = threadIdx.x;
+            for (; s + 7 * kBlock < n; s += 8 * kBlock) {
+                double v[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) v[i] = row[s + i * kBlock];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) visit(v[i]);
             }
+            for (; s < n; s += kBlock) visit(row[s]);
             __syncthreads();
             if ((int)threadIdx.x < R && !done[threadIdx.x]) {
                 const int r = threadIdx.x;

@@ -95,6 +95,43 @@ def test_stats_synthetic_match_oracle(gpu, n: int) -> None:
     np.testing.assert_array_equal(counts, oc)
 
 
+def crafted_rows(rng: np.random.Generator, n: int) -> np.ndarray:
+    """Rows whose quantiles lie outside the masses of their extremes, for each route of the selection:
+    value bins of ordinary spread, a middle value held by more samples than one bin gathers (radix
+    passes), a range of three subnormals (bin scale overflows: radix passes), ranges of infinite width
+    (radix passes), signed zeros among positives, heavy ties, and sums near overflow."""
+    third = n // 3
+    cols = [
+        rng.normal(100.0, 30.0, n),
+        np.concatenate([rng.normal(-5.0, 1.0, third), np.full(n - 2 * third, 0.5), rng.normal(5.0, 1.0, third)]),
+        rng.choice([5e-324, 1e-323, 1.5e-323], n),
+        np.where(rng.random(n) < 0.5, rng.normal(0.0, 1.0, n), rng.choice([-1.7e308, 1.7e308], n)),
+        np.concatenate([rng.normal(0.0, 1.0, n - 2), [np.inf, -np.inf]]),
+        rng.choice([-0.0, 0.0, 1e-300, 2.0, 3.0], n),
+        np.round(rng.normal(0.0, 4.0, n)),
+        rng.uniform(-1e300, 1e300, n),
+        np.exp(rng.normal(0.0, 30.0, n)),
+    ]
+    r = np.stack([rng.permutation(c) for c in cols], axis=1)
+    return r
+
+
+@pytest.mark.parametrize("n", [5000, 100_003])
+def test_quantiles_value_bins_and_radix_fallback(gpu, n: int) -> None:
+    rng = np.random.default_rng(n)
+    r = crafted_rows(rng, n)
+    dag = D.parity_long_chain(r.shape[1] - 1)
+    P = capi.Propagator(dag)
+    assert P.E == r.shape[1]
+    q = [0.001, 0.25, 0.5, 0.75, 0.9, 0.99, 0.999]
+    c = np.full(r.shape, -1, dtype=np.int32)
+    mean, quant, _ = run_stats_device(P, r, c, q, pad=1)
+    om, oq, _ = oracle.event_stats(r, c, oracle.pred_sources(dag), q)
+    assert_quantiles_equal(quant, oq)
+    finite = np.isfinite(om)
+    np.testing.assert_array_equal(bits(mean[finite]), bits(om[finite]))
+
+
 @pytest.mark.parametrize("n,pad", [(3001, 5), (4096, 0), (77, 1)])
 def test_cause_counts_wide_events_and_strays(gpu, n: int, pad: int) -> None:
     """Events with 1 to 14 predecessors (counted in registers up to 8 distinct sources, in LDS
