@@ -172,6 +172,12 @@ __device__ __forceinline__ Extremes wave_merge_extremes(Extremes x) {
     }
     return x;
 }
+// The quantile kernel's workgroup: rows whose ranks need passes beyond the
+// first are streamed again by one workgroup while the others run, so a
+// larger one keeps more of such a row in flight.
+constexpr int kQBlock = 512;
+constexpr int kQWaves = kQBlock / 64;
+
 // The quantile kernel's first pass over a row: this thread's share of the
 // extremes with their multiplicities, of the plain sum and, with kSum, of the
 // double-double sum.  Extremes are taken over values: NaNs are left out (a
@@ -180,9 +186,9 @@ __device__ __forceinline__ Extremes wave_merge_extremes(Extremes x) {
 // largest one, so that the key range brackets both zeros.  Four loads per
 // lane are in flight per iteration, and the four values are reduced to their
 // smallest and largest before they meet the running extremes.  Each thread
-// visits its samples tid, tid + kBlock, ... in that order, so the sums'
+// visits its samples tid, tid + kB, ... in that order, so the sums'
 // rounding does not depend on the unrolling.
-template <bool kSum>
+template <int kB, bool kSum>
 __device__ __forceinline__ void extremes_pass(const double *__restrict__ row, int32_t n, Extremes &x, DD &acc,
                                               double &plain) {
     double mn = __builtin_inf(), mx = -__builtin_inf();
@@ -192,8 +198,8 @@ __device__ __forceinline__ void extremes_pass(const double *__restrict__ row, in
         plain += v;
     };
     int32_t s = threadIdx.x;
-    for (; s + 3 * kBlock < n; s += 4 * kBlock) {
-        const double v0 = row[s], v1 = row[s + kBlock], v2 = row[s + 2 * kBlock], v3 = row[s + 3 * kBlock];
+    for (; s + 3 * kB < n; s += 4 * kB) {
+        const double v0 = row[s], v1 = row[s + kB], v2 = row[s + 2 * kB], v3 = row[s + 3 * kB];
         sum(v0);
         sum(v1);
         sum(v2);
@@ -211,7 +217,7 @@ __device__ __forceinline__ void extremes_pass(const double *__restrict__ row, in
         cmx = (gt ? 0u : cmx) + (v0 == mx ? 1u : 0u) + (v1 == mx ? 1u : 0u) + (v2 == mx ? 1u : 0u) +
               (v3 == mx ? 1u : 0u);
     }
-    for (; s < n; s += kBlock) {
+    for (; s < n; s += kB) {
         const double v = row[s];
         sum(v);
         const bool lt = v < mn;
@@ -230,7 +236,6 @@ __device__ __forceinline__ void extremes_pass(const double *__restrict__ row, in
 
 constexpr int kValueBins = kMaxRanks * 256;  // the radix histograms' LDS, reused
 constexpr int kGather = kValueBins / 2;      // keys of one bin gathered into the same LDS
-static_assert(kValueBins % kBlock == 0, "value bins split evenly over the block");
 
 // Rows with ranks outside the masses of their extremes, before any radix
 // pass: one pass counts the row in kValueBins bins of equal width over
@@ -242,6 +247,7 @@ static_assert(kValueBins % kBlock == 0, "value bins split evenly over the block"
 // Ranks in bins holding more than kGather keys, and rows whose range is not a
 // positive finite double with a finite bin scale, are left to the radix
 // passes (done[r] stays 0).  Rows with NaNs never get here (every rank done).
+template <int kB>
 __device__ void value_bin_select(const double *__restrict__ row, int32_t n, const Extremes &X, int R, int lane,
                                  uint32_t *vh, uint32_t *seg, uint64_t *prefix, const int32_t *krem, int32_t *done,
                                  int32_t *vbin, int32_t *vloc, int32_t *vcnt, uint32_t &cand_n) {
@@ -259,20 +265,21 @@ __device__ void value_bin_select(const double *__restrict__ row, int32_t n, cons
     // four loads per lane in flight
     auto scan = [&](auto &&visit) {
         int32_t s = threadIdx.x;
-        for (; s + 3 * kBlock < n; s += 4 * kBlock) {
+        for (; s + 3 * kB < n; s += 4 * kB) {
             double v[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) v[i] = row[s + i * kBlock];
+            for (int i = 0; i < 4; ++i) v[i] = row[s + i * kB];
 #pragma unroll
             for (int i = 0; i < 4; ++i) visit(v[i]);
         }
-        for (; s < n; s += kBlock) visit(row[s]);
+        for (; s < n; s += kB) visit(row[s]);
     };
-    for (int i = threadIdx.x; i < kValueBins; i += kBlock) vh[i] = 0;
+    for (int i = threadIdx.x; i < kValueBins; i += kB) vh[i] = 0;
     __syncthreads();
     scan([&](double v) { wave_hist_add(vh, true, (uint32_t)bin_of(v), lane); });
     __syncthreads();
-    constexpr int per = kValueBins / kBlock;
+    static_assert(kValueBins % kB == 0, "value bins split evenly over the block");
+    constexpr int per = kValueBins / kB;
     uint32_t t = 0;
     for (int i = 0; i < per; ++i) t += vh[threadIdx.x * per + i];
     seg[threadIdx.x] = t;
@@ -307,7 +314,7 @@ __device__ void value_bin_select(const double *__restrict__ row, int32_t n, cons
         });
         __syncthreads();
         const int32_t c = (int32_t)cand_n;  // == vcnt[r]
-        for (int32_t i = threadIdx.x; i < c; i += kBlock) {
+        for (int32_t i = threadIdx.x; i < c; i += kB) {
             const uint64_t ki = cand[i];
             int32_t below = 0, same = 0;
             for (int32_t j = 0; j < c; ++j) {
@@ -327,13 +334,13 @@ __device__ void value_bin_select(const double *__restrict__ row, int32_t n, cons
 // Quantiles of each row; with kMean, its first pass also sums the row (the
 // mean without a separate read).
 template <bool kMean>
-__global__ void __launch_bounds__(kBlock) quantile_kernel(const double *__restrict__ realized, int32_t n, uint32_t ld,
+__global__ void __launch_bounds__(kQBlock) quantile_kernel(const double *__restrict__ realized, int32_t n, uint32_t ld,
                                                           int32_t E, QuantileArgs qa, double *__restrict__ out,
                                                           double *__restrict__ mean) {
     __shared__ uint32_t hist[kMaxRanks][256];
-    __shared__ Extremes ext[kWaves];
-    __shared__ DD part[kWaves];
-    __shared__ double naive[kWaves];
+    __shared__ Extremes ext[kQWaves];
+    __shared__ DD part[kQWaves];
+    __shared__ double naive[kQWaves];
     __shared__ uint64_t prefix[kMaxRanks];  // per rank: its key's digits found so far
     __shared__ int32_t krem[kMaxRanks];     // per rank: its rank among keys with that prefix
     __shared__ int32_t done[kMaxRanks];     // per rank: key known
@@ -341,7 +348,7 @@ __global__ void __launch_bounds__(kBlock) quantile_kernel(const double *__restri
     __shared__ int32_t group_of[kMaxRanks];
     __shared__ int32_t n_groups;
     __shared__ int32_t first_pass;
-    __shared__ uint32_t seg[kBlock];       // value bins: counts per run of kValueBins / kBlock bins
+    __shared__ uint32_t seg[kQBlock];       // value bins: counts per run of kValueBins / kQBlock bins
     __shared__ int32_t vbin[kMaxRanks];    // per rank: its value bin, rank inside it, the bin's count
     __shared__ int32_t vloc[kMaxRanks];
     __shared__ int32_t vcnt[kMaxRanks];
@@ -357,7 +364,7 @@ __global__ void __launch_bounds__(kBlock) quantile_kernel(const double *__restri
         Extremes x{~0ull, 0ull, 0u, 0u, 0u};
         DD acc{0.0, 0.0};
         double plain = 0.0;
-        extremes_pass<kMean>(row, n, x, acc, plain);
+        extremes_pass<kQBlock, kMean>(row, n, x, acc, plain);
         x = wave_merge_extremes(x);
         if (kMean) {
             wave_sum(acc, plain);
@@ -371,10 +378,16 @@ __global__ void __launch_bounds__(kBlock) quantile_kernel(const double *__restri
             naive[threadIdx.x >> 6] = plain;
         }
         __syncthreads();
-        const Extremes X = merge_extremes(merge_extremes(ext[0], ext[1]), merge_extremes(ext[2], ext[3]));
-        const double nv = (naive[0] + naive[1]) + (naive[2] + naive[3]);
+        // the waves' results in wave order
+        Extremes X = ext[0];
+        double nv = naive[0];
+        for (int w = 1; w < kQWaves; ++w) {
+            X = merge_extremes(X, ext[w]);
+            nv += naive[w];
+        }
         if (kMean && threadIdx.x == 0) {
-            const DD sum = dd_add(dd_add(part[0], part[1]), dd_add(part[2], part[3]));
+            DD sum = part[0];
+            for (int w = 1; w < kQWaves; ++w) sum = dd_add(sum, part[w]);
             mean[e] = (__builtin_isfinite(nv) ? sum.hi : nv) / (double)n;
         }
         // A NaN sample makes the plain sum NaN, and so does +inf + -inf: only
@@ -382,7 +395,7 @@ __global__ void __launch_bounds__(kBlock) quantile_kernel(const double *__restri
         bool has_nan = false;
         if (nv != nv) {
             int found = 0;
-            for (int32_t s = threadIdx.x; s < n; s += kBlock) found |= row[s] != row[s];
+            for (int32_t s = threadIdx.x; s < n; s += kQBlock) found |= row[s] != row[s];
             has_nan = __syncthreads_or(found) != 0;
         }
         if ((int)threadIdx.x < R) {
@@ -410,7 +423,7 @@ __global__ void __launch_bounds__(kBlock) quantile_kernel(const double *__restri
         if ((int)threadIdx.x < R && !done[threadIdx.x] && p0 > 0)
             prefix[threadIdx.x] = X.mn & ~(~0ull >> (8 * p0));
         __syncthreads();
-        value_bin_select(row, n, X, R, lane, &hist[0][0], seg, prefix, krem, done, vbin, vloc, vcnt, cand_n);
+        value_bin_select<kQBlock>(row, n, X, R, lane, &hist[0][0], seg, prefix, krem, done, vbin, vloc, vcnt, cand_n);
         for (int pass = p0; pass < 8; ++pass) {
             const int shift = 56 - 8 * pass;
             __syncthreads();
@@ -433,7 +446,7 @@ __global__ void __launch_bounds__(kBlock) quantile_kernel(const double *__restri
             __syncthreads();
             const int G = n_groups;
             if (G == 0) break;
-            for (int i = threadIdx.x; i < G * 256; i += kBlock) hist[i >> 8][i & 255] = 0;
+            for (int i = threadIdx.x; i < G * 256; i += kQBlock) hist[i >> 8][i & 255] = 0;
             __syncthreads();
             auto visit = [&](double v) {
                 const bool ok = v == v;
@@ -447,14 +460,14 @@ __global__ void __launch_bounds__(kBlock) quantile_kernel(const double *__restri
             // the others have finished: eight loads per lane in flight keep
             // such a row from being bound by memory latency.
             int32_t s = threadIdx.x;
-            for (; s + 7 * kBlock < n; s += 8 * kBlock) {
+            for (; s + 7 * kQBlock < n; s += 8 * kQBlock) {
                 double v[8];
 #pragma unroll
-                for (int i = 0; i < 8; ++i) v[i] = row[s + i * kBlock];
+                for (int i = 0; i < 8; ++i) v[i] = row[s + i * kQBlock];
 #pragma unroll
                 for (int i = 0; i < 8; ++i) visit(v[i]);
             }
-            for (; s < n; s += kBlock) visit(row[s]);
+            for (; s < n; s += kQBlock) visit(row[s]);
             __syncthreads();
             if ((int)threadIdx.x < R && !done[threadIdx.x]) {
                 const int r = threadIdx.x;
@@ -827,7 +840,7 @@ hipError_t launch_event_mean(const double *realized, int32_t n, int32_t ld, int3
 hipError_t launch_event_quantiles(const double *realized, int32_t n, int32_t ld, int32_t E, const QuantileArgs &qa,
                                   double *out, double *mean, hipStream_t stream) {
     if (E > 0)
-        hipLaunchKernelGGL(mean ? quantile_kernel<true> : quantile_kernel<false>, dim3((unsigned)E), dim3(kBlock), 0,
+        hipLaunchKernelGGL(mean ? quantile_kernel<true> : quantile_kernel<false>, dim3((unsigned)E), dim3(kQBlock), 0,
                            stream, realized, n, (uint32_t)ld, E, qa, out, mean);
     return hipGetLastError();
 }
