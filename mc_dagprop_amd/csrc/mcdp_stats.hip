@@ -172,6 +172,7 @@ __device__ __forceinline__ Extremes wave_merge_extremes(Extremes x) {
     }
     return x;
 }
+
 // The quantile kernel's workgroup: rows whose ranks need passes beyond the
 // first are streamed again by one workgroup while the others run, so a
 // larger one keeps more of such a row in flight.
