@@ -172,7 +172,11 @@ int mcdp_batch_run_device(mcdp_batch *batch, const int32_t *const *seeds_dev, co
  * the graph replayed; every replay recomputes every output.  A graph holds the
  * descriptors' device address: the next prepare of the batch changes what its
  * replays compute, and may move the descriptors (then the graph must not be
- * replayed again). */
+ * replayed again).  The descriptors also point into each member's device
+ * plan, which mcdp_propagator_set_times, _configure and _autotune free: after
+ * any of them on a member, launch fails with MCDP_E_INVALID until the batch is
+ * prepared again, and a graph captured earlier must not be replayed (capture
+ * it again). */
 int mcdp_batch_prepare_device(mcdp_batch *batch, const int32_t *const *seeds_dev, const int64_t *n, const int64_t *ld,
                               double *const *realized_dev, double *const *durations_dev, int32_t *const *cause_dev);
 int mcdp_batch_launch(mcdp_batch *batch, void *stream);
@@ -362,6 +366,14 @@ int mcdp_debug_reverse_tiles(mcdp_propagator *prop, int64_t max_tiles);
  * step, every event a wave per group of 64 samples).  Results do not depend on
  * the route; tests use it to run both. */
 int mcdp_debug_tail_mode(mcdp_propagator *prop, int32_t mode);
+/* The serving footprint rule, host only: for a request of n samples over a
+ * context of n_events events and n_preds predecessor entries with n_q
+ * quantiles, writes chunk = 0 and the request's device bytes when it fits
+ * request_cap whole, else the samples per chunk and the chunked request's
+ * bytes (<= request_cap); fails with MCDP_E_NOMEM when not even chunks of 128
+ * samples fit, as mcdp_server_submit does. */
+int mcdp_debug_serve_chunk(int64_t n, int64_t n_events, int64_t n_preds, int32_t n_q, int64_t request_cap,
+                           int64_t *chunk, int64_t *bytes);
 
 #ifdef __cplusplus
 }

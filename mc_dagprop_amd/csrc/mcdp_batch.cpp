@@ -22,8 +22,10 @@ struct mcdp_batch {
     void *dev = nullptr;  // WalkParams[plans], item_dag[], item_tile[], counter
     size_t dev_bytes = 0;
     int device = -1;
-    // The prepared launch (mcdp_batch_prepare_device): descriptors in `dev`.
+    // The prepared launch (mcdp_batch_prepare_device): descriptors in `dev`,
+    // and the generation of each member's device copies they point into.
     bool prepared = false;
+    std::vector<uint64_t> gens;
     WalkBatchParams bp{};
     WalkVariant v{};
     size_t lds = 0;
@@ -91,6 +93,7 @@ int mcdp_batch_prepare_device(mcdp_batch *b, const int32_t *const *seeds_dev, co
     // Per-plan launch parameters; the fast variant only if every plan allows it.
     std::vector<WalkParams> params(D);
     std::vector<int64_t> tiles(D), cost(D);
+    std::vector<uint64_t> gens(D);
     bool all_fast = true;
     size_t lds = 0;
     const int64_t tile = (int64_t)kLanes * b->spl;
@@ -107,6 +110,7 @@ int mcdp_batch_prepare_device(mcdp_batch *b, const int32_t *const *seeds_dev, co
         DeviceCopy *dc = nullptr;
         int rc = ensure_device(P, dc);
         if (rc) return rc;
+        gens[d] = P.generation.load();
         WalkParams p = dc->base;
         p.seeds = seeds_dev[d];
         p.dur_in = nullptr;
@@ -179,6 +183,7 @@ int mcdp_batch_prepare_device(mcdp_batch *b, const int32_t *const *seeds_dev, co
     b->v = v;
     b->lds = lds;
     b->grid = (int)std::min<int64_t>(n_items, (int64_t)cus * bpc);
+    b->gens = std::move(gens);
     b->prepared = true;
     return MCDP_OK;
 }
@@ -186,6 +191,15 @@ int mcdp_batch_prepare_device(mcdp_batch *b, const int32_t *const *seeds_dev, co
 int mcdp_batch_launch(mcdp_batch *b, void *stream) {
     if (!b) return fail(MCDP_E_INVALID, "batch is null");
     if (!b->prepared) return fail(MCDP_E_INVALID, "no prepared launch: call mcdp_batch_prepare_device first");
+    // set_times / configure / autotune of a member free the plan the prepared
+    // descriptors point into: such a launch (or a graph captured from it) is
+    // refused until the batch is prepared again
+    for (size_t d = 0; d < b->props.size(); ++d)
+        if (b->props[d]->generation.load() != b->gens[d]) {
+            b->prepared = false;
+            return fail(MCDP_E_INVALID, "a propagator of the batch changed its times or shape since "
+                                        "mcdp_batch_prepare_device: prepare the batch again (and re-capture graphs)");
+        }
     if (b->bp.n_items == 0) return MCDP_OK;
     HIP_TRY(launch_walk_batch(b->bp, b->v, b->waves, b->grid, b->lds, static_cast<hipStream_t>(stream)),
             "batch kernel launch");

@@ -329,6 +329,137 @@ int ensure_workspace(DeviceCopy &d, size_t bytes) {
 
 }  // namespace
 
+int mcdp::reverse_run(mcdp_propagator *prop, const int32_t *seeds_dev, const double *weights_dev,
+                      const double *realized_dev, const double *durations_dev, const int32_t *cause_dev, int64_t n,
+                      int64_t ld, int accumulate, double *objective_dev, double *grad_dmin_dev,
+                      double *grad_earliest_dev, void *given, size_t given_bytes, hipStream_t s) {
+    if (!prop) return fail(MCDP_E_INVALID, "propagator is null");
+    if (n < 0 || (n > 0 && ld < n)) return fail(MCDP_E_INVALID, "need 0 <= n <= ld");
+    if (n > std::numeric_limits<int32_t>::max() - 256 || ld > std::numeric_limits<int32_t>::max())
+        return fail(MCDP_E_INVALID, "n and ld must be below 2^31 - 256 per call");
+    if (!objective_dev || (!grad_dmin_dev && prop->L > 0) || (!grad_earliest_dev && prop->E > 0))
+        return fail(MCDP_E_INVALID, "gradient outputs are null");
+    if (n > 0 && prop->E > 0 && (!weights_dev || !realized_dev || !cause_dev || (!durations_dev && prop->L > 0)))
+        return fail(MCDP_E_INVALID, "weights, realized, durations and cause are required");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return fail(MCDP_E_DEVICE, "no HIP device available: gradients run on the GPU only");
+    std::lock_guard<std::mutex> lock(prop->mu);
+    mcdp_propagator &P = *prop;
+    DeviceCopy *d = nullptr;
+    int rc = MCDP_OK;
+    if (given) {
+        // nothing may be allocated: both plans must be on this device already
+        int device = 0;
+        HIP_TRY(hipGetDevice(&device), "hipGetDevice");
+        for (auto &c : P.dev)
+            if (c.device == device) d = &c;
+        if (!d || !d->rev || !P.rev)
+            return fail(MCDP_E_INVALID, "the propagator's plans are not resident on this device: call "
+                                        "mcdp_reverse_device on it once first (n = 0 is enough)");
+    } else {
+        rc = ensure_device(P, d);
+        if (rc) return rc;
+        rc = ensure_reverse(P, *d);
+        if (rc) return rc;
+    }
+    const int64_t n_unit = (int64_t)P.rev->unit_links.size();
+    if (n > 0 && n_unit > 0 && !seeds_dev)
+        return fail(MCDP_E_INVALID, "seeds are required: a relative link has minimal duration 0");
+    const RevParams &base = d->rev_base;
+    const int spl = P.rev->spl;
+    const int64_t tile = (int64_t)kLanes * spl;
+    const int64_t tiles = (n + tile - 1) / tile;
+    const size_t row_bytes = (size_t)base.n_cols * sizeof(double);
+    // launches cover whole groups of the column reduction, so sums do not depend on the split
+    int64_t max_tiles = std::max<int64_t>(kReduceGroup, (int64_t)(kPartialBudget / row_bytes));
+    if (P.rev_max_tiles > 0) max_tiles = std::min(max_tiles, P.rev_max_tiles);
+    max_tiles = std::min<int64_t>(max_tiles, (int64_t)kReduceGroup * 65535);
+    max_tiles = std::max<int64_t>(kReduceGroup, max_tiles / kReduceGroup * kReduceGroup);
+    const size_t lds = (size_t)base.lds_slots * kSlotBytes * spl;
+    int cus = 0, bpc = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device), "hipDeviceGetAttribute");
+    HIP_TRY(reverse_occupancy(base.waves, lds, base.global_slots > 0, spl, &bpc),
+            "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    if (bpc < 1) return fail(MCDP_E_DEVICE, "reverse kernel cannot be resident with " + std::to_string(lds) + " bytes of LDS");
+    const int64_t resident = (int64_t)cus * bpc;
+    int64_t chunk_tiles = std::min<int64_t>(std::max<int64_t>(tiles, 1), max_tiles);
+    if (n_unit > 0) {  // unit-duration rows for the launch's samples stay within the same budget
+        const int64_t cap = (int64_t)(kPartialBudget / ((size_t)n_unit * tile * 8));
+        if (chunk_tiles > cap)
+            chunk_tiles = std::max<int64_t>(kReduceGroup, cap / kReduceGroup * kReduceGroup);
+    }
+    // Workspace of a launch of ct tiles: partial rows, global slots, unit
+    // rows, the reduction's group sums.
+    struct Layout {
+        size_t o_gslots, o_unit, o_groups, total;
+        int32_t ld_unit;
+    };
+    auto layout = [&](int64_t ct) {
+        Layout w;
+        const int64_t grid_max = std::min<int64_t>(ct, resident);
+        w.o_gslots = align_to(row_bytes * (size_t)ct, 256);
+        const size_t gslot_bytes = (size_t)grid_max * base.global_slots * tile * 2 * sizeof(double);
+        w.o_unit = w.o_gslots + align_to(std::max<size_t>(gslot_bytes, 256), 256);
+        w.ld_unit = (int32_t)(ct * tile);
+        w.o_groups = w.o_unit + align_to(std::max<size_t>((size_t)n_unit * (size_t)w.ld_unit * 8, 256), 256);
+        const size_t group_bytes = row_bytes * (size_t)((ct + kReduceGroup - 1) / kReduceGroup);
+        w.total = w.o_groups + std::max<size_t>(group_bytes, 256);
+        return w;
+    };
+    Layout w = layout(chunk_tiles);
+    void *work_base = nullptr;
+    if (given) {
+        // the largest launch (whole reduction groups) that fits the given workspace
+        while (w.total > given_bytes && chunk_tiles > kReduceGroup) {
+            chunk_tiles = std::max<int64_t>(kReduceGroup, (chunk_tiles - 1) / kReduceGroup * kReduceGroup);
+            w = layout(chunk_tiles);
+        }
+        if (w.total > given_bytes)
+            return fail(MCDP_E_NOMEM, "the reverse walk of this DAG needs " + std::to_string(w.total) +
+                                          " bytes of workspace per launch; the fixed scratch holds " +
+                                          std::to_string(given_bytes) + " bytes");
+        work_base = given;
+    } else {
+        rc = ensure_workspace(*d, w.total);
+        if (rc) return rc;
+        work_base = d->rev_work;
+    }
+    auto *work = static_cast<unsigned char *>(work_base);
+    RevParams p = base;
+    p.weights = weights_dev;
+    p.realized = realized_dev;
+    p.durations = durations_dev;
+    p.cause = cause_dev;
+    p.n = (int32_t)n;
+    p.ld = (int32_t)std::max<int64_t>(ld, 1);
+    p.partial = reinterpret_cast<double *>(work);
+    p.gslots = reinterpret_cast<double *>(work + w.o_gslots);
+    double *unit = reinterpret_cast<double *>(work + w.o_unit);
+    double *groups = reinterpret_cast<double *>(work + w.o_groups);
+    p.unit = unit;
+    p.ld_unit = w.ld_unit;
+    if (tiles == 0 || P.E == 0) {  // no samples: sums are 0 (or unchanged)
+        HIP_TRY(launch_reduce_columns(p.partial, 0, p.n_cols, groups, d->rev_map_off, d->rev_map_col, P.L, P.E,
+                                      accumulate, grad_dmin_dev, grad_earliest_dev, objective_dev, s),
+                "reduce kernel launch");
+        return MCDP_OK;
+    }
+    for (int64_t t0 = 0; t0 < tiles; t0 += chunk_tiles) {
+        p.tile0 = (int32_t)t0;
+        p.n_tiles = (int32_t)std::min<int64_t>(chunk_tiles, tiles - t0);
+        const int grid = (int)std::min<int64_t>(p.n_tiles, resident);
+        HIP_TRY(launch_unit_draws(d->rev_unit_links, (int32_t)n_unit, seeds_dev, (int32_t)(t0 * tile),
+                                  (int32_t)(p.n_tiles * tile), (int32_t)n, base.dists, base.alias, unit, w.ld_unit, s),
+                "unit draw kernel launch");
+        HIP_TRY(launch_reverse(p, spl, grid, lds, s), "reverse kernel launch");
+        HIP_TRY(launch_reduce_columns(p.partial, p.n_tiles, p.n_cols, groups, d->rev_map_off, d->rev_map_col, P.L, P.E,
+                                      accumulate || t0 > 0, grad_dmin_dev, grad_earliest_dev, objective_dev, s),
+                "reduce kernel launch");
+    }
+    return MCDP_OK;
+}
+
 extern "C" {
 
 int mcdp_propagator_set_times(mcdp_propagator *prop, const double *dmin, const double *earliest) {
@@ -362,93 +493,8 @@ int mcdp_reverse_device(mcdp_propagator *prop, const int32_t *seeds_dev, const d
                         const double *realized_dev, const double *durations_dev, const int32_t *cause_dev, int64_t n,
                         int64_t ld, int accumulate, double *objective_dev, double *grad_dmin_dev,
                         double *grad_earliest_dev, void *stream) {
-    if (!prop) return fail(MCDP_E_INVALID, "propagator is null");
-    if (n < 0 || (n > 0 && ld < n)) return fail(MCDP_E_INVALID, "need 0 <= n <= ld");
-    if (n > std::numeric_limits<int32_t>::max() - 256 || ld > std::numeric_limits<int32_t>::max())
-        return fail(MCDP_E_INVALID, "n and ld must be below 2^31 - 256 per call");
-    if (!objective_dev || (!grad_dmin_dev && prop->L > 0) || (!grad_earliest_dev && prop->E > 0))
-        return fail(MCDP_E_INVALID, "gradient outputs are null");
-    if (n > 0 && prop->E > 0 && (!weights_dev || !realized_dev || !cause_dev || (!durations_dev && prop->L > 0)))
-        return fail(MCDP_E_INVALID, "weights, realized, durations and cause are required");
-    int count = 0;
-    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
-        return fail(MCDP_E_DEVICE, "no HIP device available: gradients run on the GPU only");
-    std::lock_guard<std::mutex> lock(prop->mu);
-    mcdp_propagator &P = *prop;
-    DeviceCopy *d = nullptr;
-    int rc = ensure_device(P, d);
-    if (rc) return rc;
-    rc = ensure_reverse(P, *d);
-    if (rc) return rc;
-    const int64_t n_unit = (int64_t)P.rev->unit_links.size();
-    if (n > 0 && n_unit > 0 && !seeds_dev)
-        return fail(MCDP_E_INVALID, "seeds are required: a relative link has minimal duration 0");
-    const auto s = static_cast<hipStream_t>(stream);
-    const RevParams &base = d->rev_base;
-    const int spl = P.rev->spl;
-    const int64_t tile = (int64_t)kLanes * spl;
-    const int64_t tiles = (n + tile - 1) / tile;
-    const size_t row_bytes = (size_t)base.n_cols * sizeof(double);
-    // launches cover whole groups of the column reduction, so sums do not depend on the split
-    int64_t max_tiles = std::max<int64_t>(kReduceGroup, (int64_t)(kPartialBudget / row_bytes));
-    if (P.rev_max_tiles > 0) max_tiles = std::min(max_tiles, P.rev_max_tiles);
-    max_tiles = std::min<int64_t>(max_tiles, (int64_t)kReduceGroup * 65535);
-    max_tiles = std::max<int64_t>(kReduceGroup, max_tiles / kReduceGroup * kReduceGroup);
-    const size_t lds = (size_t)base.lds_slots * kSlotBytes * spl;
-    int cus = 0, bpc = 0;
-    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device), "hipDeviceGetAttribute");
-    HIP_TRY(reverse_occupancy(base.waves, lds, base.global_slots > 0, spl, &bpc),
-            "hipOccupancyMaxActiveBlocksPerMultiprocessor");
-    if (bpc < 1) return fail(MCDP_E_DEVICE, "reverse kernel cannot be resident with " + std::to_string(lds) + " bytes of LDS");
-    const int64_t resident = (int64_t)cus * bpc;
-    int64_t chunk_tiles = std::min<int64_t>(std::max<int64_t>(tiles, 1), max_tiles);
-    if (n_unit > 0) {  // unit-duration rows for the launch's samples stay within the same budget
-        const int64_t cap = (int64_t)(kPartialBudget / ((size_t)n_unit * tile * 8));
-        if (chunk_tiles > cap)
-            chunk_tiles = std::max<int64_t>(kReduceGroup, cap / kReduceGroup * kReduceGroup);
-    }
-    const int64_t grid_max = std::min<int64_t>(chunk_tiles, resident);
-    const size_t o_gslots = align_to(row_bytes * (size_t)chunk_tiles, 256);
-    const size_t gslot_bytes = (size_t)grid_max * base.global_slots * tile * 2 * sizeof(double);
-    const size_t o_unit = o_gslots + align_to(std::max<size_t>(gslot_bytes, 256), 256);
-    const int32_t ld_unit = (int32_t)(chunk_tiles * tile);
-    const size_t o_groups = o_unit + align_to(std::max<size_t>((size_t)n_unit * (size_t)ld_unit * 8, 256), 256);
-    const size_t group_bytes = row_bytes * (size_t)((chunk_tiles + kReduceGroup - 1) / kReduceGroup);
-    rc = ensure_workspace(*d, o_groups + std::max<size_t>(group_bytes, 256));
-    if (rc) return rc;
-    auto *work = static_cast<unsigned char *>(d->rev_work);
-    RevParams p = base;
-    p.weights = weights_dev;
-    p.realized = realized_dev;
-    p.durations = durations_dev;
-    p.cause = cause_dev;
-    p.n = (int32_t)n;
-    p.ld = (int32_t)std::max<int64_t>(ld, 1);
-    p.partial = reinterpret_cast<double *>(work);
-    p.gslots = reinterpret_cast<double *>(work + o_gslots);
-    double *unit = reinterpret_cast<double *>(work + o_unit);
-    double *groups = reinterpret_cast<double *>(work + o_groups);
-    p.unit = unit;
-    p.ld_unit = ld_unit;
-    if (tiles == 0 || P.E == 0) {  // no samples: sums are 0 (or unchanged)
-        HIP_TRY(launch_reduce_columns(p.partial, 0, p.n_cols, groups, d->rev_map_off, d->rev_map_col, P.L, P.E,
-                                      accumulate, grad_dmin_dev, grad_earliest_dev, objective_dev, s),
-                "reduce kernel launch");
-        return MCDP_OK;
-    }
-    for (int64_t t0 = 0; t0 < tiles; t0 += chunk_tiles) {
-        p.tile0 = (int32_t)t0;
-        p.n_tiles = (int32_t)std::min<int64_t>(chunk_tiles, tiles - t0);
-        const int grid = (int)std::min<int64_t>(p.n_tiles, resident);
-        HIP_TRY(launch_unit_draws(d->rev_unit_links, (int32_t)n_unit, seeds_dev, (int32_t)(t0 * tile),
-                                  (int32_t)(p.n_tiles * tile), (int32_t)n, base.dists, base.alias, unit, ld_unit, s),
-                "unit draw kernel launch");
-        HIP_TRY(launch_reverse(p, spl, grid, lds, s), "reverse kernel launch");
-        HIP_TRY(launch_reduce_columns(p.partial, p.n_tiles, p.n_cols, groups, d->rev_map_off, d->rev_map_col, P.L, P.E,
-                                      accumulate || t0 > 0, grad_dmin_dev, grad_earliest_dev, objective_dev, s),
-                "reduce kernel launch");
-    }
-    return MCDP_OK;
+    return reverse_run(prop, seeds_dev, weights_dev, realized_dev, durations_dev, cause_dev, n, ld, accumulate,
+                       objective_dev, grad_dmin_dev, grad_earliest_dev, nullptr, 0, static_cast<hipStream_t>(stream));
 }
 
 int mcdp_propagator_gradient_info(mcdp_propagator *prop, int64_t *out, int n_out) {
@@ -510,9 +556,12 @@ int mcdp_gradient_host(mcdp_propagator *prop, const int32_t *seeds, int64_t n, c
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
         return fail(MCDP_E_DEVICE, "no HIP device available: gradients run on the GPU only");
+    std::lock_guard<std::mutex> host_lock(prop->host_mu);
     void *buf = nullptr;
-    hipError_t e = hipMalloc(&buf, bytes);
-    if (e != hipSuccess) return fail(MCDP_E_NOMEM, std::string("hipMalloc (gradient): ") + hipGetErrorString(e));
+    hipStream_t st = nullptr;
+    int rc = host_workspace(*prop, bytes, buf, st);
+    if (rc) return rc;
+    hipError_t e = hipSuccess;
     auto *b = static_cast<unsigned char *>(buf);
     auto *d_seeds = reinterpret_cast<int32_t *>(b + o_seeds);
     auto *d_real = reinterpret_cast<double *>(b + o_real);
@@ -521,26 +570,26 @@ int mcdp_gradient_host(mcdp_propagator *prop, const int32_t *seeds, int64_t n, c
     auto *d_w = reinterpret_cast<double *>(b + o_w);
     auto *d_obj = reinterpret_cast<double *>(b + o_out);
     double *d_gd = d_obj + 1, *d_ge = d_obj + 1 + L;
-    int rc = MCDP_OK;
-    if (E > 0 && (e = hipMemcpy(d_w, weights, (size_t)E * 8, hipMemcpyHostToDevice)) != hipSuccess)
+    if (E > 0 && (e = hipMemcpyAsync(d_w, weights, (size_t)E * 8, hipMemcpyHostToDevice, st)) != hipSuccess)
         rc = hip_fail(e, "hipMemcpy");
     for (int64_t c0 = 0; c0 < n && !rc; c0 += chunk) {
         const int64_t nc = std::min<int64_t>(chunk, n - c0);
-        if ((e = hipMemcpy(d_seeds, seeds + c0, (size_t)nc * 4, hipMemcpyHostToDevice)) != hipSuccess) {
+        if ((e = hipMemcpyAsync(d_seeds, seeds + c0, (size_t)nc * 4, hipMemcpyHostToDevice, st)) != hipSuccess) {
             rc = hip_fail(e, "hipMemcpy");
             break;
         }
-        rc = mcdp_run_many_device(prop, d_seeds, nc, nc, d_real, d_dur, d_cause, nullptr);
+        rc = mcdp_run_many_device(prop, d_seeds, nc, nc, d_real, d_dur, d_cause, st);
         if (!rc)
             rc = mcdp_reverse_device(prop, d_seeds, d_w, d_real, d_dur, d_cause, nc, nc, c0 > 0, d_obj, d_gd, d_ge,
-                                     nullptr);
-        // the next chunk's seeds overwrite this chunk's
-        if (!rc && (e = hipDeviceSynchronize()) != hipSuccess) rc = hip_fail(e, "gradient");
+                                     st);
+        // the next chunk's seeds overwrite this chunk's (the copy is asynchronous for pageable memory)
+        if (!rc && (e = hipStreamSynchronize(st)) != hipSuccess) rc = hip_fail(e, "gradient");
     }
     std::vector<double> out((size_t)(1 + L + E));
-    if (!rc && (e = hipMemcpy(out.data(), d_obj, out.size() * 8, hipMemcpyDeviceToHost)) != hipSuccess)
+    if (!rc && (e = hipMemcpyAsync(out.data(), d_obj, out.size() * 8, hipMemcpyDeviceToHost, st)) != hipSuccess)
         rc = hip_fail(e, "hipMemcpy");
-    (void)hipFree(buf);
+    if ((e = hipStreamSynchronize(st)) != hipSuccess && !rc) rc = hip_fail(e, "gradient");
+    host_workspace_trim(*prop);
     if (rc) return rc;
     const double dn = (double)n;
     objective[0] = out[0] / dn;

@@ -921,8 +921,59 @@ int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double
     return MCDP_OK;
 }
 
+}  // namespace
+
+int mcdp::host_workspace(mcdp_propagator &P, size_t need, void *&buf, hipStream_t &stream) {
+    int device = 0;
+    HIP_TRY(hipGetDevice(&device), "hipGetDevice");
+    HostWork *hw = nullptr;
+    for (auto &h : P.host_work)
+        if (h.device == device) hw = &h;
+    if (!hw) {
+        HostWork h;
+        h.device = device;
+        HIP_TRY(hipStreamCreateWithFlags(&h.stream, hipStreamNonBlocking), "hipStreamCreate");
+        P.host_work.push_back(h);
+        hw = &P.host_work.back();
+    }
+    if (hw->bytes < need) {
+        if (hw->buf) {
+            (void)hipStreamSynchronize(hw->stream);  // idle already: every call ends by waiting on it
+            (void)hipFree(hw->buf);
+            hw->buf = nullptr;
+            hw->bytes = 0;
+        }
+        hipError_t e = hipMalloc(&hw->buf, std::max<size_t>(need, 256));
+        if (e != hipSuccess) {
+            hw->buf = nullptr;
+            return fail(MCDP_E_NOMEM, std::string("hipMalloc (host-call workspace): ") + hipGetErrorString(e));
+        }
+        hw->bytes = std::max<size_t>(need, 256);
+    }
+    buf = hw->buf;
+    stream = hw->stream;
+    return MCDP_OK;
+}
+
+void mcdp::host_workspace_trim(mcdp_propagator &P) {
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) return;
+    for (auto &h : P.host_work)
+        if (h.device == device && h.buf && h.bytes > kHostWorkKeepBytes) {
+            (void)hipStreamSynchronize(h.stream);
+            (void)hipFree(h.buf);
+            h.buf = nullptr;
+            h.bytes = 0;
+        }
+}
+
+namespace {
+
 // Host-buffer convenience: runs in chunks of samples so device memory stays
-// bounded, and copies each chunk into the caller's event-major arrays.
+// bounded, and copies each chunk into the caller's event-major arrays.  The
+// device buffers and the stream are the propagator's (HostWork), kept from
+// call to call; the call waits on that stream only, so other streams of the
+// process (a server's, a caller's own) keep running.
 int run_host(mcdp_propagator &P, bool injected, const int32_t *seeds, const double *dur_in, int64_t n,
              double *realized, double *durations, int32_t *cause) {
     if (n < 0) return fail(MCDP_E_INVALID, "n must be non-negative");
@@ -934,60 +985,52 @@ int run_host(mcdp_propagator &P, bool injected, const int32_t *seeds, const doub
     const int64_t per_sample = 12 * E + 8 * L + 8;
     int64_t chunk = std::max<int64_t>(kLanes, ((int64_t)1 << 31) / std::max<int64_t>(per_sample, 1));
     chunk = std::min<int64_t>(n, chunk / kLanes * kLanes);
-    double *d_real = nullptr, *d_dur = nullptr;
-    int32_t *d_cause = nullptr, *d_seeds = nullptr;
-    auto cleanup = [&]() {
-        if (d_real) (void)hipFree(d_real);
-        if (d_dur) (void)hipFree(d_dur);
-        if (d_cause) (void)hipFree(d_cause);
-        if (d_seeds) (void)hipFree(d_seeds);
-    };
-    auto alloc = [&](void **ptr, size_t bytes) -> int {
-        if (bytes == 0) return MCDP_OK;
-        hipError_t e = hipMalloc(ptr, bytes);
-        if (e != hipSuccess) return fail(MCDP_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
-        return MCDP_OK;
-    };
-    int rc = MCDP_OK;
-    if ((rc = alloc((void **)&d_real, sizeof(double) * E * chunk)) ||
-        (rc = alloc((void **)&d_cause, sizeof(int32_t) * E * chunk)) ||
-        (rc = alloc((void **)&d_dur, sizeof(double) * L * chunk)) ||
-        (!injected && (rc = alloc((void **)&d_seeds, sizeof(int32_t) * chunk)))) {
-        cleanup();
-        return rc;
-    }
+    const size_t o_cause = align_up(sizeof(double) * E * chunk, 256);
+    const size_t o_dur = o_cause + align_up(sizeof(int32_t) * E * chunk, 256);
+    const size_t o_seeds = o_dur + align_up(sizeof(double) * L * chunk, 256);
+    const size_t need = o_seeds + align_up(sizeof(int32_t) * chunk, 256);
+
+    std::lock_guard<std::mutex> host_lock(P.host_mu);
+    void *work = nullptr;
+    hipStream_t st = nullptr;
+    int rc = host_workspace(P, need, work, st);
+    if (rc) return rc;
+    auto *base = static_cast<unsigned char *>(work);
+    auto *d_real = reinterpret_cast<double *>(base);
+    auto *d_cause = reinterpret_cast<int32_t *>(base + o_cause);
+    auto *d_dur = reinterpret_cast<double *>(base + o_dur);
+    auto *d_seeds = reinterpret_cast<int32_t *>(base + o_seeds);
     for (int64_t c0 = 0; c0 < n && rc == MCDP_OK; c0 += chunk) {
         const int64_t m = std::min(chunk, n - c0);
         hipError_t e = hipSuccess;
         if (injected) {
             if (L > 0)
-                e = hipMemcpy2D(d_dur, sizeof(double) * chunk, dur_in + c0, sizeof(double) * n, sizeof(double) * m,
-                                (size_t)L, hipMemcpyHostToDevice);
+                e = hipMemcpy2DAsync(d_dur, sizeof(double) * chunk, dur_in + c0, sizeof(double) * n,
+                                     sizeof(double) * m, (size_t)L, hipMemcpyHostToDevice, st);
         } else {
-            e = hipMemcpy(d_seeds, seeds + c0, sizeof(int32_t) * m, hipMemcpyHostToDevice);
+            e = hipMemcpyAsync(d_seeds, seeds + c0, sizeof(int32_t) * m, hipMemcpyHostToDevice, st);
         }
         if (e != hipSuccess) {
             rc = hip_fail(e, "hipMemcpy (inputs)");
             break;
         }
-        rc = launch(P, injected, d_seeds, d_dur, m, chunk, d_real, injected ? nullptr : d_dur, d_cause, nullptr);
+        rc = launch(P, injected, d_seeds, d_dur, m, chunk, d_real, injected ? nullptr : d_dur, d_cause, st);
         if (rc) break;
-        e = hipDeviceSynchronize();
-        if (e != hipSuccess) {
-            rc = hip_fail(e, "walk kernel");
-            break;
-        }
         auto copy_out = [&](void *dst, const void *src, size_t elem, int64_t rows) -> hipError_t {
             if (!dst || rows == 0) return hipSuccess;
-            return hipMemcpy2D(static_cast<char *>(dst) + elem * c0, elem * n, src, elem * chunk, elem * m,
-                               (size_t)rows, hipMemcpyDeviceToHost);
+            return hipMemcpy2DAsync(static_cast<char *>(dst) + elem * c0, elem * n, src, elem * chunk, elem * m,
+                                    (size_t)rows, hipMemcpyDeviceToHost, st);
         };
         if ((e = copy_out(realized, d_real, sizeof(double), E)) != hipSuccess ||
             (e = copy_out(cause, d_cause, sizeof(int32_t), E)) != hipSuccess ||
-            (!injected && (e = copy_out(durations, d_dur, sizeof(double), L)) != hipSuccess))
+            (!injected && (e = copy_out(durations, d_dur, sizeof(double), L)) != hipSuccess)) {
             rc = hip_fail(e, "hipMemcpy (outputs)");
+            break;
+        }
+        if ((e = hipStreamSynchronize(st)) != hipSuccess) rc = hip_fail(e, "walk kernel");
     }
-    cleanup();
+    if (rc) (void)hipStreamSynchronize(st);  // nothing of this call may still use the buffers
+    host_workspace_trim(P);
     return rc;
 }
 

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime_api.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -73,6 +74,18 @@ struct RevPlan {
     std::vector<Record> unit_links;         // forward records of the links drawn again (kMulDraw)
 };
 
+// Device buffers and a stream of the host-buffer entry points
+// (mcdp_run_many_host, mcdp_propagate_host) on one device: kept for the next
+// call, so a call allocates only when it needs more than any call before it,
+// and waits on its own stream only.  They hold per-sample rows, no plan, so
+// new times or a new shape leave them valid.
+struct HostWork {
+    int device = -1;
+    void *buf = nullptr;
+    size_t bytes = 0;
+    hipStream_t stream = nullptr;
+};
+
 // One predecessor entry as the reference evaluates it.
 struct PredEntry {
     int32_t src;
@@ -113,8 +126,17 @@ struct mcdp_propagator {
     // device copies, one per device used
     std::mutex mu;
     std::vector<mcdp::DeviceCopy> dev;
+    // Bumped whenever the device copies are dropped (new times, a new shape):
+    // a batch launch prepared against older copies would read freed plans.
+    std::atomic<uint64_t> generation{0};
+
+    // the host-buffer entry points' buffers and streams, one per device; a
+    // call holds host_mu throughout (taken before mu)
+    std::mutex host_mu;
+    std::vector<mcdp::HostWork> host_work;
 
     void drop_device_copies() {
+        generation.fetch_add(1);
         for (auto &d : dev) {
             int cur = -1;
             (void)hipGetDevice(&cur);
@@ -131,7 +153,18 @@ struct mcdp_propagator {
         }
         dev.clear();
     }
-    ~mcdp_propagator() { drop_device_copies(); }
+    ~mcdp_propagator() {
+        drop_device_copies();
+        for (auto &h : host_work) {
+            int cur = -1;
+            (void)hipGetDevice(&cur);
+            (void)hipSetDevice(h.device);
+            if (h.stream) (void)hipStreamSynchronize(h.stream);
+            if (h.buf) (void)hipFree(h.buf);
+            if (h.stream) (void)hipStreamDestroy(h.stream);
+            if (cur >= 0) (void)hipSetDevice(cur);
+        }
+    }
 };
 
 namespace mcdp {
@@ -141,6 +174,23 @@ int ensure_device(mcdp_propagator &P, DeviceCopy *&out);
 // Walk-kernel variant (fast or general paths) and its LDS bytes for P's plan.
 WalkVariant plan_variant(const mcdp_propagator &P, bool injected);
 size_t plan_lds_bytes(const mcdp_propagator &P, WalkVariant v);
+// The host-buffer entry points' device workspace of at least `need` bytes and
+// stream on the current device (made or grown on demand).  The caller holds
+// P.host_mu until its work on the stream is done, and ends by waiting on it.
+int host_workspace(mcdp_propagator &P, size_t need, void *&buf, hipStream_t &stream);
+// Frees the current device's workspace if it is larger than
+// kHostWorkKeepBytes (a large call does not pin its buffers for good); the
+// caller holds P.host_mu and its stream is idle.
+constexpr size_t kHostWorkKeepBytes = (size_t)4 << 30;
+void host_workspace_trim(mcdp_propagator &P);
+// mcdp_reverse_device with its workspace given: `work` of `work_bytes` bytes,
+// split into as many launches as it needs, and nothing allocated (the
+// propagator's plans must be resident on the current device already); or,
+// with work == nullptr, the propagator's own workspace, grown on demand.
+int reverse_run(mcdp_propagator *prop, const int32_t *seeds_dev, const double *weights_dev, const double *realized_dev,
+                const double *durations_dev, const int32_t *cause_dev, int64_t n, int64_t ld, int accumulate,
+                double *objective_dev, double *grad_dmin_dev, double *grad_earliest_dev, void *work, size_t work_bytes,
+                hipStream_t s);
 // mcdp_propagator_configure.
 int configure_shape(mcdp_propagator *prop, int32_t waves, int32_t k, int32_t spl);
 // Uploads P's plan and the statistics' predecessor table to the current

@@ -117,6 +117,22 @@ size_t footprint(int64_t n, int64_t rows, int64_t E, int64_t P, int64_t n_q, boo
     return b;
 }
 
+// Samples per chunk of a request of n samples whose whole footprint exceeds
+// `cap`: the largest multiple of 128 whose footprint (with the chunks' running
+// statistics) fits the cap, at most n; 0 if not even 128 samples fit.  The
+// 1,024 bytes of slack cover the 256-byte alignment of the two per-sample
+// row blocks.
+int64_t chunk_samples(int64_t n, int64_t E, int64_t P, int64_t n_q, size_t cap) {
+    const size_t fixed = footprint(n, 0, E, P, n_q, true);
+    const size_t per_sample = (size_t)std::max<int64_t>(E, 1) * 12;
+    if (fixed >= cap || cap - fixed <= 1024) return 0;
+    int64_t m = (int64_t)((cap - fixed - 1024) / per_sample);
+    m = m / 128 * 128;
+    if (m < 128) return 0;
+    m = std::min<int64_t>(m, n);
+    return footprint(n, m, E, P, n_q, true) <= cap ? m : 0;
+}
+
 }  // namespace
 
 struct mcdp_server {
@@ -325,6 +341,7 @@ void mcdp_server_destroy(mcdp_server *srv) {
         }
         srv->queued.clear();
     }
+    srv->cv_done.notify_all();  // waiters on the failed requests return now
     srv->cv_work.notify_all();
     if (srv->worker.joinable()) srv->worker.join();  // drains running requests
     for (auto &kv : srv->all)
@@ -373,18 +390,21 @@ int mcdp_server_submit(mcdp_server *srv, int32_t ctx_id, const int32_t *seeds, i
             return fail(MCDP_E_NOMEM, "request needs " + std::to_string(r->bytes) + " bytes of device memory (cap " +
                                           std::to_string(srv->request_cap) +
                                           "); statistics over sample chunks need a finite max_delay");
-        const size_t fixed = footprint(n, 0, r->E, r->P, n_q, true);
-        const size_t per_sample = (size_t)r->E * 12;
-        int64_t m = fixed < srv->request_cap ? (int64_t)((srv->request_cap - fixed - 1024) / std::max<size_t>(per_sample, 1)) : 0;
-        m = m / 128 * 128;
-        if (m < 128)
+        const int64_t m = chunk_samples(n, r->E, r->P, n_q, srv->request_cap);
+        if (m < 1)
             return fail(MCDP_E_NOMEM, "request needs " + std::to_string(r->bytes) + " bytes of device memory; even "
                                       "chunks of 128 samples exceed the request cap of " +
                                       std::to_string(srv->request_cap) + " bytes (budget " +
                                       std::to_string(srv->pool->size()) + ")");
-        r->chunk = std::min<int64_t>(m, n);
+        r->chunk = m;
         r->bytes = footprint(n, r->chunk, r->E, r->P, n_q, true);
     }
+    // Admission is strictly in arrival order: a request the pool can never
+    // hold would block every later one, so none is queued.
+    if (r->bytes > srv->request_cap || r->bytes > srv->pool->size())
+        return fail(MCDP_E_NOMEM, "request footprint of " + std::to_string(r->bytes) +
+                                      " bytes exceeds the request cap of " + std::to_string(srv->request_cap) +
+                                      " bytes");
     const size_t host_doubles = (size_t)(r->E + r->E * n_q + r->E + r->P);
     hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&r->h_out), std::max<size_t>(host_doubles, 1) * 8);
     if (e != hipSuccess) return fail(MCDP_E_NOMEM, std::string("hipHostMalloc (results): ") + hipGetErrorString(e));
@@ -432,6 +452,24 @@ int mcdp_server_wait(mcdp_server *srv, int64_t request_id, double *mean, double 
     }
     (void)hipHostFree(r->h_out);
     return rc;
+}
+
+int mcdp_debug_serve_chunk(int64_t n, int64_t n_events, int64_t n_preds, int32_t n_q, int64_t request_cap,
+                           int64_t *chunk, int64_t *bytes) {
+    if (!chunk || !bytes || n < 1 || n_events < 0 || n_preds < 0 || n_q < 0 || n_q > 8 || request_cap < 0)
+        return fail(MCDP_E_INVALID, "invalid argument");
+    const size_t cap = (size_t)request_cap;
+    const size_t whole = footprint(n, n, n_events, n_preds, n_q, false);
+    if (whole <= cap) {
+        *chunk = 0;
+        *bytes = (int64_t)whole;
+        return MCDP_OK;
+    }
+    const int64_t m = chunk_samples(n, n_events, n_preds, n_q, cap);
+    if (m < 1) return fail(MCDP_E_NOMEM, "even chunks of 128 samples exceed the request cap");
+    *chunk = m;
+    *bytes = (int64_t)footprint(n, m, n_events, n_preds, n_q, true);
+    return MCDP_OK;
 }
 
 int mcdp_server_stats(mcdp_server *srv, int64_t *out, int n_out) {

@@ -210,9 +210,15 @@ int mcdp_run_stats_host(mcdp_propagator *prop, const int32_t *seeds, int64_t n, 
     const size_t o_q = o_mean + (size_t)E * 8;
     const size_t o_cnt = o_q + (size_t)E * (size_t)std::max(n_q, 0) * 8;
     const size_t bytes = o_cnt + (size_t)(E + P) * 8;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return fail(MCDP_E_DEVICE, "no HIP device available: statistics run on the GPU only");
+    std::lock_guard<std::mutex> host_lock(prop->host_mu);
     void *buf = nullptr;
-    hipError_t e = hipMalloc(&buf, bytes);
-    if (e != hipSuccess) return fail(MCDP_E_NOMEM, std::string("hipMalloc (statistics): ") + hipGetErrorString(e));
+    hipStream_t st = nullptr;
+    rc = host_workspace(*prop, bytes, buf, st);
+    if (rc) return rc;
+    hipError_t e = hipSuccess;
     auto *b = static_cast<unsigned char *>(buf);
     auto *d_seeds = reinterpret_cast<int32_t *>(b + o_seeds);
     auto *d_real = reinterpret_cast<double *>(b + o_real);
@@ -220,22 +226,23 @@ int mcdp_run_stats_host(mcdp_propagator *prop, const int32_t *seeds, int64_t n, 
     auto *d_mean = reinterpret_cast<double *>(b + o_mean);
     auto *d_q = reinterpret_cast<double *>(b + o_q);
     auto *d_cnt = reinterpret_cast<int64_t *>(b + o_cnt);
-    e = hipMemcpy(d_seeds, seeds, (size_t)n * 4, hipMemcpyHostToDevice);
+    e = hipMemcpyAsync(d_seeds, seeds, (size_t)n * 4, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy");
-    if (!rc) rc = mcdp_run_many_device(prop, d_seeds, n, n, d_real, nullptr, cause_counts ? d_cause : nullptr, nullptr);
+    if (!rc) rc = mcdp_run_many_device(prop, d_seeds, n, n, d_real, nullptr, cause_counts ? d_cause : nullptr, st);
     if (!rc)
         rc = mcdp_event_stats_device(prop, d_real, d_cause, n, n, n_q, q, mean ? d_mean : nullptr,
-                                     quantiles ? d_q : nullptr, cause_counts ? d_cnt : nullptr, nullptr);
-    if (!rc && (e = hipDeviceSynchronize()) != hipSuccess) rc = hip_fail(e, "statistics");
-    if (!rc && mean && (e = hipMemcpy(mean, d_mean, (size_t)E * 8, hipMemcpyDeviceToHost)) != hipSuccess)
+                                     quantiles ? d_q : nullptr, cause_counts ? d_cnt : nullptr, st);
+    if (!rc && mean &&
+        (e = hipMemcpyAsync(mean, d_mean, (size_t)E * 8, hipMemcpyDeviceToHost, st)) != hipSuccess)
         rc = hip_fail(e, "hipMemcpy");
     if (!rc && quantiles && n_q > 0 &&
-        (e = hipMemcpy(quantiles, d_q, (size_t)E * (size_t)n_q * 8, hipMemcpyDeviceToHost)) != hipSuccess)
+        (e = hipMemcpyAsync(quantiles, d_q, (size_t)E * (size_t)n_q * 8, hipMemcpyDeviceToHost, st)) != hipSuccess)
         rc = hip_fail(e, "hipMemcpy");
     if (!rc && cause_counts &&
-        (e = hipMemcpy(cause_counts, d_cnt, (size_t)(E + P) * 8, hipMemcpyDeviceToHost)) != hipSuccess)
+        (e = hipMemcpyAsync(cause_counts, d_cnt, (size_t)(E + P) * 8, hipMemcpyDeviceToHost, st)) != hipSuccess)
         rc = hip_fail(e, "hipMemcpy");
-    (void)hipFree(buf);
+    if ((e = hipStreamSynchronize(st)) != hipSuccess && !rc) rc = hip_fail(e, "statistics");
+    host_workspace_trim(*prop);
     return rc;
 }
 

@@ -5,6 +5,15 @@
 // Classes, constructors, keyword names, attribute names, repr formats, the
 // frozen-dataclass decoration and the exception types follow the reference;
 // the Monte Carlo work itself runs in libmcdp's gfx950 kernels.
+//
+// MCDP_TYPES_MODULE names the module whose NewType aliases annotate the data
+// classes.  Built as this package's module it is mc_dagprop_amd.types; built
+// as the reference's own mc_dagprop.monte_carlo._core (a drop-in for the
+// extension of _core.cpp) pass -DMCDP_TYPES_MODULE="mc_dagprop.types", the
+// module the reference imports at _core.cpp:453.
+#ifndef MCDP_TYPES_MODULE
+#define MCDP_TYPES_MODULE "mc_dagprop_amd.types"
+#endif
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
@@ -220,7 +229,7 @@ PYBIND11_MODULE(_core, m) {
         opts["slots"] = true;
         opts["init"] = false;
         py::object deco = dataclass(**opts);
-        py::module_ types_mod = py::module_::import("mc_dagprop_amd.types");
+        py::module_ types_mod = py::module_::import(MCDP_TYPES_MODULE);
         py::object Second = types_mod.attr("Second");
         py::dict ts_ann;
         ts_ann["earliest"] = Second;

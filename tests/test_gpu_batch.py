@@ -177,3 +177,72 @@ def test_batch_launch_replays_from_a_graph(gpu) -> None:
     finally:
         capi.check(capi.lib().mcdp_graph_destroy(graph))
         capi.check(capi.lib().mcdp_stream_destroy(stream))
+
+
+def test_batch_matches_oracle(gpu) -> None:
+    """Every DAG's batched durations, fed to the CPU oracle (the reference's propagation loop,
+    _core.cpp:332-350), reproduce the batch's realized and cause bit for bit; links that need no
+    draw carry the oracle's closed-form durations."""
+    import oracle
+
+    dags = _dags()
+    ns = [700, 300, 129, 1]
+    seeds = [np.arange(n, dtype=np.int32) * (k + 7) - 300 for k, n in enumerate(ns)]
+    B = capi.Batch([capi.Propagator(d) for d in dags])
+    for d, (r, dd, c) in zip(dags, B.run_many(seeds)):
+        ctx = oracle.OracleCtx(d)
+        r0, c0 = ctx.propagate(dd)
+        np.testing.assert_array_equal(bits(r), bits(r0))
+        np.testing.assert_array_equal(c, c0)
+        base = ctx.base_durations()
+        fixed = ~np.isnan(base)
+        np.testing.assert_array_equal(bits(dd[:, fixed]), bits(np.broadcast_to(base[fixed], dd[:, fixed].shape)))
+
+
+def test_batch_stats_match_oracle(gpu) -> None:
+    """Batch.stats (grouped launches, device statistics) equals the oracle's per-event statistics
+    (math.fsum mean, numpy.quantile, explicit cause counting; demo/monte_carlo.py:45-63 is the
+    consumer these replace) over the oracle's propagation of the same seeds' durations."""
+    import oracle
+
+    dags = _dags()
+    ns = [640, 257, 129, 3]
+    seeds = [np.arange(n, dtype=np.int32) * 3 + 11 * k for k, n in enumerate(ns)]
+    q = (0.0, 0.5, 0.9, 0.99, 1.0)
+    props = [capi.Propagator(d) for d in dags]
+    got = capi.Batch(props).stats(seeds, q, memory_budget=5 << 20)  # two groups
+    for d, P, s, (m, qq, cc) in zip(dags, props, seeds, got):
+        _, dd, _ = capi.Propagator(d).run_many(s)
+        r0, c0 = oracle.OracleCtx(d).propagate(dd)
+        m0, q0, k0 = oracle.event_stats(r0, c0, oracle.pred_sources(d), q)
+        np.testing.assert_array_equal(bits(m), bits(m0))
+        np.testing.assert_array_equal(bits(qq), bits(q0))
+        np.testing.assert_array_equal(cc, k0)
+
+
+def test_prepared_launch_refused_after_new_times(gpu) -> None:
+    """set_times frees a member's device plan, which a prepared launch (and a graph captured from it)
+    points into: the next launch must fail instead of reading freed memory; a new prepare works and
+    gives the member's own results with the new times."""
+    dags = [layered_small(), layered_network(layers=6)]
+    props = [capi.Propagator(d) for d in dags]
+    B = capi.Batch(props)
+    ns = [300, 200]
+    sd = [capi.DeviceBuffer.from_array(np.arange(n, dtype=np.int32)) for n in ns]
+    real = [capi.DeviceBuffer(8 * P.E * n) for P, n in zip(props, ns)]
+    B.prepare_device(sd, ns, ns, real)
+    B.launch()
+    capi.synchronize()
+    dmin, earliest = props[1].get_times()
+    props[1].set_times(dmin * 1.25, earliest)
+    with pytest.raises(capi.McdpError, match="prepare the batch again"):
+        B.launch()
+    with pytest.raises(capi.McdpError, match="no prepared launch"):
+        B.launch()  # refused once: the batch stays unprepared
+    B.prepare_device(sd, ns, ns, real)
+    B.launch()
+    capi.synchronize()
+    solo = capi.Propagator(dags[1])
+    solo.set_times(dmin * 1.25, earliest)
+    r0, _, _ = solo.run_many(np.arange(ns[1]))
+    np.testing.assert_array_equal(bits(real[1].to_array(np.float64, (props[1].E, ns[1])).T), bits(r0))

@@ -182,3 +182,58 @@ def test_chunked_request_needs_finite_max_delay(gpu) -> None:
     with pytest.raises(capi.McdpError, match="finite max_delay"):
         srv.submit(cid, np.arange(20_000, dtype=np.int32))
     srv.close()
+
+
+def test_served_results_match_oracle(gpu) -> None:
+    """One request per resident context (in a mix, several on the device at once) against the oracle:
+    the same seeds' device durations propagated by the CPU oracle (_core.cpp:332-350), then the
+    oracle's per-event statistics (math.fsum mean, numpy.quantile, explicit cause counting; the
+    consumer of demo/monte_carlo.py:45-63).  Bit for bit, including quantiles."""
+    import oracle
+
+    dags, props = contexts()
+    srv = capi.Server(64 << 20, max_running=4, n_streams=3, request_cap=64 << 20)
+    ids = [srv.add_context(p) for p in props]
+    q = (0.0, 0.25, 0.5, 0.9, 0.99, 1.0)
+    reqs = []
+    for k in range(2):
+        for c in range(len(props)):
+            n = 700 + 333 * c + 101 * k
+            seeds = np.arange(n, dtype=np.int32) * 5 + 1_000 * c - 77 * k
+            reqs.append((srv.submit(ids[c], seeds, q), c, seeds))
+    for rid, c, seeds in reqs:
+        mean, quant, counts, _ = srv.wait(rid)
+        _, dd, _ = capi.Propagator(dags[c]).run_many(seeds)
+        r0, c0 = oracle.OracleCtx(dags[c]).propagate(dd)
+        m0, q0, k0 = oracle.event_stats(r0, c0, oracle.pred_sources(dags[c]), q)
+        np.testing.assert_array_equal(bits(mean), bits(m0))
+        np.testing.assert_array_equal(bits(quant), bits(q0))
+        np.testing.assert_array_equal(counts, k0)
+    srv.close()
+
+
+def test_chunked_request_matches_oracle(gpu) -> None:
+    """A request above the request cap runs in chunks of samples: its mean and cause counts equal the
+    oracle's bit for bit, its extreme quantiles (0 and 1) exactly, and every other quantile lies within
+    max_delay / 4096 of numpy's."""
+    import oracle
+
+    dags, props = contexts()
+    d, p = dags[1], props[1]
+    n = 5000
+    P = int(p.pred_offsets()[-1])
+    cap = footprint(p.E, P, n, 4) // 3  # forces chunks of about a third of the samples
+    srv = capi.Server(4 * cap, max_running=2, n_streams=2, request_cap=cap)
+    cid = srv.add_context(p)
+    seeds = np.arange(n, dtype=np.int32) + 12345
+    q = (0.0, 0.5, 0.9, 1.0)
+    mean, quant, counts, _ = srv.wait(srv.submit(cid, seeds, q))
+    assert srv.stats()["chunked"] == 1
+    srv.close()
+    _, dd, _ = capi.Propagator(d).run_many(seeds)
+    r0, c0 = oracle.OracleCtx(d).propagate(dd)
+    m0, q0, k0 = oracle.event_stats(r0, c0, oracle.pred_sources(d), q)
+    np.testing.assert_array_equal(bits(mean), bits(m0))
+    np.testing.assert_array_equal(counts, k0)
+    np.testing.assert_array_equal(bits(quant[:, [0, 3]]), bits(q0[:, [0, 3]]))
+    assert np.all(np.abs(quant - q0) <= d.max_delay / BINS)

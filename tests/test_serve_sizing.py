@@ -1,0 +1,96 @@
+"""The serving footprint rule (mcdp_server_submit's chunk sizing), host only.
+
+A request whose whole footprint exceeds the request cap runs in chunks of samples.  The chunk
+is the largest multiple of 128 samples whose footprint fits the cap; a request that cannot fit
+even 128 samples is refused.  Near the boundary (the fixed part of the footprint within 1,024
+bytes of the cap) an unsigned subtraction used to wrap, giving a chunk of every sample and a
+footprint above the cap, which the strictly ordered admission could never place.
+
+The footprint below restates the device layout (256-byte aligned blocks: seeds, realized and
+cause rows, mean, quantiles, cause counts, and for chunked requests the running statistics of
+56 bytes per event and a 4,096-bin histogram per event); the test checks the library's answers
+against it: every accepted request fits the cap, and no larger chunk would.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from mc_dagprop_amd import capi
+
+CHUNK_STAT_BYTES = 56
+CHUNK_BINS = 4096
+
+
+def _a(x: int) -> int:
+    return (x + 255) // 256 * 256
+
+
+def footprint(n: int, rows: int, E: int, P: int, nq: int, chunked: bool) -> int:
+    b = _a(n * 4) + _a(E * rows * 8) + _a(E * rows * 4) + _a(E * 8) + _a(E * nq * 8) + _a((E + P) * 8)
+    if chunked:
+        b += _a(E * CHUNK_STAT_BYTES) + _a(E * CHUNK_BINS * 4)
+    return b
+
+
+def sizing(n: int, E: int, P: int, nq: int, cap: int) -> tuple[int, int, int]:
+    chunk, nbytes = C.c_int64(-1), C.c_int64(-1)
+    rc = capi.lib().mcdp_debug_serve_chunk(n, E, P, nq, cap, C.byref(chunk), C.byref(nbytes))
+    return rc, chunk.value, nbytes.value
+
+
+def test_whole_request_fits() -> None:
+    rc, chunk, nbytes = sizing(1000, 50, 80, 3, 1 << 30)
+    assert (rc, chunk, nbytes) == (0, 0, footprint(1000, 1000, 50, 80, 3, False))
+
+
+@pytest.mark.parametrize("slack", [0, 1, 255, 256, 1000, 1023, 1024, 1025])
+def test_cap_just_above_the_fixed_part_is_refused(slack: int) -> None:
+    """fixed + slack with slack <= 1024 + a few bytes: not even 128 samples fit, so NOMEM."""
+    n, E, P, nq = 262_144, 200_000, 450_000, 3
+    fixed = footprint(n, 0, E, P, nq, True)
+    rc, chunk, nbytes = sizing(n, E, P, nq, fixed + slack)
+    assert rc == 6, (rc, chunk, nbytes)
+    assert b"128 samples" in capi.lib().mcdp_last_error()
+
+
+def test_smallest_chunk_at_the_boundary() -> None:
+    n, E, P, nq = 262_144, 200_000, 450_000, 3
+    fixed = footprint(n, 0, E, P, nq, True)
+    need128 = footprint(n, 128, E, P, nq, True)
+    rc, chunk, nbytes = sizing(n, E, P, nq, need128 + 1024)
+    assert rc == 0 and chunk == 128 and nbytes == need128 and nbytes <= need128 + 1024
+    # one byte below what the rule's slack asks: refused, never an oversized chunk
+    rc, chunk, _ = sizing(n, E, P, nq, fixed + 1024 + 128 * 12 * E - 1)
+    assert rc == 6
+
+
+def test_random_requests_fit_their_cap_and_chunks_are_maximal() -> None:
+    rng = np.random.default_rng(7)
+    for _ in range(3000):
+        E = int(rng.integers(1, 250_000))
+        P = int(rng.integers(0, 3 * E + 1))
+        nq = int(rng.integers(0, 9))
+        n = int(rng.integers(1, 300_000))
+        whole = footprint(n, n, E, P, nq, False)
+        cap = int(rng.integers(1, max(2, 2 * whole)))
+        rc, chunk, nbytes = sizing(n, E, P, nq, cap)
+        if whole <= cap:
+            assert (rc, chunk, nbytes) == (0, 0, whole)
+            continue
+        if rc != 0:
+            assert rc == 6
+            assert footprint(n, min(128, n), E, P, nq, True) + 1024 > cap
+            continue
+        assert 1 <= chunk <= n and (chunk % 128 == 0 or chunk == n), chunk
+        assert nbytes == footprint(n, chunk, E, P, nq, True) and nbytes <= cap
+        if chunk < n and chunk + 128 <= n:
+            assert footprint(n, chunk + 128, E, P, nq, True) + 1024 > cap
+
+
+def test_invalid_arguments() -> None:
+    assert sizing(0, 10, 10, 0, 1 << 20)[0] == 1
+    assert sizing(10, 10, 10, 9, 1 << 20)[0] == 1
