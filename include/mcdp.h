@@ -263,6 +263,26 @@ int mcdp_reverse_device(mcdp_propagator *prop, const int32_t *seeds_dev, const d
                         const double *realized_dev, const double *durations_dev, const int32_t *cause_dev, int64_t n,
                         int64_t ld, int accumulate, double *objective_dev, double *grad_dmin_dev,
                         double *grad_earliest_dev, void *stream);
+/* mcdp_reverse_device without allocating device memory.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, serving loops or harnesses that forbid hipMalloc).  Same arguments,
+ * results bit-identical to mcdp_reverse_device's.  Its partial rows, unit rows
+ * and group sums live in one fixed scratch array of
+ * MCDP_NOALLOC_SCRATCH_BYTES per GPU (256 MiB, loaded with the library and
+ * held until the process ends), split into as many launches as it needs; the
+ * call allocates, frees and synchronises nothing.  Preconditions and limits:
+ *   - the propagator's plans must already be resident on the current device
+ *     (call mcdp_reverse_device on it once; n = 0 is enough), else
+ *     MCDP_E_INVALID;
+ *   - a DAG whose reverse walk needs more than the scratch for one launch of
+ *     16 tiles (about 1 KiB per partial-row column) fails with MCDP_E_NOMEM;
+ *   - every call on a GPU shares that GPU's scratch: calls that overlap on one
+ *     GPU (other streams or threads) race and give wrong sums. */
+#define MCDP_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int mcdp_reverse_device_noalloc(mcdp_propagator *prop, const int32_t *seeds_dev, const double *weights_dev,
+                                const double *realized_dev, const double *durations_dev, const int32_t *cause_dev,
+                                int64_t n, int64_t ld, int accumulate, double *objective_dev, double *grad_dmin_dev,
+                                double *grad_earliest_dev, void *stream);
 /* Facts about the reverse walk's plan: out[0..5] = {steps, waves per
  * workgroup, slots, slots in LDS, partial-row columns, samples per lane}. */
 int mcdp_propagator_gradient_info(mcdp_propagator *prop, int64_t *out, int n_out);

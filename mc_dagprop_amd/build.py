@@ -1,7 +1,7 @@
 """Builds the native parts of mc_dagprop_amd in place.
 
 * ``libmcdp.so``: the gfx950 kernels (``csrc/mcdp_kernels.hip``, ``csrc/mcdp_stats.hip``,
-  ``csrc/mcdp_grad.hip``; hipcc) and the host compiler + C ABI (``csrc/mcdp_host.cpp``, ``csrc/mcdp_stats.cpp``,
+  ``csrc/mcdp_grad.hip``, ``csrc/mcdp_noalloc.hip``; hipcc) and the host compiler + C ABI (``csrc/mcdp_host.cpp``, ``csrc/mcdp_stats.cpp``,
   ``csrc/mcdp_batch.cpp``, ``csrc/mcdp_grad.cpp``, ``csrc/mcdp_serve.cpp``; g++),
   declared in
   ``include/mcdp.h``.
@@ -75,7 +75,7 @@ def build(verbose: bool = False) -> None:
     """Compiles libmcdp.so and the _core extension (always rebuilds)."""
     BUILD.mkdir(exist_ok=True)
     hipcc = str(ROCM / "bin" / "hipcc")
-    kernels = ["mcdp_kernels.hip", "mcdp_stats.hip", "mcdp_grad.hip"]
+    kernels = ["mcdp_kernels.hip", "mcdp_stats.hip", "mcdp_grad.hip", "mcdp_noalloc.hip"]
     hosts = ["mcdp_host.cpp", "mcdp_stats.cpp", "mcdp_batch.cpp", "mcdp_grad.cpp", "mcdp_serve.cpp"]
     objs = [BUILD / (Path(f).stem + ("_dev.o" if f.endswith(".hip") else "_host.o")) for f in kernels + hosts]
     steps = [[hipcc, *KERNEL_FLAGS, "-c", CSRC / f, "-o", o] for f, o in zip(kernels, objs)]

@@ -86,6 +86,7 @@ _SIGNATURES = {
     "mcdp_propagator_set_times": (C.c_int, [_P, _pf64, _pf64]),
     "mcdp_propagator_get_times": (C.c_int, [_P, _pf64, _pf64]),
     "mcdp_reverse_device": (C.c_int, [_P, _P, _P, _P, _P, _P, _i64, _i64, C.c_int, _P, _P, _P, _P]),
+    "mcdp_reverse_device_noalloc": (C.c_int, [_P, _P, _P, _P, _P, _P, _i64, _i64, C.c_int, _P, _P, _P, _P]),
     "mcdp_gradient_host": (C.c_int, [_P, _pi32, _i64, _pf64, _i64, _pf64, _pf64, _pf64]),
     "mcdp_debug_reverse_tiles": (C.c_int, [_P, _i64]),
     "mcdp_debug_tail_mode": (C.c_int, [_P, C.c_int32]),
