@@ -97,14 +97,9 @@ def bench_batch(args) -> int:
     quantiles (and cause counts) reduced on the device (mcdp_batch_stats_host: DAGs in groups that fit the
     memory budget, one batched launch per group).  Against one DAG at a time (mcdp_run_stats_host each)."""
     from mc_dagprop_amd import capi
-    from mc_dagprop_amd.workloads import SplitMix64, layered_vectorized, seeded_network_dists
+    from mc_dagprop_amd.workloads import batch_workload
 
-    rng = SplitMix64(20260002)
-    dags, ns = [], []
-    for i in range(args.batch_dags):
-        events = int(round(1000.0 * 50.0 ** rng.uniform(0.0, 1.0), -2))
-        dags.append(layered_vectorized(events // 100, 100, 4, seeded_network_dists(i), 8, 20260002 + i))
-        ns.append(int(4096.0 * 16.0 ** rng.uniform(0.0, 1.0)))
+    dags, ns = batch_workload(args.batch_dags)
     props = [capi.Propagator(d) for d in dags]
     solo = [capi.Propagator(d) for d in dags]
     E_all = sum(P.E for P in props)
@@ -169,7 +164,7 @@ def bench_batch(args) -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (layered DAGs from numpy PCG64, per-DAG distribution parameters from SplitMix64; "
+        "data": "synthetic (layered DAGs and per-DAG distribution parameters from SplitMix64; "
                 "seeds 0..S-1 per DAG)",
         "config": {"workload": f"{len(dags)} layered DAGs (width 100, 1-4 preds, 8 activity types with per-DAG "
                                "parameters), 1,000-50,000 events and 4,096-65,536 samples each (log-uniform); "
@@ -192,15 +187,11 @@ def bench_serve(args) -> int:
     import threading
 
     from mc_dagprop_amd import capi
-    from mc_dagprop_amd.workloads import layered_vectorized, network_dists
+    from mc_dagprop_amd.workloads import serve_contexts, serve_stream
 
-    rng = np.random.default_rng(20260003)
     n_ctx = args.contexts
-    dists = network_dists()
-    events = np.exp(rng.uniform(np.log(1000.0), np.log(200000.0), n_ctx))
-    layers = np.maximum(10, np.round(events / 100.0)).astype(int)
     t_build = time.perf_counter()
-    props = [capi.Propagator(layered_vectorized(int(m), 100, 4, dists, 8, seed=7000 + i)) for i, m in enumerate(layers)]
+    props = [capi.Propagator(d) for d in serve_contexts(n_ctx)]
     budget = int(args.budget_gb * (1 << 30))
     cap = budget // 4  # one request's share of the pool; larger requests run in chunks of samples
     srv = capi.Server(budget, max_running=8, n_streams=4, request_cap=cap)
@@ -214,17 +205,16 @@ def bench_serve(args) -> int:
         srv.wait(srv.submit(ids[c], np.arange(256, dtype=np.int32), q))
 
     N, rate = args.requests, args.rate
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, N))
-    ctx_of = rng.integers(0, n_ctx, N)
-    n_of = np.exp(rng.uniform(np.log(16384.0), np.log(262144.0), N)).astype(np.int64)
-    seeds_of = [np.arange(1_000_000 * i, 1_000_000 * i + n_of[i], dtype=np.int32) for i in range(N)]
+    stream = serve_stream(n_ctx, N, rate)
+    arrivals, ctx_of, n_of = stream.arrivals, stream.context, stream.samples
+    seeds_of = [stream.seeds(i) for i in range(N)]
     work = float(sum(props[ctx_of[i]].L * n_of[i] for i in range(N)))
     rids = [None] * N
     lat = np.zeros(N)
     keep = {}
     # checked against the same request run alone: three that run in one piece
     fits = [i for i in range(N) if 12 * props[ctx_of[i]].E * n_of[i] < cap // 2 and n_of[i] <= 40000]
-    check_idx = set(int(i) for i in rng.choice(fits, min(3, len(fits)), replace=False))
+    check_idx = set(int(i) for i in fits[:: max(1, len(fits) // 3)][:3])
     t0 = time.perf_counter()
 
     def submitter() -> None:
@@ -269,7 +259,7 @@ def bench_serve(args) -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (layered DAGs from numpy PCG64; seeds disjoint per request)",
+        "data": "synthetic (layered DAGs and the Poisson request stream from SplitMix64; seeds disjoint per request)",
         "config": {"workload": f"Poisson requests at {rate:g}/s over {n_ctx} resident contexts (1k-200k events, "
                                "log-uniform; width 100, 1-4 preds, 8 activity types); 16,384-262,144 samples per "
                                "request (log-uniform); per-event mean, quantiles 0.5/0.9/0.99 and cause counts; "

@@ -214,53 +214,160 @@ def layered(layers: int, width: int, max_preds: int, n_types: int, variant: int,
 INF = float("inf")
 
 
+class SplitMix64Stream:
+    """The SplitMix64 sequence of :class:`SplitMix64` drawn in blocks: draw i (0-based) of the stream
+    seeded with ``seed`` is mix(seed + (i + 1) * 0x9E3779B97F4A7C15 mod 2^64), so a block is one
+    vectorized evaluation (numpy uint64 arithmetic wraps modulo 2^64 on every platform).  Uniforms
+    and bounded integers are derived exactly as :class:`SplitMix64` derives them."""
+
+    GAMMA = np.uint64(0x9E3779B97F4A7C15)
+
+    def __init__(self, seed: int) -> None:
+        self.seed = np.uint64(seed & SplitMix64.MASK)
+        self.pos = 0
+
+    def raw(self, n: int) -> np.ndarray:
+        i = np.arange(self.pos + 1, self.pos + 1 + n, dtype=np.uint64)
+        self.pos += n
+        with np.errstate(over="ignore"):
+            z = self.seed + i * self.GAMMA
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+    def uniform(self, n: int, lo: float = 0.0, hi: float = 1.0) -> np.ndarray:
+        u = (self.raw(n) >> np.uint64(11)).astype(np.float64) * 2.0**-53
+        return lo + (hi - lo) * u
+
+    def below(self, n: int, m) -> np.ndarray:
+        """n integers, the i-th uniform on [0, m_i) (m an int or an array of n ints)."""
+        m = np.asarray(m, dtype=np.int64)
+        u = (self.raw(n) >> np.uint64(11)).astype(np.float64) * 2.0**-53
+        return np.minimum((u * m).astype(np.int64), m - 1)
+
+
 def layered_vectorized(layers: int, width: int, max_preds: int, dists: list[DistSpec], n_types: int,
                        seed: int) -> FlatDag:
     """The layered construction of :func:`layered` (1..max_preds distinct predecessors from the previous
-    layer, minimal durations U[30, 600) s, earliest = latest predecessor arrival + U[0, 120) s slack,
-    max_delay 1800), drawn with numpy's PCG64 layer by layer: fast enough for hundreds of contexts of up
-    to 200,000 events.  Deterministic for a given numpy version and seed."""
-    rng = np.random.default_rng(seed)
+    layer, kept in draw order; minimal durations U[30, 600) s; earliest = latest predecessor arrival +
+    U[0, 120) s slack; max_delay 1800), drawn from one :class:`SplitMix64Stream` in blocks over the
+    whole DAG, fast enough for hundreds of contexts of up to 200,000 events.  Draw order: the
+    predecessor count of every event of layers 1.. (event order); max_preds rounds of a partial
+    Fisher-Yates shuffle of each such event's copy of the previous layer (one draw per event and
+    round, consumed whatever the event's count); the minimal duration of every activity, then its
+    type (activity order: by destination event, predecessors in draw order); the slack of every
+    event (layer 0 first).  Identical on every machine and numpy version."""
+    rng = SplitMix64Stream(seed)
     E = layers * width
-    earliest = np.zeros(E)
-    earliest[:width] = rng.uniform(0.0, 120.0, width)
-    srcs, dsts, durs, types, offs = [], [], [], [], [np.zeros(1, dtype=np.int64)]
-    n_acts = 0
-    for layer in range(1, layers):
-        k = rng.integers(1, max_preds + 1, width)
-        order = np.argsort(rng.random((width, width)), axis=1)  # a random permutation per event
-        take = np.arange(width)[None, :] < k[:, None]
-        local = order[take]  # row-major: each event's first k picks, in draw order
-        dst = layer * width + np.repeat(np.arange(width), k)
-        src = (layer - 1) * width + local
-        dmin = rng.uniform(30.0, 600.0, src.size)
-        arrive = np.full(width, -np.inf)
-        np.maximum.at(arrive, dst - layer * width, earliest[src] + dmin)
-        earliest[layer * width:(layer + 1) * width] = arrive + rng.uniform(0.0, 120.0, width)
-        srcs.append(src)
-        dsts.append(dst)
-        durs.append(dmin)
-        types.append(rng.integers(0, n_types, src.size))
-        offs.append(n_acts + np.cumsum(k))
-        n_acts += src.size
-    src = np.concatenate(srcs) if srcs else np.zeros(0, dtype=np.int64)
+    N = (layers - 1) * width  # events with predecessors
+    k = 1 + rng.below(N, max_preds)
+    pool = np.tile(np.arange(width, dtype=np.int16 if width < 32768 else np.int64), (N, 1))
+    rows = np.arange(N)
+    for i in range(min(max_preds, width)):
+        r = i + rng.below(N, width - i)
+        a, b = pool[rows, i].copy(), pool[rows, r].copy()
+        pool[rows, i], pool[rows, r] = b, a
+    take = np.arange(width)[None, :] < k[:, None]
+    dst = width + np.repeat(rows, k)  # destination event of each activity, in activity order
+    src = (dst // width - 1) * width + pool[take].astype(np.int64)
     A = src.size
+    dmin = rng.uniform(A, 30.0, 600.0)
+    typ = rng.below(A, n_types)
+    slack = rng.uniform(E, 0.0, 120.0)
+    earliest = np.zeros(E)
+    earliest[:width] = slack[:width]
+    seg = np.concatenate([[0], np.cumsum(k)])  # activities of event width + j: seg[j] .. seg[j + 1]
+    for layer in range(1, layers):
+        j0, j1 = (layer - 1) * width, layer * width
+        a0, a1 = seg[j0], seg[j1]
+        cand = earliest[src[a0:a1]] + dmin[a0:a1]
+        arrive = np.maximum.reduceat(cand, seg[j0:j1] - a0)
+        earliest[layer * width:(layer + 1) * width] = arrive + slack[layer * width:(layer + 1) * width]
     return FlatDag(
         earliest=earliest,
         latest=earliest + 3600.0,
         actual=np.zeros(E),
         act_src=src.astype(np.int32),
-        act_dst=(np.concatenate(dsts) if dsts else np.zeros(0)).astype(np.int32),
+        act_dst=dst.astype(np.int32),
         act_idx=np.arange(A, dtype=np.int32),
-        act_dur=np.concatenate(durs) if durs else np.zeros(0),
-        act_type=(np.concatenate(types) if types else np.zeros(0)).astype(np.int32),
+        act_dur=dmin,
+        act_type=typ.astype(np.int32),
         prec_target=np.arange(width, E, dtype=np.int32),
-        prec_off=np.concatenate(offs).astype(np.int64),
+        prec_off=seg.astype(np.int64),
         prec_src=src.astype(np.int32),
         prec_edge=np.arange(A, dtype=np.int32),
         max_delay=1800.0,
         dists=dists,
     )
+
+
+def dag_digest(dag: FlatDag) -> str:
+    """sha256 of a DAG's arrays and distribution parameters (fixed dtypes, little-endian)."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for a, t in ((dag.earliest, "<f8"), (dag.latest, "<f8"), (dag.actual, "<f8"), (dag.act_src, "<i4"),
+                 (dag.act_dst, "<i4"), (dag.act_idx, "<i4"), (dag.act_dur, "<f8"), (dag.act_type, "<i4"),
+                 (dag.prec_target, "<i4"), (dag.prec_off, "<i8"), (dag.prec_src, "<i4"), (dag.prec_edge, "<i4")):
+        h.update(np.ascontiguousarray(a, dtype=t).tobytes())
+    h.update(np.float64(dag.max_delay).astype("<f8").tobytes())
+    for d in dag.dists:
+        h.update(np.asarray([d.activity_type, d.kind], dtype="<i8").tobytes())
+        h.update(np.asarray(d.params, dtype="<f8").tobytes())
+        for a in (d.values, d.weights):
+            if a is not None:
+                h.update(np.ascontiguousarray(a, dtype="<f8").tobytes())
+    return h.hexdigest()
+
+
+def batch_workload(n_dags: int = 64) -> tuple[list[FlatDag], list[int]]:
+    """The batched workload: ``n_dags`` layered DAGs (width 100, 1-4 predecessors, 8 activity types
+    with per-DAG parameters from :func:`seeded_network_dists`), event counts log-uniform in
+    [1,000, 50,000] (rounded to whole layers) and sample counts log-uniform in [4,096, 65,536], all
+    drawn from SplitMix64 (seed 20260002)."""
+    rng = SplitMix64(20260002)
+    dags, ns = [], []
+    for i in range(n_dags):
+        events = int(round(1000.0 * 50.0 ** rng.uniform(0.0, 1.0), -2))
+        dags.append(layered_vectorized(events // 100, 100, 4, seeded_network_dists(i), 8, 20260002 + i))
+        ns.append(int(4096.0 * 16.0 ** rng.uniform(0.0, 1.0)))
+    return dags, ns
+
+
+@dataclass
+class ServeStream:
+    """The serving workload's request stream: arrival times (s), context of each request, and its
+    sample count; request i's seeds are 1,000,000 i + 0 .. n_i - 1."""
+
+    arrivals: np.ndarray
+    context: np.ndarray
+    samples: np.ndarray
+
+    def seeds(self, i: int) -> np.ndarray:
+        return np.arange(1_000_000 * i, 1_000_000 * i + int(self.samples[i]), dtype=np.int32)
+
+
+def serve_contexts(n_contexts: int = 200) -> list[FlatDag]:
+    """The serving workload's resident contexts: event counts log-uniform in [1,000, 200,000] (whole
+    layers of width 100, at least 10 layers), 1-4 predecessors, the network workload's eight
+    activity types; SplitMix64 (seed 20260003 for the sizes, 7000 + i for context i)."""
+    rng = SplitMix64Stream(20260003)
+    events = np.exp(rng.uniform(n_contexts, np.log(1000.0), np.log(200000.0)))
+    layers = np.maximum(10, np.round(events / 100.0)).astype(int)
+    dists = network_dists()
+    return [layered_vectorized(int(m), 100, 4, dists, 8, 7000 + i) for i, m in enumerate(layers)]
+
+
+def serve_stream(n_contexts: int, n_requests: int, rate: float, min_samples: int = 16384,
+                 max_samples: int = 262144) -> ServeStream:
+    """A Poisson stream of ``n_requests`` requests at ``rate`` per second, each on a context drawn
+    uniformly and asking for a log-uniform number of samples in [min_samples, max_samples]; SplitMix64
+    (seed 20260013)."""
+    rng = SplitMix64Stream(20260013)
+    gaps = -np.log1p(-rng.uniform(n_requests)) / rate
+    ctx = rng.below(n_requests, n_contexts)
+    n = np.exp(rng.uniform(n_requests, np.log(float(min_samples)), np.log(float(max_samples)))).astype(np.int64)
+    return ServeStream(np.cumsum(gaps), ctx, n)
 
 
 def network_dists() -> list[DistSpec]:
