@@ -63,6 +63,7 @@ def oracle_lib() -> C.CDLL:
         L.orc_get_order.argtypes = [_P, _pi32]
         L.orc_base_durations.argtypes = [_P, C.c_int, _pi32, _pi32, _pf64, _pf64]
         L.orc_propagate.argtypes = [_P, _pf64, C.c_int64, C.c_int64, _pf64, _pi32]
+        L.orc_propagate_columns.argtypes = [_P, _pf64, C.c_int64, C.c_int64, _pf64, _pi32, C.c_int64]
         L.orc_philox4x32_many.argtypes = [C.c_int, C.c_int64, _pu32, _pu32, _pu32]
         L.orc_philox2x32_many.argtypes = [C.c_int, C.c_int64, _pu32, _pu32, _pu32]
         L.orc_get_links.argtypes = [_P, _pf64, _pi32]
@@ -141,6 +142,30 @@ class OracleCtx:
         r = np.empty((n, self.E))
         c = np.empty((n, self.E), dtype=np.int32)
         oracle_lib().orc_propagate(self.h, _ptr(d, _pf64), d.shape[1], n, _ptr(r, _pf64), _ptr(c, _pi32))
+        return r, c
+
+    def propagate_columns(self, durations: np.ndarray, realized_out: np.ndarray | None = None,
+                          cause_out: np.ndarray | None = None) -> tuple[np.ndarray, np.ndarray]:
+        """Event-major form: durations (L, n) -> realized (E, n), cause (E, n).  Rows may have any
+        stride (columns contiguous); given outputs (views of one leading dimension) are written in
+        place.  The C call releases the GIL, so threads may run it on disjoint column ranges."""
+        d = np.asarray(durations, dtype=np.float64)
+        if d.ndim != 2 or d.shape[0] != self.L or (d.size and d.strides[1] != 8):
+            d = np.ascontiguousarray(d)
+        n = d.shape[1]
+        ld = d.strides[0] // 8 if d.size else n
+        if realized_out is None:
+            r = np.empty((self.E, n))
+            c = np.empty((self.E, n), dtype=np.int32)
+        else:
+            r, c = realized_out, cause_out
+            if r.shape != (self.E, n) or c.shape != (self.E, n) or r.dtype != np.float64 or c.dtype != np.int32:
+                raise ValueError("outputs must be (E, n) float64 and int32")
+            if n and (r.strides[1] != 8 or c.strides[1] != 4 or r.strides[0] // 8 != c.strides[0] // 4):
+                raise ValueError("outputs need contiguous columns and one leading dimension")
+        ldo = (r.strides[0] // 8) if (n and self.E) else max(n, 1)
+        oracle_lib().orc_propagate_columns(self.h, d.ctypes.data_as(_pf64), ld, n, r.ctypes.data_as(_pf64),
+                                           c.ctypes.data_as(_pi32), ldo)
         return r, c
 
     def links(self) -> tuple[np.ndarray, np.ndarray]:
@@ -372,18 +397,20 @@ def event_stats(realized: np.ndarray, cause: np.ndarray | None, preds: list[list
     counts = None
     if cause is not None:
         cause = np.asarray(cause)
+        # counters per event: cause -1, then one per predecessor position; a source listed twice counts at
+        # its first position (its later positions stay 0)
+        kmax = max((len(p) for p in preds), default=0)
+        first_src = np.full((kmax, E), -2, dtype=np.int64)  # -2: no counter matches (causes are >= -1)
+        for e in range(E):
+            seen = set()
+            for k, src in enumerate(preds[e]):
+                if src not in seen:
+                    seen.add(src)
+                    first_src[k, e] = src
+        per_pos = [np.count_nonzero(cause == -1, axis=0)]
+        per_pos += [np.count_nonzero(cause == first_src[k][None, :], axis=0) for k in range(kmax)]
         out = []
         for e in range(E):
-            first = {}
-            for k, src in enumerate(preds[e]):
-                first.setdefault(src, k)
-            c = np.zeros(1 + len(preds[e]), dtype=np.int64)
-            vals, cnt = np.unique(cause[:, e], return_counts=True)
-            for v, m in zip(vals.tolist(), cnt.tolist()):
-                if v == -1:
-                    c[0] += m
-                elif v in first:
-                    c[1 + first[v]] += m
-            out.append(c)
+            out.append(np.asarray([per_pos[k][e] for k in range(1 + len(preds[e]))], dtype=np.int64))
         counts = np.concatenate(out) if out else np.zeros(0, dtype=np.int64)
     return mean, quant, counts

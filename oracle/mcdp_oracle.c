@@ -16,6 +16,9 @@
  *   orc_propagate   the propagation loop, _core.cpp:316-350, given injected
  *                   durations.  Pinned bit-exactly against oracle/_ref and
  *                   the golden fixtures it produced.
+ *   orc_propagate_columns  the same loop over event-major arrays (samples
+ *                   innermost), for full-size device outputs.  Pinned
+ *                   against orc_propagate.
  *   orc_base_durations  durations that need no random numbers: no
  *                   distribution (_core.cpp:303-306) and constant
  *                   (_core.cpp:72-76 with :328).  Pinned by test KATs.
@@ -287,6 +290,47 @@ void orc_propagate(const orc_ctx *c, const double *durations, int64_t ld_dur, in
             r[e] = (upper < latest) ? upper : latest;
             ce[e] = who;
         }
+    }
+}
+
+/*
+ * orc_propagate in the device's event-major layout: durations[l * ld_dur + s],
+ * realized[e * ld_out + s], cause[e * ld_out + s] for samples s < n.  The same
+ * operations in the same order per sample (the loop over samples is
+ * innermost), so full-size device outputs can be checked without transposing
+ * them.  Pinned against orc_propagate on random DAGs (stale reads, caps, NaN
+ * and infinite durations, invalid edges).
+ */
+void orc_propagate_columns(const orc_ctx *c, const double *durations, int64_t ld_dur, int64_t n, double *realized,
+                           int *cause, int64_t ld_out) {
+    const int E = c->E, L = c->L;
+    for (int i = 0; i < E; ++i) {
+        double *r = realized + (int64_t)i * ld_out;
+        int *ce = cause + (int64_t)i * ld_out;
+        for (int64_t s = 0; s < n; ++s) {
+            r[s] = c->earliest[i];
+            ce[s] = -1;
+        }
+    }
+    for (int oi = 0; oi < E; ++oi) {
+        const int e = c->order[oi];
+        const double upper = c->earliest[e] + c->max_delay;
+        double *r = realized + (int64_t)e * ld_out; /* holds the running maximum until the cap below */
+        int *ce = cause + (int64_t)e * ld_out;
+        for (int64_t k = c->off[e]; k < c->off[e + 1]; ++k) {
+            const int src = c->src[k], edge = c->edge[k];
+            const double *rs = realized + (int64_t)src * ld_out;
+            const double *d = (edge >= 0 && edge < L) ? durations + (int64_t)edge * ld_dur : NULL;
+            for (int64_t s = 0; s < n; ++s) {
+                double t = rs[s] + (d ? d[s] : 0.0);
+                t = (upper < t) ? upper : t;
+                if (t >= r[s]) {
+                    r[s] = t;
+                    ce[s] = src;
+                }
+            }
+        }
+        for (int64_t s = 0; s < n; ++s) r[s] = (upper < r[s]) ? upper : r[s];
     }
 }
 
