@@ -1003,10 +1003,14 @@ int run_host(mcdp_propagator &P, bool injected, const int32_t *seeds, const doub
     for (int64_t c0 = 0; c0 < n && rc == MCDP_OK; c0 += chunk) {
         const int64_t m = std::min(chunk, n - c0);
         hipError_t e = hipSuccess;
+        // one chunk of all samples: host and device rows have the same pitch, so every array moves as
+        // one contiguous copy (a 2-D copy of many short rows, e.g. one sample, costs a transfer per row)
+        const bool whole = chunk == n;
         if (injected) {
             if (L > 0)
-                e = hipMemcpy2DAsync(d_dur, sizeof(double) * chunk, dur_in + c0, sizeof(double) * n,
-                                     sizeof(double) * m, (size_t)L, hipMemcpyHostToDevice, st);
+                e = whole ? hipMemcpyAsync(d_dur, dur_in, sizeof(double) * L * n, hipMemcpyHostToDevice, st)
+                          : hipMemcpy2DAsync(d_dur, sizeof(double) * chunk, dur_in + c0, sizeof(double) * n,
+                                             sizeof(double) * m, (size_t)L, hipMemcpyHostToDevice, st);
         } else {
             e = hipMemcpyAsync(d_seeds, seeds + c0, sizeof(int32_t) * m, hipMemcpyHostToDevice, st);
         }
@@ -1018,6 +1022,7 @@ int run_host(mcdp_propagator &P, bool injected, const int32_t *seeds, const doub
         if (rc) break;
         auto copy_out = [&](void *dst, const void *src, size_t elem, int64_t rows) -> hipError_t {
             if (!dst || rows == 0) return hipSuccess;
+            if (whole) return hipMemcpyAsync(dst, src, elem * (size_t)n * (size_t)rows, hipMemcpyDeviceToHost, st);
             return hipMemcpy2DAsync(static_cast<char *>(dst) + elem * c0, elem * n, src, elem * chunk, elem * m,
                                     (size_t)rows, hipMemcpyDeviceToHost, st);
         };
