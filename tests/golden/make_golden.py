@@ -7,7 +7,8 @@ seeds, and what the reference's Simulator::run returned for them, seed by seed i
 order on one Simulator (as run_many does): realized (n, E), durations (n, L),
 cause (n, E).  ``MANIFEST.txt`` lists the sha256 of every fixture.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # every case
+    python tests/golden/make_golden.py NAME ...   # only these cases (their manifest lines are replaced)
 """
 
 from __future__ import annotations
@@ -31,6 +32,7 @@ from mc_dagprop_amd.workloads import (  # noqa: E402
     EXPONENTIAL,
     GAMMA,
     DistSpec,
+    layered_network,
     layered_small,
 )
 from oracle import RefSim  # noqa: E402
@@ -59,6 +61,23 @@ def cases():
     yield "parity_long_chain", D.parity_long_chain(), list(range(32))
     yield "parity_cap", D.parity_cap(), list(range(200))
     yield "layered_small", layered_small(), list(range(64))
+    # 2,000-event slices of the network workload's generator (20 of its 200 layers): the literal eight
+    # activity types, whose extras multiply minimal durations of 30-600 s and saturate most events at their
+    # cap, and a calibrated variant (exponential means, gamma scales and relative factors divided by 315,
+    # the mean minimal duration) where caps bind less often
+    yield "network_slice", layered_network(layers=20), list(range(8))
+    calib = layered_network(layers=20)
+    scaled = []
+    for d in calib.dists:
+        if d.kind in (EXPONENTIAL,):
+            d = DistSpec(d.activity_type, d.kind, (d.params[0] / 315.0, d.params[1], d.params[2]))
+        elif d.kind == GAMMA:
+            d = DistSpec(d.activity_type, d.kind, (d.params[0], d.params[1] / 315.0, d.params[2]))
+        elif d.kind == EMPIRICAL_RELATIVE:
+            d = DistSpec(d.activity_type, d.kind, d.params, values=np.asarray(d.values) / 315.0, weights=d.weights)
+        scaled.append(d)
+    calib.dists = scaled
+    yield "network_slice_calibrated", calib, list(range(100, 108))
     rng = np.random.default_rng(20260)
     for i, (n, kw) in enumerate([(40, {}), (60, {"duplicates": True}), (50, {"gaps": True, "duplicates": True}),
                                  (80, {"max_delay": 0.0}), (30, {"max_delay": 7.0, "gaps": True})]):
@@ -66,8 +85,19 @@ def cases():
 
 
 def main() -> None:
+    only = set(sys.argv[1:])
+    manifest = HERE / "MANIFEST.txt"
+    kept = {}
+    if only and manifest.exists():
+        for line in manifest.read_text().splitlines():
+            if line.strip():
+                kept[line.split()[1]] = line
     lines = []
     for name, dag, seeds in cases():
+        if only and name not in only:
+            if f"{name}.npz" in kept:
+                lines.append(kept[f"{name}.npz"])
+            continue
         ref = RefSim(dag)
         r, d, cz = ref.run(seeds)
         path = HERE / f"{name}.npz"
