@@ -247,12 +247,54 @@ int ensure_reverse(mcdp_propagator &P, DeviceCopy &d) {
         off = align_to(off + bytes, 256);
         return o;
     };
-    const size_t o_step = reserve(R.step_off.size() * 4), o_recs = reserve(R.recs.size() * sizeof(Record));
-    const size_t o_evoff = reserve(R.ev_off.size() * 4), o_evl = reserve(R.ev_list.size() * 4);
+    // per-event blocks, one stream per (step, wave) (mcdp_grad.hpp)
+    std::vector<uint32_t> blk;
+    std::vector<int32_t> blk_off;
+    {
+        auto put_f64 = [&](double x) {
+            uint64_t u;
+            std::memcpy(&u, &x, 8);
+            blk.push_back((uint32_t)u);
+            blk.push_back((uint32_t)(u >> 32));
+        };
+        for (size_t g = 0; g + 1 < R.ev_off.size(); ++g) {
+            blk_off.push_back((int32_t)blk.size());
+            for (int32_t i = R.ev_off[g]; i < R.ev_off[g + 1]; ++i) {
+                const Record &h = R.recs[(size_t)R.ev_rec[(size_t)i]];
+                const uint32_t n_pred = h.info & kRevNpredMask, n_succ = (uint32_t)h.i2;
+                if (n_pred > 0xFFFFu || n_succ > 0xFFFFu)
+                    return fail(MCDP_E_INVALID, "too many predecessors or successors of one event for the gradient");
+                const size_t start = blk.size();
+                blk.push_back((uint32_t)h.i0);
+                blk.push_back((uint32_t)h.i1);
+                blk.push_back(n_pred | (n_succ << 16));
+                blk.push_back(kBlkHeader + kBlkPred * n_pred + kBlkSucc * n_succ);
+                put_f64(h.a);
+                put_f64(h.b);
+                for (uint32_t k = 0; k < n_pred; ++k) {
+                    const Record &q = R.recs[(size_t)R.ev_rec[(size_t)i] + 1 + k];
+                    put_f64(q.a);
+                    blk.push_back((uint32_t)q.i0);
+                    blk.push_back((uint32_t)q.i2);
+                    blk.push_back(q.info);
+                }
+                const size_t q0 = (size_t)R.ev_succ[(size_t)i];
+                for (uint32_t k = 0; k < n_succ; ++k) {
+                    blk.push_back((uint32_t)R.succ[2 * (q0 + k)]);
+                    blk.push_back((uint32_t)R.succ[2 * (q0 + k) + 1]);
+                }
+                if (blk.size() - start != kBlkHeader + kBlkPred * n_pred + kBlkSucc * n_succ)
+                    return fail(MCDP_E_INVALID, "internal error: gradient block size");
+            }
+        }
+        blk_off.push_back((int32_t)blk.size());
+        for (int k = 0; k < kBlkWindow; ++k) blk.push_back(0u);
+        if (blk.size() > (size_t)std::numeric_limits<int32_t>::max()) return fail(MCDP_E_INVALID, "plan too large");
+    }
+    const size_t o_blk = reserve(blk.size() * 4), o_boff = reserve(blk_off.size() * 4);
+    const size_t o_evoff = reserve(R.ev_off.size() * 4);
     const size_t o_evs = reserve(R.ev_slot.size() * 4), o_moff = reserve(R.map_off.size() * 4);
-    const size_t o_evr = reserve(R.ev_rec.size() * 4);
     const size_t o_unit = reserve(R.unit_links.size() * sizeof(Record));
-    const size_t o_evsucc = reserve(R.ev_succ.size() * 4), o_succ = reserve(R.succ.size() * 4 + 32);
     const size_t o_coloff = reserve(R.col_off.size() * 4);
     const size_t o_mcol = reserve(R.map_col.size() * 4);
     const size_t total = std::max<size_t>(off, 256);
@@ -260,15 +302,11 @@ int ensure_reverse(mcdp_propagator &P, DeviceCopy &d) {
     auto put = [&](size_t o, const void *src, size_t bytes) {
         if (bytes) std::memcpy(host.data() + o, src, bytes);
     };
-    put(o_step, R.step_off.data(), R.step_off.size() * 4);
-    put(o_recs, R.recs.data(), R.recs.size() * sizeof(Record));
+    put(o_blk, blk.data(), blk.size() * 4);
+    put(o_boff, blk_off.data(), blk_off.size() * 4);
     put(o_evoff, R.ev_off.data(), R.ev_off.size() * 4);
-    put(o_evl, R.ev_list.data(), R.ev_list.size() * 4);
     put(o_evs, R.ev_slot.data(), R.ev_slot.size() * 4);
-    put(o_evr, R.ev_rec.data(), R.ev_rec.size() * 4);
     put(o_unit, R.unit_links.data(), R.unit_links.size() * sizeof(Record));
-    put(o_evsucc, R.ev_succ.data(), R.ev_succ.size() * 4);
-    put(o_succ, R.succ.data(), R.succ.size() * 4);
     put(o_coloff, R.col_off.data(), R.col_off.size() * 4);
     put(o_moff, R.map_off.data(), R.map_off.size() * 4);
     put(o_mcol, R.map_col.data(), R.map_col.size() * 4);
@@ -284,14 +322,10 @@ int ensure_reverse(mcdp_propagator &P, DeviceCopy &d) {
     d.rev = blob;
     RevParams &p = d.rev_base;
     p = RevParams{};
-    p.step_off = reinterpret_cast<const int32_t *>(b + o_step);
-    p.recs = reinterpret_cast<const Record *>(b + o_recs);
+    p.blocks = reinterpret_cast<const uint32_t *>(b + o_blk);
+    p.blk_off = reinterpret_cast<const int32_t *>(b + o_boff);
     p.ev_off = reinterpret_cast<const int32_t *>(b + o_evoff);
-    p.ev_list = reinterpret_cast<const int32_t *>(b + o_evl);
     p.ev_slot = reinterpret_cast<const int32_t *>(b + o_evs);
-    p.ev_rec = reinterpret_cast<const int32_t *>(b + o_evr);
-    p.ev_succ = reinterpret_cast<const int32_t *>(b + o_evsucc);
-    p.succ = reinterpret_cast<const int2 *>(b + o_succ);
     p.col_off = reinterpret_cast<const int32_t *>(b + o_coloff);
     p.dists = d.base.dists;
     p.alias = d.base.alias;

@@ -62,6 +62,22 @@ __device__ __forceinline__ double wave_sum(double v) {
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
+// Issue priority from the events a wave has left in its step (5 or more: 3,
+// 3-4: 2, 2: 1, the last: 0).  A SIMD otherwise issues its oldest waves
+// first, so with equal work its youngest wave finishes every step last while
+// the others wait at the step's barrier; favouring the waves furthest behind
+// lets a SIMD's waves reach the barrier together.
+__device__ __forceinline__ void set_priority(int left) {
+    if (left >= 5)
+        __builtin_amdgcn_s_setprio(3);
+    else if (left >= 3)
+        __builtin_amdgcn_s_setprio(2);
+    else if (left >= 2)
+        __builtin_amdgcn_s_setprio(1);
+    else
+        __builtin_amdgcn_s_setprio(0);
+}
+
 __device__ __forceinline__ bool any_lane(bool b) { return __builtin_amdgcn_ballot_w64(b) != 0; }
 
 // A slot's adjoints and winner positions for this lane's kSpl samples
@@ -193,16 +209,12 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) reverse_kernel(RevParams p
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int W = p.waves;
     const uint32_t ld = (uint32_t)p.ld;
-    const auto recs = as_const_space(p.recs);
-    const auto succ = as_const_space(p.succ);
+    const auto blk_off = as_const_space(p.blk_off);
     const auto ev_off = as_const_space(p.ev_off);
-    const auto ev_list = as_const_space(p.ev_list);
-    const auto ev_rec = as_const_space(p.ev_rec);
-    const auto ev_succ = as_const_space(p.ev_succ);
-    const auto ev_slot = as_const_space(p.ev_slot);
     const auto col_off = as_const_space(p.col_off);
     const auto weights = as_const_space(p.weights);
     const auto earliest = as_const_space(p.earliest);
+    const auto blocks_s = as_const_space(p.blocks);
 
     Slots<kGlobal, kSpl> sl;
     sl.g_lds = lds;
@@ -211,6 +223,17 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) reverse_kernel(RevParams p
     sl.w_glb = reinterpret_cast<int32_t *>(sl.g_glb + (size_t)p.global_slots * kLanes * kSpl);
     sl.n_lds = p.lds_slots;
     sl.lane = lane;
+
+    // A block's first kBlkWindow dwords, one per lane (vector load), and its
+    // fields by lane index; dwords past the window come through scalar loads.
+    auto window = [&](int32_t pos) -> uint32_t { return p.blocks[pos + lane]; };
+    auto dw = [&](uint32_t win, int32_t pos, int k) -> uint32_t {
+        return k < kBlkWindow ? (uint32_t)__builtin_amdgcn_readlane((int)win, k) : blocks_s[pos + k];
+    };
+    auto f64 = [&](uint32_t win, int32_t pos, int k) -> double {
+        const uint64_t lo = dw(win, pos, k), hi = dw(win, pos, k + 1);
+        return __longlong_as_double((long long)((hi << 32) | lo));
+    };
 
     for (int32_t tile = blockIdx.x; tile < p.n_tiles; tile += gridDim.x) {
         // Lane l carries samples (tile0 + tile) * 64 kSpl + i * 64 + l.  Lanes
@@ -223,7 +246,6 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) reverse_kernel(RevParams p
             const int32_t s_raw = (p.tile0 + tile) * (kLanes * kSpl) + i * kLanes + lane;
             live[i] = s_raw < p.n;
             s[i] = live[i] ? (uint32_t)s_raw : (uint32_t)(p.n - 1);
-            // unit-duration rows cover this launch's tiles only
             s_unit[i] = s[i] - (uint32_t)p.tile0 * (kLanes * kSpl);
         }
         double *const prow = p.partial + (size_t)tile * p.n_cols;
@@ -231,106 +253,167 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) reverse_kernel(RevParams p
 #pragma unroll
         for (int i = 0; i < kSpl; ++i) obj[i] = 0.0;
         ColumnStage st;
-        // A predecessor's multiplier row: its duration row (times 1 / d) or its
-        // row of unit durations (times 1).
-        auto mult_values = [&](const Record &pr, double (&out)[kSpl]) {
-            const bool unit = (pr.info & kRevClassMask) == (kMulDraw << kRevClassShift);
-            const double *base = unit ? p.unit + row_offset((int32_t)((pr.info >> kInfoDistShift) & kInfoDistMask), p.ld_unit)
-                                      : p.durations + row_offset(pr.i2, ld);
+        auto mult_row = [&](uint32_t info, uint32_t edge, double (&out)[kSpl]) {
+            const bool unit = (info & kRevClassMask) == (kMulDraw << kRevClassShift);
+            const double *base = unit ? p.unit + row_offset((int32_t)((info >> kInfoDistShift) & kInfoDistMask), p.ld_unit)
+                                      : p.durations + row_offset((int32_t)edge, ld);
 #pragma unroll
             for (int i = 0; i < kSpl; ++i) out[i] = base[unit ? s_unit[i] : s[i]];
         };
-        auto reads_row = [](const Record &pr) {
-            const uint32_t cls = (pr.info & kRevClassMask) >> kRevClassShift;
+        auto reads_row = [](uint32_t info) {
+            const uint32_t cls = (info & kRevClassMask) >> kRevClassShift;
             return cls == kMulRead || cls == kMulDraw;
         };
 
-        // An event's per-sample rows (realized, cause, the multiplier rows of
-        // its first kPre predecessors), loaded one event ahead: the next event
-        // of the step, or the first event of the next step.
-        struct EvRows {
+        // The event pipeline over a wave's stream of the whole tile: blocks two
+        // events ahead, per-sample rows (realized, cause, the multiplier rows
+        // of the first kPre predecessors) and the weight one event ahead.
+        struct Rows {
             double r[kSpl];
             int32_t c[kSpl];
             double d[kPre][kSpl];
         };
-        auto load_event = [&](int idx, EvRows &x) {
-            const uint64_t row = row_offset(ev_list[idx], ld);
+        // first block and event count of step tt in this wave's stream
+        auto first_of = [&](int tt, int &ne_out) -> int32_t {
+            ne_out = ev_off[tt * W + wave + 1] - ev_off[tt * W + wave];
+            return blk_off[tt * W + wave];
+        };
+        // A cursor walks the wave's events in stream order: step t, event j of
+        // the step's ne, block at dword pos.
+        struct Cursor {
+            int t, j, ne;
+            int32_t pos;
+        };
+        auto advance = [&](Cursor &c, uint32_t size) {
+            // c is at an event whose block has `size` dwords
+            ++c.j;
+            c.pos += (int32_t)size;
+            while (c.t < p.n_steps && c.j >= c.ne) {
+                ++c.t;
+                c.j = 0;
+                if (c.t < p.n_steps) c.pos = first_of(c.t, c.ne);
+            }
+        };
+        Cursor c0{0, 0, 0, 0};
+        if (p.n_steps > 0) c0.pos = first_of(0, c0.ne);
+        while (c0.t < p.n_steps && c0.ne == 0) {
+            ++c0.t;
+            if (c0.t < p.n_steps) c0.pos = first_of(c0.t, c0.ne);
+        }
+        const bool any_event = c0.t < p.n_steps;
+        // blocks of the current event and the next one, rows of the current event
+        uint32_t win0 = any_event ? window(c0.pos) : 0u;
+        Cursor c1 = c0;
+        uint32_t win1 = 0;
+        bool have1 = false;
+        if (any_event) {
+            advance(c1, dw(win0, c0.pos, 3));
+            have1 = c1.t < p.n_steps;
+            if (have1) win1 = window(c1.pos);
+        }
+        // A fixed number of loads per event (an entry without a multiplier row
+        // reads the event's realized row instead, and its value is never used),
+        // so the waits for one event's rows can leave the next event's loads in
+        // flight.
+        auto load_rows = [&](uint32_t win, int32_t pos, Rows &x) {
+            const uint64_t row = row_offset((int32_t)dw(win, pos, 0), ld);
 #pragma unroll
             for (int i = 0; i < kSpl; ++i) {
                 x.r[i] = p.realized[row + s[i]];
                 x.c[i] = p.cause[row + s[i]];
             }
-            const int h = ev_rec[idx];
-            const int n_pred = (int)(recs[h].info & kRevNpredMask);
+            const int n_pred = (int)(dw(win, pos, 2) & 0xFFFFu);
 #pragma unroll
             for (int k = 0; k < kPre; ++k) {
+                const uint32_t info = k < n_pred ? dw(win, pos, kBlkHeader + kBlkPred * k + 4) : 0u;
+                const uint32_t edge = k < n_pred ? dw(win, pos, kBlkHeader + kBlkPred * k + 3) : 0u;
+                const bool unit = (info & kRevClassMask) == (kMulDraw << kRevClassShift);
+                const bool rd = k < n_pred && reads_row(info);
+                const double *base = !rd   ? p.realized + row
+                                     : unit ? p.unit + row_offset((int32_t)((info >> kInfoDistShift) & kInfoDistMask),
+                                                                  p.ld_unit)
+                                            : p.durations + row_offset((int32_t)edge, ld);
 #pragma unroll
-                for (int i = 0; i < kSpl; ++i) x.d[k][i] = 0.0;
-                if (k < n_pred) {
-                    const Record pr = recs[h + 1 + k];
-                    if (reads_row(pr)) mult_values(pr, x.d[k]);
-                }
+                for (int i = 0; i < kSpl; ++i) x.d[k][i] = base[(rd && unit) ? s_unit[i] : s[i]];
             }
         };
-        EvRows cur{}, nxt{};
-        int loaded = -1;  // index (in ev_list) of the event whose rows are in `cur`
+        Rows cur{}, nxt{};
+        double we_cur = 0.0, we_nxt = 0.0;
+        if (any_event) {
+            load_rows(win0, c0.pos, cur);
+            we_cur = weights[(int32_t)dw(win0, c0.pos, 0)];
+        }
 
-        for (int t = 0; t < p.n_steps; ++t) {
-            const int e0 = ev_off[t * W + wave];
-            const int ne = ev_off[t * W + wave + 1] - e0;
-            if (ne > 0 && loaded != e0) load_event(e0, cur);
-            int32_t ev_col = col_off[t * W + wave];  // the next column this wave writes
-            // the step's results until the commit; the winner positions of a lane's
-            // samples share one word (positions < 2^15 with two samples per lane)
+        // A step's event range and first column, one per lane (lanes 0-2), loaded a
+        // step ahead with one vector load; the step's slots likewise at its start.
+        auto step_meta = [&](int tt) -> int32_t {
+            const int i = (tt < p.n_steps ? tt : 0) * W + wave;
+            return lane < 2 ? p.ev_off[i + lane] : p.col_off[i];
+        };
+        int32_t meta = p.n_steps > 0 ? step_meta(0) : 0;
+        for (int ts = 0; ts < p.n_steps; ++ts) {
+            const int e0 = __builtin_amdgcn_readlane(meta, 0);
+            const int ne = __builtin_amdgcn_readlane(meta, 1) - e0;
+            int32_t ev_col = __builtin_amdgcn_readlane(meta, 2);  // the next column this wave writes
+            const int32_t slot_v = ne > 0 ? p.ev_slot[e0 + (lane < ne ? lane : 0)] : -1;
+            meta = step_meta(ts + 1);
             static_assert(K == 8, "Regs8 holds one step's winner positions");
             double gres[K][kSpl];
             Regs8 wres;
 #pragma unroll 1
-            for (int j = 0; j < ne; ++j) {
-                // the next event's rows
-                int next = -1;
-                if (j + 1 < ne) {
-                    next = e0 + j + 1;
-                } else if (t + 1 < p.n_steps) {
-                    const int f0 = ev_off[(t + 1) * W + wave];
-                    if (ev_off[(t + 1) * W + wave + 1] > f0) next = f0;
+            for (int jj = 0; jj < ne; ++jj) {
+                set_priority(ne - jj);
+                // c0 / win0 / cur: this event; c1 / win1: the next; start the one after and the next rows
+                const int32_t pos = c0.pos;
+                const uint32_t win = win0;
+                Cursor c2 = c1;
+                uint32_t win2 = 0;
+                bool have2 = false;
+                if (have1) {
+                    advance(c2, dw(win1, c1.pos, 3));
+                    have2 = c2.t < p.n_steps;
                 }
-                if (next >= 0) load_event(next, nxt);
+                // the same loads whether or not a further event exists (block 0 stands in)
+                win2 = window(have2 ? c2.pos : 0);
+                load_rows(win1, have1 ? c1.pos : 0, nxt);
+                we_nxt = weights[(int32_t)dw(win1, have1 ? c1.pos : 0, 0)];
 
-                const int h = ev_rec[e0 + j];
-                const Record hdr = recs[h];
-                const int32_t ev = hdr.i0;
-                const int32_t n_pred = (int32_t)(hdr.info & kRevNpredMask);
-                const int32_t n_succ = hdr.i2;
-                const int pb = h + 1;
-                const double we = weights[ev];
+                const uint32_t counts = dw(win, pos, 2);
+                const int32_t n_pred = (int32_t)(counts & 0xFFFFu);
+                const int32_t n_succ = (int32_t)(counts >> 16);
+                const double ea = f64(win, pos, 4), eb = f64(win, pos, 6);
+                const double we = we_cur;
+                const int sb = kBlkHeader + kBlkPred * n_pred;  // first successor dword
                 // adjoint from the successors, four entries per round
                 double g[kSpl];
 #pragma unroll
                 for (int i = 0; i < kSpl; ++i) g[i] = 0.0;
                 {
-                    const int q0 = ev_succ[e0 + j];
                     int q = 0;
                     for (; q + 4 <= n_succ; q += 4) {
-                        int2 sr[4];
+                        int32_t sx[4], sy[4];
                         double gv[4][kSpl];
                         int32_t wv[4][kSpl];
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) sr[u] = succ[q0 + q + u];
+                        for (int u = 0; u < 4; ++u) {
+                            sx[u] = (int32_t)dw(win, pos, sb + kBlkSucc * (q + u));
+                            sy[u] = (int32_t)dw(win, pos, sb + kBlkSucc * (q + u) + 1);
+                        }
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) sl.read(sr[u].x, gv[u], wv[u]);
+                        for (int u = 0; u < 4; ++u) sl.read(sx[u], gv[u], wv[u]);
 #pragma unroll
                         for (int u = 0; u < 4; ++u)
 #pragma unroll
-                            for (int i = 0; i < kSpl; ++i) g[i] += (wv[u][i] == sr[u].y) ? gv[u][i] : 0.0;
+                            for (int i = 0; i < kSpl; ++i) g[i] += (wv[u][i] == sy[u]) ? gv[u][i] : 0.0;
                     }
                     for (; q < n_succ; ++q) {
-                        const int2 sr = succ[q0 + q];
+                        const int32_t sx = (int32_t)dw(win, pos, sb + kBlkSucc * q);
+                        const int32_t sy = (int32_t)dw(win, pos, sb + kBlkSucc * q + 1);
                         double gv[kSpl];
                         int32_t wv[kSpl];
-                        sl.read(sr.x, gv, wv);
+                        sl.read(sx, gv, wv);
 #pragma unroll
-                        for (int i = 0; i < kSpl; ++i) g[i] += (wv[i] == sr.y) ? gv[i] : 0.0;
+                        for (int i = 0; i < kSpl; ++i) g[i] += (wv[i] == sy) ? gv[i] : 0.0;
                     }
                 }
                 double gt[kSpl], ge[kSpl];
@@ -338,55 +421,80 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) reverse_kernel(RevParams p
 #pragma unroll
                 for (int i = 0; i < kSpl; ++i) {
                     const double r = cur.r[i];
-                    const bool active = live[i] && (r > hdr.a);
-                    capped[i] = (cur.c[i] < 0) || (r == hdr.b);
+                    const bool active = live[i] && (r > ea);
+                    capped[i] = (cur.c[i] < 0) || (r == eb);
                     open[i] = live[i] && !capped[i];
                     gt[i] = g[i] + (active ? we : 0.0);
-                    obj[i] += active ? we * (r - hdr.a) : 0.0;
+                    obj[i] += active ? we * (r - ea) : 0.0;
                     ge[i] = active ? -we : 0.0;
                     ge[i] += capped[i] ? gt[i] : 0.0;
                     ge_nz[i] = ge[i] != 0.0;
                 }
                 const bool any_open = any_of<kSpl>(open);
 
-                // is entry q the winner of sample i: its source is the cause and,
-                // for a repeated source, its capped candidate is the maximum
-                auto wins = [&](const Record &q, bool (&m)[kSpl]) {
+                // winners: the last entry whose source is the cause (for a repeated
+                // source, whose capped candidate is the maximum)
+                int32_t win_k[kSpl];
 #pragma unroll
-                    for (int i = 0; i < kSpl; ++i) m[i] = open[i] && (cur.c[i] == q.i0);
-                    if ((q.info & kRevDup) && any_of<kSpl>(m)) {
+                for (int i = 0; i < kSpl; ++i) win_k[i] = -1;
+                if (any_open) {
+                    for (int k = 0; k < n_pred; ++k) {
+                        const int kb = kBlkHeader + kBlkPred * k;
+                        const int32_t src = (int32_t)dw(win, pos, kb + 2);
+                        const uint32_t info = dw(win, pos, kb + 4);
+                        bool m[kSpl];
 #pragma unroll
-                        for (int i = 0; i < kSpl; ++i) {
-                            const double base =
-                                (q.info & kRevStale) ? earliest[q.i0] : p.realized[row_offset(q.i0, ld) + s[i]];
-                            const double d =
-                                (q.info & kInfoEdgeValid) ? p.durations[row_offset(q.i2, ld) + s[i]] : 0.0;
-                            double tt = base + d;
-                            tt = (hdr.b < tt) ? hdr.b : tt;
-                            m[i] = m[i] && (tt == cur.r[i]);
+                        for (int i = 0; i < kSpl; ++i) m[i] = open[i] && (cur.c[i] == src);
+                        if ((info & kRevDup) && any_of<kSpl>(m)) {
+                            const uint32_t edge = dw(win, pos, kb + 3);
+#pragma unroll
+                            for (int i = 0; i < kSpl; ++i) {
+                                const double base =
+                                    (info & kRevStale) ? earliest[src] : p.realized[row_offset(src, ld) + s[i]];
+                                const double d =
+                                    (info & kInfoEdgeValid) ? p.durations[row_offset((int32_t)edge, ld) + s[i]] : 0.0;
+                                double tt = base + d;
+                                tt = (eb < tt) ? eb : tt;
+                                m[i] = m[i] && (tt == cur.r[i]);
+                            }
                         }
+#pragma unroll
+                        for (int i = 0; i < kSpl; ++i) win_k[i] = m[i] ? k : win_k[i];
                     }
-                };
-                // the derivative columns of entry k; the winner passes the adjoint on
+                }
+                // the derivative columns of each entry; the winner passes the adjoint on
                 int32_t wslot[kSpl];
 #pragma unroll
                 for (int i = 0; i < kSpl; ++i) wslot[i] = -1;
-                auto entry = [&](const Record &q, int k, const int32_t (&win)[kSpl], const double (&dk)[kSpl]) {
-                    const bool stale = (q.info & kRevStale) != 0;
+                for (int k = 0; k < n_pred; ++k) {
+                    const int kb = kBlkHeader + kBlkPred * k;
+                    const uint32_t info = dw(win, pos, kb + 4);
+                    const bool stale = (info & kRevStale) != 0;
                     bool mine[kSpl];
 #pragma unroll
                     for (int i = 0; i < kSpl; ++i) {
-                        mine[i] = win[i] == k;
+                        mine[i] = win_k[i] == k;
                         if (mine[i] && !stale) wslot[i] = k;
                     }
-                    const uint32_t cls = (q.info & kRevClassMask) >> kRevClassShift;
-                    if (cls == kMulNone && !stale) return;
+                    const uint32_t cls = (info & kRevClassMask) >> kRevClassShift;
+                    if (cls == kMulNone && !stale) continue;
                     double dm = 0.0, ds = 0.0;
                     if (any_of<kSpl>(mine)) {
                         if (cls != kMulNone) {
+                            double dk[kSpl];
+                            if (k < kPre) {
+#pragma unroll
+                                for (int i = 0; i < kSpl; ++i) dk[i] = cur.d[k < kPre ? k : 0][i];
+                            } else if (reads_row(info)) {
+                                mult_row(info, dw(win, pos, kb + 3), dk);
+                            } else {
+#pragma unroll
+                                for (int i = 0; i < kSpl; ++i) dk[i] = 0.0;
+                            }
+                            const double a = f64(win, pos, kb);
                             double v[kSpl];
 #pragma unroll
-                            for (int i = 0; i < kSpl; ++i) v[i] = mine[i] ? gt[i] * (cls == kMulOne ? 1.0 : dk[i] * q.a) : 0.0;
+                            for (int i = 0; i < kSpl; ++i) v[i] = mine[i] ? gt[i] * (cls == kMulOne ? 1.0 : dk[i] * a) : 0.0;
                             dm = tile_sum<kSpl>(v);
                         }
                         if (stale) {
@@ -396,79 +504,39 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) reverse_kernel(RevParams p
                             ds = tile_sum<kSpl>(v);
                         }
                     }
-                    st.put(q.i1, dm, prow, lane);
-                    if (stale) st.put(q.i1 + 1, ds, prow, lane);
-                    ev_col = q.i1 + (stale ? 2 : 1);
-                };
-
-                // predecessors in groups of four: the winner first, then the columns
-                int32_t win[kSpl];
-#pragma unroll
-                for (int i = 0; i < kSpl; ++i) win[i] = -1;
-                if (any_open) {
-                    for (int k0 = 0; k0 < n_pred; k0 += kFastPreds) {
-                        Record q[kFastPreds];
-#pragma unroll
-                        for (int u = 0; u < kFastPreds; ++u)
-                            if (k0 + u < n_pred) q[u] = recs[pb + k0 + u];
-#pragma unroll
-                        for (int u = 0; u < kFastPreds; ++u) {
-                            if (k0 + u < n_pred) {
-                                bool m[kSpl];
-                                wins(q[u], m);
-#pragma unroll
-                                for (int i = 0; i < kSpl; ++i) win[i] = m[i] ? k0 + u : win[i];
-                            }
-                        }
-                    }
-                }
-                for (int k0 = 0; k0 < n_pred; k0 += kFastPreds) {
-                    Record q[kFastPreds];
-#pragma unroll
-                    for (int u = 0; u < kFastPreds; ++u)
-                        if (k0 + u < n_pred) q[u] = recs[pb + k0 + u];
-#pragma unroll
-                    for (int u = 0; u < kFastPreds; ++u) {
-                        if (k0 + u < n_pred) {
-                            double dk[kSpl];
-                            if (k0 == 0 && u < kPre) {
-#pragma unroll
-                                for (int i = 0; i < kSpl; ++i) dk[i] = cur.d[u < kPre ? u : 0][i];
-                            } else if (reads_row(q[u])) {
-                                mult_values(q[u], dk);
-                            } else {
-#pragma unroll
-                                for (int i = 0; i < kSpl; ++i) dk[i] = 0.0;
-                            }
-                            entry(q[u], k0 + u, win, dk);
-                        }
-                    }
+                    st.put(ev_col, dm, prow, lane);
+                    if (stale) st.put(ev_col + 1, ds, prow, lane);
+                    ev_col += stale ? 2 : 1;
                 }
                 // the event's earliest-time column follows its entries' columns
                 st.put(ev_col, any_of<kSpl>(ge_nz) ? tile_sum<kSpl>(ge) : 0.0, prow, lane);
                 ++ev_col;
 #pragma unroll
-                for (int i = 0; i < kSpl; ++i) {
-                    gres[j][i] = gt[i];
-                }
-                wres.set(j, kSpl == 1 ? wslot[0]
-                                      : (int32_t)(((uint32_t)wslot[0] & 0xFFFFu) | ((uint32_t)wslot[kSpl - 1] << 16)));
+                for (int i = 0; i < kSpl; ++i) gres[jj][i] = gt[i];
+                wres.set(jj, kSpl == 1 ? wslot[0]
+                                       : (int32_t)(((uint32_t)wslot[0] & 0xFFFFu) | ((uint32_t)wslot[kSpl - 1] << 16)));
+                // shift the pipeline
+                c0 = c1;
+                win0 = win1;
+                c1 = c2;
+                win1 = win2;
+                have1 = have2;
                 cur = nxt;
-                loaded = next;
+                we_cur = we_nxt;
             }
             st.flush(prow, lane);
             // Every slot read of this step is done before any is overwritten,
             // and every commit is visible before the next step reads it.
             __syncthreads();
 #pragma unroll 1
-            for (int j = 0; j < ne; ++j) {
-                const int32_t slot = ev_slot[e0 + j];
+            for (int jj = 0; jj < ne; ++jj) {
+                const int32_t slot = __builtin_amdgcn_readlane(slot_v, jj);
                 if (slot >= 0) {
                     int32_t w[kSpl];
-                    const int32_t x = wres.get(j);
+                    const int32_t x = wres.get(jj);
                     w[0] = kSpl == 1 ? x : (int32_t)(int16_t)(x & 0xFFFF);
                     if constexpr (kSpl == 2) w[1] = x >> 16;
-                    sl.write(slot, gres[j], w);
+                    sl.write(slot, gres[jj], w);
                 }
             }
             __syncthreads();

@@ -53,16 +53,23 @@ enum RevMul : uint32_t {
 
 constexpr int kRevEventsPerWave = 8;  // events of one wave and step (compile-time bound)
 
+// The reverse walk's per-event blocks, one stream per (step, wave), in dwords:
+//   [0] event  [1] slot (-1: none)  [2] predecessors | successors << 16  [3] dwords of the block
+//   [4..5] earliest  [6..7] upper (earliest + max_delay, rounded once)
+//   per predecessor, in the event's input order (kBlkPred dwords): [0..1] a (1 / d for kMulRead,
+//     1 otherwise), [2] source, [3] edge, [4] info (the forward record's kind and flags, kRevClass,
+//     kRevDup, kRevStale; kMulDraw: the unit row in the distribution-index bits)
+//   per successor (kBlkSucc dwords): the target's slot, this event's position in the target's list
+// A wave loads a block's first kBlkWindow dwords with one vector load (a dword per lane) and reads
+// fields with readlane; the stream ends in kBlkWindow padding dwords.
+constexpr int kBlkHeader = 8, kBlkPred = 5, kBlkSucc = 2, kBlkWindow = 64;
+
 struct RevParams {
-    const int32_t *step_off;  // [n_steps * waves + 1]: record range of (step, wave)
-    const Record *recs;
-    const int32_t *ev_off;    // [n_steps * waves + 1]: range of (step, wave) in ev_list
-    const int32_t *ev_list;   // events in stream order
-    const int32_t *ev_rec;    // their header records
-    const int32_t *ev_succ;   // their first successor entry
-    const int2 *succ;         // successor entries: (target's slot, position in the target's list)
+    const uint32_t *blocks;   // the block streams
+    const int32_t *blk_off;   // [n_steps * waves + 1]: first dword of each (step, wave) stream
+    const int32_t *ev_off;    // [n_steps * waves + 1]: event range of (step, wave)
     const int32_t *col_off;   // [n_steps * waves]: first partial-row column (step, wave) writes
-    const int32_t *ev_slot;   // their slots (-1: none)
+    const int32_t *ev_slot;   // slots of the events in stream order (-1: none)
     const DistParams *dists;  // the forward plan's distributions and alias tables (kMulDraw)
     const AliasRec *alias;
     const double *earliest;   // [E] (repeated sources read stale predecessors' earliest time)

@@ -207,6 +207,35 @@ def continuous_dag(seed: int, n_events: int, *, max_delay: float, shared_links: 
     return d
 
 
+def long_lived_slots(seed: int, n: int = 260, fan_in: int = 20, max_delay: float = 30.0) -> FlatDag:
+    """A reverse-walk stress DAG: n sources s_i, a hub m (predecessor s_0), n events t_i
+    (predecessors s_i and m) and a sink z (predecessors t_0 .. t_{fan_in - 1}).
+
+    The t_i's adjoints stay live from their own step to the sources' step, more slots than a
+    workgroup's LDS holds at one sample per lane, so some live in global memory; the hub has n
+    successors and the sink fan_in predecessors, so their per-event records run past the 64 dwords
+    a wave reads with one vector load."""
+    rng = np.random.default_rng(seed)
+    hub, sink = n, 2 * n + 1
+    t = [n + 1 + i for i in range(n)]
+    earliest = np.concatenate([rng.uniform(0.0, 10.0, n), [20.0], rng.uniform(30.0, 45.0, n), [80.0]])
+    events = [(float(x), float(x) + 200.0, 0.0) for x in earliest]
+    acts, prec = {}, []
+    dur = lambda: float(rng.uniform(1.0, 20.0))  # noqa: E731
+    acts[(0, hub)] = (n, dur(), 1)
+    prec.append((hub, [(0, n)]))
+    for i in range(n):
+        acts[(i, t[i])] = (i, dur(), i % 5)
+        acts[(hub, t[i])] = (n + 1 + i, dur(), (i + 2) % 5)
+        prec.append((t[i], [(i, i), (hub, n + 1 + i)]))
+    zp = []
+    for j in range(fan_in):
+        acts[(t[j], sink)] = (2 * n + 1 + j, dur(), j % 5)
+        zp.append((t[j], 2 * n + 1 + j))
+    prec.append((sink, zp))
+    return FlatDag.from_lists(events, acts, prec, max_delay, list(GRAD_DISTS))
+
+
 # ---- fixture (de)serialisation ------------------------------------------------
 _ARRAYS = ("earliest", "latest", "actual", "act_src", "act_dst", "act_idx", "act_dur", "act_type", "prec_target",
            "prec_off", "prec_src", "prec_edge")

@@ -229,6 +229,15 @@ def test_tail_route_estimates() -> None:
     assert i["steps"] == 20 and i["tail_wide_cost"] > i["tail_tile_cost"] > 0
 
 
+def test_reverse_plan_with_global_slots() -> None:
+    """The long-lived-slot DAG (tests/dags.py) compiles to a reverse plan whose live adjoint slots exceed
+    what LDS holds, at one sample per lane: the GPU gradient test of that DAG runs the global-slot path."""
+    import dags as D
+
+    info = capi.Propagator(D.long_lived_slots(3)).gradient_info()
+    assert info["samples_per_lane"] == 1 and info["slots"] > info["lds_slots"] > 0
+
+
 def test_batch_stats_checks_arguments_before_the_device() -> None:
     """The batch statistics reject a DAG whose rows exceed the memory budget, and bad statistics arguments,
     before touching a device."""

@@ -59,6 +59,8 @@ CASES = [
     ("random_stale", lambda: D.random_dag(np.random.default_rng(7), 120, duplicates=True, gaps=True, oob_edges=True,
                                           max_delay=30.0, dists=D.STD_DISTS), 257),
     ("sim6", lambda: D.sim6(D.STD_DISTS), 130),
+    # slots in global memory; records past the one-load window (a hub of 260 successors, a sink of 20 inputs)
+    ("long_lived_slots", lambda: D.long_lived_slots(3), 200),
 ]
 
 
