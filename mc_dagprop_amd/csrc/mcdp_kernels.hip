@@ -56,6 +56,24 @@ __device__ __forceinline__ void lds_store(double *lds, int slot, int lane, const
 // a 32 x 32 -> 64-bit product (rows and ld are non-negative and < 2^31).
 __device__ __forceinline__ uint64_t row_offset(int32_t row, uint32_t ld) { return (uint64_t)(uint32_t)row * ld; }
 
+// Issue priority from the share of its step's records a wave has left: 3 in
+// the first quarter down to 0 in the last.  A SIMD issues its oldest waves
+// first at equal priority, so with equal work its youngest wave would finish
+// every step last while the others wait at the step's barrier.  A wave later
+// on its SIMD (waves go to the four SIMDs in turn: rank wave / 4) counts
+// itself a further 1/16 of the step behind per rank.
+__device__ __forceinline__ void step_priority(int left, int total, int wave) {
+    const int l = left + (((wave >> 2) * total) >> 4);
+    if (4 * l > 3 * total)
+        __builtin_amdgcn_s_setprio(3);
+    else if (2 * l > total)
+        __builtin_amdgcn_s_setprio(2);
+    else if (4 * l > total)
+        __builtin_amdgcn_s_setprio(1);
+    else
+        __builtin_amdgcn_s_setprio(0);
+}
+
 // Alias tables staged in LDS after the slots (fast sampled variant), so an
 // empirical draw is a ds_read rather than a vector memory load.
 template <bool kInjected, int kPaths, int kSpl>
@@ -134,6 +152,7 @@ __device__ __forceinline__ void walk_tile(const WalkParams &p, int32_t tile, dou
             for (int r = r0; r < r1; ++r) {
                 const Record nxt = recs[r + 1];
                 if (cur.info & kInfoHeader) {
+                    step_priority(r1 - r, r1 - r0, wave);
                     upper = cur.b;
                     // one vector copy per event: the caps and the final clamp
                     // select against it without rebuilding it from SGPRs
