@@ -288,8 +288,9 @@ def bench_grad(args) -> tuple[int, dict]:
     if args.shape:
         prop.configure(*map(int, args.shape.split(",")))
     elif args.autotune != 0:
-        w_, k_, spl_, ms_ = prop.autotune(S if args.autotune < 0 else args.autotune)
-        log(f"autotune: waves={w_} events_per_wave={k_} samples_per_lane={spl_} ({ms_:.3f} ms)")
+        w_, k_, spl_, ms_ = prop.autotune(S if args.autotune < 0 else args.autotune, path=args.tune_file or None)
+        src = " (from the tuning file)" if prop.last_tune_from_file else ""
+        log(f"autotune: waves={w_} events_per_wave={k_} samples_per_lane={spl_} ({ms_:.3f} ms){src}")
     seeds = capi.DeviceBuffer.from_array(np.arange(S, dtype=np.int32))
     weights = capi.DeviceBuffer.from_array(w_h)
     realized, durations, cause = capi.DeviceBuffer(8 * E * S), capi.DeviceBuffer(8 * L * S), capi.DeviceBuffer(4 * E * S)
@@ -469,6 +470,9 @@ def main() -> int:
     ap.add_argument("--autotune", type=int, default=-1, metavar="N",
                     help="before the run, time every schedule shape on N samples (default: the workload's "
                     "sample count) and keep the fastest; 0 keeps the shape chosen at the first launch")
+    ap.add_argument("--tune-file", default="", metavar="PATH",
+                    help="with --autotune: a shape recorded in PATH for this DAG structure and device is used "
+                    "without timing; a new choice is appended to PATH")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write) --steps times, to calibrate "
                     "rocprofv3 FETCH_SIZE / WRITE_SIZE against known bytes")
@@ -517,8 +521,11 @@ def main() -> int:
         prop.configure(*map(int, args.shape.split(",")))
     elif args.autotune != 0:
         args.autotune = S if args.autotune < 0 else args.autotune
-        w, k, spl, ms = prop.autotune(args.autotune)  # setup, outside the timed steps
-        log(f"autotune on {args.autotune} samples: waves={w} events_per_wave={k} samples_per_lane={spl} ({ms:.3f} ms)")
+        # setup, outside the timed steps
+        w, k, spl, ms = prop.autotune(args.autotune, path=args.tune_file or None)
+        src = " (from the tuning file)" if prop.last_tune_from_file else ""
+        log(f"autotune on {args.autotune} samples: waves={w} events_per_wave={k} samples_per_lane={spl} "
+            f"({ms:.3f} ms){src}")
     log(f"workload {args.workload}: E={E} L={L} S={S}")
 
     seeds = capi.DeviceBuffer.from_array(np.arange(S, dtype=np.int32))

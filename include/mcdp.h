@@ -138,6 +138,26 @@ int mcdp_propagator_configure(mcdp_propagator *prop, int32_t waves, int32_t even
  * reference.) */
 int mcdp_propagator_autotune(mcdp_propagator *prop, int64_t n_samples, int32_t *waves, int32_t *events_per_wave,
                              int32_t *samples_per_lane, float *ms);
+/* A 64-bit fingerprint of what the schedule shape's speed depends on: the
+ * precedence structure (events, links, every entry's source, edge and stale
+ * flag), each link's distribution and each distribution's kind, and the alias
+ * tables' size; not times or distribution parameters (FNV-1a over those
+ * values).  (No counterpart in the reference.) */
+int mcdp_propagator_fingerprint(const mcdp_propagator *prop, uint64_t *fingerprint);
+/* mcdp_propagator_autotune with its choice kept in a text file, one line per
+ * choice:  mcdp-shape 1 <fingerprint, 16 hex digits> <device architecture>
+ * <compute units> <n_samples rounded up to a power of two> <waves>
+ * <events_per_wave> <samples_per_lane> <ms>.  The last line with this
+ * propagator's key on the current device is configured without running
+ * anything (*from_file = 1); with none (or an invalid or non-resident shape)
+ * the autotune runs and appends its line (*from_file = 0), in one write to the
+ * file opened for appending.  Lines that do not parse are skipped.  A file
+ * that cannot be read counts as empty; one that cannot be written fails the
+ * call with MCDP_E_INVALID after the shape is configured.  (No counterpart in
+ * the reference.) */
+int mcdp_propagator_autotune_cached(mcdp_propagator *prop, int64_t n_samples, const char *path, int32_t *waves,
+                                    int32_t *events_per_wave, int32_t *samples_per_lane, float *ms,
+                                    int32_t *from_file);
 
 /* ---- Batched run_many over several DAGs (no counterpart in the reference) - */
 /* The reference runs one DAG per Simulator; callers run several DAGs from

@@ -9,6 +9,7 @@ that check the C ABI itself.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import re
 from pathlib import Path
 
@@ -50,6 +51,12 @@ _SIGNATURES = {
     "mcdp_propagator_autotune": (
         C.c_int,
         [_P, _i64, C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_i32), C.POINTER(C.c_float)],
+    ),
+    "mcdp_propagator_fingerprint": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "mcdp_propagator_autotune_cached": (
+        C.c_int,
+        [_P, _i64, C.c_char_p, C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_i32), C.POINTER(C.c_float),
+         C.POINTER(_i32)],
     ),
     "mcdp_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "mcdp_malloc": (C.c_int, [C.POINTER(_P), C.c_size_t]),
@@ -246,6 +253,7 @@ class Propagator:
         ]
         e, ai, ad, at, pt, po, ps, pe = self._keep
         self.h = C.c_void_p(0)
+        self.last_tune_from_file = False
         check(
             lib().mcdp_propagator_create(
                 e.size,
@@ -366,12 +374,28 @@ class Propagator:
         check(lib().mcdp_reverse_device(self.h, p(seeds), p(weights), p(realized), p(durations), p(cause), n, ld,
                                         int(accumulate), p(out_obj), p(out_dmin), p(out_earliest), stream))
 
-    def autotune(self, n_samples: int) -> tuple[int, int, int, float]:
+    def autotune(self, n_samples: int, path: str | None = None) -> tuple[int, int, int, float]:
         """Times every candidate shape with n_samples samples; keeps and returns
-        (waves, events_per_wave, samples_per_lane, ms)."""
+        (waves, events_per_wave, samples_per_lane, ms).  With a tuning file, a choice recorded there for
+        this DAG structure and device is configured without timing anything, and a new choice is added to it
+        (``last_tune_from_file`` says which happened)."""
         w, k, spl, ms = _i32(0), _i32(0), _i32(0), C.c_float(0)
-        check(lib().mcdp_propagator_autotune(self.h, n_samples, C.byref(w), C.byref(k), C.byref(spl), C.byref(ms)))
+        if path is None:
+            check(lib().mcdp_propagator_autotune(self.h, n_samples, C.byref(w), C.byref(k), C.byref(spl),
+                                                 C.byref(ms)))
+            self.last_tune_from_file = False
+        else:
+            hit = _i32(0)
+            check(lib().mcdp_propagator_autotune_cached(self.h, n_samples, os.fsencode(path), C.byref(w), C.byref(k),
+                                                        C.byref(spl), C.byref(ms), C.byref(hit)))
+            self.last_tune_from_file = bool(hit.value)
         return w.value, k.value, spl.value, float(ms.value)
+
+    def fingerprint(self) -> int:
+        """64-bit fingerprint of what the schedule shape's speed depends on (structure, distribution kinds)."""
+        out = C.c_uint64(0)
+        check(lib().mcdp_propagator_fingerprint(self.h, C.byref(out)))
+        return int(out.value)
 
     def run_many(self, seeds) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
         """Host run_many: (realized (n,E), durations (n,L), cause (n,E)) as transposed views."""

@@ -12,9 +12,17 @@
 //   * links without a distribution keep minimal_duration (:303-306).
 #include <hip/hip_runtime_api.h>
 
+#include <fcntl.h>
+#include <unistd.h>
+
 #include <algorithm>
+#include <cerrno>
+#include <cinttypes>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <limits>
 #include <map>
 #include <memory>
@@ -1280,6 +1288,128 @@ int mcdp_propagator_autotune(mcdp_propagator *prop, int64_t n_samples, int32_t *
     if (events_per_wave) *events_per_wave = best.k;
     if (samples_per_lane) *samples_per_lane = best.spl;
     if (ms) *ms = best_ms;
+    return MCDP_OK;
+}
+
+// What a schedule shape's speed depends on: the precedence structure (events,
+// links, every entry's source, edge and stale flag), which distribution each
+// link draws from and each distribution's kind and flags (the code path of a
+// draw), and the alias tables' size (LDS).  Times and distribution
+// parameters are left out.  FNV-1a, 64 bits, over those values' bytes.
+static uint64_t shape_fingerprint(const mcdp_propagator &P) {
+    uint64_t h = 0xcbf29ce484222325ull;
+    auto mix = [&](const void *data, size_t n) {
+        const auto *b = static_cast<const unsigned char *>(data);
+        for (size_t i = 0; i < n; ++i) {
+            h ^= b[i];
+            h *= 0x100000001b3ull;
+        }
+    };
+    auto mix64 = [&](int64_t v) { mix(&v, sizeof v); };
+    mix("mcdp-shape-1", 12);
+    mix64(P.E);
+    mix64(P.L);
+    for (int64_t o : P.pred_off) mix64(o);
+    for (const PredEntry &pe : P.preds) {
+        mix64(pe.src);
+        mix64(pe.edge);
+        mix64(pe.stale ? 1 : 0);
+    }
+    for (uint32_t d : P.ldist) mix64(d);
+    for (const DistRec &d : P.dist) {
+        mix64(d.kind);
+        mix64(d.flags);
+    }
+    mix64((int64_t)P.alias.size());
+    return h;
+}
+
+int mcdp_propagator_fingerprint(const mcdp_propagator *prop, uint64_t *fingerprint) {
+    if (!prop || !fingerprint) return fail(MCDP_E_INVALID, "propagator or fingerprint is null");
+    *fingerprint = shape_fingerprint(*prop);
+    return MCDP_OK;
+}
+
+int mcdp_propagator_autotune_cached(mcdp_propagator *prop, int64_t n_samples, const char *path, int32_t *waves,
+                                    int32_t *events_per_wave, int32_t *samples_per_lane, float *ms,
+                                    int32_t *from_file) {
+    if (!prop) return fail(MCDP_E_INVALID, "propagator is null");
+    if (!path || !*path) return fail(MCDP_E_INVALID, "tuning file path is empty");
+    if (n_samples < 1) return fail(MCDP_E_INVALID, "n_samples must be positive");
+    int dev = 0;
+    hipDeviceProp_t props{};
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&props, dev) != hipSuccess)
+        return fail(MCDP_E_DEVICE, "no HIP device available");
+    // the device kind: architecture name up to its first ':' or space, and compute units
+    std::string arch(props.gcnArchName);
+    arch = arch.substr(0, arch.find_first_of(": "));
+    if (arch.empty()) arch = "unknown";
+    int64_t bucket = 1;
+    while (bucket < n_samples) bucket *= 2;
+    if (arch.size() > 64) arch.resize(64);
+    char key[160];
+    std::snprintf(key, sizeof key, "mcdp-shape 1 %016" PRIx64 " %s %d %" PRId64, shape_fingerprint(*prop), arch.c_str(),
+                  props.multiProcessorCount, bucket);
+    const std::string prefix = std::string(key) + " ";
+
+    // the last valid line for this key wins (a later tuning appends a newer choice)
+    bool found = false;
+    Shape got{0, 0, 0};
+    float got_ms = 0.0f;
+    {
+        std::ifstream in(path);
+        std::string line;
+        while (in && std::getline(in, line)) {
+            if (line.compare(0, prefix.size(), prefix) != 0) continue;
+            std::istringstream rest(line.substr(prefix.size()));
+            Shape c{0, 0, 0};
+            float t = 0.0f;
+            if (!(rest >> c.waves >> c.k >> c.spl >> t) || !shape_valid(c.waves, c.k, c.spl)) continue;
+            found = true;
+            got = c;
+            got_ms = t;
+        }
+    }
+    if (found) {
+        int rc = mcdp_propagator_configure(prop, got.waves, got.k, got.spl);
+        if (rc) return rc;
+        bool resident = false;
+        {
+            std::lock_guard<std::mutex> lock(prop->mu);
+            const WalkVariant v = walk_variant(*prop, false);
+            int bpc = 0;
+            resident = walk_occupancy(v, prop->waves, walk_lds_bytes(*prop, v), &bpc) == hipSuccess && bpc >= 1;
+        }
+        if (resident) {
+            if (waves) *waves = got.waves;
+            if (events_per_wave) *events_per_wave = got.k;
+            if (samples_per_lane) *samples_per_lane = got.spl;
+            if (ms) *ms = got_ms;
+            if (from_file) *from_file = 1;
+            return MCDP_OK;
+        }
+    }
+    int32_t w = 0, k = 0, spl = 0;
+    float t = 0.0f;
+    int rc = mcdp_propagator_autotune(prop, n_samples, &w, &k, &spl, &t);
+    if (rc) return rc;
+    if (waves) *waves = w;
+    if (events_per_wave) *events_per_wave = k;
+    if (samples_per_lane) *samples_per_lane = spl;
+    if (ms) *ms = t;
+    if (from_file) *from_file = 0;
+    // one write of the whole line to a file opened for appending: concurrent
+    // tunings add whole lines
+    char line[256];
+    const int len = std::snprintf(line, sizeof line, "%s%d %d %d %.6f\n", prefix.c_str(), w, k, spl, (double)t);
+    if (len <= 0 || (size_t)len >= sizeof line) return fail(MCDP_E_INVALID, "tuning file line too long");
+    const int fd = ::open(path, O_WRONLY | O_APPEND | O_CREAT | O_CLOEXEC, 0644);
+    if (fd < 0) return fail(MCDP_E_INVALID, std::string("cannot open tuning file ") + path + ": " + std::strerror(errno));
+    const ssize_t wr = ::write(fd, line, (size_t)len);
+    const int werr = errno;
+    ::close(fd);
+    if (wr != (ssize_t)len)
+        return fail(MCDP_E_INVALID, std::string("cannot write tuning file ") + path + ": " + std::strerror(werr));
     return MCDP_OK;
 }
 

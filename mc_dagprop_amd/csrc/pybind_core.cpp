@@ -478,16 +478,20 @@ PYBIND11_MODULE(_core, m) {
             "lane); results do not depend on it")
         .def(
             "autotune",
-            [](Propagator &p, int64_t n_samples) {
-                int32_t w = 0, k = 0, spl = 0;
+            [](Propagator &p, int64_t n_samples, std::optional<std::string> path) {
+                int32_t w = 0, k = 0, spl = 0, hit = 0;
                 float ms = 0.f;
                 {
                     py::gil_scoped_release nogil;
-                    check(mcdp_propagator_autotune(p.p, n_samples, &w, &k, &spl, &ms));
+                    if (path)
+                        check(mcdp_propagator_autotune_cached(p.p, n_samples, path->c_str(), &w, &k, &spl, &ms, &hit));
+                    else
+                        check(mcdp_propagator_autotune(p.p, n_samples, &w, &k, &spl, &ms));
                 }
-                return py::make_tuple(w, k, spl, ms);
+                return py::make_tuple(w, k, spl, ms, hit != 0);
             },
-            py::arg("n_samples"),
-            "Times each candidate schedule shape on n_samples samples and keeps the fastest; "
-            "returns (waves, events_per_wave, samples_per_lane, ms)");
+            py::arg("n_samples"), py::arg("path") = py::none(),
+            "Times each candidate schedule shape on n_samples samples and keeps the fastest; with a tuning "
+            "file, a choice recorded there for this DAG structure and device is configured without timing, and "
+            "a new choice is added to it.  Returns (waves, events_per_wave, samples_per_lane, ms, from_file)");
 }

@@ -214,6 +214,41 @@ def test_plan_is_host_side() -> None:
     assert info["samples_per_lane"] == 2 and info["lds_bytes"] == info["lds_slots"] * 1024
 
 
+def test_shape_fingerprint() -> None:
+    """The tuning key covers structure and distribution kinds, not times or distribution parameters (host-side);
+    the cached autotune checks its arguments and, without a device, fails instead of guessing a shape."""
+    import dags as D
+    from mc_dagprop_amd.workloads import layered_network
+
+    dag = layered_network(layers=6)
+    fp = capi.Propagator(dag).fingerprint()
+    assert fp == capi.Propagator(dag).fingerprint()
+    other_times = D.FlatDag(**{**dag.__dict__})
+    other_times.act_dur = dag.act_dur + 3.0
+    other_times.earliest = dag.earliest * 2.0
+    assert capi.Propagator(other_times).fingerprint() == fp
+    # other means and scales (a gamma keeps its shape: shapes 2 and 1/2 have draw paths of their own)
+    other_params = D.FlatDag(**{**dag.__dict__})
+    other_params.dists = [
+        x if x.values is not None else
+        D.DistSpec(x.activity_type, x.kind, (x.params[0], 1.5 * x.params[1], x.params[2]) if x.kind == D.GAMMA
+                   else (1.5 * x.params[0], x.params[1], x.params[2]))
+        for x in dag.dists]
+    assert capi.Propagator(other_params).fingerprint() == fp
+    shapes = D.FlatDag(**{**dag.__dict__})
+    shapes.dists = [D.DistSpec(x.activity_type, x.kind, (3.0, x.params[1], x.params[2])) if x.kind == D.GAMMA else x
+                    for x in dag.dists]
+    assert capi.Propagator(shapes).fingerprint() != fp
+    assert capi.Propagator(layered_network(layers=7)).fingerprint() != fp
+    assert capi.Propagator(D.sim6()).fingerprint() != capi.Propagator(D.sim6(D.STD_DISTS)).fingerprint()
+    P = capi.Propagator(dag)
+    with pytest.raises(capi.McdpError):
+        P.autotune(100, path="")
+    if capi.device_count() == 0:
+        with pytest.raises(capi.McdpError, match="device"):
+            P.autotune(100, path="/nonexistent/dir/shapes.txt")
+
+
 def test_tail_route_estimates() -> None:
     """The schedule's latency estimates of a last partial round (host-side): the 200-step network DAG, whose
     levels are 100 events wide, favours wide steps; the 20-step small DAG favours its tiles."""

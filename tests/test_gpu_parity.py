@@ -236,6 +236,42 @@ def test_autotune_keeps_results(gpu) -> None:
     np.testing.assert_array_equal(c, c0)
 
 
+def test_autotune_choice_persists_in_a_file(gpu, tmp_path) -> None:
+    """The first tuning times the shapes and appends one line; a new propagator of the same structure (other
+    times) reads it back without timing; unparsable lines and other keys are skipped; a recorded choice for
+    another sample-count bucket does not match."""
+    dag = layered_network(layers=12)
+    path = tmp_path / "shapes.txt"
+    path.write_text("garbage line\nmcdp-shape 1 0000000000000000 gfx000 1 4096 4 4 1 1.0\n")
+    P = capi.Propagator(dag)
+    w, k, spl, ms = P.autotune(4096, path=str(path))
+    assert not P.last_tune_from_file and ms > 0
+    lines = path.read_text().splitlines()
+    assert len(lines) == 3
+    f = lines[-1].split()
+    assert f[:3] == ["mcdp-shape", "1", f"{P.fingerprint():016x}"] and f[2] != "0000000000000000"
+    assert f[5] == "4096" and (int(f[6]), int(f[7]), int(f[8])) == (w, k, spl) and float(f[9]) > 0
+    # same structure, other times: the recorded shape, without timing
+    d2 = FlatDag(**{**dag.__dict__})
+    d2.act_dur = dag.act_dur * 1.5
+    Q = capi.Propagator(d2)
+    assert Q.fingerprint() == P.fingerprint()
+    assert Q.autotune(3000, path=str(path))[:3] == (w, k, spl) and Q.last_tune_from_file
+    info = Q.plan_info()
+    assert (info["waves"], info["events_per_wave"], info["samples_per_lane"]) == (w, k, spl)
+    # a later line for the key wins
+    other = (8, 4, 1) if (w, k, spl) != (8, 4, 1) else (16, 4, 1)
+    with open(path, "a") as fh:
+        fh.write(" ".join(f[:6]) + f" {other[0]} {other[1]} {other[2]} 9.5\n")
+    R = capi.Propagator(dag)
+    assert R.autotune(4096, path=str(path)) == (*other, 9.5) and R.last_tune_from_file
+    # another bucket of sample counts tunes again and appends
+    R.autotune(5000, path=str(path))
+    assert not R.last_tune_from_file and len(path.read_text().splitlines()) == 5
+    seeds = np.arange(200)
+    np.testing.assert_array_equal(bits(R.run_many(seeds)[0]), bits(P.run_many(seeds)[0]))
+
+
 @pytest.mark.parametrize("n", [1, 63, 64, 65, 127, 128, 129, 300])
 def test_two_samples_per_lane_edges(gpu, n: int) -> None:
     """Tiles of 128 samples: sizes around the tile and half-tile boundaries, device API with ld > n,
