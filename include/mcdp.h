@@ -330,10 +330,14 @@ int mcdp_gradient_host(mcdp_propagator *prop, const int32_t *seeds, int64_t n, c
  * thread-safe. */
 typedef struct mcdp_server mcdp_server;
 /* request_cap (0: memory_budget / 2) bounds one request's share of the pool:
- * a request needing more runs in chunks of samples that fit it, with the
- * exact mean and cause counts, and each quantile within max_delay / 4096 of
- * the exact one (order statistics placed inside a 4,096-bin histogram over
- * [earliest, earliest + max_delay]; needs a finite max_delay). */
+ * a request needing more runs in chunks of samples that fit it (needs a finite
+ * max_delay).  Its mean and cause counts come from running sums over the
+ * chunks; its quantiles from further passes over the same chunks (results are
+ * a function of the seeds) that gather, or narrow down in 256 sub-bins, the
+ * values around each order statistic a 4,096-bin histogram over [earliest,
+ * earliest + max_delay] placed: every statistic equals mcdp_run_stats_host's
+ * bit for bit.  Usually one further pass; each pass shrinks a region that
+ * holds too many values 256-fold. */
 int mcdp_server_create(int64_t memory_budget, int64_t request_cap, int32_t max_running, int32_t n_streams,
                        mcdp_server **out);
 /* Waits for running requests; fails queued ones; frees the pool. */
@@ -351,9 +355,15 @@ int mcdp_server_submit(mcdp_server *srv, int32_t ctx_id, const int32_t *seeds, i
  * completion; the request is then forgotten. */
 int mcdp_server_wait(mcdp_server *srv, int64_t request_id, double *mean, double *quantiles, int64_t *cause_counts,
                      double *latency_ms);
-/* out[0..8] = {submitted, completed, failed, queued, running, most running at
- * once, pool bytes in use, peak pool bytes in use, requests run in chunks}. */
+/* out[0..9] = {submitted, completed, failed, queued, running, most running at
+ * once, pool bytes in use, peak pool bytes in use, requests run in chunks,
+ * further passes over chunks for exact quantiles}. */
 int mcdp_server_stats(mcdp_server *srv, int64_t *out, int n_out);
+/* exact != 0 (the default): chunked requests get exact quantiles (further
+ * passes over their chunks).  exact == 0: no further passes; each quantile is
+ * then within max_delay / 4096 of the exact one (the order statistics placed
+ * inside their histogram bin), for lower latency. */
+int mcdp_server_set_exact_quantiles(mcdp_server *srv, int32_t exact);
 
 /* ---- Device plumbing for callers without their own HIP allocator -------- */
 /* (no counterpart in the reference, which has no device) */
@@ -414,6 +424,11 @@ int mcdp_debug_tail_mode(mcdp_propagator *prop, int32_t mode);
  * samples fit, as mcdp_server_submit does. */
 int mcdp_debug_serve_chunk(int64_t n, int64_t n_events, int64_t n_preds, int32_t n_q, int64_t request_cap,
                            int64_t *chunk, int64_t *bytes);
+/* Lowers the exact selection's limits for later requests: values gathered per
+ * region (1..4096), candidate space per target (1..256 values) and refine
+ * slots per request (1..1024), so tests reach the refining passes and the
+ * targets that wait for a slot. */
+int mcdp_debug_server_exact_limits(mcdp_server *srv, int32_t c_max, int64_t cand_per_target, int32_t refine_slots);
 
 #ifdef __cplusplus
 }

@@ -105,6 +105,8 @@ _SIGNATURES = {
     "mcdp_server_submit": (C.c_int, [_P, _i32, _pi32, _i64, _i32, _pf64, C.POINTER(_i64)]),
     "mcdp_server_wait": (C.c_int, [_P, _i64, _pf64, _pf64, _pi64, C.POINTER(_f64)]),
     "mcdp_server_stats": (C.c_int, [_P, _pi64, C.c_int]),
+    "mcdp_server_set_exact_quantiles": (C.c_int, [_P, _i32]),
+    "mcdp_debug_server_exact_limits": (C.c_int, [_P, _i32, _i64, _i32]),
 }
 
 _lib = None
@@ -545,14 +547,23 @@ class Server:
     """Statistics requests over resident contexts, admitted by device-memory footprint (mcdp_server_*)."""
 
     STAT_FIELDS = ["submitted", "completed", "failed", "queued", "running", "max_running", "bytes_in_use",
-                   "peak_bytes", "chunked"]
+                   "peak_bytes", "chunked", "exact_rounds"]
 
-    def __init__(self, memory_budget: int, max_running: int = 8, n_streams: int = 4, request_cap: int = 0) -> None:
-        """request_cap: a request needing more device memory runs in chunks of samples (0: half the pool)."""
+    def __init__(self, memory_budget: int, max_running: int = 8, n_streams: int = 4, request_cap: int = 0,
+                 exact_chunked_quantiles: bool = True) -> None:
+        """request_cap: a request needing more device memory runs in chunks of samples (0: half the pool).
+        exact_chunked_quantiles: such requests' quantiles are exact (further passes over their chunks); False
+        places them inside a histogram bin instead (within max_delay / 4096), for lower latency."""
         self.h = C.c_void_p(0)
         check(lib().mcdp_server_create(int(memory_budget), int(request_cap), max_running, n_streams, C.byref(self.h)))
         self._ctx: list[Propagator] = []
         self._req: dict[int, tuple[int, int, int]] = {}
+        if not exact_chunked_quantiles:
+            check(lib().mcdp_server_set_exact_quantiles(self.h, 0))
+
+    def debug_exact_limits(self, c_max: int, cand_per_target: int, refine_slots: int) -> None:
+        """Lower limits of the exact selection for later requests (test hook)."""
+        check(lib().mcdp_debug_server_exact_limits(self.h, c_max, cand_per_target, refine_slots))
 
     def add_context(self, prop: Propagator) -> int:
         cid = _i32(0)

@@ -204,6 +204,18 @@ int chunk_stats_accumulate(mcdp_propagator &P, const double *realized, const int
                            bool first, ChunkStat *st, uint32_t *hist, int64_t *counts, hipStream_t s);
 int chunk_stats_finish(mcdp_propagator &P, int64_t n, int32_t n_q, const double *q, const ChunkStat *st,
                        const uint32_t *hist, double *mean, double *quantiles, hipStream_t s);
+// Exact quantiles of a run in sample chunks (mcdp_stats.hpp, ExactWork): the
+// targets from the running statistics (w.n_ranks set from n and q), one chunk
+// of a further pass, the end of a pass, and the quantiles once every target
+// is resolved; asynchronous on s.  The number of distinct ranks n and q need.
+int exact_rank_count(int64_t n, int32_t n_q, const double *q, int32_t *n_ranks);
+int exact_stats_init(mcdp_propagator &P, int64_t n, int32_t n_q, const double *q, const ChunkStat *st,
+                     const uint32_t *hist, ExactWork &w, hipStream_t s);
+int exact_stats_pass(mcdp_propagator &P, const double *realized, int32_t m, int32_t ld, const ExactWork &w,
+                     hipStream_t s);
+int exact_stats_round_end(mcdp_propagator &P, const ExactWork &w, hipStream_t s);
+int exact_stats_finish(mcdp_propagator &P, int64_t n, int32_t n_q, const double *q, const ChunkStat *st,
+                       const ExactWork &w, double *quantiles, hipStream_t s);
 // The forward plan's record of link l (kind, flags, distribution index,
 // minimal duration, p0); an edge outside the link table is kind kNone.
 Record link_record(const mcdp_propagator &P, int32_t l);

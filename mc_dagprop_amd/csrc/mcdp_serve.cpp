@@ -13,8 +13,12 @@
 //   * completion is seen through an event, which returns the memory at once.
 // A request's results are those of mcdp_run_stats_host on the same seeds, bit
 // for bit: the kernels do not depend on what else runs.  A request larger than
-// the per-request cap runs in chunks of samples: exact mean and cause counts,
-// quantiles within one histogram bin (max_delay / 4096) of the exact ones.
+// the per-request cap runs in chunks of samples: running mean, extremes, cause
+// counts and a 4,096-bin histogram per event over the chunks, then further
+// passes over the same chunks (their results are a function of the seeds) that
+// gather or narrow down the values around each wanted order statistic, so its
+// quantiles are exact too.  Between passes the scheduler reads a few counters
+// back and launches the next pass; other requests keep running meanwhile.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -41,6 +45,35 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Exact quantiles of chunked requests.  A target's histogram bin holds about
+// n / 4,096 values where the row's values spread evenly, so candidate space is
+// n / 512 values per target on average (at least 16, at most 256; one target
+// may take up to kExactMaxCollect of it); refine slots: one per 64 targets
+// (at least 16, at most 1,024).  A request may take kMaxExactRounds further
+// passes before it fails (a refining pass shrinks a region 256-fold, so a few
+// suffice).
+constexpr int64_t kExactCandPerTarget = 256;
+constexpr int32_t kExactRefineSlots = 1024;
+constexpr int32_t kMaxExactRounds = 64;
+
+struct ExactSizes {
+    int64_t targets = 0, cand_per_target = 0;
+    int32_t slots = 0;
+    size_t bytes = 0;
+};
+ExactSizes exact_sizes(int64_t n, int64_t E, int64_t n_q) {
+    ExactSizes z;
+    if (n_q <= 0) return z;
+    z.targets = E * std::min<int64_t>(2 * n_q, kMaxRanks);
+    z.cand_per_target = std::min<int64_t>(kExactCandPerTarget, std::max<int64_t>(16, (n + 511) / 512));
+    z.slots = (int32_t)std::min<int64_t>(kExactRefineSlots, std::max<int64_t>(16, (z.targets + 63) / 64));
+    z.bytes = align_up((size_t)z.targets * sizeof(ExactTarget), 256) +
+              align_up((size_t)(z.targets * z.cand_per_target) * 8, 256) +
+              align_up((size_t)z.slots * kExactSubBins * 4, 256) + 2 * align_up((size_t)z.slots * kExactSubBins * 8, 256) +
+              256;
+    return z;
+}
 
 // Best-fit allocator over [0, size) with coalescing free blocks.
 class Pool {
@@ -101,6 +134,19 @@ struct Request {
     int stream = 0;
     hipEvent_t done = nullptr;
     double *h_out = nullptr;  // pinned: mean [E], quantiles [E * n_q], cause counts [E + P] (as int64)
+    // device buffers (in the pool) and, for chunked requests with quantiles,
+    // the exact selection: phase 1 while a pass (or its set-up) runs, whose
+    // counters come back to h_status; phase 2 once the results are on the way
+    int32_t *d_seeds = nullptr, *d_cause = nullptr;
+    double *d_real = nullptr, *d_mean = nullptr, *d_quant = nullptr;
+    int64_t *d_cnt = nullptr;
+    ChunkStat *d_st = nullptr;
+    uint32_t *d_hist = nullptr;
+    ExactWork exact{};
+    int phase = 0;
+    int rounds = 0;
+    int64_t buf_c0 = 0;  // first sample of the chunk whose rows d_real holds
+    uint64_t *h_status = nullptr;  // pinned: the exact selection's counters
     Clock::time_point t_submit, t_start, t_end;
     State state = State::Queued;
     int rc = MCDP_OK;
@@ -114,6 +160,7 @@ size_t footprint(int64_t n, int64_t rows, int64_t E, int64_t P, int64_t n_q, boo
                align_up((size_t)(E * rows) * 4, 256) + align_up((size_t)E * 8, 256) +
                align_up((size_t)(E * n_q) * 8, 256) + align_up((size_t)(E + P) * 8, 256);
     if (chunked) b += align_up((size_t)E * sizeof(ChunkStat), 256) + align_up((size_t)E * kMaxChunkBins * 4, 256);
+    if (chunked) b += exact_sizes(n, E, n_q).bytes;  // targets, candidates, refine slots, counters
     return b;
 }
 
@@ -151,15 +198,21 @@ struct mcdp_server {
     std::vector<Request *> running;
     std::unordered_map<int64_t, std::unique_ptr<Request>> all;
     int64_t next_id = 1;
-    int64_t n_submitted = 0, n_completed = 0, n_failed = 0, max_running_seen = 0, n_chunked = 0;
+    int64_t n_submitted = 0, n_completed = 0, n_failed = 0, max_running_seen = 0, n_chunked = 0, n_rounds = 0;
     bool stop = false;
+    bool exact_chunked = true;  // exact quantiles for chunked requests (further passes)
+    int32_t exact_c_max = kExactMaxCollect, exact_slots = kExactRefineSlots;
+    int64_t exact_cand_per_target = kExactCandPerTarget;
     std::thread worker;
 
     void run();
     void start(Request &r);
+    void finish_failed(Request &r);
 };
 
 namespace {
+
+int copy_results(mcdp_server &srv, Request &r);
 
 // Launches a request's work on its stream; the caller holds srv.mu.
 int launch_request(mcdp_server &srv, Request &r) {
@@ -173,19 +226,19 @@ int launch_request(mcdp_server &srv, Request &r) {
         o += align_up(bytes, 256);
         return p;
     };
-    auto *d_seeds = reinterpret_cast<int32_t *>(take((size_t)n * 4));
-    auto *d_real = reinterpret_cast<double *>(take((size_t)(E * rows) * 8));
-    auto *d_cause = reinterpret_cast<int32_t *>(take((size_t)(E * rows) * 4));
-    auto *d_mean = reinterpret_cast<double *>(take((size_t)E * 8));
-    auto *d_quant = reinterpret_cast<double *>(take((size_t)(E * nq) * 8));
-    auto *d_cnt = reinterpret_cast<int64_t *>(take((size_t)(E + r.P) * 8));
+    auto *d_seeds = r.d_seeds = reinterpret_cast<int32_t *>(take((size_t)n * 4));
+    auto *d_real = r.d_real = reinterpret_cast<double *>(take((size_t)(E * rows) * 8));
+    auto *d_cause = r.d_cause = reinterpret_cast<int32_t *>(take((size_t)(E * rows) * 4));
+    auto *d_mean = r.d_mean = reinterpret_cast<double *>(take((size_t)E * 8));
+    auto *d_quant = r.d_quant = reinterpret_cast<double *>(take((size_t)(E * nq) * 8));
+    auto *d_cnt = r.d_cnt = reinterpret_cast<int64_t *>(take((size_t)(E + r.P) * 8));
     const hipStream_t s = srv.streams[(size_t)r.stream];
     HIP_TRY(hipMemcpyAsync(d_seeds, r.seeds.data(), (size_t)n * 4, hipMemcpyHostToDevice, s), "copy seeds");
     int rc;
     if (r.chunk > 0) {
         // run_many chunk by chunk into the same rows; statistics merged across chunks
-        auto *d_st = reinterpret_cast<ChunkStat *>(take((size_t)E * sizeof(ChunkStat)));
-        auto *d_hist = reinterpret_cast<uint32_t *>(take((size_t)E * kMaxChunkBins * 4));
+        auto *d_st = r.d_st = reinterpret_cast<ChunkStat *>(take((size_t)E * sizeof(ChunkStat)));
+        auto *d_hist = r.d_hist = reinterpret_cast<uint32_t *>(take((size_t)E * kMaxChunkBins * 4));
         for (int64_t c0 = 0; c0 < n; c0 += r.chunk) {
             const int64_t m = std::min<int64_t>(r.chunk, n - c0);
             rc = mcdp_run_many_device(prop, d_seeds + c0, m, r.chunk, d_real, nullptr, d_cause, s);
@@ -193,8 +246,35 @@ int launch_request(mcdp_server &srv, Request &r) {
             rc = chunk_stats_accumulate(*prop, d_real, d_cause, (int32_t)m, (int32_t)r.chunk, c0 == 0, d_st, d_hist,
                                         d_cnt, s);
             if (rc) return rc;
+            r.buf_c0 = c0;
         }
-        rc = chunk_stats_finish(*prop, n, (int32_t)nq, r.q.data(), d_st, d_hist, d_mean, d_quant, s);
+        const bool exact = srv.exact_chunked && nq > 0;
+        rc = chunk_stats_finish(*prop, n, (int32_t)nq, r.q.data(), d_st, d_hist, d_mean, exact ? nullptr : d_quant, s);
+        if (rc) return rc;
+        if (exact) {
+            // targets from the histograms; the scheduler continues once the counters are back
+            const ExactSizes z = exact_sizes(n, E, nq);
+            ExactWork &w = r.exact;
+            w.targets = reinterpret_cast<ExactTarget *>(take((size_t)z.targets * sizeof(ExactTarget)));
+            w.cand = reinterpret_cast<double *>(take((size_t)(z.targets * z.cand_per_target) * 8));
+            w.slot_cnt = reinterpret_cast<uint32_t *>(take((size_t)z.slots * kExactSubBins * 4));
+            w.slot_min = reinterpret_cast<uint64_t *>(take((size_t)z.slots * kExactSubBins * 8));
+            w.slot_max = reinterpret_cast<uint64_t *>(take((size_t)z.slots * kExactSubBins * 8));
+            w.counters = reinterpret_cast<uint64_t *>(take(256));
+            w.n_ranks = (int32_t)std::min<int64_t>(2 * nq, kMaxRanks);
+            w.n_slots = std::min(srv.exact_slots, z.slots);
+            w.c_max = std::min(srv.exact_c_max, kExactMaxCollect);
+            w.cand_cap = (uint32_t)std::min<int64_t>(z.targets * std::min(srv.exact_cand_per_target, z.cand_per_target),
+                                                     std::numeric_limits<int32_t>::max());
+            if (o > r.bytes) return fail(MCDP_E_INVALID, "internal error: request layout exceeds its footprint");
+            rc = exact_stats_init(*prop, n, (int32_t)nq, r.q.data(), d_st, d_hist, w, s);
+            if (rc) return rc;
+            HIP_TRY(hipMemcpyAsync(r.h_status, w.counters, 4 * 8, hipMemcpyDeviceToHost, s), "copy counters");
+            HIP_TRY(hipEventCreateWithFlags(&r.done, hipEventDisableTiming), "hipEventCreate");
+            HIP_TRY(hipEventRecord(r.done, s), "hipEventRecord");
+            r.phase = 1;
+            return MCDP_OK;
+        }
     } else {
         rc = mcdp_run_many_device(prop, d_seeds, n, n, d_real, nullptr, d_cause, s);
         if (rc) return rc;
@@ -202,12 +282,60 @@ int launch_request(mcdp_server &srv, Request &r) {
                                      nq ? d_quant : nullptr, d_cnt, s);
     }
     if (rc) return rc;
+    return copy_results(srv, r);
+}
+
+// Enqueues the copies of a request's results and its completion event.
+int copy_results(mcdp_server &srv, Request &r) {
+    const int64_t E = r.E, nq = (int64_t)r.q.size();
+    const hipStream_t s = srv.streams[(size_t)r.stream];
     double *h = r.h_out;
-    HIP_TRY(hipMemcpyAsync(h, d_mean, (size_t)E * 8, hipMemcpyDeviceToHost, s), "copy results");
-    if (nq) HIP_TRY(hipMemcpyAsync(h + E, d_quant, (size_t)(E * nq) * 8, hipMemcpyDeviceToHost, s), "copy results");
-    HIP_TRY(hipMemcpyAsync(h + E + E * nq, d_cnt, (size_t)(E + r.P) * 8, hipMemcpyDeviceToHost, s), "copy results");
+    HIP_TRY(hipMemcpyAsync(h, r.d_mean, (size_t)E * 8, hipMemcpyDeviceToHost, s), "copy results");
+    if (nq) HIP_TRY(hipMemcpyAsync(h + E, r.d_quant, (size_t)(E * nq) * 8, hipMemcpyDeviceToHost, s), "copy results");
+    HIP_TRY(hipMemcpyAsync(h + E + E * nq, r.d_cnt, (size_t)(E + r.P) * 8, hipMemcpyDeviceToHost, s), "copy results");
     HIP_TRY(hipEventCreateWithFlags(&r.done, hipEventDisableTiming), "hipEventCreate");
     HIP_TRY(hipEventRecord(r.done, s), "hipEventRecord");
+    r.phase = 2;
+    return MCDP_OK;
+}
+
+// The exact selection's next step, once the counters of the last one are
+// back: another pass over every chunk while targets are open, else the
+// quantiles and the results' copies.
+int continue_exact(mcdp_server &srv, Request &r) {
+    const uint64_t open = r.h_status[2], bad = r.h_status[3];
+    if (bad) return fail(MCDP_E_DEVICE, "internal error: exact quantile selection found inconsistent counts");
+    mcdp_propagator *prop = srv.contexts[(size_t)r.ctx];
+    const hipStream_t s = srv.streams[(size_t)r.stream];
+    const int64_t n = (int64_t)r.seeds.size(), nq = (int64_t)r.q.size();
+    if (open == 0) {
+        int rc = exact_stats_finish(*prop, n, (int32_t)nq, r.q.data(), r.d_st, r.exact, r.d_quant, s);
+        if (rc) return rc;
+        return copy_results(srv, r);
+    }
+    if (r.rounds >= kMaxExactRounds) return fail(MCDP_E_DEVICE, "exact quantile selection did not converge");
+    ++r.rounds;
+    ++srv.n_rounds;
+    // the chunk still in the rows first, then the others walked again (no cause rows: the counts are done)
+    const int64_t kept = r.buf_c0;
+    int rc = exact_stats_pass(*prop, r.d_real, (int32_t)std::min<int64_t>(r.chunk, n - kept), (int32_t)r.chunk,
+                              r.exact, s);
+    if (rc) return rc;
+    for (int64_t c0 = 0; c0 < n; c0 += r.chunk) {
+        if (c0 == kept) continue;
+        const int64_t m = std::min<int64_t>(r.chunk, n - c0);
+        rc = mcdp_run_many_device(prop, r.d_seeds + c0, m, r.chunk, r.d_real, nullptr, nullptr, s);
+        if (rc) return rc;
+        rc = exact_stats_pass(*prop, r.d_real, (int32_t)m, (int32_t)r.chunk, r.exact, s);
+        if (rc) return rc;
+        r.buf_c0 = c0;
+    }
+    rc = exact_stats_round_end(*prop, r.exact, s);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(r.h_status, r.exact.counters, 4 * 8, hipMemcpyDeviceToHost, s), "copy counters");
+    HIP_TRY(hipEventCreateWithFlags(&r.done, hipEventDisableTiming), "hipEventCreate");
+    HIP_TRY(hipEventRecord(r.done, s), "hipEventRecord");
+    r.phase = 1;
     return MCDP_OK;
 }
 
@@ -221,24 +349,29 @@ void mcdp_server::start(Request &r) {
     r.t_start = Clock::now();
     r.rc = launch_request(*this, r);
     if (r.rc) {
-        r.error = mcdp_last_error();
-        r.state = State::Failed;
-        // work already enqueued must finish before its memory is reused
-        (void)hipStreamSynchronize(streams[(size_t)r.stream]);
-        if (r.done) {
-            (void)hipEventDestroy(r.done);
-            r.done = nullptr;
-        }
-        pool->release(r.off, r.bytes);
-        r.t_end = Clock::now();
-        ++n_failed;
-        cv_done.notify_all();
+        finish_failed(r);
         return;
     }
     ++stream_load[(size_t)r.stream];
     r.state = State::Running;
     running.push_back(&r);
     max_running_seen = std::max<int64_t>(max_running_seen, (int64_t)running.size());
+}
+
+// A request whose launch failed: its enqueued work finishes before its memory
+// returns to the pool.  The caller holds mu; r is not in `running`.
+void mcdp_server::finish_failed(Request &r) {
+    r.error = mcdp_last_error();
+    r.state = State::Failed;
+    (void)hipStreamSynchronize(streams[(size_t)r.stream]);
+    if (r.done) {
+        (void)hipEventDestroy(r.done);
+        r.done = nullptr;
+    }
+    pool->release(r.off, r.bytes);
+    r.t_end = Clock::now();
+    ++n_failed;
+    cv_done.notify_all();
 }
 
 void mcdp_server::run() {
@@ -252,6 +385,20 @@ void mcdp_server::run() {
             const hipError_t e = hipEventQuery(r.done);
             if (e == hipErrorNotReady) {
                 ++i;
+                continue;
+            }
+            if (e == hipSuccess && r.phase == 1) {  // an exact-selection step is done: launch the next
+                (void)hipEventDestroy(r.done);
+                r.done = nullptr;
+                r.rc = continue_exact(*this, r);
+                if (r.rc == MCDP_OK) {
+                    ++i;
+                    continue;
+                }
+                --stream_load[(size_t)r.stream];
+                running.erase(running.begin() + (std::ptrdiff_t)i);
+                finish_failed(r);
+                changed = true;
                 continue;
             }
             r.t_end = Clock::now();
@@ -405,9 +552,11 @@ int mcdp_server_submit(mcdp_server *srv, int32_t ctx_id, const int32_t *seeds, i
         return fail(MCDP_E_NOMEM, "request footprint of " + std::to_string(r->bytes) +
                                       " bytes exceeds the request cap of " + std::to_string(srv->request_cap) +
                                       " bytes");
+    // results, then the exact selection's four counters
     const size_t host_doubles = (size_t)(r->E + r->E * n_q + r->E + r->P);
-    hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&r->h_out), std::max<size_t>(host_doubles, 1) * 8);
+    hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&r->h_out), (host_doubles + 4) * 8);
     if (e != hipSuccess) return fail(MCDP_E_NOMEM, std::string("hipHostMalloc (results): ") + hipGetErrorString(e));
+    r->h_status = reinterpret_cast<uint64_t *>(r->h_out + host_doubles);
     r->t_submit = Clock::now();
     {
         std::lock_guard<std::mutex> lock(srv->mu);
@@ -475,10 +624,30 @@ int mcdp_debug_serve_chunk(int64_t n, int64_t n_events, int64_t n_preds, int32_t
 int mcdp_server_stats(mcdp_server *srv, int64_t *out, int n_out) {
     if (!srv || !out) return fail(MCDP_E_INVALID, "null argument");
     std::lock_guard<std::mutex> lock(srv->mu);
-    const int64_t v[9] = {srv->n_submitted,          srv->n_completed,         srv->n_failed,
-                          (int64_t)srv->queued.size(), (int64_t)srv->running.size(), srv->max_running_seen,
-                          (int64_t)srv->pool->used(), (int64_t)srv->pool->peak(), srv->n_chunked};
-    for (int i = 0; i < n_out && i < 9; ++i) out[i] = v[i];
+    const int64_t v[10] = {srv->n_submitted,          srv->n_completed,         srv->n_failed,
+                           (int64_t)srv->queued.size(), (int64_t)srv->running.size(), srv->max_running_seen,
+                           (int64_t)srv->pool->used(), (int64_t)srv->pool->peak(), srv->n_chunked,
+                           srv->n_rounds};
+    for (int i = 0; i < n_out && i < 10; ++i) out[i] = v[i];
+    return MCDP_OK;
+}
+
+int mcdp_server_set_exact_quantiles(mcdp_server *srv, int32_t exact) {
+    if (!srv) return fail(MCDP_E_INVALID, "server is null");
+    std::lock_guard<std::mutex> lock(srv->mu);
+    srv->exact_chunked = exact != 0;
+    return MCDP_OK;
+}
+
+int mcdp_debug_server_exact_limits(mcdp_server *srv, int32_t c_max, int64_t cand_per_target, int32_t refine_slots) {
+    if (!srv) return fail(MCDP_E_INVALID, "server is null");
+    if (c_max < 1 || c_max > kExactMaxCollect || cand_per_target < 1 || cand_per_target > kExactCandPerTarget ||
+        refine_slots < 1 || refine_slots > kExactRefineSlots)
+        return fail(MCDP_E_INVALID, "limits out of range");
+    std::lock_guard<std::mutex> lock(srv->mu);
+    srv->exact_c_max = c_max;
+    srv->exact_cand_per_target = cand_per_target;
+    srv->exact_slots = refine_slots;
     return MCDP_OK;
 }
 

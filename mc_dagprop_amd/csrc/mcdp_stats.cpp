@@ -142,6 +142,55 @@ int mcdp::chunk_stats_finish(mcdp_propagator &P, int64_t n, int32_t n_q, const d
     return MCDP_OK;
 }
 
+int mcdp::exact_rank_count(int64_t n, int32_t n_q, const double *q, int32_t *n_ranks) {
+    QuantileArgs qa;
+    const int rc = quantile_args(n, n_q, q, qa);
+    if (rc) return rc;
+    *n_ranks = qa.n_ranks;
+    return MCDP_OK;
+}
+
+int mcdp::exact_stats_init(mcdp_propagator &P, int64_t n, int32_t n_q, const double *q, const ChunkStat *st,
+                           const uint32_t *hist, ExactWork &w, hipStream_t s) {
+    QuantileArgs qa;
+    int rc = quantile_args(n, n_q, q, qa);
+    if (rc) return rc;
+    if (qa.n_ranks > w.n_ranks) return fail(MCDP_E_INVALID, "internal error: exact quantile workspace too small");
+    w.n_ranks = qa.n_ranks;
+    std::lock_guard<std::mutex> lock(P.mu);
+    DeviceCopy *d = nullptr;
+    rc = ensure_device(P, d);
+    if (rc) return rc;
+    HIP_TRY(launch_exact_init(n, P.E, qa, st, hist, kMaxChunkBins, d->base.earliest, P.max_delay, w, s),
+            "exact quantile kernels");
+    return MCDP_OK;
+}
+
+int mcdp::exact_stats_pass(mcdp_propagator &P, const double *realized, int32_t m, int32_t ld, const ExactWork &w,
+                           hipStream_t s) {
+    std::lock_guard<std::mutex> lock(P.mu);
+    DeviceCopy *d = nullptr;
+    int rc = ensure_device(P, d);
+    if (rc) return rc;
+    HIP_TRY(launch_exact_pass(realized, m, ld, P.E, d->base.earliest, P.max_delay, kMaxChunkBins, w, s),
+            "exact quantile pass kernel");
+    return MCDP_OK;
+}
+
+int mcdp::exact_stats_round_end(mcdp_propagator &P, const ExactWork &w, hipStream_t s) {
+    HIP_TRY(launch_exact_round_end(P.E, w, s), "exact quantile kernels");
+    return MCDP_OK;
+}
+
+int mcdp::exact_stats_finish(mcdp_propagator &P, int64_t n, int32_t n_q, const double *q, const ChunkStat *st,
+                             const ExactWork &w, double *quantiles, hipStream_t s) {
+    QuantileArgs qa;
+    const int rc = quantile_args(n, n_q, q, qa);
+    if (rc) return rc;
+    HIP_TRY(launch_exact_finish(P.E, qa, st, w, quantiles, s), "exact quantile kernel");
+    return MCDP_OK;
+}
+
 int mcdp::prepare_stats(mcdp_propagator &P) {
     std::lock_guard<std::mutex> lock(P.mu);
     DeviceCopy *d = nullptr;

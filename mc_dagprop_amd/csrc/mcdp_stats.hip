@@ -668,6 +668,14 @@ __global__ void __launch_bounds__(kBlock) cause_kernel(const int32_t *__restrict
 }
 
 // ---- statistics over sample chunks ------------------------------------------
+// The histogram bin of v in its event's range [lo, lo + max_delay]
+// (inv_width = n_bins / max_delay): monotone in v.
+__device__ __forceinline__ int32_t value_bin(double v, double lo, double inv_width, int32_t n_bins) {
+    double f = (v - lo) * inv_width;
+    f = f < 0.0 ? 0.0 : f;
+    return f < (double)n_bins ? (int32_t)f : n_bins - 1;
+}
+
 // An event's running statistics over chunks of samples: the exactly rounded
 // sum (double-double, with the plain sum for non-finite rows) and the
 // extremes, merged in chunk order; and a histogram of `n_bins` bins over the
@@ -706,10 +714,7 @@ __global__ void __launch_bounds__(kBlock) chunk_accumulate_kernel(const double *
                 const uint64_t key = order_key(v);
                 x = merge_extremes(x, Extremes{key, key, 1u, 1u, 0u});
             }
-            double f = (v - lo) * inv_width;
-            f = f < 0.0 ? 0.0 : f;
-            int32_t b = f < (double)n_bins ? (int32_t)f : n_bins - 1;
-            wave_hist_add(h, ok, (uint32_t)b, lane);
+            wave_hist_add(h, ok, (uint32_t)value_bin(v, lo, inv_width, n_bins), lane);
         };
         int32_t s0 = 0;
         for (; s0 + 4 * kBlock <= m; s0 += 4 * kBlock) {
@@ -804,12 +809,288 @@ __global__ void __launch_bounds__(kBlock) chunk_finish_kernel(int64_t n, int32_t
                 }
                 stat[r] = v;
             }
-            for (int j = 0; j < qa.n_q; ++j)
-                out[(uint64_t)(uint32_t)e * (uint32_t)qa.n_q + j] =
-                    c.nans ? __builtin_nan("") : lerp_numpy(stat[qa.lo[j]], stat[qa.hi[j]], qa.gamma[j]);
+            if (out)
+                for (int j = 0; j < qa.n_q; ++j)
+                    out[(uint64_t)(uint32_t)e * (uint32_t)qa.n_q + j] =
+                        c.nans ? __builtin_nan("") : lerp_numpy(stat[qa.lo[j]], stat[qa.hi[j]], qa.gamma[j]);
         }
         __syncthreads();
     }
+}
+
+// ---- exact order statistics over further passes of the chunks ---------------
+// The sub-bin of a value in a refined region: even splits of its value range,
+// or of its key range once it spans few representable doubles.  Monotone in v.
+__device__ __forceinline__ int32_t exact_sub_bin(const ExactTarget &t, double v, uint64_t key) {
+    if (t.inv != 0.0) {
+        double f = (v - t.a) * t.inv;
+        f = f < 0.0 ? 0.0 : f;
+        return f < (double)kExactSubBins ? (int32_t)f : kExactSubBins - 1;
+    }
+    return (int32_t)((key - t.klo) >> t.shift);
+}
+
+// Targets from the chunks' histograms: as chunk_finish_kernel places a rank,
+// but keeping its bin and rank inside the bin instead of interpolating.
+__global__ void __launch_bounds__(kBlock) exact_init_kernel(int64_t n, int32_t E, QuantileArgs qa,
+                                                            const ChunkStat *__restrict__ st,
+                                                            const uint32_t *__restrict__ hist, int32_t n_bins,
+                                                            const double *__restrict__ earliest, double width,
+                                                            ExactWork w) {
+    __shared__ uint64_t seg[kBlock];
+    const int per = n_bins / kBlock;
+    for (int32_t e = blockIdx.x; e < E; e += gridDim.x) {
+        const ChunkStat c = st[e];
+        const uint32_t *h = hist + (uint64_t)(uint32_t)e * (uint32_t)n_bins;
+        uint64_t sum = 0;
+        for (int i = 0; i < per; ++i) sum += h[threadIdx.x * per + i];
+        seg[threadIdx.x] = sum;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const double lo = earliest[e];
+            const int64_t valid = n - (int64_t)c.nans;
+            for (int r = 0; r < w.n_ranks; ++r) {
+                ExactTarget t{};
+                t.slot = -1;
+                t.mode = kExactResolved;
+                const int64_t k = r < qa.n_ranks ? qa.rank[r] : 0;
+                if (c.nans) {
+                    t.value = __builtin_nan("");
+                } else if (k >= valid - (int64_t)c.cmx) {
+                    t.value = key_value(c.mx);
+                } else if (k < (int64_t)c.cmn) {
+                    t.value = key_value(c.mn);
+                } else {
+                    int64_t cum = 0;
+                    int g = 0;
+                    while (g < kBlock - 1 && cum + (int64_t)seg[g] <= k) cum += (int64_t)seg[g++];
+                    int b = g * per;
+                    while (b < (g + 1) * per - 1 && cum + (int64_t)h[b] <= k) cum += (int64_t)h[b++];
+                    t.bin = b;
+                    t.rank = k - cum;
+                    t.count = h[b];
+                    t.klo = 0;
+                    t.khi = ~0ull;
+                    t.a = lo + (double)b * width;
+                    t.inv = (double)kExactSubBins / width;
+                    t.mode = kExactWait;
+                    if (t.rank < 0 || t.rank >= (int64_t)t.count) {
+                        w.counters[3] = 1;  // the histogram does not hold the rank
+                        t.mode = kExactResolved;
+                        t.value = __builtin_nan("");
+                    }
+                }
+                w.targets[(uint64_t)(uint32_t)e * (uint32_t)w.n_ranks + r] = t;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Hands out this round's work: a waiting region with at most c_max values
+// gets candidate space (while the buffer lasts), otherwise a refine slot
+// (while slots last), otherwise waits for the next round.  Counts the
+// unresolved targets.
+__global__ void __launch_bounds__(kBlock) exact_assign_kernel(int32_t E, ExactWork w) {
+    __shared__ int32_t new_slot[kMaxRanks];
+    for (int32_t e = blockIdx.x; e < E; e += gridDim.x) {
+        ExactTarget *tg = w.targets + (uint64_t)(uint32_t)e * (uint32_t)w.n_ranks;
+        if (threadIdx.x == 0) {
+            for (int r = 0; r < w.n_ranks; ++r) {
+                new_slot[r] = -1;
+                ExactTarget t = tg[r];
+                if (t.mode == kExactResolved) continue;
+                t.mode = kExactWait;
+                t.slot = -1;
+                if (t.count <= (uint32_t)w.c_max) {
+                    const uint64_t off = atomicAdd((unsigned long long *)&w.counters[0], (unsigned long long)t.count);
+                    if (off + t.count <= (uint64_t)w.cand_cap) {
+                        t.mode = kExactCollect;
+                        t.off = (uint32_t)off;
+                        t.fill = 0;
+                    }
+                }
+                if (t.mode == kExactWait) {
+                    const uint64_t sl = atomicAdd((unsigned long long *)&w.counters[1], 1ull);
+                    if (sl < (uint64_t)w.n_slots) {
+                        t.mode = kExactRefine;
+                        t.slot = (int32_t)sl;
+                        new_slot[r] = (int32_t)sl;
+                    }
+                }
+                atomicAdd((unsigned long long *)&w.counters[2], 1ull);
+                tg[r] = t;
+            }
+        }
+        __syncthreads();
+        for (int r = 0; r < w.n_ranks; ++r) {
+            const int32_t sl = new_slot[r];
+            if (sl < 0) continue;
+            const uint64_t base = (uint64_t)(uint32_t)sl * kExactSubBins;
+            for (int i = threadIdx.x; i < kExactSubBins; i += kBlock) {
+                w.slot_cnt[base + i] = 0u;
+                w.slot_min[base + i] = ~0ull;
+                w.slot_max[base + i] = 0ull;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void exact_reset_kernel(uint64_t *counters, int all) {
+    counters[0] = 0;
+    counters[1] = 0;
+    counters[2] = 0;
+    if (all) counters[3] = 0;
+}
+
+// One chunk of a pass: every value of an open region is gathered (a wave's
+// matching lanes take consecutive places) or counted in its sub-bin.
+__global__ void __launch_bounds__(kBlock) exact_pass_kernel(const double *__restrict__ realized, int32_t m,
+                                                            uint32_t ld, int32_t E,
+                                                            const double *__restrict__ earliest, double inv_width,
+                                                            int32_t n_bins, ExactWork w) {
+    __shared__ ExactTarget tg[kMaxRanks];
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int32_t e = blockIdx.x; e < E; e += gridDim.x) {
+        ExactTarget *gt = w.targets + (uint64_t)(uint32_t)e * (uint32_t)w.n_ranks;
+        if ((int)threadIdx.x < w.n_ranks) tg[threadIdx.x] = gt[threadIdx.x];
+        __syncthreads();
+        uint32_t open = 0;
+        for (int r = 0; r < w.n_ranks; ++r)
+            if (tg[r].mode == kExactCollect || tg[r].mode == kExactRefine) open |= 1u << r;
+        if (open) {
+            const double *row = realized + (uint64_t)(uint32_t)e * ld;
+            const double lo = earliest[e];
+            for (int32_t s0 = 0; s0 < m; s0 += kBlock) {  // every lane takes part in each step (ballots)
+                const int32_t s = s0 + (int32_t)threadIdx.x;
+                const bool in = s < m;
+                const double v = in ? row[s] : 0.0;
+                const bool ok = in && v == v;
+                const int32_t vb = value_bin(v, lo, inv_width, n_bins);
+                const uint64_t key = order_key(v);
+                for (uint32_t o = open; o; o &= o - 1) {
+                    const int r = __ffs((int)o) - 1;
+                    const ExactTarget &t = tg[r];
+                    const bool mem = ok && vb == t.bin && key >= t.klo && key <= t.khi;
+                    if (t.mode == kExactCollect) {
+                        const uint64_t mm = __ballot(mem);
+                        if (mm == 0) continue;
+                        const int leader = __ffsll((long long)mm) - 1;
+                        uint32_t base = 0;
+                        if (lane == leader) base = atomicAdd(&gt[r].fill, (uint32_t)__popcll(mm));
+                        base = __shfl(base, leader);
+                        const uint32_t pos = base + (uint32_t)__popcll(mm & below);
+                        if (mem) {
+                            if (pos < t.count)
+                                w.cand[(uint64_t)t.off + pos] = v;
+                            else
+                                w.counters[3] = 1;  // more values than the histogram counted
+                        }
+                    } else if (mem) {
+                        const uint64_t i = (uint64_t)(uint32_t)t.slot * kExactSubBins + (uint32_t)exact_sub_bin(t, v, key);
+                        atomicAdd(&w.slot_cnt[i], 1u);
+                        atomicMin((unsigned long long *)&w.slot_min[i], (unsigned long long)key);
+                        atomicMax((unsigned long long *)&w.slot_max[i], (unsigned long long)key);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// After a pass: a gathered region is sorted (bitonic, in LDS) and its rank
+// read off; a refined region becomes the sub-bin holding its rank, bounded by
+// that sub-bin's smallest and largest key.
+__global__ void __launch_bounds__(kBlock) exact_resolve_kernel(int32_t E, ExactWork w) {
+    __shared__ uint64_t keys[kExactMaxCollect];
+    for (int32_t e = blockIdx.x; e < E; e += gridDim.x) {
+        ExactTarget *gt = w.targets + (uint64_t)(uint32_t)e * (uint32_t)w.n_ranks;
+        for (int r = 0; r < w.n_ranks; ++r) {
+            const ExactTarget t = gt[r];
+            if (t.mode == kExactCollect) {
+                const uint32_t c = t.count;
+                uint32_t p2 = 1;
+                while (p2 < c) p2 <<= 1;
+                for (uint32_t i = threadIdx.x; i < p2; i += kBlock)
+                    keys[i] = i < c ? order_key(w.cand[(uint64_t)t.off + i]) : ~0ull;
+                __syncthreads();
+                for (uint32_t k = 2; k <= p2; k <<= 1)
+                    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                        for (uint32_t i = threadIdx.x; i < p2; i += kBlock) {
+                            const uint32_t x = i ^ j;
+                            if (x > i) {
+                                const uint64_t a = keys[i], b = keys[x];
+                                if ((a > b) == ((i & k) == 0)) {
+                                    keys[i] = b;
+                                    keys[x] = a;
+                                }
+                            }
+                        }
+                        __syncthreads();
+                    }
+                if (threadIdx.x == 0) {
+                    ExactTarget u = t;
+                    if (t.fill != c) {
+                        w.counters[3] = 1;
+                        u.value = __builtin_nan("");
+                    } else {
+                        u.value = key_value(keys[t.rank]);
+                    }
+                    u.mode = kExactResolved;
+                    gt[r] = u;
+                }
+                __syncthreads();
+            } else if (t.mode == kExactRefine && threadIdx.x == 0) {
+                const uint64_t base = (uint64_t)(uint32_t)t.slot * kExactSubBins;
+                int64_t cum = 0;
+                int j = 0;
+                while (j < kExactSubBins - 1 && cum + (int64_t)w.slot_cnt[base + j] <= t.rank) cum += w.slot_cnt[base + j++];
+                ExactTarget u = t;
+                u.rank = t.rank - cum;
+                u.count = w.slot_cnt[base + j];
+                u.klo = w.slot_min[base + j];
+                u.khi = w.slot_max[base + j];
+                u.slot = -1;
+                if (u.rank < 0 || u.rank >= (int64_t)u.count || u.klo > u.khi) {
+                    w.counters[3] = 1;  // the sub-bins do not hold the rank
+                    u.mode = kExactResolved;
+                    u.value = __builtin_nan("");
+                } else if (u.klo == u.khi) {
+                    u.mode = kExactResolved;
+                    u.value = key_value(u.klo);
+                } else {
+                    u.mode = kExactWait;
+                    const uint64_t span = u.khi - u.klo;
+                    const double va = key_value(u.klo), vb = key_value(u.khi);
+                    if (span > (1ull << 20) && vb - va > 0.0) {
+                        u.a = va;
+                        u.inv = (double)kExactSubBins / (vb - va);
+                        u.shift = 0;
+                    } else {
+                        u.inv = 0.0;
+                        u.shift = 0;
+                        while ((span >> u.shift) >= (uint64_t)kExactSubBins) ++u.shift;
+                    }
+                }
+                gt[r] = u;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void exact_finish_kernel(int32_t E, QuantileArgs qa, const ChunkStat *__restrict__ st, ExactWork w,
+                                    double *__restrict__ out) {
+    const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const ExactTarget *t = w.targets + (uint64_t)(uint32_t)e * (uint32_t)w.n_ranks;
+    const bool nan_row = st[e].nans != 0;
+    for (int j = 0; j < qa.n_q; ++j)
+        out[(uint64_t)(uint32_t)e * (uint32_t)qa.n_q + j] =
+            nan_row ? __builtin_nan("") : lerp_numpy(t[qa.lo[j]].value, t[qa.hi[j]].value, qa.gamma[j]);
 }
 
 }  // namespace
@@ -829,6 +1110,41 @@ hipError_t launch_chunk_finish(int64_t n, int32_t E, const QuantileArgs &qa, con
     if (E > 0)
         hipLaunchKernelGGL(chunk_finish_kernel, dim3((unsigned)E), dim3(kBlock), 0, stream, n, E, qa, earliest,
                            max_delay / (double)n_bins, n_bins, st, hist, mean, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_exact_init(int64_t n, int32_t E, const QuantileArgs &qa, const ChunkStat *st, const uint32_t *hist,
+                             int32_t n_bins, const double *earliest, double max_delay, const ExactWork &w,
+                             hipStream_t stream) {
+    hipLaunchKernelGGL(exact_reset_kernel, dim3(1), dim3(1), 0, stream, w.counters, 1);
+    if (E > 0) {
+        hipLaunchKernelGGL(exact_init_kernel, dim3((unsigned)E), dim3(kBlock), 0, stream, n, E, qa, st, hist, n_bins,
+                           earliest, max_delay / (double)n_bins, w);
+        hipLaunchKernelGGL(exact_assign_kernel, dim3((unsigned)E), dim3(kBlock), 0, stream, E, w);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_exact_pass(const double *realized, int32_t m, int32_t ld, int32_t E, const double *earliest,
+                             double max_delay, int32_t n_bins, const ExactWork &w, hipStream_t stream) {
+    if (E > 0 && m > 0)
+        hipLaunchKernelGGL(exact_pass_kernel, dim3((unsigned)E), dim3(kBlock), 0, stream, realized, m, (uint32_t)ld, E,
+                           earliest, (double)n_bins / max_delay, n_bins, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_exact_round_end(int32_t E, const ExactWork &w, hipStream_t stream) {
+    if (E > 0) hipLaunchKernelGGL(exact_resolve_kernel, dim3((unsigned)E), dim3(kBlock), 0, stream, E, w);
+    hipLaunchKernelGGL(exact_reset_kernel, dim3(1), dim3(1), 0, stream, w.counters, 0);
+    if (E > 0) hipLaunchKernelGGL(exact_assign_kernel, dim3((unsigned)E), dim3(kBlock), 0, stream, E, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_exact_finish(int32_t E, const QuantileArgs &qa, const ChunkStat *st, const ExactWork &w,
+                               double *out, hipStream_t stream) {
+    if (E > 0)
+        hipLaunchKernelGGL(exact_finish_kernel, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, stream, E, qa, st, w,
+                           out);
     return hipGetLastError();
 }
 

@@ -50,10 +50,71 @@ struct ChunkStat {
 hipError_t launch_chunk_accumulate(const double *realized, int32_t m, int32_t ld, int32_t E, const double *earliest,
                                    double max_delay, int32_t n_bins, int first, ChunkStat *st, uint32_t *hist,
                                    hipStream_t stream);
-// mean[e] (exactly rounded sum / n) and out[e * n_q + j]: numpy-linear quantiles
-// from order statistics placed inside their bin (within one bin width).
+// mean[e] (exactly rounded sum / n) and, when out is not null, out[e * n_q + j]:
+// numpy-linear quantiles from order statistics placed inside their bin (within
+// one bin width).
 hipError_t launch_chunk_finish(int64_t n, int32_t E, const QuantileArgs &qa, const double *earliest, double max_delay,
                                int32_t n_bins, const ChunkStat *st, const uint32_t *hist, double *mean, double *out,
                                hipStream_t stream);
+
+// Exact order statistics of a request run in chunks, over further passes of
+// the same chunks (their results are a function of the seeds).  Each (event,
+// rank) target is a region of the event's values: those in histogram bin
+// `bin` with klo <= order key <= khi, holding `count` values, the wanted one
+// at `rank` among them.  A pass either gathers a region's values (at most
+// c_max; then they are sorted and the rank read off) or counts them in 256
+// sub-bins with each sub-bin's smallest and largest key (then the region
+// shrinks to the sub-bin holding the rank).  Sub-bins split the region's value
+// range evenly while it spans many representable doubles, then its key range
+// (key offsets >> shift), so a region ends as one key or few enough values.
+enum ExactMode : int32_t { kExactResolved = 0, kExactCollect = 1, kExactRefine = 2, kExactWait = 3 };
+struct ExactTarget {
+    uint64_t klo, khi;
+    double a, inv;   // value sub-bins: (v - a) * inv, clamped to [0, 255]; inv == 0: (key - klo) >> shift
+    int64_t rank;
+    uint32_t count;
+    int32_t bin;
+    int32_t mode;    // ExactMode
+    int32_t shift;
+    int32_t slot;    // kExactRefine: the refine slot
+    uint32_t off;    // kExactCollect: the first candidate
+    uint32_t fill;   // kExactCollect: candidates gathered so far
+    uint32_t pad;
+    double value;    // kExactResolved: the order statistic
+};
+static_assert(sizeof(ExactTarget) == 80, "ExactTarget layout");
+constexpr int kExactSubBins = 256;
+constexpr int kExactMaxCollect = 4096;  // values gathered per region (sorted in LDS)
+// Device state of the exact selection: targets [E * n_ranks], candidates, the
+// refine slots' sub-bin counts, smallest and largest keys [n_slots * 256]
+// each, and counters (candidates handed out, slots handed out, unresolved
+// targets, an inconsistency flag).
+struct ExactWork {
+    ExactTarget *targets;
+    double *cand;
+    uint32_t *slot_cnt;
+    uint64_t *slot_min, *slot_max;
+    uint64_t *counters;  // [4]
+    int32_t n_ranks;
+    int32_t n_slots;
+    int32_t c_max;
+    uint32_t cand_cap;
+};
+// After the chunks' running statistics: targets from the histogram (ranks on
+// an extreme's mass, or of a row with NaN, are resolved at once), then modes
+// handed out.  counters[2] = unresolved targets, counters[3] != 0 on an
+// inconsistency.
+hipError_t launch_exact_init(int64_t n, int32_t E, const QuantileArgs &qa, const ChunkStat *st, const uint32_t *hist,
+                             int32_t n_bins, const double *earliest, double max_delay, const ExactWork &w,
+                             hipStream_t stream);
+// One chunk of a pass: rows realized[e][0..m) against every open target.
+hipError_t launch_exact_pass(const double *realized, int32_t m, int32_t ld, int32_t E, const double *earliest,
+                             double max_delay, int32_t n_bins, const ExactWork &w, hipStream_t stream);
+// After a pass over every chunk: gathered regions sorted and read, refined
+// regions shrunk; then modes handed out again.
+hipError_t launch_exact_round_end(int32_t E, const ExactWork &w, hipStream_t stream);
+// out[e * n_q + j] from the resolved order statistics (numpy's "linear").
+hipError_t launch_exact_finish(int32_t E, const QuantileArgs &qa, const ChunkStat *st, const ExactWork &w,
+                               double *out, hipStream_t stream);
 
 }  // namespace mcdp

@@ -8,8 +8,9 @@ footprint above the cap, which the strictly ordered admission could never place.
 
 The footprint below restates the device layout (256-byte aligned blocks: seeds, realized and
 cause rows, mean, quantiles, cause counts, and for chunked requests the running statistics of
-56 bytes per event and a 4,096-bin histogram per event); the test checks the library's answers
-against it: every accepted request fits the cap, and no larger chunk would.
+56 bytes per event and a 4,096-bin histogram per event, and with quantiles the exact selection's
+targets, candidate values and refine slots); the test checks the library's answers against it:
+every accepted request fits the cap, and no larger chunk would.
 """
 
 from __future__ import annotations
@@ -23,6 +24,9 @@ from mc_dagprop_amd import capi
 
 CHUNK_STAT_BYTES = 56
 CHUNK_BINS = 4096
+MAX_RANKS = 16           # two order statistics per quantile, at most 8 quantiles
+EXACT_TARGET_BYTES = 80  # one (event, rank) target of the exact selection
+# candidate values per target: n / 512 (16..256); refine slots of 256 sub-bins: one per 64 targets (16..1024)
 
 
 def _a(x: int) -> int:
@@ -33,6 +37,12 @@ def footprint(n: int, rows: int, E: int, P: int, nq: int, chunked: bool) -> int:
     b = _a(n * 4) + _a(E * rows * 8) + _a(E * rows * 4) + _a(E * 8) + _a(E * nq * 8) + _a((E + P) * 8)
     if chunked:
         b += _a(E * CHUNK_STAT_BYTES) + _a(E * CHUNK_BINS * 4)
+    if chunked and nq > 0:  # exact quantiles: targets, candidates, refine slots' counts and key bounds, counters
+        T = E * min(2 * nq, MAX_RANKS)
+        per_target = min(256, max(16, (n + 511) // 512))
+        slots = min(1024, max(16, (T + 63) // 64))
+        b += _a(T * EXACT_TARGET_BYTES) + _a(T * per_target * 8) + _a(slots * 256 * 4)
+        b += 2 * _a(slots * 256 * 8) + 256
     return b
 
 
