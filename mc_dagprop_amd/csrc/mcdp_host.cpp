@@ -657,6 +657,12 @@ int ensure_device_impl(mcdp_propagator &P, DeviceCopy *&out) {
     const size_t o_recs = reserve(P.recs.size() * sizeof(Record));
     const size_t o_evoff = reserve(P.ev_off.size() * sizeof(int32_t));
     const size_t o_slots = reserve(P.slot_list.size() * sizeof(int32_t));
+    // each (step, wave)'s slots as one row of k, -1 past its events
+    std::vector<int32_t> step_slots((size_t)P.n_steps * P.waves * P.k, -1);
+    for (size_t g = 0; g + 1 < P.ev_off.size(); ++g)
+        for (int32_t i = P.ev_off[g]; i < P.ev_off[g + 1]; ++i)
+            step_slots[g * (size_t)P.k + (size_t)(i - P.ev_off[g])] = P.slot_list[(size_t)i];
+    const size_t o_sslots = reserve(step_slots.size() * sizeof(int32_t));
     const size_t o_orph = reserve(P.orph.size() * sizeof(Record));
     const size_t o_dist = reserve(P.dist.size() * sizeof(DistParams));
     const size_t o_alias = reserve(P.alias.size() * sizeof(AliasRec));
@@ -671,6 +677,7 @@ int ensure_device_impl(mcdp_propagator &P, DeviceCopy *&out) {
     put(o_recs, P.recs.data(), P.recs.size() * sizeof(Record));
     put(o_evoff, P.ev_off.data(), P.ev_off.size() * sizeof(int32_t));
     put(o_slots, P.slot_list.data(), P.slot_list.size() * sizeof(int32_t));
+    put(o_sslots, step_slots.data(), step_slots.size() * sizeof(int32_t));
     put(o_orph, P.orph.data(), P.orph.size() * sizeof(Record));
     put(o_dist, P.dist.data(), P.dist.size() * sizeof(DistParams));
     put(o_alias, P.alias.data(), P.alias.size() * sizeof(AliasRec));
@@ -703,6 +710,7 @@ int ensure_device_impl(mcdp_propagator &P, DeviceCopy *&out) {
     d.base.recs = reinterpret_cast<const Record *>(b + o_recs);
     d.base.ev_off = reinterpret_cast<const int32_t *>(b + o_evoff);
     d.base.slot_list = reinterpret_cast<const int32_t *>(b + o_slots);
+    d.base.step_slots = reinterpret_cast<const int32_t *>(b + o_sslots);
     d.base.orph = reinterpret_cast<const Record *>(b + o_orph);
     d.base.dists = reinterpret_cast<const DistParams *>(b + o_dist);
     d.base.alias = reinterpret_cast<const AliasRec *>(b + o_alias);

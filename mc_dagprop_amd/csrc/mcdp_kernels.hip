@@ -95,6 +95,12 @@ __device__ __forceinline__ void stage_alias(const WalkParams &p, double *lds) {
 // One tile of 64 x kSpl samples of one plan: every step, then the orphan
 // activities.  Every wave of the workgroup calls it; it ends with all waves
 // past the last step's barrier.
+// The LDS slots of one (step, wave)'s events.
+template <int K>
+struct SlotRow {
+    int32_t v[K];
+};
+
 template <bool kInjected, int kPaths, int K, int kSpl>
 __device__ __forceinline__ void walk_tile(const WalkParams &p, int32_t tile, double *lds, const LogTable &lt) {
     const int lane = threadIdx.x & (kLanes - 1);
@@ -107,8 +113,7 @@ __device__ __forceinline__ void walk_tile(const WalkParams &p, int32_t tile, dou
     // address space lets wave-uniform record loads become scalar loads.
     const auto step_off = as_const_space(p.step_off);
     const auto recs = as_const_space(p.recs);
-    const auto ev_off = as_const_space(p.ev_off);
-    const auto slot_list = as_const_space(p.slot_list);
+    const auto step_slots = as_const_space(reinterpret_cast<const SlotRow<K> *>(p.step_slots));
     const auto dists = as_const_space(p.dists);
     const auto earliest = as_const_space(p.earliest);
 
@@ -221,13 +226,15 @@ __device__ __forceinline__ void walk_tile(const WalkParams &p, int32_t tile, dou
                 cur = nxt;
             }
         }
+        // The step's slots in one scalar load, in flight across the barrier.
+        const SlotRow<K> srow = step_slots[t * W + wave];
         // Every read of this step is done before any slot is overwritten,
         // and every commit is visible before the next step reads it.
         __syncthreads();
-        const int e0 = ev_off[t * W + wave];
-        for (int i = 0; i < j; ++i) {
-            const int slot = slot_list[e0 + i];
-            if (slot >= 0) lds_store<kSpl>(lds, slot, lane, res[i]);
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            const int slot = srow.v[i];
+            if (i < j && slot >= 0) lds_store<kSpl>(lds, slot, lane, res[i]);
         }
         __syncthreads();
     }

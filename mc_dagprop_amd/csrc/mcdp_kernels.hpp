@@ -39,6 +39,7 @@ struct WalkParams {
     const Record *recs;        // record streams, ending with one padding record
     const int32_t *ev_off;     // [n_steps * waves + 1]: event range of (step, wave) in slot_list
     const int32_t *slot_list;  // LDS slot of each event in stream order (-1: none)
+    const int32_t *step_slots; // [n_steps * waves][k]: the slots of (step, wave)'s events, -1 past them
     const Record *orph;        // activities no predecessor entry references
     const DistParams *dists;   // compiled distributions (p1..p4 of the rarer kinds)
     const AliasRec *alias;
