@@ -293,7 +293,8 @@ int ensure_reverse(mcdp_propagator &P, DeviceCopy &d) {
     }
     const size_t o_blk = reserve(blk.size() * 4), o_boff = reserve(blk_off.size() * 4);
     const size_t o_evoff = reserve(R.ev_off.size() * 4);
-    const size_t o_evs = reserve(R.ev_slot.size() * 4), o_moff = reserve(R.map_off.size() * 4);
+    // the kernel reads a (step, wave)'s slots as one row of kRevEventsPerWave: padding after the last
+    const size_t o_evs = reserve((R.ev_slot.size() + kRevEventsPerWave) * 4), o_moff = reserve(R.map_off.size() * 4);
     const size_t o_unit = reserve(R.unit_links.size() * sizeof(Record));
     const size_t o_coloff = reserve(R.col_off.size() * 4);
     const size_t o_mcol = reserve(R.map_col.size() * 4);

@@ -181,6 +181,12 @@ struct Regs8 {
     }
 };
 
+// The slots of a (step, wave)'s events, read as one row.
+template <int K>
+struct EventSlots {
+    int32_t v[K];
+};
+
 constexpr int kPre = 3;        // multiplier rows of an event's first predecessors, loaded one event ahead
 constexpr int kFastPreds = 4;  // predecessor records read per group
 
@@ -345,7 +351,7 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) reverse_kernel(RevParams p
         }
 
         // A step's event range and first column, one per lane (lanes 0-2), loaded a
-        // step ahead with one vector load; the step's slots likewise at its start.
+        // step ahead with one vector load.
         auto step_meta = [&](int tt) -> int32_t {
             const int i = (tt < p.n_steps ? tt : 0) * W + wave;
             return lane < 2 ? p.ev_off[i + lane] : p.col_off[i];
@@ -355,7 +361,6 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) reverse_kernel(RevParams p
             const int e0 = __builtin_amdgcn_readlane(meta, 0);
             const int ne = __builtin_amdgcn_readlane(meta, 1) - e0;
             int32_t ev_col = __builtin_amdgcn_readlane(meta, 2);  // the next column this wave writes
-            const int32_t slot_v = ne > 0 ? p.ev_slot[e0 + (lane < ne ? lane : 0)] : -1;
             meta = step_meta(ts + 1);
             static_assert(K == 8, "Regs8 holds one step's winner positions");
             double gres[K][kSpl];
@@ -525,13 +530,16 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) reverse_kernel(RevParams p
                 we_cur = we_nxt;
             }
             st.flush(prow, lane);
+            // The step's slots in one scalar load (the list ends in K padding
+            // entries), in flight across the barrier: no wait on the stores.
+            const EventSlots<K> srow = *as_const_space(reinterpret_cast<const EventSlots<K> *>(p.ev_slot + e0));
             // Every slot read of this step is done before any is overwritten,
             // and every commit is visible before the next step reads it.
             __syncthreads();
-#pragma unroll 1
-            for (int jj = 0; jj < ne; ++jj) {
-                const int32_t slot = __builtin_amdgcn_readlane(slot_v, jj);
-                if (slot >= 0) {
+#pragma unroll
+            for (int jj = 0; jj < K; ++jj) {
+                const int32_t slot = srow.v[jj];
+                if (jj < ne && slot >= 0) {
                     int32_t w[kSpl];
                     const int32_t x = wres.get(jj);
                     w[0] = kSpl == 1 ? x : (int32_t)(int16_t)(x & 0xFFFF);
