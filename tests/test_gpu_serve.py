@@ -224,6 +224,23 @@ def test_chunked_exact_selection_small_limits(gpu, limits) -> None:
     srv.close()
 
 
+def test_chunked_extremes_and_no_quantiles(gpu) -> None:
+    """A chunked request whose order statistics all lie on an extreme's mass (quantile 1: rank n - 1 only; quantile
+    0 would also read rank 1, with weight 0) resolves without a further pass; a chunked request with no quantiles
+    needs no selection at all.  Both equal run_stats."""
+    d = layered_network(layers=2)
+    p = capi.Propagator(d)
+    srv = capi.Server(48 << 20, max_running=2, n_streams=2, request_cap=8 << 20)
+    cid = srv.add_context(p)
+    seeds = np.arange(20_000, dtype=np.int32) * 13 + 1
+    for q in ((1.0,), ()):
+        got = srv.wait(srv.submit(cid, seeds, q))
+        same_stats(got, p.run_stats(seeds, q))
+    st = srv.stats()
+    assert st["chunked"] == 2 and st["exact_rounds"] == 0 and st["failed"] == 0
+    srv.close()
+
+
 def test_chunked_approximate_quantiles(gpu) -> None:
     """With exact selection off, a chunked request makes no further passes: mean, counts and the extremes
     are exact, every other quantile within one histogram bin (max_delay / 4096)."""
