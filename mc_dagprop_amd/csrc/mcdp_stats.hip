@@ -887,9 +887,9 @@ __global__ void __launch_bounds__(kBlock) exact_init_kernel(int64_t n, int32_t E
     }
 }
 
-// Hands out this round's work: a waiting region with at most c_max values
+// Hands out the next pass's work: a waiting region with at most c_max values
 // gets candidate space (while the buffer lasts), otherwise a refine slot
-// (while slots last), otherwise waits for the next round.  Counts the
+// (while slots last), otherwise waits for the pass after.  Counts the
 // unresolved targets.
 __global__ void __launch_bounds__(kBlock) exact_assign_kernel(int32_t E, ExactWork w) {
     __shared__ int32_t new_slot[kMaxRanks];

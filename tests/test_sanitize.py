@@ -3,7 +3,7 @@
 tests/native/Makefile builds tests/native/sanitize_host from the host sources (DAG compiler,
 schedule and reverse-plan builders, statistics and serving bookkeeping) and oracle/mcdp_oracle.c,
 all instrumented, linked with the device objects that mc_dagprop_amd/build.py leaves in
-mc_dagprop_amd/_build (uninstrumented; nothing runs on a device here).  The driver compiles random
+mc_dagprop_amd/_build (uninstrumented; nothing runs on a device here).  The program compiles random
 DAGs with repeated precedence targets, index gaps, edges outside the link table, cycles, events out
 of range and reserved types, builds every schedule shape and the reverse plan, and runs the
 oracle's propagation and gradient; any out-of-bounds access or undefined operation aborts it.
