@@ -46,6 +46,16 @@ constexpr uint32_t kPhiloxW0 = 0x9E3779B9u, kPhiloxW1 = 0xBB67AE85u;
 struct Words4 {
     uint32_t x, y, z, w;
 };
+
+// a ^ b ^ c in one instruction: gfx950's three-input bit operation, truth
+// table 0x96 (set where an odd number of the inputs is set).
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+    return a ^ b ^ c;
+#endif
+}
 struct Words2 {
     uint32_t x, y;
 };
@@ -61,8 +71,8 @@ __device__ __forceinline__ Words4 philox4x32_10(uint32_t c0, uint32_t c1, uint32
         }
         const uint64_t p0 = (uint64_t)kPhiloxM4a * c0;
         const uint64_t p1 = (uint64_t)kPhiloxM4b * c2;
-        const uint32_t y0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t y2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t y0 = xor3((uint32_t)(p1 >> 32), c1, k0);
+        const uint32_t y2 = xor3((uint32_t)(p0 >> 32), c3, k1);
         c1 = (uint32_t)p1;
         c3 = (uint32_t)p0;
         c0 = y0;
@@ -77,7 +87,7 @@ __device__ __forceinline__ Words2 philox2x32_10(uint32_t c0, uint32_t c1, uint32
     for (int r = 0; r < 10; ++r) {
         if (r) k += kPhiloxW0;
         const uint64_t p = (uint64_t)kPhiloxM2 * c0;
-        const uint32_t y0 = (uint32_t)(p >> 32) ^ k ^ c1;
+        const uint32_t y0 = xor3((uint32_t)(p >> 32), k, c1);
         c1 = (uint32_t)p;
         c0 = y0;
     }
