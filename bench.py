@@ -213,7 +213,7 @@ def bench_serve(args) -> tuple[int, dict]:
     lat = np.zeros(N)
     keep = {}
     # checked against the same request run alone: three that run in one piece, and the smallest of those that
-    # run in chunks (exact quantiles from further passes over the chunks)
+    # run in chunks (exact quantiles from rows kept whole and, where needed, further passes over the chunks)
     fits = [i for i in range(N) if 12 * props[ctx_of[i]].E * n_of[i] < cap // 2 and n_of[i] <= 40000]
     check_idx = set(int(i) for i in fits[:: max(1, len(fits) // 3)][:3])
     chunked_idx = sorted((i for i in range(N) if 12 * props[ctx_of[i]].E * n_of[i] > cap),
@@ -250,7 +250,8 @@ def bench_serve(args) -> tuple[int, dict]:
     p50, p99 = np.percentile(lat, [50, 99])
     log(f"serve: {N} requests at {rate:g}/s over {wall:.2f} s: p50 {p50:.1f} ms, p99 {p99:.1f} ms, "
         f"{work / wall:.3e} activity-samples/s; {st['chunked']} requests ran in chunks of samples "
-        f"({st['exact_rounds']} further passes for exact quantiles); at most {st['max_running']} running at once; "
+        f"({st['kept_rows']} rows kept whole, {st['exact_rounds']} further passes for exact quantiles); at most "
+        f"{st['max_running']} running at once; "
         f"checks bit-exact ({len(keep)} requests, {len(set(keep) & set(chunked_idx))} chunked): {ok}")
     out = {
         "metric": "activity-samples/sec GPU vs host run_many (bit-exact); p50/p99 solve latency",
@@ -270,7 +271,9 @@ def bench_serve(args) -> tuple[int, dict]:
                                "log-uniform; width 100, 1-4 preds, 8 activity types); 16,384-262,144 samples per "
                                "request (log-uniform); per-event mean, quantiles 0.5/0.9/0.99 and cause counts; "
                                "requests above a quarter of the memory pool run in chunks of samples (running "
-                               "mean and counts; " + ("exact quantiles from further passes over the chunks)" if exact
+                               "mean and counts; " + ("exact quantiles: on the extremes' masses, from rows kept whole "
+                                                      "for the events whose first chunk leaves them, else from "
+                                                      "further passes over the chunks)" if exact
                                                       else "quantiles within max_delay/4096: no further passes)"),
                    "contexts": n_ctx, "requests": N, "rate_per_s": rate, "pool_gib": args.budget_gb,
                    "chunked_requests": st["chunked"]},

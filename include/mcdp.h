@@ -124,6 +124,16 @@ int mcdp_propagate_host(mcdp_propagator *prop, const double *durations, int64_t 
  * (No counterpart in the reference.) */
 int mcdp_propagator_plan_info(const mcdp_propagator *prop, int64_t *out, int n_out);
 
+/* Bounds that hold for every seed: link_lo/link_hi [L] the range of the
+ * durations each link can draw, event_lo/event_hi [E] the range of each
+ * event's realized time (the propagation recurrence over those ranges).  Any
+ * pointer may be NULL.  MCDP_E_INVALID, with the event ranges left unwritten,
+ * when an earliest time or max_delay is not finite.  Serving uses them to
+ * tell which extremes of a sample chunk later chunks cannot move.  (No
+ * counterpart in the reference.) */
+int mcdp_propagator_ranges(const mcdp_propagator *prop, double *link_lo, double *link_hi, double *event_lo,
+                           double *event_hi);
+
 /* Fixes the schedule shape: `waves` waves per workgroup (1..16), at most
  * `events_per_wave` events per wave and step (4, 8, 12 or 16) and
  * `samples_per_lane` samples per lane (1, or 2 with events_per_wave <= 8).
@@ -332,12 +342,17 @@ typedef struct mcdp_server mcdp_server;
 /* request_cap (0: memory_budget / 2) bounds one request's share of the pool:
  * a request needing more runs in chunks of samples that fit it (needs a finite
  * max_delay).  Its mean and cause counts come from running sums over the
- * chunks; its quantiles from further passes over the same chunks (results are
- * a function of the seeds) that gather, or narrow down in 256 sub-bins, the
- * values around each order statistic a 4,096-bin histogram over [earliest,
- * earliest + max_delay] placed: every statistic equals mcdp_run_stats_host's
- * bit for bit.  Usually one further pass; each pass shrinks a region that
- * holds too many values 256-fold. */
+ * chunks.  Its quantiles: an order statistic on the mass of an extreme is
+ * that extreme; the events whose first chunk puts an order statistic
+ * elsewhere (as far as a chunk tells, with the event's bounds from
+ * mcdp_propagator_ranges saying which extremes later chunks cannot move)
+ * keep their whole rows across the chunks, and their quantiles come from
+ * those rows; any other order statistic takes further passes over the same
+ * chunks (results are a function of the seeds) that gather, or narrow down
+ * in 256 sub-bins, the values around it that a 4,096-bin histogram over
+ * [earliest, earliest + max_delay] placed.  Every statistic equals
+ * mcdp_run_stats_host's bit for bit.  Usually no further pass is needed;
+ * each pass shrinks a region that holds too many values 256-fold. */
 int mcdp_server_create(int64_t memory_budget, int64_t request_cap, int32_t max_running, int32_t n_streams,
                        mcdp_server **out);
 /* Waits for running requests; fails queued ones; frees the pool. */
@@ -355,9 +370,9 @@ int mcdp_server_submit(mcdp_server *srv, int32_t ctx_id, const int32_t *seeds, i
  * completion; the request is then forgotten. */
 int mcdp_server_wait(mcdp_server *srv, int64_t request_id, double *mean, double *quantiles, int64_t *cause_counts,
                      double *latency_ms);
-/* out[0..9] = {submitted, completed, failed, queued, running, most running at
+/* out[0..10] = {submitted, completed, failed, queued, running, most running at
  * once, pool bytes in use, peak pool bytes in use, requests run in chunks,
- * further passes over chunks for exact quantiles}. */
+ * further passes over chunks for exact quantiles, rows kept whole}. */
 int mcdp_server_stats(mcdp_server *srv, int64_t *out, int n_out);
 /* exact != 0 (the default): chunked requests get exact quantiles (further
  * passes over their chunks).  exact == 0: no further passes; each quantile is
@@ -429,6 +444,11 @@ int mcdp_debug_serve_chunk(int64_t n, int64_t n_events, int64_t n_preds, int32_t
  * slots per request (1..1024), so tests reach the refining passes and the
  * targets that wait for a slot. */
 int mcdp_debug_server_exact_limits(mcdp_server *srv, int32_t c_max, int64_t cand_per_target, int32_t refine_slots);
+/* Rows kept whole per chunked request, for later requests: rows < 0 restores
+ * the default (one per 128 events, at least 16, within a quarter of the
+ * request cap), 0 keeps none (every order statistic off the extremes' masses
+ * then takes further passes), rows > 0 at most that many. */
+int mcdp_debug_server_kept_rows(mcdp_server *srv, int32_t rows);
 
 #ifdef __cplusplus
 }

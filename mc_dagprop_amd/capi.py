@@ -47,6 +47,7 @@ _SIGNATURES = {
     "mcdp_propagate_device": (C.c_int, [_P, _P, _i64, _i64, _P, _P, _P]),
     "mcdp_propagate_host": (C.c_int, [_P, _pf64, _i64, _pf64, _pi32]),
     "mcdp_propagator_plan_info": (C.c_int, [_P, _pi64, C.c_int]),
+    "mcdp_propagator_ranges": (C.c_int, [_P, _pf64, _pf64, _pf64, _pf64]),
     "mcdp_propagator_configure": (C.c_int, [_P, _i32, _i32, _i32]),
     "mcdp_propagator_autotune": (
         C.c_int,
@@ -107,6 +108,7 @@ _SIGNATURES = {
     "mcdp_server_stats": (C.c_int, [_P, _pi64, C.c_int]),
     "mcdp_server_set_exact_quantiles": (C.c_int, [_P, _i32]),
     "mcdp_debug_server_exact_limits": (C.c_int, [_P, _i32, _i64, _i32]),
+    "mcdp_debug_server_kept_rows": (C.c_int, [_P, _i32]),
 }
 
 _lib = None
@@ -284,6 +286,19 @@ class Propagator:
         v = np.zeros(len(self.PLAN_FIELDS), dtype=np.int64)
         check(lib().mcdp_propagator_plan_info(self.h, _ptr(v, _pi64), v.size))
         return dict(zip(self.PLAN_FIELDS, map(int, v)))
+
+    def ranges(self, events: bool = True) -> tuple:
+        """Bounds that hold for every seed: (link_lo [L], link_hi [L], event_lo [E], event_hi [E]); the
+        event bounds are None when an earliest time or max_delay is not finite (or events=False)."""
+        llo, lhi = np.empty(self.L), np.empty(self.L)
+        check(lib().mcdp_propagator_ranges(self.h, _ptr(llo, _pf64), _ptr(lhi, _pf64), None, None))
+        if not events:
+            return llo, lhi, None, None
+        elo, ehi = np.empty(self.E), np.empty(self.E)
+        rc = lib().mcdp_propagator_ranges(self.h, None, None, _ptr(elo, _pf64), _ptr(ehi, _pf64))
+        if rc:
+            return llo, lhi, None, None
+        return llo, lhi, elo, ehi
 
     def configure(self, waves: int, events_per_wave: int, samples_per_lane: int = 1) -> None:
         """Fixes the schedule shape (results do not depend on it)."""
@@ -547,7 +562,7 @@ class Server:
     """Statistics requests over resident contexts, admitted by device-memory footprint (mcdp_server_*)."""
 
     STAT_FIELDS = ["submitted", "completed", "failed", "queued", "running", "max_running", "bytes_in_use",
-                   "peak_bytes", "chunked", "exact_rounds"]
+                   "peak_bytes", "chunked", "exact_rounds", "kept_rows"]
 
     def __init__(self, memory_budget: int, max_running: int = 8, n_streams: int = 4, request_cap: int = 0,
                  exact_chunked_quantiles: bool = True) -> None:
@@ -564,6 +579,10 @@ class Server:
     def debug_exact_limits(self, c_max: int, cand_per_target: int, refine_slots: int) -> None:
         """Lower limits of the exact selection for later requests (test hook)."""
         check(lib().mcdp_debug_server_exact_limits(self.h, c_max, cand_per_target, refine_slots))
+
+    def debug_kept_rows(self, rows: int) -> None:
+        """Rows kept whole per chunked request, for later requests (test hook): -1 the default, 0 none."""
+        check(lib().mcdp_debug_server_kept_rows(self.h, rows))
 
     def add_context(self, prop: Propagator) -> int:
         cid = _i32(0)

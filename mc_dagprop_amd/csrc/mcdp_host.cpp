@@ -1077,6 +1077,123 @@ int mcdp::configure_shape(mcdp_propagator *prop, int32_t waves, int32_t k, int32
 }
 mcdp::Record mcdp::link_record(const mcdp_propagator &P, int32_t l) { return edge_record(P, l); }
 
+// The range of every duration the sampler can draw for link l, as the device
+// rounds it (mcdp_sampler.hpp: base + extra, extra = X * d, one rounding
+// each): X in [x0, x1] for the multiplier kinds (exponential and gamma X >= 0,
+// at most max_scale), the reachable alias-table values for the empirical
+// kinds.  Rounding is monotone, so the two ends bound every draw.  A link
+// that can draw NaN gets [-inf, +inf]: a NaN candidate never wins the max, so
+// it acts as the smallest candidate.
+void mcdp::link_duration_range(const mcdp_propagator &P, int32_t l, double &lo, double &hi) {
+    constexpr double inf = std::numeric_limits<double>::infinity();
+    const double d = P.dmin[(size_t)l];
+    lo = -inf;
+    hi = inf;
+    if (std::isnan(d)) return;
+    const uint32_t di = P.ldist[(size_t)l];
+    if (di == kNoDistIndex) {
+        lo = hi = d;
+        return;
+    }
+    const DistRec &D = P.dist[di];
+    auto rel = [&](double x) {  // d + x d, rounded as the device does
+        const double extra = x * d;
+        return d + extra;
+    };
+    auto ends = [&](double a, double b) {
+        if (std::isnan(a) || std::isnan(b)) return;
+        lo = std::min(a, b);
+        hi = std::max(a, b);
+    };
+    // multiplier X in [x0, x1] (x1 may be +inf while every draw is finite)
+    auto mult = [&](double x0, double x1) {
+        if (!std::isfinite(d)) return;
+        if (d == 0.0) {  // a finite X times a zero is a zero
+            lo = hi = rel(0.0);
+            return;
+        }
+        ends(rel(x0), std::isinf(x1) ? (d > 0.0 ? inf : -inf) : rel(x1));
+    };
+    switch (D.kind) {
+        case kNone: lo = hi = d; return;
+        case kConstant:
+        case kEmpiricalRel1: ends(rel(D.p[0]), rel(D.p[0])); return;
+        case kEmpiricalAbs1: ends(d + D.p[0], d + D.p[0]); return;
+        case kExpInf: mult(0.0, inf); return;
+        case kExpTrunc: mult(0.0, D.p[2]); return;
+        case kGammaErlang2:
+        case kGammaHalf:
+        case kGammaMT:
+        case kGammaPow: mult(0.0, D.p[3]); return;
+        case kEmpiricalAbs:
+        case kEmpiricalRel: {
+            uint64_t packed;
+            std::memcpy(&packed, &D.p[0], sizeof(double));
+            const size_t n = (uint32_t)packed, off = (size_t)(uint32_t)(packed >> 32);
+            double vmin = inf, vmax = -inf;
+            bool nan = false;
+            for (size_t b = 0; b < n; ++b) {  // a bin's own value needs prob > 0; its alias is always reachable
+                const AliasRec &a = P.alias[off + b];
+                for (int side = 0; side < 2; ++side) {
+                    if (side == 0 && !(a.prob > 0.0)) continue;
+                    const double v = side == 0 ? a.v_self : a.v_alias;
+                    if (std::isnan(v)) nan = true;
+                    vmin = std::min(vmin, v);
+                    vmax = std::max(vmax, v);
+                }
+            }
+            if (nan) return;
+            if (D.kind == kEmpiricalAbs) {
+                if (std::isfinite(d)) ends(d + vmin, d + vmax);
+            } else if (std::isfinite(d)) {
+                if (d == 0.0) {
+                    if (std::isfinite(vmin) && std::isfinite(vmax)) lo = hi = rel(0.0);
+                } else {
+                    ends(rel(vmin), rel(vmax));
+                }
+            }
+            return;
+        }
+        default: return;
+    }
+}
+
+// Every event's realized time lies in [ev_lo[e], ev_hi[e]] for every seed:
+// the walk's recurrence (upper = earliest + max_delay; per predecessor in
+// input order t = min(upper, r_src + duration), best = t when t >= best;
+// realized = min(upper, best); a stale predecessor reads its earliest time)
+// over the per-link duration ranges, which it preserves because each step is
+// monotone.  False when an earliest time or max_delay is not finite.
+bool mcdp::event_ranges(const mcdp_propagator &P, std::vector<double> &ev_lo, std::vector<double> &ev_hi) {
+    if (!std::isfinite(P.max_delay)) return false;
+    for (double e : P.earliest)
+        if (!std::isfinite(e)) return false;
+    std::vector<double> dlo((size_t)P.L), dhi((size_t)P.L);
+    for (int32_t l = 0; l < P.L; ++l) link_duration_range(P, l, dlo[(size_t)l], dhi[(size_t)l]);
+    ev_lo.assign((size_t)P.E, 0.0);
+    ev_hi.assign((size_t)P.E, 0.0);
+    for (const auto &level : P.levels)
+        for (int32_t e : level) {
+            const double upper = P.earliest[(size_t)e] + P.max_delay;
+            double blo = P.earliest[(size_t)e], bhi = blo;
+            for (int64_t k = P.pred_off[(size_t)e]; k < P.pred_off[(size_t)e + 1]; ++k) {
+                const PredEntry &pe = P.preds[(size_t)k];
+                const bool valid = pe.edge >= 0 && pe.edge < P.L;
+                const double rlo = pe.stale ? P.earliest[(size_t)pe.src] : ev_lo[(size_t)pe.src];
+                const double rhi = pe.stale ? P.earliest[(size_t)pe.src] : ev_hi[(size_t)pe.src];
+                double tlo = rlo + (valid ? dlo[(size_t)pe.edge] : 0.0);
+                double thi = rhi + (valid ? dhi[(size_t)pe.edge] : 0.0);
+                tlo = (upper < tlo) ? upper : tlo;
+                thi = (upper < thi) ? upper : thi;
+                if (tlo >= blo) blo = tlo;
+                if (thi >= bhi) bhi = thi;
+            }
+            ev_lo[(size_t)e] = (upper < blo) ? upper : blo;
+            ev_hi[(size_t)e] = (upper < bhi) ? upper : bhi;
+        }
+    return true;
+}
+
 // ============================================================================
 // C ABI
 // ============================================================================
@@ -1208,6 +1325,24 @@ int mcdp_propagator_plan_info(const mcdp_propagator *prop, int64_t *out, int n_o
                           prop->tile_cost,
                           prop->wide_cost};
     for (int i = 0; i < n_out && i < 12; ++i) out[i] = v[i];
+    return MCDP_OK;
+}
+
+int mcdp_propagator_ranges(const mcdp_propagator *prop, double *link_lo, double *link_hi, double *event_lo,
+                           double *event_hi) {
+    if (!prop) return fail(MCDP_E_INVALID, "propagator is null");
+    for (int32_t l = 0; l < prop->L; ++l) {
+        double lo, hi;
+        link_duration_range(*prop, l, lo, hi);
+        if (link_lo) link_lo[l] = lo;
+        if (link_hi) link_hi[l] = hi;
+    }
+    if (!event_lo && !event_hi) return MCDP_OK;
+    std::vector<double> elo, ehi;
+    if (!event_ranges(*prop, elo, ehi))
+        return fail(MCDP_E_INVALID, "event ranges need finite earliest times and a finite max_delay");
+    if (event_lo) std::copy(elo.begin(), elo.end(), event_lo);
+    if (event_hi) std::copy(ehi.begin(), ehi.end(), event_hi);
     return MCDP_OK;
 }
 

@@ -210,13 +210,29 @@ int chunk_stats_finish(mcdp_propagator &P, int64_t n, int32_t n_q, const double 
 // is resolved; asynchronous on s.  The number of distinct ranks n and q need.
 int exact_rank_count(int64_t n, int32_t n_q, const double *q, int32_t *n_ranks);
 int exact_stats_init(mcdp_propagator &P, int64_t n, int32_t n_q, const double *q, const ChunkStat *st,
-                     const uint32_t *hist, ExactWork &w, hipStream_t s);
+                     const uint32_t *hist, ExactWork &w, const WatchWork *watch, hipStream_t s);
 int exact_stats_pass(mcdp_propagator &P, const double *realized, int32_t m, int32_t ld, const ExactWork &w,
                      hipStream_t s);
 int exact_stats_round_end(mcdp_propagator &P, const ExactWork &w, hipStream_t s);
 int exact_stats_finish(mcdp_propagator &P, int64_t n, int32_t n_q, const double *q, const ChunkStat *st,
                        const ExactWork &w, double *quantiles, hipStream_t s);
+// Rows kept whole across a request's chunks (mcdp_stats.hpp, WatchWork): c[0..n)
+// = 0; the rows chosen after the first chunk (ranges: [E][2] event bounds);
+// one chunk's samples c0 .. c0 + m appended; and the quantiles of the first
+// `rows` kept rows (ld = n) written over theirs in quantiles[E][n_q].
+// Asynchronous on s.
+int counters_clear(uint64_t *c, int n, hipStream_t s);
+int watch_select(mcdp_propagator &P, int64_t n, int32_t m, int32_t n_q, const double *q, const ChunkStat *st,
+                 const double *ranges, const WatchWork &w, hipStream_t s);
+int watch_copy(const double *realized, int32_t m, int32_t ld, int64_t c0, const WatchWork &w, hipStream_t s);
+int watch_finish(int64_t n, int32_t n_q, const double *q, const WatchWork &w, int32_t rows, double *quantiles,
+                 hipStream_t s);
 // The forward plan's record of link l (kind, flags, distribution index,
 // minimal duration, p0); an edge outside the link table is kind kNone.
 Record link_record(const mcdp_propagator &P, int32_t l);
+// Bounds that hold for every seed: the durations link l can draw, and every
+// event's realized time (false when an earliest time or max_delay is not
+// finite).
+void link_duration_range(const mcdp_propagator &P, int32_t l, double &lo, double &hi);
+bool event_ranges(const mcdp_propagator &P, std::vector<double> &ev_lo, std::vector<double> &ev_hi);
 }  // namespace mcdp

@@ -19,6 +19,10 @@
 // gather or narrow down the values around each wanted order statistic, so its
 // quantiles are exact too.  Between passes the scheduler reads a few counters
 // back and launches the next pass; other requests keep running meanwhile.
+// Most events' order statistics lie on the masses of their extremes (a cap
+// that binds, an earliest time that holds); the events whose first chunk
+// says otherwise keep their whole rows across the chunks, so their quantiles
+// need no further pass either, and a request usually needs none.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -62,7 +66,8 @@ struct ExactSizes {
     int32_t slots = 0;
     size_t bytes = 0;
 };
-ExactSizes exact_sizes(int64_t n, int64_t E, int64_t n_q) {
+// `kept`: rows kept whole (WatchWork), each with its n samples and quantiles.
+ExactSizes exact_sizes(int64_t n, int64_t E, int64_t n_q, int64_t kept) {
     ExactSizes z;
     if (n_q <= 0) return z;
     z.targets = E * std::min<int64_t>(2 * n_q, kMaxRanks);
@@ -72,7 +77,21 @@ ExactSizes exact_sizes(int64_t n, int64_t E, int64_t n_q) {
               align_up((size_t)(z.targets * z.cand_per_target) * 8, 256) +
               align_up((size_t)z.slots * kExactSubBins * 4, 256) + 2 * align_up((size_t)z.slots * kExactSubBins * 8, 256) +
               256;
+    if (kept > 0)
+        z.bytes += align_up((size_t)kept * 4, 256) + align_up((size_t)kept * (size_t)n * 8, 256) +
+                   align_up((size_t)(kept * n_q) * 8, 256);
     return z;
+}
+
+// Rows kept whole for a chunked request: one per 128 events (at least 16, at
+// most E; `limit` >= 0 replaces that), within a quarter of the request cap.
+// Events whose first chunk puts a rank outside the extremes' masses are
+// rare where caps bind (a few per thousand events), and those past the
+// limit take further passes instead.
+int64_t kept_rows(int64_t n, int64_t E, int64_t n_q, size_t cap, int32_t limit) {
+    if (n_q <= 0 || E <= 0 || n <= 0) return 0;
+    int64_t k = limit >= 0 ? std::min<int64_t>(limit, E) : std::min<int64_t>(E, std::max<int64_t>(16, (E + 127) / 128));
+    return std::min<int64_t>(k, (int64_t)(cap / 4 / ((size_t)n * 8)));
 }
 
 // Best-fit allocator over [0, size) with coalescing free blocks.
@@ -143,10 +162,12 @@ struct Request {
     ChunkStat *d_st = nullptr;
     uint32_t *d_hist = nullptr;
     ExactWork exact{};
+    int64_t kept_cap = 0;  // rows kept whole (0: none)
+    WatchWork watch{};
     int phase = 0;
     int rounds = 0;
     int64_t buf_c0 = 0;  // first sample of the chunk whose rows d_real holds
-    uint64_t *h_status = nullptr;  // pinned: the exact selection's counters
+    uint64_t *h_status = nullptr;  // pinned: the exact selection's counters and the kept-row count
     Clock::time_point t_submit, t_start, t_end;
     State state = State::Queued;
     int rc = MCDP_OK;
@@ -154,13 +175,14 @@ struct Request {
 };
 
 // Device bytes of a request: seeds, realized and cause rows for `rows`
-// samples at a time, statistics (and, run in chunks, their running state).
-size_t footprint(int64_t n, int64_t rows, int64_t E, int64_t P, int64_t n_q, bool chunked) {
+// samples at a time, statistics (and, run in chunks, their running state, the
+// exact selection's workspace and `kept` rows kept whole).
+size_t footprint(int64_t n, int64_t rows, int64_t E, int64_t P, int64_t n_q, bool chunked, int64_t kept) {
     size_t b = align_up((size_t)n * 4, 256) + align_up((size_t)(E * rows) * 8, 256) +
                align_up((size_t)(E * rows) * 4, 256) + align_up((size_t)E * 8, 256) +
                align_up((size_t)(E * n_q) * 8, 256) + align_up((size_t)(E + P) * 8, 256);
     if (chunked) b += align_up((size_t)E * sizeof(ChunkStat), 256) + align_up((size_t)E * kMaxChunkBins * 4, 256);
-    if (chunked) b += exact_sizes(n, E, n_q).bytes;  // targets, candidates, refine slots, counters
+    if (chunked) b += exact_sizes(n, E, n_q, kept).bytes;  // targets, candidates, refine slots, counters
     return b;
 }
 
@@ -169,15 +191,27 @@ size_t footprint(int64_t n, int64_t rows, int64_t E, int64_t P, int64_t n_q, boo
 // statistics) fits the cap, at most n; 0 if not even 128 samples fit.  The
 // 1,024 bytes of slack cover the 256-byte alignment of the two per-sample
 // row blocks.
-int64_t chunk_samples(int64_t n, int64_t E, int64_t P, int64_t n_q, size_t cap) {
-    const size_t fixed = footprint(n, 0, E, P, n_q, true);
+int64_t chunk_samples(int64_t n, int64_t E, int64_t P, int64_t n_q, size_t cap, int64_t kept) {
+    const size_t fixed = footprint(n, 0, E, P, n_q, true, kept);
     const size_t per_sample = (size_t)std::max<int64_t>(E, 1) * 12;
     if (fixed >= cap || cap - fixed <= 1024) return 0;
     int64_t m = (int64_t)((cap - fixed - 1024) / per_sample);
     m = m / 128 * 128;
     if (m < 128) return 0;
     m = std::min<int64_t>(m, n);
-    return footprint(n, m, E, P, n_q, true) <= cap ? m : 0;
+    return footprint(n, m, E, P, n_q, true, kept) <= cap ? m : 0;
+}
+
+// A chunked request's samples per chunk and rows kept whole: the kept rows
+// give way (to none) when the chunks would not fit beside them.
+void chunk_plan(int64_t n, int64_t E, int64_t P, int64_t n_q, size_t cap, bool exact, int32_t limit, int64_t &m,
+                int64_t &kept) {
+    kept = exact ? kept_rows(n, E, n_q, cap, limit) : 0;
+    m = chunk_samples(n, E, P, n_q, cap, kept);
+    if (m < 1 && kept > 0) {
+        kept = 0;
+        m = chunk_samples(n, E, P, n_q, cap, 0);
+    }
 }
 
 }  // namespace
@@ -191,6 +225,7 @@ struct mcdp_server {
     std::vector<hipStream_t> streams;
     std::vector<int> stream_load;
     std::vector<mcdp_propagator *> contexts;
+    std::vector<double *> ranges;  // per context: [E][2] bounds of realized times (null: not finite)
 
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -198,11 +233,13 @@ struct mcdp_server {
     std::vector<Request *> running;
     std::unordered_map<int64_t, std::unique_ptr<Request>> all;
     int64_t next_id = 1;
-    int64_t n_submitted = 0, n_completed = 0, n_failed = 0, max_running_seen = 0, n_chunked = 0, n_rounds = 0;
+    int64_t n_submitted = 0, n_completed = 0, n_failed = 0, max_running_seen = 0, n_chunked = 0, n_rounds = 0,
+            n_kept = 0;
     bool stop = false;
     bool exact_chunked = true;  // exact quantiles for chunked requests (further passes)
     int32_t exact_c_max = kExactMaxCollect, exact_slots = kExactRefineSlots;
     int64_t exact_cand_per_target = kExactCandPerTarget;
+    int32_t kept_limit = -1;  // rows kept whole per chunked request: -1 by kept_rows, else at most this many
     std::thread worker;
 
     void run();
@@ -239,22 +276,12 @@ int launch_request(mcdp_server &srv, Request &r) {
         // run_many chunk by chunk into the same rows; statistics merged across chunks
         auto *d_st = r.d_st = reinterpret_cast<ChunkStat *>(take((size_t)E * sizeof(ChunkStat)));
         auto *d_hist = r.d_hist = reinterpret_cast<uint32_t *>(take((size_t)E * kMaxChunkBins * 4));
-        for (int64_t c0 = 0; c0 < n; c0 += r.chunk) {
-            const int64_t m = std::min<int64_t>(r.chunk, n - c0);
-            rc = mcdp_run_many_device(prop, d_seeds + c0, m, r.chunk, d_real, nullptr, d_cause, s);
-            if (rc) return rc;
-            rc = chunk_stats_accumulate(*prop, d_real, d_cause, (int32_t)m, (int32_t)r.chunk, c0 == 0, d_st, d_hist,
-                                        d_cnt, s);
-            if (rc) return rc;
-            r.buf_c0 = c0;
-        }
         const bool exact = srv.exact_chunked && nq > 0;
-        rc = chunk_stats_finish(*prop, n, (int32_t)nq, r.q.data(), d_st, d_hist, d_mean, exact ? nullptr : d_quant, s);
-        if (rc) return rc;
+        ExactWork &w = r.exact;
+        WatchWork &v = r.watch;
         if (exact) {
-            // targets from the histograms; the scheduler continues once the counters are back
-            const ExactSizes z = exact_sizes(n, E, nq);
-            ExactWork &w = r.exact;
+            // the exact selection's workspace, and the rows kept whole
+            const ExactSizes z = exact_sizes(n, E, nq, r.kept_cap);
             w.targets = reinterpret_cast<ExactTarget *>(take((size_t)z.targets * sizeof(ExactTarget)));
             w.cand = reinterpret_cast<double *>(take((size_t)(z.targets * z.cand_per_target) * 8));
             w.slot_cnt = reinterpret_cast<uint32_t *>(take((size_t)z.slots * kExactSubBins * 4));
@@ -266,10 +293,45 @@ int launch_request(mcdp_server &srv, Request &r) {
             w.c_max = std::min(srv.exact_c_max, kExactMaxCollect);
             w.cand_cap = (uint32_t)std::min<int64_t>(z.targets * std::min(srv.exact_cand_per_target, z.cand_per_target),
                                                      std::numeric_limits<int32_t>::max());
+            v.count = w.counters + 4;
+            v.ld = n;
+            v.cap = 0;
+            if (r.kept_cap > 0) {
+                v.list = reinterpret_cast<int32_t *>(take((size_t)r.kept_cap * 4));
+                v.store = reinterpret_cast<double *>(take((size_t)r.kept_cap * (size_t)n * 8));
+                v.wq = reinterpret_cast<double *>(take((size_t)(r.kept_cap * nq) * 8));
+                if (srv.ranges[(size_t)r.ctx]) v.cap = (int32_t)r.kept_cap;
+            }
             if (o > r.bytes) return fail(MCDP_E_INVALID, "internal error: request layout exceeds its footprint");
-            rc = exact_stats_init(*prop, n, (int32_t)nq, r.q.data(), d_st, d_hist, w, s);
+            rc = counters_clear(w.counters, 5, s);
             if (rc) return rc;
-            HIP_TRY(hipMemcpyAsync(r.h_status, w.counters, 4 * 8, hipMemcpyDeviceToHost, s), "copy counters");
+        }
+        for (int64_t c0 = 0; c0 < n; c0 += r.chunk) {
+            const int64_t m = std::min<int64_t>(r.chunk, n - c0);
+            rc = mcdp_run_many_device(prop, d_seeds + c0, m, r.chunk, d_real, nullptr, d_cause, s);
+            if (rc) return rc;
+            rc = chunk_stats_accumulate(*prop, d_real, d_cause, (int32_t)m, (int32_t)r.chunk, c0 == 0, d_st, d_hist,
+                                        d_cnt, s);
+            if (rc) return rc;
+            if (v.cap > 0) {
+                if (c0 == 0) {
+                    rc = watch_select(*prop, n, (int32_t)m, (int32_t)nq, r.q.data(), d_st, srv.ranges[(size_t)r.ctx],
+                                      v, s);
+                    if (rc) return rc;
+                }
+                rc = watch_copy(d_real, (int32_t)m, (int32_t)r.chunk, c0, v, s);
+                if (rc) return rc;
+            }
+            r.buf_c0 = c0;
+        }
+        rc = chunk_stats_finish(*prop, n, (int32_t)nq, r.q.data(), d_st, d_hist, d_mean, exact ? nullptr : d_quant, s);
+        if (rc) return rc;
+        if (exact) {
+            // targets from the histograms (those of kept rows resolved); the
+            // scheduler continues once the counters are back
+            rc = exact_stats_init(*prop, n, (int32_t)nq, r.q.data(), d_st, d_hist, w, v.cap > 0 ? &v : nullptr, s);
+            if (rc) return rc;
+            HIP_TRY(hipMemcpyAsync(r.h_status, w.counters, 5 * 8, hipMemcpyDeviceToHost, s), "copy counters");
             HIP_TRY(hipEventCreateWithFlags(&r.done, hipEventDisableTiming), "hipEventCreate");
             HIP_TRY(hipEventRecord(r.done, s), "hipEventRecord");
             r.phase = 1;
@@ -311,6 +373,11 @@ int continue_exact(mcdp_server &srv, Request &r) {
     if (open == 0) {
         int rc = exact_stats_finish(*prop, n, (int32_t)nq, r.q.data(), r.d_st, r.exact, r.d_quant, s);
         if (rc) return rc;
+        // the kept rows' quantiles, from their whole rows, over those
+        const int32_t rows = (int32_t)std::min<uint64_t>(r.h_status[4], (uint64_t)r.watch.cap);
+        rc = watch_finish(n, (int32_t)nq, r.q.data(), r.watch, rows, r.d_quant, s);
+        if (rc) return rc;
+        srv.n_kept += rows;
         return copy_results(srv, r);
     }
     if (r.rounds >= kMaxExactRounds) return fail(MCDP_E_DEVICE, "exact quantile selection did not converge");
@@ -332,7 +399,7 @@ int continue_exact(mcdp_server &srv, Request &r) {
     }
     rc = exact_stats_round_end(*prop, r.exact, s);
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(r.h_status, r.exact.counters, 4 * 8, hipMemcpyDeviceToHost, s), "copy counters");
+    HIP_TRY(hipMemcpyAsync(r.h_status, r.exact.counters, 5 * 8, hipMemcpyDeviceToHost, s), "copy counters");
     HIP_TRY(hipEventCreateWithFlags(&r.done, hipEventDisableTiming), "hipEventCreate");
     HIP_TRY(hipEventRecord(r.done, s), "hipEventRecord");
     r.phase = 1;
@@ -494,6 +561,8 @@ void mcdp_server_destroy(mcdp_server *srv) {
     for (auto &kv : srv->all)
         if (kv.second->h_out) (void)hipHostFree(kv.second->h_out);
     for (hipStream_t s : srv->streams) (void)hipStreamDestroy(s);
+    for (double *p : srv->ranges)
+        if (p) (void)hipFree(p);
     (void)hipFree(srv->pool_base);
     delete srv;
 }
@@ -502,11 +571,31 @@ int mcdp_server_add_context(mcdp_server *srv, mcdp_propagator *prop, int32_t *ct
     if (!srv || !prop || !ctx_id) return fail(MCDP_E_INVALID, "null argument");
     int rc = prepare_stats(*prop);  // plan and predecessor table resident from now on
     if (rc) return rc;
+    // bounds of every event's realized time, for the rows kept whole
+    double *d_ranges = nullptr;
+    std::vector<double> lo, hi;
+    if (prop->E > 0 && event_ranges(*prop, lo, hi)) {
+        std::vector<double> pairs(2 * (size_t)prop->E);
+        for (size_t e = 0; e < lo.size(); ++e) {
+            pairs[2 * e] = lo[e];
+            pairs[2 * e + 1] = hi[e];
+        }
+        hipError_t e = hipMalloc(&d_ranges, pairs.size() * 8);
+        if (e != hipSuccess) return fail(MCDP_E_NOMEM, std::string("hipMalloc (event bounds): ") + hipGetErrorString(e));
+        e = hipMemcpy(d_ranges, pairs.data(), pairs.size() * 8, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(d_ranges);
+            return hip_fail(e, "upload event bounds");
+        }
+    }
     std::lock_guard<std::mutex> lock(srv->mu);
-    if (srv->contexts.size() >= (size_t)std::numeric_limits<int32_t>::max())
+    if (srv->contexts.size() >= (size_t)std::numeric_limits<int32_t>::max()) {
+        if (d_ranges) (void)hipFree(d_ranges);
         return fail(MCDP_E_INVALID, "too many contexts");
+    }
     *ctx_id = (int32_t)srv->contexts.size();
     srv->contexts.push_back(prop);
+    srv->ranges.push_back(d_ranges);
     return MCDP_OK;
 }
 
@@ -520,31 +609,37 @@ int mcdp_server_submit(mcdp_server *srv, int32_t ctx_id, const int32_t *seeds, i
         if (!(q[j] >= 0.0 && q[j] <= 1.0)) return fail(MCDP_E_INVALID, "quantiles must be in the range [0, 1]");
     auto r = std::make_unique<Request>();
     mcdp_propagator *prop;
+    bool exact;
+    int32_t kept_limit;
     {
         std::lock_guard<std::mutex> lock(srv->mu);
         if (ctx_id < 0 || (size_t)ctx_id >= srv->contexts.size()) return fail(MCDP_E_INVALID, "unknown context");
         prop = srv->contexts[(size_t)ctx_id];
+        exact = srv->exact_chunked;
+        kept_limit = srv->kept_limit;
     }
     r->ctx = ctx_id;
     r->E = prop->E;
     r->P = (int64_t)prop->preds.size();
     r->seeds.assign(seeds, seeds + n);
     r->q.assign(q, q + n_q);
-    r->bytes = footprint(n, n, r->E, r->P, n_q, false);
+    r->bytes = footprint(n, n, r->E, r->P, n_q, false, 0);
     if (r->bytes > srv->request_cap) {
         // too large to hold all samples at once: chunks of samples, the largest that fit the cap
         if (!(prop->max_delay < std::numeric_limits<double>::infinity()))
             return fail(MCDP_E_NOMEM, "request needs " + std::to_string(r->bytes) + " bytes of device memory (cap " +
                                           std::to_string(srv->request_cap) +
                                           "); statistics over sample chunks need a finite max_delay");
-        const int64_t m = chunk_samples(n, r->E, r->P, n_q, srv->request_cap);
+        int64_t m, kept;
+        chunk_plan(n, r->E, r->P, n_q, srv->request_cap, exact, kept_limit, m, kept);
         if (m < 1)
             return fail(MCDP_E_NOMEM, "request needs " + std::to_string(r->bytes) + " bytes of device memory; even "
                                       "chunks of 128 samples exceed the request cap of " +
                                       std::to_string(srv->request_cap) + " bytes (budget " +
                                       std::to_string(srv->pool->size()) + ")");
         r->chunk = m;
-        r->bytes = footprint(n, r->chunk, r->E, r->P, n_q, true);
+        r->kept_cap = kept;
+        r->bytes = footprint(n, r->chunk, r->E, r->P, n_q, true, kept);
     }
     // Admission is strictly in arrival order: a request the pool can never
     // hold would block every later one, so none is queued.
@@ -552,9 +647,9 @@ int mcdp_server_submit(mcdp_server *srv, int32_t ctx_id, const int32_t *seeds, i
         return fail(MCDP_E_NOMEM, "request footprint of " + std::to_string(r->bytes) +
                                       " bytes exceeds the request cap of " + std::to_string(srv->request_cap) +
                                       " bytes");
-    // results, then the exact selection's four counters
+    // results, then the exact selection's four counters and the kept-row count
     const size_t host_doubles = (size_t)(r->E + r->E * n_q + r->E + r->P);
-    hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&r->h_out), (host_doubles + 4) * 8);
+    hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&r->h_out), (host_doubles + 8) * 8);
     if (e != hipSuccess) return fail(MCDP_E_NOMEM, std::string("hipHostMalloc (results): ") + hipGetErrorString(e));
     r->h_status = reinterpret_cast<uint64_t *>(r->h_out + host_doubles);
     r->t_submit = Clock::now();
@@ -608,27 +703,28 @@ int mcdp_debug_serve_chunk(int64_t n, int64_t n_events, int64_t n_preds, int32_t
     if (!chunk || !bytes || n < 1 || n_events < 0 || n_preds < 0 || n_q < 0 || n_q > 8 || request_cap < 0)
         return fail(MCDP_E_INVALID, "invalid argument");
     const size_t cap = (size_t)request_cap;
-    const size_t whole = footprint(n, n, n_events, n_preds, n_q, false);
+    const size_t whole = footprint(n, n, n_events, n_preds, n_q, false, 0);
     if (whole <= cap) {
         *chunk = 0;
         *bytes = (int64_t)whole;
         return MCDP_OK;
     }
-    const int64_t m = chunk_samples(n, n_events, n_preds, n_q, cap);
+    int64_t m, kept;
+    chunk_plan(n, n_events, n_preds, n_q, cap, true, -1, m, kept);
     if (m < 1) return fail(MCDP_E_NOMEM, "even chunks of 128 samples exceed the request cap");
     *chunk = m;
-    *bytes = (int64_t)footprint(n, m, n_events, n_preds, n_q, true);
+    *bytes = (int64_t)footprint(n, m, n_events, n_preds, n_q, true, kept);
     return MCDP_OK;
 }
 
 int mcdp_server_stats(mcdp_server *srv, int64_t *out, int n_out) {
     if (!srv || !out) return fail(MCDP_E_INVALID, "null argument");
     std::lock_guard<std::mutex> lock(srv->mu);
-    const int64_t v[10] = {srv->n_submitted,          srv->n_completed,         srv->n_failed,
+    const int64_t v[11] = {srv->n_submitted,          srv->n_completed,         srv->n_failed,
                            (int64_t)srv->queued.size(), (int64_t)srv->running.size(), srv->max_running_seen,
                            (int64_t)srv->pool->used(), (int64_t)srv->pool->peak(), srv->n_chunked,
-                           srv->n_rounds};
-    for (int i = 0; i < n_out && i < 10; ++i) out[i] = v[i];
+                           srv->n_rounds,               srv->n_kept};
+    for (int i = 0; i < n_out && i < 11; ++i) out[i] = v[i];
     return MCDP_OK;
 }
 
@@ -648,6 +744,13 @@ int mcdp_debug_server_exact_limits(mcdp_server *srv, int32_t c_max, int64_t cand
     srv->exact_c_max = c_max;
     srv->exact_cand_per_target = cand_per_target;
     srv->exact_slots = refine_slots;
+    return MCDP_OK;
+}
+
+int mcdp_debug_server_kept_rows(mcdp_server *srv, int32_t rows) {
+    if (!srv) return fail(MCDP_E_INVALID, "server is null");
+    std::lock_guard<std::mutex> lock(srv->mu);
+    srv->kept_limit = rows < 0 ? -1 : rows;
     return MCDP_OK;
 }
 

@@ -151,7 +151,7 @@ int mcdp::exact_rank_count(int64_t n, int32_t n_q, const double *q, int32_t *n_r
 }
 
 int mcdp::exact_stats_init(mcdp_propagator &P, int64_t n, int32_t n_q, const double *q, const ChunkStat *st,
-                           const uint32_t *hist, ExactWork &w, hipStream_t s) {
+                           const uint32_t *hist, ExactWork &w, const WatchWork *watch, hipStream_t s) {
     QuantileArgs qa;
     int rc = quantile_args(n, n_q, q, qa);
     if (rc) return rc;
@@ -161,8 +161,40 @@ int mcdp::exact_stats_init(mcdp_propagator &P, int64_t n, int32_t n_q, const dou
     DeviceCopy *d = nullptr;
     rc = ensure_device(P, d);
     if (rc) return rc;
-    HIP_TRY(launch_exact_init(n, P.E, qa, st, hist, kMaxChunkBins, d->base.earliest, P.max_delay, w, s),
+    HIP_TRY(launch_exact_init(n, P.E, qa, st, hist, kMaxChunkBins, d->base.earliest, P.max_delay, w, watch, s),
             "exact quantile kernels");
+    return MCDP_OK;
+}
+
+int mcdp::counters_clear(uint64_t *c, int n, hipStream_t s) {
+    HIP_TRY(launch_counters_clear(c, n, s), "counter clearing kernel");
+    return MCDP_OK;
+}
+
+int mcdp::watch_select(mcdp_propagator &P, int64_t n, int32_t m, int32_t n_q, const double *q, const ChunkStat *st,
+                       const double *ranges, const WatchWork &w, hipStream_t s) {
+    QuantileArgs qa;
+    const int rc = quantile_args(n, n_q, q, qa);
+    if (rc) return rc;
+    HIP_TRY(launch_watch_select(n, m, P.E, qa, st, ranges, w, s), "kept-row selection kernel");
+    return MCDP_OK;
+}
+
+int mcdp::watch_copy(const double *realized, int32_t m, int32_t ld, int64_t c0, const WatchWork &w, hipStream_t s) {
+    HIP_TRY(launch_watch_copy(realized, m, ld, c0, w, s), "kept-row copy kernel");
+    return MCDP_OK;
+}
+
+int mcdp::watch_finish(int64_t n, int32_t n_q, const double *q, const WatchWork &w, int32_t rows, double *quantiles,
+                       hipStream_t s) {
+    if (rows <= 0 || n_q <= 0) return MCDP_OK;
+    if (rows > w.cap || n > std::numeric_limits<int32_t>::max() || w.ld != n)
+        return fail(MCDP_E_INVALID, "internal error: kept rows do not match the request");
+    QuantileArgs qa;
+    const int rc = quantile_args(n, n_q, q, qa);
+    if (rc) return rc;
+    HIP_TRY(launch_event_quantiles(w.store, (int32_t)n, (int32_t)n, rows, qa, w.wq, nullptr, s), "quantile kernel");
+    HIP_TRY(launch_watch_scatter(w, rows, n_q, quantiles, s), "kept-row scatter kernel");
     return MCDP_OK;
 }
 

@@ -1093,7 +1093,109 @@ __global__ void exact_finish_kernel(int32_t E, QuantileArgs qa, const ChunkStat 
             nan_row ? __builtin_nan("") : lerp_numpy(t[qa.lo[j]].value, t[qa.hi[j]].value, qa.gamma[j]);
 }
 
+// ---- rows kept whole across the chunks of a request -------------------------
+__global__ void counters_clear_kernel(uint64_t *c, int n) {
+    if ((int)threadIdx.x < n) c[threadIdx.x] = 0;
+}
+
+// One event per thread: is any rank of the n samples outside the masses of
+// the extremes, as far as the first chunk's m samples tell?  A mass counts
+// only where its extreme is the event's bound; its fraction f is widened by
+// 4 sqrt(max(f (1 - f), 1/m) / m) + 2/m.  Listed in the order the atomic
+// hands out places (the order does not change any result).
+__global__ void __launch_bounds__(256) watch_select_kernel(int64_t n, int32_t m, int32_t E, QuantileArgs qa,
+                                                           const ChunkStat *__restrict__ st,
+                                                           const double *__restrict__ ranges, WatchWork w) {
+    const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const ChunkStat c = st[e];
+    if (c.nans || c.cmn == 0) return;  // a row with NaN has NaN quantiles, resolved at once
+    const double inv_m = 1.0 / (double)m;
+    const double f_lo = key_value(c.mn) == ranges[2 * (int64_t)e] ? (double)c.cmn * inv_m : 0.0;
+    const double f_hi = key_value(c.mx) == ranges[2 * (int64_t)e + 1] ? (double)c.cmx * inv_m : 0.0;
+    auto widen = [&](double f) {
+        const double v = f * (1.0 - f);
+        return 4.0 * sqrt((v > inv_m ? v : inv_m) * inv_m) + 2.0 * inv_m;
+    };
+    const double k_lo = (f_lo - widen(f_lo)) * (double)n;
+    const double k_hi = (1.0 - f_hi + widen(f_hi)) * (double)n;
+    bool open = false;
+    for (int r = 0; r < qa.n_ranks; ++r) {
+        const double k = (double)qa.rank[r];
+        open = open || (k >= k_lo && k < k_hi);
+    }
+    if (!open) return;
+    const uint64_t i = atomicAdd((unsigned long long *)w.count, 1ull);
+    if (i < (uint64_t)w.cap) w.list[i] = e;
+}
+
+__device__ __forceinline__ uint32_t watch_rows(const WatchWork &w) {
+    const uint64_t c = *w.count;
+    return c < (uint64_t)w.cap ? (uint32_t)c : (uint32_t)w.cap;
+}
+
+__global__ void __launch_bounds__(256) watch_copy_kernel(const double *__restrict__ realized, int32_t m, uint32_t ld,
+                                                         int64_t c0, WatchWork w) {
+    const uint32_t rows = watch_rows(w);
+    for (uint32_t i = blockIdx.x; i < rows; i += gridDim.x) {
+        const double *src = realized + (uint64_t)(uint32_t)w.list[i] * ld;
+        double *dst = w.store + (uint64_t)i * (uint64_t)w.ld + (uint64_t)c0;
+        for (int32_t s = threadIdx.x; s < m; s += blockDim.x) dst[s] = src[s];
+    }
+}
+
+// The targets of kept rows need no selection: resolved (their quantiles are
+// written over exact_finish_kernel's by launch_watch_scatter).
+__global__ void __launch_bounds__(256) watch_mark_kernel(WatchWork w, ExactWork x) {
+    const uint32_t rows = watch_rows(w);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rows; i += gridDim.x * blockDim.x) {
+        ExactTarget *t = x.targets + (uint64_t)(uint32_t)w.list[i] * (uint32_t)x.n_ranks;
+        for (int r = 0; r < x.n_ranks; ++r) {
+            t[r].mode = kExactResolved;
+            t[r].slot = -1;
+            t[r].value = 0.0;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) watch_scatter_kernel(WatchWork w, int32_t rows, int32_t n_q,
+                                                            double *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)rows * n_q) return;
+    const int32_t r = (int32_t)(i / n_q), j = (int32_t)(i % n_q);
+    out[(uint64_t)(uint32_t)w.list[r] * (uint32_t)n_q + j] = w.wq[i];
+}
+
 }  // namespace
+
+hipError_t launch_counters_clear(uint64_t *c, int n, hipStream_t stream) {
+    hipLaunchKernelGGL(counters_clear_kernel, dim3(1), dim3(64), 0, stream, c, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_watch_select(int64_t n, int32_t m, int32_t E, const QuantileArgs &qa, const ChunkStat *st,
+                               const double *ranges, const WatchWork &w, hipStream_t stream) {
+    if (E > 0 && m > 0 && w.cap > 0)
+        hipLaunchKernelGGL(watch_select_kernel, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, stream, n, m, E, qa,
+                           st, ranges, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_watch_copy(const double *realized, int32_t m, int32_t ld, int64_t c0, const WatchWork &w,
+                             hipStream_t stream) {
+    if (m > 0 && w.cap > 0)
+        hipLaunchKernelGGL(watch_copy_kernel, dim3((unsigned)(w.cap < 1024 ? w.cap : 1024)), dim3(256), 0, stream, realized, m,
+                           (uint32_t)ld, c0, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_watch_scatter(const WatchWork &w, int32_t rows, int32_t n_q, double *out, hipStream_t stream) {
+    const int64_t total = (int64_t)rows * n_q;
+    if (total > 0)
+        hipLaunchKernelGGL(watch_scatter_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, w, rows,
+                           n_q, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_chunk_accumulate(const double *realized, int32_t m, int32_t ld, int32_t E, const double *earliest,
                                    double max_delay, int32_t n_bins, int first, ChunkStat *st, uint32_t *hist,
@@ -1115,11 +1217,14 @@ hipError_t launch_chunk_finish(int64_t n, int32_t E, const QuantileArgs &qa, con
 
 hipError_t launch_exact_init(int64_t n, int32_t E, const QuantileArgs &qa, const ChunkStat *st, const uint32_t *hist,
                              int32_t n_bins, const double *earliest, double max_delay, const ExactWork &w,
-                             hipStream_t stream) {
+                             const WatchWork *watch, hipStream_t stream) {
     hipLaunchKernelGGL(exact_reset_kernel, dim3(1), dim3(1), 0, stream, w.counters, 1);
     if (E > 0) {
         hipLaunchKernelGGL(exact_init_kernel, dim3((unsigned)E), dim3(kBlock), 0, stream, n, E, qa, st, hist, n_bins,
                            earliest, max_delay / (double)n_bins, w);
+        if (watch && watch->cap > 0)
+            hipLaunchKernelGGL(watch_mark_kernel, dim3((unsigned)((watch->cap + 255) / 256 < 1024 ? (watch->cap + 255) / 256 : 1024)), dim3(256), 0,
+                               stream, *watch, w);
         hipLaunchKernelGGL(exact_assign_kernel, dim3((unsigned)E), dim3(kBlock), 0, stream, E, w);
     }
     return hipGetLastError();

@@ -100,13 +100,41 @@ struct ExactWork {
     int32_t c_max;
     uint32_t cand_cap;
 };
+// Rows kept whole for a request run in chunks.  After the first chunk, the
+// events whose order statistics may leave the masses of their extremes are
+// listed; each later chunk appends their samples to a row of n, so their
+// quantiles come from the whole row (the one-piece kernel) and their targets
+// need no further pass.  list [cap] event ids; *count the events chosen (past
+// cap they are not kept and take further passes if they need them); store
+// [cap][ld] with ld the request's samples; wq [cap][n_q] their quantiles.
+struct WatchWork {
+    int32_t *list;
+    uint64_t *count;
+    double *store;
+    double *wq;
+    int32_t cap;
+    int64_t ld;
+};
+// c[0..n) = 0 (a kernel, so the clearing may sit in a captured graph).
+hipError_t launch_counters_clear(uint64_t *c, int n, hipStream_t stream);
+// Chooses the rows to keep from the first chunk's statistics st (m samples of
+// n): an extreme's mass counts only where the extreme equals the event's bound
+// (ranges[2e], ranges[2e + 1]: no later value can pass it), and each mass
+// fraction is widened by four standard errors and two samples.
+hipError_t launch_watch_select(int64_t n, int32_t m, int32_t E, const QuantileArgs &qa, const ChunkStat *st,
+                               const double *ranges, const WatchWork &w, hipStream_t stream);
+// Appends samples c0 .. c0 + m of the kept rows (realized[e * ld + s], s < m).
+hipError_t launch_watch_copy(const double *realized, int32_t m, int32_t ld, int64_t c0, const WatchWork &w,
+                             hipStream_t stream);
+// out[e * n_q + j] = wq[i * n_q + j] for the first `rows` kept events e = list[i].
+hipError_t launch_watch_scatter(const WatchWork &w, int32_t rows, int32_t n_q, double *out, hipStream_t stream);
 // After the chunks' running statistics: targets from the histogram (ranks on
-// an extreme's mass, or of a row with NaN, are resolved at once), then modes
-// handed out.  counters[2] = unresolved targets, counters[3] != 0 on an
-// inconsistency.
+// an extreme's mass, or of a row with NaN, are resolved at once; so are those
+// of kept rows, when watch is given), then modes handed out.  counters[2] =
+// unresolved targets, counters[3] != 0 on an inconsistency.
 hipError_t launch_exact_init(int64_t n, int32_t E, const QuantileArgs &qa, const ChunkStat *st, const uint32_t *hist,
                              int32_t n_bins, const double *earliest, double max_delay, const ExactWork &w,
-                             hipStream_t stream);
+                             const WatchWork *watch, hipStream_t stream);
 // One chunk of a pass: rows realized[e][0..m) against every open target.
 hipError_t launch_exact_pass(const double *realized, int32_t m, int32_t ld, int32_t E, const double *earliest,
                              double max_delay, int32_t n_bins, const ExactWork &w, hipStream_t stream);

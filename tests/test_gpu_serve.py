@@ -4,10 +4,12 @@ Each request's statistics must equal those of the same seeds run alone (``run_st
 whatever else is queued or running; the pool's peak use stays within its budget; requests wait for
 memory instead of failing; requests that could never fit are refused at submission.  Requests above
 the request cap run in chunks of samples and still equal ``run_stats`` bit for bit, quantiles
-included (further passes over the chunks select the exact order statistics): with the default
-limits, with limits small enough that regions are refined over several passes and targets wait for
-refine slots, and on rows full of ties.  With exact selection off, quantiles are within one
-histogram bin.
+included: rows kept whole for the events whose order statistics may leave their extremes' masses
+(chosen from the first chunk and the events' bounds, which must enclose every draw), and further
+passes over the chunks that select the other order statistics exactly; with the default limits,
+with no kept rows and limits small enough that regions are refined over several passes and targets
+wait for refine slots, with too few kept rows, and on rows full of ties.  With exact selection off,
+quantiles are within one histogram bin.
 """
 
 from __future__ import annotations
@@ -150,6 +152,7 @@ def test_chunked_requests(gpu) -> None:
     dags, props = contexts()
     big, small = props[1], props[0]
     srv = capi.Server(96 << 20, max_running=4, n_streams=3, request_cap=24 << 20)
+    srv.debug_kept_rows(0)  # every order statistic off the extremes' masses takes further passes
     cb, cs = srv.add_context(big), srv.add_context(small)
     rng = np.random.default_rng(12)
     q = [0.0, 0.25, 0.5, 0.9, 0.99, 1.0]
@@ -181,12 +184,14 @@ def jitter_chain(n_events: int = 60, max_delay: float = 500.0):
     return D.FlatDag.from_lists(events, acts, prec, max_delay, dists)
 
 
-def test_chunked_exact_selection(gpu) -> None:
+@pytest.mark.parametrize("kept", [0, -1], ids=["passes", "kept_rows"])
+def test_chunked_exact_selection(gpu, kept) -> None:
     """Bit-exact chunked quantiles with the default limits: a layered DAG and a chain of clustered, tied values,
-    seven quantiles each."""
+    seven quantiles each; by further passes alone, and with the default kept rows."""
     dags = [layered_network(layers=2), jitter_chain()]
     props = [capi.Propagator(d) for d in dags]
     srv = capi.Server(48 << 20, max_running=2, n_streams=2, request_cap=8 << 20)
+    srv.debug_kept_rows(kept)
     ids = [srv.add_context(p) for p in props]
     q = (0.0, 0.1, 0.5, 0.75, 0.9, 0.99, 1.0)
     reqs = []
@@ -197,7 +202,11 @@ def test_chunked_exact_selection(gpu) -> None:
     for rid, p, seeds in reqs:
         same_stats(srv.wait(rid), p.run_stats(seeds, q))
     st = srv.stats()
-    assert st["completed"] == 2 and st["failed"] == 0 and st["exact_rounds"] >= 2
+    assert st["completed"] == 2 and st["failed"] == 0
+    if kept == 0:
+        assert st["exact_rounds"] >= 2 and st["kept_rows"] == 0
+    else:
+        assert st["kept_rows"] > 0
     srv.close()
 
 
@@ -210,6 +219,7 @@ def test_chunked_exact_selection_small_limits(gpu, limits) -> None:
     props = [capi.Propagator(d) for d in dags]
     srv = capi.Server(16 << 20, max_running=2, n_streams=2, request_cap=2 << 20)
     srv.debug_exact_limits(*limits)
+    srv.debug_kept_rows(0)
     ids = [srv.add_context(p) for p in props]
     q = (0.1, 0.5, 0.9)
     reqs = []
@@ -222,6 +232,79 @@ def test_chunked_exact_selection_small_limits(gpu, limits) -> None:
     st = srv.stats()
     assert st["completed"] == 2 and st["failed"] == 0 and st["exact_rounds"] >= 4
     srv.close()
+
+
+def test_ranges_bound_every_draw(gpu) -> None:
+    """Every duration the device draws lies in its link's range, and every realized time in its event's range
+    (mcdp_propagator_ranges), over every sampling method the compiler picks: serving trusts a chunk's extreme
+    that equals its event's bound never to be passed by a later chunk."""
+    from test_ranges import mixed_dag
+
+    for dag in (mixed_dag(0), mixed_dag(1, 500, 1e3), layered_network(layers=12)):
+        p = capi.Propagator(dag)
+        llo, lhi, elo, ehi = p.ranges()
+        seeds = np.arange(8192, dtype=np.int32) * 31 + 7
+        real, dur, _ = p.run_many(seeds)
+        assert np.all((dur >= llo[None, :]) & (dur <= lhi[None, :]))
+        assert np.all((real >= elo[None, :]) & (real <= ehi[None, :]))
+        # the bounds are attained where they are point masses (constants, caps, earliest times)
+        assert np.any(real == elo[None, :]) and np.any(real == ehi[None, :])
+
+
+def kept_case():
+    """The network workload's literal distributions saturate most events at their cap: a few events per
+    hundred have order statistics off their extremes' masses, the rest on them."""
+    d = layered_network(layers=20)
+    p = capi.Propagator(d)
+    seeds = (np.arange(24_000, dtype=np.int64) * 7919 + 3).astype(np.int32)
+    return d, p, seeds
+
+
+def test_kept_rows_need_no_further_pass(gpu) -> None:
+    """With the default kept rows a chunked request on the network workload needs no further pass, and its
+    statistics equal run_stats bit for bit."""
+    d, p, seeds = kept_case()
+    q = (0.5, 0.9, 0.99)
+    srv = capi.Server(256 << 20, max_running=2, n_streams=2, request_cap=64 << 20)
+    cid = srv.add_context(p)
+    got = srv.wait(srv.submit(cid, seeds, q))
+    st = srv.stats()
+    srv.close()
+    same_stats(got, p.run_stats(seeds, q))
+    assert st["chunked"] == 1 and st["failed"] == 0
+    assert 0 < st["kept_rows"] <= max(16, p.E // 128)
+    assert st["exact_rounds"] == 0
+
+
+@pytest.mark.parametrize("rows", [1, 2])
+def test_too_few_kept_rows(gpu, rows) -> None:
+    """Kept rows past the limit fall back to further passes: still bit for bit."""
+    d, p, seeds = kept_case()
+    q = (0.5, 0.9, 0.99)
+    srv = capi.Server(256 << 20, max_running=2, n_streams=2, request_cap=64 << 20)
+    srv.debug_kept_rows(rows)
+    cid = srv.add_context(p)
+    got = srv.wait(srv.submit(cid, seeds, q))
+    st = srv.stats()
+    srv.close()
+    same_stats(got, p.run_stats(seeds, q))
+    assert st["chunked"] == 1 and st["failed"] == 0 and st["kept_rows"] == rows and st["exact_rounds"] >= 1
+
+
+def test_kept_rows_on_continuous_rows(gpu) -> None:
+    """Continuous rows put every order statistic off the extremes: the kept rows fill up (16 of them) and the
+    other events take further passes; bit for bit."""
+    d = D.continuous_dag(5, 120, max_delay=30.0, shared_links=True)
+    p = capi.Propagator(d)
+    seeds = (np.arange(20_000, dtype=np.int64) * 104729 + 9).astype(np.int32)
+    q = (0.1, 0.5, 0.9)
+    srv = capi.Server(64 << 20, max_running=2, n_streams=2, request_cap=16 << 20)
+    cid = srv.add_context(p)
+    got = srv.wait(srv.submit(cid, seeds, q))
+    st = srv.stats()
+    srv.close()
+    same_stats(got, p.run_stats(seeds, q))
+    assert st["chunked"] == 1 and st["kept_rows"] == 16 and st["exact_rounds"] >= 1
 
 
 def test_chunked_extremes_and_no_quantiles(gpu) -> None:

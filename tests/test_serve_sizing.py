@@ -9,8 +9,10 @@ footprint above the cap, which the strictly ordered admission could never place.
 The footprint below restates the device layout (256-byte aligned blocks: seeds, realized and
 cause rows, mean, quantiles, cause counts, and for chunked requests the running statistics of
 56 bytes per event and a 4,096-bin histogram per event, and with quantiles the exact selection's
-targets, candidate values and refine slots); the test checks the library's answers against it:
-every accepted request fits the cap, and no larger chunk would.
+targets, candidate values and refine slots, and the rows kept whole: one per 128 events, at least
+16 and at most E, within a quarter of the cap, each with its n samples, its quantiles and its event
+id; they give way to none when the chunks would not fit beside them); the test checks the
+library's answers against it: every accepted request fits the cap, and no larger chunk would.
 """
 
 from __future__ import annotations
@@ -33,7 +35,7 @@ def _a(x: int) -> int:
     return (x + 255) // 256 * 256
 
 
-def footprint(n: int, rows: int, E: int, P: int, nq: int, chunked: bool) -> int:
+def footprint(n: int, rows: int, E: int, P: int, nq: int, chunked: bool, kept: int = 0) -> int:
     b = _a(n * 4) + _a(E * rows * 8) + _a(E * rows * 4) + _a(E * 8) + _a(E * nq * 8) + _a((E + P) * 8)
     if chunked:
         b += _a(E * CHUNK_STAT_BYTES) + _a(E * CHUNK_BINS * 4)
@@ -43,7 +45,33 @@ def footprint(n: int, rows: int, E: int, P: int, nq: int, chunked: bool) -> int:
         slots = min(1024, max(16, (T + 63) // 64))
         b += _a(T * EXACT_TARGET_BYTES) + _a(T * per_target * 8) + _a(slots * 256 * 4)
         b += 2 * _a(slots * 256 * 8) + 256
+        if kept:  # rows kept whole: event ids, samples, quantiles
+            b += _a(kept * 4) + _a(kept * n * 8) + _a(kept * nq * 8)
     return b
+
+
+def kept_rows(n: int, E: int, nq: int, cap: int) -> int:
+    if nq <= 0:
+        return 0
+    return min(E, max(16, (E + 127) // 128), cap // 4 // (n * 8))
+
+
+def chunk_of(n: int, E: int, P: int, nq: int, cap: int, kept: int) -> int:
+    """The largest multiple of 128 samples (at most n) whose footprint, with 1,024 bytes of slack, fits."""
+    fixed = footprint(n, 0, E, P, nq, True, kept)
+    if fixed + 1024 >= cap:
+        return 0
+    m = (cap - fixed - 1024) // (12 * max(E, 1)) // 128 * 128
+    m = min(m, n)
+    return m if m >= 128 and footprint(n, m, E, P, nq, True, kept) <= cap else 0
+
+
+def plan(n: int, E: int, P: int, nq: int, cap: int) -> tuple[int, int]:
+    kept = kept_rows(n, E, nq, cap)
+    m = chunk_of(n, E, P, nq, cap, kept)
+    if m < 1 and kept > 0:
+        kept, m = 0, chunk_of(n, E, P, nq, cap, 0)
+    return m, kept
 
 
 def sizing(n: int, E: int, P: int, nq: int, cap: int) -> tuple[int, int, int]:
@@ -92,13 +120,15 @@ def test_random_requests_fit_their_cap_and_chunks_are_maximal() -> None:
             assert (rc, chunk, nbytes) == (0, 0, whole)
             continue
         if rc != 0:
-            assert rc == 6
+            assert rc == 6 and plan(n, E, P, nq, cap)[0] == 0
             assert footprint(n, min(128, n), E, P, nq, True) + 1024 > cap
             continue
         assert 1 <= chunk <= n and (chunk % 128 == 0 or chunk == n), chunk
-        assert nbytes == footprint(n, chunk, E, P, nq, True) and nbytes <= cap
+        m, kept = plan(n, E, P, nq, cap)
+        assert m == chunk
+        assert nbytes == footprint(n, chunk, E, P, nq, True, kept) and nbytes <= cap
         if chunk < n and chunk + 128 <= n:
-            assert footprint(n, chunk + 128, E, P, nq, True) + 1024 > cap
+            assert footprint(n, chunk + 128, E, P, nq, True, kept) + 1024 > cap
 
 
 def test_invalid_arguments() -> None:
