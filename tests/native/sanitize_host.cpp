@@ -211,6 +211,29 @@ void check_dag(Rng &r, int trial) {
         CHECK(mcdp_propagator_get_times(p, dmin2.data(), e2.data()) == MCDP_OK, "get_times");
         CHECK(dmin2 == dmin && e2 == e0, "times round trip");
         CHECK(mcdp_propagator_gradient_info(p, ginfo, 6) == MCDP_OK, "gradient_info after set_times");
+        // bounds for every seed: the event ranges are the oracle's propagation of the link ranges' two ends
+        {
+            std::vector<double> llo((size_t)std::max(L, 1)), lhi((size_t)std::max(L, 1));
+            std::vector<double> elo((size_t)std::max(E, 1)), ehi((size_t)std::max(E, 1));
+            const int rrc = mcdp_propagator_ranges(p, llo.data(), lhi.data(), elo.data(), ehi.data());
+            bool finite = std::isfinite(d.max_delay);
+            for (double x : d.earliest) finite = finite && std::isfinite(x);
+            CHECK((rrc == MCDP_OK) == finite, "trial %d: ranges rc %d", trial, rrc);
+            if (rrc == MCDP_OK && E > 0) {
+                std::vector<double> ends((size_t)std::max(L, 1) * 2), r2((size_t)E * 2);
+                std::vector<int> c2((size_t)E * 2);
+                for (int l = 0; l < L; ++l) {
+                    CHECK(!(llo[(size_t)l] > lhi[(size_t)l]), "trial %d: link %d range", trial, l);
+                    ends[(size_t)l] = llo[(size_t)l];
+                    ends[(size_t)(L + l)] = lhi[(size_t)l];
+                }
+                orc_propagate(o, ends.data(), L, 2, r2.data(), c2.data());
+                for (int e = 0; e < E; ++e)
+                    CHECK(std::memcmp(&r2[(size_t)e], &elo[(size_t)e], 8) == 0 &&
+                              std::memcmp(&r2[(size_t)(E + e)], &ehi[(size_t)e], 8) == 0,
+                          "trial %d: event %d range", trial, e);
+            }
+        }
         // no device here: every run entry point fails cleanly
         int32_t seeds[3] = {1, 2, 3};
         std::vector<double> rr((size_t)E * 3), dd((size_t)L * 3);

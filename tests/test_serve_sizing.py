@@ -108,6 +108,7 @@ def test_smallest_chunk_at_the_boundary() -> None:
 
 def test_random_requests_fit_their_cap_and_chunks_are_maximal() -> None:
     rng = np.random.default_rng(7)
+    with_kept = 0
     for _ in range(3000):
         E = int(rng.integers(1, 250_000))
         P = int(rng.integers(0, 3 * E + 1))
@@ -126,9 +127,11 @@ def test_random_requests_fit_their_cap_and_chunks_are_maximal() -> None:
         assert 1 <= chunk <= n and (chunk % 128 == 0 or chunk == n), chunk
         m, kept = plan(n, E, P, nq, cap)
         assert m == chunk
+        with_kept += kept > 0
         assert nbytes == footprint(n, chunk, E, P, nq, True, kept) and nbytes <= cap
         if chunk < n and chunk + 128 <= n:
             assert footprint(n, chunk + 128, E, P, nq, True, kept) + 1024 > cap
+    assert with_kept > 100  # the kept rows' share of the footprint is exercised, not only its fallback
 
 
 def test_invalid_arguments() -> None:
