@@ -644,8 +644,13 @@ def main() -> int:
         if baseline:
             baseline["host_nproc"] = os.cpu_count()
             baseline["host_affinity_cpus"] = affinity
+            # the same loop on one thread (a smaller bounded sample), for the per-core rate
+            one = cpu_baseline(dag, L, 1, 256 if args.workload == "network" else 8192, min_seconds=2.0, max_chunks=4)
+            if one:
+                baseline["one_thread"] = {"value": one["value"], "sample": one["sample"]}
             log(f"cpu baseline: {baseline['value']:.4g} activity-samples/s on {threads} threads "
-                f"(nproc {os.cpu_count()}, affinity {affinity})")
+                f"(nproc {os.cpu_count()}, affinity {affinity})"
+                + (f"; {one['value']:.4g} on one thread" if one else ""))
 
     out = {
         "metric": "activity-samples/sec GPU vs host run_many (bit-exact); p50/p99 solve latency",
