@@ -253,7 +253,8 @@ int mcdp_batch_stats_host(mcdp_batch *b, const int32_t *const *seeds, const int6
         if (bytes[d] > (size_t)memory_budget)
             return fail(MCDP_E_NOMEM, "one DAG's realized and cause rows exceed memory_budget");
     }
-    // Groups in batch order, each as large as the budget allows.
+    // Groups in batch order, each as large as the budget allows; the statistics'
+    // per-DAG descriptors follow the largest group in the workspace.
     std::vector<size_t> group_start{0};
     size_t max_group = 0;
     for (size_t d = 0, used = 0; d < D; ++d) {
@@ -265,39 +266,36 @@ int mcdp_batch_stats_host(mcdp_batch *b, const int32_t *const *seeds, const int6
         max_group = std::max(max_group, used);
     }
     group_start.push_back(D);
+    const size_t o_sets = align_to(max_group, a);
+    const size_t need = o_sets + align_to(std::max<size_t>(D, 1) * sizeof(RowSet), a);
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
         return fail(MCDP_E_DEVICE, "no HIP device available: the Monte Carlo engine runs on the GPU only");
     int device = 0;
     HIP_TRY(hipGetDevice(&device), "hipGetDevice");
-    if (!b->stats_work || b->stats_bytes < max_group || b->stats_device != device) {
+    if (!b->stats_work || b->stats_bytes < need || b->stats_device != device) {
         if (b->stats_work) {
             (void)hipDeviceSynchronize();
             (void)hipFree(b->stats_work);
             b->stats_work = nullptr;
             b->stats_bytes = 0;
         }
-        hipError_t e = hipMalloc(&b->stats_work, max_group);
+        hipError_t e = hipMalloc(&b->stats_work, need);
         if (e != hipSuccess) {
             b->stats_work = nullptr;
             return fail(MCDP_E_NOMEM, std::string("hipMalloc (batch statistics): ") + hipGetErrorString(e));
         }
-        b->stats_bytes = max_group;
+        b->stats_bytes = need;
         b->stats_device = device;
     }
-    // the launch on one stream; the DAGs' statistics spread over a few, so
-    // that the small DAGs' kernels fill the device together
-    constexpr int kStatStreams = 4;
-    hipStream_t s = nullptr, ss[kStatStreams] = {};
+    // the walk and then the group's statistics, every DAG's rows in one launch
+    // per statistics kernel (small DAGs alone would leave most CUs idle)
+    hipStream_t s = nullptr;
     hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
-    for (int i = 0; e == hipSuccess && i < kStatStreams; ++i) e = hipStreamCreateWithFlags(&ss[i], hipStreamNonBlocking);
-    if (e != hipSuccess) {
-        if (s) (void)hipStreamDestroy(s);
-        for (hipStream_t x : ss)
-            if (x) (void)hipStreamDestroy(x);
-        return hip_fail(e, "hipStreamCreateWithFlags");
-    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamCreateWithFlags");
     auto *base = static_cast<unsigned char *>(b->stats_work);
+    auto *sets_dev = reinterpret_cast<RowSet *>(base + o_sets);
+    std::vector<RowSet> sets;
     int rc = MCDP_OK;
     std::vector<const int32_t *> sd(D, nullptr);
     std::vector<int64_t> gn(D, 0), gld(D, 1);
@@ -325,27 +323,41 @@ int mcdp_batch_stats_host(mcdp_batch *b, const int32_t *const *seeds, const int6
         if (rc) break;
         // returns with the launch done (its stream synchronised)
         rc = mcdp_batch_run_device(b, sd.data(), gn.data(), gld.data(), real.data(), nullptr, cause.data(), s);
+        sets.clear();
+        int64_t rows = 0;
+        bool any_counts = false;
         for (size_t d = group_start[g]; rc == MCDP_OK && d < group_start[g + 1]; ++d) {
-            const hipStream_t sx = ss[(d - group_start[g]) % kStatStreams];
             unsigned char *p = part[d];
             auto *dm = reinterpret_cast<double *>(p + o_mean[d]);
             auto *dq = n_q > 0 ? reinterpret_cast<double *>(p + o_quant[d]) : nullptr;
             auto *dc = n_counts[d] ? reinterpret_cast<int64_t *>(p + o_cnt[d]) : nullptr;
-            rc = mcdp_event_stats_device(b->props[d], real[d], cause[d], n[d], n[d], n_q, q, dm, dq, dc, sx);
+            RowSet rs;
+            rc = row_set(*b->props[d], real[d], cause[d], n[d], n_q, q, dm, dq, dc, rows, rs);
+            sets.push_back(rs);
+            rows += b->props[d]->E;
+            any_counts = any_counts || dc != nullptr;
+        }
+        if (rc == MCDP_OK &&
+            (e = hipMemcpyAsync(sets_dev, sets.data(), sets.size() * sizeof(RowSet), hipMemcpyHostToDevice, s)) !=
+                hipSuccess)
+            rc = hip_fail(e, "copy statistics descriptors");
+        if (rc == MCDP_OK) rc = set_stats(sets_dev, (int32_t)sets.size(), rows, n_q > 0, any_counts, s);
+        for (size_t d = group_start[g]; rc == MCDP_OK && d < group_start[g + 1]; ++d) {
+            unsigned char *p = part[d];
             const size_t E = (size_t)b->props[d]->E;
-            if (rc == MCDP_OK && (e = hipMemcpyAsync(mean[d], dm, 8 * E, hipMemcpyDeviceToHost, sx)) != hipSuccess)
+            if ((e = hipMemcpyAsync(mean[d], p + o_mean[d], 8 * E, hipMemcpyDeviceToHost, s)) != hipSuccess)
                 rc = hip_fail(e, "copy means");
-            if (rc == MCDP_OK && dq &&
-                (e = hipMemcpyAsync(quantiles[d], dq, 8 * E * (size_t)n_q, hipMemcpyDeviceToHost, sx)) != hipSuccess)
-                rc = hip_fail(e, "copy quantiles");
-            if (rc == MCDP_OK && dc &&
-                (e = hipMemcpyAsync(cause_counts[d], dc, 8 * (size_t)n_counts[d], hipMemcpyDeviceToHost, sx)) !=
+            if (rc == MCDP_OK && n_q > 0 &&
+                (e = hipMemcpyAsync(quantiles[d], p + o_quant[d], 8 * E * (size_t)n_q, hipMemcpyDeviceToHost, s)) !=
                     hipSuccess)
+                rc = hip_fail(e, "copy quantiles");
+            if (rc == MCDP_OK && n_counts[d] &&
+                (e = hipMemcpyAsync(cause_counts[d], p + o_cnt[d], 8 * (size_t)n_counts[d], hipMemcpyDeviceToHost,
+                                    s)) != hipSuccess)
                 rc = hip_fail(e, "copy cause counts");
         }
-        // the next group reuses the workspace: every statistic of this one is done first
-        for (int i = 0; i < kStatStreams; ++i)
-            if ((e = hipStreamSynchronize(ss[i])) != hipSuccess && rc == MCDP_OK) rc = hip_fail(e, "hipStreamSynchronize");
+        // the next group reuses the workspace (and the descriptors' host copy goes away): all of this one is done
+        if ((e = hipStreamSynchronize(s)) != hipSuccess && rc == MCDP_OK) rc = hip_fail(e, "hipStreamSynchronize");
         for (size_t d = group_start[g]; d < group_start[g + 1]; ++d) {
             sd[d] = nullptr;
             real[d] = nullptr;
@@ -354,10 +366,6 @@ int mcdp_batch_stats_host(mcdp_batch *b, const int32_t *const *seeds, const int6
     }
     (void)hipStreamSynchronize(s);
     (void)hipStreamDestroy(s);
-    for (hipStream_t x : ss) {
-        (void)hipStreamSynchronize(x);
-        (void)hipStreamDestroy(x);
-    }
     return rc;
 }
 

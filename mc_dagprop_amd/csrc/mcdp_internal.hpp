@@ -216,6 +216,13 @@ int exact_stats_pass(mcdp_propagator &P, const double *realized, int32_t m, int3
 int exact_stats_round_end(mcdp_propagator &P, const ExactWork &w, hipStream_t s);
 int exact_stats_finish(mcdp_propagator &P, int64_t n, int32_t n_q, const double *q, const ChunkStat *st,
                        const ExactWork &w, double *quantiles, hipStream_t s);
+// Statistics of several DAGs in one launch per kernel (RowSet, mcdp_stats.hpp):
+// one DAG's set (rows realized / cause with ld = n, its global rows from
+// row0, the outputs of mcdp_event_stats_device; counts null: none), then the
+// launch over sets already in device memory; asynchronous on s.
+int row_set(mcdp_propagator &P, const double *realized, const int32_t *cause, int64_t n, int32_t n_q, const double *q,
+            double *mean, double *quantiles, int64_t *counts, int64_t row0, RowSet &rs);
+int set_stats(const RowSet *sets_dev, int32_t n_sets, int64_t rows, bool quantiles, bool counts, hipStream_t s);
 // Rows kept whole across a request's chunks (mcdp_stats.hpp, WatchWork): c[0..n)
 // = 0; the rows chosen after the first chunk (ranges: [E][2] event bounds);
 // one chunk's samples c0 .. c0 + m appended; and the quantiles of the first

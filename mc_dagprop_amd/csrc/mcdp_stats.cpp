@@ -223,6 +223,48 @@ int mcdp::exact_stats_finish(mcdp_propagator &P, int64_t n, int32_t n_q, const d
     return MCDP_OK;
 }
 
+int mcdp::row_set(mcdp_propagator &P, const double *realized, const int32_t *cause, int64_t n, int32_t n_q,
+                  const double *q, double *mean, double *quantiles, int64_t *counts, int64_t row0, RowSet &rs) {
+    if (n < 1 || n > std::numeric_limits<int32_t>::max())
+        return fail(MCDP_E_INVALID, "statistics need between 1 and 2^31 - 1 samples");
+    if (row0 < 0 || row0 + P.E > std::numeric_limits<int32_t>::max())
+        return fail(MCDP_E_INVALID, "too many events for one statistics launch");
+    if (!realized || !mean || (n_q > 0 && !quantiles) || (counts && !cause))
+        return fail(MCDP_E_INVALID, "null argument");
+    rs = RowSet{};
+    int rc = quantile_args(n, n_q, q, rs.qa);
+    if (rc) return rc;
+    rs.realized = realized;
+    rs.cause = cause;
+    rs.mean = mean;
+    rs.quant = quantiles;
+    rs.counts = counts;
+    rs.n = (int32_t)n;
+    rs.ld = (uint32_t)n;
+    rs.E = P.E;
+    rs.row0 = (int32_t)row0;
+    if (counts) {
+        std::lock_guard<std::mutex> lock(P.mu);
+        DeviceCopy *d = nullptr;
+        rc = ensure_device(P, d);
+        if (rc) return rc;
+        rc = ensure_topo(P, *d);
+        if (rc) return rc;
+        rs.pred_off = d->pred_off;
+        rs.pred_key = d->pred_key;
+        rs.pred_pos = d->pred_pos;
+        rs.pred_nkey = d->pred_nkey;
+    }
+    return MCDP_OK;
+}
+
+int mcdp::set_stats(const RowSet *sets_dev, int32_t n_sets, int64_t rows, bool quantiles, bool counts,
+                    hipStream_t s) {
+    if (rows > std::numeric_limits<int32_t>::max()) return fail(MCDP_E_INVALID, "too many rows for one launch");
+    HIP_TRY(launch_set_stats(sets_dev, n_sets, (int32_t)rows, quantiles, counts, s), "statistics kernels");
+    return MCDP_OK;
+}
+
 int mcdp::prepare_stats(mcdp_propagator &P) {
     std::lock_guard<std::mutex> lock(P.mu);
     DeviceCopy *d = nullptr;

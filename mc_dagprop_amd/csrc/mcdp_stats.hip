@@ -78,12 +78,11 @@ __device__ __forceinline__ double key_value(uint64_t k) {
     return __longlong_as_double((long long)u);
 }
 
-__global__ void __launch_bounds__(kBlock) mean_kernel(const double *__restrict__ realized, int32_t n, uint32_t ld,
-                                                      int32_t E, double *__restrict__ mean) {
-    __shared__ DD part[kWaves];
-    __shared__ double naive[kWaves];
-    for (int32_t e = blockIdx.x; e < E; e += gridDim.x) {
-        const double *row = realized + (uint64_t)(uint32_t)e * ld;
+// The mean of one row by a block of kBlock threads (part, naive: LDS of
+// kWaves entries each).
+__device__ __forceinline__ void mean_row(const double *__restrict__ row, int32_t n, double *__restrict__ mean,
+                                         DD *part, double *naive) {
+    {
         DD acc{0.0, 0.0};
         double plain = 0.0;
         // four loads per lane in flight; samples tid, tid + kBlock, ... in order
@@ -111,10 +110,18 @@ __global__ void __launch_bounds__(kBlock) mean_kernel(const double *__restrict__
             // carries no rounding to correct
             const DD sum = dd_add(dd_add(part[0], part[1]), dd_add(part[2], part[3]));
             const double nv = (naive[0] + naive[1]) + (naive[2] + naive[3]);
-            mean[e] = (__builtin_isfinite(nv) ? sum.hi : nv) / (double)n;
+            *mean = (__builtin_isfinite(nv) ? sum.hi : nv) / (double)n;
         }
         __syncthreads();
     }
+}
+
+__global__ void __launch_bounds__(kBlock) mean_kernel(const double *__restrict__ realized, int32_t n, uint32_t ld,
+                                                      int32_t E, double *__restrict__ mean) {
+    __shared__ DD part[kWaves];
+    __shared__ double naive[kWaves];
+    for (int32_t e = blockIdx.x; e < E; e += gridDim.x)
+        mean_row(realized + (uint64_t)(uint32_t)e * ld, n, mean + e, part, naive);
 }
 
 // numpy.quantile(row, q, method="linear") for each q, given the row's order
@@ -332,32 +339,51 @@ __device__ void value_bin_select(const double *__restrict__ row, int32_t n, cons
     }
 }
 
-// Quantiles of each row; with kMean, its first pass also sums the row (the
-// mean without a separate read).
+// LDS of the quantile kernels' work on one row.
+struct QuantileLds {
+    uint32_t hist[kMaxRanks][256];
+    Extremes ext[kQWaves];
+    DD part[kQWaves];
+    double naive[kQWaves];
+    uint64_t prefix[kMaxRanks];   // per rank: its key's digits found so far
+    int32_t krem[kMaxRanks];      // per rank: its rank among keys with that prefix
+    int32_t done[kMaxRanks];      // per rank: key known
+    uint64_t gprefix[kMaxRanks];  // ranks whose prefixes agree share one histogram
+    int32_t group_of[kMaxRanks];
+    int32_t n_groups;
+    int32_t first_pass;
+    uint32_t seg[kQBlock];        // value bins: counts per run of kValueBins / kQBlock bins
+    int32_t vbin[kMaxRanks];      // per rank: its value bin, rank inside it, the bin's count
+    int32_t vloc[kMaxRanks];
+    int32_t vcnt[kMaxRanks];
+    uint32_t cand_n;
+};
+
+// The quantiles of one row by a block of kQBlock threads, out[0 .. n_q); with
+// kMean, its first pass also sums the row into *mean (the mean without a
+// separate read).
 template <bool kMean>
-__global__ void __launch_bounds__(kQBlock) quantile_kernel(const double *__restrict__ realized, int32_t n, uint32_t ld,
-                                                          int32_t E, QuantileArgs qa, double *__restrict__ out,
-                                                          double *__restrict__ mean) {
-    __shared__ uint32_t hist[kMaxRanks][256];
-    __shared__ Extremes ext[kQWaves];
-    __shared__ DD part[kQWaves];
-    __shared__ double naive[kQWaves];
-    __shared__ uint64_t prefix[kMaxRanks];  // per rank: its key's digits found so far
-    __shared__ int32_t krem[kMaxRanks];     // per rank: its rank among keys with that prefix
-    __shared__ int32_t done[kMaxRanks];     // per rank: key known
-    __shared__ uint64_t gprefix[kMaxRanks]; // ranks whose prefixes agree share one histogram
-    __shared__ int32_t group_of[kMaxRanks];
-    __shared__ int32_t n_groups;
-    __shared__ int32_t first_pass;
-    __shared__ uint32_t seg[kQBlock];       // value bins: counts per run of kValueBins / kQBlock bins
-    __shared__ int32_t vbin[kMaxRanks];    // per rank: its value bin, rank inside it, the bin's count
-    __shared__ int32_t vloc[kMaxRanks];
-    __shared__ int32_t vcnt[kMaxRanks];
-    __shared__ uint32_t cand_n;
+__device__ __forceinline__ void quantile_row(const double *__restrict__ row, int32_t n, const QuantileArgs &qa,
+                                             double *__restrict__ out, double *__restrict__ mean, QuantileLds &L) {
+    auto &hist = L.hist;
+    auto &ext = L.ext;
+    auto &part = L.part;
+    auto &naive = L.naive;
+    auto &prefix = L.prefix;
+    auto &krem = L.krem;
+    auto &done = L.done;
+    auto &gprefix = L.gprefix;
+    auto &group_of = L.group_of;
+    auto &n_groups = L.n_groups;
+    auto &first_pass = L.first_pass;
+    auto &seg = L.seg;
+    auto &vbin = L.vbin;
+    auto &vloc = L.vloc;
+    auto &vcnt = L.vcnt;
+    auto &cand_n = L.cand_n;
     const int R = qa.n_ranks;
     const int lane = threadIdx.x & 63;
-    for (int32_t e = blockIdx.x; e < E; e += gridDim.x) {
-        const double *row = realized + (uint64_t)(uint32_t)e * ld;
+    {
         // Pass 0: extremes.  Ranks inside the mass of the smallest or the
         // largest value (a row saturated at its cap puts its high quantiles
         // there) are known at once; the others need only the key bytes in
@@ -389,7 +415,7 @@ __global__ void __launch_bounds__(kQBlock) quantile_kernel(const double *__restr
         if (kMean && threadIdx.x == 0) {
             DD sum = part[0];
             for (int w = 1; w < kQWaves; ++w) sum = dd_add(sum, part[w]);
-            mean[e] = (__builtin_isfinite(nv) ? sum.hi : nv) / (double)n;
+            *mean = (__builtin_isfinite(nv) ? sum.hi : nv) / (double)n;
         }
         // A NaN sample makes the plain sum NaN, and so does +inf + -inf: only
         // then is the row read again for NaNs (nv is the same in every thread).
@@ -495,11 +521,22 @@ __global__ void __launch_bounds__(kQBlock) quantile_kernel(const double *__restr
                     const double b = key_value(prefix[qa.hi[j]]);
                     res = lerp_numpy(a, b, qa.gamma[j]);
                 }
-                out[(uint64_t)(uint32_t)e * (uint32_t)qa.n_q + j] = res;
+                out[j] = res;
             }
         }
-        __syncthreads();  // prefix[] and ext[] are read above before the next event resets them
+        __syncthreads();  // prefix[] and ext[] are read above before the next row resets them
     }
+}
+
+// Quantiles of each row; with kMean, also the row means.
+template <bool kMean>
+__global__ void __launch_bounds__(kQBlock) quantile_kernel(const double *__restrict__ realized, int32_t n, uint32_t ld,
+                                                          int32_t E, QuantileArgs qa, double *__restrict__ out,
+                                                          double *__restrict__ mean) {
+    __shared__ QuantileLds L;
+    for (int32_t e = blockIdx.x; e < E; e += gridDim.x)
+        quantile_row<kMean>(realized + (uint64_t)(uint32_t)e * ld, n, qa, out + (uint64_t)(uint32_t)e * (uint32_t)qa.n_q,
+                            kMean ? mean + e : nullptr, L);
 }
 
 // Events with at most this many distinct sources count their row in registers.
@@ -577,15 +614,22 @@ __device__ __forceinline__ int32_t find_source(const int32_t *__restrict__ keys,
     return (lo < nk && keys[lo] == c) ? lo : -1;
 }
 
-__global__ void __launch_bounds__(kBlock) cause_kernel(const int32_t *__restrict__ cause, int32_t n, uint32_t ld,
-                                                       int32_t E, const int32_t *__restrict__ pred_off,
-                                                       const int32_t *__restrict__ pred_key,
-                                                       const int32_t *__restrict__ pred_pos,
-                                                       const int32_t *__restrict__ pred_nkey,
-                                                       int64_t *__restrict__ counts, int accumulate) {
-    __shared__ uint32_t c_lds[kCauseLdsCounters];
-    __shared__ int32_t k_lds[kCauseLdsKeys], p_lds[kCauseLdsKeys];
-    for (int32_t e = blockIdx.x; e < E; e += gridDim.x) {
+// LDS of the cause kernels' work on one row.
+struct CauseLds {
+    uint32_t c[kCauseLdsCounters];
+    int32_t k[kCauseLdsKeys], p[kCauseLdsKeys];
+};
+
+// The cause counts of event e's row (n samples) by a block of kBlock threads.
+__device__ __forceinline__ void cause_row(const int32_t *__restrict__ row, int32_t n, int32_t e,
+                                          const int32_t *__restrict__ pred_off, const int32_t *__restrict__ pred_key,
+                                          const int32_t *__restrict__ pred_pos,
+                                          const int32_t *__restrict__ pred_nkey, int64_t *__restrict__ counts,
+                                          int accumulate, CauseLds &L) {
+    auto &c_lds = L.c;
+    auto &k_lds = L.k;
+    auto &p_lds = L.p;
+    {
         const int32_t p0 = pred_off[e], np = pred_off[e + 1] - p0, nk = pred_nkey[e];
         // the event's distinct sources and positions, in LDS when they fit
         const bool keys_in_lds = nk <= kCauseLdsKeys;
@@ -603,7 +647,6 @@ __global__ void __launch_bounds__(kBlock) cause_kernel(const int32_t *__restrict
             for (int i = threadIdx.x; i <= np; i += kBlock) out[i] = 0;
         }
         __syncthreads();
-        const int32_t *row = cause + (uint64_t)(uint32_t)e * ld;
         if (in_lds && nk <= kCauseRegKeys) {
             const int32_t *k0 = pred_key + p0, *q0 = pred_pos + p0;
             switch (nk) {
@@ -620,7 +663,7 @@ __global__ void __launch_bounds__(kBlock) cause_kernel(const int32_t *__restrict
             __syncthreads();
             for (int i = threadIdx.x; i <= np; i += kBlock) out[i] = (accumulate ? out[i] : 0) + (int64_t)c_lds[i];
             __syncthreads();
-            continue;
+            return;
         }
         auto count = [&](int32_t c) {
             int32_t slot = -1;
@@ -664,6 +707,66 @@ __global__ void __launch_bounds__(kBlock) cause_kernel(const int32_t *__restrict
         if (in_lds)
             for (int i = threadIdx.x; i <= np; i += kBlock) out[i] = (accumulate ? out[i] : 0) + (int64_t)c_lds[i];
         __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) cause_kernel(const int32_t *__restrict__ cause, int32_t n, uint32_t ld,
+                                                       int32_t E, const int32_t *__restrict__ pred_off,
+                                                       const int32_t *__restrict__ pred_key,
+                                                       const int32_t *__restrict__ pred_pos,
+                                                       const int32_t *__restrict__ pred_nkey,
+                                                       int64_t *__restrict__ counts, int accumulate) {
+    __shared__ CauseLds L;
+    for (int32_t e = blockIdx.x; e < E; e += gridDim.x)
+        cause_row(cause + (uint64_t)(uint32_t)e * ld, n, e, pred_off, pred_key, pred_pos, pred_nkey, counts,
+                  accumulate, L);
+}
+
+// ---- several DAGs' statistics in one launch ----------------------------------
+// The set holding global row g: the last with row0 <= g (sets in row order).
+__device__ __forceinline__ int32_t find_set(const RowSet *__restrict__ sets, int32_t n_sets, int32_t g) {
+    int32_t lo = 0, hi = n_sets - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (sets[mid].row0 <= g)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(kQBlock) quantile_sets_kernel(const RowSet *__restrict__ sets, int32_t n_sets,
+                                                                int32_t rows) {
+    __shared__ QuantileLds L;
+    for (int32_t g = blockIdx.x; g < rows; g += gridDim.x) {
+        const RowSet &S = sets[find_set(sets, n_sets, g)];
+        const int32_t e = g - S.row0;
+        quantile_row<true>(S.realized + (uint64_t)(uint32_t)e * S.ld, S.n, S.qa,
+                           S.quant + (uint64_t)(uint32_t)e * (uint32_t)S.qa.n_q, S.mean + e, L);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) mean_sets_kernel(const RowSet *__restrict__ sets, int32_t n_sets,
+                                                           int32_t rows) {
+    __shared__ DD part[kWaves];
+    __shared__ double naive[kWaves];
+    for (int32_t g = blockIdx.x; g < rows; g += gridDim.x) {
+        const RowSet &S = sets[find_set(sets, n_sets, g)];
+        const int32_t e = g - S.row0;
+        mean_row(S.realized + (uint64_t)(uint32_t)e * S.ld, S.n, S.mean + e, part, naive);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) cause_sets_kernel(const RowSet *__restrict__ sets, int32_t n_sets,
+                                                            int32_t rows) {
+    __shared__ CauseLds L;
+    for (int32_t g = blockIdx.x; g < rows; g += gridDim.x) {
+        const RowSet &S = sets[find_set(sets, n_sets, g)];
+        if (!S.counts) continue;  // uniform over the block
+        const int32_t e = g - S.row0;
+        cause_row(S.cause + (uint64_t)(uint32_t)e * S.ld, S.n, e, S.pred_off, S.pred_key, S.pred_pos, S.pred_nkey,
+                  S.counts, 0, L);
     }
 }
 
@@ -1250,6 +1353,17 @@ hipError_t launch_exact_finish(int32_t E, const QuantileArgs &qa, const ChunkSta
     if (E > 0)
         hipLaunchKernelGGL(exact_finish_kernel, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, stream, E, qa, st, w,
                            out);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_stats(const RowSet *sets, int32_t n_sets, int32_t rows, bool quantiles, bool counts,
+                            hipStream_t stream) {
+    if (rows <= 0 || n_sets <= 0) return hipSuccess;
+    if (quantiles)
+        hipLaunchKernelGGL(quantile_sets_kernel, dim3((unsigned)rows), dim3(kQBlock), 0, stream, sets, n_sets, rows);
+    else
+        hipLaunchKernelGGL(mean_sets_kernel, dim3((unsigned)rows), dim3(kBlock), 0, stream, sets, n_sets, rows);
+    if (counts) hipLaunchKernelGGL(cause_sets_kernel, dim3((unsigned)rows), dim3(kBlock), 0, stream, sets, n_sets, rows);
     return hipGetLastError();
 }
 

@@ -30,6 +30,29 @@ hipError_t launch_event_mean(const double *realized, int32_t n, int32_t ld, int3
 // `mean` (nullable): the row means, computed in the quantile kernel's first pass.
 hipError_t launch_event_quantiles(const double *realized, int32_t n, int32_t ld, int32_t E, const QuantileArgs &qa,
                                   double *out, double *mean, hipStream_t stream);
+// One DAG's rows for statistics of several DAGs in one launch: rows
+// realized[e * ld + s] and cause likewise (s < n, e < E), global rows
+// row0 .. row0 + E - 1 (the sets in row order), outputs mean [E], quant
+// [E][qa.n_q] and, when counts is not null, the cause counts (the layout of
+// launch_cause_counts, from the DAG's predecessor table).
+struct RowSet {
+    const double *realized;
+    const int32_t *cause;
+    double *mean;
+    double *quant;
+    int64_t *counts;
+    const int32_t *pred_off, *pred_key, *pred_pos, *pred_nkey;
+    int32_t n;
+    uint32_t ld;
+    int32_t E;
+    int32_t row0;
+    QuantileArgs qa;
+};
+// The statistics of launch_event_quantiles (quantiles: with the means) or
+// launch_event_mean, and of launch_cause_counts (counts), for `rows` rows of
+// n_sets sets in device memory; each row's results equal those kernels'.
+hipError_t launch_set_stats(const RowSet *sets, int32_t n_sets, int32_t rows, bool quantiles, bool counts,
+                            hipStream_t stream);
 // accumulate != 0 adds to the counts already in `counts`.
 hipError_t launch_cause_counts(const int32_t *cause, int32_t n, int32_t ld, int32_t E, const int32_t *pred_off,
                                const int32_t *pred_key, const int32_t *pred_pos, const int32_t *pred_nkey,
