@@ -1222,10 +1222,11 @@ __global__ void __launch_bounds__(256) watch_select_kernel(int64_t n, int32_t m,
     };
     const double k_lo = (f_lo - widen(f_lo)) * (double)n;
     const double k_hi = (1.0 - f_hi + widen(f_hi)) * (double)n;
+    // ranks 0 and n - 1 always lie on the masses (each holds a sample at least)
     bool open = false;
     for (int r = 0; r < qa.n_ranks; ++r) {
         const double k = (double)qa.rank[r];
-        open = open || (k >= k_lo && k < k_hi);
+        open = open || (k > 0.0 && k < (double)(n - 1) && k >= k_lo && k < k_hi);
     }
     if (!open) return;
     const uint64_t i = atomicAdd((unsigned long long *)w.count, 1ull);

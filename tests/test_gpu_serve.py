@@ -321,6 +321,7 @@ def test_chunked_extremes_and_no_quantiles(gpu) -> None:
         same_stats(got, p.run_stats(seeds, q))
     st = srv.stats()
     assert st["chunked"] == 2 and st["exact_rounds"] == 0 and st["failed"] == 0
+    assert st["kept_rows"] == 0  # ranks 0 and n - 1 always lie on the extremes' masses: no row is kept for them
     srv.close()
 
 
