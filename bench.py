@@ -194,7 +194,8 @@ def bench_serve(args) -> tuple[int, dict]:
     budget = int(args.budget_gb * (1 << 30))
     cap = budget // 4  # one request's share of the pool; larger requests run in chunks of samples
     exact = not getattr(args, "serve_approx", False)
-    srv = capi.Server(budget, max_running=8, n_streams=4, request_cap=cap, exact_chunked_quantiles=exact)
+    srv = capi.Server(budget, max_running=args.serve_running, n_streams=args.serve_streams, request_cap=cap,
+                      exact_chunked_quantiles=exact)
     ids = [srv.add_context(p) for p in props]
     log(f"serve: {n_ctx} contexts ({sum(p.E for p in props)} events) compiled and resident in "
         f"{time.perf_counter() - t_build:.1f} s; pool {args.budget_gb:.0f} GiB, requests above "
@@ -276,6 +277,7 @@ def bench_serve(args) -> tuple[int, dict]:
                                                       "further passes over the chunks)" if exact
                                                       else "quantiles within max_delay/4096: no further passes)"),
                    "contexts": n_ctx, "requests": N, "rate_per_s": rate, "pool_gib": args.budget_gb,
+                   "max_running": args.serve_running, "streams": args.serve_streams,
                    "chunked_requests": st["chunked"]},
         "latency_ms": {"p50": float(p50), "p99": float(p99), "mean": float(lat.mean()), "max": float(lat.max())},
         "server": st,
@@ -472,6 +474,8 @@ def main() -> int:
     ap.add_argument("--contexts", type=int, default=200, help="serve: resident contexts")
     ap.add_argument("--requests", type=int, default=300, help="serve: requests in the stream")
     ap.add_argument("--rate", type=float, default=12.0, help="serve: Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-running", type=int, default=8, help="serve: requests on the device at most")
+    ap.add_argument("--serve-streams", type=int, default=4, help="serve: streams the server spreads requests over")
     ap.add_argument("--budget-gb", type=float, default=200.0,
                     help="serve: device memory pool; batch: device memory per group of DAGs (GiB)")
     ap.add_argument("--batch-dags", type=int, default=64, help="batch: number of DAGs")
