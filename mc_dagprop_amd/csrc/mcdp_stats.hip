@@ -863,8 +863,16 @@ __global__ void __launch_bounds__(kBlock) chunk_accumulate_kernel(const double *
             c.pad = 0;
             st[e] = c;
         }
+        // the first chunk writes every bin; later ones touch only the bins they
+        // hit (a row saturated at its cap hits one), not 16 KiB per row
         uint32_t *gh = hist + (uint64_t)(uint32_t)e * (uint32_t)n_bins;
-        for (int i = threadIdx.x; i < n_bins; i += kBlock) gh[i] = (first ? 0u : gh[i]) + h[i];
+        for (int i = threadIdx.x; i < n_bins; i += kBlock) {
+            const uint32_t c = h[i];
+            if (first)
+                gh[i] = c;
+            else if (c)
+                gh[i] += c;
+        }
         __syncthreads();
     }
 }
