@@ -432,13 +432,15 @@ int mcdp_debug_reverse_tiles(mcdp_propagator *prop, int64_t max_tiles);
  * the route; tests use it to run both. */
 int mcdp_debug_tail_mode(mcdp_propagator *prop, int32_t mode);
 /* The serving footprint rule, host only: for a request of n samples over a
- * context of n_events events and n_preds predecessor entries with n_q
- * quantiles, writes chunk = 0 and the request's device bytes when it fits
- * request_cap whole, else the samples per chunk and the chunked request's
- * bytes (<= request_cap); fails with MCDP_E_NOMEM when not even chunks of 128
- * samples fit, as mcdp_server_submit does. */
-int mcdp_debug_serve_chunk(int64_t n, int64_t n_events, int64_t n_preds, int32_t n_q, int64_t request_cap,
-                           int64_t *chunk, int64_t *bytes);
+ * context of n_events events and n_preds predecessor entries (at most
+ * max_entries for one event: up to 254, chunks store one-byte winner
+ * positions instead of four-byte cause events) with n_q quantiles, writes
+ * chunk = 0 and the request's device bytes when it fits request_cap whole,
+ * else the samples per chunk and the chunked request's bytes (<= request_cap);
+ * fails with MCDP_E_NOMEM when not even chunks of 128 samples fit, as
+ * mcdp_server_submit does. */
+int mcdp_debug_serve_chunk(int64_t n, int64_t n_events, int64_t n_preds, int32_t max_entries, int32_t n_q,
+                           int64_t request_cap, int64_t *chunk, int64_t *bytes);
 /* Lowers the exact selection's limits for later requests: values gathered per
  * region (1..4096), candidate space per target (1..256 values) and refine
  * slots per request (1..1024), so tests reach the refining passes and the

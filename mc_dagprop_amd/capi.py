@@ -98,7 +98,7 @@ _SIGNATURES = {
     "mcdp_gradient_host": (C.c_int, [_P, _pi32, _i64, _pf64, _i64, _pf64, _pf64, _pf64]),
     "mcdp_debug_reverse_tiles": (C.c_int, [_P, _i64]),
     "mcdp_debug_tail_mode": (C.c_int, [_P, C.c_int32]),
-    "mcdp_debug_serve_chunk": (C.c_int, [_i64, _i64, _i64, _i32, _i64, _pi64, _pi64]),
+    "mcdp_debug_serve_chunk": (C.c_int, [_i64, _i64, _i64, _i32, _i32, _i64, _pi64, _pi64]),
     "mcdp_propagator_gradient_info": (C.c_int, [_P, _pi64, C.c_int]),
     "mcdp_server_create": (C.c_int, [_i64, _i64, _i32, _i32, C.POINTER(_P)]),
     "mcdp_server_destroy": (None, [_P]),

@@ -809,9 +809,12 @@ int choose_shape(mcdp_propagator &P) {
     return build_schedule(P, best.waves, best.k);
 }
 
+// cause_pos (sampled walks, no cause): the winning entry's position per
+// sample instead of the cause event.
 int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double *dur_in, int64_t n, int64_t ld,
-           double *realized, double *durations, int32_t *cause, hipStream_t stream) {
+           double *realized, double *durations, int32_t *cause, hipStream_t stream, uint8_t *cause_pos = nullptr) {
     if (n < 0) return fail(MCDP_E_INVALID, "n must be non-negative");
+    if (cause_pos && (injected || cause)) return fail(MCDP_E_INVALID, "internal error: positions replace sampled causes");
     if (n > 0 && ld < n) return fail(MCDP_E_INVALID, "ld must be >= n");
     if (n > kMaxSamplesPerLaunch || ld > std::numeric_limits<int32_t>::max())
         return fail(MCDP_E_INVALID, "n and ld must be below 2^31 - 256 per call");
@@ -838,7 +841,9 @@ int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double
     p.realized = realized;
     p.durations = injected ? nullptr : durations;
     p.cause = cause;
-    const WalkVariant v = walk_variant(P, injected);
+    p.cause_pos = cause_pos;
+    WalkVariant v = walk_variant(P, injected);
+    v.pos = cause_pos != nullptr;
     const size_t lds = walk_lds_bytes(P, v);
     int cus = 0, bpc = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device), "hipDeviceGetAttribute");
@@ -916,6 +921,7 @@ int launch(mcdp_propagator &P, bool injected, const int32_t *seeds, const double
         q.realized = realized + n_main;
         q.durations = p.durations ? p.durations + n_main : nullptr;
         q.cause = cause ? cause + n_main : nullptr;
+        q.cause_pos = cause_pos ? cause_pos + n_main : nullptr;
         q.n = (int32_t)n_tail;
         q.n_tiles = (int32_t)((n_tail + kLanes - 1) / kLanes);
         if (wide) {
@@ -1076,6 +1082,21 @@ int mcdp::configure_shape(mcdp_propagator *prop, int32_t waves, int32_t k, int32
     return mcdp_propagator_configure(prop, waves, k, spl);
 }
 mcdp::Record mcdp::link_record(const mcdp_propagator &P, int32_t l) { return edge_record(P, l); }
+
+int32_t mcdp::max_entries(const mcdp_propagator &P) {
+    int64_t m = 0;
+    for (int32_t e = 0; e < P.E; ++e) m = std::max(m, P.pred_off[(size_t)e + 1] - P.pred_off[(size_t)e]);
+    return (int32_t)std::min<int64_t>(m, std::numeric_limits<int32_t>::max());
+}
+
+int mcdp::run_many_positions(mcdp_propagator &P, const int32_t *seeds_dev, int64_t n, int64_t ld, double *realized,
+                             uint8_t *cause_pos, hipStream_t s) {
+    if (!cause_pos) return fail(MCDP_E_INVALID, "positions output is null");
+    if (max_entries(P) > kMaxPositionEntries)
+        return fail(MCDP_E_INVALID, "winner positions need events of at most 254 predecessor entries");
+    if (n > 0 && !seeds_dev) return fail(MCDP_E_INVALID, "seeds is null");
+    return launch(P, false, seeds_dev, nullptr, n, ld, realized, nullptr, nullptr, s, cause_pos);
+}
 
 // The range of every duration the sampler can draw for link l, as the device
 // rounds it (mcdp_sampler.hpp: base + extra, extra = X * d, one rounding

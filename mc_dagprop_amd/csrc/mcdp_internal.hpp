@@ -46,6 +46,7 @@ struct DeviceCopy {
     const int32_t *pred_key = nullptr;   // [P] per event: its distinct sources, ascending
     const int32_t *pred_pos = nullptr;   // [P] first position of pred_key in the event's list
     const int32_t *pred_nkey = nullptr;  // [E] distinct sources per event
+    const int32_t *pred_first = nullptr; // [P] per entry: the first position of its source in the event's list
     void *rev = nullptr;                 // gradients: reverse plan, uploaded on first use
     RevParams rev_base{};
     const int32_t *rev_map_off = nullptr;  // [L + E + 2] columns of each output
@@ -197,11 +198,12 @@ int configure_shape(mcdp_propagator *prop, int32_t waves, int32_t k, int32_t spl
 // device (what a resident context needs before serving requests).
 int prepare_stats(mcdp_propagator &P);
 // Statistics of a run in sample chunks (serving): adds one chunk's rows
-// (realized[e * ld + s], cause likewise, s < m) to the running statistics and
-// cause counts (first: start them), then the means and quantiles of all n
-// samples; asynchronous on s.
-int chunk_stats_accumulate(mcdp_propagator &P, const double *realized, const int32_t *cause, int32_t m, int32_t ld,
-                           bool first, ChunkStat *st, uint32_t *hist, int64_t *counts, hipStream_t s);
+// (realized[e * ld + s], cause likewise, or cause_pos when not null: winner
+// positions; s < m) to the running statistics and cause counts (first: start
+// them), then the means and quantiles of all n samples; asynchronous on s.
+int chunk_stats_accumulate(mcdp_propagator &P, const double *realized, const int32_t *cause, const uint8_t *cause_pos,
+                           int32_t m, int32_t ld, bool first, ChunkStat *st, uint32_t *hist, int64_t *counts,
+                           hipStream_t s);
 int chunk_stats_finish(mcdp_propagator &P, int64_t n, int32_t n_q, const double *q, const ChunkStat *st,
                        const uint32_t *hist, double *mean, double *quantiles, hipStream_t s);
 // Exact quantiles of a run in sample chunks (mcdp_stats.hpp, ExactWork): the
@@ -237,6 +239,14 @@ int watch_finish(int64_t n, int32_t n_q, const double *q, const WatchWork &w, in
 // The forward plan's record of link l (kind, flags, distribution index,
 // minimal duration, p0); an edge outside the link table is kind kNone.
 Record link_record(const mcdp_propagator &P, int32_t l);
+// run_many with each event's winning entry position (one byte per sample,
+// 0xFF: none) in place of its cause event, for events of at most
+// kMaxPositionEntries entries (max_entries: the most any event has);
+// asynchronous on s.
+constexpr int32_t kMaxPositionEntries = 254;
+int32_t max_entries(const mcdp_propagator &P);
+int run_many_positions(mcdp_propagator &P, const int32_t *seeds_dev, int64_t n, int64_t ld, double *realized,
+                       uint8_t *cause_pos, hipStream_t s);
 // Bounds that hold for every seed: the durations link l can draw, and every
 // event's realized time (false when an earliest time or max_delay is not
 // finite).

@@ -101,7 +101,8 @@ struct SlotRow {
     int32_t v[K];
 };
 
-template <bool kInjected, int kPaths, int K, int kSpl>
+// kPos: the cause output is cause_pos (the winning entry's position), not cause.
+template <bool kInjected, int kPaths, int K, int kSpl, bool kPos = false>
 __device__ __forceinline__ void walk_tile(const WalkParams &p, int32_t tile, double *lds, const LogTable &lt) {
     const int lane = threadIdx.x & (kLanes - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -149,6 +150,7 @@ __device__ __forceinline__ void walk_tile(const WalkParams &p, int32_t tile, dou
             Record cur = recs[r0];
             double best[kSpl], upper = 0.0;
             int32_t who[kSpl], event = 0;
+            int hdr = r0;  // kPos: the current event's header record (entry k is record hdr + 1 + k)
 #pragma unroll
             for (int i = 0; i < kSpl; ++i) {
                 best[i] = 0.0;
@@ -163,6 +165,7 @@ __device__ __forceinline__ void walk_tile(const WalkParams &p, int32_t tile, dou
                     // select against it without rebuilding it from SGPRs
                     asm volatile("" : "+v"(upper));
                     event = cur.i0;
+                    if (kPos) hdr = r;
 #pragma unroll
                     for (int i = 0; i < kSpl; ++i) {
                         best[i] = cur.a;
@@ -202,13 +205,15 @@ __device__ __forceinline__ void walk_tile(const WalkParams &p, int32_t tile, dou
 #pragma unroll
                         for (int i = 0; i < kSpl; ++i) rs[i] = e;
                     }
+                    // the cause: the source event, or the entry's position
+                    const int32_t tag = kPos ? r - hdr - 1 : cur.i0;
 #pragma unroll
                     for (int i = 0; i < kSpl; ++i) {
                         double tt = rs[i] + dur[i];
                         tt = (upper < tt) ? upper : tt;
                         if (tt >= best[i]) {
                             best[i] = tt;
-                            who[i] = cur.i0;
+                            who[i] = tag;
                         }
                     }
                 }
@@ -218,7 +223,10 @@ __device__ __forceinline__ void walk_tile(const WalkParams &p, int32_t tile, dou
                     for (int i = 0; i < kSpl; ++i) {
                         const double rr = (upper < best[i]) ? upper : best[i];
                         __builtin_nontemporal_store(rr, p.realized + row + s[i]);
-                        if (p.cause) __builtin_nontemporal_store(who[i], p.cause + row + s[i]);
+                        if (kPos)
+                            __builtin_nontemporal_store((uint8_t)who[i], p.cause_pos + row + s[i]);
+                        else if (p.cause)
+                            __builtin_nontemporal_store(who[i], p.cause + row + s[i]);
                         res[j][i] = rr;
                     }
                     ++j;
@@ -254,14 +262,14 @@ __device__ __forceinline__ void walk_tile(const WalkParams &p, int32_t tile, dou
     }
 }
 
-template <bool kInjected, int kPaths, int K, int kSpl>
+template <bool kInjected, int kPaths, int K, int kSpl, bool kPos = false>
 __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) {
     extern __shared__ double lds[];
     __shared__ LogTableLds log_lds;
     if (!kInjected) stage_log_table(log_lds);
     stage_alias<kInjected, kPaths, kSpl>(p, lds);  // its barrier also publishes the log table
     for (int32_t tile = blockIdx.x; tile < p.n_tiles; tile += gridDim.x)
-        walk_tile<kInjected, kPaths, K, kSpl>(p, tile, lds, log_lds.view());
+        walk_tile<kInjected, kPaths, K, kSpl, kPos>(p, tile, lds, log_lds.view());
 }
 
 // Several plans in one launch: workgroups take (plan, tile) items from a
@@ -353,14 +361,17 @@ __global__ void __launch_bounds__(kLanes) walk_wide_kernel(WalkParams p, int32_t
             tt = (upper < tt) ? upper : tt;
             if (tt >= best) {
                 best = tt;
-                who = c[u].i0;
+                who = p.cause_pos ? q0 + u : c[u].i0;  // the entry's position, or the source event
             }
         }
     }
     // realized is read by later steps: an ordinary store; cause is not
     const uint64_t row = row_offset(h.i0, ld);
     p.realized[row + s] = (upper < best) ? upper : best;
-    if (p.cause) __builtin_nontemporal_store(who, p.cause + row + s);
+    if (p.cause_pos)
+        __builtin_nontemporal_store((uint8_t)who, p.cause_pos + row + s);
+    else if (p.cause)
+        __builtin_nontemporal_store(who, p.cause + row + s);
 }
 
 // Durations of the activities no predecessor entry references, for the wide
@@ -420,28 +431,35 @@ hipError_t launch_copy(double *dst, const double *src, int64_t n, hipStream_t st
     return hipGetLastError();
 }
 
-template <bool kInjected, int kPaths, int kSpl>
+template <bool kInjected, int kPaths, int kSpl, bool kPos>
 const void *walk_symbol(int k) {
     switch (k) {
-        case 4: return (const void *)walk_kernel<kInjected, kPaths, 4, kSpl>;
-        case 8: return (const void *)walk_kernel<kInjected, kPaths, 8, kSpl>;
-        case 12: return kSpl == 1 ? (const void *)walk_kernel<kInjected, kPaths, 12, 1> : nullptr;
-        case 16: return kSpl == 1 ? (const void *)walk_kernel<kInjected, kPaths, 16, 1> : nullptr;
+        case 4: return (const void *)walk_kernel<kInjected, kPaths, 4, kSpl, kPos>;
+        case 8: return (const void *)walk_kernel<kInjected, kPaths, 8, kSpl, kPos>;
+        case 12: return kSpl == 1 ? (const void *)walk_kernel<kInjected, kPaths, 12, 1, kPos> : nullptr;
+        case 16: return kSpl == 1 ? (const void *)walk_kernel<kInjected, kPaths, 16, 1, kPos> : nullptr;
         default: return nullptr;
     }
 }
 
-template <bool kInjected, int kPaths>
+template <bool kInjected, int kPaths, bool kPos>
 const void *walk_symbol(int k, int spl) {
-    return spl == 2 ? walk_symbol<kInjected, kPaths, 2>(k) : spl == 1 ? walk_symbol<kInjected, kPaths, 1>(k) : nullptr;
+    return spl == 2   ? walk_symbol<kInjected, kPaths, 2, kPos>(k)
+           : spl == 1 ? walk_symbol<kInjected, kPaths, 1, kPos>(k)
+                      : nullptr;
 }
 
+// Winner positions (v.pos) only for sampled walks.
 const void *walk_symbol(WalkVariant v) {
     if (v.injected)
-        return v.paths == kPathsFast ? walk_symbol<true, kPathsFast>(v.k, v.spl)
-                                     : walk_symbol<true, kPathsGeneral>(v.k, v.spl);
-    return v.paths == kPathsFast ? walk_symbol<false, kPathsFast>(v.k, v.spl)
-                                 : walk_symbol<false, kPathsGeneral>(v.k, v.spl);
+        return v.pos ? nullptr
+               : v.paths == kPathsFast ? walk_symbol<true, kPathsFast, false>(v.k, v.spl)
+                                       : walk_symbol<true, kPathsGeneral, false>(v.k, v.spl);
+    if (v.pos)
+        return v.paths == kPathsFast ? walk_symbol<false, kPathsFast, true>(v.k, v.spl)
+                                     : walk_symbol<false, kPathsGeneral, true>(v.k, v.spl);
+    return v.paths == kPathsFast ? walk_symbol<false, kPathsFast, false>(v.k, v.spl)
+                                 : walk_symbol<false, kPathsGeneral, false>(v.k, v.spl);
 }
 
 template <bool kInjected, int kPaths, int kSpl>

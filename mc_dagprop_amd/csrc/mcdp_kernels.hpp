@@ -58,6 +58,8 @@ struct WalkParams {
     double *realized;       // [E][ld] (read within the launch when a plan spills slots)
     double *durations;      // [L][ld] or null (sampled mode)
     int32_t *cause;         // [E][ld] or null
+    uint8_t *cause_pos;     // [E][ld] or null: the winning entry's position in the event's list (0xFF: none),
+                            // instead of cause, for events of at most 254 entries
 };
 
 // A batch: several plans' WalkParams and a queue of (plan, tile) items.
@@ -74,6 +76,7 @@ struct WalkVariant {
     int paths;      // WalkPaths
     int k;          // one of kEventsPerWaveChoices
     int spl;        // samples per lane: one of kSplChoices (spl 2: k <= kMaxEventsPerWaveSpl2)
+    bool pos = false;  // sampled walks only: the cause output is WalkParams::cause_pos
 };
 
 hipError_t launch_walk(const WalkParams &p, WalkVariant v, int grid, size_t lds_bytes, hipStream_t stream);

@@ -156,13 +156,14 @@ struct Request {
     // device buffers (in the pool) and, for chunked requests with quantiles,
     // the exact selection: phase 1 while a pass (or its set-up) runs, whose
     // counters come back to h_status; phase 2 once the results are on the way
-    int32_t *d_seeds = nullptr, *d_cause = nullptr;
+    int32_t *d_seeds = nullptr, *d_cause = nullptr;  // d_cause holds winner positions when cause_bytes == 1
     double *d_real = nullptr, *d_mean = nullptr, *d_quant = nullptr;
     int64_t *d_cnt = nullptr;
     ChunkStat *d_st = nullptr;
     uint32_t *d_hist = nullptr;
     ExactWork exact{};
     int64_t kept_cap = 0;  // rows kept whole (0: none)
+    int cause_bytes = 4;   // chunked: 1 when the walks write winner positions
     WatchWork watch{};
     int phase = 0;
     int rounds = 0;
@@ -175,11 +176,12 @@ struct Request {
 };
 
 // Device bytes of a request: seeds, realized and cause rows for `rows`
-// samples at a time, statistics (and, run in chunks, their running state, the
-// exact selection's workspace and `kept` rows kept whole).
-size_t footprint(int64_t n, int64_t rows, int64_t E, int64_t P, int64_t n_q, bool chunked, int64_t kept) {
+// samples at a time (cb bytes per cause: 4 for cause events, 1 for winner
+// positions), statistics (and, run in chunks, their running state, the exact
+// selection's workspace and `kept` rows kept whole).
+size_t footprint(int64_t n, int64_t rows, int64_t E, int64_t P, int64_t n_q, bool chunked, int64_t kept, int cb) {
     size_t b = align_up((size_t)n * 4, 256) + align_up((size_t)(E * rows) * 8, 256) +
-               align_up((size_t)(E * rows) * 4, 256) + align_up((size_t)E * 8, 256) +
+               align_up((size_t)(E * rows) * (size_t)cb, 256) + align_up((size_t)E * 8, 256) +
                align_up((size_t)(E * n_q) * 8, 256) + align_up((size_t)(E + P) * 8, 256);
     if (chunked) b += align_up((size_t)E * sizeof(ChunkStat), 256) + align_up((size_t)E * kMaxChunkBins * 4, 256);
     if (chunked) b += exact_sizes(n, E, n_q, kept).bytes;  // targets, candidates, refine slots, counters
@@ -191,26 +193,31 @@ size_t footprint(int64_t n, int64_t rows, int64_t E, int64_t P, int64_t n_q, boo
 // statistics) fits the cap, at most n; 0 if not even 128 samples fit.  The
 // 1,024 bytes of slack cover the 256-byte alignment of the two per-sample
 // row blocks.
-int64_t chunk_samples(int64_t n, int64_t E, int64_t P, int64_t n_q, size_t cap, int64_t kept) {
-    const size_t fixed = footprint(n, 0, E, P, n_q, true, kept);
-    const size_t per_sample = (size_t)std::max<int64_t>(E, 1) * 12;
+int64_t chunk_samples(int64_t n, int64_t E, int64_t P, int64_t n_q, size_t cap, int64_t kept, int cb) {
+    const size_t fixed = footprint(n, 0, E, P, n_q, true, kept, cb);
+    const size_t per_sample = (size_t)std::max<int64_t>(E, 1) * (size_t)(8 + cb);
     if (fixed >= cap || cap - fixed <= 1024) return 0;
     int64_t m = (int64_t)((cap - fixed - 1024) / per_sample);
     m = m / 128 * 128;
     if (m < 128) return 0;
     m = std::min<int64_t>(m, n);
-    return footprint(n, m, E, P, n_q, true, kept) <= cap ? m : 0;
+    return footprint(n, m, E, P, n_q, true, kept, cb) <= cap ? m : 0;
 }
+
+// Cause bytes per event and sample of a chunked request: its walks write the
+// winning entry's position (one byte) when no event has more than 254
+// entries, the cause event (four) otherwise.  The counts come out the same.
+int chunk_cause_bytes(int32_t max_entries) { return max_entries <= kMaxPositionEntries ? 1 : 4; }
 
 // A chunked request's samples per chunk and rows kept whole: the kept rows
 // give way (to none) when the chunks would not fit beside them.
-void chunk_plan(int64_t n, int64_t E, int64_t P, int64_t n_q, size_t cap, bool exact, int32_t limit, int64_t &m,
-                int64_t &kept) {
+void chunk_plan(int64_t n, int64_t E, int64_t P, int64_t n_q, size_t cap, bool exact, int32_t limit, int cb,
+                int64_t &m, int64_t &kept) {
     kept = exact ? kept_rows(n, E, n_q, cap, limit) : 0;
-    m = chunk_samples(n, E, P, n_q, cap, kept);
+    m = chunk_samples(n, E, P, n_q, cap, kept, cb);
     if (m < 1 && kept > 0) {
         kept = 0;
-        m = chunk_samples(n, E, P, n_q, cap, 0);
+        m = chunk_samples(n, E, P, n_q, cap, 0, cb);
     }
 }
 
@@ -226,6 +233,7 @@ struct mcdp_server {
     std::vector<int> stream_load;
     std::vector<mcdp_propagator *> contexts;
     std::vector<double *> ranges;  // per context: [E][2] bounds of realized times (null: not finite)
+    std::vector<int32_t> max_entries;  // per context: the most predecessor entries of an event
 
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -265,7 +273,8 @@ int launch_request(mcdp_server &srv, Request &r) {
     };
     auto *d_seeds = r.d_seeds = reinterpret_cast<int32_t *>(take((size_t)n * 4));
     auto *d_real = r.d_real = reinterpret_cast<double *>(take((size_t)(E * rows) * 8));
-    auto *d_cause = r.d_cause = reinterpret_cast<int32_t *>(take((size_t)(E * rows) * 4));
+    auto *d_cause = r.d_cause = reinterpret_cast<int32_t *>(take((size_t)(E * rows) * (size_t)r.cause_bytes));
+    auto *d_pos = r.cause_bytes == 1 ? reinterpret_cast<uint8_t *>(d_cause) : nullptr;
     auto *d_mean = r.d_mean = reinterpret_cast<double *>(take((size_t)E * 8));
     auto *d_quant = r.d_quant = reinterpret_cast<double *>(take((size_t)(E * nq) * 8));
     auto *d_cnt = r.d_cnt = reinterpret_cast<int64_t *>(take((size_t)(E + r.P) * 8));
@@ -308,10 +317,11 @@ int launch_request(mcdp_server &srv, Request &r) {
         }
         for (int64_t c0 = 0; c0 < n; c0 += r.chunk) {
             const int64_t m = std::min<int64_t>(r.chunk, n - c0);
-            rc = mcdp_run_many_device(prop, d_seeds + c0, m, r.chunk, d_real, nullptr, d_cause, s);
+            rc = d_pos ? run_many_positions(*prop, d_seeds + c0, m, r.chunk, d_real, d_pos, s)
+                       : mcdp_run_many_device(prop, d_seeds + c0, m, r.chunk, d_real, nullptr, d_cause, s);
             if (rc) return rc;
-            rc = chunk_stats_accumulate(*prop, d_real, d_cause, (int32_t)m, (int32_t)r.chunk, c0 == 0, d_st, d_hist,
-                                        d_cnt, s);
+            rc = chunk_stats_accumulate(*prop, d_real, d_pos ? nullptr : d_cause, d_pos, (int32_t)m, (int32_t)r.chunk,
+                                        c0 == 0, d_st, d_hist, d_cnt, s);
             if (rc) return rc;
             if (v.cap > 0) {
                 if (c0 == 0) {
@@ -572,6 +582,7 @@ int mcdp_server_add_context(mcdp_server *srv, mcdp_propagator *prop, int32_t *ct
     int rc = prepare_stats(*prop);  // plan and predecessor table resident from now on
     if (rc) return rc;
     // bounds of every event's realized time, for the rows kept whole
+    const int32_t entries = max_entries(*prop);
     double *d_ranges = nullptr;
     std::vector<double> lo, hi;
     if (prop->E > 0 && event_ranges(*prop, lo, hi)) {
@@ -596,6 +607,7 @@ int mcdp_server_add_context(mcdp_server *srv, mcdp_propagator *prop, int32_t *ct
     *ctx_id = (int32_t)srv->contexts.size();
     srv->contexts.push_back(prop);
     srv->ranges.push_back(d_ranges);
+    srv->max_entries.push_back(entries);
     return MCDP_OK;
 }
 
@@ -610,20 +622,21 @@ int mcdp_server_submit(mcdp_server *srv, int32_t ctx_id, const int32_t *seeds, i
     auto r = std::make_unique<Request>();
     mcdp_propagator *prop;
     bool exact;
-    int32_t kept_limit;
+    int32_t kept_limit, entries;
     {
         std::lock_guard<std::mutex> lock(srv->mu);
         if (ctx_id < 0 || (size_t)ctx_id >= srv->contexts.size()) return fail(MCDP_E_INVALID, "unknown context");
         prop = srv->contexts[(size_t)ctx_id];
         exact = srv->exact_chunked;
         kept_limit = srv->kept_limit;
+        entries = srv->max_entries[(size_t)ctx_id];
     }
     r->ctx = ctx_id;
     r->E = prop->E;
     r->P = (int64_t)prop->preds.size();
     r->seeds.assign(seeds, seeds + n);
     r->q.assign(q, q + n_q);
-    r->bytes = footprint(n, n, r->E, r->P, n_q, false, 0);
+    r->bytes = footprint(n, n, r->E, r->P, n_q, false, 0, 4);
     if (r->bytes > srv->request_cap) {
         // too large to hold all samples at once: chunks of samples, the largest that fit the cap
         if (!(prop->max_delay < std::numeric_limits<double>::infinity()))
@@ -631,7 +644,8 @@ int mcdp_server_submit(mcdp_server *srv, int32_t ctx_id, const int32_t *seeds, i
                                           std::to_string(srv->request_cap) +
                                           "); statistics over sample chunks need a finite max_delay");
         int64_t m, kept;
-        chunk_plan(n, r->E, r->P, n_q, srv->request_cap, exact, kept_limit, m, kept);
+        const int cb = chunk_cause_bytes(entries);
+        chunk_plan(n, r->E, r->P, n_q, srv->request_cap, exact, kept_limit, cb, m, kept);
         if (m < 1)
             return fail(MCDP_E_NOMEM, "request needs " + std::to_string(r->bytes) + " bytes of device memory; even "
                                       "chunks of 128 samples exceed the request cap of " +
@@ -639,7 +653,8 @@ int mcdp_server_submit(mcdp_server *srv, int32_t ctx_id, const int32_t *seeds, i
                                       std::to_string(srv->pool->size()) + ")");
         r->chunk = m;
         r->kept_cap = kept;
-        r->bytes = footprint(n, r->chunk, r->E, r->P, n_q, true, kept);
+        r->cause_bytes = cb;
+        r->bytes = footprint(n, r->chunk, r->E, r->P, n_q, true, kept, cb);
     }
     // Admission is strictly in arrival order: a request the pool can never
     // hold would block every later one, so none is queued.
@@ -698,22 +713,24 @@ int mcdp_server_wait(mcdp_server *srv, int64_t request_id, double *mean, double 
     return rc;
 }
 
-int mcdp_debug_serve_chunk(int64_t n, int64_t n_events, int64_t n_preds, int32_t n_q, int64_t request_cap,
-                           int64_t *chunk, int64_t *bytes) {
-    if (!chunk || !bytes || n < 1 || n_events < 0 || n_preds < 0 || n_q < 0 || n_q > 8 || request_cap < 0)
+int mcdp_debug_serve_chunk(int64_t n, int64_t n_events, int64_t n_preds, int32_t max_entries, int32_t n_q,
+                           int64_t request_cap, int64_t *chunk, int64_t *bytes) {
+    if (!chunk || !bytes || n < 1 || n_events < 0 || n_preds < 0 || max_entries < 0 || n_q < 0 || n_q > 8 ||
+        request_cap < 0)
         return fail(MCDP_E_INVALID, "invalid argument");
     const size_t cap = (size_t)request_cap;
-    const size_t whole = footprint(n, n, n_events, n_preds, n_q, false, 0);
+    const size_t whole = footprint(n, n, n_events, n_preds, n_q, false, 0, 4);
     if (whole <= cap) {
         *chunk = 0;
         *bytes = (int64_t)whole;
         return MCDP_OK;
     }
     int64_t m, kept;
-    chunk_plan(n, n_events, n_preds, n_q, cap, true, -1, m, kept);
+    const int cb = chunk_cause_bytes(max_entries);
+    chunk_plan(n, n_events, n_preds, n_q, cap, true, -1, cb, m, kept);
     if (m < 1) return fail(MCDP_E_NOMEM, "even chunks of 128 samples exceed the request cap");
     *chunk = m;
-    *bytes = (int64_t)footprint(n, m, n_events, n_preds, n_q, true, kept);
+    *bytes = (int64_t)footprint(n, m, n_events, n_preds, n_q, true, kept, cb);
     return MCDP_OK;
 }
 

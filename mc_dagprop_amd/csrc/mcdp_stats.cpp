@@ -22,13 +22,14 @@ namespace {
 // The predecessor table the cause counts need, on the device of `d`: per
 // event, its distinct sources in ascending order with the first position of
 // each in the event's predecessor list (a binary search maps a cause to its
-// position).
+// position), and per entry the first position of its source (maps a winner
+// position to the same count).
 int ensure_topo(const mcdp_propagator &P, DeviceCopy &d) {
     if (d.topo) return MCDP_OK;
     const int64_t np = (int64_t)P.preds.size();
     if (np > std::numeric_limits<int32_t>::max() - P.E - 1)
         return fail(MCDP_E_INVALID, "too many predecessor entries for cause counts");
-    std::vector<int32_t> off((size_t)P.E + 1), key((size_t)np), pos((size_t)np), nkey((size_t)P.E);
+    std::vector<int32_t> off((size_t)P.E + 1), key((size_t)np), pos((size_t)np), nkey((size_t)P.E), first((size_t)np);
     std::vector<std::pair<int32_t, int32_t>> tmp;
     for (int32_t e = 0; e < P.E; ++e) {
         const int64_t a = P.pred_off[(size_t)e], b = P.pred_off[(size_t)e + 1];
@@ -40,7 +41,10 @@ int ensure_topo(const mcdp_propagator &P, DeviceCopy &d) {
                              return x.first < y.first;
                          });
         int32_t m = 0;
+        int32_t lead = 0;
         for (size_t i = 0; i < tmp.size(); ++i) {
+            if (i == 0 || tmp[i].first != tmp[i - 1].first) lead = tmp[i].second;  // the stable sort keeps list order
+            first[(size_t)a + tmp[i].second] = lead;
             if (i > 0 && tmp[i].first == tmp[i - 1].first) continue;  // keep the first position
             key[(size_t)a + m] = tmp[i].first;
             pos[(size_t)a + m] = tmp[i].second;
@@ -49,16 +53,18 @@ int ensure_topo(const mcdp_propagator &P, DeviceCopy &d) {
         nkey[(size_t)e] = m;
     }
     off[(size_t)P.E] = (int32_t)np;
-    const size_t bytes = sizeof(int32_t) * (off.size() + key.size() + pos.size() + nkey.size());
+    const size_t bytes = sizeof(int32_t) * (off.size() + key.size() + pos.size() + nkey.size() + first.size());
     void *blob = nullptr;
     hipError_t e = hipMalloc(&blob, std::max<size_t>(bytes, 4));
     if (e != hipSuccess) return fail(MCDP_E_NOMEM, std::string("hipMalloc (predecessor table): ") + hipGetErrorString(e));
     auto *p = static_cast<int32_t *>(blob);
     int32_t *d_off = p, *d_key = d_off + off.size(), *d_pos = d_key + key.size(), *d_nkey = d_pos + pos.size();
+    int32_t *d_first = d_nkey + nkey.size();
     e = hipMemcpy(d_off, off.data(), sizeof(int32_t) * off.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && np) e = hipMemcpy(d_key, key.data(), sizeof(int32_t) * key.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && np) e = hipMemcpy(d_pos, pos.data(), sizeof(int32_t) * pos.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && P.E) e = hipMemcpy(d_nkey, nkey.data(), sizeof(int32_t) * nkey.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && np) e = hipMemcpy(d_first, first.data(), sizeof(int32_t) * first.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         (void)hipFree(blob);
         return hip_fail(e, "upload predecessor table");
@@ -68,6 +74,7 @@ int ensure_topo(const mcdp_propagator &P, DeviceCopy &d) {
     d.pred_key = d_key;
     d.pred_pos = d_pos;
     d.pred_nkey = d_nkey;
+    d.pred_first = d_first;
     return MCDP_OK;
 }
 
@@ -110,9 +117,9 @@ int quantile_args(int64_t n, int32_t n_q, const double *q, QuantileArgs &qa) {
 
 }  // namespace
 
-int mcdp::chunk_stats_accumulate(mcdp_propagator &P, const double *realized, const int32_t *cause, int32_t m,
-                                 int32_t ld, bool first, ChunkStat *st, uint32_t *hist, int64_t *counts,
-                                 hipStream_t s) {
+int mcdp::chunk_stats_accumulate(mcdp_propagator &P, const double *realized, const int32_t *cause,
+                                 const uint8_t *cause_pos, int32_t m, int32_t ld, bool first, ChunkStat *st,
+                                 uint32_t *hist, int64_t *counts, hipStream_t s) {
     std::lock_guard<std::mutex> lock(P.mu);
     DeviceCopy *d = nullptr;
     int rc = ensure_device(P, d);
@@ -122,9 +129,13 @@ int mcdp::chunk_stats_accumulate(mcdp_propagator &P, const double *realized, con
     HIP_TRY(launch_chunk_accumulate(realized, m, ld, P.E, d->base.earliest, P.max_delay, kMaxChunkBins, first ? 1 : 0,
                                     st, hist, s),
             "chunk statistics kernel");
-    HIP_TRY(launch_cause_counts(cause, m, ld, P.E, d->pred_off, d->pred_key, d->pred_pos, d->pred_nkey, counts,
-                                first ? 0 : 1, s),
-            "cause count kernel");
+    if (cause_pos)
+        HIP_TRY(launch_cause_pos_counts(cause_pos, m, ld, P.E, d->pred_off, d->pred_first, counts, first ? 0 : 1, s),
+                "cause count kernel");
+    else
+        HIP_TRY(launch_cause_counts(cause, m, ld, P.E, d->pred_off, d->pred_key, d->pred_pos, d->pred_nkey, counts,
+                                    first ? 0 : 1, s),
+                "cause count kernel");
     return MCDP_OK;
 }
 

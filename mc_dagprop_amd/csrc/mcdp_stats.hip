@@ -722,6 +722,104 @@ __global__ void __launch_bounds__(kBlock) cause_kernel(const int32_t *__restrict
                   accumulate, L);
 }
 
+// ---- cause counts from winner positions --------------------------------------
+// Rows of one byte per sample: the position of the entry that won the event's
+// max (0xFF: none).  Counted by position in LDS (slot 0: none, 1 + k: entry
+// k), then each entry's count moves to the first entry of the same source, so
+// the counts equal cause_kernel's on the cause events.
+constexpr int kPosRegs = 8;  // events with at most this many entries count in registers
+
+template <int NP>
+__device__ __forceinline__ void count_pos_regs(const uint8_t *__restrict__ row, int32_t n, uint32_t *c_lds) {
+    uint32_t cnt[NP + 1];
+#pragma unroll
+    for (int k = 0; k <= NP; ++k) cnt[k] = 0;
+    auto count = [&](uint32_t b) {
+        cnt[0] += b == 0xFFu ? 1u : 0u;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) cnt[1 + k] += b == (uint32_t)k ? 1u : 0u;
+    };
+    auto count4 = [&](uint32_t w) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) count((w >> (8 * j)) & 0xFFu);
+    };
+    // 16-byte loads over the row's aligned middle, single bytes around it
+    const int32_t head = min(n, (int32_t)((16u - ((uint32_t)(uintptr_t)row & 15u)) & 15u));
+    const int32_t n_vec = (n - head) / 16;
+    const uint4 *vrow = reinterpret_cast<const uint4 *>(row + head);
+    for (int32_t s = threadIdx.x; s < head; s += kBlock) count(row[s]);
+    for (int32_t v = threadIdx.x; v < n_vec; v += kBlock) {
+        const uint4 w = vrow[v];
+        count4(w.x);
+        count4(w.y);
+        count4(w.z);
+        count4(w.w);
+    }
+    for (int32_t s = head + 16 * n_vec + (int32_t)threadIdx.x; s < n; s += kBlock) count(row[s]);
+#pragma unroll
+    for (int k = 0; k <= NP; ++k) {
+        uint32_t t = cnt[k];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+        if ((threadIdx.x & 63) == 0 && t) atomicAdd(&c_lds[k], t);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) cause_pos_kernel(const uint8_t *__restrict__ pos, int32_t n, uint32_t ld,
+                                                           int32_t E, const int32_t *__restrict__ pred_off,
+                                                           const int32_t *__restrict__ pred_first,
+                                                           int64_t *__restrict__ counts, int accumulate) {
+    __shared__ uint32_t c_lds[1 + 256];
+    for (int32_t e = blockIdx.x; e < E; e += gridDim.x) {
+        const int32_t p0 = pred_off[e], np = pred_off[e + 1] - p0;  // np <= 254
+        int64_t *out = counts + (int64_t)p0 + e;                   // [1 + np]
+        for (int i = threadIdx.x; i <= np; i += kBlock) c_lds[i] = 0;
+        __syncthreads();
+        const uint8_t *row = pos + (uint64_t)(uint32_t)e * ld;
+        switch (np) {
+            case 0: count_pos_regs<0>(row, n, c_lds); break;
+            case 1: count_pos_regs<1>(row, n, c_lds); break;
+            case 2: count_pos_regs<2>(row, n, c_lds); break;
+            case 3: count_pos_regs<3>(row, n, c_lds); break;
+            case 4: count_pos_regs<4>(row, n, c_lds); break;
+            case 5: count_pos_regs<5>(row, n, c_lds); break;
+            case 6: count_pos_regs<6>(row, n, c_lds); break;
+            case 7: count_pos_regs<7>(row, n, c_lds); break;
+            case 8: count_pos_regs<8>(row, n, c_lds); break;
+            default: {
+                // a wave's positions take few distinct values: one atomic per value
+                const int32_t n_up = (n + kBlock - 1) / kBlock * kBlock;  // every lane takes part in the ballots
+                for (int32_t s = threadIdx.x; s < n_up; s += kBlock) {
+                    const uint32_t b = s < n ? row[s] : 0xFFFFu;  // past the row: counted nowhere
+                    const int32_t slot = b == 0xFFu ? 0 : (b < (uint32_t)np ? 1 + (int32_t)b : -1);
+                    bool pending = slot >= 0;
+                    while (__ballot(pending)) {
+                        const uint64_t mm = __ballot(pending);
+                        const int leader = __ffsll((long long)mm) - 1;
+                        const int32_t s0 = __shfl(slot, leader);
+                        const uint64_t same = __ballot(pending && slot == s0);
+                        if ((int)(threadIdx.x & 63) == leader) atomicAdd(&c_lds[s0], (uint32_t)__popcll(same));
+                        pending = pending && slot != s0;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // an entry whose source occurs earlier in the list counts for that first entry
+        for (int k = threadIdx.x; k < np; k += kBlock) {
+            const int32_t f = pred_first[p0 + k];
+            if (f != k) {
+                const uint32_t v = c_lds[1 + k];
+                c_lds[1 + k] = 0;
+                if (v) atomicAdd(&c_lds[1 + f], v);
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i <= np; i += kBlock) out[i] = (accumulate ? out[i] : 0) + (int64_t)c_lds[i];
+        __syncthreads();
+    }
+}
+
 // ---- several DAGs' statistics in one launch ----------------------------------
 // The set holding global row g: the last with row0 <= g (sets in row order).
 __device__ __forceinline__ int32_t find_set(const RowSet *__restrict__ sets, int32_t n_sets, int32_t g) {
@@ -1396,6 +1494,14 @@ hipError_t launch_cause_counts(const int32_t *cause, int32_t n, int32_t ld, int3
     if (E > 0)
         hipLaunchKernelGGL(cause_kernel, dim3((unsigned)E), dim3(kBlock), 0, stream, cause, n, (uint32_t)ld, E, pred_off,
                            pred_key, pred_pos, pred_nkey, counts, accumulate);
+    return hipGetLastError();
+}
+
+hipError_t launch_cause_pos_counts(const uint8_t *pos, int32_t n, int32_t ld, int32_t E, const int32_t *pred_off,
+                                   const int32_t *pred_first, int64_t *counts, int accumulate, hipStream_t stream) {
+    if (E > 0)
+        hipLaunchKernelGGL(cause_pos_kernel, dim3((unsigned)E), dim3(kBlock), 0, stream, pos, n, (uint32_t)ld, E,
+                           pred_off, pred_first, counts, accumulate);
     return hipGetLastError();
 }
 

@@ -53,6 +53,12 @@ struct RowSet {
 // n_sets sets in device memory; each row's results equal those kernels'.
 hipError_t launch_set_stats(const RowSet *sets, int32_t n_sets, int32_t rows, bool quantiles, bool counts,
                             hipStream_t stream);
+// The cause counts of launch_cause_counts from winner positions (one byte per
+// sample: the winning entry's position in the event's list, 0xFF: none; events
+// of at most 254 entries): pred_first[p0 + k] is the first entry of entry k's
+// source in the event's list.
+hipError_t launch_cause_pos_counts(const uint8_t *pos, int32_t n, int32_t ld, int32_t E, const int32_t *pred_off,
+                                   const int32_t *pred_first, int64_t *counts, int accumulate, hipStream_t stream);
 // accumulate != 0 adds to the counts already in `counts`.
 hipError_t launch_cause_counts(const int32_t *cause, int32_t n, int32_t ld, int32_t E, const int32_t *pred_off,
                                const int32_t *pred_key, const int32_t *pred_pos, const int32_t *pred_nkey,

@@ -323,12 +323,13 @@ void check_arguments() {
     // serving's footprint rule over a grid of sizes
     for (int64_t E : {1, 1000, 200000})
         for (int64_t n : {1, 127, 128, 100000, 262144})
-            for (int64_t cap : {(int64_t)1024, (int64_t)1 << 20, (int64_t)1 << 30, (int64_t)1 << 36}) {
-                int64_t chunk = -1, bytes = -1;
-                const int rc = mcdp_debug_serve_chunk(n, E, 3 * E, 3, cap, &chunk, &bytes);
-                CHECK(rc == MCDP_OK || rc == MCDP_E_NOMEM, "serve chunk rc %d", rc);
-                if (rc == MCDP_OK) CHECK(bytes <= cap && chunk >= 0 && chunk <= n, "serve chunk bounds");
-            }
+            for (int64_t cap : {(int64_t)1024, (int64_t)1 << 20, (int64_t)1 << 30, (int64_t)1 << 36})
+                for (int32_t entries : {4, 300}) {
+                    int64_t chunk = -1, bytes = -1;
+                    const int rc = mcdp_debug_serve_chunk(n, E, 3 * E, entries, 3, cap, &chunk, &bytes);
+                    CHECK(rc == MCDP_OK || rc == MCDP_E_NOMEM, "serve chunk rc %d", rc);
+                    if (rc == MCDP_OK) CHECK(bytes <= cap && chunk >= 0 && chunk <= n, "serve chunk bounds");
+                }
 }
 
 }  // namespace

@@ -307,6 +307,32 @@ def test_kept_rows_on_continuous_rows(gpu) -> None:
     assert st["chunked"] == 1 and st["kept_rows"] == 16 and st["exact_rounds"] >= 1
 
 
+def wide_fan_in(n_src: int = 300, max_delay: float = 40.0):
+    """One event with n_src entries (more than a byte can number) and a successor."""
+    events = [(float(i % 7), 1e6, 0.0) for i in range(n_src + 2)]
+    acts = {(i, n_src): (i, 1.0 + (i % 5), i % 4) for i in range(n_src)}
+    acts[(n_src, n_src + 1)] = (n_src, 2.0, 1)
+    prec = [(n_src, [(i, i) for i in range(n_src)]), (n_src + 1, [(n_src, n_src)])]
+    return D.FlatDag.from_lists(events, acts, prec, max_delay, list(D.STD_DISTS))
+
+
+@pytest.mark.parametrize("fan_in", [254, 300], ids=["byte_positions", "cause_events"])
+def test_chunked_causes_at_the_position_limit(gpu, fan_in) -> None:
+    """Chunks store the winning entry's position in a byte while no event has more than 254 entries, the cause
+    event otherwise; the cause counts (and every other statistic) equal run_stats either way."""
+    d = wide_fan_in(fan_in)
+    p = capi.Propagator(d)
+    seeds = (np.arange(6_000, dtype=np.int64) * 31 + 17).astype(np.int32)
+    q = (0.25, 0.5, 0.9)
+    srv = capi.Server(32 << 20, max_running=2, n_streams=2, request_cap=8 << 20)  # chunks of 512-1,152 samples
+    cid = srv.add_context(p)
+    got = srv.wait(srv.submit(cid, seeds, q))
+    st = srv.stats()
+    srv.close()
+    same_stats(got, p.run_stats(seeds, q))
+    assert st["chunked"] == 1 and st["failed"] == 0
+
+
 def test_chunked_extremes_and_no_quantiles(gpu) -> None:
     """A chunked request whose order statistics all lie on an extreme's mass (quantile 1: rank n - 1 only; quantile
     0 would also read rank 1, with weight 0) resolves without a further pass; a chunked request with no quantiles

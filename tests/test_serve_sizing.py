@@ -7,7 +7,9 @@ bytes of the cap) an unsigned subtraction used to wrap, giving a chunk of every 
 footprint above the cap, which the strictly ordered admission could never place.
 
 The footprint below restates the device layout (256-byte aligned blocks: seeds, realized and
-cause rows, mean, quantiles, cause counts, and for chunked requests the running statistics of
+cause rows (for chunked requests one byte per cause, the winning entry's position, when no event
+has more than 254 entries; four otherwise), mean, quantiles, cause counts, and for chunked
+requests the running statistics of
 56 bytes per event and a 4,096-bin histogram per event, and with quantiles the exact selection's
 targets, candidate values and refine slots, and the rows kept whole: one per 128 events, at least
 16 and at most E, within a quarter of the cap, each with its n samples, its quantiles and its event
@@ -35,8 +37,8 @@ def _a(x: int) -> int:
     return (x + 255) // 256 * 256
 
 
-def footprint(n: int, rows: int, E: int, P: int, nq: int, chunked: bool, kept: int = 0) -> int:
-    b = _a(n * 4) + _a(E * rows * 8) + _a(E * rows * 4) + _a(E * 8) + _a(E * nq * 8) + _a((E + P) * 8)
+def footprint(n: int, rows: int, E: int, P: int, nq: int, chunked: bool, kept: int = 0, cb: int = 4) -> int:
+    b = _a(n * 4) + _a(E * rows * 8) + _a(E * rows * cb) + _a(E * 8) + _a(E * nq * 8) + _a((E + P) * 8)
     if chunked:
         b += _a(E * CHUNK_STAT_BYTES) + _a(E * CHUNK_BINS * 4)
     if chunked and nq > 0:  # exact quantiles: targets, candidates, refine slots' counts and key bounds, counters
@@ -56,27 +58,28 @@ def kept_rows(n: int, E: int, nq: int, cap: int) -> int:
     return min(E, max(16, (E + 127) // 128), cap // 4 // (n * 8))
 
 
-def chunk_of(n: int, E: int, P: int, nq: int, cap: int, kept: int) -> int:
+def chunk_of(n: int, E: int, P: int, nq: int, cap: int, kept: int, cb: int) -> int:
     """The largest multiple of 128 samples (at most n) whose footprint, with 1,024 bytes of slack, fits."""
-    fixed = footprint(n, 0, E, P, nq, True, kept)
+    fixed = footprint(n, 0, E, P, nq, True, kept, cb)
     if fixed + 1024 >= cap:
         return 0
-    m = (cap - fixed - 1024) // (12 * max(E, 1)) // 128 * 128
+    m = (cap - fixed - 1024) // ((8 + cb) * max(E, 1)) // 128 * 128
     m = min(m, n)
-    return m if m >= 128 and footprint(n, m, E, P, nq, True, kept) <= cap else 0
+    return m if m >= 128 and footprint(n, m, E, P, nq, True, kept, cb) <= cap else 0
 
 
-def plan(n: int, E: int, P: int, nq: int, cap: int) -> tuple[int, int]:
+def plan(n: int, E: int, P: int, nq: int, cap: int, cb: int) -> tuple[int, int]:
     kept = kept_rows(n, E, nq, cap)
-    m = chunk_of(n, E, P, nq, cap, kept)
+    m = chunk_of(n, E, P, nq, cap, kept, cb)
     if m < 1 and kept > 0:
-        kept, m = 0, chunk_of(n, E, P, nq, cap, 0)
+        kept, m = 0, chunk_of(n, E, P, nq, cap, 0, cb)
     return m, kept
 
 
-def sizing(n: int, E: int, P: int, nq: int, cap: int) -> tuple[int, int, int]:
+def sizing(n: int, E: int, P: int, nq: int, cap: int, max_entries: int = 300) -> tuple[int, int, int]:
+    """The library's answer; max_entries above 254 keeps four-byte causes in chunks."""
     chunk, nbytes = C.c_int64(-1), C.c_int64(-1)
-    rc = capi.lib().mcdp_debug_serve_chunk(n, E, P, nq, cap, C.byref(chunk), C.byref(nbytes))
+    rc = capi.lib().mcdp_debug_serve_chunk(n, E, P, max_entries, nq, cap, C.byref(chunk), C.byref(nbytes))
     return rc, chunk.value, nbytes.value
 
 
@@ -108,7 +111,7 @@ def test_smallest_chunk_at_the_boundary() -> None:
 
 def test_random_requests_fit_their_cap_and_chunks_are_maximal() -> None:
     rng = np.random.default_rng(7)
-    with_kept = 0
+    with_kept = with_bytes = 0
     for _ in range(3000):
         E = int(rng.integers(1, 250_000))
         P = int(rng.integers(0, 3 * E + 1))
@@ -116,22 +119,34 @@ def test_random_requests_fit_their_cap_and_chunks_are_maximal() -> None:
         n = int(rng.integers(1, 300_000))
         whole = footprint(n, n, E, P, nq, False)
         cap = int(rng.integers(1, max(2, 2 * whole)))
-        rc, chunk, nbytes = sizing(n, E, P, nq, cap)
+        entries = int(rng.choice([1, 4, 254, 255, 1000]))
+        cb = 1 if entries <= 254 else 4
+        rc, chunk, nbytes = sizing(n, E, P, nq, cap, entries)
         if whole <= cap:
             assert (rc, chunk, nbytes) == (0, 0, whole)
             continue
         if rc != 0:
-            assert rc == 6 and plan(n, E, P, nq, cap)[0] == 0
-            assert footprint(n, min(128, n), E, P, nq, True) + 1024 > cap
+            assert rc == 6 and plan(n, E, P, nq, cap, cb)[0] == 0
+            assert footprint(n, min(128, n), E, P, nq, True, 0, cb) + 1024 > cap
             continue
         assert 1 <= chunk <= n and (chunk % 128 == 0 or chunk == n), chunk
-        m, kept = plan(n, E, P, nq, cap)
+        m, kept = plan(n, E, P, nq, cap, cb)
         assert m == chunk
         with_kept += kept > 0
-        assert nbytes == footprint(n, chunk, E, P, nq, True, kept) and nbytes <= cap
+        with_bytes += cb == 1
+        assert nbytes == footprint(n, chunk, E, P, nq, True, kept, cb) and nbytes <= cap
         if chunk < n and chunk + 128 <= n:
-            assert footprint(n, chunk + 128, E, P, nq, True, kept) + 1024 > cap
+            assert footprint(n, chunk + 128, E, P, nq, True, kept, cb) + 1024 > cap
     assert with_kept > 100  # the kept rows' share of the footprint is exercised, not only its fallback
+    assert with_bytes > 100  # and so are chunks of one-byte causes
+
+
+def test_byte_causes_hold_more_samples_per_chunk() -> None:
+    """The network-scale serving request: 9 bytes per event and sample instead of 12."""
+    n, E, P, nq, cap = 262_144, 200_000, 500_000, 3, 50 << 30
+    _, m4, _ = sizing(n, E, P, nq, cap, 300)
+    _, m1, b1 = sizing(n, E, P, nq, cap, 4)
+    assert m1 > m4 * 1.25 and b1 <= cap
 
 
 def test_invalid_arguments() -> None:
