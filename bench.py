@@ -215,10 +215,21 @@ def bench_serve(args) -> tuple[int, dict]:
     keep = {}
     # checked against the same request run alone: three that run in one piece, and the smallest of those that
     # run in chunks (exact quantiles from rows kept whole and, where needed, further passes over the chunks)
-    fits = [i for i in range(N) if 12 * props[ctx_of[i]].E * n_of[i] < cap // 2 and n_of[i] <= 40000]
+    # the server's own sizing rule says which requests run in chunks
+    offs = [P.pred_offsets() for P in props]
+    entries = [int(np.max(np.diff(o))) if o.size > 1 else 0 for o in offs]
+
+    def runs_in_chunks(i: int) -> bool:
+        c = ctx_of[i]
+        chunk, nbytes = capi.C.c_int64(0), capi.C.c_int64(0)
+        capi.check(capi.lib().mcdp_debug_serve_chunk(int(n_of[i]), props[c].E, int(offs[c][-1]), entries[c], len(q),
+                                                     cap, capi.C.byref(chunk), capi.C.byref(nbytes)))
+        return chunk.value > 0
+
+    chunked = [runs_in_chunks(i) for i in range(N)]
+    fits = [i for i in range(N) if not chunked[i] and n_of[i] <= 40000]
     check_idx = set(int(i) for i in fits[:: max(1, len(fits) // 3)][:3])
-    chunked_idx = sorted((i for i in range(N) if 12 * props[ctx_of[i]].E * n_of[i] > cap),
-                         key=lambda i: props[ctx_of[i]].E * n_of[i])
+    chunked_idx = sorted((i for i in range(N) if chunked[i]), key=lambda i: props[ctx_of[i]].E * n_of[i])
     if exact and chunked_idx and 12 * props[ctx_of[chunked_idx[0]]].E * n_of[chunked_idx[0]] <= (96 << 30):
         check_idx.add(int(chunked_idx[0]))
     t0 = time.perf_counter()

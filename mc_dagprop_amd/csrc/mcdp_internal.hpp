@@ -218,6 +218,11 @@ int exact_stats_pass(mcdp_propagator &P, const double *realized, int32_t m, int3
 int exact_stats_round_end(mcdp_propagator &P, const ExactWork &w, hipStream_t s);
 int exact_stats_finish(mcdp_propagator &P, int64_t n, int32_t n_q, const double *q, const ChunkStat *st,
                        const ExactWork &w, double *quantiles, hipStream_t s);
+// mcdp_event_stats_device with winner positions (run_many_positions) in place
+// of cause events; the same results.
+int event_stats_positions(mcdp_propagator &P, const double *realized, const uint8_t *cause_pos, int64_t n, int64_t ld,
+                          int32_t n_q, const double *q, double *mean, double *quantiles, int64_t *counts,
+                          hipStream_t s);
 // Statistics of several DAGs in one launch per kernel (RowSet, mcdp_stats.hpp):
 // one DAG's set (rows realized / cause with ld = n, its global rows from
 // row0, the outputs of mcdp_event_stats_device; counts null: none), then the

@@ -163,7 +163,7 @@ struct Request {
     uint32_t *d_hist = nullptr;
     ExactWork exact{};
     int64_t kept_cap = 0;  // rows kept whole (0: none)
-    int cause_bytes = 4;   // chunked: 1 when the walks write winner positions
+    int cause_bytes = 4;   // 1 when the walks write winner positions
     WatchWork watch{};
     int phase = 0;
     int rounds = 0;
@@ -204,10 +204,10 @@ int64_t chunk_samples(int64_t n, int64_t E, int64_t P, int64_t n_q, size_t cap, 
     return footprint(n, m, E, P, n_q, true, kept, cb) <= cap ? m : 0;
 }
 
-// Cause bytes per event and sample of a chunked request: its walks write the
-// winning entry's position (one byte) when no event has more than 254
-// entries, the cause event (four) otherwise.  The counts come out the same.
-int chunk_cause_bytes(int32_t max_entries) { return max_entries <= kMaxPositionEntries ? 1 : 4; }
+// Cause bytes per event and sample of a request: its walks write the winning
+// entry's position (one byte) when no event has more than 254 entries, the
+// cause event (four) otherwise.  The counts come out the same.
+int cause_bytes_for(int32_t max_entries) { return max_entries <= kMaxPositionEntries ? 1 : 4; }
 
 // A chunked request's samples per chunk and rows kept whole: the kept rows
 // give way (to none) when the chunks would not fit beside them.
@@ -347,6 +347,11 @@ int launch_request(mcdp_server &srv, Request &r) {
             r.phase = 1;
             return MCDP_OK;
         }
+    } else if (d_pos) {
+        rc = run_many_positions(*prop, d_seeds, n, n, d_real, d_pos, s);
+        if (rc) return rc;
+        rc = event_stats_positions(*prop, d_real, d_pos, n, n, (int32_t)nq, r.q.data(), d_mean, nq ? d_quant : nullptr,
+                                   d_cnt, s);
     } else {
         rc = mcdp_run_many_device(prop, d_seeds, n, n, d_real, nullptr, d_cause, s);
         if (rc) return rc;
@@ -636,7 +641,9 @@ int mcdp_server_submit(mcdp_server *srv, int32_t ctx_id, const int32_t *seeds, i
     r->P = (int64_t)prop->preds.size();
     r->seeds.assign(seeds, seeds + n);
     r->q.assign(q, q + n_q);
-    r->bytes = footprint(n, n, r->E, r->P, n_q, false, 0, 4);
+    const int cb = cause_bytes_for(entries);
+    r->cause_bytes = cb;
+    r->bytes = footprint(n, n, r->E, r->P, n_q, false, 0, cb);
     if (r->bytes > srv->request_cap) {
         // too large to hold all samples at once: chunks of samples, the largest that fit the cap
         if (!(prop->max_delay < std::numeric_limits<double>::infinity()))
@@ -644,7 +651,6 @@ int mcdp_server_submit(mcdp_server *srv, int32_t ctx_id, const int32_t *seeds, i
                                           std::to_string(srv->request_cap) +
                                           "); statistics over sample chunks need a finite max_delay");
         int64_t m, kept;
-        const int cb = chunk_cause_bytes(entries);
         chunk_plan(n, r->E, r->P, n_q, srv->request_cap, exact, kept_limit, cb, m, kept);
         if (m < 1)
             return fail(MCDP_E_NOMEM, "request needs " + std::to_string(r->bytes) + " bytes of device memory; even "
@@ -653,7 +659,6 @@ int mcdp_server_submit(mcdp_server *srv, int32_t ctx_id, const int32_t *seeds, i
                                       std::to_string(srv->pool->size()) + ")");
         r->chunk = m;
         r->kept_cap = kept;
-        r->cause_bytes = cb;
         r->bytes = footprint(n, r->chunk, r->E, r->P, n_q, true, kept, cb);
     }
     // Admission is strictly in arrival order: a request the pool can never
@@ -719,14 +724,14 @@ int mcdp_debug_serve_chunk(int64_t n, int64_t n_events, int64_t n_preds, int32_t
         request_cap < 0)
         return fail(MCDP_E_INVALID, "invalid argument");
     const size_t cap = (size_t)request_cap;
-    const size_t whole = footprint(n, n, n_events, n_preds, n_q, false, 0, 4);
+    const int cb = cause_bytes_for(max_entries);
+    const size_t whole = footprint(n, n, n_events, n_preds, n_q, false, 0, cb);
     if (whole <= cap) {
         *chunk = 0;
         *bytes = (int64_t)whole;
         return MCDP_OK;
     }
     int64_t m, kept;
-    const int cb = chunk_cause_bytes(max_entries);
     chunk_plan(n, n_events, n_preds, n_q, cap, true, -1, cb, m, kept);
     if (m < 1) return fail(MCDP_E_NOMEM, "even chunks of 128 samples exceed the request cap");
     *chunk = m;

@@ -276,6 +276,34 @@ int mcdp::set_stats(const RowSet *sets_dev, int32_t n_sets, int64_t rows, bool q
     return MCDP_OK;
 }
 
+int mcdp::event_stats_positions(mcdp_propagator &P, const double *realized, const uint8_t *cause_pos, int64_t n,
+                                int64_t ld, int32_t n_q, const double *q, double *mean, double *quantiles,
+                                int64_t *counts, hipStream_t s) {
+    if (n < 1 || ld < n || ld > std::numeric_limits<int32_t>::max())
+        return fail(MCDP_E_INVALID, "statistics need 1 <= n <= ld < 2^31");
+    if (!realized || (counts && !cause_pos)) return fail(MCDP_E_INVALID, "null argument");
+    QuantileArgs qa;
+    int rc = quantile_args(n, quantiles ? n_q : 0, q, qa);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(P.mu);
+    if (quantiles && qa.n_q > 0)
+        HIP_TRY(launch_event_quantiles(realized, (int32_t)n, (int32_t)ld, P.E, qa, quantiles, mean, s),
+                "quantile kernel");
+    else if (mean)
+        HIP_TRY(launch_event_mean(realized, (int32_t)n, (int32_t)ld, P.E, mean, s), "mean kernel");
+    if (counts) {
+        DeviceCopy *d = nullptr;
+        rc = ensure_device(P, d);
+        if (rc) return rc;
+        rc = ensure_topo(P, *d);
+        if (rc) return rc;
+        HIP_TRY(launch_cause_pos_counts(cause_pos, (int32_t)n, (int32_t)ld, P.E, d->pred_off, d->pred_first, counts, 0,
+                                        s),
+                "cause count kernel");
+    }
+    return MCDP_OK;
+}
+
 int mcdp::prepare_stats(mcdp_propagator &P) {
     std::lock_guard<std::mutex> lock(P.mu);
     DeviceCopy *d = nullptr;

@@ -42,9 +42,11 @@ def same_stats(got, want) -> None:
 BINS = 4096  # histogram bins of statistics over sample chunks
 
 
-def footprint(E: int, P: int, n: int, nq: int) -> int:
+def footprint(E: int, P: int, n: int, nq: int, cb: int = 1) -> int:
+    """A request's device bytes run whole (cb: bytes per cause, 1 for winner positions, which every context
+    here allows: no event has more than 254 entries)."""
     up = lambda x: (x + 255) // 256 * 256  # noqa: E731
-    return up(4 * n) + up(8 * E * n) + up(4 * E * n) + up(8 * E) + up(8 * E * nq) + up(8 * (E + P))
+    return up(4 * n) + up(8 * E * n) + up(cb * E * n) + up(8 * E) + up(8 * E * nq) + up(8 * (E + P))
 
 
 def contexts():

@@ -7,9 +7,9 @@ bytes of the cap) an unsigned subtraction used to wrap, giving a chunk of every 
 footprint above the cap, which the strictly ordered admission could never place.
 
 The footprint below restates the device layout (256-byte aligned blocks: seeds, realized and
-cause rows (for chunked requests one byte per cause, the winning entry's position, when no event
-has more than 254 entries; four otherwise), mean, quantiles, cause counts, and for chunked
-requests the running statistics of
+cause rows (one byte per cause, the winning entry's position, when no event has more than 254
+entries; four otherwise), mean, quantiles, cause counts, and for chunked requests the running
+statistics of
 56 bytes per event and a 4,096-bin histogram per event, and with quantiles the exact selection's
 targets, candidate values and refine slots, and the rows kept whole: one per 128 events, at least
 16 and at most E, within a quarter of the cap, each with its n samples, its quantiles and its event
@@ -86,6 +86,8 @@ def sizing(n: int, E: int, P: int, nq: int, cap: int, max_entries: int = 300) ->
 def test_whole_request_fits() -> None:
     rc, chunk, nbytes = sizing(1000, 50, 80, 3, 1 << 30)
     assert (rc, chunk, nbytes) == (0, 0, footprint(1000, 1000, 50, 80, 3, False))
+    rc, chunk, nbytes = sizing(1000, 50, 80, 3, 1 << 30, max_entries=4)  # one-byte winner positions
+    assert (rc, chunk, nbytes) == (0, 0, footprint(1000, 1000, 50, 80, 3, False, 0, 1))
 
 
 @pytest.mark.parametrize("slack", [0, 1, 255, 256, 1000, 1023, 1024, 1025])
@@ -117,10 +119,10 @@ def test_random_requests_fit_their_cap_and_chunks_are_maximal() -> None:
         P = int(rng.integers(0, 3 * E + 1))
         nq = int(rng.integers(0, 9))
         n = int(rng.integers(1, 300_000))
-        whole = footprint(n, n, E, P, nq, False)
-        cap = int(rng.integers(1, max(2, 2 * whole)))
         entries = int(rng.choice([1, 4, 254, 255, 1000]))
         cb = 1 if entries <= 254 else 4
+        whole = footprint(n, n, E, P, nq, False, 0, cb)
+        cap = int(rng.integers(1, max(2, 2 * whole)))
         rc, chunk, nbytes = sizing(n, E, P, nq, cap, entries)
         if whole <= cap:
             assert (rc, chunk, nbytes) == (0, 0, whole)
