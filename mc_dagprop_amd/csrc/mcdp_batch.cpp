@@ -78,9 +78,17 @@ int mcdp_batch_create(mcdp_propagator *const *props, int32_t n_props, int32_t wa
 
 void mcdp_batch_destroy(mcdp_batch *b) { delete b; }
 
-int mcdp_batch_prepare_device(mcdp_batch *b, const int32_t *const *seeds_dev, const int64_t *n, const int64_t *ld,
-                              double *const *realized_dev, double *const *durations_dev, int32_t *const *cause_dev) {
+}  // extern "C"
+
+namespace {
+
+// mcdp_batch_prepare_device; cause_pos_dev (instead of cause_dev, every plan
+// of at most kMaxPositionEntries entries per event): the winner positions.
+int prepare(mcdp_batch *b, const int32_t *const *seeds_dev, const int64_t *n, const int64_t *ld,
+            double *const *realized_dev, double *const *durations_dev, int32_t *const *cause_dev,
+            uint8_t *const *cause_pos_dev) {
     if (!b) return fail(MCDP_E_INVALID, "batch is null");
+    if (cause_pos_dev && cause_dev) return fail(MCDP_E_INVALID, "internal error: positions replace causes");
     b->prepared = false;
     if (!seeds_dev || !n || !ld || !realized_dev) return fail(MCDP_E_INVALID, "null argument");
     const size_t D = b->props.size();
@@ -119,6 +127,9 @@ int mcdp_batch_prepare_device(mcdp_batch *b, const int32_t *const *seeds_dev, co
         p.realized = realized_dev[d];
         p.durations = durations_dev ? durations_dev[d] : nullptr;
         p.cause = cause_dev ? cause_dev[d] : nullptr;
+        p.cause_pos = cause_pos_dev ? cause_pos_dev[d] : nullptr;
+        if (p.cause_pos && max_entries(P) > kMaxPositionEntries)
+            return fail(MCDP_E_INVALID, "internal error: winner positions need events of at most 254 entries");
         const bool has_work = n[d] > 0 && (P.E > 0 || (!P.orph.empty() && p.durations));
         tiles[d] = has_work ? (n[d] + tile - 1) / tile : 0;
         p.n_tiles = (int32_t)tiles[d];
@@ -128,6 +139,7 @@ int mcdp_batch_prepare_device(mcdp_batch *b, const int32_t *const *seeds_dev, co
         cost[d] = (int64_t)P.recs.size() + (int64_t)P.orph.size();
     }
     WalkVariant v{false, all_fast ? kPathsFast : kPathsGeneral, b->k, b->spl};
+    v.pos = cause_pos_dev != nullptr;
     for (size_t d = 0; d < D; ++d) lds = std::max(lds, plan_lds_bytes(*b->props[d], v));
 
     // Work items: every tile of every plan, plans with the most records first.
@@ -188,6 +200,15 @@ int mcdp_batch_prepare_device(mcdp_batch *b, const int32_t *const *seeds_dev, co
     return MCDP_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int mcdp_batch_prepare_device(mcdp_batch *b, const int32_t *const *seeds_dev, const int64_t *n, const int64_t *ld,
+                              double *const *realized_dev, double *const *durations_dev, int32_t *const *cause_dev) {
+    return prepare(b, seeds_dev, n, ld, realized_dev, durations_dev, cause_dev, nullptr);
+}
+
 int mcdp_batch_launch(mcdp_batch *b, void *stream) {
     if (!b) return fail(MCDP_E_INVALID, "batch is null");
     if (!b->prepared) return fail(MCDP_E_INVALID, "no prepared launch: call mcdp_batch_prepare_device first");
@@ -226,6 +247,11 @@ int mcdp_batch_stats_host(mcdp_batch *b, const int32_t *const *seeds, const int6
     if (memory_budget <= 0) return fail(MCDP_E_INVALID, "memory_budget must be positive");
     const size_t D = b->props.size();
     const size_t a = 256;
+    // The walks store one-byte winner positions instead of cause events when
+    // no event of any DAG has more than 254 entries (the same counts).
+    bool pos = true;
+    for (size_t d = 0; d < D; ++d) pos = pos && max_entries(*b->props[d]) <= kMaxPositionEntries;
+    const size_t cb = pos ? 1 : 4;
     // Device bytes of one DAG's part of a group: seeds, realized and cause
     // rows (ld = n), and its statistics.
     std::vector<size_t> bytes(D), o_seed(D), o_real(D), o_cause(D), o_mean(D), o_quant(D), o_cnt(D);
@@ -245,7 +271,7 @@ int mcdp_batch_stats_host(mcdp_batch *b, const int32_t *const *seeds, const int6
         };
         o_seed[d] = take(4 * m);
         o_real[d] = take(8 * E * m);
-        o_cause[d] = take(counts ? 4 * E * m : 0);
+        o_cause[d] = take(counts ? cb * E * m : 0);
         o_mean[d] = take(8 * E);
         o_quant[d] = take(8 * E * (size_t)n_q);
         o_cnt[d] = take(8 * (size_t)n_counts[d]);
@@ -302,6 +328,7 @@ int mcdp_batch_stats_host(mcdp_batch *b, const int32_t *const *seeds, const int6
     std::vector<unsigned char *> part(D, nullptr);  // a DAG's part of the workspace in its group
     std::vector<double *> real(D, nullptr);
     std::vector<int32_t *> cause(D, nullptr);
+    std::vector<uint8_t *> cpos(D, nullptr);
     for (size_t g = 0; rc == MCDP_OK && g + 1 < group_start.size(); ++g) {
         std::fill(gn.begin(), gn.end(), 0);
         std::fill(gld.begin(), gld.end(), 1);
@@ -310,7 +337,8 @@ int mcdp_batch_stats_host(mcdp_batch *b, const int32_t *const *seeds, const int6
             unsigned char *p = part[d] = base + at;
             sd[d] = reinterpret_cast<const int32_t *>(p + o_seed[d]);
             real[d] = reinterpret_cast<double *>(p + o_real[d]);
-            cause[d] = n_counts[d] ? reinterpret_cast<int32_t *>(p + o_cause[d]) : nullptr;
+            cause[d] = n_counts[d] && !pos ? reinterpret_cast<int32_t *>(p + o_cause[d]) : nullptr;
+            cpos[d] = n_counts[d] && pos ? reinterpret_cast<uint8_t *>(p + o_cause[d]) : nullptr;
             gn[d] = n[d];
             gld[d] = n[d];
             e = hipMemcpyAsync(p + o_seed[d], seeds[d], 4 * (size_t)n[d], hipMemcpyHostToDevice, s);
@@ -321,8 +349,10 @@ int mcdp_batch_stats_host(mcdp_batch *b, const int32_t *const *seeds, const int6
             at += bytes[d];
         }
         if (rc) break;
-        // returns with the launch done (its stream synchronised)
-        rc = mcdp_batch_run_device(b, sd.data(), gn.data(), gld.data(), real.data(), nullptr, cause.data(), s);
+        // the walk, then (same stream) the statistics
+        rc = prepare(b, sd.data(), gn.data(), gld.data(), real.data(), nullptr, pos ? nullptr : cause.data(),
+                     pos ? cpos.data() : nullptr);
+        if (rc == MCDP_OK) rc = mcdp_batch_launch(b, s);
         sets.clear();
         int64_t rows = 0;
         bool any_counts = false;
@@ -332,7 +362,7 @@ int mcdp_batch_stats_host(mcdp_batch *b, const int32_t *const *seeds, const int6
             auto *dq = n_q > 0 ? reinterpret_cast<double *>(p + o_quant[d]) : nullptr;
             auto *dc = n_counts[d] ? reinterpret_cast<int64_t *>(p + o_cnt[d]) : nullptr;
             RowSet rs;
-            rc = row_set(*b->props[d], real[d], cause[d], n[d], n_q, q, dm, dq, dc, rows, rs);
+            rc = row_set(*b->props[d], real[d], cause[d], cpos[d], n[d], n_q, q, dm, dq, dc, rows, rs);
             sets.push_back(rs);
             rows += b->props[d]->E;
             any_counts = any_counts || dc != nullptr;
@@ -362,6 +392,7 @@ int mcdp_batch_stats_host(mcdp_batch *b, const int32_t *const *seeds, const int6
             sd[d] = nullptr;
             real[d] = nullptr;
             cause[d] = nullptr;
+            cpos[d] = nullptr;
         }
     }
     (void)hipStreamSynchronize(s);

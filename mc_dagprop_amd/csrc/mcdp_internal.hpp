@@ -227,8 +227,8 @@ int event_stats_positions(mcdp_propagator &P, const double *realized, const uint
 // one DAG's set (rows realized / cause with ld = n, its global rows from
 // row0, the outputs of mcdp_event_stats_device; counts null: none), then the
 // launch over sets already in device memory; asynchronous on s.
-int row_set(mcdp_propagator &P, const double *realized, const int32_t *cause, int64_t n, int32_t n_q, const double *q,
-            double *mean, double *quantiles, int64_t *counts, int64_t row0, RowSet &rs);
+int row_set(mcdp_propagator &P, const double *realized, const int32_t *cause, const uint8_t *cause_pos, int64_t n,
+            int32_t n_q, const double *q, double *mean, double *quantiles, int64_t *counts, int64_t row0, RowSet &rs);
 int set_stats(const RowSet *sets_dev, int32_t n_sets, int64_t rows, bool quantiles, bool counts, hipStream_t s);
 // Rows kept whole across a request's chunks (mcdp_stats.hpp, WatchWork): c[0..n)
 // = 0; the rows chosen after the first chunk (ranges: [E][2] event bounds);

@@ -223,10 +223,11 @@ __device__ __forceinline__ void walk_tile(const WalkParams &p, int32_t tile, dou
                     for (int i = 0; i < kSpl; ++i) {
                         const double rr = (upper < best[i]) ? upper : best[i];
                         __builtin_nontemporal_store(rr, p.realized + row + s[i]);
-                        if (kPos)
-                            __builtin_nontemporal_store((uint8_t)who[i], p.cause_pos + row + s[i]);
-                        else if (p.cause)
+                        if (kPos) {
+                            if (p.cause_pos) __builtin_nontemporal_store((uint8_t)who[i], p.cause_pos + row + s[i]);
+                        } else if (p.cause) {
                             __builtin_nontemporal_store(who[i], p.cause + row + s[i]);
+                        }
                         res[j][i] = rr;
                     }
                     ++j;
@@ -275,7 +276,7 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_kernel(WalkParams p) 
 // Several plans in one launch: workgroups take (plan, tile) items from a
 // queue (largest plans first, as ordered by the host) until it is empty;
 // every workgroup leaves through the same check of the queue head.
-template <bool kInjected, int kPaths, int K, int kSpl>
+template <bool kInjected, int kPaths, int K, int kSpl, bool kPos = false>
 __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_batch_kernel(WalkBatchParams b) {
     extern __shared__ double lds[];
     __shared__ int32_t next_item;
@@ -297,7 +298,7 @@ __global__ void __launch_bounds__(kMaxWaves * kLanes) walk_batch_kernel(WalkBatc
             stage_alias<kInjected, kPaths, kSpl>(p, lds);
             staged = d;
         }
-        walk_tile<kInjected, kPaths, K, kSpl>(p, item_tile[item], lds, log_lds.view());
+        walk_tile<kInjected, kPaths, K, kSpl, kPos>(p, item_tile[item], lds, log_lds.view());
     }
 }
 
@@ -462,27 +463,33 @@ const void *walk_symbol(WalkVariant v) {
                                  : walk_symbol<false, kPathsGeneral, false>(v.k, v.spl);
 }
 
-template <bool kInjected, int kPaths, int kSpl>
+template <bool kInjected, int kPaths, int kSpl, bool kPos>
 const void *walk_batch_symbol(int k) {
     switch (k) {
-        case 4: return (const void *)walk_batch_kernel<kInjected, kPaths, 4, kSpl>;
-        case 8: return (const void *)walk_batch_kernel<kInjected, kPaths, 8, kSpl>;
-        case 12: return kSpl == 1 ? (const void *)walk_batch_kernel<kInjected, kPaths, 12, 1> : nullptr;
-        case 16: return kSpl == 1 ? (const void *)walk_batch_kernel<kInjected, kPaths, 16, 1> : nullptr;
+        case 4: return (const void *)walk_batch_kernel<kInjected, kPaths, 4, kSpl, kPos>;
+        case 8: return (const void *)walk_batch_kernel<kInjected, kPaths, 8, kSpl, kPos>;
+        case 12: return kSpl == 1 ? (const void *)walk_batch_kernel<kInjected, kPaths, 12, 1, kPos> : nullptr;
+        case 16: return kSpl == 1 ? (const void *)walk_batch_kernel<kInjected, kPaths, 16, 1, kPos> : nullptr;
         default: return nullptr;
     }
 }
 
-// Batches run sampled plans (the injected variant has no batch form).
+template <bool kPos>
+const void *walk_batch_symbol_of(WalkVariant v) {
+    if (v.spl == 1)
+        return v.paths == kPathsFast ? walk_batch_symbol<false, kPathsFast, 1, kPos>(v.k)
+                                     : walk_batch_symbol<false, kPathsGeneral, 1, kPos>(v.k);
+    if (v.spl == 2)
+        return v.paths == kPathsFast ? walk_batch_symbol<false, kPathsFast, 2, kPos>(v.k)
+                                     : walk_batch_symbol<false, kPathsGeneral, 2, kPos>(v.k);
+    return nullptr;
+}
+
+// Batches run sampled plans (the injected variant has no batch form); v.pos:
+// every plan's cause output is cause_pos.
 const void *walk_batch_symbol(WalkVariant v) {
     if (v.injected) return nullptr;
-    if (v.spl == 1)
-        return v.paths == kPathsFast ? walk_batch_symbol<false, kPathsFast, 1>(v.k)
-                                     : walk_batch_symbol<false, kPathsGeneral, 1>(v.k);
-    if (v.spl == 2)
-        return v.paths == kPathsFast ? walk_batch_symbol<false, kPathsFast, 2>(v.k)
-                                     : walk_batch_symbol<false, kPathsGeneral, 2>(v.k);
-    return nullptr;
+    return v.pos ? walk_batch_symbol_of<true>(v) : walk_batch_symbol_of<false>(v);
 }
 
 hipError_t walk_batch_occupancy(WalkVariant v, int waves, size_t lds_bytes, int *blocks_per_cu) {

@@ -234,19 +234,21 @@ int mcdp::exact_stats_finish(mcdp_propagator &P, int64_t n, int32_t n_q, const d
     return MCDP_OK;
 }
 
-int mcdp::row_set(mcdp_propagator &P, const double *realized, const int32_t *cause, int64_t n, int32_t n_q,
-                  const double *q, double *mean, double *quantiles, int64_t *counts, int64_t row0, RowSet &rs) {
+int mcdp::row_set(mcdp_propagator &P, const double *realized, const int32_t *cause, const uint8_t *cause_pos,
+                  int64_t n, int32_t n_q, const double *q, double *mean, double *quantiles, int64_t *counts,
+                  int64_t row0, RowSet &rs) {
     if (n < 1 || n > std::numeric_limits<int32_t>::max())
         return fail(MCDP_E_INVALID, "statistics need between 1 and 2^31 - 1 samples");
     if (row0 < 0 || row0 + P.E > std::numeric_limits<int32_t>::max())
         return fail(MCDP_E_INVALID, "too many events for one statistics launch");
-    if (!realized || !mean || (n_q > 0 && !quantiles) || (counts && !cause))
+    if (!realized || !mean || (n_q > 0 && !quantiles) || (counts && !cause && !cause_pos))
         return fail(MCDP_E_INVALID, "null argument");
     rs = RowSet{};
     int rc = quantile_args(n, n_q, q, rs.qa);
     if (rc) return rc;
     rs.realized = realized;
-    rs.cause = cause;
+    rs.cause = cause_pos ? nullptr : cause;
+    rs.cause_pos = cause_pos;
     rs.mean = mean;
     rs.quant = quantiles;
     rs.counts = counts;
@@ -265,6 +267,7 @@ int mcdp::row_set(mcdp_propagator &P, const double *realized, const int32_t *cau
         rs.pred_key = d->pred_key;
         rs.pred_pos = d->pred_pos;
         rs.pred_nkey = d->pred_nkey;
+        rs.pred_first = d->pred_first;
     }
     return MCDP_OK;
 }

@@ -765,17 +765,17 @@ __device__ __forceinline__ void count_pos_regs(const uint8_t *__restrict__ row, 
     }
 }
 
-__global__ void __launch_bounds__(kBlock) cause_pos_kernel(const uint8_t *__restrict__ pos, int32_t n, uint32_t ld,
-                                                           int32_t E, const int32_t *__restrict__ pred_off,
-                                                           const int32_t *__restrict__ pred_first,
-                                                           int64_t *__restrict__ counts, int accumulate) {
-    __shared__ uint32_t c_lds[1 + 256];
-    for (int32_t e = blockIdx.x; e < E; e += gridDim.x) {
+// The counts of event e from its row of n positions, by a block of kBlock
+// threads; c_lds holds at least 256 counters.
+__device__ __forceinline__ void cause_pos_row(const uint8_t *__restrict__ row, int32_t n, int32_t e,
+                                              const int32_t *__restrict__ pred_off,
+                                              const int32_t *__restrict__ pred_first, int64_t *__restrict__ counts,
+                                              int accumulate, uint32_t *c_lds) {
+    {
         const int32_t p0 = pred_off[e], np = pred_off[e + 1] - p0;  // np <= 254
         int64_t *out = counts + (int64_t)p0 + e;                   // [1 + np]
         for (int i = threadIdx.x; i <= np; i += kBlock) c_lds[i] = 0;
         __syncthreads();
-        const uint8_t *row = pos + (uint64_t)(uint32_t)e * ld;
         switch (np) {
             case 0: count_pos_regs<0>(row, n, c_lds); break;
             case 1: count_pos_regs<1>(row, n, c_lds); break;
@@ -818,6 +818,15 @@ __global__ void __launch_bounds__(kBlock) cause_pos_kernel(const uint8_t *__rest
         for (int i = threadIdx.x; i <= np; i += kBlock) out[i] = (accumulate ? out[i] : 0) + (int64_t)c_lds[i];
         __syncthreads();
     }
+}
+
+__global__ void __launch_bounds__(kBlock) cause_pos_kernel(const uint8_t *__restrict__ pos, int32_t n, uint32_t ld,
+                                                           int32_t E, const int32_t *__restrict__ pred_off,
+                                                           const int32_t *__restrict__ pred_first,
+                                                           int64_t *__restrict__ counts, int accumulate) {
+    __shared__ uint32_t c_lds[1 + 256];
+    for (int32_t e = blockIdx.x; e < E; e += gridDim.x)
+        cause_pos_row(pos + (uint64_t)(uint32_t)e * ld, n, e, pred_off, pred_first, counts, accumulate, c_lds);
 }
 
 // ---- several DAGs' statistics in one launch ----------------------------------
@@ -863,8 +872,12 @@ __global__ void __launch_bounds__(kBlock) cause_sets_kernel(const RowSet *__rest
         const RowSet &S = sets[find_set(sets, n_sets, g)];
         if (!S.counts) continue;  // uniform over the block
         const int32_t e = g - S.row0;
-        cause_row(S.cause + (uint64_t)(uint32_t)e * S.ld, S.n, e, S.pred_off, S.pred_key, S.pred_pos, S.pred_nkey,
-                  S.counts, 0, L);
+        if (S.cause_pos)
+            cause_pos_row(S.cause_pos + (uint64_t)(uint32_t)e * S.ld, S.n, e, S.pred_off, S.pred_first, S.counts, 0,
+                          L.c);
+        else
+            cause_row(S.cause + (uint64_t)(uint32_t)e * S.ld, S.n, e, S.pred_off, S.pred_key, S.pred_pos, S.pred_nkey,
+                      S.counts, 0, L);
     }
 }
 

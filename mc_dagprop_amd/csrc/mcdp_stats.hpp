@@ -34,10 +34,13 @@ hipError_t launch_event_quantiles(const double *realized, int32_t n, int32_t ld,
 // realized[e * ld + s] and cause likewise (s < n, e < E), global rows
 // row0 .. row0 + E - 1 (the sets in row order), outputs mean [E], quant
 // [E][qa.n_q] and, when counts is not null, the cause counts (the layout of
-// launch_cause_counts, from the DAG's predecessor table).
+// launch_cause_counts, from the DAG's predecessor table; from winner positions
+// when cause_pos is not null).
 struct RowSet {
     const double *realized;
     const int32_t *cause;
+    const uint8_t *cause_pos;  // instead of cause when not null: winner positions (launch_cause_pos_counts)
+    const int32_t *pred_first;
     double *mean;
     double *quant;
     int64_t *counts;
