@@ -369,9 +369,13 @@ int mcdp_run_stats_host(mcdp_propagator *prop, const int32_t *seeds, int64_t n, 
     QuantileArgs qa;
     int rc = quantile_args(n, quantiles ? n_q : 0, q, qa);
     if (rc) return rc;
+    // causes as one-byte winner positions when every event has at most 254
+    // entries (the same counts, a quarter of the bytes)
+    const bool pos = max_entries(*prop) <= kMaxPositionEntries;
+    const size_t cb = pos ? 1 : 4;
     const size_t o_seeds = 0, o_real = 256 * (((size_t)n * 4 + 255) / 256);
     const size_t o_cause = o_real + (size_t)(E * n) * 8;
-    const size_t o_mean = o_cause + 256 * (((size_t)(E * n) * 4 + 255) / 256);
+    const size_t o_mean = o_cause + 256 * (((size_t)(E * n) * cb + 255) / 256);
     const size_t o_q = o_mean + (size_t)E * 8;
     const size_t o_cnt = o_q + (size_t)E * (size_t)std::max(n_q, 0) * 8;
     const size_t bytes = o_cnt + (size_t)(E + P) * 8;
@@ -393,10 +397,19 @@ int mcdp_run_stats_host(mcdp_propagator *prop, const int32_t *seeds, int64_t n, 
     auto *d_cnt = reinterpret_cast<int64_t *>(b + o_cnt);
     e = hipMemcpyAsync(d_seeds, seeds, (size_t)n * 4, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy");
-    if (!rc) rc = mcdp_run_many_device(prop, d_seeds, n, n, d_real, nullptr, cause_counts ? d_cause : nullptr, st);
-    if (!rc)
-        rc = mcdp_event_stats_device(prop, d_real, d_cause, n, n, n_q, q, mean ? d_mean : nullptr,
-                                     quantiles ? d_q : nullptr, cause_counts ? d_cnt : nullptr, st);
+    auto *d_pos = reinterpret_cast<uint8_t *>(d_cause);
+    if (!rc && pos && cause_counts) {
+        rc = run_many_positions(*prop, d_seeds, n, n, d_real, d_pos, st);
+        if (!rc)
+            rc = event_stats_positions(*prop, d_real, d_pos, n, n, n_q, q, mean ? d_mean : nullptr,
+                                       quantiles ? d_q : nullptr, d_cnt, st);
+    } else {
+        if (!rc)
+            rc = mcdp_run_many_device(prop, d_seeds, n, n, d_real, nullptr, cause_counts ? d_cause : nullptr, st);
+        if (!rc)
+            rc = mcdp_event_stats_device(prop, d_real, d_cause, n, n, n_q, q, mean ? d_mean : nullptr,
+                                         quantiles ? d_q : nullptr, cause_counts ? d_cnt : nullptr, st);
+    }
     if (!rc && mean &&
         (e = hipMemcpyAsync(mean, d_mean, (size_t)E * 8, hipMemcpyDeviceToHost, st)) != hipSuccess)
         rc = hip_fail(e, "hipMemcpy");
