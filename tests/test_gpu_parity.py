@@ -136,6 +136,15 @@ def test_global_memory_fallback(gpu) -> None:
     rr, cc = ctx.propagate(d)
     np.testing.assert_array_equal(bits(r), bits(rr))
     np.testing.assert_array_equal(c, cc)
+    # run_stats walks the same plan storing one-byte winner positions (the general path's variant)
+    seeds = np.arange(300)
+    q = (0.1, 0.5, 0.9)
+    mean, quant, counts = P.run_stats(seeds, q)
+    r, _, c = P.run_many(seeds)
+    om, oq, oc = oracle.event_stats(r, c, oracle.pred_sources(dag), q)
+    np.testing.assert_array_equal(bits(mean), bits(om))
+    np.testing.assert_array_equal(quant, oq)
+    np.testing.assert_array_equal(counts, oc)
 
 
 def test_device_api_leading_dimension(gpu) -> None:
