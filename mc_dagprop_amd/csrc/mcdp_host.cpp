@@ -13,6 +13,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <fcntl.h>
+#include <sys/mman.h>
 #include <unistd.h>
 
 #include <algorithm>
@@ -29,6 +30,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mcdp.h"
@@ -991,6 +993,71 @@ void mcdp::host_workspace_trim(mcdp_propagator &P) {
 
 namespace {
 
+// The caller's output arrays, faulted in by several threads while the device
+// works: a page's first touch costs a fault, which the copies from the device
+// would otherwise take one page at a time on one thread (for the network DAG's
+// 2,048 samples, 1.3 GB: about 50 of 130 ms).  The pages get zeros, which the
+// copies then overwrite; regions under 64 MB are left alone.
+class Prefault {
+   public:
+    void start(const std::vector<std::pair<char *, size_t>> &regions) {
+        size_t total = 0;
+        for (const auto &r : regions) total += r.second;
+        if (total < ((size_t)64 << 20)) return;
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const size_t nt = std::min<size_t>(16, hw);
+        const size_t share = (total + nt - 1) / nt;
+        std::vector<std::pair<char *, size_t>> pieces;  // (start, bytes) of every thread, in order
+        size_t taken = 0;
+        for (const auto &r : regions)
+            for (size_t o = 0; o < r.second;) {
+                const size_t room = share - taken % share;
+                const size_t len = std::min(room, r.second - o);
+                pieces.emplace_back(r.first + o, len);
+                o += len;
+                taken += len;
+            }
+        try {
+            threads_.reserve(nt);
+            for (size_t t = 0; t < nt; ++t) spawn(pieces, t, share);
+        } catch (...) {  // no threads to spare: the copies fault the pages in themselves
+            join();
+        }
+    }
+    void join() {
+        for (auto &t : threads_) t.join();
+        threads_.clear();
+    }
+    ~Prefault() { join(); }
+
+   private:
+    void spawn(const std::vector<std::pair<char *, size_t>> &pieces, size_t t, size_t share) {
+        threads_.emplace_back([pieces, t, share] {
+            size_t at = 0;
+            for (const auto &pc : pieces) {  // the pieces whose bytes fall in this thread's share
+                const size_t lo = at, hi = at + pc.second;
+                at = hi;
+                const size_t a = std::max(lo, t * share), b = std::min(hi, (t + 1) * share);
+                if (a < b) {
+                    char *p = pc.first + (a - lo);
+#ifdef MADV_POPULATE_WRITE
+                    // whole pages inside the range in one call; the edges by hand
+                    const uintptr_t pg = (uintptr_t)sysconf(_SC_PAGESIZE), x = (uintptr_t)p, y = x + (b - a);
+                    const uintptr_t x1 = (x + pg - 1) / pg * pg, y1 = y / pg * pg;
+                    if (y1 > x1 && madvise((void *)x1, y1 - x1, MADV_POPULATE_WRITE) == 0) {
+                        std::memset(p, 0, x1 - x);
+                        std::memset((void *)y1, 0, y - y1);
+                        continue;
+                    }
+#endif
+                    std::memset(p, 0, b - a);
+                }
+            }
+        });
+    }
+    std::vector<std::thread> threads_;
+};
+
 // Host-buffer convenience: runs in chunks of samples so device memory stays
 // bounded, and copies each chunk into the caller's event-major arrays.  The
 // device buffers and the stream are the propagator's (HostWork), kept from
@@ -1022,6 +1089,11 @@ int run_host(mcdp_propagator &P, bool injected, const int32_t *seeds, const doub
     auto *d_cause = reinterpret_cast<int32_t *>(base + o_cause);
     auto *d_dur = reinterpret_cast<double *>(base + o_dur);
     auto *d_seeds = reinterpret_cast<int32_t *>(base + o_seeds);
+    Prefault prefault;
+    prefault.start({{reinterpret_cast<char *>(realized), realized ? sizeof(double) * (size_t)(E * n) : 0},
+                    {reinterpret_cast<char *>(cause), cause ? sizeof(int32_t) * (size_t)(E * n) : 0},
+                    {reinterpret_cast<char *>(durations),
+                     (durations && !injected) ? sizeof(double) * (size_t)(L * n) : 0}});
     for (int64_t c0 = 0; c0 < n && rc == MCDP_OK; c0 += chunk) {
         const int64_t m = std::min(chunk, n - c0);
         hipError_t e = hipSuccess;
@@ -1042,6 +1114,7 @@ int run_host(mcdp_propagator &P, bool injected, const int32_t *seeds, const doub
         }
         rc = launch(P, injected, d_seeds, d_dur, m, chunk, d_real, injected ? nullptr : d_dur, d_cause, st);
         if (rc) break;
+        prefault.join();  // the pages are ready before the first copy writes them
         auto copy_out = [&](void *dst, const void *src, size_t elem, int64_t rows) -> hipError_t {
             if (!dst || rows == 0) return hipSuccess;
             if (whole) return hipMemcpyAsync(dst, src, elem * (size_t)n * (size_t)rows, hipMemcpyDeviceToHost, st);
