@@ -17,6 +17,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cerrno>
 #include <cinttypes>
 #include <cmath>
@@ -994,10 +995,11 @@ void mcdp::host_workspace_trim(mcdp_propagator &P) {
 namespace {
 
 // The caller's output arrays, faulted in by several threads while the device
-// works: a page's first touch costs a fault, which the copies from the device
-// would otherwise take one page at a time on one thread (for the network DAG's
-// 2,048 samples, 1.3 GB: about 50 of 130 ms).  The pages get zeros, which the
-// copies then overwrite; regions under 64 MB are left alone.
+// works, for calls whose results copy directly (no staging buffers): a page's
+// first touch costs a fault, which those copies would otherwise take one page at
+// a time on one thread (for the network DAG's 2,048 samples, 1.3 GB: about 50 of
+// 130 ms).  The pages get zeros, which the copies then overwrite; regions under
+// 64 MB are left alone.
 class Prefault {
    public:
     void start(const std::vector<std::pair<char *, size_t>> &regions) {
@@ -1058,6 +1060,131 @@ class Prefault {
     std::vector<std::thread> threads_;
 };
 
+// Large device-to-host copies through pinned staging: kStageThreads threads,
+// each with its own stream and pinned buffer of kStageBytes, take pieces of the
+// results in turn, copy a piece into their buffer and from there into the
+// caller's array.  A copy straight into pageable memory goes through one
+// staging path in the runtime whatever the number of threads issuing it; this
+// way several copies run at once, and the host-side copies overlap them.  The
+// buffers and streams exist once per device (64 MB of pinned memory, held until
+// the process ends); calls take turns on them.
+constexpr int kStageThreads = 8;
+constexpr size_t kStageBytes = (size_t)8 << 20;
+constexpr size_t kStageMinBytes = (size_t)64 << 20;  // smaller results copy directly
+
+struct StageSet {
+    int device = -1;
+    void *buf[kStageThreads] = {};
+    hipStream_t stream[kStageThreads] = {};
+};
+std::mutex g_stage_mu;
+std::vector<StageSet> g_stage;
+
+// One output array of a chunk: rows of `width` bytes, `src_pitch` apart on the
+// device and `dst_pitch` apart in the caller's array.
+struct CopyPlane {
+    char *dst;
+    const char *src;
+    size_t rows, width, src_pitch, dst_pitch;
+};
+
+// A piece: rows [r0, r0 + k) and bytes [b0, b0 + len) of each of them.  Its
+// device bytes are the contiguous span from row r0's byte b0 to row r0+k-1's
+// byte b0+len, at most kStageBytes.
+struct CopyPiece {
+    const CopyPlane *plane;
+    size_t r0, k, b0, len;
+};
+
+// Returns an mcdp_status, or -1 when the staging buffers cannot be had (then
+// nothing was copied).
+// The current device's staging set, created on first use; nullptr when its
+// buffers cannot be had.  Call with g_stage_mu held.
+StageSet *stage_set() {
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) return nullptr;
+    for (auto &g : g_stage)
+        if (g.device == device) return &g;
+    StageSet g;
+    g.device = device;
+    for (int t = 0; t < kStageThreads; ++t) {
+        hipError_t e = hipHostMalloc(&g.buf[t], kStageBytes, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&g.stream[t], hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            for (int u = 0; u <= t; ++u) {
+                if (g.buf[u]) (void)hipHostFree(g.buf[u]);
+                if (g.stream[u]) (void)hipStreamDestroy(g.stream[u]);
+            }
+            return nullptr;
+        }
+    }
+    g_stage.push_back(g);
+    return &g_stage.back();
+}
+
+// True when large results of this device can go through the staging buffers.
+bool staging_available() {
+    std::lock_guard<std::mutex> lock(g_stage_mu);
+    return stage_set() != nullptr;
+}
+
+int staged_copy(const std::vector<CopyPlane> &planes) {
+    std::lock_guard<std::mutex> lock(g_stage_mu);
+    StageSet *set = stage_set();
+    if (!set) return -1;
+    std::vector<CopyPiece> pieces;
+    for (const auto &pl : planes) {
+        if (pl.rows == 0 || pl.width == 0) continue;
+        if (pl.width == pl.src_pitch && pl.width == pl.dst_pitch) {
+            // rows back to back on both sides: one long row
+            const size_t total = pl.rows * pl.width;
+            for (size_t b = 0; b < total; b += kStageBytes)
+                pieces.push_back({&pl, 0, 1, b, std::min(kStageBytes, total - b)});
+        } else if (pl.src_pitch <= kStageBytes) {
+            const size_t per = kStageBytes / pl.src_pitch;
+            for (size_t r = 0; r < pl.rows; r += per) pieces.push_back({&pl, r, std::min(per, pl.rows - r), 0, pl.width});
+        } else {
+            for (size_t r = 0; r < pl.rows; ++r)
+                for (size_t b = 0; b < pl.width; b += kStageBytes)
+                    pieces.push_back({&pl, r, 1, b, std::min(kStageBytes, pl.width - b)});
+        }
+    }
+    std::atomic<size_t> next{0};
+    std::atomic<int> failed{0};
+    hipError_t first_error = hipSuccess;
+    std::mutex err_mu;
+    auto work = [&](int t) {
+        char *stage = static_cast<char *>(set->buf[t]);
+        for (size_t i; !failed.load(std::memory_order_relaxed) && (i = next.fetch_add(1)) < pieces.size();) {
+            const CopyPiece &pc = pieces[i];
+            const CopyPlane &pl = *pc.plane;
+            const bool flat = pl.width == pl.src_pitch && pl.width == pl.dst_pitch;
+            const size_t pitch = flat ? 0 : pl.src_pitch;
+            const size_t span = (pc.k - 1) * pitch + pc.len;
+            hipError_t e = hipMemcpyAsync(stage, pl.src + pc.r0 * pitch + pc.b0, span, hipMemcpyDeviceToHost,
+                                          set->stream[t]);
+            if (e == hipSuccess) e = hipStreamSynchronize(set->stream[t]);
+            if (e != hipSuccess) {
+                std::lock_guard<std::mutex> g(err_mu);
+                if (first_error == hipSuccess) first_error = e;
+                failed.store(1);
+                return;
+            }
+            for (size_t j = 0; j < pc.k; ++j)
+                std::memcpy(pl.dst + (pc.r0 + j) * (flat ? 0 : pl.dst_pitch) + pc.b0, stage + j * pitch, pc.len);
+        }
+    };
+    std::vector<std::thread> threads;
+    try {
+        for (int t = 1; t < kStageThreads; ++t) threads.emplace_back(work, t);
+    } catch (...) {  // fewer threads: the ones running take every piece
+    }
+    work(0);
+    for (auto &th : threads) th.join();
+    if (failed.load()) return hip_fail(first_error, "hipMemcpy (outputs)");
+    return MCDP_OK;
+}
+
 // Host-buffer convenience: runs in chunks of samples so device memory stays
 // bounded, and copies each chunk into the caller's event-major arrays.  The
 // device buffers and the stream are the propagator's (HostWork), kept from
@@ -1089,11 +1216,17 @@ int run_host(mcdp_propagator &P, bool injected, const int32_t *seeds, const doub
     auto *d_cause = reinterpret_cast<int32_t *>(base + o_cause);
     auto *d_dur = reinterpret_cast<double *>(base + o_dur);
     auto *d_seeds = reinterpret_cast<int32_t *>(base + o_seeds);
+    // With staging, the copying threads fault the caller's pages in themselves, in parallel and while
+    // other pieces are in flight; without it the copies would take them one at a time.
+    const size_t total_out = (size_t)n * ((realized ? 8 * E : 0) + (cause ? 4 * E : 0) +
+                                          (durations && !injected ? 8 * L : 0));
+    const bool staged = total_out >= kStageMinBytes && staging_available();
     Prefault prefault;
-    prefault.start({{reinterpret_cast<char *>(realized), realized ? sizeof(double) * (size_t)(E * n) : 0},
-                    {reinterpret_cast<char *>(cause), cause ? sizeof(int32_t) * (size_t)(E * n) : 0},
-                    {reinterpret_cast<char *>(durations),
-                     (durations && !injected) ? sizeof(double) * (size_t)(L * n) : 0}});
+    if (!staged)
+        prefault.start({{reinterpret_cast<char *>(realized), realized ? sizeof(double) * (size_t)(E * n) : 0},
+                        {reinterpret_cast<char *>(cause), cause ? sizeof(int32_t) * (size_t)(E * n) : 0},
+                        {reinterpret_cast<char *>(durations),
+                         (durations && !injected) ? sizeof(double) * (size_t)(L * n) : 0}});
     for (int64_t c0 = 0; c0 < n && rc == MCDP_OK; c0 += chunk) {
         const int64_t m = std::min(chunk, n - c0);
         hipError_t e = hipSuccess;
@@ -1115,6 +1248,28 @@ int run_host(mcdp_propagator &P, bool injected, const int32_t *seeds, const doub
         rc = launch(P, injected, d_seeds, d_dur, m, chunk, d_real, injected ? nullptr : d_dur, d_cause, st);
         if (rc) break;
         prefault.join();  // the pages are ready before the first copy writes them
+        const size_t out_bytes = (size_t)m * ((realized ? 8 * E : 0) + (cause ? 4 * E : 0) +
+                                              (durations && !injected ? 8 * L : 0));
+        if (staged && out_bytes >= kStageMinBytes) {
+            if ((e = hipStreamSynchronize(st)) != hipSuccess) {
+                rc = hip_fail(e, "walk kernel");
+                break;
+            }
+            std::vector<CopyPlane> planes;
+            auto plane = [&](void *dst, const void *src, size_t elem, int64_t rows) {
+                if (dst && rows > 0)
+                    planes.push_back({static_cast<char *>(dst) + elem * c0, static_cast<const char *>(src),
+                                      (size_t)rows, elem * m, elem * chunk, elem * n});
+            };
+            plane(realized, d_real, sizeof(double), E);
+            plane(cause, d_cause, sizeof(int32_t), E);
+            if (!injected) plane(durations, d_dur, sizeof(double), L);
+            const int sc = staged_copy(planes);
+            if (sc >= 0) {
+                rc = sc;
+                continue;
+            }
+        }
         auto copy_out = [&](void *dst, const void *src, size_t elem, int64_t rows) -> hipError_t {
             if (!dst || rows == 0) return hipSuccess;
             if (whole) return hipMemcpyAsync(dst, src, elem * (size_t)n * (size_t)rows, hipMemcpyDeviceToHost, st);

@@ -206,6 +206,36 @@ def test_network_slice_consistent(gpu) -> None:
     assert np.all(np.isfinite(d)) and np.all(d >= dag.act_dur[None, :])
 
 
+def _host_chunk(dag: FlatDag) -> int:
+    """Samples per device chunk of the host calls: 2 GiB of results and seeds, rounded down to whole waves."""
+    per_sample = 12 * dag.n_events + 8 * len(dag.act_idx) + 8
+    return (2**31 // per_sample) // 64 * 64
+
+
+@pytest.mark.parametrize("name", ["network", "long_chain"])
+def test_host_run_many_over_several_chunks(gpu, name: str) -> None:
+    """A host run_many over more samples than one device chunk equals the same seeds in calls of at most one chunk,
+    bit for bit.  Its results reach the caller's arrays in pieces through pinned staging buffers: row groups of the
+    network DAG (26 KB device rows), and rows longer than a staging buffer, split, for the 100-event chain (8.6 MB
+    device rows of its 1,073,728-sample chunks)."""
+    dag = layered_network() if name == "network" else D.parity_long_chain(99)
+    P = capi.Propagator(dag)
+    chunk = _host_chunk(dag)
+    n = chunk + (700 if name == "network" else 5000)
+    seeds = np.arange(n, dtype=np.int32)
+    r, d, c = P.run_many(seeds)
+    for lo, hi in ((0, chunk), (chunk, n)):
+        r1, d1, c1 = P.run_many(seeds[lo:hi])
+        np.testing.assert_array_equal(bits(r1), bits(r[lo:hi]))
+        np.testing.assert_array_equal(bits(d1), bits(d[lo:hi]))
+        np.testing.assert_array_equal(c1, c[lo:hi])
+        del r1, d1, c1
+    if name == "long_chain":  # injected durations over several chunks: the same results again
+        rr, cc = P.propagate(d)
+        np.testing.assert_array_equal(bits(rr), bits(r))
+        np.testing.assert_array_equal(cc, c)
+
+
 SHAPES = [(1, 4, 1), (2, 16, 1), (5, 8, 1), (16, 12, 1), (16, 16, 1), (1, 4, 2), (3, 8, 2), (7, 4, 2), (16, 8, 2)]
 
 
