@@ -1154,6 +1154,7 @@ int staged_copy(const std::vector<CopyPlane> &planes) {
     hipError_t first_error = hipSuccess;
     std::mutex err_mu;
     auto work = [&](int t) {
+        if (t > 0 && hipSetDevice(set->device) != hipSuccess) return;  // a new thread starts on device 0
         char *stage = static_cast<char *>(set->buf[t]);
         for (size_t i; !failed.load(std::memory_order_relaxed) && (i = next.fetch_add(1)) < pieces.size();) {
             const CopyPiece &pc = pieces[i];
