@@ -104,7 +104,12 @@ int mcdp_run_many_device(mcdp_propagator *prop, const int32_t *seeds_dev, int64_
                          double *realized_dev, double *durations_dev, int32_t *cause_dev, void *stream);
 
 /* Simulator::run_many (_core.cpp:355-361) on host buffers, synchronous.
- * Outputs event-major with ld = n: realized[E*n], durations[L*n], cause[E*n]. */
+ * Outputs event-major with ld = n: realized[E*n], durations[L*n], cause[E*n].
+ * The device work runs in chunks of samples (about 2 GiB of device buffers,
+ * kept by the propagator up to 4 GiB).  Results of 64 MB and more reach the
+ * outputs through pinned staging buffers, 64 MB per device, created on the
+ * first such call and held until the process ends; calls from several threads
+ * take turns on them.  Any output may be NULL to skip it. */
 int mcdp_run_many_host(mcdp_propagator *prop, const int32_t *seeds, int64_t n, double *realized,
                        double *durations, int32_t *cause);
 
