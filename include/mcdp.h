@@ -106,7 +106,7 @@ int mcdp_run_many_device(mcdp_propagator *prop, const int32_t *seeds_dev, int64_
 /* Simulator::run_many (_core.cpp:355-361) on host buffers, synchronous.
  * Outputs event-major with ld = n: realized[E*n], durations[L*n], cause[E*n].
  * The device work runs in chunks of samples (about 2 GiB of device buffers,
- * kept by the propagator up to 4 GiB).  Results of 64 MB and more reach the
+ * kept by the propagator up to 4 GiB).  Results of 16 MB and more reach the
  * outputs through pinned staging buffers, 64 MB per device, created on the
  * first such call and held until the process ends; calls from several threads
  * take turns on them.  Any output may be NULL to skip it. */

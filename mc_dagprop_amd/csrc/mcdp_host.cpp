@@ -1070,7 +1070,7 @@ class Prefault {
 // the process ends); calls take turns on them.
 constexpr int kStageThreads = 8;
 constexpr size_t kStageBytes = (size_t)8 << 20;
-constexpr size_t kStageMinBytes = (size_t)64 << 20;  // smaller results copy directly
+constexpr size_t kStageMinBytes = (size_t)16 << 20;  // smaller results copy directly
 
 struct StageSet {
     int device = -1;
