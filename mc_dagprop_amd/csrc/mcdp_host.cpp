@@ -1060,17 +1060,17 @@ class Prefault {
     std::vector<std::thread> threads_;
 };
 
-// Large device-to-host copies through pinned staging: kStageThreads threads,
-// each with its own stream and pinned buffer of kStageBytes, take pieces of the
-// results in turn, copy a piece into their buffer and from there into the
-// caller's array.  A copy straight into pageable memory goes through one
-// staging path in the runtime whatever the number of threads issuing it; this
-// way several copies run at once, and the host-side copies overlap them.  The
-// buffers and streams exist once per device (64 MB of pinned memory, held until
-// the process ends); calls take turns on them.
+// Large copies between the device and the caller's (pageable) arrays through
+// pinned staging: kStageThreads threads, each with its own stream and pinned
+// buffer of kStageBytes, take pieces of the arrays in turn and move each
+// through their buffer.  A copy straight to or from pageable memory goes
+// through one staging path in the runtime whatever the number of threads
+// issuing it; this way several transfers run at once, and the host-side copies
+// overlap them.  The buffers and streams exist once per device (64 MB of pinned
+// memory, held until the process ends); calls take turns on them.
 constexpr int kStageThreads = 8;
 constexpr size_t kStageBytes = (size_t)8 << 20;
-constexpr size_t kStageMinBytes = (size_t)16 << 20;  // smaller results copy directly
+constexpr size_t kStageMinBytes = (size_t)16 << 20;  // smaller arrays copy directly
 
 struct StageSet {
     int device = -1;
@@ -1080,24 +1080,23 @@ struct StageSet {
 std::mutex g_stage_mu;
 std::vector<StageSet> g_stage;
 
-// One output array of a chunk: rows of `width` bytes, `src_pitch` apart on the
-// device and `dst_pitch` apart in the caller's array.
+// One array of a chunk: rows of `width` bytes, `dev_pitch` apart on the device
+// and `host_pitch` apart in the caller's array.
 struct CopyPlane {
-    char *dst;
-    const char *src;
-    size_t rows, width, src_pitch, dst_pitch;
+    char *host;
+    char *dev;
+    size_t rows, width, dev_pitch, host_pitch;
 };
 
 // A piece: rows [r0, r0 + k) and bytes [b0, b0 + len) of each of them.  Its
 // device bytes are the contiguous span from row r0's byte b0 to row r0+k-1's
-// byte b0+len, at most kStageBytes.
+// byte b0+len, at most kStageBytes; an upload also writes the span's bytes
+// between rows (columns past `width`, which the chunk does not use).
 struct CopyPiece {
     const CopyPlane *plane;
     size_t r0, k, b0, len;
 };
 
-// Returns an mcdp_status, or -1 when the staging buffers cannot be had (then
-// nothing was copied).
 // The current device's staging set, created on first use; nullptr when its
 // buffers cannot be had.  Call with g_stage_mu held.
 StageSet *stage_set() {
@@ -1122,26 +1121,30 @@ StageSet *stage_set() {
     return &g_stage.back();
 }
 
-// True when large results of this device can go through the staging buffers.
+// True when large arrays of this device can go through the staging buffers.
 bool staging_available() {
     std::lock_guard<std::mutex> lock(g_stage_mu);
     return stage_set() != nullptr;
 }
 
-int staged_copy(const std::vector<CopyPlane> &planes) {
+// Moves every plane device -> host (to_host) or host -> device, synchronously.
+// The device side must be ready (no kernel still writing or reading it).
+// Returns an mcdp_status, or -1 when the staging buffers cannot be had (then
+// nothing was copied).
+int staged_copy(const std::vector<CopyPlane> &planes, bool to_host) {
     std::lock_guard<std::mutex> lock(g_stage_mu);
     StageSet *set = stage_set();
     if (!set) return -1;
     std::vector<CopyPiece> pieces;
     for (const auto &pl : planes) {
         if (pl.rows == 0 || pl.width == 0) continue;
-        if (pl.width == pl.src_pitch && pl.width == pl.dst_pitch) {
+        if (pl.width == pl.dev_pitch && pl.width == pl.host_pitch) {
             // rows back to back on both sides: one long row
             const size_t total = pl.rows * pl.width;
             for (size_t b = 0; b < total; b += kStageBytes)
                 pieces.push_back({&pl, 0, 1, b, std::min(kStageBytes, total - b)});
-        } else if (pl.src_pitch <= kStageBytes) {
-            const size_t per = kStageBytes / pl.src_pitch;
+        } else if (pl.dev_pitch <= kStageBytes) {
+            const size_t per = kStageBytes / pl.dev_pitch;
             for (size_t r = 0; r < pl.rows; r += per) pieces.push_back({&pl, r, std::min(per, pl.rows - r), 0, pl.width});
         } else {
             for (size_t r = 0; r < pl.rows; ++r)
@@ -1159,11 +1162,16 @@ int staged_copy(const std::vector<CopyPlane> &planes) {
         for (size_t i; !failed.load(std::memory_order_relaxed) && (i = next.fetch_add(1)) < pieces.size();) {
             const CopyPiece &pc = pieces[i];
             const CopyPlane &pl = *pc.plane;
-            const bool flat = pl.width == pl.src_pitch && pl.width == pl.dst_pitch;
-            const size_t pitch = flat ? 0 : pl.src_pitch;
+            const bool flat = pl.width == pl.dev_pitch && pl.width == pl.host_pitch;
+            const size_t pitch = flat ? 0 : pl.dev_pitch;  // of the rows in the staging buffer, as on the device
+            const size_t host_pitch = flat ? 0 : pl.host_pitch;
             const size_t span = (pc.k - 1) * pitch + pc.len;
-            hipError_t e = hipMemcpyAsync(stage, pl.src + pc.r0 * pitch + pc.b0, span, hipMemcpyDeviceToHost,
-                                          set->stream[t]);
+            char *dev = pl.dev + pc.r0 * pitch + pc.b0;
+            if (!to_host)
+                for (size_t j = 0; j < pc.k; ++j)
+                    std::memcpy(stage + j * pitch, pl.host + (pc.r0 + j) * host_pitch + pc.b0, pc.len);
+            hipError_t e = to_host ? hipMemcpyAsync(stage, dev, span, hipMemcpyDeviceToHost, set->stream[t])
+                                   : hipMemcpyAsync(dev, stage, span, hipMemcpyHostToDevice, set->stream[t]);
             if (e == hipSuccess) e = hipStreamSynchronize(set->stream[t]);
             if (e != hipSuccess) {
                 std::lock_guard<std::mutex> g(err_mu);
@@ -1171,8 +1179,9 @@ int staged_copy(const std::vector<CopyPlane> &planes) {
                 failed.store(1);
                 return;
             }
-            for (size_t j = 0; j < pc.k; ++j)
-                std::memcpy(pl.dst + (pc.r0 + j) * (flat ? 0 : pl.dst_pitch) + pc.b0, stage + j * pitch, pc.len);
+            if (to_host)
+                for (size_t j = 0; j < pc.k; ++j)
+                    std::memcpy(pl.host + (pc.r0 + j) * host_pitch + pc.b0, stage + j * pitch, pc.len);
         }
     };
     std::vector<std::thread> threads;
@@ -1182,7 +1191,7 @@ int staged_copy(const std::vector<CopyPlane> &planes) {
     }
     work(0);
     for (auto &th : threads) th.join();
-    if (failed.load()) return hip_fail(first_error, "hipMemcpy (outputs)");
+    if (failed.load()) return hip_fail(first_error, to_host ? "hipMemcpy (outputs)" : "hipMemcpy (inputs)");
     return MCDP_OK;
 }
 
@@ -1235,7 +1244,19 @@ int run_host(mcdp_propagator &P, bool injected, const int32_t *seeds, const doub
         // one contiguous copy (a 2-D copy of many short rows, e.g. one sample, costs a transfer per row)
         const bool whole = chunk == n;
         if (injected) {
-            if (L > 0)
+            // the workspace is idle here: every chunk ends by waiting on the stream
+            const size_t in_bytes = sizeof(double) * (size_t)L * (size_t)m;
+            int sc = -1;
+            if (L > 0 && in_bytes >= kStageMinBytes)
+                sc = staged_copy({{const_cast<char *>(reinterpret_cast<const char *>(dur_in + c0)),
+                                   reinterpret_cast<char *>(d_dur), (size_t)L, sizeof(double) * m,
+                                   sizeof(double) * chunk, sizeof(double) * n}},
+                                 false);
+            if (sc > 0) {
+                rc = sc;
+                break;
+            }
+            if (L > 0 && sc < 0)
                 e = whole ? hipMemcpyAsync(d_dur, dur_in, sizeof(double) * L * n, hipMemcpyHostToDevice, st)
                           : hipMemcpy2DAsync(d_dur, sizeof(double) * chunk, dur_in + c0, sizeof(double) * n,
                                              sizeof(double) * m, (size_t)L, hipMemcpyHostToDevice, st);
@@ -1257,15 +1278,15 @@ int run_host(mcdp_propagator &P, bool injected, const int32_t *seeds, const doub
                 break;
             }
             std::vector<CopyPlane> planes;
-            auto plane = [&](void *dst, const void *src, size_t elem, int64_t rows) {
-                if (dst && rows > 0)
-                    planes.push_back({static_cast<char *>(dst) + elem * c0, static_cast<const char *>(src),
-                                      (size_t)rows, elem * m, elem * chunk, elem * n});
+            auto plane = [&](void *host, void *dev, size_t elem, int64_t rows) {
+                if (host && rows > 0)
+                    planes.push_back({static_cast<char *>(host) + elem * c0, static_cast<char *>(dev), (size_t)rows,
+                                      elem * m, elem * chunk, elem * n});
             };
             plane(realized, d_real, sizeof(double), E);
             plane(cause, d_cause, sizeof(int32_t), E);
             if (!injected) plane(durations, d_dur, sizeof(double), L);
-            const int sc = staged_copy(planes);
+            const int sc = staged_copy(planes, true);
             if (sc >= 0) {
                 rc = sc;
                 continue;

@@ -215,9 +215,10 @@ def _host_chunk(dag: FlatDag) -> int:
 @pytest.mark.parametrize("name", ["network", "long_chain"])
 def test_host_run_many_over_several_chunks(gpu, name: str) -> None:
     """A host run_many over more samples than one device chunk equals the same seeds in calls of at most one chunk,
-    bit for bit.  Its results reach the caller's arrays in pieces through pinned staging buffers: row groups of the
-    network DAG (26 KB device rows), and rows longer than a staging buffer, split, for the 100-event chain (8.6 MB
-    device rows of its 1,073,728-sample chunks)."""
+    bit for bit, and so does propagating its durations.  Results and injected durations move between the caller's
+    arrays and the device in pieces through pinned staging buffers: row groups of the network DAG (26 KB device
+    rows), and rows longer than a staging buffer, split, for the 100-event chain (8.6 MB device rows of its
+    1,073,728-sample chunks)."""
     dag = layered_network() if name == "network" else D.parity_long_chain(99)
     P = capi.Propagator(dag)
     chunk = _host_chunk(dag)
@@ -230,10 +231,10 @@ def test_host_run_many_over_several_chunks(gpu, name: str) -> None:
         np.testing.assert_array_equal(bits(d1), bits(d[lo:hi]))
         np.testing.assert_array_equal(c1, c[lo:hi])
         del r1, d1, c1
-    if name == "long_chain":  # injected durations over several chunks: the same results again
-        rr, cc = P.propagate(d)
-        np.testing.assert_array_equal(bits(rr), bits(r))
-        np.testing.assert_array_equal(cc, c)
+    # injected durations over several chunks, uploaded through the same staging: the same results again
+    rr, cc = P.propagate(d)
+    np.testing.assert_array_equal(bits(rr), bits(r))
+    np.testing.assert_array_equal(cc, c)
 
 
 SHAPES = [(1, 4, 1), (2, 16, 1), (5, 8, 1), (16, 12, 1), (16, 16, 1), (1, 4, 2), (3, 8, 2), (7, 4, 2), (16, 8, 2)]
