@@ -22,7 +22,7 @@ Also reported:
   check         device durations of a few samples fed to the CPU oracle
                 reproduce the device's realized / cause bit-exactly.
 
-    python bench.py [--steps K] [--warmup W] [--samples S] [--workload network|small|batch|grad|serve]
+    python bench.py [--steps K] [--warmup W] [--samples S] [--workload network|small|batch|grad|serve|host]
 
 --workload grad times the pathwise-gradient step (run_many + the reverse walk) on the
 network DAG with 65,536 samples, with the reverse walk's roofline and a check of its
@@ -30,7 +30,9 @@ gradient against the FP64 CPU oracle.  --workload serve runs a Poisson stream of
 statistics requests over 200 resident contexts through the serving loop and reports
 p50/p99 latency beside the aggregate throughput.  --workload batch runs 64 DAGs of
 1,000-50,000 events and 4,096-65,536 samples with per-event statistics reduced on the
-device, batched, against one DAG at a time.
+device, batched, against one DAG at a time.  --workload host times the Python API's
+run_many over 100,000 network samples with every result in host memory (64 GB through
+pinned staging), and propagate_host with 2,048 samples' durations injected.
 """
 
 from __future__ import annotations
@@ -429,6 +431,79 @@ def bench_grad(args) -> tuple[int, dict]:
     return (0 if check is None or check["ok"] else 3), out
 
 
+def bench_host(args) -> tuple[int, dict]:
+    """The drop-in API with results in host memory (the PCIe-inclusive rate): mc_dagprop_amd.Simulator.run_many
+    over S seeds of the network DAG (100,000 by default: 64 GB of realized / durations / cause in host memory), and
+    mcdp_propagate_host with 2,048 samples' durations injected.  A spread of the returned samples goes through the
+    oracle from their own durations (bit-exact)."""
+    import gc
+
+    import mc_dagprop_amd as api
+    import oracle
+    from mc_dagprop_amd import capi
+    from mc_dagprop_amd.workloads import layered_network
+
+    dag = layered_network()
+    S = args.samples or 100_000
+    steps = max(1, min(args.steps, 3))
+    ctx, gen = dag.to_api(api)
+    sim = api.Simulator(ctx, gen)
+    L, E = sim.activity_count(), sim.node_count()
+    sim.run_many(range(64))  # plan upload, staging buffers
+    times, check = [], None
+    for k in range(steps):
+        t0 = time.perf_counter()
+        res = sim.run_many(range(S))
+        times.append(time.perf_counter() - t0)
+        log(f"host run_many {k}: {1000 * times[-1]:.0f} ms")
+        if k == steps - 1:
+            pick = sorted({int(c) for c in np.linspace(0, S - 1, 64).round()})
+            d = np.stack([np.asarray(res[i].durations) for i in pick])
+            r = np.stack([np.asarray(res[i].realized) for i in pick])
+            c = np.stack([np.asarray(res[i].cause_event) for i in pick])
+            rr, cc = oracle.OracleCtx(dag).propagate(d)
+            check = {"samples_checked": len(pick),
+                     "bit_exact": bool(np.array_equal(r.view(np.uint64), rr.view(np.uint64)) and np.array_equal(c, cc))}
+            log(f"host check: {check}")
+        del res
+        gc.collect()
+    t = float(np.median(times))
+    P = capi.Propagator(dag)
+    n_inj = 2048
+    _, dur, _ = P.run_many(np.arange(n_inj))
+    P.propagate(dur)
+    inj = []
+    for _ in range(4):
+        t0 = time.perf_counter()
+        P.propagate(dur)  # (n, L) view of the [L][n] array: no transpose on the way in
+        inj.append(1000.0 * (time.perf_counter() - t0))
+    out_bytes = S * (12 * E + 8 * L)
+    out = {
+        "metric": "activity-samples/s of the Python API's run_many with every result in host memory",
+        "value": L * S / t,
+        "unit": "activity-samples/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 1,
+        "ms_per_step": 1000.0 * t,
+        "mode": "host",
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (deterministic SplitMix64 layered DAG; seeds 0..S-1)",
+        "config": {"workload": f"layered_network ({E} events, {L} activities), mc_dagprop_amd.Simulator.run_many("
+                               f"range({S})) returning SimResults in host memory",
+                   "events": E, "activities": L, "samples": S},
+        "result_bytes": out_bytes,
+        "host_GB_per_s": out_bytes / t / 1e9,
+        "propagate_host_2048_ms": {"p50": float(np.median(inj)), "bytes_in": 8 * L * n_inj,
+                                   "bytes_out": 12 * E * n_inj},
+        "check": check,
+    }
+    return (0 if check and check["bit_exact"] else 3), out
+
+
 def bench_latency(args) -> dict:
     """Latency of the drop-in Python API (mc_dagprop_amd.Simulator, the reference's
     MonteCarloPropagator): run(seed) on the 200-event and the 20,000-event DAGs, and run_many over
@@ -472,12 +547,13 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--samples", type=int, default=None, help="samples per step (100,000; 65,536 for grad)")
-    ap.add_argument("--workload", choices=["network", "small", "batch", "grad", "serve"], default="network",
+    ap.add_argument("--workload", choices=["network", "small", "batch", "grad", "serve", "host"], default="network",
                     help="network (default): the headline line, with latency / serve / batch / grad blocks; "
                     "batch: 64 layered DAGs of 1,000-50,000 events (log-uniform) with 4,096-65,536 samples each "
                     "(log-uniform), per-event statistics in batched launches and, for comparison, one DAG at a "
                     "time; grad: run_many + the reverse walk (pathwise gradient) on the network DAG, 65,536 "
-                    "samples; serve: a Poisson stream of statistics requests over 200 resident contexts")
+                    "samples; serve: a Poisson stream of statistics requests over 200 resident contexts; host: the "
+                    "Python API's run_many of 100,000 network samples with results in host memory")
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--serve-approx", action="store_true",
                     help="serve workload: chunked requests' quantiles placed inside their histogram bin (no further "
@@ -542,8 +618,8 @@ def main() -> int:
                           "GB/s": 16 * n / ms / 1e6}), flush=True)
         return 0
 
-    if args.workload in ("batch", "grad", "serve"):
-        rc, out = {"batch": bench_batch, "grad": bench_grad, "serve": bench_serve}[args.workload](args)
+    if args.workload in ("batch", "grad", "serve", "host"):
+        rc, out = {"batch": bench_batch, "grad": bench_grad, "serve": bench_serve, "host": bench_host}[args.workload](args)
         print(json.dumps(out), flush=True)
         return rc
 
