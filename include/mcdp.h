@@ -109,7 +109,10 @@ int mcdp_run_many_device(mcdp_propagator *prop, const int32_t *seeds_dev, int64_
  * kept by the propagator up to 4 GiB).  Results of 16 MB and more reach the
  * outputs through pinned staging buffers, 64 MB per device, created on the
  * first such call and held until the process ends; calls from several threads
- * take turns on them.  Any output may be NULL to skip it. */
+ * take turns on them.  A run of at most 1,024 samples on a plan of 32 steps
+ * or more replays a graph of its launches, captured when the same sample count
+ * comes twice in a row (the propagator keeps one).  Any output may be NULL to
+ * skip it. */
 int mcdp_run_many_host(mcdp_propagator *prop, const int32_t *seeds, int64_t n, double *realized,
                        double *durations, int32_t *cause);
 

@@ -964,6 +964,7 @@ int mcdp::host_workspace(mcdp_propagator &P, size_t need, void *&buf, hipStream_
     if (hw->bytes < need) {
         if (hw->buf) {
             (void)hipStreamSynchronize(hw->stream);  // idle already: every call ends by waiting on it
+            hw->drop_graph();
             (void)hipFree(hw->buf);
             hw->buf = nullptr;
             hw->bytes = 0;
@@ -986,6 +987,7 @@ void mcdp::host_workspace_trim(mcdp_propagator &P) {
     for (auto &h : P.host_work)
         if (h.device == device && h.buf && h.bytes > kHostWorkKeepBytes) {
             (void)hipStreamSynchronize(h.stream);
+            h.drop_graph();
             (void)hipFree(h.buf);
             h.buf = nullptr;
             h.bytes = 0;
@@ -1195,6 +1197,72 @@ int staged_copy(const std::vector<CopyPlane> &planes, bool to_host) {
     return MCDP_OK;
 }
 
+// Short runs through the host path replay a captured graph of their launches
+// (HostWork): for up to this many samples, which stay below one round of tiles
+// on any device, so the launches are one stream's sequence; and for plans of at
+// least kGraphMinSteps steps.  A replay spends about 1.7 us per dependent launch
+// against about 3 us for launches one by one, but a graph of a single kernel
+// (the tiles' route) is slower to launch than the kernel alone.
+constexpr int64_t kGraphMaxSamples = 1024;
+constexpr int32_t kGraphMinSteps = 32;
+
+// The chunk's launches (launch()).  A run of one chunk of at most
+// kGraphMaxSamples samples whose key (ShortRunKey) matches the previous call's
+// is captured into a graph and launched as one; while the key stays, later
+// calls replay that graph.  Any other call, or a capture that fails, launches
+// directly.  The caller holds P.host_mu.
+int launch_host_chunk(mcdp_propagator &P, HostWork *hw, bool injected, const int32_t *d_seeds, double *d_dur,
+                      int64_t m, int64_t ld, double *d_real, int32_t *d_cause, hipStream_t st) {
+    auto direct = [&] {
+        return launch(P, injected, d_seeds, d_dur, m, ld, d_real, injected ? nullptr : d_dur, d_cause, st);
+    };
+    if (!hw || m != ld || m > kGraphMaxSamples || P.n_steps < kGraphMinSteps) return direct();
+    ShortRunKey key;
+    key.buf = hw->buf;
+    key.n = m;
+    key.ld = ld;
+    key.injected = injected;
+    key.generation = P.generation.load();
+    key.tail_mode = P.tail_mode;
+    if (hw->graph && hw->captured == key) {
+        HIP_TRY(hipGraphLaunch(hw->graph, st), "hipGraphLaunch");
+        return MCDP_OK;
+    }
+    if (!(hw->seen == key)) {  // first sight: the shape and the plan's upload happen here, not in a capture
+        hw->seen = key;
+        return direct();
+    }
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        return direct();
+    }
+    const int rc = direct();
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(st, &g);
+    hipGraphExec_t exec = nullptr;
+    const bool ok = rc == MCDP_OK && ee == hipSuccess && g && hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) == hipSuccess;
+    if (g) (void)hipGraphDestroy(g);
+    if (!ok) {  // launch directly; this key is not captured again until it is seen twice anew
+        (void)hipGetLastError();
+        hw->seen = ShortRunKey{};
+        return direct();
+    }
+    if (hw->graph) (void)hipGraphExecDestroy(hw->graph);
+    hw->graph = exec;
+    hw->captured = key;
+    HIP_TRY(hipGraphLaunch(exec, st), "hipGraphLaunch");
+    return MCDP_OK;
+}
+
+// The host-path state of the current device (after host_workspace).
+HostWork *current_host_work(mcdp_propagator &P) {
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) return nullptr;
+    for (auto &h : P.host_work)
+        if (h.device == device) return &h;
+    return nullptr;
+}
+
 // Host-buffer convenience: runs in chunks of samples so device memory stays
 // bounded, and copies each chunk into the caller's event-major arrays.  The
 // device buffers and the stream are the propagator's (HostWork), kept from
@@ -1221,6 +1289,7 @@ int run_host(mcdp_propagator &P, bool injected, const int32_t *seeds, const doub
     hipStream_t st = nullptr;
     int rc = host_workspace(P, need, work, st);
     if (rc) return rc;
+    HostWork *const hw = current_host_work(P);
     auto *base = static_cast<unsigned char *>(work);
     auto *d_real = reinterpret_cast<double *>(base);
     auto *d_cause = reinterpret_cast<int32_t *>(base + o_cause);
@@ -1267,7 +1336,7 @@ int run_host(mcdp_propagator &P, bool injected, const int32_t *seeds, const doub
             rc = hip_fail(e, "hipMemcpy (inputs)");
             break;
         }
-        rc = launch(P, injected, d_seeds, d_dur, m, chunk, d_real, injected ? nullptr : d_dur, d_cause, st);
+        rc = launch_host_chunk(P, hw, injected, d_seeds, d_dur, m, chunk, d_real, d_cause, st);
         if (rc) break;
         prefault.join();  // the pages are ready before the first copy writes them
         const size_t out_bytes = (size_t)m * ((realized ? 8 * E : 0) + (cause ? 4 * E : 0) +

@@ -80,11 +80,37 @@ struct RevPlan {
 // call, so a call allocates only when it needs more than any call before it,
 // and waits on its own stream only.  They hold per-sample rows, no plan, so
 // new times or a new shape leave them valid.
+// What a short run's launches hold fixed: the workspace, the sample count and
+// leading dimension (which place every buffer in it), the mode, the plan's
+// generation and the route hook.  Equal keys launch the same kernels with the
+// same arguments.
+struct ShortRunKey {
+    const void *buf = nullptr;
+    int64_t n = -1, ld = -1;
+    bool injected = false;
+    uint64_t generation = 0;
+    int32_t tail_mode = 0;
+    bool operator==(const ShortRunKey &o) const {
+        return buf == o.buf && n == o.n && ld == o.ld && injected == o.injected && generation == o.generation &&
+               tail_mode == o.tail_mode;
+    }
+};
+
 struct HostWork {
     int device = -1;
     void *buf = nullptr;
     size_t bytes = 0;
     hipStream_t stream = nullptr;
+    // Short runs: the launches of a key seen twice in a row are captured into
+    // a graph, replayed while the key stays (one graph launch instead of a
+    // launch per step).
+    ShortRunKey seen, captured;
+    hipGraphExec_t graph = nullptr;
+    void drop_graph() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        seen = captured = ShortRunKey{};
+    }
 };
 
 // One predecessor entry as the reference evaluates it.
@@ -161,6 +187,7 @@ struct mcdp_propagator {
             (void)hipGetDevice(&cur);
             (void)hipSetDevice(h.device);
             if (h.stream) (void)hipStreamSynchronize(h.stream);
+            h.drop_graph();
             if (h.buf) (void)hipFree(h.buf);
             if (h.stream) (void)hipStreamDestroy(h.stream);
             if (cur >= 0) (void)hipSetDevice(cur);

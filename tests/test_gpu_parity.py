@@ -237,6 +237,46 @@ def test_host_run_many_over_several_chunks(gpu, name: str) -> None:
     np.testing.assert_array_equal(cc, c)
 
 
+@pytest.mark.parametrize("name", ["network_slice", "long_chain"])
+def test_short_host_runs_replay_a_graph(gpu, name: str) -> None:
+    """Host runs of a few samples on plans of 32 steps or more: the first call with a given sample count launches
+    directly, the second captures its launches into a graph, later calls replay it (the network slice's 41 wide
+    launches; the 100-step chain's one tile launch).  Every call, with other work and synchronizations between
+    them, equals the same seeds in one direct run_many, bit for bit; injected runs too, and a change of times is
+    not hidden by a replay."""
+    if name == "network_slice":
+        dag = layered_network(layers=40)
+    else:
+        dag = D.parity_long_chain(99)
+    P = capi.Propagator(dag)
+    other = capi.Propagator(layered_small())
+    seeds = np.arange(48, dtype=np.int32) * 13 + 5
+    r0, d0, c0 = P.run_many(seeds)  # 48 samples: a direct launch
+    for k in range(8):
+        sl = slice(6 * k, 6 * k + 6)
+        r, d, c = P.run_many(seeds[sl])
+        np.testing.assert_array_equal(bits(r), bits(r0[sl]), err_msg=f"call {k}")
+        np.testing.assert_array_equal(bits(d), bits(d0[sl]), err_msg=f"call {k}")
+        np.testing.assert_array_equal(c, c0[sl], err_msg=f"call {k}")
+        rr, cc = P.propagate(d0[sl])
+        np.testing.assert_array_equal(bits(rr), bits(r0[sl]), err_msg=f"injected call {k}")
+        np.testing.assert_array_equal(cc, c0[sl], err_msg=f"injected call {k}")
+        other.run_many(np.arange(50 + k))  # other work between the replays
+        capi.synchronize()
+    # new times: the next calls use the new plan, not the graph of the old one
+    _, e0 = P.get_times()
+    earliest = e0 + 7.0
+    P.set_times(earliest=earliest)
+    moved = FlatDag(**{**dag.__dict__})
+    moved.earliest = earliest.copy()
+    rf, df, cf = capi.Propagator(moved).run_many(seeds[:6])
+    for k in range(3):
+        r, d, c = P.run_many(seeds[:6])
+        np.testing.assert_array_equal(bits(r), bits(rf), err_msg=f"call {k} after new times")
+        np.testing.assert_array_equal(bits(d), bits(df), err_msg=f"call {k} after new times")
+        np.testing.assert_array_equal(c, cf, err_msg=f"call {k} after new times")
+
+
 SHAPES = [(1, 4, 1), (2, 16, 1), (5, 8, 1), (16, 12, 1), (16, 16, 1), (1, 4, 2), (3, 8, 2), (7, 4, 2), (16, 8, 2)]
 
 
